@@ -1,0 +1,205 @@
+"""ctypes binding of libfzh.so (the C ABI declared in include/fzh.h).
+
+This is the Python-side view of the drop-in boundary: every wrapper calls one
+``fzh_*`` entry point with plain device pointers.  There is no CPU fallback:
+if libfzh.so is missing or no GPU is visible, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libfzh.so")
+
+_lib = None
+
+_P = C.c_void_p
+_Z = C.c_size_t
+_I = C.c_int
+_U32 = C.c_uint32
+_F = C.c_float
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "fzh_last_error": (C.c_char_p, []),
+    "fzh_version": (C.c_char_p, []),
+    "fzh_workspace_init": (_I, [_P, _P, _Z]),
+    "fzh_lorenzo_quant_fwd_ws_bytes": (_Z, [_Z, _Z, _Z, _Z]),
+    "fzh_lorenzo_quant_fwd_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _F, _P, _U32, _I, _I, _P, _P, _P, _P, _Z, _P, _Z]),
+    "fzh_lorenzo_quant_inv_ws_bytes": (_Z, [_Z, _Z, _Z, _Z, _Z]),
+    "fzh_lorenzo_quant_inv_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _F, _I, _I, _P, _P, _Z]),
+    "fzh_value_base_ws_bytes": (_Z, [_Z]),
+    "fzh_value_base_f32": (_I, [_P, _P, _Z, _I, _F, _P, _P, _Z]),
+    "fzh_bitshuffle_enc": (_I, [_P, _P, _P, _Z, _U32, _I]),
+    "fzh_bitshuffle_dec": (_I, [_P, _P, _P, _Z, _U32, _I]),
+    "fzh_rze_enc_bound": (_Z, [_Z]),
+    "fzh_rze_enc_ws_bytes": (_Z, [_Z]),
+    "fzh_rze_enc": (_I, [_P, _P, _Z, _P, _Z, _P, _P, _Z]),
+    "fzh_rze_dec_ws_bytes": (_Z, [_Z]),
+    "fzh_rze_dec": (_I, [_P, _P, _Z, _P, _Z, _P, _P, _Z]),
+    "fzh_rle_enc_bound": (_Z, [_Z, _I]),
+    "fzh_rle_enc_ws_bytes": (_Z, [_Z]),
+    "fzh_rle_enc": (_I, [_P, _P, _Z, _I, _P, _Z, _P, _P, _Z]),
+    "fzh_rle_dec_ws_bytes": (_Z, [_Z]),
+    "fzh_rle_dec": (_I, [_P, _P, _Z, _I, _P, _Z, _P, _P, _Z]),
+    "fzh_histogram": (_I, [_P, _P, _Z, _I, _U32, _P]),
+    "fzh_huffman_revbook_bytes": (_Z, [_U32, _I]),
+    "fzh_huffman_build_book": (_I, [_P, _U32, _I, _P, _P]),
+    "fzh_huffman_enc_bound": (_Z, [_Z, _U32, _I]),
+    "fzh_huffman_enc_ws_bytes": (_Z, [_Z]),
+    "fzh_huffman_encode": (_I, [_P, _P, _Z, _I, _U32, _P, _P, _P, _Z, _P, _P, _Z]),
+    "fzh_huffman_decode": (_I, [_P, _P, _Z, _I, _P, _Z, _P]),
+    "fzh_quant_abs_fwd_f32": (_I, [_P, _P, _Z, _F, _F, _U32, _I, _I, _P, _P, _P, _P, _Z, _P, _Z]),
+    "fzh_quant_abs_fwd_ws_bytes": (_Z, [_Z]),
+    "fzh_quant_abs_inv_f32": (_I, [_P, _P, _Z, _F, _I, _I, _P, _P, _P, _Z, _P]),
+    "fzh_max_u32": (_I, [_P, _P, _Z, _I, _P]),
+    "fzh_bitpack_enc": (_I, [_P, _P, _Z, _I, _I, _P]),
+    "fzh_bitpack_dec": (_I, [_P, _P, _Z, _I, _I, _P]),
+    "fzh_gather": (_I, [_P, _P, _I]),
+    "fzh_device_alloc": (_I, [C.POINTER(_P), _Z]),
+    "fzh_device_free": (_I, [_P]),
+    "fzh_memcpy_htod": (_I, [_P, _P, _Z]),
+    "fzh_memcpy_dtoh": (_I, [_P, _P, _Z]),
+    "fzh_memset": (_I, [_P, _I, _Z]),
+    "fzh_synchronize": (_I, []),
+    "fzh_stream_create": (_I, [C.POINTER(_P)]),
+    "fzh_stream_destroy": (_I, [_P]),
+    "fzh_stream_sync": (_I, [_P]),
+    "fzh_event_create": (_I, [C.POINTER(_P)]),
+    "fzh_event_destroy": (_I, [_P]),
+    "fzh_event_record": (_I, [_P, _P]),
+    "fzh_event_elapsed_ms": (_I, [_P, _P, C.POINTER(_F)]),
+    "fzh_device_count": (_I, [C.POINTER(_I)]),
+}
+
+
+class FzhError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Load libfzh.so; raise loudly if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FzhError(f"{LIB_PATH} is missing: build it with `python -m fzgpumodules_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return list(SIGNATURES)
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().fzh_last_error().decode(errors="replace")
+        raise FzhError(f"{what or 'fzh call'} failed ({rc}): {msg}")
+
+
+def call(name: str, *args):
+    rc = getattr(lib(), name)(*args)
+    check(rc, name)
+    return rc
+
+
+# ---------------------------------------------------------------------------
+# Device memory helper
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    """A plain hipMalloc'd device buffer (test/bench plumbing)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = _P()
+        call("fzh_device_alloc", C.byref(p), max(self.nbytes, 16))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            call("fzh_memcpy_htod", b.ptr, a.ctypes.data, a.nbytes)
+        return b
+
+    @classmethod
+    def zeros(cls, nbytes: int) -> "DeviceBuffer":
+        b = cls(nbytes)
+        call("fzh_memset", b.ptr, 0, max(int(nbytes), 16))
+        return b
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        if a.nbytes:
+            call("fzh_memcpy_htod", self.ptr + offset, a.ctypes.data, a.nbytes)
+
+    def download(self, dtype=np.uint8, count: int | None = None, offset: int = 0) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            assert offset + out.nbytes <= self.nbytes
+            call("fzh_memcpy_dtoh", out.ctypes.data, self.ptr + offset, out.nbytes)
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().fzh_device_free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def workspace(nbytes: int, stream=None) -> DeviceBuffer:
+    """A zero-initialised workspace (fzh_workspace_init)."""
+    b = DeviceBuffer(max(int(nbytes), 256))
+    call("fzh_workspace_init", stream, b.ptr, b.nbytes)
+    return b
+
+
+def synchronize() -> None:
+    call("fzh_synchronize")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = lib().fzh_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class Event:
+    def __init__(self):
+        p = _P()
+        call("fzh_event_create", C.byref(p))
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        call("fzh_event_record", self.ptr, stream)
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = _F(0)
+        call("fzh_event_elapsed_ms", self.ptr, end.ptr, C.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().fzh_event_destroy(self.ptr)
+        except Exception:
+            pass

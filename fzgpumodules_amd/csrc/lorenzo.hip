@@ -1,0 +1,904 @@
+// lorenzo.hip — fused Lorenzo predict+quantize (1D/2D/3D) and its inverse,
+// plus the REL/NOA value-base reduction, for gfx950 (wave64).
+//
+// Format geometry follows the reference exactly: 1D tiles of 1024 elements
+// (modules/fused/lorenzo_quant/lorenzo_quant.cu:262-364), 2D tiles of 32x32
+// (lorenzo_quant_nd.cu:29-105), 3D tiles of 8x8x8 (lorenzo_quant_nd.cu:199-274).
+// One wave64 owns one tile.  Lane layouts are chosen for wave64:
+//   1D  lane l, step q: elements 4(l+64q) .. 4(l+64q)+3 of the tile
+//   2D  lane l: column x = l&31, rows 16*(l>>5) .. +15 of the tile
+//   3D  lane l: x = l&7, z = l>>3, rows y = 0..7 of the tile
+// The arithmetic per element is the reference's (same FP32 operation order),
+// and the inverse scans reproduce the reference's addition trees exactly.
+//
+// Outliers are compacted per tile and placed with an ordered single-pass
+// tile scan, so the outlier list is deterministic: tile-major, and sorted by
+// element index inside each tile (the reference appends them with atomics in
+// arbitrary order).
+#include "fzh_internal.h"
+
+namespace fzh {
+namespace {
+
+template <typename TCode> struct SignedOf;
+template <> struct SignedOf<uint8_t> { using type = int8_t; };
+template <> struct SignedOf<uint16_t> { using type = int16_t; };
+
+// (TCode)(int)delta, or zigzag((SCode)(int)delta)   (lorenzo_quant.cu:328-334)
+template <typename TCode, bool ZZ>
+__device__ __forceinline__ TCode enc_code(float d) {
+    using SC = typename SignedOf<TCode>::type;
+    constexpr int W = 8 * sizeof(TCode);
+    const int q = (int)d;
+    if constexpr (ZZ) {
+        const SC s = (SC)q;
+        return (TCode)(((TCode)s << 1) ^ (TCode)(s >> (W - 1)));
+    } else {
+        return (TCode)q;
+    }
+}
+
+// (TIn)(SCode)code, or zigzag-decode  (lorenzo_quant.cu:195-202)
+template <typename TCode, bool ZZ>
+__device__ __forceinline__ float dec_code(TCode c) {
+    using SC = typename SignedOf<TCode>::type;
+    if constexpr (ZZ) {
+        return (float)(SC)((TCode)(c >> 1) ^ (TCode)(-(SC)(c & 1u)));
+    } else {
+        return (float)(SC)c;
+    }
+}
+
+__device__ __forceinline__ float quant_round(float x, float r) { return __builtin_roundf(x * r); }
+
+struct Tiles {
+    uint32_t gx, gy, gz;  // tile grid
+};
+
+// ===========================================================================
+// Forward kernels
+// ===========================================================================
+
+template <typename TCode, bool ZZ>
+__global__ void __launch_bounds__(256)
+lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, float r_host, const float* __restrict__ d_r,
+                float rad, TCode* __restrict__ codes, float* __restrict__ oerr,
+                uint32_t* __restrict__ oidx, uint32_t* __restrict__ ocount, uint64_t max_out,
+                ScanCtx* ctx, uint64_t* status, uint32_t T, uint32_t total_waves) {
+    const TileTicket tk = take_ticket(ctx);
+    const float r = d_r ? *d_r : r_host;
+    if (tk.tile < T) {
+        const int l = lane_id();
+        const uint64_t base = (uint64_t)tk.tile * 1024;
+        const bool full = base + 1024 <= n;
+        float s[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
+            if (full) {
+                const float4 v = *reinterpret_cast<const float4*>(in + e0);
+                s[q][0] = quant_round(v.x, r);
+                s[q][1] = quant_round(v.y, r);
+                s[q][2] = quant_round(v.z, r);
+                s[q][3] = quant_round(v.w, r);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) s[q][k] = (e0 + k < n) ? quant_round(in[e0 + k], r) : 0.0f;
+            }
+        }
+        // delta_i = s_i - s_{i-1}, chain restarts at the tile start.
+        float d[4][4];
+        uint32_t mask = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            float prev = __shfl_up(s[q][3], 1, 64);
+            const float last_prev_q = __shfl(q > 0 ? s[q > 0 ? q - 1 : 0][3] : 0.0f, 63, 64);
+            if (l == 0) prev = (q > 0) ? last_prev_q : 0.0f;
+            d[q][0] = s[q][0] - prev;
+#pragma unroll
+            for (int k = 1; k < 4; k++) d[q][k] = s[q][k] - s[q][k - 1];
+        }
+        TCode c[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
+                const bool ok = fabsf(d[q][k]) < rad;
+                c[q][k] = ok ? enc_code<TCode, ZZ>(d[q][k]) : (TCode)0;
+                if (!ok && e < n) mask |= 1u << (q * 4 + k);
+            }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
+            if (full) {
+                if constexpr (sizeof(TCode) == 2) {
+                    uint2 w;
+                    w.x = (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 16);
+                    w.y = (uint32_t)c[q][2] | ((uint32_t)c[q][3] << 16);
+                    *reinterpret_cast<uint2*>(codes + e0) = w;
+                } else {
+                    *reinterpret_cast<uint32_t*>(codes + e0) =
+                        (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 8) | ((uint32_t)c[q][2] << 16) |
+                        ((uint32_t)c[q][3] << 24);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (e0 + k < n) codes[e0 + k] = c[q][k];
+            }
+        }
+        // Outliers in index order: key (q, lane, k).
+        const uint32_t agg = wave_sum(__popc(mask));
+        const uint32_t obase = lookback_exclusive(status, tk.tile, tk.epoch, agg);
+        const bool lb_ok = obase != kLookbackFailed;
+        if (agg && lb_ok) {
+            uint32_t run = obase;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t mq = (mask >> (4 * q)) & 0xFu;
+                const uint32_t cq = __popc(mq);
+                const uint32_t incl = wave_incl_scan(cq);
+                uint32_t pos = run + incl - cq;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (mq & (1u << k)) {
+                        if (pos < max_out) {
+                            oerr[pos] = d[q][k];
+                            oidx[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
+                        }
+                        pos++;
+                    }
+                run += __shfl(incl, 63, 64);
+            }
+        }
+        if (tk.tile == T - 1 && l == 0) *ocount = lb_ok ? obase + agg : 0xFFFFFFFFu;
+    }
+    release_ticket(ctx, tk.epoch, total_waves);
+}
+
+// Zero codes[from, to) (tail beyond nx*ny[*nz] when the input was padded).
+template <typename TCode>
+__device__ __forceinline__ void zero_code_tail(TCode* codes, uint64_t from, uint64_t to) {
+    for (uint64_t i = from + lane_id(); i < to; i += 64) codes[i] = (TCode)0;
+}
+
+template <typename TCode, bool ZZ>
+__global__ void __launch_bounds__(256)
+lz2d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n, float r_host,
+                const float* __restrict__ d_r, float rad, TCode* __restrict__ codes,
+                float* __restrict__ oerr, uint32_t* __restrict__ oidx, uint32_t* __restrict__ ocount,
+                uint64_t max_out, ScanCtx* ctx, uint64_t* status, uint32_t gx, uint32_t T,
+                uint32_t total_waves) {
+    const TileTicket tk = take_ticket(ctx);
+    const float r = d_r ? *d_r : r_host;
+    if (tk.tile < T) {
+        const int l = lane_id();
+        const int x = l & 31, h = l >> 5;
+        const uint32_t bx = tk.tile % gx, by = tk.tile / gx;
+        const uint32_t X = bx * 32 + x;
+        const uint32_t Y0 = by * 32 + 16 * h;
+        const bool xin = X < nx;
+        float s[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t Y = Y0 + i;
+            s[i] = (xin && Y < ny) ? quant_round(in[(uint64_t)Y * nx + X], r) : 0.0f;
+        }
+        // t = s(y) - s(y-1) inside the tile (row 0 of the tile subtracts 0).
+        const float north = __shfl(s[15], x, 64);  // row 15 of the tile, for h = 1
+        float t[16];
+        t[0] = s[0] - (h ? north : 0.0f);
+#pragma unroll
+        for (int i = 1; i < 16; i++) t[i] = s[i] - s[i - 1];
+        uint32_t mask = 0;
+        uint64_t bal[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const float w = __shfl_up(t[i], 1, 32);
+            const float d = (x > 0) ? t[i] - w : t[i];
+            const bool valid = xin && (Y0 + i) < ny;
+            const bool ok = fabsf(d) < rad;
+            if (valid) codes[(uint64_t)(Y0 + i) * nx + X] = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
+            const bool is_out = valid && !ok;
+            if (is_out) mask |= 1u << i;
+            bal[i] = __ballot(is_out);
+            t[i] = d;  // keep the delta for the outlier pass
+        }
+        uint32_t agg = 0, lo_tot = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            agg += __popcll(bal[i]);
+            lo_tot += __popc((uint32_t)bal[i]);
+        }
+        const uint32_t obase = lookback_exclusive(status, tk.tile, tk.epoch, agg);
+        const bool lb_ok = obase != kLookbackFailed;
+        if (agg && lb_ok) {
+            // Row-major order inside the tile: rows 0..15 (h=0) then 16..31 (h=1).
+            uint32_t run = h ? lo_tot : 0u;
+            const uint32_t below = (1u << x) - 1u;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t half = h ? (uint32_t)(bal[i] >> 32) : (uint32_t)bal[i];
+                if (mask & (1u << i)) {
+                    const uint32_t pos = obase + run + __popc(half & below);
+                    if (pos < max_out) {
+                        oerr[pos] = t[i];
+                        oidx[pos] = (Y0 + i) * nx + X;
+                    }
+                }
+                run += __popc(half);
+            }
+        }
+        if (tk.tile == T - 1) {
+            if (l == 0) *ocount = lb_ok ? obase + agg : 0xFFFFFFFFu;
+            zero_code_tail(codes, (uint64_t)nx * ny, n);
+        }
+    }
+    release_ticket(ctx, tk.epoch, total_waves);
+}
+
+template <typename TCode, bool ZZ>
+__global__ void __launch_bounds__(256)
+lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, uint64_t n,
+                float r_host, const float* __restrict__ d_r, float rad, TCode* __restrict__ codes,
+                float* __restrict__ oerr, uint32_t* __restrict__ oidx, uint32_t* __restrict__ ocount,
+                uint64_t max_out, ScanCtx* ctx, uint64_t* status, Tiles tg, uint32_t T,
+                uint32_t total_waves) {
+    const TileTicket tk = take_ticket(ctx);
+    const float r = d_r ? *d_r : r_host;
+    if (tk.tile < T) {
+        const int l = lane_id();
+        const int x = l & 7, z = l >> 3;
+        const uint32_t bx = tk.tile % tg.gx, by = (tk.tile / tg.gx) % tg.gy, bz = tk.tile / (tg.gx * tg.gy);
+        const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
+        const bool xz = X < nx && Z < nz;
+        const uint64_t plane = (uint64_t)nx * ny;
+        float s[8];
+#pragma unroll
+        for (int y = 0; y < 8; y++)
+            s[y] = (xz && Y0 + y < ny) ? quant_round(in[Z * plane + (uint64_t)(Y0 + y) * nx + X], r) : 0.0f;
+        // a = s - s(z-1); b = a - a(x-1); d = b - b(y-1)   (lorenzo_quant_nd.cu:234-246)
+        float b[8];
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            const float sz = __shfl_up(s[y], 8, 64);
+            const float a = s[y] - (z > 0 ? sz : 0.0f);
+            const float aw = __shfl_up(a, 1, 8);
+            b[y] = (x > 0) ? a - aw : a;
+        }
+        float d[8];
+        uint32_t mask = 0;
+        uint64_t bal[8];
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            d[y] = (y > 0) ? b[y] - b[y - 1] : b[y];
+            const bool valid = xz && (Y0 + y) < ny;
+            const bool ok = fabsf(d[y]) < rad;
+            if (valid) codes[Z * plane + (uint64_t)(Y0 + y) * nx + X] = ok ? enc_code<TCode, ZZ>(d[y]) : (TCode)0;
+            const bool is_out = valid && !ok;
+            if (is_out) mask |= 1u << y;
+            bal[y] = __ballot(is_out);
+        }
+        uint32_t agg = 0;
+#pragma unroll
+        for (int y = 0; y < 8; y++) agg += __popcll(bal[y]);
+        const uint32_t obase = lookback_exclusive(status, tk.tile, tk.epoch, agg);
+        const bool lb_ok = obase != kLookbackFailed;
+        if (agg && lb_ok) {
+            // Order inside the tile: (z, y, x).
+            const uint64_t zlow = (z == 0) ? 0ull : ((1ull << (8 * z)) - 1ull);
+            uint32_t run = 0;
+#pragma unroll
+            for (int y = 0; y < 8; y++) run += __popcll(bal[y] & zlow);
+            const uint32_t below = (1u << x) - 1u;
+#pragma unroll
+            for (int y = 0; y < 8; y++) {
+                const uint32_t row = (uint32_t)(bal[y] >> (8 * z)) & 0xFFu;
+                if (mask & (1u << y)) {
+                    const uint32_t pos = obase + run + __popc(row & below);
+                    if (pos < max_out) {
+                        oerr[pos] = d[y];
+                        oidx[pos] = (uint32_t)(Z * plane + (uint64_t)(Y0 + y) * nx + X);
+                    }
+                }
+                run += __popc(row);
+            }
+        }
+        if (tk.tile == T - 1) {
+            if (l == 0) *ocount = lb_ok ? obase + agg : 0xFFFFFFFFu;
+            zero_code_tail(codes, (uint64_t)nx * ny * nz, n);
+        }
+    }
+    release_ticket(ctx, tk.epoch, total_waves);
+}
+
+// ===========================================================================
+// REL / NOA value base (predictor_utils.cuh:27-117), resolved on the device
+// ===========================================================================
+__global__ void __launch_bounds__(256)
+minmax_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ partial,
+              uint32_t* __restrict__ counter, int mode, float user_eb, float* __restrict__ params) {
+    __shared__ float smn[4], smx[4];
+    const float first = in[0];
+    float mn = first, mx = first;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t n4 = n / 4;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        const float4 v = reinterpret_cast<const float4*>(in)[i];
+        if (v.x < mn) mn = v.x;
+        if (v.x > mx) mx = v.x;
+        if (v.y < mn) mn = v.y;
+        if (v.y > mx) mx = v.y;
+        if (v.z < mn) mn = v.z;
+        if (v.z > mx) mx = v.z;
+        if (v.w < mn) mn = v.w;
+        if (v.w > mx) mx = v.w;
+    }
+    for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float v = in[i];
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float a = __shfl_xor(mn, off, 64), b = __shfl_xor(mx, off, 64);
+        if (a < mn) mn = a;
+        if (b > mx) mx = b;
+    }
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) { smn[w] = mn; smx[w] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < 4; i++) {
+            if (smn[i] < mn) mn = smn[i];
+            if (smx[i] > mx) mx = smx[i];
+        }
+        partial[blockIdx.x] = make_float2(mn, mx);
+        __threadfence();
+        const uint32_t prev = atomicAdd(counter, 1u);
+        if (prev == gridDim.x - 1) {
+            __threadfence();
+            float gmn = partial[0].x, gmx = partial[0].y;
+            for (uint32_t b = 1; b < gridDim.x; b++) {
+                const uint64_t raw = __hip_atomic_load(reinterpret_cast<const uint64_t*>(&partial[b]),
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const float px = __uint_as_float((uint32_t)raw), py = __uint_as_float((uint32_t)(raw >> 32));
+                if (px < gmn) gmn = px;
+                if (py > gmx) gmx = py;
+            }
+            float vb;
+            if (mode == 2) {
+                vb = gmx - gmn;
+            } else {
+                const float amin = gmn < 0.0f ? -gmn : gmn;
+                const float amax = gmx < 0.0f ? -gmx : gmx;
+                vb = amin > amax ? amin : amax;
+            }
+            const float abs_eb = (vb <= 0.0f) ? user_eb : user_eb * vb;
+            params[0] = vb;
+            params[1] = abs_eb;
+            params[2] = 1.0f / (2.0f * abs_eb);
+            params[3] = 2.0f * abs_eb;
+            *counter = 0u;  // ready for the next call
+        }
+    }
+}
+
+// ===========================================================================
+// Inverse: outlier binning by tile (counting sort), then per-tile decode
+// ===========================================================================
+struct TileMap {
+    int ndim;
+    uint32_t nx, ny, nz;
+    uint32_t gx, gy;
+    uint64_t total;  // elements covered by the tiles' valid region
+};
+
+__device__ __forceinline__ bool tile_of(const TileMap& m, uint32_t idx, uint32_t& tile, uint32_t& local) {
+    if ((uint64_t)idx >= m.total) return false;
+    if (m.ndim == 1) {
+        tile = idx >> 10;
+        local = idx & 1023u;
+    } else if (m.ndim == 2) {
+        const uint32_t x = idx % m.nx, y = idx / m.nx;
+        tile = (y >> 5) * m.gx + (x >> 5);
+        local = ((y & 31u) << 5) | (x & 31u);
+    } else {
+        const uint32_t x = idx % m.nx, yz = idx / m.nx;
+        const uint32_t y = yz % m.ny, z = yz / m.ny;
+        tile = ((z >> 3) * m.gy + (y >> 3)) * m.gx + (x >> 3);
+        local = ((z & 7u) << 6) | ((y & 7u) << 3) | (x & 7u);
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(256)
+bin_count_kernel(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ count, uint64_t max_out,
+                 TileMap m, uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ slot) {
+    const uint64_t k = min((uint64_t)*count, max_out);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t t, loc;
+        if (tile_of(m, idx[i], t, loc)) slot[i] = atomicAdd(&tile_cnt[t], 1u);
+    }
+}
+
+// Exclusive scan of tile_cnt[0..T) into off[0..T] (off[T] = total), zeroing
+// tile_cnt as it goes so the array is clear for the next call.  One wave per
+// 1024 entries.
+__global__ void __launch_bounds__(256)
+bin_scan_kernel(uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ off, uint32_t T, ScanCtx* ctx,
+                uint64_t* status, uint32_t nseg, uint32_t total_waves) {
+    const TileTicket tk = take_ticket(ctx);
+    if (tk.tile < nseg) {
+        const int l = lane_id();
+        const uint32_t b = tk.tile * 1024 + 16 * l;
+        uint32_t v[16], sum = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            v[i] = (b + i < T) ? tile_cnt[b + i] : 0u;
+            sum += v[i];
+        }
+        const uint32_t incl = wave_incl_scan(sum);
+        const uint32_t agg = __shfl(incl, 63, 64);
+        uint32_t pre = lookback_exclusive(status, tk.tile, tk.epoch, agg);
+        if (pre == kLookbackFailed) pre = 0u;  // offsets stay in range; counts are bounded by the list
+        uint32_t run = pre + incl - sum;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            if (b + i < T) {
+                off[b + i] = run;
+                tile_cnt[b + i] = 0u;
+            }
+            run += v[i];
+        }
+        if (tk.tile == nseg - 1 && l == 0) off[T] = pre + agg;
+    }
+    release_ticket(ctx, tk.epoch, total_waves);
+}
+
+__global__ void __launch_bounds__(256)
+bin_place_kernel(const float* __restrict__ err, const uint32_t* __restrict__ idx,
+                 const uint32_t* __restrict__ count, uint64_t max_out, TileMap m,
+                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ slot,
+                 float* __restrict__ serr, uint32_t* __restrict__ sloc) {
+    const uint64_t k = min((uint64_t)*count, max_out);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t t, loc;
+        if (tile_of(m, idx[i], t, loc)) {
+            const uint32_t p = off[t] + slot[i];
+            serr[p] = err[i];
+            sloc[p] = loc;
+        }
+    }
+}
+
+// Adds the tile's outliers into the per-lane registers.  `reg_of(loc, lane, r)`
+// maps a local index to (lane, register); registers are updated with a select
+// chain so no dynamic register indexing is needed.
+#define FZH_APPLY_OUTLIERS(NREG, DVAR, LANE_OF, REG_OF)                      \
+    do {                                                                     \
+        const uint32_t o0 = off[tile], o1 = off[tile + 1];                   \
+        for (uint32_t j = o0; j < o1; j++) {                                 \
+            const uint32_t loc = sloc[j];                                    \
+            const float e = serr[j];                                         \
+            const int tl = (LANE_OF);                                        \
+            const int tr = (REG_OF);                                         \
+            if (l == tl) {                                                   \
+                _Pragma("unroll") for (int rr = 0; rr < NREG; rr++)          \
+                    if (rr == tr) DVAR[rr] = e + DVAR[rr];                   \
+            }                                                                \
+        }                                                                    \
+    } while (0)
+
+// 1D inverse, scan tree of lorenzo_quant.cu:17-87 (see header comment).
+template <typename TCode, bool ZZ>
+__global__ void __launch_bounds__(256)
+lz1d_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, const uint32_t* __restrict__ off,
+                const float* __restrict__ serr, const uint32_t* __restrict__ sloc, float* __restrict__ out,
+                uint32_t T) {
+    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= T) return;
+    const int l = lane_id();
+    const int h = l >> 5, lam = l & 31;
+    const uint64_t base = (uint64_t)tile * 1024;
+    const bool full = base + 1024 <= n;
+    float d[16];  // d[4q + k]
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
+        if (full) {
+            TCode c[4];
+            if constexpr (sizeof(TCode) == 2) {
+                const uint2 w = *reinterpret_cast<const uint2*>(codes + e0);
+                c[0] = (TCode)w.x; c[1] = (TCode)(w.x >> 16); c[2] = (TCode)w.y; c[3] = (TCode)(w.y >> 16);
+            } else {
+                const uint32_t w = *reinterpret_cast<const uint32_t*>(codes + e0);
+                c[0] = (TCode)w; c[1] = (TCode)(w >> 8); c[2] = (TCode)(w >> 16); c[3] = (TCode)(w >> 24);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) d[4 * q + k] = dec_code<TCode, ZZ>(c[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) d[4 * q + k] = (e0 + k < n) ? dec_code<TCode, ZZ>(codes[e0 + k]) : 0.0f;
+        }
+    }
+    // local index li -> reference thread t = li/4 = l + 64q, k = li%4
+    FZH_APPLY_OUTLIERS(16, d, (int)((loc >> 2) & 63u), (int)(((loc >> 8) << 2) | (loc & 3u)));
+    // per reference thread: sequential prefix of 4
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int k = 1; k < 4; k++) d[4 * q + k] += d[4 * q + k - 1];
+    // per 32-lane warp: Hillis-Steele over thread totals, add lane-1's inclusive
+    float wt[8];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float ts = d[4 * q + 3];
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const float v = __shfl_up(ts, o, 32);
+            if (lam >= o) ts += v;
+        }
+        const float pre = __shfl_up(ts, 1, 32);
+        if (lam > 0) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) d[4 * q + k] += pre;
+        }
+        wt[2 * q] = __shfl(d[4 * q + 3], 31, 64);
+        wt[2 * q + 1] = __shfl(d[4 * q + 3], 63, 64);
+    }
+    // per block: Hillis-Steele over the 8 warp totals
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        float nw[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) nw[i] = (i >= o) ? wt[i] + wt[i - o] : wt[i];
+#pragma unroll
+        for (int i = 0; i < 8; i++) wt[i] = nw[i];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int w = 2 * q + h;
+        if (w > 0) {
+            float add = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 7; i++)
+                if (i == w - 1) add = wt[i];
+#pragma unroll
+            for (int k = 0; k < 4; k++) d[4 * q + k] += add;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
+        float4 v = make_float4(d[4 * q] * ebx2, d[4 * q + 1] * ebx2, d[4 * q + 2] * ebx2, d[4 * q + 3] * ebx2);
+        if (full) {
+            *reinterpret_cast<float4*>(out + e0) = v;
+        } else {
+            if (e0 < n) out[e0] = v.x;
+            if (e0 + 1 < n) out[e0 + 1] = v.y;
+            if (e0 + 2 < n) out[e0 + 2] = v.z;
+            if (e0 + 3 < n) out[e0 + 3] = v.w;
+        }
+    }
+}
+
+// 2D inverse, scan tree of lorenzo_quant_nd.cu:112-190.
+template <typename TCode, bool ZZ>
+__global__ void __launch_bounds__(256)
+lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float ebx2,
+                const uint32_t* __restrict__ off, const float* __restrict__ serr,
+                const uint32_t* __restrict__ sloc, float* __restrict__ out, uint32_t gx, uint32_t T) {
+    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= T) return;
+    const int l = lane_id();
+    const int x = l & 31, h = l >> 5;
+    const uint32_t bx = tile % gx, by = tile / gx;
+    const uint32_t X = bx * 32 + x, Y0 = by * 32 + 16 * h;
+    const bool xin = X < nx;
+    float d[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        d[i] = (xin && Y0 + i < ny) ? dec_code<TCode, ZZ>(codes[(uint64_t)(Y0 + i) * nx + X]) : 0.0f;
+    // local = (ty << 5) | tx ; lane = tx + 32*(ty >= 16), register = ty & 15
+    FZH_APPLY_OUTLIERS(16, d, (int)((loc & 31u) | ((loc >> 4) & 32u)), (int)((loc >> 5) & 15u));
+    // y: sequential inside each 8-row group
+#pragma unroll
+    for (int i = 1; i < 8; i++) d[i] += d[i - 1];
+#pragma unroll
+    for (int i = 9; i < 16; i++) d[i] += d[i - 1];
+    // cross-group addends: A1 = T0, A2 = T0+T1, A3 = A2+T2
+    const float tlo = d[7], thi = d[15];
+    const float a2_own = thi + tlo;                 // meaningful in h = 0 lanes
+    const float a2 = __shfl(a2_own, x, 64);         // lanes of h = 1 read A2 from h = 0
+    if (h == 0) {
+#pragma unroll
+        for (int i = 8; i < 16; i++) d[i] += tlo;
+    } else {
+        const float a3 = tlo + a2;
+#pragma unroll
+        for (int i = 0; i < 8; i++) d[i] += a2;
+#pragma unroll
+        for (int i = 8; i < 16; i++) d[i] += a3;
+    }
+    // x: Hillis-Steele over the 32 columns, then scale
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        float v = d[i];
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const float u = __shfl_up(v, o, 32);
+            if (x >= o) v += u;
+        }
+        d[i] = v * ebx2;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (xin && Y0 + i < ny) out[(uint64_t)(Y0 + i) * nx + X] = d[i];
+}
+
+// 3D inverse, scan tree of lorenzo_quant_nd.cu:281-364 (y sequential, x
+// Hillis-Steele over 8, z Hillis-Steele over 8).
+template <typename TCode, bool ZZ>
+__global__ void __launch_bounds__(256)
+lz3d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, uint32_t nz, float ebx2,
+                const uint32_t* __restrict__ off, const float* __restrict__ serr,
+                const uint32_t* __restrict__ sloc, float* __restrict__ out, Tiles tg, uint32_t T) {
+    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= T) return;
+    const int l = lane_id();
+    const int x = l & 7, z = l >> 3;
+    const uint32_t bx = tile % tg.gx, by = (tile / tg.gx) % tg.gy, bz = tile / (tg.gx * tg.gy);
+    const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
+    const bool xz = X < nx && Z < nz;
+    const uint64_t plane = (uint64_t)nx * ny;
+    float d[8];
+#pragma unroll
+    for (int y = 0; y < 8; y++)
+        d[y] = (xz && Y0 + y < ny) ? dec_code<TCode, ZZ>(codes[Z * plane + (uint64_t)(Y0 + y) * nx + X]) : 0.0f;
+    // local = (z << 6) | (y << 3) | x ; lane = x + 8z, register = y
+    FZH_APPLY_OUTLIERS(8, d, (int)((loc & 7u) | ((loc >> 3) & 56u)), (int)((loc >> 3) & 7u));
+#pragma unroll
+    for (int y = 1; y < 8; y++) d[y] += d[y - 1];
+#pragma unroll
+    for (int y = 0; y < 8; y++) {
+        float v = d[y];
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            const float u = __shfl_up(v, o, 8);
+            if (x >= o) v += u;
+        }
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            const float u = __shfl_up(v, 8 * o, 64);
+            if (z >= o) v += u;
+        }
+        d[y] = v * ebx2;
+    }
+#pragma unroll
+    for (int y = 0; y < 8; y++)
+        if (xz && Y0 + y < ny) out[Z * plane + (uint64_t)(Y0 + y) * nx + X] = d[y];
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+int ndim_of(size_t ny, size_t nz) { return nz > 1 ? 3 : (ny > 1 ? 2 : 1); }
+
+size_t tile_count(size_t n, size_t nx, size_t ny, size_t nz) {
+    switch (ndim_of(ny, nz)) {
+    case 3: return cdiv(nx, 8) * cdiv(ny, 8) * cdiv(nz, 8);
+    case 2: return cdiv(nx, 32) * cdiv(ny, 32);
+    default: return cdiv(n, 1024);
+    }
+}
+
+int check_dims(size_t n, size_t nx, size_t ny, size_t nz, const char* who) {
+    const int nd = ndim_of(ny, nz);
+    if (n > 0xFFFFFFFFull) return fail(FZH_E_INVALID, std::string(who) + ": more than 2^32 elements");
+    if (nd >= 2) {
+        if (nx == 0) return fail(FZH_E_INVALID, std::string(who) + ": dims[0] must be set for 2D/3D");
+        const size_t cover = nx * ny * (nd == 3 ? nz : 1);
+        if (cover > n) return fail(FZH_E_INVALID, std::string(who) + ": dims exceed the element count");
+    }
+    return FZH_OK;
+}
+
+template <typename TCode>
+int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, size_t nz, float r,
+               const float* d_r, uint32_t radius, int zz, void* codes, float* oerr, uint32_t* oidx,
+               uint32_t* ocount, size_t max_out, void* ws, size_t ws_bytes) {
+    const size_t T = tile_count(n, nx, ny, nz);
+    if (ws_bytes < scan_state_bytes(T)) return fail(FZH_E_CAPACITY, "lorenzo fwd: workspace too small");
+    ScanCtx* ctx = (ScanCtx*)ws;
+    uint64_t* status = (uint64_t*)((uint8_t*)ws + sizeof(ScanCtx));
+    const float rad = (float)(TCode)radius;
+    const unsigned blocks = (unsigned)cdiv(T, 4);
+    const uint32_t waves = blocks * 4;
+    TCode* c = (TCode*)codes;
+    const int nd = ndim_of(ny, nz);
+#define FZH_LZ_FWD(ZZV)                                                                               \
+    if (nd == 1)                                                                                      \
+        lz1d_fwd_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(in, n, r, d_r, rad, c, oerr, oidx, ocount, \
+                                                           max_out, ctx, status, (uint32_t)T, waves); \
+    else if (nd == 2)                                                                                 \
+        lz2d_fwd_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(                                           \
+            in, (uint32_t)nx, (uint32_t)ny, n, r, d_r, rad, c, oerr, oidx, ocount, max_out, ctx,      \
+            status, (uint32_t)cdiv(nx, 32), (uint32_t)T, waves);                                      \
+    else                                                                                              \
+        lz3d_fwd_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(                                           \
+            in, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, n, r, d_r, rad, c, oerr, oidx, ocount,      \
+            max_out, ctx, status, Tiles{(uint32_t)cdiv(nx, 8), (uint32_t)cdiv(ny, 8), 0}, (uint32_t)T, \
+            waves);
+    if (zz) {
+        FZH_LZ_FWD(true)
+    } else {
+        FZH_LZ_FWD(false)
+    }
+#undef FZH_LZ_FWD
+    FZH_CHECK_LAUNCH("lorenzo fwd");
+    return FZH_OK;
+}
+
+struct InvWs {
+    size_t T, nseg, max_out;
+    size_t o_ctx, o_cnt, o_off, o_slot, o_err, o_loc, total;
+};
+
+InvWs inv_layout(size_t n, size_t nx, size_t ny, size_t nz, size_t max_out) {
+    InvWs w;
+    w.T = tile_count(n, nx, ny, nz);
+    w.nseg = cdiv(w.T, 1024);
+    w.max_out = max_out;
+    size_t o = 0;
+    w.o_ctx = o;  o = align_up(o + scan_state_bytes(w.nseg), 256);
+    w.o_cnt = o;  o = align_up(o + 4 * w.T, 256);
+    w.o_off = o;  o = align_up(o + 4 * (w.T + 1), 256);
+    w.o_slot = o; o = align_up(o + 4 * std::max<size_t>(max_out, 1), 256);
+    w.o_err = o;  o = align_up(o + 4 * std::max<size_t>(max_out, 1), 256);
+    w.o_loc = o;  o = align_up(o + 4 * std::max<size_t>(max_out, 1), 256);
+    w.total = o;
+    return w;
+}
+
+template <typename TCode>
+int launch_inv(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny, size_t nz,
+               const float* err, const uint32_t* idx, const uint32_t* count, size_t max_out,
+               float ebx2, int zz, float* out, void* ws, size_t ws_bytes) {
+    const InvWs L = inv_layout(n, nx, ny, nz, max_out);
+    if (ws_bytes < L.total) return fail(FZH_E_CAPACITY, "lorenzo inv: workspace too small");
+    uint8_t* w = (uint8_t*)ws;
+    ScanCtx* ctx = (ScanCtx*)(w + L.o_ctx);
+    uint64_t* status = (uint64_t*)(w + L.o_ctx + sizeof(ScanCtx));
+    uint32_t* cnt = (uint32_t*)(w + L.o_cnt);
+    uint32_t* off = (uint32_t*)(w + L.o_off);
+    uint32_t* slot = (uint32_t*)(w + L.o_slot);
+    float* serr = (float*)(w + L.o_err);
+    uint32_t* sloc = (uint32_t*)(w + L.o_loc);
+
+    const int nd = ndim_of(ny, nz);
+    TileMap m;
+    m.ndim = nd;
+    m.nx = (uint32_t)nx; m.ny = (uint32_t)ny; m.nz = (uint32_t)nz;
+    m.gx = (uint32_t)(nd == 2 ? cdiv(nx, 32) : cdiv(nx, 8));
+    m.gy = (uint32_t)cdiv(ny, 8);
+    m.total = nd == 3 ? (uint64_t)nx * ny * nz : nd == 2 ? (uint64_t)nx * ny : n;
+
+    const bool has_out = count != nullptr && max_out > 0;
+    const unsigned ob = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(max_out, 256), 1024));
+    if (has_out) {
+        bin_count_kernel<<<ob, 256, 0, s>>>(idx, count, max_out, m, cnt, slot);
+    }
+    {
+        const unsigned sb = (unsigned)cdiv(L.nseg, 4);
+        bin_scan_kernel<<<sb, 256, 0, s>>>(cnt, off, (uint32_t)L.T, ctx, status, (uint32_t)L.nseg, sb * 4);
+    }
+    if (has_out) {
+        bin_place_kernel<<<ob, 256, 0, s>>>(err, idx, count, max_out, m, off, slot, serr, sloc);
+    }
+    FZH_CHECK_LAUNCH("lorenzo inv binning");
+    const unsigned blocks = (unsigned)cdiv(L.T, 4);
+    const TCode* c = (const TCode*)codes;
+#define FZH_LZ_INV(ZZV)                                                                                \
+    if (nd == 1)                                                                                       \
+        lz1d_inv_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(c, n, ebx2, off, serr, sloc, out, (uint32_t)L.T); \
+    else if (nd == 2)                                                                                  \
+        lz2d_inv_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(c, (uint32_t)nx, (uint32_t)ny, ebx2, off,   \
+                                                           serr, sloc, out, m.gx, (uint32_t)L.T);       \
+    else                                                                                               \
+        lz3d_inv_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(c, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, \
+                                                           ebx2, off, serr, sloc, out,                 \
+                                                           Tiles{m.gx, m.gy, 0}, (uint32_t)L.T);
+    if (zz) {
+        FZH_LZ_INV(true)
+    } else {
+        FZH_LZ_INV(false)
+    }
+#undef FZH_LZ_INV
+    FZH_CHECK_LAUNCH("lorenzo inv");
+    return FZH_OK;
+}
+
+}  // namespace
+}  // namespace fzh
+
+using namespace fzh;
+
+extern "C" {
+
+size_t fzh_lorenzo_quant_fwd_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz) {
+    return align_up(scan_state_bytes(tile_count(n, nx, ny, nz)), 256);
+}
+
+int fzh_lorenzo_quant_fwd_f32(void* stream, const float* d_in, size_t n, size_t nx, size_t ny, size_t nz,
+                              float ebx2_r, const float* d_ebx2_r, uint32_t radius, int zigzag,
+                              int code_bytes, void* d_codes, float* d_outlier_err, uint32_t* d_outlier_idx,
+                              uint32_t* d_outlier_count, size_t max_outliers, void* d_ws, size_t ws_bytes) {
+    if (int rc = check_dims(n, nx, ny, nz, "lorenzo fwd")) return rc;
+    if (n == 0) return FZH_OK;
+    if (!d_in || !d_codes || !d_outlier_count) return fail(FZH_E_INVALID, "lorenzo fwd: null pointer");
+    if (max_outliers > 0 && (!d_outlier_err || !d_outlier_idx))
+        return fail(FZH_E_INVALID, "lorenzo fwd: null outlier buffers");
+    if (ndim_of(ny, nz) == 1 && reinterpret_cast<uintptr_t>(d_in) % 16 != 0)
+        return fail(FZH_E_INVALID, "lorenzo fwd: input must be 16-byte aligned");
+    switch (code_bytes) {
+    case 1:
+        return launch_fwd<uint8_t>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag, d_codes,
+                                   d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers, d_ws, ws_bytes);
+    case 2:
+        return launch_fwd<uint16_t>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag, d_codes,
+                                    d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers, d_ws, ws_bytes);
+    default:
+        return fail(FZH_E_INVALID, "lorenzo fwd: code_bytes must be 1 or 2");
+    }
+}
+
+size_t fzh_lorenzo_quant_inv_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz, size_t max_outliers) {
+    return inv_layout(n, nx, ny, nz, max_outliers).total;
+}
+
+int fzh_lorenzo_quant_inv_f32(void* stream, const void* d_codes, size_t n, size_t nx, size_t ny, size_t nz,
+                              const float* d_outlier_err, const uint32_t* d_outlier_idx,
+                              const uint32_t* d_outlier_count, size_t max_outliers, float ebx2, int zigzag,
+                              int code_bytes, float* d_out, void* d_ws, size_t ws_bytes) {
+    if (int rc = check_dims(n, nx, ny, nz, "lorenzo inv")) return rc;
+    if (n == 0) return FZH_OK;
+    if (!d_codes || !d_out) return fail(FZH_E_INVALID, "lorenzo inv: null pointer");
+    if (ndim_of(ny, nz) == 1 &&
+        (reinterpret_cast<uintptr_t>(d_out) % 16 != 0 || reinterpret_cast<uintptr_t>(d_codes) % 8 != 0))
+        return fail(FZH_E_INVALID, "lorenzo inv: 1D buffers must be 16/8-byte aligned");
+    if (max_outliers > 0 && d_outlier_count && (!d_outlier_err || !d_outlier_idx))
+        return fail(FZH_E_INVALID, "lorenzo inv: null outlier buffers");
+    switch (code_bytes) {
+    case 1:
+        return launch_inv<uint8_t>(S(stream), d_codes, n, nx, ny, nz, d_outlier_err, d_outlier_idx, d_outlier_count,
+                                   max_outliers, ebx2, zigzag, d_out, d_ws, ws_bytes);
+    case 2:
+        return launch_inv<uint16_t>(S(stream), d_codes, n, nx, ny, nz, d_outlier_err, d_outlier_idx,
+                                    d_outlier_count, max_outliers, ebx2, zigzag, d_out, d_ws, ws_bytes);
+    default:
+        return fail(FZH_E_INVALID, "lorenzo inv: code_bytes must be 1 or 2");
+    }
+}
+
+size_t fzh_value_base_ws_bytes(size_t n) {
+    (void)n;
+    return 256 + 8 * 1024;
+}
+
+int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode, float user_eb, float* d_params,
+                       void* d_ws, size_t ws_bytes) {
+    if (ws_bytes < fzh_value_base_ws_bytes(n)) return fail(FZH_E_CAPACITY, "value base: workspace too small");
+    if (mode != 1 && mode != 2) return fail(FZH_E_INVALID, "value base: mode must be 1 (REL) or 2 (NOA)");
+    if (n == 0) return fail(FZH_E_INVALID, "value base: empty input");
+    if (reinterpret_cast<uintptr_t>(d_in) % 16 != 0) return fail(FZH_E_INVALID, "value base: input must be 16-byte aligned");
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256 * 16), 1024));
+    uint32_t* counter = (uint32_t*)d_ws;
+    float2* partial = (float2*)((uint8_t*)d_ws + 256);
+    minmax_kernel<<<blocks, 256, 0, S(stream)>>>(d_in, n, partial, counter, mode, user_eb, d_params);
+    FZH_CHECK_LAUNCH("minmax_kernel");
+    return FZH_OK;
+}
+
+}  // extern "C"

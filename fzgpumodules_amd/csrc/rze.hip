@@ -1,0 +1,500 @@
+// rze.hip — Recursive Zero-byte Elimination, 16 KiB chunks, 4 levels (gfx950).
+//
+// Stream format (modules/coders/rze/rze_stage.cu:774-843, 324-445):
+//   [orig_bytes u32][n_chunks u32][size_with_flag u32 x n_chunks][chunks...]
+//   zero-padded to a multiple of 4 bytes.  size bit 31 = chunk stored raw.
+// Chunk: [nonzero bytes][bm1 non-repeated][bm2 non-repeated][bm3 non-repeated]
+//        [bm4 raw 4 B][in_size u16 LE]; an all-zero chunk is the 2-byte tag.
+//   bm1 bit i of byte i/8 = (data byte i != 0)            (zero elimination)
+//   bm(k+1) bit i = (bm(k) byte i != bm(k) byte i-1)      (repeat elimination,
+//                                                          byte -1 := 0)
+// The raw-fallback thresholds are the reference's, verbatim (:369-431).
+//
+// MI355X design
+//   encode: one wave64 per chunk (4 waves per workgroup).  The chunk is held
+//   in registers (16 x 16 B per lane), bitmaps are built with per-lane masks
+//   in LDS, the compressed chunk is assembled in LDS, and its byte offset
+//   comes from the ordered tile scan — so the stream is written once, already
+//   packed (the reference writes per-chunk scratch, then scans on the host
+//   stream with CUB and copies again).
+//   decode: one wave64 per chunk; chunk input offsets come from the same
+//   ordered scan over the header's sizes (no host round trip), and the
+//   repeat-elimination decode is a parallel prefix-popcount gather instead of
+//   a single-thread loop.
+#include "fzh_internal.h"
+
+namespace fzh {
+namespace {
+
+constexpr int CS = 16384;
+constexpr int WAVES = 2;
+// LDS per wave: stream/staging buffer (+ slack for the aligned dword reads
+// that straddle its end) and bm1.
+constexpr int LDS_STREAM = CS + 64;
+constexpr int LDS_BM1 = CS / 8;
+constexpr int LDS_PER_WAVE = LDS_STREAM + LDS_BM1;
+
+__device__ __forceinline__ uint32_t byte_of(const uint4& v, int k) {
+    const uint32_t w = k < 4 ? v.x : k < 8 ? v.y : k < 12 ? v.z : v.w;
+    return (w >> (8 * (k & 3))) & 0xFFu;
+}
+
+// Copy `size` bytes from LDS buffer `lds` (dword aligned) to global `dst`
+// (any alignment), touching no byte outside [dst, dst+size).
+__device__ __forceinline__ void lds_to_global(const uint8_t* lds, uint8_t* dst, uint32_t size) {
+    if (size == 0) return;
+    const int l = lane_id();
+    const uintptr_t a0 = (uintptr_t)dst;
+    const uint32_t head = (uint32_t)((4 - (a0 & 3)) & 3);  // bytes before the first aligned dword
+    const uint32_t hb = head < size ? head : size;
+    if ((uint32_t)l < hb) dst[l] = lds[l];
+    if (size <= hb) return;
+    uint32_t* dw = reinterpret_cast<uint32_t*>(dst + hb);
+    const uint32_t nd = (size - hb) / 4;
+    const uint32_t sh = hb & 3;  // stream position hb is at lds byte offset hb
+    const uint32_t* lw = reinterpret_cast<const uint32_t*>(lds);
+    for (uint32_t i = l; i < nd; i += 64) {
+        const uint32_t p = hb + 4 * i;  // stream byte position of this dword
+        const uint32_t q = p >> 2;
+        uint32_t v;
+        if (sh == 0) {
+            v = lw[q];
+        } else {
+            const uint32_t lo = lw[q], hi = lw[q + 1];
+            v = __builtin_amdgcn_alignbyte(hi, lo, sh);
+        }
+        dw[i] = v;
+    }
+    const uint32_t done = hb + 4 * nd;
+    if ((uint32_t)l < size - done) dst[done + l] = lds[done + l];
+}
+
+// ---------------------------------------------------------------------------
+// Encode
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restrict__ out,
+               uint64_t* __restrict__ out_size, ScanCtx* ctx, uint64_t* status, uint32_t nc,
+               uint32_t total_waves) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int l = lane_id();
+    const int wv = threadIdx.x >> 6;
+    uint8_t* st = smem + wv * LDS_PER_WAVE;  // assembled chunk stream
+    uint8_t* bm1 = st + LDS_STREAM;
+
+    const TileTicket tk = take_ticket(ctx);
+    if (tk.tile < nc) {
+        const uint32_t c = tk.tile;
+        const uint64_t in_off = (uint64_t)c * CS;
+        const int in_size = (int)((nbytes - in_off) < (uint64_t)CS ? (nbytes - in_off) : CS);
+        const bool full = in_size == CS;
+
+        // -- load the chunk: lane l, step j holds bytes [1024 j + 16 l, +16)
+        uint4 v[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int p = 1024 * j + 16 * l;
+            if (full || p + 16 <= in_size) {
+                v[j] = *reinterpret_cast<const uint4*>(in + in_off + p);
+            } else {
+                uint32_t wds[4] = {0, 0, 0, 0};
+                for (int k = 0; k < 16; k++)
+                    if (p + k < in_size) wds[k >> 2] |= (uint32_t)in[in_off + p + k] << (8 * (k & 3));
+                v[j] = make_uint4(wds[0], wds[1], wds[2], wds[3]);
+            }
+        }
+        // -- level 1: nonzero masks (16 bits per lane per step) -> bm1 in LDS
+        uint32_t m[16];
+        uint32_t my_nz = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            uint32_t mm = 0;
+#pragma unroll
+            for (int k = 0; k < 16; k++) mm |= (byte_of(v[j], k) != 0u ? 1u : 0u) << k;
+            m[j] = mm;
+            my_nz += __popc(mm);
+            *reinterpret_cast<uint16_t*>(bm1 + 128 * j + 2 * l) = (uint16_t)mm;
+        }
+        const uint32_t data_nz = wave_sum(my_nz);
+        wave_lds_sync();
+
+        const int avail = CS - 2;
+        const int bm1_bytes = (in_size + 7) / 8;
+        const int bm2_bytes = (bm1_bytes + 7) / 8;
+        const int bm3_bytes = (bm2_bytes + 7) / 8;
+        uint32_t size = 0;
+        bool raw = false;
+        uint32_t bm2_nr = 0, bm3_nr = 0, bm4_nr = 0;
+        uint32_t rmask1 = 0;   // bm1 bytes 32l..32l+31 non-repeated
+        uint32_t rmask2 = 0;   // bm2 bytes 4l..4l+3 non-repeated (4 bits)
+        uint32_t bm3_byte = 0; // lanes 0..31: bm3 byte l
+        uint32_t rmask3 = 0;   // ballot over bm3 bytes (= bm4 as u32)
+        uint32_t bm2w = 0;     // bm2 bytes 4l..4l+3 as a word
+        if (data_nz == 0) {
+            size = 2;
+        } else {
+            int wpos = (int)data_nz;
+            if (wpos >= avail || wpos > avail - bm1_bytes) raw = true;
+            if (!raw) {
+                // -- level 2: RE over bm1 (N = bm1_bytes); lane l owns bytes 32l..32l+31
+                const uint4 a = *reinterpret_cast<const uint4*>(bm1 + 32 * l);
+                const uint4 b = *reinterpret_cast<const uint4*>(bm1 + 32 * l + 16);
+                const uint32_t last_prev = __shfl_up(b.w >> 24, 1, 64);
+                uint32_t prev = (l == 0) ? 0u : last_prev;
+                const uint32_t wds[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+                for (int k = 0; k < 32; k++) {
+                    const uint32_t cur = (wds[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+                    if (cur != prev && 32 * l + k < bm1_bytes) rmask1 |= 1u << k;
+                    prev = cur;
+                }
+                bm2w = rmask1;  // bm2 bytes 4l..4l+3 (bit k <-> bm1 byte 32l+k)
+                bm2_nr = wave_sum(__popc(rmask1));
+                wpos += (int)bm2_nr;
+                if (wpos >= avail) raw = true;
+                if (!raw && (wpos >= avail || wpos > avail - bm2_bytes)) raw = true;
+            }
+            if (!raw) {
+                // -- level 3: RE over bm2 (N = bm2_bytes); lane l owns bytes 4l..4l+3
+                uint32_t prev = __shfl_up(bm2w >> 24, 1, 64);
+                if (l == 0) prev = 0u;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t cur = (bm2w >> (8 * k)) & 0xFFu;
+                    if (cur != prev && 4 * l + k < bm2_bytes) rmask2 |= 1u << k;
+                    prev = cur;
+                }
+                bm3_nr = wave_sum(__popc(rmask2));
+                // bm3 byte m = nibble(2m) | nibble(2m+1) << 4
+                const uint32_t nlo = __shfl(rmask2, (2 * l) & 63, 64);
+                const uint32_t nhi = __shfl(rmask2, (2 * l + 1) & 63, 64);
+                bm3_byte = (l < 32) ? (nlo | (nhi << 4)) : 0u;
+                wpos += (int)bm3_nr;
+                if (wpos >= avail) raw = true;
+                if (!raw && (wpos >= avail - 4 || wpos > (avail - 4) - bm3_bytes)) raw = true;
+            }
+            if (!raw) {
+                // -- level 4: RE over bm3 (N = bm3_bytes); lanes 0..31 own bm3 bytes
+                uint32_t prev = __shfl_up(bm3_byte, 1, 64);
+                if (l == 0) prev = 0u;
+                const bool nr = (l < bm3_bytes) && bm3_byte != prev;
+                rmask3 = (uint32_t)__ballot(nr);
+                bm4_nr = __popc(rmask3);
+                wpos += (int)bm4_nr;
+                if (wpos >= avail - 4) raw = true;
+            }
+            size = raw ? ((1u << 31) | (uint32_t)in_size) : (uint32_t)(wpos + 4 + 2);
+        }
+        const uint32_t clean = size & 0x7FFFFFFFu;
+
+        // -- assemble the chunk stream in LDS (before the look-back so the
+        //    wait overlaps useful work)
+        if (size == 2) {
+            if (l == 0) { st[0] = (uint8_t)(in_size & 0xFF); st[1] = (uint8_t)((in_size >> 8) & 0xFF); }
+        } else if (raw) {
+#pragma unroll
+            for (int j = 0; j < 16; j++) *reinterpret_cast<uint4*>(st + 1024 * j + 16 * l) = v[j];
+        } else {
+            // nonzero bytes in order (j, lane, k)
+            uint32_t run = 0;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const uint32_t cnt = __popc(m[j]);
+                const uint32_t incl = wave_incl_scan(cnt);
+                uint32_t pos = run + incl - cnt;
+#pragma unroll
+                for (int k = 0; k < 16; k++)
+                    if (m[j] & (1u << k)) st[pos++] = (uint8_t)byte_of(v[j], k);
+                run += __shfl(incl, 63, 64);
+            }
+            // bm1 non-repeated bytes
+            {
+                const uint32_t cnt = __popc(rmask1);
+                const uint32_t incl = wave_incl_scan(cnt);
+                uint32_t pos = data_nz + incl - cnt;
+                for (int k = 0; k < 32; k++)
+                    if (rmask1 & (1u << k)) st[pos++] = bm1[32 * l + k];
+            }
+            // bm2 non-repeated bytes
+            {
+                const uint32_t cnt = __popc(rmask2);
+                const uint32_t incl = wave_incl_scan(cnt);
+                uint32_t pos = data_nz + bm2_nr + incl - cnt;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (rmask2 & (1u << k)) st[pos++] = (uint8_t)((bm2w >> (8 * k)) & 0xFFu);
+            }
+            // bm3 non-repeated bytes, bm4, tag
+            const uint32_t base3 = data_nz + bm2_nr + bm3_nr;
+            if (rmask3 & (1u << (l & 31)) && l < 32) st[base3 + __popc(rmask3 & ((1u << l) - 1u))] = (uint8_t)bm3_byte;
+            const uint32_t base4 = base3 + bm4_nr;
+            if (l < 4) st[base4 + l] = (uint8_t)((rmask3 >> (8 * l)) & 0xFFu);
+            if (l == 0) {
+                st[base4 + 4] = (uint8_t)(in_size & 0xFF);
+                st[base4 + 5] = (uint8_t)((in_size >> 8) & 0xFF);
+            }
+        }
+        wave_lds_sync();
+        const uint32_t hdr = 8u + 4u * nc;
+        const uint32_t off = lookback_exclusive(status, c, tk.epoch, clean);
+        if (off == kLookbackFailed) {
+            if (l == 0 && c == nc - 1) *out_size = 0;  // signals failure to the host
+        } else {
+        if (l == 0) {
+            reinterpret_cast<uint32_t*>(out + 8)[c] = size;
+            if (c == 0) {
+                reinterpret_cast<uint32_t*>(out)[0] = (uint32_t)nbytes;
+                reinterpret_cast<uint32_t*>(out)[1] = nc;
+            }
+        }
+        lds_to_global(st, out + hdr + off, clean);
+        if (c == nc - 1) {
+            const uint64_t end = (uint64_t)hdr + off + clean;
+            const uint64_t padded = (end + 3) & ~3ull;
+            if ((uint64_t)l < padded - end) out[end + l] = 0;
+            if (l == 0) *out_size = padded;
+        }
+        }
+    }
+    release_ticket(ctx, tk.epoch, total_waves);
+}
+
+// ---------------------------------------------------------------------------
+// Decode
+// ---------------------------------------------------------------------------
+// Repeat-elimination decode of `n` bytes: out[i] = data[rank(i)-1], or 0 when
+// no bit of bm[0..i] is set (rze_stage.cu:283-298, done with prefix popcounts).
+
+__global__ void __launch_bounds__(256)
+rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __restrict__ out,
+               uint64_t orig_bytes, int* __restrict__ d_status, ScanCtx* ctx, uint64_t* status, uint32_t nc,
+               uint32_t total_waves) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int l = lane_id();
+    const int wv = threadIdx.x >> 6;
+    uint8_t* st = smem + wv * LDS_PER_WAVE;
+    uint8_t* bm1 = st + LDS_STREAM;
+
+    const TileTicket tk = take_ticket(ctx);
+    if (tk.tile < nc) {
+        const uint32_t c = tk.tile;
+        const uint32_t hdr = 8u + 4u * nc;
+        bool bad = in_bytes < hdr;
+        uint32_t entry = 0;
+        if (!bad) {
+            const uint32_t h0 = reinterpret_cast<const uint32_t*>(in)[0];
+            const uint32_t h1 = reinterpret_cast<const uint32_t*>(in)[1];
+            bad = (uint64_t)h0 != orig_bytes || h1 != nc;
+            entry = reinterpret_cast<const uint32_t*>(in + 8)[c];
+        }
+        const uint32_t stored = bad ? 0u : (entry & 0x7FFFFFFFu);
+        const uint32_t off = lookback_exclusive(status, c, tk.epoch, stored);
+        if (off == kLookbackFailed) bad = true;
+        const uint64_t src = (uint64_t)hdr + off;
+        const uint32_t o = (c + 1 < nc) ? (uint32_t)CS : (uint32_t)(orig_bytes - (uint64_t)c * CS);
+        const bool israw = (entry & 0x80000000u) != 0u;
+        if (!bad) bad = src + stored > in_bytes || stored > (uint32_t)CS || (israw && stored < o) ||
+                        (!israw && stored != 2 && stored < 7);
+        uint8_t* dst = out + (uint64_t)c * CS;
+        if (bad) {
+            if (l == 0) *d_status = 1;
+        } else if (!israw && stored == 2) {
+            for (uint32_t i = 16 * l; i < o; i += 1024) {
+                if (i + 16 <= o) *reinterpret_cast<uint4*>(dst + i) = make_uint4(0, 0, 0, 0);
+                else for (uint32_t k = i; k < o; k++) dst[k] = 0;
+            }
+        } else {
+            // -- stage the chunk's bytes into LDS (dword aligned start)
+            const uint64_t a0 = src & ~3ull;
+            const uint32_t shift = (uint32_t)(src - a0);
+            const uint32_t span = (uint32_t)(((src + stored + 3) & ~3ull) - a0);  // bytes to fetch
+            const uint32_t nwd = span / 4;
+            uint32_t* stw = reinterpret_cast<uint32_t*>(st);
+            for (uint32_t i = l; i < nwd; i += 64) {
+                const uint64_t ga = a0 + 4ull * i;
+                uint32_t w;
+                if (ga + 4 <= in_bytes) {
+                    w = *reinterpret_cast<const uint32_t*>(in + ga);
+                } else {
+                    w = 0;
+                    for (uint32_t k = 0; k < 4; k++)
+                        if (ga + k < in_bytes) w |= (uint32_t)in[ga + k] << (8 * k);
+                }
+                stw[i] = w;
+            }
+            wave_lds_sync();
+            const uint8_t* cs = st + shift;  // chunk byte 0
+            if (israw) {
+                // raw copy: cs[0 .. o) -> dst (dst 16-byte aligned)
+                for (uint32_t i = 16 * l; i < o; i += 1024) {
+                    uint32_t w4[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t p = i + 4 * q + shift;
+                        const uint32_t lo = stw[p >> 2], hi = stw[(p >> 2) + 1];
+                        w4[q] = (p & 3) ? __builtin_amdgcn_alignbyte(hi, lo, p & 3) : lo;
+                    }
+                    if (i + 16 <= o) *reinterpret_cast<uint4*>(dst + i) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+                    else for (uint32_t k = i; k < o; k++) dst[k] = cs[k];
+                }
+            } else {
+                const int bm1_bytes = ((int)o + 7) / 8;
+                const int bm2_bytes = (bm1_bytes + 7) / 8;
+                const int bm3_bytes = (bm2_bytes + 7) / 8;
+                const int bm4_start = (int)stored - 6;
+                const uint32_t bm4 = (uint32_t)cs[bm4_start] | ((uint32_t)cs[bm4_start + 1] << 8) |
+                                     ((uint32_t)cs[bm4_start + 2] << 16) | ((uint32_t)cs[bm4_start + 3] << 24);
+                const int bm3_start = bm4_start - __popc(bm4);
+                // bm3: lanes 0..31, byte l
+                uint32_t b3 = 0;
+                {
+                    const uint32_t rank = __popc(bm4 & (l >= 31 ? 0xFFFFFFFFu : ((2u << l) - 1u)));
+                    const int p = bm3_start + (int)rank - 1;
+                    b3 = (l < bm3_bytes && rank > 0 && p >= 0) ? cs[p] : 0u;
+                    if (l < bm3_bytes && rank > 0 && p < 0) bad = true;
+                }
+                const uint32_t bm3_nr = wave_sum(l < 32 ? __popc(b3) : 0u);
+                const int bm2_start = bm3_start - (int)bm3_nr;
+                // bm2: lane l owns bytes 4l..4l+3; bits 4l..4l+3 of bm3 are its nibble
+                uint32_t b2w = 0;
+                {
+                    const uint32_t b3byte = __shfl(b3, (l >> 1) & 63, 64);
+                    const uint32_t nib = (b3byte >> (4 * (l & 1))) & 0xFu;
+                    const uint32_t cnt = __popc(nib);
+                    const uint32_t excl = wave_incl_scan(cnt) - cnt;
+                    uint32_t r = excl;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (nib & (1u << k)) r++;
+                        const int p = bm2_start + (int)r - 1;
+                        uint32_t bv = 0;
+                        if (4 * l + k < bm2_bytes && r > 0) {
+                            if (p < 0) bad = true; else bv = cs[p];
+                        }
+                        b2w |= bv << (8 * k);
+                    }
+                }
+                const uint32_t bm2_nr = wave_sum(__popc(b2w));
+                const int bm1_start = bm2_start - (int)bm2_nr;
+                // bm1: lane l owns bytes 32l..32l+31; bits = bm2 bytes 4l..4l+3 = b2w
+                {
+                    const uint32_t cnt = __popc(b2w);
+                    const uint32_t excl = wave_incl_scan(cnt) - cnt;
+                    uint32_t r = excl;
+                    uint32_t wds[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        uint32_t wq = 0;
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            const int bi = 4 * q + k;
+                            if (b2w & (1u << bi)) r++;
+                            const int p = bm1_start + (int)r - 1;
+                            uint32_t bv = 0;
+                            if (32 * l + bi < bm1_bytes && r > 0) {
+                                if (p < 0) bad = true; else bv = cs[p];
+                            }
+                            wq |= bv << (8 * k);
+                        }
+                        wds[q] = wq;
+                    }
+                    *reinterpret_cast<uint4*>(bm1 + 32 * l) = make_uint4(wds[0], wds[1], wds[2], wds[3]);
+                    *reinterpret_cast<uint4*>(bm1 + 32 * l + 16) = make_uint4(wds[4], wds[5], wds[6], wds[7]);
+                }
+                const uint32_t data_nz = wave_sum(__popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l)) +
+                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 4)) +
+                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 8)) +
+                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 12)) +
+                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 16)) +
+                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 20)) +
+                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 24)) +
+                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 28)));
+                if ((int)data_nz > bm1_start) bad = true;
+                wave_lds_sync();
+                // zero-elimination decode: lane l, step j -> output bytes [1024 j + 16 l, +16)
+                uint32_t run = 0;
+                for (int j = 0; j < 16; j++) {
+                    const uint32_t mm = *reinterpret_cast<const uint16_t*>(bm1 + 128 * j + 2 * l);
+                    const uint32_t cnt = __popc(mm);
+                    const uint32_t incl = wave_incl_scan(cnt);
+                    uint32_t r = run + incl - cnt;
+                    uint32_t w4[4] = {0, 0, 0, 0};
+#pragma unroll
+                    for (int k = 0; k < 16; k++) {
+                        if (mm & (1u << k)) {
+                            const uint32_t bv = (r < data_nz && (int)r < bm1_start) ? cs[r] : 0u;
+                            w4[k >> 2] |= bv << (8 * (k & 3));
+                            r++;
+                        }
+                    }
+                    run += __shfl(incl, 63, 64);
+                    const uint32_t p = 1024 * j + 16 * l;
+                    if (p + 16 <= o) *reinterpret_cast<uint4*>(dst + p) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+                    else
+                        for (uint32_t k = p; k < o; k++) dst[k] = (uint8_t)(w4[(k - p) >> 2] >> (8 * ((k - p) & 3)));
+                }
+                if (__ballot(bad) != 0ull && l == 0) *d_status = 1;
+            }
+        }
+    }
+    release_ticket(ctx, tk.epoch, total_waves);
+}
+
+constexpr size_t kLds = (size_t)WAVES * LDS_PER_WAVE;
+
+}  // namespace
+}  // namespace fzh
+
+using namespace fzh;
+
+extern "C" {
+
+size_t fzh_rze_enc_bound(size_t nbytes) {
+    const size_t nc = cdiv(nbytes, CS);
+    // A partial last chunk may grow by up to ~1/7 (the thresholds compare
+    // against the full 16 KiB chunk), plus the header and 4-byte padding.
+    return nbytes + 8 + 4 * nc + (nbytes % CS) / 7 + 64;
+}
+
+size_t fzh_rze_enc_ws_bytes(size_t nbytes) { return align_up(scan_state_bytes(cdiv(nbytes, CS)), 256); }
+
+int fzh_rze_enc(void* stream, const void* d_in, size_t nbytes, void* d_out, size_t out_cap, uint64_t* d_out_size,
+                void* d_ws, size_t ws_bytes) {
+    if (nbytes == 0) {
+        return fail(FZH_E_INVALID, "RZE: empty input");
+    }
+    if (nbytes > 0xFFFFFFFFull) return fail(FZH_E_INVALID, "RZE: input larger than 4 GiB");
+    if (out_cap < fzh_rze_enc_bound(nbytes)) return fail(FZH_E_CAPACITY, "RZE: output buffer below fzh_rze_enc_bound()");
+    if (ws_bytes < fzh_rze_enc_ws_bytes(nbytes)) return fail(FZH_E_CAPACITY, "RZE: workspace too small");
+    if (((uintptr_t)d_in | (uintptr_t)d_out) % 16 != 0) return fail(FZH_E_INVALID, "RZE: buffers must be 16-byte aligned");
+    const uint32_t nc = (uint32_t)cdiv(nbytes, CS);
+    const unsigned blocks = (unsigned)cdiv(nc, WAVES);
+    ScanCtx* ctx = (ScanCtx*)d_ws;
+    uint64_t* status = (uint64_t*)((uint8_t*)d_ws + sizeof(ScanCtx));
+    rze_enc_kernel<<<blocks, 64 * WAVES, kLds, S(stream)>>>((const uint8_t*)d_in, nbytes, (uint8_t*)d_out, d_out_size,
+                                                            ctx, status, nc, blocks * WAVES);
+    FZH_CHECK_LAUNCH("rze_enc_kernel");
+    return FZH_OK;
+}
+
+size_t fzh_rze_dec_ws_bytes(size_t orig_bytes) { return align_up(scan_state_bytes(cdiv(orig_bytes, CS)), 256); }
+
+int fzh_rze_dec(void* stream, const void* d_in, size_t in_bytes, void* d_out, size_t orig_bytes, int* d_status,
+                void* d_ws, size_t ws_bytes) {
+    if (orig_bytes == 0) return FZH_OK;
+    if (orig_bytes > 0xFFFFFFFFull) return fail(FZH_E_INVALID, "RZE: output larger than 4 GiB");
+    if (ws_bytes < fzh_rze_dec_ws_bytes(orig_bytes)) return fail(FZH_E_CAPACITY, "RZE: workspace too small");
+    if (((uintptr_t)d_in % 4) != 0 || ((uintptr_t)d_out % 16) != 0)
+        return fail(FZH_E_INVALID, "RZE: input must be 4-byte and output 16-byte aligned");
+    if (!d_status) return fail(FZH_E_INVALID, "RZE: null status pointer");
+    const uint32_t nc = (uint32_t)cdiv(orig_bytes, CS);
+    const unsigned blocks = (unsigned)cdiv(nc, WAVES);
+    ScanCtx* ctx = (ScanCtx*)d_ws;
+    uint64_t* status = (uint64_t*)((uint8_t*)d_ws + sizeof(ScanCtx));
+    rze_dec_kernel<<<blocks, 64 * WAVES, kLds, S(stream)>>>((const uint8_t*)d_in, in_bytes, (uint8_t*)d_out,
+                                                            orig_bytes, d_status, ctx, status, nc, blocks * WAVES);
+    FZH_CHECK_LAUNCH("rze_dec_kernel");
+    return FZH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+/*
+ * fzh.h — C ABI of the MI355X (gfx950) stage kernels.
+ *
+ * This is the drop-in boundary underneath the fz::Stage classes.  Every entry
+ * point takes plain device pointers and byte/element counts, enqueues work on
+ * the given HIP stream (passed as void*; NULL = the null stream), allocates no
+ * device memory, throws nothing, and returns 0 on success or a negative
+ * FZH_E* code (message in fzh_last_error(), thread-local).
+ *
+ * Workspaces: entry points that need scratch take (d_ws, ws_bytes).  Query the
+ * size with the matching *_ws_bytes() function.  A workspace must be zeroed
+ * once with fzh_workspace_init() before first use; after that it can be reused
+ * by any number of calls on one stream (its ordered-scan state persists from
+ * call to call, so no per-call clearing is needed), but never by two calls
+ * that may run concurrently.
+ *
+ * Reference interfaces replaced (file:line in szcompressor/FZGPUModules):
+ *   fzh_lorenzo_quant_fwd_f32   launchLorenzoKernel / launchLorenzoKernel2D /
+ *                               launchLorenzoKernel3D
+ *                               (modules/fused/lorenzo_quant/lorenzo_quant.h:343-401)
+ *   fzh_lorenzo_quant_inv_f32   launchLorenzoInverseKernel{,2D,3D}
+ *                               (lorenzo_quant.h:354-414)
+ *   fzh_value_base_f32          computeValueBase
+ *                               (modules/predictors/predictor_utils.cuh:74-117)
+ *   fzh_bitshuffle_{enc,dec}    BitshuffleStage::execute kernels
+ *                               (modules/shufflers/bitshuffle/bitshuffle_stage.cu:355-475)
+ *   fzh_rze_{enc,dec}           RZEStage::execute
+ *                               (modules/coders/rze/rze_stage.cu:774-994)
+ *   fzh_rle_{enc,dec}           RLEStage<T>::execute (modules/coders/rle/rle.cu:185-385)
+ *   fzh_histogram               fz::module::GPU_histogram_generic
+ *                               (modules/util/histogram/histogram.cu:100-110)
+ *   fzh_huffman_build_book      phf_CPU_build_canonized_codebook_v2
+ *                               (modules/coders/huffman/phf/hf_bk.cc:84-140)
+ *   fzh_huffman_{encode,decode} phf::high_level<E>::encode / decode
+ *                               (modules/coders/huffman/phf/hf_hl.cc:43-137)
+ *   fzh_quant_abs_{fwd,inv}_f32 quantizer_abs_fwd_kernel / quantizer_abs_inv_kernel
+ *                               (modules/quantizers/quantizer/quantizer.cu:30-137)
+ *   fzh_bitpack_{enc,dec}       BitpackStage launchEncode / launchDecode
+ *                               (modules/coders/bitpack/bitpack_stage.cu:130-165)
+ *   fzh_max_u32                 cub::DeviceReduce::Max call in BitpackStage::execute
+ *                               (bitpack_stage.cu:190-205)
+ *   fzh_gather                  launch_gather_kernel (src/pipeline/concat_kernel.cu:63-71)
+ */
+#ifndef FZH_H
+#define FZH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FZH_OK 0
+#define FZH_E_INVALID (-1)   /* bad argument / unsupported configuration   */
+#define FZH_E_RUNTIME (-2)   /* HIP runtime or launch failure              */
+#define FZH_E_CAPACITY (-3)  /* output or workspace too small              */
+
+/* ---- library ------------------------------------------------------------ */
+const char* fzh_last_error(void);
+const char* fzh_version(void);
+/* Zero a workspace (fill kernel, safe under stream capture). */
+int fzh_workspace_init(void* stream, void* d_ws, size_t ws_bytes);
+
+/* ---- Lorenzo predict + quantize ------------------------------------------
+ * dims (nx, ny, nz): ndim = nz>1 ? 3 : ny>1 ? 2 : 1 (lorenzo_quant.h:181-185).
+ * n = element count of d_in/d_codes (the 1D path uses n; 2D/3D cover
+ * nx*ny[*nz] and write zeros into codes[nx*ny*nz .. n)).
+ * ebx2_r = 1/(2*abs_eb) in float.  If d_ebx2_r is non-NULL the kernel reads
+ * the value from device memory instead (set by fzh_value_base_f32).
+ * code_bytes: 1 (u8) or 2 (u16) codes.  radius is cast to the code type.
+ * Outliers: d_outlier_count receives the total count (may exceed
+ * max_outliers; only the first max_outliers are stored), in tile-major,
+ * row-major-within-tile order.                                              */
+size_t fzh_lorenzo_quant_fwd_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz);
+int fzh_lorenzo_quant_fwd_f32(void* stream, const float* d_in, size_t n,
+                              size_t nx, size_t ny, size_t nz,
+                              float ebx2_r, const float* d_ebx2_r,
+                              uint32_t radius, int zigzag, int code_bytes,
+                              void* d_codes, float* d_outlier_err,
+                              uint32_t* d_outlier_idx, uint32_t* d_outlier_count,
+                              size_t max_outliers, void* d_ws, size_t ws_bytes);
+
+/* Inverse: uses min(*d_outlier_count, max_outliers) outliers in any order.
+ * ebx2 = 2*abs_eb in float.                                                 */
+size_t fzh_lorenzo_quant_inv_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz,
+                                      size_t max_outliers);
+int fzh_lorenzo_quant_inv_f32(void* stream, const void* d_codes, size_t n,
+                              size_t nx, size_t ny, size_t nz,
+                              const float* d_outlier_err,
+                              const uint32_t* d_outlier_idx,
+                              const uint32_t* d_outlier_count,
+                              size_t max_outliers, float ebx2, int zigzag,
+                              int code_bytes, float* d_out, void* d_ws,
+                              size_t ws_bytes);
+
+/* REL (mode 1: max|x|) / NOA (mode 2: max-min) value base, resolved on the
+ * device.  d_params (4 floats) receives
+ *   [0] value_base, [1] abs_eb, [2] ebx2_r = 1/(2*abs_eb), [3] ebx2 = 2*abs_eb
+ * with abs_eb = user_eb*value_base, or user_eb when value_base <= 0
+ * (lorenzo_quant.cu:612-644).                                                */
+size_t fzh_value_base_ws_bytes(size_t n);
+int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode,
+                       float user_eb, float* d_params, void* d_ws, size_t ws_bytes);
+
+/* ---- Bitshuffle (block_size multiple of 1024*elem_width; tail copied) ---- */
+int fzh_bitshuffle_enc(void* stream, const void* d_in, void* d_out,
+                       size_t nbytes, uint32_t block_size, int elem_width);
+int fzh_bitshuffle_dec(void* stream, const void* d_in, void* d_out,
+                       size_t nbytes, uint32_t block_size, int elem_width);
+
+/* ---- RZE (chunk 16384, 4 levels) ------------------------------------------
+ * Stream: [orig u32][n_chunks u32][size|raw-flag u32 x n][chunks], padded
+ * with zeros to 4 bytes.  d_out_size (device u64) receives the stream size. */
+size_t fzh_rze_enc_bound(size_t nbytes);
+size_t fzh_rze_enc_ws_bytes(size_t nbytes);
+int fzh_rze_enc(void* stream, const void* d_in, size_t nbytes, void* d_out,
+                size_t out_cap, uint64_t* d_out_size, void* d_ws, size_t ws_bytes);
+/* orig_bytes: expected decoded size (from the stage header).  *d_status is
+ * set to 0 on success, nonzero if the stream is malformed or disagrees.    */
+size_t fzh_rze_dec_ws_bytes(size_t orig_bytes);
+int fzh_rze_dec(void* stream, const void* d_in, size_t in_bytes, void* d_out,
+                size_t orig_bytes, int* d_status, void* d_ws, size_t ws_bytes);
+
+/* ---- RLE: [R u32][values T x R, padded to 4][lengths u32 x R] ------------ */
+size_t fzh_rle_enc_bound(size_t n, int elem_bytes);
+size_t fzh_rle_enc_ws_bytes(size_t n);
+int fzh_rle_enc(void* stream, const void* d_in, size_t n, int elem_bytes,
+                void* d_out, size_t out_cap, uint64_t* d_out_size,
+                void* d_ws, size_t ws_bytes);
+size_t fzh_rle_dec_ws_bytes(size_t n);
+int fzh_rle_dec(void* stream, const void* d_in, size_t in_bytes,
+                int elem_bytes, void* d_out, size_t n, int* d_status,
+                void* d_ws, size_t ws_bytes);
+
+/* ---- Histogram / Huffman (coarse, 768-symbol partitions) ----------------- */
+int fzh_histogram(void* stream, const void* d_in, size_t n, int elem_bytes,
+                  uint32_t bklen, uint32_t* d_freq);
+/* Host routine: canonical codebook from a host histogram. */
+size_t fzh_huffman_revbook_bytes(uint32_t bklen, int elem_bytes);
+int fzh_huffman_build_book(const uint32_t* h_freq, uint32_t bklen,
+                           int elem_bytes, uint32_t* h_bk4, uint8_t* h_revbook);
+size_t fzh_huffman_enc_bound(size_t n, uint32_t bklen, int elem_bytes);
+size_t fzh_huffman_enc_ws_bytes(size_t n);
+int fzh_huffman_encode(void* stream, const void* d_in, size_t n, int elem_bytes,
+                       uint32_t bklen, const uint32_t* d_bk4,
+                       const uint8_t* d_revbook, void* d_out, size_t out_cap,
+                       uint64_t* d_out_size, void* d_ws, size_t ws_bytes);
+int fzh_huffman_decode(void* stream, const void* d_in, size_t in_bytes,
+                       int elem_bytes, void* d_out, size_t n, int* d_status);
+
+/* ---- Quantizer (ABS) ------------------------------------------------------ */
+int fzh_quant_abs_fwd_f32(void* stream, const float* d_in, size_t n,
+                          float ebx2_r, float threshold, uint32_t radius,
+                          int zigzag, int code_bytes, void* d_codes,
+                          float* d_outlier_val, uint32_t* d_outlier_idx,
+                          uint32_t* d_outlier_count, size_t max_outliers,
+                          void* d_ws, size_t ws_bytes);
+size_t fzh_quant_abs_fwd_ws_bytes(size_t n);
+int fzh_quant_abs_inv_f32(void* stream, const void* d_codes, size_t n,
+                          float ebx2, int zigzag, int code_bytes,
+                          const float* d_outlier_val,
+                          const uint32_t* d_outlier_idx,
+                          const uint32_t* d_outlier_count,
+                          size_t max_outliers, float* d_out);
+
+/* ---- Bitpack -------------------------------------------------------------- */
+int fzh_max_u32(void* stream, const void* d_in, size_t n, int elem_bytes,
+                uint32_t* d_max);
+int fzh_bitpack_enc(void* stream, const void* d_in, size_t n, int elem_bytes,
+                    int nbits, void* d_out);
+int fzh_bitpack_dec(void* stream, const void* d_in, size_t n, int elem_bytes,
+                    int nbits, void* d_out);
+
+/* ---- Gather (multi-output concat) ----------------------------------------- */
+typedef struct fzh_copy_desc {
+    const void* src;
+    void* dst;
+    uint64_t bytes;
+} fzh_copy_desc;
+int fzh_gather(void* stream, const fzh_copy_desc* d_descs, int n_segs);
+
+/* ---- Device helpers for bindings (not on the hot path) ------------------- */
+int fzh_device_alloc(void** d_ptr, size_t bytes);
+int fzh_device_free(void* d_ptr);
+int fzh_memcpy_htod(void* d_dst, const void* h_src, size_t bytes);
+int fzh_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
+int fzh_memset(void* d_dst, int value, size_t bytes);
+int fzh_synchronize(void);
+int fzh_stream_create(void** stream);
+int fzh_stream_destroy(void* stream);
+int fzh_stream_sync(void* stream);
+int fzh_event_create(void** ev);
+int fzh_event_destroy(void* ev);
+int fzh_event_record(void* ev, void* stream);
+int fzh_event_elapsed_ms(void* ev_start, void* ev_end, float* ms);
+int fzh_device_count(int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FZH_H */
