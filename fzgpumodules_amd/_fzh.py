@@ -55,7 +55,8 @@ SIGNATURES = {
     "fzh_quant_abs_fwd_f32": (_I, [_P, _P, _Z, _F, _F, _U32, _I, _I, _P, _P, _P, _P, _Z, _P, _Z]),
     "fzh_quant_abs_fwd_ws_bytes": (_Z, [_Z]),
     "fzh_quant_abs_inv_f32": (_I, [_P, _P, _Z, _F, _I, _I, _P, _P, _P, _Z, _P]),
-    "fzh_max_u32": (_I, [_P, _P, _Z, _I, _P]),
+    "fzh_max_u32_ws_bytes": (_Z, []),
+    "fzh_max_u32": (_I, [_P, _P, _Z, _I, _P, _P, _Z]),
     "fzh_bitpack_enc": (_I, [_P, _P, _Z, _I, _I, _P]),
     "fzh_bitpack_dec": (_I, [_P, _P, _Z, _I, _I, _P]),
     "fzh_gather": (_I, [_P, _P, _I]),

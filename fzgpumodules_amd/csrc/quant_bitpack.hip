@@ -1,0 +1,326 @@
+// quant_bitpack.hip — direct-value quantizer (ABS/NOA) and power-of-two bit
+// packing (gfx950).
+//
+// Quantizer (modules/quantizers/quantizer/quantizer.cu:30-67, 116-137):
+//   q = __float2int_rn(x * ebx2_r)  (round half to even, saturating)
+//   |q| < radius and |x| < threshold -> code = q (two's complement in TCode)
+//   or zigzag(q); otherwise code 0 and an outlier (x, i).
+//   inverse: out = (float)q * ebx2, then outliers assign their value.
+// Outliers are placed in index order with the ordered tile scan (the
+// reference appends with atomics).
+//
+// Bitpack (modules/coders/bitpack/bitpack_stage.cu:30-124): the low `nbits`
+// bits of each element, nbits in {1,2,4,8,16,32}; sub-byte: element k of a
+// byte at bit k*nbits; multi-byte: little endian.
+#include "fzh_internal.h"
+
+namespace fzh {
+namespace {
+
+template <typename TCode> struct SignedOf;
+template <> struct SignedOf<uint8_t> { using type = int8_t; };
+template <> struct SignedOf<uint16_t> { using type = int16_t; };
+template <> struct SignedOf<uint32_t> { using type = int32_t; };
+
+// __float2int_rn: nearest-even; out of range saturates; NaN -> 0.
+__device__ __forceinline__ int f2i_rn(float f) {
+    if (f != f) return 0;
+    if (f >= 2147483647.0f) return 2147483647;
+    if (f <= -2147483648.0f) return (int)0x80000000;
+    return (int)__builtin_rintf(f);
+}
+
+constexpr int QTILE = 1024;  // elements per wave (16 per lane)
+
+template <typename TCode, bool ZZ>
+__global__ void __launch_bounds__(256)
+quant_fwd_kernel(const float* __restrict__ in, uint64_t n, float r, float threshold, int rad,
+                 TCode* __restrict__ codes, float* __restrict__ oval, uint32_t* __restrict__ oidx,
+                 uint32_t* __restrict__ ocount, uint64_t max_out, ScanCtx* ctx, uint64_t* status, uint32_t T,
+                 uint32_t total_waves) {
+    using SC = typename SignedOf<TCode>::type;
+    constexpr int W = 8 * sizeof(TCode);
+    const TileTicket tk = take_ticket(ctx);
+    if (tk.tile < T) {
+        const int l = lane_id();
+        const uint64_t base = (uint64_t)tk.tile * QTILE;
+        float x[16];
+        uint32_t mask = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
+            if (e0 + 4 <= n) {
+                const float4 v = *reinterpret_cast<const float4*>(in + e0);
+                x[4 * q] = v.x; x[4 * q + 1] = v.y; x[4 * q + 2] = v.z; x[4 * q + 3] = v.w;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) x[4 * q + k] = (e0 + k < n) ? in[e0 + k] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t e = e0 + k;
+                const int qv = f2i_rn(x[4 * q + k] * r);
+                const bool ok = qv > -rad && qv < rad && fabsf(x[4 * q + k]) < threshold;
+                TCode c = 0;
+                if (ok) {
+                    if constexpr (ZZ) {
+                        const SC s = (SC)qv;
+                        c = (TCode)(((TCode)s << 1) ^ (TCode)(s >> (W - 1)));
+                    } else {
+                        c = (TCode)qv;
+                    }
+                }
+                if (e < n) {
+                    codes[e] = c;
+                    if (!ok) mask |= 1u << (4 * q + k);
+                }
+            }
+        }
+        const uint32_t agg = wave_sum(__popc(mask));
+        const uint32_t ob = lookback_exclusive(status, tk.tile, tk.epoch, agg);
+        if (agg && ob != kLookbackFailed) {
+            uint32_t run = ob;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t mq = (mask >> (4 * q)) & 0xFu;
+                const uint32_t c = __popc(mq);
+                const uint32_t incl = wave_incl_scan(c);
+                uint32_t pos = run + incl - c;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (mq & (1u << k)) {
+                        if (pos < max_out) {
+                            oval[pos] = x[4 * q + k];
+                            oidx[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
+                        }
+                        pos++;
+                    }
+                run += __shfl(incl, 63, 64);
+            }
+        }
+        if (tk.tile == T - 1 && l == 0) *ocount = (ob == kLookbackFailed) ? 0xFFFFFFFFu : ob + agg;
+    }
+    release_ticket(ctx, tk.epoch, total_waves);
+}
+
+template <typename TCode, bool ZZ>
+__global__ void __launch_bounds__(256)
+quant_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, float* __restrict__ out) {
+    using SC = typename SignedOf<TCode>::type;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const TCode c = codes[i];
+        int q;
+        if constexpr (ZZ) q = (int)(SC)((TCode)(c >> 1) ^ (TCode)(-(SC)(c & 1u)));
+        else q = (int)(SC)c;
+        out[i] = (float)q * ebx2;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+scatter_assign_kernel(const float* __restrict__ val, const uint32_t* __restrict__ idx,
+                      const uint32_t* __restrict__ count, uint64_t max_out, uint64_t n, float* __restrict__ out) {
+    const uint64_t k = min((uint64_t)*count, max_out);
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < k; t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = idx[t];
+        if (i < n) out[i] = val[t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bitpack
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ uint32_t ldv(const T* p, uint64_t i) { return (uint32_t)p[i]; }
+
+// one thread per output byte (sub-byte), per element (>= 8 bits)
+template <typename T>
+__global__ void __launch_bounds__(256)
+bitpack_enc_kernel(const T* __restrict__ in, uint64_t n, int nbits, uint8_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (nbits < 8) {
+        const int epb = 8 / nbits;
+        const uint32_t mask = (1u << nbits) - 1u;
+        const uint64_t nout = (n * nbits + 7) / 8;
+        for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nout; b += stride) {
+            uint32_t packed = 0;
+            for (int k = 0; k < epb; k++) {
+                const uint64_t e = b * epb + k;
+                if (e < n) packed |= (ldv(in, e) & mask) << (k * nbits);
+            }
+            out[b] = (uint8_t)packed;
+        }
+    } else {
+        const int bytes = nbits / 8;
+        const uint32_t mask = nbits >= 32 ? 0xFFFFFFFFu : ((1u << nbits) - 1u);
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+            uint32_t v = ldv(in, i) & mask;
+            for (int b = 0; b < bytes; b++) {
+                out[i * bytes + b] = (uint8_t)(v & 0xFFu);
+                v >>= 8;
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+bitpack_dec_kernel(const uint8_t* __restrict__ in, uint64_t n, int nbits, T* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (nbits < 8) {
+            const int epb = 8 / nbits;
+            const uint32_t mask = (1u << nbits) - 1u;
+            out[i] = (T)((in[i / epb] >> ((i % epb) * nbits)) & mask);
+        } else {
+            const int bytes = nbits / 8;
+            uint32_t v = 0;
+            for (int b = 0; b < bytes; b++) v |= (uint32_t)in[i * bytes + b] << (8 * b);
+            out[i] = (T)v;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+max_kernel(const T* __restrict__ in, uint64_t n, uint32_t* __restrict__ partial, uint32_t* __restrict__ counter,
+           uint32_t* __restrict__ result) {
+    __shared__ uint32_t sm[4];
+    uint32_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = (uint32_t)in[i];
+        m = v > m ? v : m;
+    }
+    m = wave_max(m);
+    if (lane_id() == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < 4; i++) m = sm[i] > m ? sm[i] : m;
+        m = sm[0] > m ? sm[0] : m;
+        partial[blockIdx.x] = m;
+        __threadfence();
+        if (atomicAdd(counter, 1u) == gridDim.x - 1) {
+            __threadfence();
+            uint32_t g = 0;
+            for (uint32_t b = 0; b < gridDim.x; b++) {
+                const uint32_t v = __hip_atomic_load(&partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                g = v > g ? v : g;
+            }
+            *result = g;
+            *counter = 0u;
+        }
+    }
+}
+
+}  // namespace
+}  // namespace fzh
+
+using namespace fzh;
+
+extern "C" {
+
+size_t fzh_quant_abs_fwd_ws_bytes(size_t n) {
+    return align_up(scan_state_bytes(cdiv(std::max<size_t>(n, 1), QTILE)), 256) + 4096 + 256;
+}
+
+int fzh_quant_abs_fwd_f32(void* stream, const float* d_in, size_t n, float ebx2_r, float threshold, uint32_t radius,
+                          int zigzag, int code_bytes, void* d_codes, float* d_oval, uint32_t* d_oidx,
+                          uint32_t* d_ocount, size_t max_outliers, void* d_ws, size_t ws_bytes) {
+    if (ws_bytes < fzh_quant_abs_fwd_ws_bytes(n)) return fail(FZH_E_CAPACITY, "quantizer: workspace too small");
+    if (n == 0) return FZH_OK;
+    if (n > 0xFFFFFFFFull) return fail(FZH_E_INVALID, "quantizer: more than 2^32 elements");
+    if ((uintptr_t)d_in % 16) return fail(FZH_E_INVALID, "quantizer: input must be 16-byte aligned");
+    const uint32_t T = (uint32_t)cdiv(n, QTILE);
+    const unsigned blocks = (unsigned)cdiv(T, 4);
+    ScanCtx* ctx = (ScanCtx*)d_ws;
+    uint64_t* status = (uint64_t*)((uint8_t*)d_ws + sizeof(ScanCtx));
+    hipStream_t s = S(stream);
+#define FZH_Q(TC, ZZV)                                                                                              \
+    quant_fwd_kernel<TC, ZZV><<<blocks, 256, 0, s>>>(d_in, n, ebx2_r, threshold, (int)(TC)radius, (TC*)d_codes,   \
+                                                     d_oval, d_oidx, d_ocount, max_outliers, ctx, status, T,      \
+                                                     blocks * 4)
+    switch (code_bytes) {
+    case 1: if (zigzag) FZH_Q(uint8_t, true); else FZH_Q(uint8_t, false); break;
+    case 2: if (zigzag) FZH_Q(uint16_t, true); else FZH_Q(uint16_t, false); break;
+    case 4: if (zigzag) FZH_Q(uint32_t, true); else FZH_Q(uint32_t, false); break;
+    default: return fail(FZH_E_INVALID, "quantizer: code_bytes must be 1, 2 or 4");
+    }
+#undef FZH_Q
+    FZH_CHECK_LAUNCH("quant_fwd_kernel");
+    return FZH_OK;
+}
+
+int fzh_quant_abs_inv_f32(void* stream, const void* d_codes, size_t n, float ebx2, int zigzag, int code_bytes,
+                          const float* d_oval, const uint32_t* d_oidx, const uint32_t* d_ocount, size_t max_outliers,
+                          float* d_out) {
+    if (n == 0) return FZH_OK;
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256), 4096));
+    hipStream_t s = S(stream);
+#define FZH_QI(TC, ZZV) quant_inv_kernel<TC, ZZV><<<blocks, 256, 0, s>>>((const TC*)d_codes, n, ebx2, d_out)
+    switch (code_bytes) {
+    case 1: if (zigzag) FZH_QI(uint8_t, true); else FZH_QI(uint8_t, false); break;
+    case 2: if (zigzag) FZH_QI(uint16_t, true); else FZH_QI(uint16_t, false); break;
+    case 4: if (zigzag) FZH_QI(uint32_t, true); else FZH_QI(uint32_t, false); break;
+    default: return fail(FZH_E_INVALID, "quantizer: code_bytes must be 1, 2 or 4");
+    }
+#undef FZH_QI
+    if (d_ocount && max_outliers > 0) {
+        const unsigned ob = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(max_outliers, 256), 1024));
+        scatter_assign_kernel<<<ob, 256, 0, s>>>(d_oval, d_oidx, d_ocount, max_outliers, n, d_out);
+    }
+    FZH_CHECK_LAUNCH("quantizer inverse");
+    return FZH_OK;
+}
+
+size_t fzh_max_u32_ws_bytes(void) { return 256 + 4 * 1024; }
+
+int fzh_max_u32(void* stream, const void* d_in, size_t n, int elem_bytes, uint32_t* d_max, void* d_ws,
+                   size_t ws_bytes) {
+    if (ws_bytes < fzh_max_u32_ws_bytes()) return fail(FZH_E_CAPACITY, "max: workspace too small");
+    uint32_t* counter = (uint32_t*)d_ws;
+    uint32_t* partial = (uint32_t*)((uint8_t*)d_ws + 256);
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(std::max<size_t>(n, 1), 256 * 8), 1024));
+    hipStream_t s = S(stream);
+    switch (elem_bytes) {
+    case 1: max_kernel<uint8_t><<<blocks, 256, 0, s>>>((const uint8_t*)d_in, n, partial, counter, d_max); break;
+    case 2: max_kernel<uint16_t><<<blocks, 256, 0, s>>>((const uint16_t*)d_in, n, partial, counter, d_max); break;
+    case 4: max_kernel<uint32_t><<<blocks, 256, 0, s>>>((const uint32_t*)d_in, n, partial, counter, d_max); break;
+    default: return fail(FZH_E_INVALID, "max: elem_bytes must be 1, 2 or 4");
+    }
+    FZH_CHECK_LAUNCH("max_kernel");
+    return FZH_OK;
+}
+
+int fzh_bitpack_enc(void* stream, const void* d_in, size_t n, int elem_bytes, int nbits, void* d_out) {
+    if (nbits <= 0 || nbits > 8 * elem_bytes || (nbits & (nbits - 1)) != 0)
+        return fail(FZH_E_INVALID, "bitpack: nbits must be a power of two <= 8*sizeof(T)");
+    if (n == 0) return FZH_OK;
+    const size_t work = nbits < 8 ? (n * nbits + 7) / 8 : n;
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(work, 256), 4096));
+    hipStream_t s = S(stream);
+    switch (elem_bytes) {
+    case 1: bitpack_enc_kernel<uint8_t><<<blocks, 256, 0, s>>>((const uint8_t*)d_in, n, nbits, (uint8_t*)d_out); break;
+    case 2: bitpack_enc_kernel<uint16_t><<<blocks, 256, 0, s>>>((const uint16_t*)d_in, n, nbits, (uint8_t*)d_out); break;
+    case 4: bitpack_enc_kernel<uint32_t><<<blocks, 256, 0, s>>>((const uint32_t*)d_in, n, nbits, (uint8_t*)d_out); break;
+    default: return fail(FZH_E_INVALID, "bitpack: elem_bytes must be 1, 2 or 4");
+    }
+    FZH_CHECK_LAUNCH("bitpack_enc_kernel");
+    return FZH_OK;
+}
+
+int fzh_bitpack_dec(void* stream, const void* d_in, size_t n, int elem_bytes, int nbits, void* d_out) {
+    if (nbits <= 0 || nbits > 8 * elem_bytes || (nbits & (nbits - 1)) != 0)
+        return fail(FZH_E_INVALID, "bitpack: nbits must be a power of two <= 8*sizeof(T)");
+    if (n == 0) return FZH_OK;
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256), 4096));
+    hipStream_t s = S(stream);
+    switch (elem_bytes) {
+    case 1: bitpack_dec_kernel<uint8_t><<<blocks, 256, 0, s>>>((const uint8_t*)d_in, n, nbits, (uint8_t*)d_out); break;
+    case 2: bitpack_dec_kernel<uint16_t><<<blocks, 256, 0, s>>>((const uint8_t*)d_in, n, nbits, (uint16_t*)d_out); break;
+    case 4: bitpack_dec_kernel<uint32_t><<<blocks, 256, 0, s>>>((const uint8_t*)d_in, n, nbits, (uint32_t*)d_out); break;
+    default: return fail(FZH_E_INVALID, "bitpack: elem_bytes must be 1, 2 or 4");
+    }
+    FZH_CHECK_LAUNCH("bitpack_dec_kernel");
+    return FZH_OK;
+}
+
+}  // extern "C"

@@ -204,7 +204,8 @@ def max_u32(v):
     v = np.ascontiguousarray(v)
     din = DeviceBuffer.from_array(v)
     m = DeviceBuffer(4)
-    call("fzh_max_u32", None, din.ptr, v.size, v.dtype.itemsize, m.ptr)
+    ws = workspace(lib().fzh_max_u32_ws_bytes())
+    call("fzh_max_u32", None, din.ptr, v.size, v.dtype.itemsize, m.ptr, ws.ptr, ws.nbytes)
     return int(m.download(np.uint32, 1)[0])
 
 

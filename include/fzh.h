@@ -165,8 +165,9 @@ int fzh_quant_abs_inv_f32(void* stream, const void* d_codes, size_t n,
                           size_t max_outliers, float* d_out);
 
 /* ---- Bitpack -------------------------------------------------------------- */
+size_t fzh_max_u32_ws_bytes(void);
 int fzh_max_u32(void* stream, const void* d_in, size_t n, int elem_bytes,
-                uint32_t* d_max);
+                uint32_t* d_max, void* d_ws, size_t ws_bytes);
 int fzh_bitpack_enc(void* stream, const void* d_in, size_t n, int elem_bytes,
                     int nbits, void* d_out);
 int fzh_bitpack_dec(void* stream, const void* d_in, size_t n, int elem_bytes,
