@@ -60,6 +60,38 @@ SIGNATURES = {
     "fzh_bitpack_enc": (_I, [_P, _P, _Z, _I, _I, _P]),
     "fzh_bitpack_dec": (_I, [_P, _P, _Z, _I, _I, _P]),
     "fzh_gather": (_I, [_P, _P, _I]),
+    "fzh_concat_ws_bytes": (_Z, []),
+    "fzh_concat_bound": (C.c_uint64, [_P, _I]),
+    "fzh_concat": (_I, [_P, _P, _I, _P, C.c_uint64, _P, _P, _Z]),
+    "fzh_readback": (_I, [_P, _P, _I, _P]),
+    # pipeline C ABI (include/fzp.h)
+    "fzp_create": (_I, [_Z, _F, C.POINTER(_P)]),
+    "fzp_destroy": (_I, [_P]),
+    "fzp_add_lorenzo": (_I, [_P, _F, _I, _U32, _F, _I, _I, _F, C.POINTER(_I)]),
+    "fzp_add_bitshuffle": (_I, [_P, _U32, _I, C.POINTER(_I)]),
+    "fzp_add_rze": (_I, [_P, _U32, _I, C.POINTER(_I)]),
+    "fzp_add_rle": (_I, [_P, _I, C.POINTER(_I)]),
+    "fzp_add_huffman": (_I, [_P, _I, _U32, C.POINTER(_I)]),
+    "fzp_add_quantizer": (_I, [_P, _F, _I, _U32, _F, _I, _I, _F, C.POINTER(_I)]),
+    "fzp_add_bitpack": (_I, [_P, _I, _I, C.POINTER(_I)]),
+    "fzp_connect": (_I, [_P, _I, _I, C.c_char_p]),
+    "fzp_set_dims": (_I, [_P, _Z, _Z, _Z]),
+    "fzp_finalize": (_I, [_P]),
+    "fzp_enable_graph_mode": (_I, [_P, _I]),
+    "fzp_enable_profiling": (_I, [_P, _I]),
+    "fzp_set_pool_managed_decomp_output": (_I, [_P, _I]),
+    "fzp_max_compressed_size": (_I, [_P, _Z, C.POINTER(_Z)]),
+    "fzp_compress": (_I, [_P, _P, _P, _Z, C.POINTER(_P), C.POINTER(_Z)]),
+    "fzp_decompress": (_I, [_P, _P, _P, _Z, C.POINTER(_P), C.POINTER(_Z)]),
+    "fzp_compress_into": (_I, [_P, _P, _P, _Z, _P, _Z, C.POINTER(_Z)]),
+    "fzp_decompress_into": (_I, [_P, _P, _P, _Z, _P, _Z, C.POINTER(_Z)]),
+    "fzp_write_file": (_I, [_P, C.c_char_p]),
+    "fzp_decompress_file": (_I, [C.c_char_p, _P, C.POINTER(_P), C.POINTER(_Z)]),
+    "fzp_last_perf": (_I, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_Z), C.POINTER(_Z)]),
+    "fzp_last_stage_perf": (_I, [_P, _I, C.c_char_p, _Z, C.POINTER(_F)]),
+    "fzp_num_outputs": (_I, [_P, C.POINTER(_I)]),
+    "fzp_output_info": (_I, [_P, _I, C.c_char_p, _Z, C.POINTER(_Z)]),
+    "fzp_stage_abs_error_bound": (_I, [_P, _I, C.POINTER(_F)]),
     "fzh_device_alloc": (_I, [C.POINTER(_P), _Z]),
     "fzh_device_free": (_I, [_P]),
     "fzh_memcpy_htod": (_I, [_P, _P, _Z]),

@@ -57,6 +57,82 @@ __global__ void __launch_bounds__(256) gather_kernel(const fzh_copy_desc* __rest
     }
 }
 
+struct ConcatArgs {
+    fzh_concat_slot slot[FZH_CONCAT_MAX_SLOTS];
+};
+struct ReadbackArgs {
+    fzh_readback_entry e[FZH_READBACK_MAX];
+};
+
+__device__ __forceinline__ uint64_t slot_size(const fzh_concat_slot& s) {
+    uint64_t v = s.size;
+    if (s.kind == 1) v = *reinterpret_cast<const uint64_t*>(s.size_ptr);
+    else if (s.kind == 2) v = (uint64_t)*reinterpret_cast<const uint32_t*>(s.size_ptr) * s.elem;
+    return v < s.cap ? v : s.cap;
+}
+
+// One wave: sizes, 16-byte-aligned offsets, the header bytes and the total.
+// ws: u64 off[n] then u64 size[n].
+__global__ void __launch_bounds__(64)
+concat_header_kernel(ConcatArgs a, int n, uint8_t* __restrict__ out, uint64_t* __restrict__ total,
+                     uint64_t* __restrict__ ws) {
+    const int l = threadIdx.x;
+    const uint64_t hdr = (4 + 8 * (uint64_t)n + 15) & ~uint64_t(15);
+    const uint64_t sz = l < n ? slot_size(a.slot[l]) : 0;
+    const uint64_t padded = (sz + 15) & ~uint64_t(15);
+    uint64_t incl = padded;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t t = __shfl_up(incl, off, 64);
+        if (l >= off) incl += t;
+    }
+    if (l < n) {
+        ws[l] = hdr + incl - padded;
+        ws[FZH_CONCAT_MAX_SLOTS + l] = sz;
+        uint32_t* w = reinterpret_cast<uint32_t*>(out + 4 + 8 * l);
+        w[0] = (uint32_t)sz;
+        w[1] = (uint32_t)(sz >> 32);
+    }
+    if (l == 0) *reinterpret_cast<uint32_t*>(out) = (uint32_t)n;
+    // zero the header pad
+    for (uint64_t b = 4 + 8 * (uint64_t)n + l; b < hdr; b += 64) out[b] = 0;
+    if (l == 63) *total = hdr + incl;
+}
+
+// blockIdx.y = slot, blockIdx.x = 16 KiB piece; pieces past the slot's
+// padded size exit.  Pad bytes of the last 16-byte word are written as 0.
+__global__ void __launch_bounds__(256)
+concat_copy_kernel(ConcatArgs a, uint8_t* __restrict__ out, const uint64_t* __restrict__ ws) {
+    const int sidx = blockIdx.y;
+    const uint64_t sz = ws[FZH_CONCAT_MAX_SLOTS + sidx];
+    const uint64_t nw = (sz + 15) / 16;
+    const uint64_t w0 = (uint64_t)blockIdx.x * 1024;
+    if (w0 >= nw) return;
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(a.slot[sidx].src);
+    uint4* dst = reinterpret_cast<uint4*>(out + ws[sidx]);
+    const uint64_t full = sz / 16;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint64_t w = w0 + threadIdx.x + 256 * k;
+        if (w < full) {
+            dst[w] = reinterpret_cast<const uint4*>(src)[w];
+        } else if (w < nw) {
+            uint32_t v[4] = {0, 0, 0, 0};
+            for (uint64_t b = 16 * w; b < sz; b++) v[(b & 15) >> 2] |= (uint32_t)src[b] << (8 * (b & 3));
+            dst[w] = make_uint4(v[0], v[1], v[2], v[3]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) readback_kernel(ReadbackArgs a, int n, uint8_t* __restrict__ dst) {
+    for (int i = 0; i < n; i++) {
+        const uint32_t words = a.e[i].bytes / 4;
+        const uint32_t* s = reinterpret_cast<const uint32_t*>(a.e[i].src);
+        uint32_t* d = reinterpret_cast<uint32_t*>(dst + a.e[i].dst_offset);
+        for (uint32_t w = threadIdx.x; w < words; w += 64) d[w] = s[w];
+    }
+}
+
 }  // namespace fzh
 
 using namespace fzh;
@@ -75,6 +151,62 @@ int fzh_gather(void* stream, const fzh_copy_desc* d_descs, int n_segs) {
     if (n_segs <= 0) return FZH_OK;
     gather_kernel<<<n_segs, 256, 0, S(stream)>>>(d_descs, n_segs);
     FZH_CHECK_LAUNCH("gather_kernel");
+    return FZH_OK;
+}
+
+size_t fzh_concat_ws_bytes(void) { return 2 * FZH_CONCAT_MAX_SLOTS * sizeof(uint64_t); }
+
+uint64_t fzh_concat_bound(const fzh_concat_slot* slots, int n) {
+    uint64_t t = (4 + 8 * (uint64_t)n + 15) & ~uint64_t(15);
+    for (int i = 0; i < n; i++) t += (slots[i].cap + 15) & ~uint64_t(15);
+    return t;
+}
+
+int fzh_concat(void* stream, const fzh_concat_slot* slots, int n, void* d_out, uint64_t out_cap,
+               uint64_t* d_total, void* d_ws, size_t ws_bytes) {
+    if (n < 0 || n > FZH_CONCAT_MAX_SLOTS) return fail(FZH_E_INVALID, "concat: 0..32 slots");
+    if (!d_out || !d_total || !d_ws) return fail(FZH_E_INVALID, "concat: null pointer");
+    if (ws_bytes < fzh_concat_ws_bytes()) return fail(FZH_E_CAPACITY, "concat: workspace too small");
+    if (reinterpret_cast<uintptr_t>(d_out) % 16 != 0 || reinterpret_cast<uintptr_t>(d_ws) % 8 != 0)
+        return fail(FZH_E_INVALID, "concat: output must be 16-byte aligned");
+    ConcatArgs a;
+    std::memset(&a, 0, sizeof(a));
+    uint64_t max_words = 0;
+    for (int i = 0; i < n; i++) {
+        const fzh_concat_slot& s = slots[i];
+        if (s.kind > 2) return fail(FZH_E_INVALID, "concat: slot kind must be 0, 1 or 2");
+        if (s.kind != 0 && !s.size_ptr) return fail(FZH_E_INVALID, "concat: null size pointer");
+        if (s.cap > 0 && (!s.src || reinterpret_cast<uintptr_t>(s.src) % 16 != 0))
+            return fail(FZH_E_INVALID, "concat: slot sources must be 16-byte aligned");
+        if (s.kind == 0 && s.size > s.cap) return fail(FZH_E_INVALID, "concat: slot size exceeds its capacity");
+        a.slot[i] = s;
+        max_words = std::max<uint64_t>(max_words, (s.cap + 15) / 16);
+    }
+    if (fzh_concat_bound(slots, n) > out_cap) return fail(FZH_E_CAPACITY, "concat: output buffer too small");
+    concat_header_kernel<<<1, 64, 0, S(stream)>>>(a, n, (uint8_t*)d_out, d_total, (uint64_t*)d_ws);
+    if (n > 0 && max_words > 0) {
+        const dim3 grid((unsigned)cdiv(max_words, 1024), (unsigned)n);
+        concat_copy_kernel<<<grid, 256, 0, S(stream)>>>(a, (uint8_t*)d_out, (const uint64_t*)d_ws);
+    }
+    FZH_CHECK_LAUNCH("concat");
+    return FZH_OK;
+}
+
+int fzh_readback(void* stream, const fzh_readback_entry* entries, int n, void* d_mapped_dst) {
+    if (n <= 0) return FZH_OK;
+    if (n > FZH_READBACK_MAX) return fail(FZH_E_INVALID, "readback: at most 32 entries");
+    if (!d_mapped_dst || reinterpret_cast<uintptr_t>(d_mapped_dst) % 4 != 0)
+        return fail(FZH_E_INVALID, "readback: destination must be 4-byte aligned");
+    ReadbackArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (int i = 0; i < n; i++) {
+        const fzh_readback_entry& e = entries[i];
+        if (!e.src || e.bytes % 4 != 0 || e.dst_offset % 4 != 0 || reinterpret_cast<uintptr_t>(e.src) % 4 != 0)
+            return fail(FZH_E_INVALID, "readback: entries must be non-null 4-byte multiples");
+        a.e[i] = e;
+    }
+    readback_kernel<<<1, 64, 0, S(stream)>>>(a, n, (uint8_t*)d_mapped_dst);
+    FZH_CHECK_LAUNCH("readback_kernel");
     return FZH_OK;
 }
 

@@ -1,0 +1,671 @@
+// host_stages.cpp — fz:: stage classes on top of the fzh_* kernels.
+//
+// Each execute() enqueues kernels only; values the host needs afterwards
+// (outlier counts, stream sizes, error status, resolved REL/NOA bounds) are
+// read in postStreamSync(), after the pipeline's one stream synchronisation.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "fz/stages.h"
+#include "fzh_internal.h"
+
+namespace fz {
+
+void throwIfFzh(int rc, const char* what) {
+    if (rc != FZH_OK) throw std::runtime_error(std::string(what) + ": " + fzh_last_error());
+}
+
+#define FZ_HIP(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+void DeviceScratch::ensure(size_t need, MemoryPool* p, hipStream_t s) {
+    need = std::max<size_t>(need, 256);
+    if (ptr && bytes >= need) return;
+    release();
+    pool = p;
+    if (pool) ptr = pool->allocatePersistentDevice(need, "stage scratch");
+    else FZ_HIP(hipMalloc(&ptr, need));
+    if (!ptr) throw std::runtime_error("stage scratch allocation failed");
+    bytes = need;
+    throwIfFzh(fzh_workspace_init(s, ptr, bytes), "workspace init");
+}
+
+void DeviceScratch::release() {
+    if (!ptr) return;
+    if (pool) pool->freePersistentDevice(ptr);
+    else (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+// ===========================================================================
+// LorenzoQuantStage
+// ===========================================================================
+template <typename TI, typename TC>
+LorenzoQuantStage<TI, TC>::LorenzoQuantStage(const Config& c)
+    : config_(c), computed_abs_eb_(c.error_bound), computed_value_base_(c.precomputed_value_base) {}
+
+template <typename TI, typename TC>
+void LorenzoQuantStage<TI, TC>::resolveDims(size_t n, size_t& nx, size_t& ny, size_t& nz) const {
+    const int nd = ndim();
+    ny = nd >= 2 ? config_.dims[1] : 1;
+    nz = nd >= 3 ? config_.dims[2] : 1;
+    nx = config_.dims[0] > 0 ? config_.dims[0] : n / (ny * nz);
+    if (nd == 1) nx = n;
+}
+
+template <typename TI, typename TC>
+std::vector<size_t> LorenzoQuantStage<TI, TC>::estimateOutputSizes(const std::vector<size_t>& in) const {
+    if (is_inverse_) return {num_elements_ * sizeof(TI)};
+    const size_t n = in.empty() ? 0 : in[0] / sizeof(TI);
+    const size_t m = maxOutliers(n);
+    return {n * sizeof(TC), m * sizeof(TI), m * sizeof(uint32_t), sizeof(uint32_t)};
+}
+
+template <typename TI, typename TC>
+std::unordered_map<std::string, size_t> LorenzoQuantStage<TI, TC>::getActualOutputSizesByName() const {
+    std::unordered_map<std::string, size_t> r;
+    const auto names = getOutputNames();
+    for (size_t i = 0; i < names.size() && i < actual_output_sizes_.size(); i++) r[names[i]] = actual_output_sizes_[i];
+    return r;
+}
+
+template <typename TI, typename TC>
+void LorenzoQuantStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                                        const std::vector<void*>& outputs, const std::vector<size_t>& sizes) {
+    if (is_inverse_) {
+        if (inputs.size() < 4 || outputs.empty() || sizes.size() < 4)
+            throw std::runtime_error("LorenzoQuantStage (inverse): Requires 4 inputs and 1 output");
+        const size_t n = sizes[0] / sizeof(TC);
+        actual_output_sizes_.assign(1, n * sizeof(TI));
+        if (n == 0) return;
+        size_t nx, ny, nz;
+        resolveDims(n, nx, ny, nz);
+        const size_t max_out = sizes[1] / sizeof(TI);
+        const float ebx2 = 2.0f * computed_abs_eb_;
+        ws_.ensure(fzh_lorenzo_quant_inv_ws_bytes(n, nx, ny, nz, max_out), pool, stream);
+        throwIfFzh(fzh_lorenzo_quant_inv_f32(stream, inputs[0], n, nx, ny, nz, (const float*)inputs[1],
+                                             (const uint32_t*)inputs[2], (const uint32_t*)inputs[3], max_out, ebx2,
+                                             config_.zigzag_codes ? 1 : 0, (int)sizeof(TC), (float*)outputs[0],
+                                             ws_.ptr, ws_.bytes),
+                   "LorenzoQuantStage (inverse)");
+        return;
+    }
+    if (inputs.empty() || outputs.size() < 4 || sizes.empty())
+        throw std::runtime_error("LorenzoQuantStage: Requires 1 input and 4 outputs");
+    const size_t n = sizes[0] / sizeof(TI);
+    const size_t valid = (valid_elems_ > 0 && valid_elems_ < n) ? valid_elems_ : n;
+    valid_elems_ = 0;
+    num_elements_ = n;
+    const size_t max_out = maxOutliers(n);
+    actual_outlier_count_ = 0;
+    clearReadbacks();
+    d_count_ptr_ = nullptr;
+    if (n == 0) {
+        actual_output_sizes_.assign(4, 0);
+        return;
+    }
+    size_t nx, ny, nz;
+    resolveDims(n, nx, ny, nz);
+    if (ndim() == 1) nx = valid;
+    else if (nx * ny * nz > valid)
+        throw std::runtime_error("LorenzoQuantStage: dims exceed the unpadded input");
+    // Resolve the absolute bound (lorenzo_quant.cu:612-644).
+    const float* d_r = nullptr;
+    float r = 0.0f;
+    params_pending_ = false;
+    if (config_.eb_mode == ErrorBoundMode::ABS) {
+        computed_abs_eb_ = config_.error_bound;
+        computed_value_base_ = 0.0f;
+        r = 1.0f / (2.0f * computed_abs_eb_);
+    } else if (config_.precomputed_value_base > 0.0f) {
+        computed_value_base_ = config_.precomputed_value_base;
+        computed_abs_eb_ = config_.error_bound * computed_value_base_;
+        r = 1.0f / (2.0f * computed_abs_eb_);
+    } else {
+        // value base reduced on the device; no host round trip before the kernel
+        params_.ensure(256 + fzh_value_base_ws_bytes(n), pool, stream);
+        float* d_params = (float*)params_.ptr;
+        const int mode = (config_.eb_mode == ErrorBoundMode::NOA ? 2 : 1) | (valid < n ? FZH_VB_ZERO_PADDED : 0);
+        throwIfFzh(fzh_value_base_f32(stream, (const float*)inputs[0], valid, mode, config_.error_bound, d_params,
+                                      (uint8_t*)params_.ptr + 256, params_.bytes - 256),
+                   "LorenzoQuantStage value base");
+        d_r = d_params + 2;
+        params_pending_ = true;
+        addReadback(d_params, h_params_, sizeof(h_params_));
+    }
+    ws_.ensure(fzh_lorenzo_quant_fwd_ws_bytes(n, nx, ny, nz), pool, stream);
+    throwIfFzh(fzh_lorenzo_quant_fwd_f32(stream, (const float*)inputs[0], n, nx, ny, nz, r, d_r,
+                                         (uint32_t)config_.quant_radius, config_.zigzag_codes ? 1 : 0, (int)sizeof(TC),
+                                         outputs[0], (float*)outputs[1], (uint32_t*)outputs[2], (uint32_t*)outputs[3],
+                                         max_out, ws_.ptr, ws_.bytes),
+               "LorenzoQuantStage");
+    d_count_ptr_ = outputs[3];
+    addReadback(d_count_ptr_, &h_count_, 4);
+    actual_output_sizes_ = {n * sizeof(TC), max_out * sizeof(TI), max_out * sizeof(uint32_t), sizeof(uint32_t)};
+}
+
+template <typename TI, typename TC>
+void LorenzoQuantStage<TI, TC>::postStreamSync(hipStream_t) {
+    if (is_inverse_ || !d_count_ptr_) return;
+    completeReadbacks();
+    if (params_pending_) {
+        computed_value_base_ = h_params_[0];
+        computed_abs_eb_ = h_params_[1];
+    }
+    uint32_t cnt = h_count_;
+    if (cnt == 0xFFFFFFFFu) throw std::runtime_error("LorenzoQuantStage: ordered scan failed (workspace state)");
+    const size_t max_out = maxOutliers(num_elements_);
+    if (cnt > max_out) {
+        if (config_.outlier_capacity != 0.0f)
+            std::fprintf(stderr,
+                         "[fz] LorenzoQuant outlier overflow: %u outliers, capacity %zu; the excess was dropped\n",
+                         cnt, max_out);
+        cnt = (uint32_t)max_out;
+    }
+    actual_outlier_count_ = cnt;
+    actual_output_sizes_[1] = (size_t)cnt * sizeof(TI);
+    actual_output_sizes_[2] = (size_t)cnt * sizeof(uint32_t);
+}
+
+template <typename TI, typename TC>
+size_t LorenzoQuantStage<TI, TC>::serializeHeader(size_t, uint8_t* buf, size_t max_size) const {
+    if (max_size < sizeof(LorenzoQuantConfig)) throw std::runtime_error("Insufficient buffer for Lorenzo config");
+    LorenzoQuantConfig c;
+    c.error_bound = computed_abs_eb_;
+    c.quant_radius = (uint32_t)config_.quant_radius;
+    c.num_elements = (uint32_t)num_elements_;
+    c.outlier_count = actual_outlier_count_;
+    c.input_type = DataType::FLOAT32;
+    c.code_type = dataTypeOf<TC>();
+    c.ndim = (uint8_t)ndim();
+    c.eb_mode = (uint8_t)config_.eb_mode;
+    c.dim_x = (uint32_t)config_.dims[0];
+    c.dim_y = (uint32_t)config_.dims[1];
+    c.dim_z = (uint32_t)config_.dims[2];
+    c.user_eb = config_.error_bound;
+    c.value_base = computed_value_base_;
+    c.zigzag_codes = config_.zigzag_codes ? 1 : 0;
+    std::memcpy(buf, &c, sizeof(c));
+    return sizeof(c);
+}
+
+// Accepts the 32/40/44-byte layouts (lorenzo_quant.h:243-298).
+template <typename TI, typename TC>
+void LorenzoQuantStage<TI, TC>::deserializeHeader(const uint8_t* buf, size_t size) {
+    if (size < 32) throw std::runtime_error("Invalid Lorenzo config size");
+    LorenzoQuantConfig c;
+    std::memcpy(&c, buf, std::min(size, sizeof(LorenzoQuantConfig)));
+    config_.error_bound = c.error_bound;
+    computed_abs_eb_ = c.error_bound;
+    config_.quant_radius = (TC)c.quant_radius;
+    num_elements_ = c.num_elements;
+    actual_outlier_count_ = c.outlier_count;
+    if (size >= 40) {
+        config_.eb_mode = (ErrorBoundMode)c.eb_mode;
+        config_.precomputed_value_base = c.value_base;
+        computed_value_base_ = c.value_base;
+    } else {
+        config_.eb_mode = ErrorBoundMode::ABS;
+        config_.precomputed_value_base = 0.0f;
+        computed_value_base_ = 0.0f;
+    }
+    config_.zigzag_codes = size >= sizeof(LorenzoQuantConfig) ? c.zigzag_codes != 0 : false;
+    const int nd = c.ndim == 0 ? 1 : c.ndim;
+    if (c.dim_x > 0) config_.dims[0] = c.dim_x;
+    else if (c.num_elements > 0) config_.dims[0] = c.num_elements / (std::max<size_t>(1, c.dim_y) * std::max<size_t>(1, c.dim_z));
+    else config_.dims[0] = 0;
+    config_.dims[1] = nd >= 2 ? (c.dim_y > 0 ? c.dim_y : 1) : 1;
+    config_.dims[2] = nd >= 3 ? (c.dim_z > 0 ? c.dim_z : 1) : 1;
+}
+
+template class LorenzoQuantStage<float, uint16_t>;
+template class LorenzoQuantStage<float, uint8_t>;
+
+// ===========================================================================
+// Bitshuffle
+// ===========================================================================
+void BitshuffleStage::execute(hipStream_t stream, MemoryPool*, const std::vector<void*>& inputs,
+                              const std::vector<void*>& outputs, const std::vector<size_t>& sizes) {
+    if (inputs.empty() || outputs.empty() || sizes.empty()) throw std::runtime_error("BitshuffleStage: invalid inputs/outputs");
+    if (element_width_ != 1 && element_width_ != 2 && element_width_ != 4 && element_width_ != 8)
+        throw std::invalid_argument("BitshuffleStage: element_width must be 1, 2, 4, or 8");
+    if (block_size_ == 0 || block_size_ % (1024u * element_width_) != 0)
+        throw std::invalid_argument(
+            "BitshuffleStage: block_size must be a positive multiple of 1024 * element_width");
+    actual_output_size_ = sizes[0];
+    if (sizes[0] == 0) return;
+    const int rc = is_inverse_ ? fzh_bitshuffle_dec(stream, inputs[0], outputs[0], sizes[0], block_size_, element_width_)
+                               : fzh_bitshuffle_enc(stream, inputs[0], outputs[0], sizes[0], block_size_, element_width_);
+    throwIfFzh(rc, "BitshuffleStage");
+}
+
+// ===========================================================================
+// RZE
+// ===========================================================================
+std::vector<size_t> RZEStage::estimateOutputSizes(const std::vector<size_t>& in) const {
+    if (is_inverse_) return {cached_orig_bytes_ ? (size_t)cached_orig_bytes_ : (in.empty() ? 0 : in[0])};
+    return {fzh_rze_enc_bound(in.empty() ? 0 : in[0])};
+}
+
+void RZEStage::execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                       const std::vector<void*>& outputs, const std::vector<size_t>& sizes) {
+    if (inputs.empty() || outputs.empty() || sizes.empty()) throw std::runtime_error("RZEStage: invalid inputs/outputs");
+    if (chunk_size_ != 16384)
+        throw std::runtime_error("RZEStage: only chunk_size == 16384 is currently supported; got " + std::to_string(chunk_size_));
+    if (levels_ != 4)
+        throw std::runtime_error("RZEStage: only levels == 4 is currently supported; got " + std::to_string((int)levels_));
+    small_.ensure(256, pool, stream);
+    clearReadbacks();
+    size_pending_ = status_pending_ = false;
+    if (!is_inverse_) {
+        const size_t in_bytes = sizes[0];
+        if (in_bytes == 0) { actual_output_size_ = 0; return; }
+        cached_orig_bytes_ = (uint32_t)in_bytes;
+        ws_.ensure(fzh_rze_enc_ws_bytes(in_bytes), pool, stream);
+        throwIfFzh(fzh_rze_enc(stream, inputs[0], in_bytes, outputs[0], fzh_rze_enc_bound(in_bytes), (uint64_t*)small_.ptr,
+                               ws_.ptr, ws_.bytes),
+                   "RZEStage");
+        actual_output_size_ = fzh_rze_enc_bound(in_bytes);
+        size_pending_ = true;
+        addReadback(small_.ptr, &h_size_, 8);
+    } else {
+        const size_t orig = cached_orig_bytes_;
+        if (orig == 0) { actual_output_size_ = 0; return; }
+        ws_.ensure(fzh_rze_dec_ws_bytes(orig), pool, stream);
+        throwIfFzh(fzh_rze_dec(stream, inputs[0], sizes[0], outputs[0], orig, (int*)((uint8_t*)small_.ptr + 8), ws_.ptr,
+                               ws_.bytes),
+                   "RZEStage (inverse)");
+        actual_output_size_ = orig;
+        status_pending_ = true;
+        addReadback((uint8_t*)small_.ptr + 8, &h_status_, 4);
+    }
+}
+
+void RZEStage::postStreamSync(hipStream_t) {
+    if (!size_pending_ && !status_pending_) return;
+    completeReadbacks();
+    if (size_pending_) {
+        if (h_size_ == 0) throw std::runtime_error("RZEStage: ordered scan failed (workspace state)");
+        actual_output_size_ = h_size_;
+    }
+    if (status_pending_) {
+        const int st = h_status_;
+        if (st != 0) {
+            const int zero = 0;
+            FZ_HIP(hipMemcpy((uint8_t*)small_.ptr + 8, &zero, 4, hipMemcpyHostToDevice));
+            throw std::runtime_error("RZEStage (inverse): malformed or mismatching RZE stream");
+        }
+    }
+}
+
+// ===========================================================================
+// RLE
+// ===========================================================================
+template <typename T>
+void RLEStage<T>::execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                          const std::vector<void*>& outputs, const std::vector<size_t>& sizes) {
+    if (inputs.empty() || outputs.empty() || sizes.empty()) throw std::runtime_error("RLEStage: Invalid inputs/outputs");
+    small_.ensure(256, pool, stream);
+    clearReadbacks();
+    size_pending_ = status_pending_ = false;
+    if (!is_inverse_) {
+        const size_t n = sizes[0] / sizeof(T);
+        cached_num_elements_ = (uint32_t)n;
+        ws_.ensure(fzh_rle_enc_ws_bytes(n), pool, stream);
+        throwIfFzh(fzh_rle_enc(stream, inputs[0], n, (int)sizeof(T), outputs[0], fzh_rle_enc_bound(n, sizeof(T)),
+                               (uint64_t*)small_.ptr, ws_.ptr, ws_.bytes),
+                   "RLEStage");
+        actual_output_size_ = fzh_rle_enc_bound(n, sizeof(T));
+        size_pending_ = true;
+        addReadback(small_.ptr, &h_size_, 8);
+    } else {
+        const size_t n = cached_num_elements_;
+        actual_output_size_ = n * sizeof(T);
+        if (n == 0) return;
+        ws_.ensure(fzh_rle_dec_ws_bytes(n), pool, stream);
+        throwIfFzh(fzh_rle_dec(stream, inputs[0], sizes[0], (int)sizeof(T), outputs[0], n,
+                               (int*)((uint8_t*)small_.ptr + 8), ws_.ptr, ws_.bytes),
+                   "RLEStage (inverse)");
+        status_pending_ = true;
+        addReadback((uint8_t*)small_.ptr + 8, &h_status_, 4);
+    }
+}
+
+template <typename T>
+void RLEStage<T>::postStreamSync(hipStream_t) {
+    if (!size_pending_ && !status_pending_) return;
+    completeReadbacks();
+    if (size_pending_) {
+        if (h_size_ == 0) throw std::runtime_error("RLEStage: ordered scan failed (workspace state)");
+        actual_output_size_ = h_size_;
+    }
+    if (status_pending_) {
+        const int st = h_status_;
+        if (st != 0) {
+            const int zero = 0;
+            FZ_HIP(hipMemcpy((uint8_t*)small_.ptr + 8, &zero, 4, hipMemcpyHostToDevice));
+            throw std::runtime_error("RLEStage (inverse): malformed RLE stream or wrong element count");
+        }
+    }
+}
+
+template class RLEStage<uint8_t>;
+template class RLEStage<uint16_t>;
+template class RLEStage<uint32_t>;
+template class RLEStage<int32_t>;
+
+// ===========================================================================
+// Huffman
+// ===========================================================================
+template <typename T>
+std::vector<size_t> HuffmanStage<T>::estimateOutputSizes(const std::vector<size_t>& in) const {
+    if (is_inverse_) return {(size_t)original_len_ * sizeof(T)};
+    const size_t n = in.empty() ? 0 : in[0] / sizeof(T);
+    return {fzh_huffman_enc_bound(n, bklen_, sizeof(T))};
+}
+
+template <typename T>
+void HuffmanStage<T>::execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                              const std::vector<void*>& outputs, const std::vector<size_t>& sizes) {
+    if (inputs.empty() || outputs.empty() || sizes.empty())
+        throw std::runtime_error("HuffmanStage: inputs, outputs, and sizes must be non-empty");
+    clearReadbacks();
+    size_pending_ = status_pending_ = false;
+    if (sizes[0] == 0) { actual_output_size_ = 0; return; }
+    small_.ensure(256, pool, stream);
+    if (!is_inverse_) {
+        const size_t n = sizes[0] / sizeof(T);
+        freq_.ensure(4 * (size_t)bklen_, pool, stream);
+        throwIfFzh(fzh_histogram(stream, inputs[0], n, (int)sizeof(T), bklen_, (uint32_t*)freq_.ptr), "HuffmanStage histogram");
+        std::vector<uint32_t> h(bklen_);
+        FZ_HIP(hipMemcpyAsync(h.data(), freq_.ptr, 4 * (size_t)bklen_, hipMemcpyDeviceToHost, stream));
+        FZ_HIP(hipStreamSynchronize(stream));
+        uint64_t sum = 0;
+        for (uint32_t v : h) sum += v;
+        if (sum != (uint64_t)n)
+            throw std::runtime_error("HuffmanStage: " + std::to_string(n - sum) +
+                                     " out-of-range symbol(s) detected; all symbols must be in [0, " +
+                                     std::to_string(bklen_) +
+                                     "). Increase bklen or use setZigzagCodes(true) with LorenzoQuantStage.");
+        const size_t rev_bytes = fzh_huffman_revbook_bytes(bklen_, sizeof(T));
+        std::vector<uint8_t> book(4 * (size_t)bklen_ + rev_bytes);
+        throwIfFzh(fzh_huffman_build_book(h.data(), bklen_, (int)sizeof(T), (uint32_t*)book.data(),
+                                          book.data() + 4 * (size_t)bklen_),
+                   "HuffmanStage codebook");
+        book_.ensure(book.size(), pool, stream);
+        FZ_HIP(hipMemcpyAsync(book_.ptr, book.data(), book.size(), hipMemcpyHostToDevice, stream));
+        ws_.ensure(fzh_huffman_enc_ws_bytes(n), pool, stream);
+        throwIfFzh(fzh_huffman_encode(stream, inputs[0], n, (int)sizeof(T), bklen_, (const uint32_t*)book_.ptr,
+                                      (const uint8_t*)book_.ptr + 4 * (size_t)bklen_, outputs[0],
+                                      fzh_huffman_enc_bound(n, bklen_, sizeof(T)), (uint64_t*)small_.ptr, ws_.ptr,
+                                      ws_.bytes),
+                   "HuffmanStage encode");
+        // keep the host copy alive until the copy has been consumed
+        FZ_HIP(hipStreamSynchronize(stream));
+        original_len_ = n;
+        actual_output_size_ = fzh_huffman_enc_bound(n, bklen_, sizeof(T));
+        size_pending_ = true;
+        addReadback(small_.ptr, &h_size_, 8);
+    } else {
+        if (original_len_ == 0)
+            throw std::runtime_error("HuffmanStage: inverse called with original_len_=0; call deserializeHeader() before decompressing");
+        throwIfFzh(fzh_huffman_decode(stream, inputs[0], sizes[0], (int)sizeof(T), outputs[0], original_len_,
+                                      (int*)((uint8_t*)small_.ptr + 8)),
+                   "HuffmanStage decode");
+        actual_output_size_ = (size_t)original_len_ * sizeof(T);
+        status_pending_ = true;
+        addReadback((uint8_t*)small_.ptr + 8, &h_status_, 4);
+    }
+}
+
+template <typename T>
+void HuffmanStage<T>::postStreamSync(hipStream_t) {
+    if (!size_pending_ && !status_pending_) return;
+    completeReadbacks();
+    if (size_pending_) {
+        if (h_size_ == 0) throw std::runtime_error("HuffmanStage: ordered scan failed (workspace state)");
+        actual_output_size_ = h_size_;
+    }
+    if (status_pending_) {
+        const int st = h_status_;
+        if (st != 0) {
+            const int zero = 0;
+            FZ_HIP(hipMemcpy((uint8_t*)small_.ptr + 8, &zero, 4, hipMemcpyHostToDevice));
+            throw std::runtime_error("HuffmanStage (inverse): malformed Huffman stream");
+        }
+    }
+}
+
+template class HuffmanStage<uint8_t>;
+template class HuffmanStage<uint16_t>;
+template class HuffmanStage<uint32_t>;
+
+// ===========================================================================
+// Quantizer (ABS / NOA)
+// ===========================================================================
+template <typename TI, typename TC>
+std::vector<size_t> QuantizerStage<TI, TC>::estimateOutputSizes(const std::vector<size_t>& in) const {
+    if (is_inverse_) return {num_elements_ * sizeof(TI)};
+    const size_t n = in.empty() ? 0 : in[0] / sizeof(TI);
+    const size_t m = maxOutliers(n);
+    return {n * sizeof(TC), m * sizeof(TI), m * 4, 4};
+}
+
+template <typename TI, typename TC>
+std::unordered_map<std::string, size_t> QuantizerStage<TI, TC>::getActualOutputSizesByName() const {
+    std::unordered_map<std::string, size_t> r;
+    const auto names = getOutputNames();
+    for (size_t i = 0; i < names.size() && i < actual_output_sizes_.size(); i++) r[names[i]] = actual_output_sizes_[i];
+    return r;
+}
+
+template <typename TI, typename TC>
+void QuantizerStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                                     const std::vector<void*>& outputs, const std::vector<size_t>& sizes) {
+    if (config_.eb_mode == ErrorBoundMode::REL)
+        throw std::runtime_error("QuantizerStage: REL (log-space) mode is not provided by this build");
+    if (is_inverse_) {
+        if (inputs.size() < 4 || outputs.empty() || sizes.size() < 4)
+            throw std::runtime_error("QuantizerStage (inverse): requires 4 inputs");
+        const size_t n = sizes[0] / sizeof(TC);
+        actual_output_sizes_.assign(1, n * sizeof(TI));
+        throwIfFzh(fzh_quant_abs_inv_f32(stream, inputs[0], n, 2.0f * computed_abs_eb_, config_.zigzag_codes ? 1 : 0,
+                                         (int)sizeof(TC), (const float*)inputs[1], (const uint32_t*)inputs[2],
+                                         (const uint32_t*)inputs[3], sizes[1] / sizeof(TI), (float*)outputs[0]),
+                   "QuantizerStage (inverse)");
+        return;
+    }
+    if (inputs.empty() || outputs.size() < 4 || sizes.empty())
+        throw std::runtime_error("QuantizerStage: requires 1 input and 4 outputs");
+    const size_t n = sizes[0] / sizeof(TI);
+    num_elements_ = n;
+    const size_t m = maxOutliers(n);
+    clearReadbacks();
+    d_count_ptr_ = nullptr;
+    if (n == 0) { actual_output_sizes_.assign(4, 0); return; }
+    if (config_.eb_mode == ErrorBoundMode::ABS) {
+        computed_abs_eb_ = config_.error_bound;
+        computed_value_base_ = 0.0f;
+    } else {
+        float vb = config_.precomputed_value_base;
+        if (vb <= 0.0f) {
+            ws_.ensure(256 + fzh_value_base_ws_bytes(n), pool, stream);
+            throwIfFzh(fzh_value_base_f32(stream, (const float*)inputs[0], n, 2, config_.error_bound, (float*)ws_.ptr,
+                                          (uint8_t*)ws_.ptr + 256, ws_.bytes - 256),
+                       "QuantizerStage value range");
+            float p[4];
+            FZ_HIP(hipMemcpyAsync(p, ws_.ptr, 16, hipMemcpyDeviceToHost, stream));
+            FZ_HIP(hipStreamSynchronize(stream));
+            vb = p[0];
+        }
+        computed_value_base_ = vb;
+        computed_abs_eb_ = vb > 0.0f ? config_.error_bound * vb : config_.error_bound;
+    }
+    const float r = 1.0f / (2.0f * computed_abs_eb_);
+    ws_.ensure(256 + std::max(fzh_quant_abs_fwd_ws_bytes(n), fzh_value_base_ws_bytes(n)), pool, stream);
+    throwIfFzh(fzh_quant_abs_fwd_f32(stream, (const float*)inputs[0], n, r, config_.outlier_threshold,
+                                     (uint32_t)config_.quant_radius, config_.zigzag_codes ? 1 : 0, (int)sizeof(TC),
+                                     outputs[0], (float*)outputs[1], (uint32_t*)outputs[2], (uint32_t*)outputs[3], m,
+                                     (uint8_t*)ws_.ptr + 256, ws_.bytes - 256),
+               "QuantizerStage");
+    d_count_ptr_ = outputs[3];
+    addReadback(d_count_ptr_, &h_count_, 4);
+    actual_output_sizes_ = {n * sizeof(TC), m * sizeof(TI), m * 4, 4};
+}
+
+template <typename TI, typename TC>
+void QuantizerStage<TI, TC>::postStreamSync(hipStream_t) {
+    if (is_inverse_ || !d_count_ptr_) return;
+    completeReadbacks();
+    uint32_t cnt = h_count_;
+    if (cnt == 0xFFFFFFFFu) throw std::runtime_error("QuantizerStage: ordered scan failed (workspace state)");
+    cnt = (uint32_t)std::min<size_t>(cnt, maxOutliers(num_elements_));
+    outlier_count_ = cnt;
+    actual_output_sizes_[1] = (size_t)cnt * sizeof(TI);
+    actual_output_sizes_[2] = (size_t)cnt * 4;
+}
+
+template <typename TI, typename TC>
+size_t QuantizerStage<TI, TC>::serializeHeader(size_t, uint8_t* buf, size_t max_size) const {
+    if (max_size < sizeof(QuantizerConfig)) throw std::runtime_error("Insufficient buffer for Quantizer config");
+    QuantizerConfig c;
+    c.abs_error_bound = computed_abs_eb_;
+    c.user_error_bound = config_.error_bound;
+    c.value_base = computed_value_base_;
+    c.quant_radius = (uint32_t)config_.quant_radius;
+    c.num_elements = (uint32_t)num_elements_;
+    c.outlier_count = outlier_count_;
+    c.input_type = DataType::FLOAT32;
+    c.code_type = dataTypeOf<TC>();
+    c.eb_mode = (uint8_t)config_.eb_mode;
+    c.zigzag_codes = config_.zigzag_codes ? 1 : 0;
+    c.outlier_threshold = config_.outlier_threshold;
+    std::memcpy(buf, &c, sizeof(c));
+    return sizeof(c);
+}
+
+template <typename TI, typename TC>
+void QuantizerStage<TI, TC>::deserializeHeader(const uint8_t* buf, size_t size) {
+    QuantizerConfig c;
+    std::memcpy(&c, buf, std::min(size, sizeof(c)));
+    computed_abs_eb_ = c.abs_error_bound;
+    config_.error_bound = c.user_error_bound;
+    computed_value_base_ = c.value_base;
+    config_.quant_radius = (int)c.quant_radius;
+    num_elements_ = c.num_elements;
+    outlier_count_ = c.outlier_count;
+    config_.eb_mode = (ErrorBoundMode)c.eb_mode;
+    config_.zigzag_codes = c.zigzag_codes != 0;
+    config_.outlier_threshold = c.outlier_threshold;
+}
+
+template class QuantizerStage<float, uint8_t>;
+template class QuantizerStage<float, uint16_t>;
+template class QuantizerStage<float, uint32_t>;
+
+// ===========================================================================
+// Bitpack
+// ===========================================================================
+template <typename T>
+void BitpackStage<T>::execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                              const std::vector<void*>& outputs, const std::vector<size_t>& sizes) {
+    if (inputs.empty() || outputs.empty() || sizes.empty())
+        throw std::runtime_error("BitpackStage: inputs, outputs, and sizes must be non-empty");
+    if (!is_inverse_) {
+        const size_t n = sizes[0] / sizeof(T);
+        num_elements_ = n;
+        if (n == 0) { actual_output_size_ = 0; return; }
+        if (auto_detect_) {
+            ws_.ensure(256 + fzh_max_u32_ws_bytes(), pool, stream);
+            throwIfFzh(fzh_max_u32(stream, inputs[0], n, (int)sizeof(T), (uint32_t*)ws_.ptr, (uint8_t*)ws_.ptr + 256,
+                                   ws_.bytes - 256),
+                       "BitpackStage max");
+            uint32_t mx = 0;
+            FZ_HIP(hipMemcpyAsync(&mx, ws_.ptr, 4, hipMemcpyDeviceToHost, stream));
+            FZ_HIP(hipStreamSynchronize(stream));
+            uint8_t nb = 8 * sizeof(T);
+            for (uint8_t b = 1; b < 8 * sizeof(T); b = (uint8_t)(b * 2))
+                if ((uint64_t)mx < (1ull << b)) { nb = b; break; }
+            nbits_ = nb;
+        }
+        throwIfFzh(fzh_bitpack_enc(stream, inputs[0], n, (int)sizeof(T), nbits_, outputs[0]), "BitpackStage");
+        actual_output_size_ = (n * nbits_ + 7) / 8;
+    } else {
+        const size_t n = num_elements_;
+        actual_output_size_ = n * sizeof(T);
+        if (n == 0) return;
+        throwIfFzh(fzh_bitpack_dec(stream, inputs[0], n, (int)sizeof(T), nbits_, outputs[0]), "BitpackStage (inverse)");
+    }
+}
+
+template class BitpackStage<uint8_t>;
+template class BitpackStage<uint16_t>;
+template class BitpackStage<uint32_t>;
+
+// ===========================================================================
+// Factory (stage_factory.h:36-248): types supported by this build
+// ===========================================================================
+Stage* createStage(StageType type, const uint8_t* config, size_t config_size) {
+    Stage* s = nullptr;
+    switch (type) {
+    case StageType::LORENZO_QUANT: {
+        if (config_size < sizeof(LorenzoQuantConfig))
+            throw std::runtime_error("Lorenzo config too small: " + std::to_string(config_size));
+        LorenzoQuantConfig lc;
+        std::memcpy(&lc, config, sizeof(lc));
+        if (lc.input_type == DataType::FLOAT32 && lc.code_type == DataType::UINT16) s = new LorenzoQuantStage<float, uint16_t>();
+        else if (lc.input_type == DataType::FLOAT32 && lc.code_type == DataType::UINT8) s = new LorenzoQuantStage<float, uint8_t>();
+        else throw std::runtime_error("Unsupported Lorenzo template instantiation: input_type=" +
+                                      std::to_string((int)lc.input_type) + " code_type=" + std::to_string((int)lc.code_type));
+        break;
+    }
+    case StageType::BITSHUFFLE: s = new BitshuffleStage(); break;
+    case StageType::RZE: s = new RZEStage(); break;
+    case StageType::RLE: {
+        const DataType dt = config_size >= 1 ? (DataType)config[0] : DataType::UINT16;
+        switch (dt) {
+        case DataType::UINT8: s = new RLEStage<uint8_t>(); break;
+        case DataType::UINT16: s = new RLEStage<uint16_t>(); break;
+        case DataType::UINT32: s = new RLEStage<uint32_t>(); break;
+        case DataType::INT32: s = new RLEStage<int32_t>(); break;
+        default: throw std::runtime_error("Unsupported RLE data type " + std::to_string((int)dt));
+        }
+        break;
+    }
+    case StageType::HUFFMAN: {
+        const DataType dt = config_size >= 1 ? (DataType)config[0] : DataType::UINT16;
+        if (dt == DataType::UINT8) s = new HuffmanStage<uint8_t>();
+        else if (dt == DataType::UINT32) s = new HuffmanStage<uint32_t>();
+        else s = new HuffmanStage<uint16_t>();
+        break;
+    }
+    case StageType::QUANTIZER: {
+        QuantizerConfig qc;
+        std::memcpy(&qc, config, std::min(config_size, sizeof(qc)));
+        if (qc.code_type == DataType::UINT8) s = new QuantizerStage<float, uint8_t>();
+        else if (qc.code_type == DataType::UINT32) s = new QuantizerStage<float, uint32_t>();
+        else s = new QuantizerStage<float, uint16_t>();
+        break;
+    }
+    case StageType::BITPACK: {
+        const DataType dt = config_size >= 1 ? (DataType)config[0] : DataType::UINT16;
+        if (dt == DataType::UINT8) s = new BitpackStage<uint8_t>();
+        else if (dt == DataType::UINT32) s = new BitpackStage<uint32_t>();
+        else s = new BitpackStage<uint16_t>();
+        break;
+    }
+    default:
+        throw std::runtime_error("createStage: stage type " + stageTypeToString(type) + " (" +
+                                 std::to_string((int)type) + ") is not part of this build");
+    }
+    s->deserializeHeader(config, config_size);
+    return s;
+}
+
+}  // namespace fz

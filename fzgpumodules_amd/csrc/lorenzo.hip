@@ -61,7 +61,7 @@ struct Tiles {
 
 template <typename TCode, bool ZZ>
 __global__ void __launch_bounds__(256)
-lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, float r_host, const float* __restrict__ d_r,
+lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, uint64_t nv, float r_host, const float* __restrict__ d_r,
                 float rad, TCode* __restrict__ codes, float* __restrict__ oerr,
                 uint32_t* __restrict__ oidx, uint32_t* __restrict__ ocount, uint64_t max_out,
                 ScanCtx* ctx, uint64_t* status, uint32_t T, uint32_t total_waves) {
@@ -70,7 +70,7 @@ lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, float r_host, const fl
     if (tk.tile < T) {
         const int l = lane_id();
         const uint64_t base = (uint64_t)tk.tile * 1024;
-        const bool full = base + 1024 <= n;
+        const bool full = base + 1024 <= nv;  // elements in [nv, n) read as zero (padding)
         float s[4][4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -83,7 +83,7 @@ lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, float r_host, const fl
                 s[q][3] = quant_round(v.w, r);
             } else {
 #pragma unroll
-                for (int k = 0; k < 4; k++) s[q][k] = (e0 + k < n) ? quant_round(in[e0 + k], r) : 0.0f;
+                for (int k = 0; k < 4; k++) s[q][k] = (e0 + k < nv) ? quant_round(in[e0 + k], r) : 0.0f;
             }
         }
         // delta_i = s_i - s_{i-1}, chain restarts at the tile start.
@@ -111,7 +111,7 @@ lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, float r_host, const fl
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
-            if (full) {
+            if (base + 1024 <= n) {
                 if constexpr (sizeof(TCode) == 2) {
                     uint2 w;
                     w.x = (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 16);
@@ -318,10 +318,14 @@ lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t
 // ===========================================================================
 __global__ void __launch_bounds__(256)
 minmax_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ partial,
-              uint32_t* __restrict__ counter, int mode, float user_eb, float* __restrict__ params) {
+              uint32_t* __restrict__ counter, int mode, bool zero_pad, float user_eb, float* __restrict__ params) {
     __shared__ float smn[4], smx[4];
     const float first = in[0];
     float mn = first, mx = first;
+    if (zero_pad) {  // the buffer continues with zeros up to the padded length
+        if (0.0f < mn) mn = 0.0f;
+        if (0.0f > mx) mx = 0.0f;
+    }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t n4 = n / 4;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
@@ -697,6 +701,7 @@ size_t tile_count(size_t n, size_t nx, size_t ny, size_t nz) {
 int check_dims(size_t n, size_t nx, size_t ny, size_t nz, const char* who) {
     const int nd = ndim_of(ny, nz);
     if (n > 0xFFFFFFFFull) return fail(FZH_E_INVALID, std::string(who) + ": more than 2^32 elements");
+    if (nd == 1 && nx > n) return fail(FZH_E_INVALID, std::string(who) + ": dims[0] exceeds the element count");
     if (nd >= 2) {
         if (nx == 0) return fail(FZH_E_INVALID, std::string(who) + ": dims[0] must be set for 2D/3D");
         const size_t cover = nx * ny * (nd == 3 ? nz : 1);
@@ -718,9 +723,10 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
     const uint32_t waves = blocks * 4;
     TCode* c = (TCode*)codes;
     const int nd = ndim_of(ny, nz);
+    const size_t nv = (nd == 1 && nx > 0 && nx < n) ? nx : n;
 #define FZH_LZ_FWD(ZZV)                                                                               \
     if (nd == 1)                                                                                      \
-        lz1d_fwd_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(in, n, r, d_r, rad, c, oerr, oidx, ocount, \
+        lz1d_fwd_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(in, n, nv, r, d_r, rad, c, oerr, oidx, ocount, \
                                                            max_out, ctx, status, (uint32_t)T, waves); \
     else if (nd == 2)                                                                                 \
         lz2d_fwd_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(                                           \
@@ -890,13 +896,15 @@ size_t fzh_value_base_ws_bytes(size_t n) {
 int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode, float user_eb, float* d_params,
                        void* d_ws, size_t ws_bytes) {
     if (ws_bytes < fzh_value_base_ws_bytes(n)) return fail(FZH_E_CAPACITY, "value base: workspace too small");
+    const bool zero_pad = (mode & FZH_VB_ZERO_PADDED) != 0;
+    mode &= ~FZH_VB_ZERO_PADDED;
     if (mode != 1 && mode != 2) return fail(FZH_E_INVALID, "value base: mode must be 1 (REL) or 2 (NOA)");
     if (n == 0) return fail(FZH_E_INVALID, "value base: empty input");
     if (reinterpret_cast<uintptr_t>(d_in) % 16 != 0) return fail(FZH_E_INVALID, "value base: input must be 16-byte aligned");
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256 * 16), 1024));
     uint32_t* counter = (uint32_t*)d_ws;
     float2* partial = (float2*)((uint8_t*)d_ws + 256);
-    minmax_kernel<<<blocks, 256, 0, S(stream)>>>(d_in, n, partial, counter, mode, user_eb, d_params);
+    minmax_kernel<<<blocks, 256, 0, S(stream)>>>(d_in, n, partial, counter, mode, zero_pad, user_eb, d_params);
     FZH_CHECK_LAUNCH("minmax_kernel");
     return FZH_OK;
 }

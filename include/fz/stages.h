@@ -1,0 +1,612 @@
+// fz/stages.h — the hot-path stages, backed by the fzh_* HIP kernels.
+//
+// Class names, configuration fields, setters/getters, port names and header
+// layouts follow the reference:
+//   LorenzoQuantStage   modules/fused/lorenzo_quant/lorenzo_quant.h:43-334
+//   BitshuffleStage     modules/shufflers/bitshuffle/bitshuffle_stage.h:38-151
+//   RZEStage            modules/coders/rze/rze_stage.h:47-249
+//   RLEStage            modules/coders/rle/rle.h:31-207
+//   HuffmanStage        modules/coders/huffman/huffman_stage.h:61-256
+//   QuantizerStage      modules/quantizers/quantizer/quantizer.h:25-260 (ABS/NOA)
+//   BitpackStage        modules/coders/bitpack/bitpack_stage.h (nbits / auto)
+// Differences in behaviour are confined to where the reference is
+// nondeterministic or waits on the host: outlier lists come out in a fixed
+// order, REL/NOA error bounds and all inverse paths resolve on the device
+// without a host round trip, and padding bytes are always zero.
+#pragma once
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+
+#include "fz/mempool.h"
+#include "fz/stage.h"
+
+namespace fz {
+
+enum class ErrorBoundMode : uint8_t { ABS = 0, REL = 1, NOA = 2 };
+
+// Device scratch owned by a stage, zero-initialised once on allocation.
+struct DeviceScratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    MemoryPool* pool = nullptr;
+    // Grows to at least `need` bytes (re-zeroed when it grows).
+    void ensure(size_t need, MemoryPool* p, hipStream_t s);
+    void release();
+    ~DeviceScratch() { release(); }
+};
+
+template <typename T> constexpr DataType dataTypeOf() {
+    if constexpr (std::is_same_v<T, uint8_t>) return DataType::UINT8;
+    else if constexpr (std::is_same_v<T, uint16_t>) return DataType::UINT16;
+    else if constexpr (std::is_same_v<T, uint32_t>) return DataType::UINT32;
+    else if constexpr (std::is_same_v<T, uint64_t>) return DataType::UINT64;
+    else if constexpr (std::is_same_v<T, int8_t>) return DataType::INT8;
+    else if constexpr (std::is_same_v<T, int16_t>) return DataType::INT16;
+    else if constexpr (std::is_same_v<T, int32_t>) return DataType::INT32;
+    else if constexpr (std::is_same_v<T, int64_t>) return DataType::INT64;
+    else if constexpr (std::is_same_v<T, float>) return DataType::FLOAT32;
+    else return DataType::FLOAT64;
+}
+
+// ---------------------------------------------------------------------------
+// LorenzoQuant
+// ---------------------------------------------------------------------------
+// 44-byte header (lorenzo_quant.h:43-67).  reserved[0] bit 0 is set when the
+// stage carries the strict-error-bound patch list (an extension; zero
+// otherwise, so headers without it are byte-identical to the reference's).
+struct LorenzoQuantConfig {
+    float error_bound;
+    uint32_t quant_radius;
+    uint32_t num_elements;
+    uint32_t outlier_count;
+    DataType input_type;
+    DataType code_type;
+    uint8_t ndim;
+    uint8_t eb_mode;
+    uint32_t dim_x;
+    uint32_t dim_y;
+    uint32_t dim_z;
+    float user_eb;
+    float value_base;
+    uint8_t zigzag_codes;
+    uint8_t reserved[3];
+    LorenzoQuantConfig() { std::memset(this, 0, sizeof(*this)); input_type = DataType::FLOAT32; code_type = DataType::UINT16; ndim = 1; dim_y = 1; dim_z = 1; }
+};
+static_assert(sizeof(LorenzoQuantConfig) == 44, "LorenzoQuantConfig is 44 bytes");
+
+template <typename TInput = float, typename TCode = uint16_t>
+class LorenzoQuantStage : public Stage {
+    static_assert(std::is_same_v<TInput, float>, "LorenzoQuantStage: float input on gfx950");
+    static_assert(std::is_same_v<TCode, uint16_t> || std::is_same_v<TCode, uint8_t>,
+                  "LorenzoQuantStage: uint8_t or uint16_t codes");
+
+  public:
+    struct Config {
+        float error_bound = 1e-3f;
+        int quant_radius = 32768;
+        float outlier_capacity = 0.2f;
+        std::array<size_t, 3> dims = {0, 1, 1};
+        ErrorBoundMode eb_mode = ErrorBoundMode::ABS;
+        float precomputed_value_base = 0.0f;
+        bool zigzag_codes = false;
+        Config() = default;
+        Config(TInput eb, TCode radius = (TCode)32768, float outlier_cap = 0.2f, std::array<size_t, 3> d = {0, 1, 1})
+            : error_bound((float)eb), quant_radius(radius), outlier_capacity(outlier_cap), dims(d) {}
+    };
+
+    explicit LorenzoQuantStage(const Config& config = Config());
+
+    void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                 const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
+    void postStreamSync(hipStream_t stream) override;
+
+    std::string getName() const override { return "LorenzoQuant"; }
+    size_t getNumInputs() const override { return is_inverse_ ? 4 : 1; }
+    size_t getNumOutputs() const override { return is_inverse_ ? 1 : 4; }
+    std::vector<std::string> getOutputNames() const override {
+        return {"codes", "outlier_errors", "outlier_indices", "outlier_count"};
+    }
+    std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& input_sizes) const override;
+    std::unordered_map<std::string, size_t> getActualOutputSizesByName() const override;
+    size_t getActualOutputSize(int index) const override {
+        return (index >= 0 && index < (int)actual_output_sizes_.size()) ? actual_output_sizes_[index] : 0;
+    }
+    void saveState() override { saved_output_sizes_ = actual_output_sizes_; }
+    void restoreState() override { actual_output_sizes_ = saved_output_sizes_; }
+
+    void setErrorBound(TInput eb) { config_.error_bound = (float)eb; }
+    void setQuantRadius(TCode r) { config_.quant_radius = r; }
+    void setOutlierCapacity(float c) { config_.outlier_capacity = c; }
+    void setDims(const std::array<size_t, 3>& dims) override { config_.dims = dims; }
+    void setDims(size_t x, size_t y = 1, size_t z = 1) { config_.dims = {x, y, z}; }
+    void setErrorBoundMode(ErrorBoundMode m) { config_.eb_mode = m; }
+    void setValueBase(float vb) { config_.precomputed_value_base = vb; }
+    void setZigzagCodes(bool on) { config_.zigzag_codes = on; }
+
+    TInput getErrorBound() const { return (TInput)config_.error_bound; }
+    TCode getQuantRadius() const { return (TCode)config_.quant_radius; }
+    float getOutlierCapacity() const { return config_.outlier_capacity; }
+    std::array<size_t, 3> getDims() const { return config_.dims; }
+    ErrorBoundMode getErrorBoundMode() const { return config_.eb_mode; }
+    float getValueBase() const { return config_.precomputed_value_base; }
+    bool getZigzagCodes() const { return config_.zigzag_codes; }
+    // Absolute bound of the last compression (REL/NOA resolved) and its base.
+    float getComputedErrorBound() const { return computed_abs_eb_; }
+    float getComputedValueBase() const { return computed_value_base_; }
+    uint32_t getOutlierCount() const { return actual_outlier_count_; }
+
+    int ndim() const { return config_.dims[2] > 1 ? 3 : (config_.dims[1] > 1 ? 2 : 1); }
+    bool acceptZeroPaddedInput(size_t valid_bytes) override {
+        valid_elems_ = valid_bytes / sizeof(TInput);
+        return !is_inverse_ && valid_bytes % sizeof(TInput) == 0;
+    }
+    DeviceSize getDeviceOutputSize(int index) const override {
+        if (is_inverse_ || (index != 1 && index != 2) || !d_count_ptr_) return {};
+        return {d_count_ptr_, 2, 4};
+    }
+    void setInverse(bool inv) override { is_inverse_ = inv; }
+    bool isInverse() const override { return is_inverse_; }
+
+    uint16_t getStageTypeId() const override { return (uint16_t)StageType::LORENZO_QUANT; }
+    uint8_t getOutputDataType(size_t i) const override {
+        switch (i) {
+        case 0: return (uint8_t)dataTypeOf<TCode>();
+        case 1: return (uint8_t)DataType::FLOAT32;
+        case 2: case 3: return (uint8_t)DataType::UINT32;
+        default: return (uint8_t)DataType::UINT8;
+        }
+    }
+    uint8_t getInputDataType(size_t) const override { return (uint8_t)DataType::FLOAT32; }
+    size_t serializeHeader(size_t output_index, uint8_t* buf, size_t max_size) const override;
+    void deserializeHeader(const uint8_t* buf, size_t size) override;
+    size_t getMaxHeaderSize(size_t) const override { return sizeof(LorenzoQuantConfig); }
+
+  private:
+    size_t maxOutliers(size_t n) const { return (size_t)std::ceil((double)n * config_.outlier_capacity); }
+    void resolveDims(size_t n, size_t& nx, size_t& ny, size_t& nz) const;
+
+    Config config_;
+    bool is_inverse_ = false;
+    std::vector<size_t> actual_output_sizes_ = std::vector<size_t>(4, 0), saved_output_sizes_;
+    size_t num_elements_ = 0;
+    uint32_t actual_outlier_count_ = 0;
+    float computed_abs_eb_ = 0.0f;
+    float computed_value_base_ = 0.0f;
+    size_t valid_elems_ = 0;             // > 0: input elements past this read as zero (one call)
+    const void* d_count_ptr_ = nullptr;  // forward: outlier count output
+    bool params_pending_ = false;        // REL/NOA resolved on the device
+    uint32_t h_count_ = 0;
+    float h_params_[4] = {0, 0, 0, 0};
+    DeviceScratch ws_, params_;
+};
+
+// ---------------------------------------------------------------------------
+// Bitshuffle
+// ---------------------------------------------------------------------------
+class BitshuffleStage : public Stage {
+  public:
+    void setInverse(bool inv) override { is_inverse_ = inv; }
+    bool isInverse() const override { return is_inverse_; }
+    void setBlockSize(size_t bytes) { block_size_ = (uint32_t)bytes; }
+    void setElementWidth(size_t bytes) { element_width_ = (uint8_t)bytes; }
+    size_t getBlockSize() const { return block_size_; }
+    size_t getElementWidth() const { return element_width_; }
+    size_t getRequiredInputAlignment() const override { return block_size_; }
+
+    void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                 const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
+    std::string getName() const override { return "Bitshuffle"; }
+    size_t getNumInputs() const override { return 1; }
+    size_t getNumOutputs() const override { return 1; }
+    std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& in) const override { return {in[0]}; }
+    std::unordered_map<std::string, size_t> getActualOutputSizesByName() const override {
+        return {{"output", actual_output_size_}};
+    }
+    size_t getActualOutputSize(int i) const override { return i == 0 ? actual_output_size_ : 0; }
+    uint16_t getStageTypeId() const override { return (uint16_t)StageType::BITSHUFFLE; }
+    uint8_t getOutputDataType(size_t) const override { return (uint8_t)DataType::UINT8; }
+    // [0..3] block_size u32 LE, [4] element_width  (bitshuffle_stage.h:98-113)
+    size_t serializeHeader(size_t, uint8_t* buf, size_t max_size) const override {
+        if (max_size < 5) return 0;
+        std::memcpy(buf, &block_size_, 4);
+        buf[4] = element_width_;
+        return 5;
+    }
+    void deserializeHeader(const uint8_t* buf, size_t size) override {
+        if (size >= 4) std::memcpy(&block_size_, buf, 4);
+        if (size >= 5) element_width_ = buf[4];
+    }
+    size_t getMaxHeaderSize(size_t) const override { return 5; }
+    void saveState() override { saved_ = {block_size_, element_width_, actual_output_size_}; }
+    void restoreState() override {
+        block_size_ = saved_.bs;
+        element_width_ = saved_.ew;
+        actual_output_size_ = saved_.out;
+    }
+
+  private:
+    bool is_inverse_ = false;
+    uint32_t block_size_ = 16384;
+    uint8_t element_width_ = 4;
+    size_t actual_output_size_ = 0;
+    struct Saved { uint32_t bs; uint8_t ew; size_t out; } saved_{16384, 4, 0};
+};
+
+// ---------------------------------------------------------------------------
+// RZE
+// ---------------------------------------------------------------------------
+class RZEStage : public Stage {
+  public:
+    void setInverse(bool inv) override { is_inverse_ = inv; }
+    bool isInverse() const override { return is_inverse_; }
+    void setChunkSize(size_t bytes) { chunk_size_ = (uint32_t)bytes; }
+    void setLevels(int n) { levels_ = (uint8_t)n; }
+    size_t getChunkSize() const { return chunk_size_; }
+    int getLevels() const { return levels_; }
+    uint32_t getCachedOrigBytes() const { return cached_orig_bytes_; }
+    size_t getRequiredInputAlignment() const override { return chunk_size_; }
+
+    void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                 const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
+    void postStreamSync(hipStream_t stream) override;
+    std::string getName() const override { return "RZE"; }
+    size_t getNumInputs() const override { return 1; }
+    size_t getNumOutputs() const override { return 1; }
+    std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& in) const override;
+    std::unordered_map<std::string, size_t> getActualOutputSizesByName() const override {
+        return {{"output", actual_output_size_}};
+    }
+    size_t getActualOutputSize(int i) const override { return i == 0 ? actual_output_size_ : 0; }
+    uint16_t getStageTypeId() const override { return (uint16_t)StageType::RZE; }
+    uint8_t getOutputDataType(size_t) const override { return (uint8_t)DataType::UINT8; }
+    // [0..3] chunk_size, [4] levels, [5..8] cached_orig_bytes  (rze_stage.h:169-190)
+    size_t serializeHeader(size_t, uint8_t* buf, size_t max_size) const override {
+        if (max_size < 9) return 0;
+        std::memcpy(buf, &chunk_size_, 4);
+        buf[4] = levels_;
+        std::memcpy(buf + 5, &cached_orig_bytes_, 4);
+        return 9;
+    }
+    void deserializeHeader(const uint8_t* buf, size_t size) override {
+        if (size >= 4) std::memcpy(&chunk_size_, buf, 4);
+        if (size >= 5) levels_ = buf[4];
+        if (size >= 9) std::memcpy(&cached_orig_bytes_, buf + 5, 4);
+    }
+    size_t getMaxHeaderSize(size_t) const override { return 9; }
+    void saveState() override { saved_ = {chunk_size_, levels_, cached_orig_bytes_}; }
+    void restoreState() override {
+        chunk_size_ = saved_.cs;
+        levels_ = saved_.lv;
+        cached_orig_bytes_ = saved_.ob;
+    }
+
+  private:
+    bool is_inverse_ = false;
+    uint32_t chunk_size_ = 16384;
+    uint8_t levels_ = 4;
+    uint32_t cached_orig_bytes_ = 0;
+    size_t actual_output_size_ = 0;
+    bool size_pending_ = false, status_pending_ = false;
+    uint64_t h_size_ = 0;
+    int32_t h_status_ = 0;
+    struct Saved { uint32_t cs; uint8_t lv; uint32_t ob; } saved_{16384, 4, 0};
+    DeviceScratch ws_, small_;  // small_: [0] u64 out size, [8] i32 status
+  public:
+    DeviceSize getDeviceOutputSize(int index) const override {
+        if (is_inverse_ || index != 0 || !size_pending_) return {};
+        return {small_.ptr, 1, 1};
+    }
+};
+
+// ---------------------------------------------------------------------------
+// RLE
+// ---------------------------------------------------------------------------
+template <typename T = uint16_t>
+class RLEStage : public Stage {
+    static_assert(sizeof(T) == 1 || sizeof(T) == 2 || sizeof(T) == 4, "RLEStage: 1/2/4-byte elements");
+
+  public:
+    void setInverse(bool inv) override { is_inverse_ = inv; }
+    bool isInverse() const override { return is_inverse_; }
+    void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                 const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
+    void postStreamSync(hipStream_t stream) override;
+    std::string getName() const override { return "RLE"; }
+    size_t getNumInputs() const override { return 1; }
+    size_t getNumOutputs() const override { return 1; }
+    std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& in) const override {
+        if (is_inverse_) return {cached_num_elements_ ? (size_t)cached_num_elements_ * sizeof(T) : in[0] * 2};
+        const size_t n = in[0] / sizeof(T);
+        return {4 + ((n * sizeof(T) + 3) & ~size_t(3)) + 4 * n};
+    }
+    std::unordered_map<std::string, size_t> getActualOutputSizesByName() const override {
+        return {{"output", actual_output_size_}};
+    }
+    size_t getActualOutputSize(int i) const override { return i == 0 ? actual_output_size_ : 0; }
+    uint16_t getStageTypeId() const override { return (uint16_t)StageType::RLE; }
+    uint8_t getOutputDataType(size_t) const override { return (uint8_t)dataTypeOf<T>(); }
+    uint8_t getInputDataType(size_t) const override { return (uint8_t)dataTypeOf<T>(); }
+    // [0] DataType, [1..4] cached_num_elements  (rle.h:123-141)
+    size_t serializeHeader(size_t, uint8_t* buf, size_t max_size) const override {
+        if (max_size < 5) return 0;
+        buf[0] = (uint8_t)dataTypeOf<T>();
+        std::memcpy(buf + 1, &cached_num_elements_, 4);
+        return 5;
+    }
+    void deserializeHeader(const uint8_t* buf, size_t size) override {
+        if (size >= 5) std::memcpy(&cached_num_elements_, buf + 1, 4);
+    }
+    size_t getMaxHeaderSize(size_t) const override { return 5; }
+    void saveState() override { saved_n_ = cached_num_elements_; saved_out_ = actual_output_size_; }
+    void restoreState() override { cached_num_elements_ = saved_n_; actual_output_size_ = saved_out_; }
+
+  private:
+    bool is_inverse_ = false;
+    uint32_t cached_num_elements_ = 0, saved_n_ = 0;
+    size_t actual_output_size_ = 0, saved_out_ = 0;
+    bool size_pending_ = false, status_pending_ = false;
+    uint64_t h_size_ = 0;
+    int32_t h_status_ = 0;
+    DeviceScratch ws_, small_;
+  public:
+    DeviceSize getDeviceOutputSize(int index) const override {
+        if (is_inverse_ || index != 0 || !size_pending_) return {};
+        return {small_.ptr, 1, 1};
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Huffman (coarse mode)
+// ---------------------------------------------------------------------------
+enum class HuffmanEncodeMode { Coarse, Fine };
+
+template <typename T = uint16_t>
+class HuffmanStage : public Stage {
+    static_assert(std::is_same_v<T, uint8_t> || std::is_same_v<T, uint16_t> || std::is_same_v<T, uint32_t>,
+                  "HuffmanStage: T must be uint8_t, uint16_t, or uint32_t.");
+
+  public:
+    void setBklen(uint32_t b) { bklen_ = b; }
+    uint32_t getBklen() const { return bklen_; }
+    // Both modes emit the coarse stream here (768-symbol partitions); the
+    // encode is device-side either way.
+    void setEncodeMode(HuffmanEncodeMode m) { encode_mode_ = m; }
+    HuffmanEncodeMode getEncodeMode() const { return encode_mode_; }
+    void setInverse(bool inv) override { is_inverse_ = inv; }
+    bool isInverse() const override { return is_inverse_; }
+    // The histogram has to reach the host-side codebook builder.
+    bool isGraphCompatible() const override { return is_inverse_; }
+
+    void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                 const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
+    void postStreamSync(hipStream_t stream) override;
+    std::string getName() const override { return "Huffman"; }
+    size_t getNumInputs() const override { return 1; }
+    size_t getNumOutputs() const override { return 1; }
+    std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& in) const override;
+    std::unordered_map<std::string, size_t> getActualOutputSizesByName() const override {
+        return {{"output", actual_output_size_}};
+    }
+    size_t getActualOutputSize(int i) const override { return i == 0 ? actual_output_size_ : 0; }
+    uint16_t getStageTypeId() const override { return (uint16_t)StageType::HUFFMAN; }
+    uint8_t getOutputDataType(size_t) const override { return (uint8_t)DataType::UNKNOWN; }
+    uint8_t getInputDataType(size_t) const override { return (uint8_t)DataType::UNKNOWN; }
+    // [0] DataType, [1..2] bklen u16, [3..10] original_len u64  (huffman_stage.h:174-193)
+    size_t serializeHeader(size_t, uint8_t* buf, size_t max_size) const override {
+        if (max_size < 11) return 0;
+        buf[0] = (uint8_t)dataTypeOf<T>();
+        const uint16_t bk = (uint16_t)bklen_;
+        std::memcpy(buf + 1, &bk, 2);
+        std::memcpy(buf + 3, &original_len_, 8);
+        return 11;
+    }
+    void deserializeHeader(const uint8_t* buf, size_t size) override {
+        if (size >= 3) {
+            uint16_t bk;
+            std::memcpy(&bk, buf + 1, 2);
+            bklen_ = bk;
+        }
+        if (size >= 11) std::memcpy(&original_len_, buf + 3, 8);
+    }
+    size_t getMaxHeaderSize(size_t) const override { return 11; }
+    void saveState() override { saved_ = {bklen_, original_len_, actual_output_size_}; }
+    void restoreState() override {
+        bklen_ = saved_.bk;
+        original_len_ = saved_.ol;
+        actual_output_size_ = saved_.out;
+    }
+
+  private:
+    bool is_inverse_ = false;
+    uint32_t bklen_ = std::is_same_v<T, uint8_t> ? 256 : 1024;
+    HuffmanEncodeMode encode_mode_ = HuffmanEncodeMode::Coarse;
+    uint64_t original_len_ = 0;
+    size_t actual_output_size_ = 0;
+    bool size_pending_ = false, status_pending_ = false;
+    uint64_t h_size_ = 0;
+    int32_t h_status_ = 0;
+    struct Saved { uint32_t bk; uint64_t ol; size_t out; } saved_{1024, 0, 0};
+    DeviceScratch ws_, small_, book_, freq_;
+  public:
+    DeviceSize getDeviceOutputSize(int index) const override {
+        if (is_inverse_ || index != 0 || !size_pending_) return {};
+        return {small_.ptr, 1, 1};
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Quantizer (ABS / NOA)
+// ---------------------------------------------------------------------------
+struct QuantizerConfig {
+    float abs_error_bound;
+    float user_error_bound;
+    float value_base;
+    uint32_t quant_radius;
+    uint32_t num_elements;
+    uint32_t outlier_count;
+    DataType input_type;
+    DataType code_type;
+    uint8_t eb_mode;
+    uint8_t zigzag_codes;
+    float outlier_threshold;
+    uint8_t inplace_outliers;
+    uint8_t pad[3];
+    QuantizerConfig() {
+        std::memset(this, 0, sizeof(*this));
+        input_type = DataType::FLOAT32;
+        code_type = DataType::UINT16;
+        outlier_threshold = INFINITY;
+    }
+};
+static_assert(sizeof(QuantizerConfig) == 36, "QuantizerConfig is 36 bytes");
+
+template <typename TInput = float, typename TCode = uint16_t>
+class QuantizerStage : public Stage {
+    static_assert(std::is_same_v<TInput, float>, "QuantizerStage: float input on gfx950");
+
+  public:
+    struct Config {
+        float error_bound = 1e-4f;
+        int quant_radius = 32768;
+        float outlier_capacity = 0.05f;
+        ErrorBoundMode eb_mode = ErrorBoundMode::ABS;
+        float precomputed_value_base = 0.0f;
+        bool zigzag_codes = false;
+        float outlier_threshold = INFINITY;
+        Config() = default;
+        Config(float eb, int radius = 32768, float cap = 0.05f) : error_bound(eb), quant_radius(radius), outlier_capacity(cap) {}
+    };
+    explicit QuantizerStage(const Config& c = Config()) : config_(c) {}
+
+    void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                 const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
+    void postStreamSync(hipStream_t stream) override;
+    std::string getName() const override { return "Quantizer"; }
+    size_t getNumInputs() const override { return is_inverse_ ? 4 : 1; }
+    size_t getNumOutputs() const override { return is_inverse_ ? 1 : 4; }
+    std::vector<std::string> getOutputNames() const override {
+        if (is_inverse_) return {"reconstructed"};
+        return {"codes", "outlier_vals", "outlier_idxs", "outlier_count"};
+    }
+    std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& in) const override;
+    std::unordered_map<std::string, size_t> getActualOutputSizesByName() const override;
+    size_t getActualOutputSize(int i) const override {
+        return (i >= 0 && i < (int)actual_output_sizes_.size()) ? actual_output_sizes_[i] : 0;
+    }
+    void setInverse(bool inv) override { is_inverse_ = inv; }
+    bool isInverse() const override { return is_inverse_; }
+    void saveState() override { saved_sizes_ = actual_output_sizes_; }
+    void restoreState() override { actual_output_sizes_ = saved_sizes_; }
+    void setErrorBound(TInput eb) { config_.error_bound = (float)eb; }
+    void setQuantRadius(int r) { config_.quant_radius = r; }
+    void setOutlierCapacity(float c) { config_.outlier_capacity = c; }
+    void setErrorBoundMode(ErrorBoundMode m) { config_.eb_mode = m; }
+    void setValueBase(float vb) { config_.precomputed_value_base = vb; }
+    void setZigzagCodes(bool on) { config_.zigzag_codes = on; }
+    void setOutlierThreshold(float t) { config_.outlier_threshold = t; }
+    uint16_t getStageTypeId() const override { return (uint16_t)StageType::QUANTIZER; }
+    uint8_t getOutputDataType(size_t i) const override {
+        switch (i) {
+        case 0: return (uint8_t)dataTypeOf<TCode>();
+        case 1: return (uint8_t)DataType::FLOAT32;
+        default: return (uint8_t)DataType::UINT32;
+        }
+    }
+    uint8_t getInputDataType(size_t) const override { return (uint8_t)DataType::FLOAT32; }
+    size_t serializeHeader(size_t, uint8_t* buf, size_t max_size) const override;
+    void deserializeHeader(const uint8_t* buf, size_t size) override;
+    size_t getMaxHeaderSize(size_t) const override { return sizeof(QuantizerConfig); }
+
+  private:
+    size_t maxOutliers(size_t n) const { return (size_t)std::ceil((double)n * config_.outlier_capacity); }
+    Config config_;
+    bool is_inverse_ = false;
+    std::vector<size_t> actual_output_sizes_ = std::vector<size_t>(4, 0), saved_sizes_;
+    size_t num_elements_ = 0;
+    uint32_t outlier_count_ = 0;
+    float computed_abs_eb_ = 0.0f, computed_value_base_ = 0.0f;
+    const void* d_count_ptr_ = nullptr;
+    uint32_t h_count_ = 0;
+    DeviceScratch ws_;
+  public:
+    DeviceSize getDeviceOutputSize(int index) const override {
+        if (is_inverse_ || (index != 1 && index != 2) || !d_count_ptr_) return {};
+        return {d_count_ptr_, 2, 4};
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Bitpack
+// ---------------------------------------------------------------------------
+template <typename T = uint16_t>
+class BitpackStage : public Stage {
+  public:
+    void setInverse(bool inv) override { is_inverse_ = inv; }
+    bool isInverse() const override { return is_inverse_; }
+    void setNBits(uint8_t nbits) {
+        if (nbits == 0 || nbits > 8 * sizeof(T) || (nbits & (nbits - 1)) != 0)
+            throw std::invalid_argument("BitpackStage::setNBits: nbits must be a power of two in [1, " +
+                                        std::to_string(8 * sizeof(T)) + "], got " + std::to_string(nbits));
+        nbits_ = nbits;
+    }
+    uint8_t getNBits() const { return nbits_; }
+    void setAutoDetect(bool on) { auto_detect_ = on; }
+    bool isAutoDetect() const { return auto_detect_; }
+    bool isGraphCompatible() const override { return !auto_detect_ || is_inverse_; }
+
+    void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                 const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
+    std::string getName() const override { return "Bitpack"; }
+    size_t getNumInputs() const override { return 1; }
+    size_t getNumOutputs() const override { return 1; }
+    std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& in) const override {
+        if (is_inverse_) return {num_elements_ * sizeof(T)};
+        return {in[0]};
+    }
+    std::unordered_map<std::string, size_t> getActualOutputSizesByName() const override {
+        return {{"output", actual_output_size_}};
+    }
+    size_t getActualOutputSize(int i) const override { return i == 0 ? actual_output_size_ : 0; }
+    uint16_t getStageTypeId() const override { return (uint16_t)StageType::BITPACK; }
+    uint8_t getOutputDataType(size_t) const override { return (uint8_t)DataType::UINT8; }
+    uint8_t getInputDataType(size_t) const override { return (uint8_t)dataTypeOf<T>(); }
+    // [0] DataType, [1] nbits, [2..9] num_elements u64
+    size_t serializeHeader(size_t, uint8_t* buf, size_t max_size) const override {
+        if (max_size < 10) return 0;
+        buf[0] = (uint8_t)dataTypeOf<T>();
+        buf[1] = nbits_;
+        const uint64_t n = num_elements_;
+        std::memcpy(buf + 2, &n, 8);
+        return 10;
+    }
+    void deserializeHeader(const uint8_t* buf, size_t size) override {
+        if (size >= 2) nbits_ = buf[1];
+        if (size >= 10) {
+            uint64_t n;
+            std::memcpy(&n, buf + 2, 8);
+            num_elements_ = n;
+        }
+    }
+    size_t getMaxHeaderSize(size_t) const override { return 10; }
+    void saveState() override { saved_ = {nbits_, num_elements_, actual_output_size_}; }
+    void restoreState() override {
+        nbits_ = saved_.nb;
+        num_elements_ = saved_.n;
+        actual_output_size_ = saved_.out;
+    }
+
+  private:
+    bool is_inverse_ = false, auto_detect_ = false;
+    uint8_t nbits_ = 8 * sizeof(T);
+    size_t num_elements_ = 0, actual_output_size_ = 0;
+    struct Saved { uint8_t nb; size_t n; size_t out; } saved_{8 * sizeof(T), 0, 0};
+    DeviceScratch ws_;
+};
+
+// Rebuilds a stage from its FZM record (include/stage/stage_factory.h:36-248).
+Stage* createStage(StageType type, const uint8_t* config, size_t config_size);
+
+}  // namespace fz

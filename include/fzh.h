@@ -64,8 +64,10 @@ int fzh_workspace_init(void* stream, void* d_ws, size_t ws_bytes);
 
 /* ---- Lorenzo predict + quantize ------------------------------------------
  * dims (nx, ny, nz): ndim = nz>1 ? 3 : ny>1 ? 2 : 1 (lorenzo_quant.h:181-185).
- * n = element count of d_in/d_codes (the 1D path uses n; 2D/3D cover
- * nx*ny[*nz] and write zeros into codes[nx*ny*nz .. n)).
+ * n = element count of d_codes.  1D: elements [nx, n) of the input are read
+ * as zeros when 0 < nx < n (the reference's zero padding, without a padded
+ * copy; d_in needs only nx elements), else d_in holds n elements.  2D/3D
+ * read nx*ny[*nz] elements and write zeros into codes[nx*ny*nz .. n).
  * ebx2_r = 1/(2*abs_eb) in float.  If d_ebx2_r is non-NULL the kernel reads
  * the value from device memory instead (set by fzh_value_base_f32).
  * code_bytes: 1 (u8) or 2 (u16) codes.  radius is cast to the code type.
@@ -98,7 +100,9 @@ int fzh_lorenzo_quant_inv_f32(void* stream, const void* d_codes, size_t n,
  * device.  d_params (4 floats) receives
  *   [0] value_base, [1] abs_eb, [2] ebx2_r = 1/(2*abs_eb), [3] ebx2 = 2*abs_eb
  * with abs_eb = user_eb*value_base, or user_eb when value_base <= 0
- * (lorenzo_quant.cu:612-644).                                                */
+ * (lorenzo_quant.cu:612-644).  mode | FZH_VB_ZERO_PADDED folds a 0.0 into
+ * the min/max, as the reference sees when it reduces a zero-padded copy.    */
+#define FZH_VB_ZERO_PADDED 4
 size_t fzh_value_base_ws_bytes(size_t n);
 int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode,
                        float user_eb, float* d_params, void* d_ws, size_t ws_bytes);
@@ -180,6 +184,42 @@ typedef struct fzh_copy_desc {
     uint64_t bytes;
 } fzh_copy_desc;
 int fzh_gather(void* stream, const fzh_copy_desc* d_descs, int n_segs);
+
+/* Device-side concat of up to FZH_CONCAT_MAX_SLOTS outputs whose sizes may
+ * only be known on the device (replaces Pipeline::writeConcatBuffer,
+ * src/pipeline/compressor.cpp:805-851, which needs the sizes on the host).
+ * Layout: [u32 n][u64 size_0..n-1] zero-padded to 16 B, then each slot's
+ * bytes zero-padded to 16 B (the reference leaves the pads unwritten).
+ * Slot size: kind 0 = `size`; kind 1 = *(const uint64_t*)size_ptr;
+ * kind 2 = *(const uint32_t*)size_ptr * elem; every size is clamped to `cap`
+ * (the slot's source capacity).  slots is a host array (passed by value to
+ * the kernels, so the call is capture-safe); src must be 16-byte aligned.
+ * d_total (u64) receives the concat length.  d_ws: fzh_concat_ws_bytes().  */
+#define FZH_CONCAT_MAX_SLOTS 32
+typedef struct fzh_concat_slot {
+    const void* src;
+    const void* size_ptr;
+    uint64_t size;
+    uint64_t cap;
+    uint32_t kind;
+    uint32_t elem;
+} fzh_concat_slot;
+size_t fzh_concat_ws_bytes(void);
+uint64_t fzh_concat_bound(const fzh_concat_slot* slots, int n);
+int fzh_concat(void* stream, const fzh_concat_slot* slots, int n, void* d_out,
+               uint64_t out_cap, uint64_t* d_total, void* d_ws, size_t ws_bytes);
+
+/* Copies up to FZH_READBACK_MAX small device values (4-byte multiples,
+ * 4-byte aligned) into one host-mapped pinned buffer with a single kernel;
+ * visible on the host after the stream is synchronised.  Entries are passed
+ * by value (capture-safe).                                                  */
+#define FZH_READBACK_MAX 32
+typedef struct fzh_readback_entry {
+    const void* src;
+    uint32_t bytes;
+    uint32_t dst_offset;
+} fzh_readback_entry;
+int fzh_readback(void* stream, const fzh_readback_entry* entries, int n, void* d_mapped_dst);
 
 /* ---- Device helpers for bindings (not on the hot path) ------------------- */
 int fzh_device_alloc(void** d_ptr, size_t bytes);

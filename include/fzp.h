@@ -1,0 +1,104 @@
+/* fzp.h — C ABI over fz::Pipeline (include/fz/pipeline.h).
+ *
+ * For bindings that cannot use the C++ classes (ctypes, cgo, JNI).  Every
+ * call returns 0 on success or a negative FZH_E_* code with the message in
+ * fzh_last_error(); C++ exceptions never cross this boundary.  Handles are
+ * not thread-safe (one pipeline per host thread, as in the reference,
+ * include/mem/mempool.h:80).
+ *
+ * Entry point                     replaces (reference)
+ *   fzp_create / fzp_destroy      Pipeline(size_t, MemoryStrategy, float)
+ *                                 (include/pipeline/compressor.h:41-54)
+ *   fzp_add_lorenzo ...           Pipeline::addStage<T>(...) (compressor.h:82-83,
+ *                                 733-758) with the stage's setters
+ *   fzp_connect                   Pipeline::connect (compressor.h:92)
+ *   fzp_set_dims                  Pipeline::setDims (compressor.h:72)
+ *   fzp_finalize                  Pipeline::finalize (compressor.h:102)
+ *   fzp_compress                  Pipeline::compress (compressor.h:175-181)
+ *   fzp_decompress                Pipeline::decompress (compressor.h:232-238)
+ *   fzp_write_file                Pipeline::writeToFile (compressor.h:357)
+ *   fzp_decompress_file           Pipeline::decompressFromFile (compressor.h:379-386)
+ *   fzp_enable_graph_mode         Pipeline::enableGraphMode (compressor.h:327)
+ *   fzp_enable_profiling /        Pipeline::enableProfiling /
+ *   fzp_last_perf                 getLastPerfResult (compressor.h:278-282)
+ */
+#ifndef FZP_H
+#define FZP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fzh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fzp_pipeline fzp_pipeline;
+
+/* error-bound modes (lorenzo_quant.h ErrorBoundMode) */
+#define FZP_EB_ABS 0
+#define FZP_EB_REL 1
+#define FZP_EB_NOA 2
+
+int fzp_create(size_t input_size_hint, float pool_multiplier, fzp_pipeline** out);
+int fzp_destroy(fzp_pipeline* p);
+
+/* Stage constructors; *stage_id receives the stage's index in this pipeline. */
+int fzp_add_lorenzo(fzp_pipeline* p, float error_bound, int eb_mode, uint32_t quant_radius,
+                    float outlier_capacity, int zigzag, int code_bytes, float value_base,
+                    int* stage_id);
+int fzp_add_bitshuffle(fzp_pipeline* p, uint32_t block_size, int elem_width, int* stage_id);
+int fzp_add_rze(fzp_pipeline* p, uint32_t chunk_size, int levels, int* stage_id);
+int fzp_add_rle(fzp_pipeline* p, int elem_bytes, int* stage_id);
+int fzp_add_huffman(fzp_pipeline* p, int elem_bytes, uint32_t bklen, int* stage_id);
+int fzp_add_quantizer(fzp_pipeline* p, float error_bound, int eb_mode, uint32_t quant_radius,
+                      float outlier_capacity, int zigzag, int code_bytes, float value_base,
+                      int* stage_id);
+/* nbits 0 = auto-detect from the maximum */
+int fzp_add_bitpack(fzp_pipeline* p, int elem_bytes, int nbits, int* stage_id);
+
+int fzp_connect(fzp_pipeline* p, int dependent_id, int producer_id, const char* output_name);
+int fzp_set_dims(fzp_pipeline* p, size_t x, size_t y, size_t z);
+int fzp_finalize(fzp_pipeline* p);
+int fzp_enable_graph_mode(fzp_pipeline* p, int on);
+int fzp_enable_profiling(fzp_pipeline* p, int on);
+int fzp_set_pool_managed_decomp_output(fzp_pipeline* p, int on);
+int fzp_max_compressed_size(fzp_pipeline* p, size_t input_bytes, size_t* out);
+
+/* Pool-owned result, valid until the next fzp_compress on this pipeline. */
+int fzp_compress(fzp_pipeline* p, void* stream, const void* d_in, size_t in_bytes,
+                 void** d_out, size_t* out_bytes);
+/* Result is pool-owned if fzp_set_pool_managed_decomp_output(p, 1), else
+ * caller-owned (release with fzh_device_free). */
+int fzp_decompress(fzp_pipeline* p, void* stream, const void* d_in, size_t in_bytes,
+                   void** d_out, size_t* out_bytes);
+/* Caller-owned output buffers. */
+int fzp_compress_into(fzp_pipeline* p, void* stream, const void* d_in, size_t in_bytes,
+                      void* d_out, size_t out_cap, size_t* out_bytes);
+int fzp_decompress_into(fzp_pipeline* p, void* stream, const void* d_in, size_t in_bytes,
+                        void* d_out, size_t out_cap, size_t* out_bytes);
+
+int fzp_write_file(fzp_pipeline* p, const char* path);
+/* Result is caller-owned (fzh_device_free). */
+int fzp_decompress_file(const char* path, void* stream, void** d_out, size_t* out_bytes);
+
+/* After a profiled call: dag_ms = device time between the first and last
+ * stage (events), host_ms = whole call. */
+int fzp_last_perf(fzp_pipeline* p, float* dag_ms, float* host_ms, size_t* in_bytes,
+                  size_t* out_bytes);
+/* Name and device time of stage i of the last profiled eager compress. */
+int fzp_last_stage_perf(fzp_pipeline* p, int i, char* name, size_t name_cap, float* ms);
+
+/* Results of the last compress: pipeline output count and each output's
+ * name/size, and the resolved absolute error bound of a Lorenzo/Quantizer
+ * stage (REL/NOA resolved on the device). */
+int fzp_num_outputs(fzp_pipeline* p, int* n);
+int fzp_output_info(fzp_pipeline* p, int i, char* name, size_t name_cap, size_t* bytes);
+int fzp_stage_abs_error_bound(fzp_pipeline* p, int stage_id, float* abs_eb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FZP_H */
