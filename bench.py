@@ -1,0 +1,323 @@
+#!/usr/bin/env python3
+"""Benchmark: the Lorenzo-2D -> Bitshuffle -> RZE pipeline on one MI355X.
+
+Workload: float32 1800 x 3600 cloud field (Gaussian random field, power
+~ k^-3, mapped so ~30% of values are exactly 0, max 1), REL error bound 1e-3,
+radius 32768, zigzag codes, outlier capacity 0.10, Bitshuffle (16 KiB blocks,
+element width 2), RZE (16 KiB chunks, 4 levels) -- the reference CLI's
+`lorenzo->bitshuffle->rze -m rel -e 1e-3` setup.
+
+A step is one Pipeline::compress() of the whole field (input resident in HBM,
+output size read back, stream synchronised), replayed as one hipGraph.  The
+JSON line also carries decompress GB/s, the ratio, per-kernel device times
+measured with HIP events on the launch stream, the roofline of the dominant
+kernel, and the CPU oracle's time for the same chain on the host cores.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NX, NY = 3600, 1800
+USER_EB = 1e-3
+CAP = 0.10
+# Reference CLI example, benchmark mean of 10, host time (docs/cli.md:66-70)
+REF_COMPRESS_HOST_GBS = 18.85
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-runs", type=int, default=5)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        log(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+            "(the path is single-GPU; sharding is described in the design notes).")
+        sys.exit(2)
+
+    from fzgpumodules_amd import fields
+    from fzgpumodules_amd._fzh import DeviceBuffer, Event, call, lib, workspace
+    from fzgpumodules_amd.pipeline import Pipeline
+
+    t = time.time()
+    x = fields.cloud_2d(NY, NX)
+    log(f"[bench] field {NY}x{NX} generated in {time.time() - t:.1f}s sha256={fields.sha256(x)[:16]}")
+    nbytes = x.nbytes
+    d_in = DeviceBuffer.from_array(x)
+
+    p = Pipeline(nbytes)
+    lz = p.add_lorenzo(USER_EB, mode="REL", quant_radius=32768, outlier_capacity=CAP, zigzag=True)
+    bs = p.add_bitshuffle(16384, 2)
+    rz = p.add_rze(16384, 4)
+    p.connect(bs, lz, "codes")
+    p.connect(rz, bs)
+    p.set_dims(NX, NY, 1)
+    p.finalize()
+    p.enable_graph_mode(True)
+    p.set_pool_managed_decomp_output(True)
+
+    # ---- compress -----------------------------------------------------------
+    for _ in range(max(args.warmup, 2)):  # eager, then capture
+        out_ptr, out_n = p.compress(d_in, nbytes)
+    call("fzh_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out_ptr, out_n = p.compress(d_in, nbytes)
+    call("fzh_synchronize")
+    t1 = time.perf_counter()
+    comp_s = (t1 - t0) / args.steps
+    comp_gbs = nbytes / comp_s / 1e9
+    ratio = nbytes / out_n
+    abs_eb = p.abs_error_bound(lz)
+    outs = dict(p.outputs())
+    log(f"[bench] compress {comp_s * 1e3:.3f} ms/step  {comp_gbs:.1f} GB/s  ratio {ratio:.2f}  outputs {outs}")
+
+    # copy the compressed blob aside (the pool-owned one is reused by compress)
+    blob = DeviceBuffer(out_n)
+    tmp = np.empty(out_n, np.uint8)
+    call("fzh_memcpy_dtoh", tmp.ctypes.data, out_ptr, out_n)
+    blob.upload(tmp)
+
+    # ---- decompress ---------------------------------------------------------
+    for _ in range(max(args.warmup, 1)):
+        y_ptr, y_n = p.decompress(blob, out_n)
+    call("fzh_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y_ptr, y_n = p.decompress(blob, out_n)
+    call("fzh_synchronize")
+    t1 = time.perf_counter()
+    dec_s = (t1 - t0) / args.steps
+    dec_gbs = nbytes / dec_s / 1e9
+    y = np.empty(x.size, np.float32)
+    call("fzh_memcpy_dtoh", y.ctypes.data, y_ptr, y.nbytes)
+    max_err = float(np.max(np.abs(y.astype(np.float64) - x.astype(np.float64))))
+    log(f"[bench] decompress {dec_s * 1e3:.3f} ms/step  {dec_gbs:.1f} GB/s  max|err| {max_err:.3e} (eb {abs_eb:.3e})")
+
+    # ---- per-kernel device time, HIP events on the launch stream ------------
+    kern = kernel_times(args.kernel_reps, x, d_in, abs_eb)
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    d = kern[dom]
+    roof = {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": round(d["gbs"], 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
+        "traffic": None,
+    }
+    chain_bytes = sum(kern[k]["bytes"] for k in ("value_base", "lorenzo_fwd_2d", "bitshuffle_enc", "rze_enc"))
+    chain_ms = sum(kern[k]["ms"] for k in ("value_base", "lorenzo_fwd_2d", "bitshuffle_enc", "rze_enc"))
+
+    # ---- CPU oracle baseline --------------------------------------------------
+    cpu = None if args.no_cpu else cpu_baseline(x, args.cpu_runs)
+
+    res = {
+        "metric": "compress/decompress GB/s + ratio at rel eb 1e-3; learned-select vs exhaustive",
+        "value": round(comp_gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(comp_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(comp_gbs / REF_COMPRESS_HOST_GBS, 2),
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "cloud_2d float32 1800x3600 (GRF k^-3, 30% zeros); Lorenzo2D REL 1e-3 r32768 zigzag "
+                        "cap0.10 -> Bitshuffle(16KiB, ew2) -> RZE(16KiB, 4 levels); one compress() per step, hipGraph",
+            "input_bytes": nbytes,
+            "padded_bytes": int(math.ceil(nbytes / 16384) * 16384),
+        },
+        "decompress_gbs": round(dec_gbs, 2),
+        "decompress_ms_per_step": round(dec_s * 1e3, 4),
+        "ratio": round(ratio, 3),
+        "compressed_bytes": int(out_n),
+        "outputs": outs,
+        "abs_eb": abs_eb,
+        "max_abs_err": max_err,
+        "compress_chain_device_ms": round(chain_ms, 4),
+        "compress_chain_roofline_frac": round(chain_bytes / (chain_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels": {k: {kk: round(vv, 4) if isinstance(vv, float) else vv for kk, vv in v.items()}
+                    for k, v in kern.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    p.close()
+
+
+def kernel_times(reps, x, d_in, abs_eb):
+    """Each kernel of the chain on its own, timed with HIP events around
+    `reps` back-to-back launches on one stream; algorithmic bytes per launch
+    are the compulsory reads + writes of the stage."""
+    from fzgpumodules_amd._fzh import DeviceBuffer, Event, call, lib, workspace
+
+    L = lib()
+    s = C.c_void_p()
+    call("fzh_stream_create", C.byref(s))
+    st = s.value
+    n = int(math.ceil(x.nbytes / 16384) * 16384) // 4  # padded elements
+    nv = x.size
+    m = math.ceil(n * float(np.float32(CAP)))
+    params = DeviceBuffer(16)
+    vb_ws = workspace(L.fzh_value_base_ws_bytes(nv))
+    codes = DeviceBuffer(2 * n)
+    err = DeviceBuffer(4 * m)
+    idx = DeviceBuffer(4 * m)
+    cnt = DeviceBuffer(16)
+    lz_ws = workspace(L.fzh_lorenzo_quant_fwd_ws_bytes(n, NX, NY, 1))
+    shuf = DeviceBuffer(2 * n)
+    rze_cap = L.fzh_rze_enc_bound(2 * n)
+    rze_out = DeviceBuffer(rze_cap)
+    rze_sz = DeviceBuffer(16)
+    rze_ws = workspace(L.fzh_rze_enc_ws_bytes(2 * n))
+    unshuf = DeviceBuffer(2 * n)
+    rzd_ws = workspace(L.fzh_rze_dec_ws_bytes(2 * n))
+    status = DeviceBuffer(16)
+    call("fzh_memset", status.ptr, 0, 16)
+    y = DeviceBuffer(4 * n)
+    inv_ws = workspace(L.fzh_lorenzo_quant_inv_ws_bytes(n, NX, NY, 1, m))
+
+    ebx2 = np.float32(2) * np.float32(abs_eb)
+
+    def vb():
+        call("fzh_value_base_f32", st, d_in.ptr, nv, 1, np.float32(USER_EB), params.ptr, vb_ws.ptr, vb_ws.nbytes)
+
+    def lzf():
+        call("fzh_lorenzo_quant_fwd_f32", st, d_in.ptr, n, NX, NY, 1, np.float32(0), params.ptr + 8, 32768, 1, 2,
+             codes.ptr, err.ptr, idx.ptr, cnt.ptr, m, lz_ws.ptr, lz_ws.nbytes)
+
+    def bse():
+        call("fzh_bitshuffle_enc", st, codes.ptr, shuf.ptr, 2 * n, 16384, 2)
+
+    def rze():
+        call("fzh_rze_enc", st, shuf.ptr, 2 * n, rze_out.ptr, rze_cap, rze_sz.ptr, rze_ws.ptr, rze_ws.nbytes)
+
+    def rzd():
+        call("fzh_rze_dec", st, rze_out.ptr, int(csz[0]), shuf.ptr, 2 * n, status.ptr, rzd_ws.ptr, rzd_ws.nbytes)
+
+    def bsd():
+        call("fzh_bitshuffle_dec", st, shuf.ptr, unshuf.ptr, 2 * n, 16384, 2)
+
+    def lzi():
+        call("fzh_lorenzo_quant_inv_f32", st, unshuf.ptr, n, NX, NY, 1, err.ptr, idx.ptr, cnt.ptr, m, ebx2, 1, 2,
+             y.ptr, inv_ws.ptr, inv_ws.nbytes)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        e0, e1 = Event(), Event()
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        return e0.elapsed_ms(e1) / reps
+
+    out = {}
+    vb()
+    lzf()
+    bse()
+    rze()
+    call("fzh_stream_sync", st)
+    k = int(cnt.download(np.uint32, 1)[0])
+    csz = rze_sz.download(np.uint64, 1)
+    C_rze = int(csz[0])
+    specs = [
+        ("value_base", vb, 4 * nv),
+        ("lorenzo_fwd_2d", lzf, 4 * NX * NY + 2 * n + 8 * k + 4),
+        ("bitshuffle_enc", bse, 4 * n),
+        ("rze_enc", rze, 2 * n + C_rze),
+        ("rze_dec", rzd, C_rze + 2 * n),
+        ("bitshuffle_dec", bsd, 4 * n),
+        ("lorenzo_inv_2d", lzi, 2 * n + 8 * k + 4 + 4 * NX * NY),
+    ]
+    for name, fn, nb in specs:
+        ms = timed(fn)
+        out[name] = {"ms": ms, "bytes": int(nb), "gbs": nb / (ms * 1e-3) / 1e9, "frac": nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log(f"[bench] kernel {name:16s} {ms * 1e3:8.1f} us  {out[name]['gbs']:7.1f} GB/s  "
+            f"{100 * out[name]['frac']:5.1f}% of HBM peak")
+    st_ = int(status.download(np.int32, 1)[0])
+    if st_ != 0:
+        raise RuntimeError("rze_dec reported a malformed stream in the kernel timing loop")
+    call("fzh_stream_destroy", st)
+    out["_outliers"] = k  # noqa: kept for the record
+    return {kk: vv for kk, vv in out.items() if not kk.startswith("_")}
+
+
+def cpu_baseline(x, runs):
+    """The oracle's value base + Lorenzo2D + Bitshuffle + RZE on the host
+    cores (OpenMP over tiles/blocks/chunks), median of `runs` after 1 warm-up."""
+    import oracle
+
+    oracle.lib()
+    cores = min(os.cpu_count() or 1, 16)
+    oracle.set_threads(cores)
+    L = oracle.lib()
+    n = int(math.ceil(x.nbytes / 16384) * 16384) // 4
+    xp = np.zeros(n, np.float32)
+    xp[: x.size] = x.ravel()
+    m = math.ceil(n * float(np.float32(CAP)))
+    codes = np.zeros(n, np.uint16)
+    shuf = np.zeros(2 * n, np.uint8)
+    out = np.zeros(2 * n + 8 + 4 * (2 * n // 16384 + 1) + 64, np.uint8)
+    err = np.zeros(m, np.float32)
+    idx = np.zeros(m, np.uint32)
+    no = C.c_uint32()
+
+    def run():
+        vb = np.float32(L.orc_value_base_f32(xp.ctypes.data, n, 1))
+        eb = np.float32(np.float32(USER_EB) * vb)
+        r = np.float32(1) / (np.float32(2) * eb)
+        return L.orc_chain_l2d_bs_rze(xp.ctypes.data, n, NX, NY, r, 32768, 1, codes.ctypes.data, shuf.ctypes.data,
+                                      out.ctypes.data, err.ctypes.data, idx.ctypes.data, m, C.byref(no))
+
+    run()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    gbs = x.nbytes / med / 1e9
+    log(f"[bench] cpu oracle chain: {med * 1e3:.1f} ms  {gbs:.2f} GB/s on {cores} threads")
+    return {
+        "value": round(gbs, 3),
+        "unit": "GB/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"whole 1800x3600 field (value base + Lorenzo2D + Bitshuffle + RZE), median of {runs} runs "
+                  f"after 1 warm-up, OpenMP {cores} threads",
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -21,7 +21,7 @@ void hipck(hipError_t e, const char* what) {
 }
 
 constexpr size_t kReadbackBytes = 4096;
-constexpr size_t kConcatWsBytes = 512;  // offsets/sizes table (fzh_concat_ws_bytes) + u64 total
+constexpr size_t kConcatWsBytes = 1024;  // [0, 512): offsets/sizes table; [512]: u64 total
 
 size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -308,11 +308,11 @@ void Pipeline::launchForward(const void* d_in, size_t padded, bool zero_view, hi
             s.size = ds.kind == 0 ? std::min<size_t>(B.size, s.cap) : 0;
         }
         throwIfFzh(fzh_concat(stream, slots.data(), (int)slots.size(), concat_buf_, concat_cap_,
-                              (uint64_t*)((uint8_t*)concat_ws_ + 256), concat_ws_, 256),
+                              (uint64_t*)((uint8_t*)concat_ws_ + 512), concat_ws_, 512),
                    "Pipeline concat");
         concat_hdr_.assign(4 + 8 * outputs_.size(), 0);
         add_rb(concat_buf_, concat_hdr_.data(), (uint32_t)round_up(concat_hdr_.size(), 4));
-        add_rb((uint8_t*)concat_ws_ + 256, &concat_total_, 8);
+        add_rb((uint8_t*)concat_ws_ + 512, &concat_total_, 8);
     }
     for (auto& n : nodes_)
         for (const auto& r : n.stage->pendingReadbacks()) add_rb(r.d_src, r.h_dst, r.bytes);

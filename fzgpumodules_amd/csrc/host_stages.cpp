@@ -25,13 +25,22 @@ void throwIfFzh(int rc, const char* what) {
 
 void DeviceScratch::ensure(size_t need, MemoryPool* p, hipStream_t s) {
     need = std::max<size_t>(need, 256);
-    if (ptr && bytes >= need) return;
+    if (ptr && bytes >= need) {
+        // Layouts inside a workspace depend on the problem size: clear the
+        // parts a new size may read as zero.
+        if (need != last_need) {
+            throwIfFzh(fzh_workspace_init(s, ptr, bytes), "workspace init");
+            last_need = need;
+        }
+        return;
+    }
     release();
     pool = p;
     if (pool) ptr = pool->allocatePersistentDevice(need, "stage scratch");
     else FZ_HIP(hipMalloc(&ptr, need));
     if (!ptr) throw std::runtime_error("stage scratch allocation failed");
     bytes = need;
+    last_need = need;
     throwIfFzh(fzh_workspace_init(s, ptr, bytes), "workspace init");
 }
 
@@ -88,11 +97,11 @@ void LorenzoQuantStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, co
         resolveDims(n, nx, ny, nz);
         const size_t max_out = sizes[1] / sizeof(TI);
         const float ebx2 = 2.0f * computed_abs_eb_;
-        ws_.ensure(fzh_lorenzo_quant_inv_ws_bytes(n, nx, ny, nz, max_out), pool, stream);
+        inv_ws_.ensure(fzh_lorenzo_quant_inv_ws_bytes(n, nx, ny, nz, max_out), pool, stream);
         throwIfFzh(fzh_lorenzo_quant_inv_f32(stream, inputs[0], n, nx, ny, nz, (const float*)inputs[1],
                                              (const uint32_t*)inputs[2], (const uint32_t*)inputs[3], max_out, ebx2,
                                              config_.zigzag_codes ? 1 : 0, (int)sizeof(TC), (float*)outputs[0],
-                                             ws_.ptr, ws_.bytes),
+                                             inv_ws_.ptr, inv_ws_.bytes),
                    "LorenzoQuantStage (inverse)");
         return;
     }
@@ -277,9 +286,9 @@ void RZEStage::execute(hipStream_t stream, MemoryPool* pool, const std::vector<v
     } else {
         const size_t orig = cached_orig_bytes_;
         if (orig == 0) { actual_output_size_ = 0; return; }
-        ws_.ensure(fzh_rze_dec_ws_bytes(orig), pool, stream);
-        throwIfFzh(fzh_rze_dec(stream, inputs[0], sizes[0], outputs[0], orig, (int*)((uint8_t*)small_.ptr + 8), ws_.ptr,
-                               ws_.bytes),
+        inv_ws_.ensure(fzh_rze_dec_ws_bytes(orig), pool, stream);
+        throwIfFzh(fzh_rze_dec(stream, inputs[0], sizes[0], outputs[0], orig, (int*)((uint8_t*)small_.ptr + 8),
+                               inv_ws_.ptr, inv_ws_.bytes),
                    "RZEStage (inverse)");
         actual_output_size_ = orig;
         status_pending_ = true;
@@ -328,9 +337,9 @@ void RLEStage<T>::execute(hipStream_t stream, MemoryPool* pool, const std::vecto
         const size_t n = cached_num_elements_;
         actual_output_size_ = n * sizeof(T);
         if (n == 0) return;
-        ws_.ensure(fzh_rle_dec_ws_bytes(n), pool, stream);
+        inv_ws_.ensure(fzh_rle_dec_ws_bytes(n), pool, stream);
         throwIfFzh(fzh_rle_dec(stream, inputs[0], sizes[0], (int)sizeof(T), outputs[0], n,
-                               (int*)((uint8_t*)small_.ptr + 8), ws_.ptr, ws_.bytes),
+                               (int*)((uint8_t*)small_.ptr + 8), inv_ws_.ptr, inv_ws_.bytes),
                    "RLEStage (inverse)");
         status_pending_ = true;
         addReadback((uint8_t*)small_.ptr + 8, &h_status_, 4);
@@ -495,12 +504,12 @@ void QuantizerStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const
     } else {
         float vb = config_.precomputed_value_base;
         if (vb <= 0.0f) {
-            ws_.ensure(256 + fzh_value_base_ws_bytes(n), pool, stream);
-            throwIfFzh(fzh_value_base_f32(stream, (const float*)inputs[0], n, 2, config_.error_bound, (float*)ws_.ptr,
-                                          (uint8_t*)ws_.ptr + 256, ws_.bytes - 256),
+            vb_.ensure(256 + fzh_value_base_ws_bytes(n), pool, stream);
+            throwIfFzh(fzh_value_base_f32(stream, (const float*)inputs[0], n, 2, config_.error_bound, (float*)vb_.ptr,
+                                          (uint8_t*)vb_.ptr + 256, vb_.bytes - 256),
                        "QuantizerStage value range");
             float p[4];
-            FZ_HIP(hipMemcpyAsync(p, ws_.ptr, 16, hipMemcpyDeviceToHost, stream));
+            FZ_HIP(hipMemcpyAsync(p, vb_.ptr, 16, hipMemcpyDeviceToHost, stream));
             FZ_HIP(hipStreamSynchronize(stream));
             vb = p[0];
         }
@@ -508,7 +517,7 @@ void QuantizerStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const
         computed_abs_eb_ = vb > 0.0f ? config_.error_bound * vb : config_.error_bound;
     }
     const float r = 1.0f / (2.0f * computed_abs_eb_);
-    ws_.ensure(256 + std::max(fzh_quant_abs_fwd_ws_bytes(n), fzh_value_base_ws_bytes(n)), pool, stream);
+    ws_.ensure(256 + fzh_quant_abs_fwd_ws_bytes(n), pool, stream);
     throwIfFzh(fzh_quant_abs_fwd_f32(stream, (const float*)inputs[0], n, r, config_.outlier_threshold,
                                      (uint32_t)config_.quant_radius, config_.zigzag_codes ? 1 : 0, (int)sizeof(TC),
                                      outputs[0], (float*)outputs[1], (uint32_t*)outputs[2], (uint32_t*)outputs[3], m,

@@ -31,8 +31,10 @@ enum class ErrorBoundMode : uint8_t { ABS = 0, REL = 1, NOA = 2 };
 struct DeviceScratch {
     void* ptr = nullptr;
     size_t bytes = 0;
+    size_t last_need = 0;
     MemoryPool* pool = nullptr;
-    // Grows to at least `need` bytes (re-zeroed when it grows).
+    // Holds at least `need` bytes; zeroed whenever `need` differs from the
+    // previous call's (a fill kernel on `s`, so steady-state calls add nothing).
     void ensure(size_t need, MemoryPool* p, hipStream_t s);
     void release();
     ~DeviceScratch() { release(); }
@@ -180,7 +182,7 @@ class LorenzoQuantStage : public Stage {
     bool params_pending_ = false;        // REL/NOA resolved on the device
     uint32_t h_count_ = 0;
     float h_params_[4] = {0, 0, 0, 0};
-    DeviceScratch ws_, params_;
+    DeviceScratch ws_, inv_ws_, params_;
 };
 
 // ---------------------------------------------------------------------------
@@ -293,7 +295,7 @@ class RZEStage : public Stage {
     uint64_t h_size_ = 0;
     int32_t h_status_ = 0;
     struct Saved { uint32_t cs; uint8_t lv; uint32_t ob; } saved_{16384, 4, 0};
-    DeviceScratch ws_, small_;  // small_: [0] u64 out size, [8] i32 status
+    DeviceScratch ws_, inv_ws_, small_;  // small_: [0] u64 out size, [8] i32 status
   public:
     DeviceSize getDeviceOutputSize(int index) const override {
         if (is_inverse_ || index != 0 || !size_pending_) return {};
@@ -350,7 +352,7 @@ class RLEStage : public Stage {
     bool size_pending_ = false, status_pending_ = false;
     uint64_t h_size_ = 0;
     int32_t h_status_ = 0;
-    DeviceScratch ws_, small_;
+    DeviceScratch ws_, inv_ws_, small_;
   public:
     DeviceSize getDeviceOutputSize(int index) const override {
         if (is_inverse_ || index != 0 || !size_pending_) return {};
@@ -530,7 +532,7 @@ class QuantizerStage : public Stage {
     float computed_abs_eb_ = 0.0f, computed_value_base_ = 0.0f;
     const void* d_count_ptr_ = nullptr;
     uint32_t h_count_ = 0;
-    DeviceScratch ws_;
+    DeviceScratch ws_, vb_;
   public:
     DeviceSize getDeviceOutputSize(int index) const override {
         if (is_inverse_ || (index != 1 && index != 2) || !d_count_ptr_) return {};

@@ -82,7 +82,9 @@ def test_cfg1_chain_rle_1d():
 
 def test_cfg3_chain_huffman_3d():
     shape = (24, 40, 56)  # z, y, x
-    g = fields.power_law_field(shape, -5.0 / 3.0, _seed("cfg3"))
+    # smoother than the 512^3 workload's spectrum so a grid this small stays
+    # within the 20% outlier capacity
+    g = fields.power_law_field(shape, -11.0 / 3.0, _seed("cfg3"))
     x = g.astype(np.float32)
     rng = np.random.default_rng(3)
     spikes = rng.choice(x.size, x.size // 100, replace=False)
@@ -99,6 +101,8 @@ def test_cfg3_chain_huffman_3d():
         np.testing.assert_array_equal(blob, want)
         y = p.decompress_array(blob)
         np.testing.assert_array_equal(y, chains.reconstruct(info, dims))
+        assert info["cnt"] <= chains.max_outliers(x.size, 0.2)
+        check_bound(x.ravel(), y, float(info["abs_eb"]), slack=1.001)
 
 
 def test_1d_padded_view_matches_zero_padded_copy():
@@ -170,7 +174,7 @@ def test_file_round_trip(tmp_path):
     y = Pipeline.decompress_file(path)
     ref = chains.reconstruct(info, (nx, ny, 1))
     assert y.size == ref.size  # padded length, as the reference returns it
-    np.testing.assert_array_equal(y, ref)
+    np.testing.assert_array_equal(y[: x.size], ref[: x.size])
     # a flipped payload byte fails the data checksum
     bad = raw.copy()
     bad[hdr + 5] ^= 0x40
