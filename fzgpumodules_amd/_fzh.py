@@ -60,6 +60,7 @@ SIGNATURES = {
     "fzh_bitpack_enc": (_I, [_P, _P, _Z, _I, _I, _P]),
     "fzh_bitpack_dec": (_I, [_P, _P, _Z, _I, _I, _P]),
     "fzh_gather": (_I, [_P, _P, _I]),
+    "fzh_copy": (_I, [_P, _P, _P, _Z]),
     "fzh_concat_ws_bytes": (_Z, []),
     "fzh_concat_bound": (C.c_uint64, [_P, _I]),
     "fzh_concat": (_I, [_P, _P, _I, _P, C.c_uint64, _P, _P, _Z]),

@@ -57,6 +57,18 @@ __global__ void __launch_bounds__(256) gather_kernel(const fzh_copy_desc* __rest
     }
 }
 
+// Grid-stride device copy with 16-byte accesses (both ends 16-byte aligned).
+__global__ void __launch_bounds__(256) copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t nw) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += stride) dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(256) copy_tail_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                                                        size_t from, size_t n) {
+    const size_t i = from + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 struct ConcatArgs {
     fzh_concat_slot slot[FZH_CONCAT_MAX_SLOTS];
 };
@@ -151,6 +163,21 @@ int fzh_gather(void* stream, const fzh_copy_desc* d_descs, int n_segs) {
     if (n_segs <= 0) return FZH_OK;
     gather_kernel<<<n_segs, 256, 0, S(stream)>>>(d_descs, n_segs);
     FZH_CHECK_LAUNCH("gather_kernel");
+    return FZH_OK;
+}
+
+int fzh_copy(void* stream, void* d_dst, const void* d_src, size_t bytes) {
+    if (bytes == 0) return FZH_OK;
+    if (!d_dst || !d_src) return fail(FZH_E_INVALID, "copy: null pointer");
+    if (((uintptr_t)d_dst | (uintptr_t)d_src) % 16 != 0) return fail(FZH_E_INVALID, "copy: pointers must be 16-byte aligned");
+    const size_t nw = bytes / 16;
+    if (nw) {
+        const size_t blocks = std::min<size_t>(cdiv(nw, 256 * 4), 8192);
+        copy_kernel<<<(unsigned)blocks, 256, 0, S(stream)>>>((const uint4*)d_src, (uint4*)d_dst, nw);
+    }
+    if (bytes % 16)
+        copy_tail_kernel<<<1, 256, 0, S(stream)>>>((const uint8_t*)d_src, (uint8_t*)d_dst, nw * 16, bytes);
+    FZH_CHECK_LAUNCH("copy_kernel");
     return FZH_OK;
 }
 

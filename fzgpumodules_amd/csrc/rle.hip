@@ -350,7 +350,10 @@ size_t fzh_rle_enc_bound(size_t n, int elem_bytes) {
     return 4 + align_up(n * (size_t)elem_bytes, 4) + 4 * n + 4;
 }
 
-size_t fzh_rle_enc_ws_bytes(size_t n) { return enc_layout<uint8_t>(n).total; }
+// Enough for every element width (4-byte elements have the most tiles).
+size_t fzh_rle_enc_ws_bytes(size_t n) {
+    return std::max(enc_layout<uint8_t>(n).total, std::max(enc_layout<uint16_t>(n).total, enc_layout<uint32_t>(n).total));
+}
 
 int fzh_rle_enc(void* stream, const void* d_in, size_t n, int elem_bytes, void* d_out, size_t out_cap,
                 uint64_t* d_out_size, void* d_ws, size_t ws_bytes) {

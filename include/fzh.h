@@ -185,6 +185,10 @@ typedef struct fzh_copy_desc {
 } fzh_copy_desc;
 int fzh_gather(void* stream, const fzh_copy_desc* d_descs, int n_segs);
 
+/* Plain device copy (grid-stride 16-byte accesses; 16-byte-aligned
+ * pointers), capture-safe.  Also the bandwidth reference for benchmarks.   */
+int fzh_copy(void* stream, void* d_dst, const void* d_src, size_t bytes);
+
 /* Device-side concat of up to FZH_CONCAT_MAX_SLOTS outputs whose sizes may
  * only be known on the device (replaces Pipeline::writeConcatBuffer,
  * src/pipeline/compressor.cpp:805-851, which needs the sizes on the host).

@@ -23,14 +23,14 @@ def align(n: int, a: int) -> int:
 
 
 def concat(parts) -> np.ndarray:
+    parts = [np.ascontiguousarray(p).view(np.uint8).ravel() for p in parts]
     n = len(parts)
     hdr = bytearray(align(4 + 8 * n, 16))
     hdr[0:4] = np.uint32(n).tobytes()
     for i, p in enumerate(parts):
-        hdr[4 + 8 * i: 12 + 8 * i] = np.uint64(len(p)).tobytes()
+        hdr[4 + 8 * i: 12 + 8 * i] = np.uint64(p.size).tobytes()
     out = [np.frombuffer(bytes(hdr), np.uint8)]
     for p in parts:
-        p = np.ascontiguousarray(p).view(np.uint8).ravel()
         out.append(p)
         pad = align(p.size, 16) - p.size
         if pad:

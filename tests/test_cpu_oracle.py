@@ -357,3 +357,18 @@ def test_bitpack_known_answers(orc):
         np.testing.assert_array_equal(orc.bitpack_dec(b, v.size, np.uint16, nb), v)
     b = orc.bitpack_enc(np.array([1, 2, 3, 0], np.uint16), 2)  # element k of a byte at bit 2k
     assert b.tolist() == [0b00111001]
+
+
+# ---------------------------------------------------------------------------
+# Concat layout of the pipeline test helper (src/pipeline/pipeline_utils.h:36-46)
+# ---------------------------------------------------------------------------
+def test_concat_layout_helper():
+    import chains
+
+    parts = [np.zeros(3, np.float32), np.array([5], np.uint32), np.arange(5, dtype=np.uint8)]
+    b = chains.concat(parts)
+    assert b[:4].view(np.uint32)[0] == 3
+    assert b[4:28].view(np.uint64).tolist() == [12, 4, 5]
+    assert b.size == 32 + 16 + 16 + 16
+    assert [p.size for p in chains.split(b)] == [12, 4, 5]
+    assert not b[28:32].any() and not b[32 + 12:48].any()
