@@ -72,6 +72,9 @@ def main():
 
     p = Pipeline(nbytes)
     lz = p.add_lorenzo(USER_EB, mode="REL", quant_radius=32768, outlier_capacity=CAP, zigzag=True)
+    # strict error bound: reference-identical codes and streams, plus the few
+    # values the reference arithmetic would reconstruct outside eb
+    p.set_strict(lz, True)
     bs = p.add_bitshuffle(16384, 2)
     rz = p.add_rze(16384, 4)
     p.connect(bs, lz, "codes")
@@ -153,7 +156,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "cloud_2d float32 1800x3600 (GRF k^-3, 30% zeros); Lorenzo2D REL 1e-3 r32768 zigzag "
-                        "cap0.10 -> Bitshuffle(16KiB, ew2) -> RZE(16KiB, 4 levels); one compress() per step, hipGraph",
+                        "cap0.10 strict-eb -> Bitshuffle(16KiB, ew2) -> RZE(16KiB, 4 levels); one compress() per "
+                        "step, hipGraph",
             "input_bytes": nbytes,
             "padded_bytes": int(math.ceil(nbytes / 16384) * 16384),
         },

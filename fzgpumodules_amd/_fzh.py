@@ -29,6 +29,9 @@ SIGNATURES = {
     "fzh_workspace_init": (_I, [_P, _P, _Z]),
     "fzh_lorenzo_quant_fwd_ws_bytes": (_Z, [_Z, _Z, _Z, _Z]),
     "fzh_lorenzo_quant_fwd_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _F, _P, _U32, _I, _I, _P, _P, _P, _P, _Z, _P, _Z]),
+    "fzh_lorenzo_quant_fwd_strict_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _F, _P, _U32, _I, _I, _P, _P, _P, _P, _Z,
+                                             _F, _F, _P, _P, _P, _P, _Z, _P, _Z]),
+    "fzh_lorenzo_apply_patches": (_I, [_P, _P, _P, _P, _P, _Z, _Z]),
     "fzh_lorenzo_quant_inv_ws_bytes": (_Z, [_Z, _Z, _Z, _Z, _Z]),
     "fzh_lorenzo_quant_inv_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _F, _I, _I, _P, _P, _Z]),
     "fzh_value_base_ws_bytes": (_Z, [_Z]),
@@ -69,6 +72,7 @@ SIGNATURES = {
     "fzp_create": (_I, [_Z, _F, C.POINTER(_P)]),
     "fzp_destroy": (_I, [_P]),
     "fzp_add_lorenzo": (_I, [_P, _F, _I, _U32, _F, _I, _I, _F, C.POINTER(_I)]),
+    "fzp_set_lorenzo_strict": (_I, [_P, _I, _I, _F]),
     "fzp_add_bitshuffle": (_I, [_P, _U32, _I, C.POINTER(_I)]),
     "fzp_add_rze": (_I, [_P, _U32, _I, C.POINTER(_I)]),
     "fzp_add_rle": (_I, [_P, _I, C.POINTER(_I)]),

@@ -103,6 +103,21 @@ int fzp_add_lorenzo(fzp_pipeline* p, float eb, int eb_mode, uint32_t radius, flo
     });
 }
 
+int fzp_set_lorenzo_strict(fzp_pipeline* p, int id, int on, float cap) {
+    return guard([&] {
+        fz::Stage* s = stage_at(p, id);
+        if (auto* a = dynamic_cast<fz::LorenzoQuantStage<float, uint16_t>*>(s)) {
+            a->setStrictErrorBound(on != 0);
+            if (cap > 0) a->setPatchCapacity(cap);
+        } else if (auto* b = dynamic_cast<fz::LorenzoQuantStage<float, uint8_t>*>(s)) {
+            b->setStrictErrorBound(on != 0);
+            if (cap > 0) b->setPatchCapacity(cap);
+        } else {
+            throw std::invalid_argument("stage is not a LorenzoQuant stage");
+        }
+    });
+}
+
 int fzp_add_bitshuffle(fzp_pipeline* p, uint32_t block_size, int ew, int* id) {
     return guard([&] {
         need(p);

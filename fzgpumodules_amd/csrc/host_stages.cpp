@@ -73,7 +73,12 @@ std::vector<size_t> LorenzoQuantStage<TI, TC>::estimateOutputSizes(const std::ve
     if (is_inverse_) return {num_elements_ * sizeof(TI)};
     const size_t n = in.empty() ? 0 : in[0] / sizeof(TI);
     const size_t m = maxOutliers(n);
-    return {n * sizeof(TC), m * sizeof(TI), m * sizeof(uint32_t), sizeof(uint32_t)};
+    std::vector<size_t> r = {n * sizeof(TC), m * sizeof(TI), m * sizeof(uint32_t), sizeof(uint32_t)};
+    if (config_.strict_error_bound) {
+        const size_t pm = maxPatches(n);
+        r.insert(r.end(), {pm * sizeof(TI), pm * sizeof(uint32_t), sizeof(uint32_t)});
+    }
+    return r;
 }
 
 template <typename TI, typename TC>
@@ -88,8 +93,9 @@ template <typename TI, typename TC>
 void LorenzoQuantStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
                                         const std::vector<void*>& outputs, const std::vector<size_t>& sizes) {
     if (is_inverse_) {
-        if (inputs.size() < 4 || outputs.empty() || sizes.size() < 4)
-            throw std::runtime_error("LorenzoQuantStage (inverse): Requires 4 inputs and 1 output");
+        if (inputs.size() < numPorts() || outputs.empty() || sizes.size() < numPorts())
+            throw std::runtime_error("LorenzoQuantStage (inverse): Requires " + std::to_string(numPorts()) +
+                                     " inputs and 1 output");
         const size_t n = sizes[0] / sizeof(TC);
         actual_output_sizes_.assign(1, n * sizeof(TI));
         if (n == 0) return;
@@ -103,10 +109,15 @@ void LorenzoQuantStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, co
                                              config_.zigzag_codes ? 1 : 0, (int)sizeof(TC), (float*)outputs[0],
                                              inv_ws_.ptr, inv_ws_.bytes),
                    "LorenzoQuantStage (inverse)");
+        if (config_.strict_error_bound)
+            throwIfFzh(fzh_lorenzo_apply_patches(stream, (float*)outputs[0], (const float*)inputs[4],
+                                                 (const uint32_t*)inputs[5], (const uint32_t*)inputs[6],
+                                                 sizes[4] / sizeof(TI), n),
+                       "LorenzoQuantStage (inverse) patches");
         return;
     }
-    if (inputs.empty() || outputs.size() < 4 || sizes.empty())
-        throw std::runtime_error("LorenzoQuantStage: Requires 1 input and 4 outputs");
+    if (inputs.empty() || outputs.size() < numPorts() || sizes.empty())
+        throw std::runtime_error("LorenzoQuantStage: Requires 1 input and " + std::to_string(numPorts()) + " outputs");
     const size_t n = sizes[0] / sizeof(TI);
     const size_t valid = (valid_elems_ > 0 && valid_elems_ < n) ? valid_elems_ : n;
     valid_elems_ = 0;
@@ -115,8 +126,10 @@ void LorenzoQuantStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, co
     actual_outlier_count_ = 0;
     clearReadbacks();
     d_count_ptr_ = nullptr;
+    d_pcount_ptr_ = nullptr;
+    h_pcount_ = 0;
     if (n == 0) {
-        actual_output_sizes_.assign(4, 0);
+        actual_output_sizes_.assign(numPorts(), 0);
         return;
     }
     size_t nx, ny, nz;
@@ -149,14 +162,32 @@ void LorenzoQuantStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, co
         addReadback(d_params, h_params_, sizeof(h_params_));
     }
     ws_.ensure(fzh_lorenzo_quant_fwd_ws_bytes(n, nx, ny, nz), pool, stream);
-    throwIfFzh(fzh_lorenzo_quant_fwd_f32(stream, (const float*)inputs[0], n, nx, ny, nz, r, d_r,
-                                         (uint32_t)config_.quant_radius, config_.zigzag_codes ? 1 : 0, (int)sizeof(TC),
-                                         outputs[0], (float*)outputs[1], (uint32_t*)outputs[2], (uint32_t*)outputs[3],
-                                         max_out, ws_.ptr, ws_.bytes),
-               "LorenzoQuantStage");
+    if (!config_.strict_error_bound) {
+        throwIfFzh(fzh_lorenzo_quant_fwd_f32(stream, (const float*)inputs[0], n, nx, ny, nz, r, d_r,
+                                             (uint32_t)config_.quant_radius, config_.zigzag_codes ? 1 : 0,
+                                             (int)sizeof(TC), outputs[0], (float*)outputs[1], (uint32_t*)outputs[2],
+                                             (uint32_t*)outputs[3], max_out, ws_.ptr, ws_.bytes),
+                   "LorenzoQuantStage");
+    } else {
+        const size_t pm = maxPatches(n);
+        const float eb = computed_abs_eb_;
+        throwIfFzh(fzh_lorenzo_quant_fwd_strict_f32(
+                       stream, (const float*)inputs[0], n, nx, ny, nz, r, d_r, (uint32_t)config_.quant_radius,
+                       config_.zigzag_codes ? 1 : 0, (int)sizeof(TC), outputs[0], (float*)outputs[1],
+                       (uint32_t*)outputs[2], (uint32_t*)outputs[3], max_out, eb, 2.0f * eb,
+                       params_pending_ ? (const float*)params_.ptr : nullptr, (float*)outputs[4],
+                       (uint32_t*)outputs[5], (uint32_t*)outputs[6], pm, ws_.ptr, ws_.bytes),
+                   "LorenzoQuantStage (strict)");
+        d_pcount_ptr_ = outputs[6];
+        addReadback(d_pcount_ptr_, &h_pcount_, 4);
+    }
     d_count_ptr_ = outputs[3];
     addReadback(d_count_ptr_, &h_count_, 4);
     actual_output_sizes_ = {n * sizeof(TC), max_out * sizeof(TI), max_out * sizeof(uint32_t), sizeof(uint32_t)};
+    if (config_.strict_error_bound) {
+        const size_t pm = maxPatches(n);
+        actual_output_sizes_.insert(actual_output_sizes_.end(), {pm * sizeof(TI), pm * sizeof(uint32_t), sizeof(uint32_t)});
+    }
 }
 
 template <typename TI, typename TC>
@@ -180,6 +211,20 @@ void LorenzoQuantStage<TI, TC>::postStreamSync(hipStream_t) {
     actual_outlier_count_ = cnt;
     actual_output_sizes_[1] = (size_t)cnt * sizeof(TI);
     actual_output_sizes_[2] = (size_t)cnt * sizeof(uint32_t);
+    if (config_.strict_error_bound && d_pcount_ptr_) {
+        const uint32_t pc = h_pcount_;
+        if (pc == 0xFFFFFFFFu) throw std::runtime_error("LorenzoQuantStage: ordered scan failed (workspace state)");
+        if (pc > maxPatches(num_elements_))
+            throw std::runtime_error("LorenzoQuantStage: strict error bound needs " + std::to_string(pc) +
+                                     " patched values but has room for " + std::to_string(maxPatches(num_elements_)) +
+                                     "; raise setPatchCapacity()");
+        if (cnt < h_count_ && h_count_ != 0xFFFFFFFFu)
+            throw std::runtime_error("LorenzoQuantStage: strict error bound cannot hold with " +
+                                     std::to_string(h_count_) + " outliers for " + std::to_string(cnt) +
+                                     " slots; raise setOutlierCapacity()");
+        actual_output_sizes_[4] = (size_t)pc * sizeof(TI);
+        actual_output_sizes_[5] = (size_t)pc * sizeof(uint32_t);
+    }
 }
 
 template <typename TI, typename TC>
@@ -200,6 +245,7 @@ size_t LorenzoQuantStage<TI, TC>::serializeHeader(size_t, uint8_t* buf, size_t m
     c.user_eb = config_.error_bound;
     c.value_base = computed_value_base_;
     c.zigzag_codes = config_.zigzag_codes ? 1 : 0;
+    c.reserved[0] = config_.strict_error_bound ? 1 : 0;
     std::memcpy(buf, &c, sizeof(c));
     return sizeof(c);
 }
@@ -225,6 +271,7 @@ void LorenzoQuantStage<TI, TC>::deserializeHeader(const uint8_t* buf, size_t siz
         computed_value_base_ = 0.0f;
     }
     config_.zigzag_codes = size >= sizeof(LorenzoQuantConfig) ? c.zigzag_codes != 0 : false;
+    config_.strict_error_bound = size >= sizeof(LorenzoQuantConfig) && (c.reserved[0] & 1u) != 0;
     const int nd = c.ndim == 0 ? 1 : c.ndim;
     if (c.dim_x > 0) config_.dims[0] = c.dim_x;
     else if (c.num_elements > 0) config_.dims[0] = c.num_elements / (std::max<size_t>(1, c.dim_y) * std::max<size_t>(1, c.dim_z));

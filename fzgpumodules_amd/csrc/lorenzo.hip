@@ -51,6 +51,31 @@ __device__ __forceinline__ float dec_code(TCode c) {
 
 __device__ __forceinline__ float quant_round(float x, float r) { return __builtin_roundf(x * r); }
 
+// Strict error bound (an extension; no counterpart in the reference).  The
+// inverse reconstructs element i as fl(S_i * ebx2), S_i the exact integer
+// prefix of the tile's deltas, whenever every partial sum of the inverse's
+// addition tree is exact.  Partial sums are sums of deltas over sub-boxes of
+// the tile, i.e. signed sums of at most 8 quantised values, so |s| < 2^21
+// across the tile keeps them below 2^24 (exact in FP32).  An element is
+// patched (its original value stored) when |fl(s*ebx2) - x| > eb, or when
+// its tile breaks the exactness condition (then every element of the tile).
+struct StrictOut {
+    const float* d_params;  // device eb params ([1] abs_eb, [3] ebx2) or null
+    float eb, ebx2;         // used when d_params is null
+    float* val;
+    uint32_t* idx;
+    uint32_t* count;
+    uint64_t cap;
+    uint64_t* status;       // second ordered-scan status array
+};
+constexpr float kExactS = 2097152.0f;  // 2^21
+
+__device__ __forceinline__ bool needs_patch(float x, float s, float eb, float ebx2) {
+    const float xh = s * ebx2;
+    const double e = fabs((double)xh - (double)x);
+    return !(e <= (double)eb);  // NaN compares false: patched
+}
+
 struct Tiles {
     uint32_t gx, gy, gz;  // tile grid
 };
@@ -59,32 +84,34 @@ struct Tiles {
 // Forward kernels
 // ===========================================================================
 
-template <typename TCode, bool ZZ>
+template <typename TCode, bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256)
 lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, uint64_t nv, float r_host, const float* __restrict__ d_r,
                 float rad, TCode* __restrict__ codes, float* __restrict__ oerr,
                 uint32_t* __restrict__ oidx, uint32_t* __restrict__ ocount, uint64_t max_out,
-                ScanCtx* ctx, uint64_t* status, uint32_t T, uint32_t total_waves) {
+                ScanCtx* ctx, uint64_t* status, uint32_t T, uint32_t total_waves, StrictOut so) {
     const TileTicket tk = take_ticket(ctx);
     const float r = d_r ? *d_r : r_host;
     if (tk.tile < T) {
         const int l = lane_id();
         const uint64_t base = (uint64_t)tk.tile * 1024;
         const bool full = base + 1024 <= nv;  // elements in [nv, n) read as zero (padding)
-        float s[4][4];
+        float s[4][4], xv[4][4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
             if (full) {
                 const float4 v = *reinterpret_cast<const float4*>(in + e0);
-                s[q][0] = quant_round(v.x, r);
-                s[q][1] = quant_round(v.y, r);
-                s[q][2] = quant_round(v.z, r);
-                s[q][3] = quant_round(v.w, r);
+                xv[q][0] = v.x;
+                xv[q][1] = v.y;
+                xv[q][2] = v.z;
+                xv[q][3] = v.w;
             } else {
 #pragma unroll
-                for (int k = 0; k < 4; k++) s[q][k] = (e0 + k < nv) ? quant_round(in[e0 + k], r) : 0.0f;
+                for (int k = 0; k < 4; k++) xv[q][k] = (e0 + k < nv) ? in[e0 + k] : 0.0f;
             }
+#pragma unroll
+            for (int k = 0; k < 4; k++) s[q][k] = quant_round(xv[q][k], r);
         }
         // delta_i = s_i - s_{i-1}, chain restarts at the tile start.
         float d[4][4];
@@ -153,6 +180,48 @@ lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, uint64_t nv, float r_h
             }
         }
         if (tk.tile == T - 1 && l == 0) *ocount = lb_ok ? obase + agg : 0xFFFFFFFFu;
+        if constexpr (STRICT) {
+            const float eb = so.d_params ? so.d_params[1] : so.eb;
+            const float e2 = so.d_params ? so.d_params[3] : so.ebx2;
+            bool huge = false;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) huge |= !(fabsf(s[q][k]) < kExactS);
+            const bool tile_huge = __ballot(huge) != 0ull;
+            uint32_t pm = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
+                    if (e < nv && (tile_huge || needs_patch(xv[q][k], s[q][k], eb, e2))) pm |= 1u << (q * 4 + k);
+                }
+            const uint32_t pagg = wave_sum(__popc(pm));
+            const uint32_t pbase = lookback_exclusive(so.status, tk.tile, tk.epoch, pagg);
+            const bool p_ok = pbase != kLookbackFailed;
+            if (pagg && p_ok) {
+                uint32_t run = pbase;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t mq = (pm >> (4 * q)) & 0xFu;
+                    const uint32_t cq = __popc(mq);
+                    const uint32_t incl = wave_incl_scan(cq);
+                    uint32_t pos = run + incl - cq;
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if (mq & (1u << k)) {
+                            if (pos < so.cap) {
+                                so.val[pos] = xv[q][k];
+                                so.idx[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
+                            }
+                            pos++;
+                        }
+                    run += __shfl(incl, 63, 64);
+                }
+            }
+            if (tk.tile == T - 1 && l == 0) *so.count = p_ok ? pbase + pagg : 0xFFFFFFFFu;
+        }
     }
     release_ticket(ctx, tk.epoch, total_waves);
 }
@@ -163,13 +232,13 @@ __device__ __forceinline__ void zero_code_tail(TCode* codes, uint64_t from, uint
     for (uint64_t i = from + lane_id(); i < to; i += 64) codes[i] = (TCode)0;
 }
 
-template <typename TCode, bool ZZ>
+template <typename TCode, bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256)
 lz2d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n, float r_host,
                 const float* __restrict__ d_r, float rad, TCode* __restrict__ codes,
                 float* __restrict__ oerr, uint32_t* __restrict__ oidx, uint32_t* __restrict__ ocount,
                 uint64_t max_out, ScanCtx* ctx, uint64_t* status, uint32_t gx, uint32_t T,
-                uint32_t total_waves) {
+                uint32_t total_waves, StrictOut so) {
     const TileTicket tk = take_ticket(ctx);
     const float r = d_r ? *d_r : r_host;
     if (tk.tile < T) {
@@ -179,11 +248,17 @@ lz2d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t
         const uint32_t X = bx * 32 + x;
         const uint32_t Y0 = by * 32 + 16 * h;
         const bool xin = X < nx;
-        float s[16];
+        float s[16], xv[16];
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const uint32_t Y = Y0 + i;
-            s[i] = (xin && Y < ny) ? quant_round(in[(uint64_t)Y * nx + X], r) : 0.0f;
+            xv[i] = (xin && Y < ny) ? in[(uint64_t)Y * nx + X] : 0.0f;
+            s[i] = quant_round(xv[i], r);
+        }
+        bool huge = false;
+        if constexpr (STRICT) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) huge |= !(fabsf(s[i]) < kExactS);
         }
         // t = s(y) - s(y-1) inside the tile (row 0 of the tile subtracts 0).
         const float north = __shfl(s[15], x, 64);  // row 15 of the tile, for h = 1
@@ -234,17 +309,53 @@ lz2d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t
             if (l == 0) *ocount = lb_ok ? obase + agg : 0xFFFFFFFFu;
             zero_code_tail(codes, (uint64_t)nx * ny, n);
         }
+        if constexpr (STRICT) {
+            const float eb = so.d_params ? so.d_params[1] : so.eb;
+            const float e2 = so.d_params ? so.d_params[3] : so.ebx2;
+            const bool tile_huge = __ballot(huge) != 0ull;
+            uint32_t pm = 0;
+            uint64_t pbal[16];
+            uint32_t pagg = 0, plo = 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const bool valid = xin && (Y0 + i) < ny;
+                const bool p = valid && (tile_huge || needs_patch(xv[i], s[i], eb, e2));
+                if (p) pm |= 1u << i;
+                pbal[i] = __ballot(p);
+                pagg += __popcll(pbal[i]);
+                plo += __popc((uint32_t)pbal[i]);
+            }
+            const uint32_t pbase = lookback_exclusive(so.status, tk.tile, tk.epoch, pagg);
+            const bool p_ok = pbase != kLookbackFailed;
+            if (pagg && p_ok) {
+                uint32_t run = h ? plo : 0u;
+                const uint32_t below = (1u << x) - 1u;
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const uint32_t half = h ? (uint32_t)(pbal[i] >> 32) : (uint32_t)pbal[i];
+                    if (pm & (1u << i)) {
+                        const uint32_t pos = pbase + run + __popc(half & below);
+                        if (pos < so.cap) {
+                            so.val[pos] = xv[i];
+                            so.idx[pos] = (Y0 + i) * nx + X;
+                        }
+                    }
+                    run += __popc(half);
+                }
+            }
+            if (tk.tile == T - 1 && l == 0) *so.count = p_ok ? pbase + pagg : 0xFFFFFFFFu;
+        }
     }
     release_ticket(ctx, tk.epoch, total_waves);
 }
 
-template <typename TCode, bool ZZ>
+template <typename TCode, bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256)
 lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, uint64_t n,
                 float r_host, const float* __restrict__ d_r, float rad, TCode* __restrict__ codes,
                 float* __restrict__ oerr, uint32_t* __restrict__ oidx, uint32_t* __restrict__ ocount,
                 uint64_t max_out, ScanCtx* ctx, uint64_t* status, Tiles tg, uint32_t T,
-                uint32_t total_waves) {
+                uint32_t total_waves, StrictOut so) {
     const TileTicket tk = take_ticket(ctx);
     const float r = d_r ? *d_r : r_host;
     if (tk.tile < T) {
@@ -254,10 +365,12 @@ lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t
         const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
         const bool xz = X < nx && Z < nz;
         const uint64_t plane = (uint64_t)nx * ny;
-        float s[8];
+        float s[8], xv[8];
 #pragma unroll
-        for (int y = 0; y < 8; y++)
-            s[y] = (xz && Y0 + y < ny) ? quant_round(in[Z * plane + (uint64_t)(Y0 + y) * nx + X], r) : 0.0f;
+        for (int y = 0; y < 8; y++) {
+            xv[y] = (xz && Y0 + y < ny) ? in[Z * plane + (uint64_t)(Y0 + y) * nx + X] : 0.0f;
+            s[y] = quant_round(xv[y], r);
+        }
         // a = s - s(z-1); b = a - a(x-1); d = b - b(y-1)   (lorenzo_quant_nd.cu:234-246)
         float b[8];
 #pragma unroll
@@ -308,6 +421,46 @@ lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t
         if (tk.tile == T - 1) {
             if (l == 0) *ocount = lb_ok ? obase + agg : 0xFFFFFFFFu;
             zero_code_tail(codes, (uint64_t)nx * ny * nz, n);
+        }
+        if constexpr (STRICT) {
+            const float eb = so.d_params ? so.d_params[1] : so.eb;
+            const float e2 = so.d_params ? so.d_params[3] : so.ebx2;
+            bool huge = false;
+#pragma unroll
+            for (int y = 0; y < 8; y++) huge |= !(fabsf(s[y]) < kExactS);
+            const bool tile_huge = __ballot(huge) != 0ull;
+            uint32_t pm = 0, pagg = 0;
+            uint64_t pbal[8];
+#pragma unroll
+            for (int y = 0; y < 8; y++) {
+                const bool valid = xz && (Y0 + y) < ny;
+                const bool p = valid && (tile_huge || needs_patch(xv[y], s[y], eb, e2));
+                if (p) pm |= 1u << y;
+                pbal[y] = __ballot(p);
+                pagg += __popcll(pbal[y]);
+            }
+            const uint32_t pbase = lookback_exclusive(so.status, tk.tile, tk.epoch, pagg);
+            const bool p_ok = pbase != kLookbackFailed;
+            if (pagg && p_ok) {
+                const uint64_t zlow = (z == 0) ? 0ull : ((1ull << (8 * z)) - 1ull);
+                uint32_t run = 0;
+#pragma unroll
+                for (int y = 0; y < 8; y++) run += __popcll(pbal[y] & zlow);
+                const uint32_t below = (1u << x) - 1u;
+#pragma unroll
+                for (int y = 0; y < 8; y++) {
+                    const uint32_t row = (uint32_t)(pbal[y] >> (8 * z)) & 0xFFu;
+                    if (pm & (1u << y)) {
+                        const uint32_t pos = pbase + run + __popc(row & below);
+                        if (pos < so.cap) {
+                            so.val[pos] = xv[y];
+                            so.idx[pos] = (uint32_t)(Z * plane + (uint64_t)(Y0 + y) * nx + X);
+                        }
+                    }
+                    run += __popc(row);
+                }
+            }
+            if (tk.tile == T - 1 && l == 0) *so.count = p_ok ? pbase + pagg : 0xFFFFFFFFu;
         }
     }
     release_ticket(ctx, tk.epoch, total_waves);
@@ -685,6 +838,17 @@ lz3d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, uint3
         if (xz && Y0 + y < ny) out[Z * plane + (uint64_t)(Y0 + y) * nx + X] = d[y];
 }
 
+// Strict-mode patches: out[idx[i]] = val[i] for i < min(*count, cap).
+__global__ void __launch_bounds__(256)
+apply_patches_kernel(float* __restrict__ out, const float* __restrict__ val, const uint32_t* __restrict__ idx,
+                     const uint32_t* __restrict__ count, uint64_t cap, uint64_t n_out) {
+    const uint64_t k = min((uint64_t)*count, cap);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = idx[i];
+        if (j < n_out) out[j] = val[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host helpers
 // ---------------------------------------------------------------------------
@@ -710,14 +874,18 @@ int check_dims(size_t n, size_t nx, size_t ny, size_t nz, const char* who) {
     return FZH_OK;
 }
 
-template <typename TCode>
+// Forward workspace: [ScanCtx + outlier status words][patch status words].
+size_t fwd_ws_bytes(size_t T) { return align_up(scan_state_bytes(T), 256) + align_up(8 * (T + 1), 256); }
+
+template <typename TCode, bool STRICT>
 int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, size_t nz, float r,
                const float* d_r, uint32_t radius, int zz, void* codes, float* oerr, uint32_t* oidx,
-               uint32_t* ocount, size_t max_out, void* ws, size_t ws_bytes) {
+               uint32_t* ocount, size_t max_out, void* ws, size_t ws_bytes, StrictOut so) {
     const size_t T = tile_count(n, nx, ny, nz);
-    if (ws_bytes < scan_state_bytes(T)) return fail(FZH_E_CAPACITY, "lorenzo fwd: workspace too small");
+    if (ws_bytes < fwd_ws_bytes(T)) return fail(FZH_E_CAPACITY, "lorenzo fwd: workspace too small");
     ScanCtx* ctx = (ScanCtx*)ws;
     uint64_t* status = (uint64_t*)((uint8_t*)ws + sizeof(ScanCtx));
+    so.status = (uint64_t*)((uint8_t*)ws + align_up(scan_state_bytes(T), 256));
     const float rad = (float)(TCode)radius;
     const unsigned blocks = (unsigned)cdiv(T, 4);
     const uint32_t waves = blocks * 4;
@@ -726,17 +894,17 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
     const size_t nv = (nd == 1 && nx > 0 && nx < n) ? nx : n;
 #define FZH_LZ_FWD(ZZV)                                                                               \
     if (nd == 1)                                                                                      \
-        lz1d_fwd_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(in, n, nv, r, d_r, rad, c, oerr, oidx, ocount, \
-                                                           max_out, ctx, status, (uint32_t)T, waves); \
+        lz1d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(in, n, nv, r, d_r, rad, c, oerr, oidx, ocount, \
+                                                           max_out, ctx, status, (uint32_t)T, waves, so); \
     else if (nd == 2)                                                                                 \
-        lz2d_fwd_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(                                           \
+        lz2d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(                                   \
             in, (uint32_t)nx, (uint32_t)ny, n, r, d_r, rad, c, oerr, oidx, ocount, max_out, ctx,      \
-            status, (uint32_t)cdiv(nx, 32), (uint32_t)T, waves);                                      \
+            status, (uint32_t)cdiv(nx, 32), (uint32_t)T, waves, so);                                  \
     else                                                                                              \
-        lz3d_fwd_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(                                           \
+        lz3d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(                                   \
             in, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, n, r, d_r, rad, c, oerr, oidx, ocount,      \
             max_out, ctx, status, Tiles{(uint32_t)cdiv(nx, 8), (uint32_t)cdiv(ny, 8), 0}, (uint32_t)T, \
-            waves);
+            waves, so);
     if (zz) {
         FZH_LZ_FWD(true)
     } else {
@@ -834,7 +1002,7 @@ using namespace fzh;
 extern "C" {
 
 size_t fzh_lorenzo_quant_fwd_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz) {
-    return align_up(scan_state_bytes(tile_count(n, nx, ny, nz)), 256);
+    return fwd_ws_bytes(tile_count(n, nx, ny, nz));
 }
 
 int fzh_lorenzo_quant_fwd_f32(void* stream, const float* d_in, size_t n, size_t nx, size_t ny, size_t nz,
@@ -848,16 +1016,68 @@ int fzh_lorenzo_quant_fwd_f32(void* stream, const float* d_in, size_t n, size_t 
         return fail(FZH_E_INVALID, "lorenzo fwd: null outlier buffers");
     if (ndim_of(ny, nz) == 1 && reinterpret_cast<uintptr_t>(d_in) % 16 != 0)
         return fail(FZH_E_INVALID, "lorenzo fwd: input must be 16-byte aligned");
+    const StrictOut none{};
     switch (code_bytes) {
     case 1:
-        return launch_fwd<uint8_t>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag, d_codes,
-                                   d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers, d_ws, ws_bytes);
+        return launch_fwd<uint8_t, false>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag, d_codes,
+                                          d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers, d_ws,
+                                          ws_bytes, none);
     case 2:
-        return launch_fwd<uint16_t>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag, d_codes,
-                                    d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers, d_ws, ws_bytes);
+        return launch_fwd<uint16_t, false>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag,
+                                           d_codes, d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers,
+                                           d_ws, ws_bytes, none);
     default:
         return fail(FZH_E_INVALID, "lorenzo fwd: code_bytes must be 1 or 2");
     }
+}
+
+int fzh_lorenzo_quant_fwd_strict_f32(void* stream, const float* d_in, size_t n, size_t nx, size_t ny, size_t nz,
+                                     float ebx2_r, const float* d_ebx2_r, uint32_t radius, int zigzag,
+                                     int code_bytes, void* d_codes, float* d_outlier_err, uint32_t* d_outlier_idx,
+                                     uint32_t* d_outlier_count, size_t max_outliers, float eb, float ebx2,
+                                     const float* d_eb_params, float* d_patch_val, uint32_t* d_patch_idx,
+                                     uint32_t* d_patch_count, size_t max_patches, void* d_ws, size_t ws_bytes) {
+    if (int rc = check_dims(n, nx, ny, nz, "lorenzo fwd (strict)")) return rc;
+    if (n == 0) return FZH_OK;
+    if (!d_in || !d_codes || !d_outlier_count || !d_patch_count) return fail(FZH_E_INVALID, "lorenzo fwd (strict): null pointer");
+    if (max_outliers > 0 && (!d_outlier_err || !d_outlier_idx))
+        return fail(FZH_E_INVALID, "lorenzo fwd (strict): null outlier buffers");
+    if (max_patches > 0 && (!d_patch_val || !d_patch_idx))
+        return fail(FZH_E_INVALID, "lorenzo fwd (strict): null patch buffers");
+    if (ndim_of(ny, nz) == 1 && reinterpret_cast<uintptr_t>(d_in) % 16 != 0)
+        return fail(FZH_E_INVALID, "lorenzo fwd (strict): input must be 16-byte aligned");
+    StrictOut so{};
+    so.d_params = d_eb_params;
+    so.eb = eb;
+    so.ebx2 = ebx2;
+    so.val = d_patch_val;
+    so.idx = d_patch_idx;
+    so.count = d_patch_count;
+    so.cap = max_patches;
+    switch (code_bytes) {
+    case 1:
+        return launch_fwd<uint8_t, true>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag, d_codes,
+                                         d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers, d_ws,
+                                         ws_bytes, so);
+    case 2:
+        return launch_fwd<uint16_t, true>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag,
+                                          d_codes, d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers,
+                                          d_ws, ws_bytes, so);
+    default:
+        return fail(FZH_E_INVALID, "lorenzo fwd (strict): code_bytes must be 1 or 2");
+    }
+}
+
+int fzh_lorenzo_apply_patches(void* stream, float* d_out, const float* d_patch_val, const uint32_t* d_patch_idx,
+                              const uint32_t* d_patch_count, size_t max_patches, size_t n_out) {
+    if (max_patches == 0) return FZH_OK;
+    if (!d_out || !d_patch_val || !d_patch_idx || !d_patch_count)
+        return fail(FZH_E_INVALID, "lorenzo patches: null pointer");
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(max_patches, 256), 1024));
+    apply_patches_kernel<<<blocks, 256, 0, S(stream)>>>(d_out, d_patch_val, d_patch_idx, d_patch_count, max_patches,
+                                                         n_out);
+    FZH_CHECK_LAUNCH("apply_patches_kernel");
+    return FZH_OK;
 }
 
 size_t fzh_lorenzo_quant_inv_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz, size_t max_outliers) {

@@ -63,6 +63,10 @@ class Pipeline:
         return self._id("fzp_add_lorenzo", float(error_bound), _mode(mode), int(quant_radius),
                         float(outlier_capacity), int(bool(zigzag)), int(code_bytes), float(value_base))
 
+    def set_strict(self, stage_id: int, on: bool = True, patch_capacity: float = 0.0) -> None:
+        """Strict error bound on a LorenzoQuant stage (extension)."""
+        call("fzp_set_lorenzo_strict", self.h, stage_id, int(bool(on)), float(patch_capacity))
+
     def add_quantizer(self, error_bound, mode="ABS", quant_radius=32768, outlier_capacity=0.05, zigzag=False,
                       code_bytes=2, value_base=0.0) -> int:
         return self._id("fzp_add_quantizer", float(error_bound), _mode(mode), int(quant_radius),

@@ -94,6 +94,8 @@ class LorenzoQuantStage : public Stage {
         ErrorBoundMode eb_mode = ErrorBoundMode::ABS;
         float precomputed_value_base = 0.0f;
         bool zigzag_codes = false;
+        bool strict_error_bound = false;  // extension: patch list, |x - x_hat| <= eb everywhere
+        float patch_capacity = 0.01f;     // patch slots = max(256, ceil(n * patch_capacity))
         Config() = default;
         Config(TInput eb, TCode radius = (TCode)32768, float outlier_cap = 0.2f, std::array<size_t, 3> d = {0, 1, 1})
             : error_bound((float)eb), quant_radius(radius), outlier_capacity(outlier_cap), dims(d) {}
@@ -106,9 +108,12 @@ class LorenzoQuantStage : public Stage {
     void postStreamSync(hipStream_t stream) override;
 
     std::string getName() const override { return "LorenzoQuant"; }
-    size_t getNumInputs() const override { return is_inverse_ ? 4 : 1; }
-    size_t getNumOutputs() const override { return is_inverse_ ? 1 : 4; }
+    size_t getNumInputs() const override { return is_inverse_ ? numPorts() : 1; }
+    size_t getNumOutputs() const override { return is_inverse_ ? 1 : numPorts(); }
     std::vector<std::string> getOutputNames() const override {
+        if (config_.strict_error_bound)
+            return {"codes", "outlier_errors", "outlier_indices", "outlier_count",
+                    "patch_values", "patch_indices", "patch_count"};
         return {"codes", "outlier_errors", "outlier_indices", "outlier_count"};
     }
     std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& input_sizes) const override;
@@ -127,6 +132,13 @@ class LorenzoQuantStage : public Stage {
     void setErrorBoundMode(ErrorBoundMode m) { config_.eb_mode = m; }
     void setValueBase(float vb) { config_.precomputed_value_base = vb; }
     void setZigzagCodes(bool on) { config_.zigzag_codes = on; }
+    // Extension: every decompressed value within eb (the reference's FP32
+    // arithmetic overshoots by a few ULPs on a few elements); codes and
+    // outliers are unchanged, three more outputs carry the patched values.
+    void setStrictErrorBound(bool on) { config_.strict_error_bound = on; }
+    void setPatchCapacity(float c) { config_.patch_capacity = c; }
+    bool getStrictErrorBound() const { return config_.strict_error_bound; }
+    uint32_t getPatchCount() const { return h_pcount_; }
 
     TInput getErrorBound() const { return (TInput)config_.error_bound; }
     TCode getQuantRadius() const { return (TCode)config_.quant_radius; }
@@ -146,8 +158,10 @@ class LorenzoQuantStage : public Stage {
         return !is_inverse_ && valid_bytes % sizeof(TInput) == 0;
     }
     DeviceSize getDeviceOutputSize(int index) const override {
-        if (is_inverse_ || (index != 1 && index != 2) || !d_count_ptr_) return {};
-        return {d_count_ptr_, 2, 4};
+        if (is_inverse_) return {};
+        if ((index == 1 || index == 2) && d_count_ptr_) return {d_count_ptr_, 2, 4};
+        if ((index == 4 || index == 5) && d_pcount_ptr_) return {d_pcount_ptr_, 2, 4};
+        return {};
     }
     void setInverse(bool inv) override { is_inverse_ = inv; }
     bool isInverse() const override { return is_inverse_; }
@@ -156,8 +170,8 @@ class LorenzoQuantStage : public Stage {
     uint8_t getOutputDataType(size_t i) const override {
         switch (i) {
         case 0: return (uint8_t)dataTypeOf<TCode>();
-        case 1: return (uint8_t)DataType::FLOAT32;
-        case 2: case 3: return (uint8_t)DataType::UINT32;
+        case 1: case 4: return (uint8_t)DataType::FLOAT32;
+        case 2: case 3: case 5: case 6: return (uint8_t)DataType::UINT32;
         default: return (uint8_t)DataType::UINT8;
         }
     }
@@ -168,6 +182,10 @@ class LorenzoQuantStage : public Stage {
 
   private:
     size_t maxOutliers(size_t n) const { return (size_t)std::ceil((double)n * config_.outlier_capacity); }
+    size_t maxPatches(size_t n) const {
+        return std::max<size_t>(256, (size_t)std::ceil((double)n * config_.patch_capacity));
+    }
+    size_t numPorts() const { return config_.strict_error_bound ? 7 : 4; }
     void resolveDims(size_t n, size_t& nx, size_t& ny, size_t& nz) const;
 
     Config config_;
@@ -179,8 +197,9 @@ class LorenzoQuantStage : public Stage {
     float computed_value_base_ = 0.0f;
     size_t valid_elems_ = 0;             // > 0: input elements past this read as zero (one call)
     const void* d_count_ptr_ = nullptr;  // forward: outlier count output
+    const void* d_pcount_ptr_ = nullptr; // forward (strict): patch count output
     bool params_pending_ = false;        // REL/NOA resolved on the device
-    uint32_t h_count_ = 0;
+    uint32_t h_count_ = 0, h_pcount_ = 0;
     float h_params_[4] = {0, 0, 0, 0};
     DeviceScratch ws_, inv_ws_, params_;
 };

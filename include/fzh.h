@@ -83,6 +83,31 @@ int fzh_lorenzo_quant_fwd_f32(void* stream, const float* d_in, size_t n,
                               uint32_t* d_outlier_idx, uint32_t* d_outlier_count,
                               size_t max_outliers, void* d_ws, size_t ws_bytes);
 
+/* Strict error bound (an extension; no counterpart in the reference): the
+ * same codes and outliers as fzh_lorenzo_quant_fwd_f32, plus a patch list of
+ * the elements the inverse would reconstruct with |x - x_hat| > eb (the
+ * reference's FP32 arithmetic overshoots eb by a few ULPs on a few elements):
+ * d_patch_val[i] = x[d_patch_idx[i]], ordered like the outliers, count in
+ * *d_patch_count (may exceed max_patches; only max_patches stored).  eb and
+ * ebx2 = 2*eb are read from d_eb_params[1] / [3] (fzh_value_base_f32's
+ * layout) when it is non-NULL.  Guaranteed for finite inputs; a tile with
+ * |x/(2 eb)| >= 2^21 anywhere is patched whole.                             */
+int fzh_lorenzo_quant_fwd_strict_f32(void* stream, const float* d_in, size_t n,
+                                     size_t nx, size_t ny, size_t nz,
+                                     float ebx2_r, const float* d_ebx2_r,
+                                     uint32_t radius, int zigzag, int code_bytes,
+                                     void* d_codes, float* d_outlier_err,
+                                     uint32_t* d_outlier_idx, uint32_t* d_outlier_count,
+                                     size_t max_outliers, float eb, float ebx2,
+                                     const float* d_eb_params, float* d_patch_val,
+                                     uint32_t* d_patch_idx, uint32_t* d_patch_count,
+                                     size_t max_patches, void* d_ws, size_t ws_bytes);
+/* After the inverse: d_out[idx[i]] = val[i] for i < min(*count, max_patches)
+ * and idx[i] < n_out.                                                      */
+int fzh_lorenzo_apply_patches(void* stream, float* d_out, const float* d_patch_val,
+                              const uint32_t* d_patch_idx, const uint32_t* d_patch_count,
+                              size_t max_patches, size_t n_out);
+
 /* Inverse: uses min(*d_outlier_count, max_outliers) outliers in any order.
  * ebx2 = 2*abs_eb in float.                                                 */
 size_t fzh_lorenzo_quant_inv_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz,

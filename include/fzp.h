@@ -48,6 +48,10 @@ int fzp_destroy(fzp_pipeline* p);
 int fzp_add_lorenzo(fzp_pipeline* p, float error_bound, int eb_mode, uint32_t quant_radius,
                     float outlier_capacity, int zigzag, int code_bytes, float value_base,
                     int* stage_id);
+/* Extension (no counterpart in the reference): strict error bound on a
+ * LorenzoQuant stage -- codes and outliers unchanged, plus a patch list so
+ * every decompressed value is within eb (LorenzoQuantStage::setStrictErrorBound). */
+int fzp_set_lorenzo_strict(fzp_pipeline* p, int stage_id, int on, float patch_capacity);
 int fzp_add_bitshuffle(fzp_pipeline* p, uint32_t block_size, int elem_width, int* stage_id);
 int fzp_add_rze(fzp_pipeline* p, uint32_t chunk_size, int levels, int* stage_id);
 int fzp_add_rle(fzp_pipeline* p, int elem_bytes, int* stage_id);

@@ -111,3 +111,31 @@ def reconstruct(parts_info, dims, zigzag=True):
     if ny == 1 and nz == 1:
         nx = n
     return orc.lorenzo_inv(p["codes"], (nx, ny, nz), p["err"], p["idx"], p["cnt"], p["ebx2"], zigzag)
+
+
+def tile_key(idx, dims):
+    """Sort key of the canonical outlier/patch order: tile-major, then the
+    element's position inside its tile (1D 1024, 2D 32x32, 3D 8x8x8)."""
+    idx = np.asarray(idx, np.int64)
+    nx, ny, nz = dims
+    if ny <= 1 and nz <= 1:
+        return idx
+    if nz <= 1:
+        y, x = idx // nx, idx % nx
+        gx = (nx + 31) // 32
+        return ((y // 32) * gx + x // 32) * 1024 + (y % 32) * 32 + x % 32
+    x = idx % nx
+    y = (idx // nx) % ny
+    z = idx // (nx * ny)
+    gx, gy = (nx + 7) // 8, (ny + 7) // 8
+    return (((z // 8) * gy + y // 8) * gx + x // 8) * 512 + (z % 8) * 64 + (y % 8) * 8 + x % 8
+
+
+def strict_patches(x, recon, abs_eb, dims):
+    """Elements the oracle's inverse reconstructs outside eb, in canonical
+    order, with their original values (the strict-mode patch list)."""
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    err = np.abs(recon[: x.size].astype(np.float64) - x.astype(np.float64))
+    bad = np.nonzero(~(err <= float(abs_eb)))[0]
+    bad = bad[np.argsort(tile_key(bad, dims), kind="stable")]
+    return bad.astype(np.uint32), x[bad]

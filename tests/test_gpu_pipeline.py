@@ -62,7 +62,7 @@ def test_cfg2_chain_bit_exact(shape):
         assert y.size == x.size
         ref = chains.reconstruct(info, (nx, ny, 1))
         np.testing.assert_array_equal(y, ref[: x.size])
-        check_bound(x, y, float(info["abs_eb"]), slack=1.001)
+        check_bound(x.ravel(), y, float(info["abs_eb"]), slack=1.001)
 
 
 def test_cfg1_chain_rle_1d():
@@ -236,3 +236,60 @@ def test_empty_and_errors():
         p.finalize()
         with pytest.raises(FzhError, match="compress"):
             p.decompress(0x1000, 64)
+
+
+@pytest.mark.parametrize("case", ["1d_rle", "2d_rze", "3d_rze"])
+def test_strict_error_bound(case):
+    """Strict mode: codes/outliers/coder streams identical to the plain chain,
+    plus the patch list = exactly the elements the reference arithmetic
+    reconstructs outside eb; every decompressed value is within eb."""
+    if case == "1d_rle":
+        x = fields.sinusoid_1d(1 << 20, seed=_seed("strict1"))
+        dims = (x.size, 1, 1)
+        want, info = chains.lorenzo_rle(x)
+    elif case == "2d_rze":
+        ny, nx = 300, 700
+        x = cloud(ny, nx, _seed("strict2")).ravel()
+        dims = (nx, ny, 1)
+        want, info = chains.lorenzo_bitshuffle_rze(x, dims)
+    else:
+        shape = (24, 40, 56)
+        x = fields.power_law_field(shape, -11.0 / 3.0, _seed("strict3")).astype(np.float32).ravel()
+        dims = (56, 40, 24)
+        want, info = chains.lorenzo_bitshuffle_rze(x, dims, user_eb=1e-5, cap=0.2)
+    recon = chains.reconstruct(info, dims if case != "1d_rle" else (0, 1, 1), zigzag=case != "1d_rle")
+    pidx, pval = chains.strict_patches(x, recon, info["abs_eb"], dims)
+    with Pipeline(x.nbytes) as p:
+        if case == "1d_rle":
+            lz = p.add_lorenzo(1e-4, mode="ABS", outlier_capacity=0.2)
+            co = p.add_rle(2)
+        else:
+            lz = p.add_lorenzo(1e-3 if case == "2d_rze" else 1e-5, mode="REL",
+                               outlier_capacity=0.1 if case == "2d_rze" else 0.2, zigzag=True)
+            bs = p.add_bitshuffle(16384, 2)
+            co = p.add_rze()
+        p.set_strict(lz, True)
+        if case != "1d_rle":
+            p.connect(bs, lz, "codes")
+            p.connect(co, bs)
+        else:
+            p.connect(co, lz, "codes")
+        if case != "1d_rle":
+            p.set_dims(*dims)
+        p.finalize()
+        blob = p.compress_array(x)
+        got = chains.split(blob)
+        ref = chains.split(want)
+        assert len(got) == 7
+        assert pidx.size > 0  # the data exercise the patch path
+        for i, j in ((0, 0), (1, 1), (2, 2), (6, 3)):
+            np.testing.assert_array_equal(got[i], ref[j], err_msg=f"part {i}")
+        np.testing.assert_array_equal(got[4].view(np.uint32), pidx)
+        np.testing.assert_array_equal(got[3].view(np.float32), pval)
+        assert got[5].view(np.uint32)[0] == pidx.size
+        y = p.decompress_array(blob)[: x.size]
+        err = np.abs(y.astype(np.float64) - x.astype(np.float64))
+        assert err.max() <= float(info["abs_eb"])  # exhaustive, no slack
+        expect = recon[: x.size].copy()
+        expect[pidx] = pval
+        np.testing.assert_array_equal(y, expect)
