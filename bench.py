@@ -119,8 +119,25 @@ def main():
     dec_gbs = nbytes / dec_s / 1e9
     y = np.empty(x.size, np.float32)
     call("fzh_memcpy_dtoh", y.ctypes.data, y_ptr, y.nbytes)
-    max_err = float(np.max(np.abs(y.astype(np.float64) - x.astype(np.float64))))
+    max_err = float(np.max(np.abs(y.astype(np.float64) - x.ravel().astype(np.float64))))
     log(f"[bench] decompress {dec_s * 1e3:.3f} ms/step  {dec_gbs:.1f} GB/s  max|err| {max_err:.3e} (eb {abs_eb:.3e})")
+
+    # ---- per-stage device time of one eager compress (pipeline profiling) ---
+    p.enable_graph_mode(False)
+    p.enable_profiling(True)
+    for _ in range(3):
+        p.compress(d_in, nbytes)
+    perf = p.last_perf()
+    stages = []
+    for i in range(8):
+        name = C.create_string_buffer(64)
+        ms = C.c_float()
+        if lib().fzp_last_stage_perf(p.h, i, name, 64, C.byref(ms)) != 0:
+            break
+        stages.append((name.value.decode(), ms.value))
+    log(f"[bench] eager compress: device {perf['dag_ms'] * 1e3:.1f} us, host {perf['host_ms'] * 1e3:.1f} us; "
+        + ", ".join(f"{n} {m * 1e3:.1f} us" for n, m in stages))
+    p.enable_profiling(False)
 
     # ---- per-kernel device time, HIP events on the launch stream ------------
     kern = kernel_times(args.kernel_reps, x, d_in, abs_eb)
@@ -161,6 +178,9 @@ def main():
             "input_bytes": nbytes,
             "padded_bytes": int(math.ceil(nbytes / 16384) * 16384),
         },
+        "eager_compress_device_us": round(perf["dag_ms"] * 1e3, 2),
+        "eager_compress_host_us": round(perf["host_ms"] * 1e3, 2),
+        "stage_device_us": {n: round(m * 1e3, 2) for n, m in stages},
         "decompress_gbs": round(dec_gbs, 2),
         "decompress_ms_per_step": round(dec_s * 1e3, 4),
         "ratio": round(ratio, 3),
