@@ -29,6 +29,7 @@ SIGNATURES = {
     "fzh_workspace_init": (_I, [_P, _P, _Z]),
     "fzh_lorenzo_quant_fwd_ws_bytes": (_Z, [_Z, _Z, _Z, _Z]),
     "fzh_lorenzo_quant_fwd_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _F, _P, _U32, _I, _I, _P, _P, _P, _P, _Z, _P, _Z]),
+    "fzh_lorenzo_quant_fwd_strict_ws_bytes": (_Z, [_Z, _Z, _Z, _Z]),
     "fzh_lorenzo_quant_fwd_strict_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _F, _P, _U32, _I, _I, _P, _P, _P, _P, _Z,
                                              _F, _F, _P, _P, _P, _P, _Z, _P, _Z]),
     "fzh_lorenzo_apply_patches": (_I, [_P, _P, _P, _P, _P, _Z, _Z]),

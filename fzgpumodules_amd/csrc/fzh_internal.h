@@ -206,4 +206,39 @@ __device__ __forceinline__ uint32_t lookback_exclusive(uint64_t* status, uint32_
     return prefix;
 }
 
+// Exclusive prefix for `unit` from the aggregates of ALL earlier units
+// (published with flag 1, read 64 at a time).  Unlike lookback_exclusive
+// there is no chain of inclusive prefixes: every unit publishes at once and
+// waits only for its predecessors' aggregates, so the latency is
+// ceil(unit/64) rounds of device-scope loads.  Meant for a few hundred
+// coarse units (O(units^2) loads overall).  Whole wave must call; bounded
+// spin, returns kLookbackFailed like lookback_exclusive.
+__device__ __forceinline__ uint32_t aggregate_prefix(uint64_t* status, uint32_t unit, uint32_t epoch,
+                                                     uint32_t aggregate) {
+    const int l = lane_id();
+    const uint32_t ep = epoch & 0x3FFFFFFFu;
+    if (l == 0) st_status(&status[unit], pack_status(epoch, 1u, aggregate));
+    uint32_t sum = 0;
+    for (uint32_t b = 0; b < unit; b += 64) {
+        const uint32_t j = b + (uint32_t)l;
+        uint32_t v = 0;
+        bool timed_out = false;
+        if (j < unit) {
+            uint32_t spins = 0;
+            while (true) {
+                const uint64_t s = ld_status(&status[j]);
+                if ((uint32_t)(s >> 34) == ep && ((s >> 32) & 3u) != 0u) {
+                    v = (uint32_t)s;
+                    break;
+                }
+                if (++spins > kSpinLimit) { timed_out = true; break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        if (__ballot(timed_out) != 0ull) return kLookbackFailed;
+        sum += wave_sum(v);
+    }
+    return sum;
+}
+
 }  // namespace fzh

@@ -161,7 +161,9 @@ void LorenzoQuantStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, co
         params_pending_ = true;
         addReadback(d_params, h_params_, sizeof(h_params_));
     }
-    ws_.ensure(fzh_lorenzo_quant_fwd_ws_bytes(n, nx, ny, nz), pool, stream);
+    ws_.ensure(config_.strict_error_bound ? fzh_lorenzo_quant_fwd_strict_ws_bytes(n, nx, ny, nz)
+                                          : fzh_lorenzo_quant_fwd_ws_bytes(n, nx, ny, nz),
+               pool, stream);
     if (!config_.strict_error_bound) {
         throwIfFzh(fzh_lorenzo_quant_fwd_f32(stream, (const float*)inputs[0], n, nx, ny, nz, r, d_r,
                                              (uint32_t)config_.quant_radius, config_.zigzag_codes ? 1 : 0,

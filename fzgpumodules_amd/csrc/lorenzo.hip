@@ -59,15 +59,6 @@ __device__ __forceinline__ float quant_round(float x, float r) { return __builti
 // across the tile keeps them below 2^24 (exact in FP32).  An element is
 // patched (its original value stored) when |fl(s*ebx2) - x| > eb, or when
 // its tile breaks the exactness condition (then every element of the tile).
-struct StrictOut {
-    const float* d_params;  // device eb params ([1] abs_eb, [3] ebx2) or null
-    float eb, ebx2;         // used when d_params is null
-    float* val;
-    uint32_t* idx;
-    uint32_t* count;
-    uint64_t cap;
-    uint64_t* status;       // second ordered-scan status array
-};
 constexpr float kExactS = 2097152.0f;  // 2^21
 
 __device__ __forceinline__ bool needs_patch(float x, float s, float eb, float ebx2) {
@@ -76,154 +67,142 @@ __device__ __forceinline__ bool needs_patch(float x, float s, float eb, float eb
     return !(e <= (double)eb);  // NaN compares false: patched
 }
 
+// Per-tile lists written by the forward kernels: tile t's outliers occupy
+// err/idx[t*E, t*E + cnt[t]) in the tile's order (E = elements per tile), and
+// likewise its strict-mode patches.  The compaction kernel then places every
+// tile's list at its exclusive prefix, so no wave of the forward kernels
+// waits on another (a single-pass ordered scan over thousands of tiles is
+// limited by device-scope latency on this part).
+struct TileLists {
+    float* err;
+    uint32_t* idx;
+    uint32_t* cnt;
+    float* pval;
+    uint32_t* pidx;
+    uint32_t* pcnt;
+    const float* d_params;  // device eb params ([1] abs_eb, [3] ebx2) or null
+    float eb, ebx2;         // used when d_params is null
+};
+
 struct Tiles {
     uint32_t gx, gy, gz;  // tile grid
 };
 
 // ===========================================================================
-// Forward kernels
+// Forward kernels: one wave64 per tile, four tiles per 256-thread workgroup
 // ===========================================================================
 
 template <typename TCode, bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256)
 lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, uint64_t nv, float r_host, const float* __restrict__ d_r,
-                float rad, TCode* __restrict__ codes, float* __restrict__ oerr,
-                uint32_t* __restrict__ oidx, uint32_t* __restrict__ ocount, uint64_t max_out,
-                ScanCtx* ctx, uint64_t* status, uint32_t T, uint32_t total_waves, StrictOut so) {
-    const TileTicket tk = take_ticket(ctx);
+                float rad, TCode* __restrict__ codes, TileLists tl, uint32_t T) {
+    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
-    if (tk.tile < T) {
-        const int l = lane_id();
-        const uint64_t base = (uint64_t)tk.tile * 1024;
-        const bool full = base + 1024 <= nv;  // elements in [nv, n) read as zero (padding)
-        float s[4][4], xv[4][4];
+    const int l = lane_id();
+    const uint64_t base = (uint64_t)tile * 1024;
+    const bool full = base + 1024 <= nv;  // elements in [nv, n) read as zero (padding)
+    float s[4][4], xv[4][4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
-            if (full) {
-                const float4 v = *reinterpret_cast<const float4*>(in + e0);
-                xv[q][0] = v.x;
-                xv[q][1] = v.y;
-                xv[q][2] = v.z;
-                xv[q][3] = v.w;
+    for (int q = 0; q < 4; q++) {
+        const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
+        if (full) {
+            const float4 v = *reinterpret_cast<const float4*>(in + e0);
+            xv[q][0] = v.x;
+            xv[q][1] = v.y;
+            xv[q][2] = v.z;
+            xv[q][3] = v.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) xv[q][k] = (e0 + k < nv) ? in[e0 + k] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[q][k] = quant_round(xv[q][k], r);
+    }
+    // delta_i = s_i - s_{i-1}, chain restarts at the tile start.
+    float d[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float prev = __shfl_up(s[q][3], 1, 64);
+        const float last_prev_q = __shfl(q > 0 ? s[q > 0 ? q - 1 : 0][3] : 0.0f, 63, 64);
+        if (l == 0) prev = (q > 0) ? last_prev_q : 0.0f;
+        d[q][0] = s[q][0] - prev;
+#pragma unroll
+        for (int k = 1; k < 4; k++) d[q][k] = s[q][k] - s[q][k - 1];
+    }
+    TCode c[4][4];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
+            const bool ok = fabsf(d[q][k]) < rad;
+            c[q][k] = ok ? enc_code<TCode, ZZ>(d[q][k]) : (TCode)0;
+            if (!ok && e < n) mask |= 1u << (q * 4 + k);
+        }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
+        if (base + 1024 <= n) {
+            if constexpr (sizeof(TCode) == 2) {
+                uint2 w;
+                w.x = (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 16);
+                w.y = (uint32_t)c[q][2] | ((uint32_t)c[q][3] << 16);
+                *reinterpret_cast<uint2*>(codes + e0) = w;
             } else {
-#pragma unroll
-                for (int k = 0; k < 4; k++) xv[q][k] = (e0 + k < nv) ? in[e0 + k] : 0.0f;
+                *reinterpret_cast<uint32_t*>(codes + e0) =
+                    (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 8) | ((uint32_t)c[q][2] << 16) |
+                    ((uint32_t)c[q][3] << 24);
             }
+        } else {
 #pragma unroll
-            for (int k = 0; k < 4; k++) s[q][k] = quant_round(xv[q][k], r);
+            for (int k = 0; k < 4; k++)
+                if (e0 + k < n) codes[e0 + k] = c[q][k];
         }
-        // delta_i = s_i - s_{i-1}, chain restarts at the tile start.
-        float d[4][4];
-        uint32_t mask = 0;
+    }
+    // Tile-local lists in index order: key (q, lane, k).
+    auto emit = [&](uint32_t m, bool patch, float* vdst, uint32_t* idst) {
+        uint32_t run = 0;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            float prev = __shfl_up(s[q][3], 1, 64);
-            const float last_prev_q = __shfl(q > 0 ? s[q > 0 ? q - 1 : 0][3] : 0.0f, 63, 64);
-            if (l == 0) prev = (q > 0) ? last_prev_q : 0.0f;
-            d[q][0] = s[q][0] - prev;
+            const uint32_t mq = (m >> (4 * q)) & 0xFu;
+            const uint32_t cq = __popc(mq);
+            const uint32_t incl = wave_incl_scan(cq);
+            uint32_t pos = run + incl - cq;
 #pragma unroll
-            for (int k = 1; k < 4; k++) d[q][k] = s[q][k] - s[q][k - 1];
+            for (int k = 0; k < 4; k++)
+                if (mq & (1u << k)) {
+                    vdst[pos] = patch ? xv[q][k] : d[q][k];
+                    idst[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
+                    pos++;
+                }
+            run += __shfl(incl, 63, 64);
         }
-        TCode c[4][4];
+        return run;
+    };
+    const uint32_t cnt = emit(mask, false, tl.err + base, tl.idx + base);
+    if (l == 0) tl.cnt[tile] = cnt;
+    if constexpr (STRICT) {
+        const float eb = tl.d_params ? tl.d_params[1] : tl.eb;
+        const float e2 = tl.d_params ? tl.d_params[3] : tl.ebx2;
+        bool huge = false;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) huge |= !(fabsf(s[q][k]) < kExactS);
+        const bool tile_huge = __ballot(huge) != 0ull;
+        uint32_t pm = 0;
 #pragma unroll
         for (int q = 0; q < 4; q++)
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
-                const bool ok = fabsf(d[q][k]) < rad;
-                c[q][k] = ok ? enc_code<TCode, ZZ>(d[q][k]) : (TCode)0;
-                if (!ok && e < n) mask |= 1u << (q * 4 + k);
+                if (e < nv && (tile_huge || needs_patch(xv[q][k], s[q][k], eb, e2))) pm |= 1u << (q * 4 + k);
             }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
-            if (base + 1024 <= n) {
-                if constexpr (sizeof(TCode) == 2) {
-                    uint2 w;
-                    w.x = (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 16);
-                    w.y = (uint32_t)c[q][2] | ((uint32_t)c[q][3] << 16);
-                    *reinterpret_cast<uint2*>(codes + e0) = w;
-                } else {
-                    *reinterpret_cast<uint32_t*>(codes + e0) =
-                        (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 8) | ((uint32_t)c[q][2] << 16) |
-                        ((uint32_t)c[q][3] << 24);
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (e0 + k < n) codes[e0 + k] = c[q][k];
-            }
-        }
-        // Outliers in index order: key (q, lane, k).
-        const uint32_t agg = wave_sum(__popc(mask));
-        const uint32_t obase = lookback_exclusive(status, tk.tile, tk.epoch, agg);
-        const bool lb_ok = obase != kLookbackFailed;
-        if (agg && lb_ok) {
-            uint32_t run = obase;
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint32_t mq = (mask >> (4 * q)) & 0xFu;
-                const uint32_t cq = __popc(mq);
-                const uint32_t incl = wave_incl_scan(cq);
-                uint32_t pos = run + incl - cq;
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (mq & (1u << k)) {
-                        if (pos < max_out) {
-                            oerr[pos] = d[q][k];
-                            oidx[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
-                        }
-                        pos++;
-                    }
-                run += __shfl(incl, 63, 64);
-            }
-        }
-        if (tk.tile == T - 1 && l == 0) *ocount = lb_ok ? obase + agg : 0xFFFFFFFFu;
-        if constexpr (STRICT) {
-            const float eb = so.d_params ? so.d_params[1] : so.eb;
-            const float e2 = so.d_params ? so.d_params[3] : so.ebx2;
-            bool huge = false;
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-#pragma unroll
-                for (int k = 0; k < 4; k++) huge |= !(fabsf(s[q][k]) < kExactS);
-            const bool tile_huge = __ballot(huge) != 0ull;
-            uint32_t pm = 0;
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
-                    if (e < nv && (tile_huge || needs_patch(xv[q][k], s[q][k], eb, e2))) pm |= 1u << (q * 4 + k);
-                }
-            const uint32_t pagg = wave_sum(__popc(pm));
-            const uint32_t pbase = lookback_exclusive(so.status, tk.tile, tk.epoch, pagg);
-            const bool p_ok = pbase != kLookbackFailed;
-            if (pagg && p_ok) {
-                uint32_t run = pbase;
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t mq = (pm >> (4 * q)) & 0xFu;
-                    const uint32_t cq = __popc(mq);
-                    const uint32_t incl = wave_incl_scan(cq);
-                    uint32_t pos = run + incl - cq;
-#pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        if (mq & (1u << k)) {
-                            if (pos < so.cap) {
-                                so.val[pos] = xv[q][k];
-                                so.idx[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
-                            }
-                            pos++;
-                        }
-                    run += __shfl(incl, 63, 64);
-                }
-            }
-            if (tk.tile == T - 1 && l == 0) *so.count = p_ok ? pbase + pagg : 0xFFFFFFFFu;
-        }
+        const uint32_t pc = emit(pm, true, tl.pval + base, tl.pidx + base);
+        if (l == 0) tl.pcnt[tile] = pc;
     }
-    release_ticket(ctx, tk.epoch, total_waves);
 }
 
 // Zero codes[from, to) (tail beyond nx*ny[*nz] when the input was padded).
@@ -235,235 +214,279 @@ __device__ __forceinline__ void zero_code_tail(TCode* codes, uint64_t from, uint
 template <typename TCode, bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256)
 lz2d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n, float r_host,
-                const float* __restrict__ d_r, float rad, TCode* __restrict__ codes,
-                float* __restrict__ oerr, uint32_t* __restrict__ oidx, uint32_t* __restrict__ ocount,
-                uint64_t max_out, ScanCtx* ctx, uint64_t* status, uint32_t gx, uint32_t T,
-                uint32_t total_waves, StrictOut so) {
-    const TileTicket tk = take_ticket(ctx);
+                const float* __restrict__ d_r, float rad, TCode* __restrict__ codes, TileLists tl, uint32_t gx,
+                uint32_t T) {
+    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
-    if (tk.tile < T) {
-        const int l = lane_id();
-        const int x = l & 31, h = l >> 5;
-        const uint32_t bx = tk.tile % gx, by = tk.tile / gx;
-        const uint32_t X = bx * 32 + x;
-        const uint32_t Y0 = by * 32 + 16 * h;
-        const bool xin = X < nx;
-        float s[16], xv[16];
+    const int l = lane_id();
+    const int x = l & 31, h = l >> 5;
+    const uint32_t bx = tile % gx, by = tile / gx;
+    const uint32_t X = bx * 32 + x;
+    const uint32_t Y0 = by * 32 + 16 * h;
+    const bool xin = X < nx;
+    float s[16], xv[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t Y = Y0 + i;
+        xv[i] = (xin && Y < ny) ? in[(uint64_t)Y * nx + X] : 0.0f;
+        s[i] = quant_round(xv[i], r);
+    }
+    // t = s(y) - s(y-1) inside the tile (row 0 of the tile subtracts 0).
+    const float north = __shfl(s[15], x, 64);  // row 15 of the tile, for h = 1
+    float t[16];
+    t[0] = s[0] - (h ? north : 0.0f);
+#pragma unroll
+    for (int i = 1; i < 16; i++) t[i] = s[i] - s[i - 1];
+    uint32_t mask = 0;
+    uint64_t bal[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const float w = __shfl_up(t[i], 1, 32);
+        const float d = (x > 0) ? t[i] - w : t[i];
+        const bool valid = xin && (Y0 + i) < ny;
+        const bool ok = fabsf(d) < rad;
+        if (valid) codes[(uint64_t)(Y0 + i) * nx + X] = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
+        const bool is_out = valid && !ok;
+        if (is_out) mask |= 1u << i;
+        bal[i] = __ballot(is_out);
+        t[i] = d;  // keep the delta for the outlier pass
+    }
+    const uint64_t lbase = (uint64_t)tile * 1024;
+    // Row-major order inside the tile: rows 0..15 (h=0) then 16..31 (h=1).
+    auto emit = [&](const uint64_t (&bb)[16], uint32_t m, bool patch, float* vdst, uint32_t* idst) {
+        uint32_t tot = 0, lo = 0;
 #pragma unroll
         for (int i = 0; i < 16; i++) {
-            const uint32_t Y = Y0 + i;
-            xv[i] = (xin && Y < ny) ? in[(uint64_t)Y * nx + X] : 0.0f;
-            s[i] = quant_round(xv[i], r);
+            tot += __popcll(bb[i]);
+            lo += __popc((uint32_t)bb[i]);
         }
-        bool huge = false;
-        if constexpr (STRICT) {
-#pragma unroll
-            for (int i = 0; i < 16; i++) huge |= !(fabsf(s[i]) < kExactS);
-        }
-        // t = s(y) - s(y-1) inside the tile (row 0 of the tile subtracts 0).
-        const float north = __shfl(s[15], x, 64);  // row 15 of the tile, for h = 1
-        float t[16];
-        t[0] = s[0] - (h ? north : 0.0f);
-#pragma unroll
-        for (int i = 1; i < 16; i++) t[i] = s[i] - s[i - 1];
-        uint32_t mask = 0;
-        uint64_t bal[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const float w = __shfl_up(t[i], 1, 32);
-            const float d = (x > 0) ? t[i] - w : t[i];
-            const bool valid = xin && (Y0 + i) < ny;
-            const bool ok = fabsf(d) < rad;
-            if (valid) codes[(uint64_t)(Y0 + i) * nx + X] = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
-            const bool is_out = valid && !ok;
-            if (is_out) mask |= 1u << i;
-            bal[i] = __ballot(is_out);
-            t[i] = d;  // keep the delta for the outlier pass
-        }
-        uint32_t agg = 0, lo_tot = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            agg += __popcll(bal[i]);
-            lo_tot += __popc((uint32_t)bal[i]);
-        }
-        const uint32_t obase = lookback_exclusive(status, tk.tile, tk.epoch, agg);
-        const bool lb_ok = obase != kLookbackFailed;
-        if (agg && lb_ok) {
-            // Row-major order inside the tile: rows 0..15 (h=0) then 16..31 (h=1).
-            uint32_t run = h ? lo_tot : 0u;
+        if (tot) {
+            uint32_t run = h ? lo : 0u;
             const uint32_t below = (1u << x) - 1u;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const uint32_t half = h ? (uint32_t)(bal[i] >> 32) : (uint32_t)bal[i];
-                if (mask & (1u << i)) {
-                    const uint32_t pos = obase + run + __popc(half & below);
-                    if (pos < max_out) {
-                        oerr[pos] = t[i];
-                        oidx[pos] = (Y0 + i) * nx + X;
-                    }
+                const uint32_t half = h ? (uint32_t)(bb[i] >> 32) : (uint32_t)bb[i];
+                if (m & (1u << i)) {
+                    const uint32_t pos = run + __popc(half & below);
+                    vdst[pos] = patch ? xv[i] : t[i];
+                    idst[pos] = (Y0 + i) * nx + X;
                 }
                 run += __popc(half);
             }
         }
-        if (tk.tile == T - 1) {
-            if (l == 0) *ocount = lb_ok ? obase + agg : 0xFFFFFFFFu;
-            zero_code_tail(codes, (uint64_t)nx * ny, n);
-        }
-        if constexpr (STRICT) {
-            const float eb = so.d_params ? so.d_params[1] : so.eb;
-            const float e2 = so.d_params ? so.d_params[3] : so.ebx2;
-            const bool tile_huge = __ballot(huge) != 0ull;
-            uint32_t pm = 0;
-            uint64_t pbal[16];
-            uint32_t pagg = 0, plo = 0;
+        return tot;
+    };
+    const uint32_t cnt = emit(bal, mask, false, tl.err + lbase, tl.idx + lbase);
+    if (l == 0) tl.cnt[tile] = cnt;
+    if (tile == T - 1) zero_code_tail(codes, (uint64_t)nx * ny, n);
+    if constexpr (STRICT) {
+        const float eb = tl.d_params ? tl.d_params[1] : tl.eb;
+        const float e2 = tl.d_params ? tl.d_params[3] : tl.ebx2;
+        bool huge = false;
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const bool valid = xin && (Y0 + i) < ny;
-                const bool p = valid && (tile_huge || needs_patch(xv[i], s[i], eb, e2));
-                if (p) pm |= 1u << i;
-                pbal[i] = __ballot(p);
-                pagg += __popcll(pbal[i]);
-                plo += __popc((uint32_t)pbal[i]);
-            }
-            const uint32_t pbase = lookback_exclusive(so.status, tk.tile, tk.epoch, pagg);
-            const bool p_ok = pbase != kLookbackFailed;
-            if (pagg && p_ok) {
-                uint32_t run = h ? plo : 0u;
-                const uint32_t below = (1u << x) - 1u;
+        for (int i = 0; i < 16; i++) huge |= !(fabsf(s[i]) < kExactS);
+        const bool tile_huge = __ballot(huge) != 0ull;
+        uint32_t pm = 0;
+        uint64_t pbal[16];
 #pragma unroll
-                for (int i = 0; i < 16; i++) {
-                    const uint32_t half = h ? (uint32_t)(pbal[i] >> 32) : (uint32_t)pbal[i];
-                    if (pm & (1u << i)) {
-                        const uint32_t pos = pbase + run + __popc(half & below);
-                        if (pos < so.cap) {
-                            so.val[pos] = xv[i];
-                            so.idx[pos] = (Y0 + i) * nx + X;
-                        }
-                    }
-                    run += __popc(half);
-                }
-            }
-            if (tk.tile == T - 1 && l == 0) *so.count = p_ok ? pbase + pagg : 0xFFFFFFFFu;
+        for (int i = 0; i < 16; i++) {
+            const bool valid = xin && (Y0 + i) < ny;
+            const bool p = valid && (tile_huge || needs_patch(xv[i], s[i], eb, e2));
+            if (p) pm |= 1u << i;
+            pbal[i] = __ballot(p);
         }
+        const uint32_t pc = emit(pbal, pm, true, tl.pval + lbase, tl.pidx + lbase);
+        if (l == 0) tl.pcnt[tile] = pc;
     }
-    release_ticket(ctx, tk.epoch, total_waves);
 }
 
 template <typename TCode, bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256)
 lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, uint64_t n,
-                float r_host, const float* __restrict__ d_r, float rad, TCode* __restrict__ codes,
-                float* __restrict__ oerr, uint32_t* __restrict__ oidx, uint32_t* __restrict__ ocount,
-                uint64_t max_out, ScanCtx* ctx, uint64_t* status, Tiles tg, uint32_t T,
-                uint32_t total_waves, StrictOut so) {
-    const TileTicket tk = take_ticket(ctx);
+                float r_host, const float* __restrict__ d_r, float rad, TCode* __restrict__ codes, TileLists tl,
+                Tiles tg, uint32_t T) {
+    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
-    if (tk.tile < T) {
-        const int l = lane_id();
-        const int x = l & 7, z = l >> 3;
-        const uint32_t bx = tk.tile % tg.gx, by = (tk.tile / tg.gx) % tg.gy, bz = tk.tile / (tg.gx * tg.gy);
-        const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
-        const bool xz = X < nx && Z < nz;
-        const uint64_t plane = (uint64_t)nx * ny;
-        float s[8], xv[8];
+    const int l = lane_id();
+    const int x = l & 7, z = l >> 3;
+    const uint32_t bx = tile % tg.gx, by = (tile / tg.gx) % tg.gy, bz = tile / (tg.gx * tg.gy);
+    const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
+    const bool xz = X < nx && Z < nz;
+    const uint64_t plane = (uint64_t)nx * ny;
+    float s[8], xv[8];
 #pragma unroll
-        for (int y = 0; y < 8; y++) {
-            xv[y] = (xz && Y0 + y < ny) ? in[Z * plane + (uint64_t)(Y0 + y) * nx + X] : 0.0f;
-            s[y] = quant_round(xv[y], r);
-        }
-        // a = s - s(z-1); b = a - a(x-1); d = b - b(y-1)   (lorenzo_quant_nd.cu:234-246)
-        float b[8];
+    for (int y = 0; y < 8; y++) {
+        xv[y] = (xz && Y0 + y < ny) ? in[Z * plane + (uint64_t)(Y0 + y) * nx + X] : 0.0f;
+        s[y] = quant_round(xv[y], r);
+    }
+    // a = s - s(z-1); b = a - a(x-1); d = b - b(y-1)   (lorenzo_quant_nd.cu:234-246)
+    float b[8];
 #pragma unroll
-        for (int y = 0; y < 8; y++) {
-            const float sz = __shfl_up(s[y], 8, 64);
-            const float a = s[y] - (z > 0 ? sz : 0.0f);
-            const float aw = __shfl_up(a, 1, 8);
-            b[y] = (x > 0) ? a - aw : a;
-        }
-        float d[8];
-        uint32_t mask = 0;
-        uint64_t bal[8];
+    for (int y = 0; y < 8; y++) {
+        const float sz = __shfl_up(s[y], 8, 64);
+        const float a = s[y] - (z > 0 ? sz : 0.0f);
+        const float aw = __shfl_up(a, 1, 8);
+        b[y] = (x > 0) ? a - aw : a;
+    }
+    float d[8];
+    uint32_t mask = 0;
+    uint64_t bal[8];
 #pragma unroll
-        for (int y = 0; y < 8; y++) {
-            d[y] = (y > 0) ? b[y] - b[y - 1] : b[y];
-            const bool valid = xz && (Y0 + y) < ny;
-            const bool ok = fabsf(d[y]) < rad;
-            if (valid) codes[Z * plane + (uint64_t)(Y0 + y) * nx + X] = ok ? enc_code<TCode, ZZ>(d[y]) : (TCode)0;
-            const bool is_out = valid && !ok;
-            if (is_out) mask |= 1u << y;
-            bal[y] = __ballot(is_out);
-        }
-        uint32_t agg = 0;
+    for (int y = 0; y < 8; y++) {
+        d[y] = (y > 0) ? b[y] - b[y - 1] : b[y];
+        const bool valid = xz && (Y0 + y) < ny;
+        const bool ok = fabsf(d[y]) < rad;
+        if (valid) codes[Z * plane + (uint64_t)(Y0 + y) * nx + X] = ok ? enc_code<TCode, ZZ>(d[y]) : (TCode)0;
+        const bool is_out = valid && !ok;
+        if (is_out) mask |= 1u << y;
+        bal[y] = __ballot(is_out);
+    }
+    const uint64_t lbase = (uint64_t)tile * 512;
+    // Order inside the tile: (z, y, x).
+    auto emit = [&](const uint64_t (&bb)[8], uint32_t m, bool patch, float* vdst, uint32_t* idst) {
+        uint32_t tot = 0;
 #pragma unroll
-        for (int y = 0; y < 8; y++) agg += __popcll(bal[y]);
-        const uint32_t obase = lookback_exclusive(status, tk.tile, tk.epoch, agg);
-        const bool lb_ok = obase != kLookbackFailed;
-        if (agg && lb_ok) {
-            // Order inside the tile: (z, y, x).
+        for (int y = 0; y < 8; y++) tot += __popcll(bb[y]);
+        if (tot) {
             const uint64_t zlow = (z == 0) ? 0ull : ((1ull << (8 * z)) - 1ull);
             uint32_t run = 0;
 #pragma unroll
-            for (int y = 0; y < 8; y++) run += __popcll(bal[y] & zlow);
+            for (int y = 0; y < 8; y++) run += __popcll(bb[y] & zlow);
             const uint32_t below = (1u << x) - 1u;
 #pragma unroll
             for (int y = 0; y < 8; y++) {
-                const uint32_t row = (uint32_t)(bal[y] >> (8 * z)) & 0xFFu;
-                if (mask & (1u << y)) {
-                    const uint32_t pos = obase + run + __popc(row & below);
-                    if (pos < max_out) {
-                        oerr[pos] = d[y];
-                        oidx[pos] = (uint32_t)(Z * plane + (uint64_t)(Y0 + y) * nx + X);
-                    }
+                const uint32_t row = (uint32_t)(bb[y] >> (8 * z)) & 0xFFu;
+                if (m & (1u << y)) {
+                    const uint32_t pos = run + __popc(row & below);
+                    vdst[pos] = patch ? xv[y] : d[y];
+                    idst[pos] = (uint32_t)(Z * plane + (uint64_t)(Y0 + y) * nx + X);
                 }
                 run += __popc(row);
             }
         }
-        if (tk.tile == T - 1) {
-            if (l == 0) *ocount = lb_ok ? obase + agg : 0xFFFFFFFFu;
-            zero_code_tail(codes, (uint64_t)nx * ny * nz, n);
+        return tot;
+    };
+    const uint32_t cnt = emit(bal, mask, false, tl.err + lbase, tl.idx + lbase);
+    if (l == 0) tl.cnt[tile] = cnt;
+    if (tile == T - 1) zero_code_tail(codes, (uint64_t)nx * ny * nz, n);
+    if constexpr (STRICT) {
+        const float eb = tl.d_params ? tl.d_params[1] : tl.eb;
+        const float e2 = tl.d_params ? tl.d_params[3] : tl.ebx2;
+        bool huge = false;
+#pragma unroll
+        for (int y = 0; y < 8; y++) huge |= !(fabsf(s[y]) < kExactS);
+        const bool tile_huge = __ballot(huge) != 0ull;
+        uint32_t pm = 0;
+        uint64_t pbal[8];
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            const bool valid = xz && (Y0 + y) < ny;
+            const bool p = valid && (tile_huge || needs_patch(xv[y], s[y], eb, e2));
+            if (p) pm |= 1u << y;
+            pbal[y] = __ballot(p);
         }
-        if constexpr (STRICT) {
-            const float eb = so.d_params ? so.d_params[1] : so.eb;
-            const float e2 = so.d_params ? so.d_params[3] : so.ebx2;
-            bool huge = false;
-#pragma unroll
-            for (int y = 0; y < 8; y++) huge |= !(fabsf(s[y]) < kExactS);
-            const bool tile_huge = __ballot(huge) != 0ull;
-            uint32_t pm = 0, pagg = 0;
-            uint64_t pbal[8];
-#pragma unroll
-            for (int y = 0; y < 8; y++) {
-                const bool valid = xz && (Y0 + y) < ny;
-                const bool p = valid && (tile_huge || needs_patch(xv[y], s[y], eb, e2));
-                if (p) pm |= 1u << y;
-                pbal[y] = __ballot(p);
-                pagg += __popcll(pbal[y]);
-            }
-            const uint32_t pbase = lookback_exclusive(so.status, tk.tile, tk.epoch, pagg);
-            const bool p_ok = pbase != kLookbackFailed;
-            if (pagg && p_ok) {
-                const uint64_t zlow = (z == 0) ? 0ull : ((1ull << (8 * z)) - 1ull);
-                uint32_t run = 0;
-#pragma unroll
-                for (int y = 0; y < 8; y++) run += __popcll(pbal[y] & zlow);
-                const uint32_t below = (1u << x) - 1u;
-#pragma unroll
-                for (int y = 0; y < 8; y++) {
-                    const uint32_t row = (uint32_t)(pbal[y] >> (8 * z)) & 0xFFu;
-                    if (pm & (1u << y)) {
-                        const uint32_t pos = pbase + run + __popc(row & below);
-                        if (pos < so.cap) {
-                            so.val[pos] = xv[y];
-                            so.idx[pos] = (uint32_t)(Z * plane + (uint64_t)(Y0 + y) * nx + X);
-                        }
-                    }
-                    run += __popc(row);
-                }
-            }
-            if (tk.tile == T - 1 && l == 0) *so.count = p_ok ? pbase + pagg : 0xFFFFFFFFu;
+        const uint32_t pc = emit(pbal, pm, true, tl.pval + lbase, tl.pidx + lbase);
+        if (l == 0) tl.pcnt[tile] = pc;
+    }
+}
+
+// ===========================================================================
+// Compaction of the per-tile lists: workgroup g owns tiles [1024g, 1024g+1024),
+// scans their counts, takes its base from the aggregates of the earlier
+// workgroups and copies the entries.  Up to two lists (outliers, patches).
+// ===========================================================================
+struct CompactList {
+    const float* s_val;
+    const uint32_t* s_idx;
+    const uint32_t* cnt;
+    float* val;
+    uint32_t* idx;
+    uint32_t* total;
+    uint64_t cap;
+    uint64_t* status;
+};
+
+template <int NL>
+__global__ void __launch_bounds__(256)
+compact_kernel(CompactList L0, CompactList L1, uint32_t T, uint32_t E, ScanCtx* ctx, uint32_t G) {
+    __shared__ uint32_t s_off[NL][1024];
+    __shared__ uint32_t s_cnt[NL][1024];
+    __shared__ uint32_t s_wsum[NL][4];
+    __shared__ uint32_t s_base[NL];
+    __shared__ uint32_t s_unit, s_epoch;
+    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    if (w == 0) {
+        const TileTicket tk = take_ticket(ctx);
+        if (l == 0) {
+            s_unit = tk.tile;
+            s_epoch = tk.epoch;
         }
     }
-    release_ticket(ctx, tk.epoch, total_waves);
+    __syncthreads();
+    const uint32_t g = s_unit, ep = s_epoch;
+    if (g < G) {
+#pragma unroll
+        for (int li = 0; li < NL; li++) {
+            const CompactList& L = li == 0 ? L0 : L1;
+            uint32_t c[4], sum = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t tile = (uint64_t)g * 1024 + 4 * tid + k;
+                c[k] = tile < T ? L.cnt[tile] : 0u;
+                sum += c[k];
+            }
+            const uint32_t incl = wave_incl_scan(sum);
+            if (l == 63) s_wsum[li][w] = incl;
+            __syncthreads();
+            uint32_t wbase = 0;
+            for (int j = 0; j < w; j++) wbase += s_wsum[li][j];
+            uint32_t run = wbase + incl - sum;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                s_off[li][4 * tid + k] = run;
+                s_cnt[li][4 * tid + k] = c[k];
+                run += c[k];
+            }
+        }
+        __syncthreads();
+        if (w == 0) {
+#pragma unroll
+            for (int li = 0; li < NL; li++) {
+                const CompactList& L = li == 0 ? L0 : L1;
+                const uint32_t agg = s_wsum[li][0] + s_wsum[li][1] + s_wsum[li][2] + s_wsum[li][3];
+                const uint32_t base = aggregate_prefix(L.status, g, ep, agg);
+                if (l == 0) {
+                    s_base[li] = base;
+                    if (g == G - 1) *L.total = base == kLookbackFailed ? 0xFFFFFFFFu : base + agg;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int li = 0; li < NL; li++) {
+            const CompactList& L = li == 0 ? L0 : L1;
+            const uint32_t base = s_base[li];
+            if (base == kLookbackFailed) continue;
+            for (int t = w; t < 1024; t += 4) {
+                const uint64_t tile = (uint64_t)g * 1024 + t;
+                if (tile >= T) break;
+                const uint32_t c = s_cnt[li][t];
+                if (c == 0) continue;
+                const uint64_t dst0 = (uint64_t)base + s_off[li][t];
+                const uint64_t src0 = tile * E;
+                for (uint32_t j = l; j < c; j += 64) {
+                    const uint64_t pos = dst0 + j;
+                    if (pos < L.cap) {
+                        L.val[pos] = L.s_val[src0 + j];
+                        L.idx[pos] = L.s_idx[src0 + j];
+                    }
+                }
+            }
+        }
+    }
+    if (w == 0) release_ticket(ctx, ep, G);
 }
 
 // ===========================================================================
@@ -473,6 +496,7 @@ __global__ void __launch_bounds__(256)
 minmax_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ partial,
               uint32_t* __restrict__ counter, int mode, bool zero_pad, float user_eb, float* __restrict__ params) {
     __shared__ float smn[4], smx[4];
+    __shared__ bool s_last;
     const float first = in[0];
     float mn = first, mx = first;
     if (zero_pad) {  // the buffer continues with zeros up to the padded length
@@ -513,16 +537,32 @@ minmax_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ par
         }
         partial[blockIdx.x] = make_float2(mn, mx);
         __threadfence();
-        const uint32_t prev = atomicAdd(counter, 1u);
-        if (prev == gridDim.x - 1) {
-            __threadfence();
-            float gmn = partial[0].x, gmx = partial[0].y;
-            for (uint32_t b = 1; b < gridDim.x; b++) {
-                const uint64_t raw = __hip_atomic_load(reinterpret_cast<const uint64_t*>(&partial[b]),
-                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const float px = __uint_as_float((uint32_t)raw), py = __uint_as_float((uint32_t)(raw >> 32));
-                if (px < gmn) gmn = px;
-                if (py > gmx) gmx = py;
+        s_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s_last) {
+        // The last workgroup reduces the partials in parallel.
+        __threadfence();
+        float gmn = INFINITY, gmx = -INFINITY;
+        for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+            const uint64_t raw = __hip_atomic_load(reinterpret_cast<const uint64_t*>(&partial[b]),
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const float px = __uint_as_float((uint32_t)raw), py = __uint_as_float((uint32_t)(raw >> 32));
+            if (px < gmn) gmn = px;
+            if (py > gmx) gmx = py;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const float a = __shfl_xor(gmn, off, 64), b = __shfl_xor(gmx, off, 64);
+            if (a < gmn) gmn = a;
+            if (b > gmx) gmx = b;
+        }
+        if (lane_id() == 0) { smn[w] = gmn; smx[w] = gmx; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 0; i < 4; i++) {
+                if (smn[i] < gmn) gmn = smn[i];
+                if (smx[i] > gmx) gmx = smx[i];
             }
             float vb;
             if (mode == 2) {
@@ -598,7 +638,7 @@ bin_scan_kernel(uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ off, uin
         }
         const uint32_t incl = wave_incl_scan(sum);
         const uint32_t agg = __shfl(incl, 63, 64);
-        uint32_t pre = lookback_exclusive(status, tk.tile, tk.epoch, agg);
+        uint32_t pre = aggregate_prefix(status, tk.tile, tk.epoch, agg);
         if (pre == kLookbackFailed) pre = 0u;  // offsets stay in range; counts are bounded by the list
         uint32_t run = pre + incl - sum;
 #pragma unroll
@@ -874,43 +914,90 @@ int check_dims(size_t n, size_t nx, size_t ny, size_t nz, const char* who) {
     return FZH_OK;
 }
 
-// Forward workspace: [ScanCtx + outlier status words][patch status words].
-size_t fwd_ws_bytes(size_t T) { return align_up(scan_state_bytes(T), 256) + align_up(8 * (T + 1), 256); }
+// Forward workspace: compaction scan state (ScanCtx + one status array per
+// list), per-tile counts, and the per-tile list scratch (T x E entries per
+// list; written only where outliers/patches occur).
+struct FwdWs {
+    size_t T, E, G;
+    size_t o_ctx, o_st0, o_st1, o_cnt, o_pcnt, o_err, o_idx, o_pval, o_pidx, total;
+};
+
+FwdWs fwd_layout(size_t n, size_t nx, size_t ny, size_t nz, bool strict) {
+    FwdWs w;
+    w.T = tile_count(n, nx, ny, nz);
+    w.E = ndim_of(ny, nz) == 3 ? 512 : 1024;
+    w.G = std::max<size_t>(1, cdiv(w.T, 1024));
+    size_t o = 0;
+    w.o_ctx = o;  o = align_up(o + sizeof(ScanCtx), 256);
+    w.o_st0 = o;  o = align_up(o + 8 * (w.G + 1), 256);
+    w.o_st1 = o;  o = align_up(o + 8 * (w.G + 1), 256);
+    w.o_cnt = o;  o = align_up(o + 4 * std::max<size_t>(w.T, 1), 256);
+    w.o_pcnt = o; if (strict) o = align_up(o + 4 * std::max<size_t>(w.T, 1), 256);
+    w.o_err = o;  o = align_up(o + 4 * w.T * w.E, 256);
+    w.o_idx = o;  o = align_up(o + 4 * w.T * w.E, 256);
+    w.o_pval = o; if (strict) o = align_up(o + 4 * w.T * w.E, 256);
+    w.o_pidx = o; if (strict) o = align_up(o + 4 * w.T * w.E, 256);
+    w.total = o;
+    return w;
+}
+
+struct PatchOut {
+    const float* d_params;
+    float eb, ebx2;
+    float* val;
+    uint32_t* idx;
+    uint32_t* count;
+    size_t cap;
+};
 
 template <typename TCode, bool STRICT>
 int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, size_t nz, float r,
                const float* d_r, uint32_t radius, int zz, void* codes, float* oerr, uint32_t* oidx,
-               uint32_t* ocount, size_t max_out, void* ws, size_t ws_bytes, StrictOut so) {
-    const size_t T = tile_count(n, nx, ny, nz);
-    if (ws_bytes < fwd_ws_bytes(T)) return fail(FZH_E_CAPACITY, "lorenzo fwd: workspace too small");
-    ScanCtx* ctx = (ScanCtx*)ws;
-    uint64_t* status = (uint64_t*)((uint8_t*)ws + sizeof(ScanCtx));
-    so.status = (uint64_t*)((uint8_t*)ws + align_up(scan_state_bytes(T), 256));
+               uint32_t* ocount, size_t max_out, void* ws, size_t ws_bytes, const PatchOut& po) {
+    const FwdWs L = fwd_layout(n, nx, ny, nz, STRICT);
+    if (ws_bytes < L.total) return fail(FZH_E_CAPACITY, "lorenzo fwd: workspace too small");
+    const size_t T = L.T;
+    uint8_t* w = (uint8_t*)ws;
+    ScanCtx* ctx = (ScanCtx*)(w + L.o_ctx);
+    TileLists tl{};
+    tl.err = (float*)(w + L.o_err);
+    tl.idx = (uint32_t*)(w + L.o_idx);
+    tl.cnt = (uint32_t*)(w + L.o_cnt);
+    if (STRICT) {
+        tl.pval = (float*)(w + L.o_pval);
+        tl.pidx = (uint32_t*)(w + L.o_pidx);
+        tl.pcnt = (uint32_t*)(w + L.o_pcnt);
+        tl.d_params = po.d_params;
+        tl.eb = po.eb;
+        tl.ebx2 = po.ebx2;
+    }
     const float rad = (float)(TCode)radius;
     const unsigned blocks = (unsigned)cdiv(T, 4);
-    const uint32_t waves = blocks * 4;
     TCode* c = (TCode*)codes;
     const int nd = ndim_of(ny, nz);
     const size_t nv = (nd == 1 && nx > 0 && nx < n) ? nx : n;
-#define FZH_LZ_FWD(ZZV)                                                                               \
-    if (nd == 1)                                                                                      \
-        lz1d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(in, n, nv, r, d_r, rad, c, oerr, oidx, ocount, \
-                                                           max_out, ctx, status, (uint32_t)T, waves, so); \
-    else if (nd == 2)                                                                                 \
-        lz2d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(                                   \
-            in, (uint32_t)nx, (uint32_t)ny, n, r, d_r, rad, c, oerr, oidx, ocount, max_out, ctx,      \
-            status, (uint32_t)cdiv(nx, 32), (uint32_t)T, waves, so);                                  \
-    else                                                                                              \
-        lz3d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(                                   \
-            in, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, n, r, d_r, rad, c, oerr, oidx, ocount,      \
-            max_out, ctx, status, Tiles{(uint32_t)cdiv(nx, 8), (uint32_t)cdiv(ny, 8), 0}, (uint32_t)T, \
-            waves, so);
+#define FZH_LZ_FWD(ZZV)                                                                                     \
+    if (nd == 1)                                                                                            \
+        lz1d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(in, n, nv, r, d_r, rad, c, tl, (uint32_t)T); \
+    else if (nd == 2)                                                                                       \
+        lz2d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(in, (uint32_t)nx, (uint32_t)ny, n, r, d_r, \
+                                                                   rad, c, tl, (uint32_t)cdiv(nx, 32),     \
+                                                                   (uint32_t)T);                           \
+    else                                                                                                    \
+        lz3d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(                                         \
+            in, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, n, r, d_r, rad, c, tl,                            \
+            Tiles{(uint32_t)cdiv(nx, 8), (uint32_t)cdiv(ny, 8), 0}, (uint32_t)T);
     if (zz) {
         FZH_LZ_FWD(true)
     } else {
         FZH_LZ_FWD(false)
     }
 #undef FZH_LZ_FWD
+    CompactList L0{tl.err, tl.idx, tl.cnt, oerr, oidx, ocount, max_out, (uint64_t*)(w + L.o_st0)};
+    CompactList L1{};
+    if (STRICT) L1 = CompactList{tl.pval, tl.pidx, tl.pcnt, po.val, po.idx, po.count, po.cap, (uint64_t*)(w + L.o_st1)};
+    compact_kernel<STRICT ? 2 : 1><<<(unsigned)L.G, 256, 0, s>>>(L0, L1, (uint32_t)T, (uint32_t)L.E, ctx,
+                                                                  (uint32_t)L.G);
     FZH_CHECK_LAUNCH("lorenzo fwd");
     return FZH_OK;
 }
@@ -1002,7 +1089,11 @@ using namespace fzh;
 extern "C" {
 
 size_t fzh_lorenzo_quant_fwd_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz) {
-    return fwd_ws_bytes(tile_count(n, nx, ny, nz));
+    return fwd_layout(n, nx, ny, nz, false).total;
+}
+
+size_t fzh_lorenzo_quant_fwd_strict_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz) {
+    return fwd_layout(n, nx, ny, nz, true).total;
 }
 
 int fzh_lorenzo_quant_fwd_f32(void* stream, const float* d_in, size_t n, size_t nx, size_t ny, size_t nz,
@@ -1016,7 +1107,7 @@ int fzh_lorenzo_quant_fwd_f32(void* stream, const float* d_in, size_t n, size_t 
         return fail(FZH_E_INVALID, "lorenzo fwd: null outlier buffers");
     if (ndim_of(ny, nz) == 1 && reinterpret_cast<uintptr_t>(d_in) % 16 != 0)
         return fail(FZH_E_INVALID, "lorenzo fwd: input must be 16-byte aligned");
-    const StrictOut none{};
+    const PatchOut none{};
     switch (code_bytes) {
     case 1:
         return launch_fwd<uint8_t, false>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag, d_codes,
@@ -1046,23 +1137,23 @@ int fzh_lorenzo_quant_fwd_strict_f32(void* stream, const float* d_in, size_t n, 
         return fail(FZH_E_INVALID, "lorenzo fwd (strict): null patch buffers");
     if (ndim_of(ny, nz) == 1 && reinterpret_cast<uintptr_t>(d_in) % 16 != 0)
         return fail(FZH_E_INVALID, "lorenzo fwd (strict): input must be 16-byte aligned");
-    StrictOut so{};
-    so.d_params = d_eb_params;
-    so.eb = eb;
-    so.ebx2 = ebx2;
-    so.val = d_patch_val;
-    so.idx = d_patch_idx;
-    so.count = d_patch_count;
-    so.cap = max_patches;
+    PatchOut po{};
+    po.d_params = d_eb_params;
+    po.eb = eb;
+    po.ebx2 = ebx2;
+    po.val = d_patch_val;
+    po.idx = d_patch_idx;
+    po.count = d_patch_count;
+    po.cap = max_patches;
     switch (code_bytes) {
     case 1:
         return launch_fwd<uint8_t, true>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag, d_codes,
                                          d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers, d_ws,
-                                         ws_bytes, so);
+                                         ws_bytes, po);
     case 2:
         return launch_fwd<uint16_t, true>(S(stream), d_in, n, nx, ny, nz, ebx2_r, d_ebx2_r, radius, zigzag,
                                           d_codes, d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers,
-                                          d_ws, ws_bytes, so);
+                                          d_ws, ws_bytes, po);
     default:
         return fail(FZH_E_INVALID, "lorenzo fwd (strict): code_bytes must be 1 or 2");
     }

@@ -11,16 +11,18 @@
 // The raw-fallback thresholds are the reference's, verbatim (:369-431).
 //
 // MI355X design
-//   encode: one wave64 per chunk (4 waves per workgroup).  The chunk is held
-//   in registers (16 x 16 B per lane), bitmaps are built with per-lane masks
-//   in LDS, the compressed chunk is assembled in LDS, and its byte offset
-//   comes from the ordered tile scan — so the stream is written once, already
-//   packed (the reference writes per-chunk scratch, then scans on the host
-//   stream with CUB and copies again).
-//   decode: one wave64 per chunk; chunk input offsets come from the same
-//   ordered scan over the header's sizes (no host round trip), and the
-//   repeat-elimination decode is a parallel prefix-popcount gather instead of
-//   a single-thread loop.
+//   encode: one wave64 per chunk.  The chunk is held in registers (16 x 16 B
+//   per lane), bitmaps are built with per-lane masks in LDS and the
+//   compressed chunk is assembled in LDS, then written to its slot of a
+//   chunk-sized scratch (raw chunks are not copied at all).  A pack kernel
+//   places every chunk at its offset, which each workgroup sums directly from
+//   the sizes the encode kernel wrote -- no host round trip and no wave
+//   waiting on another (the reference stages per-chunk scratch too, then
+//   scans on the host stream with CUB and copies).
+//   decode: a one-workgroup kernel scans the header's sizes into chunk
+//   offsets; then one wave64 per chunk decodes, with the repeat-elimination
+//   decode done as a parallel prefix-popcount gather instead of a
+//   single-thread loop.
 #include "fzh_internal.h"
 
 namespace fzh {
@@ -72,19 +74,17 @@ __device__ __forceinline__ void lds_to_global(const uint8_t* lds, uint8_t* dst, 
 // ---------------------------------------------------------------------------
 // Encode
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * WAVES)
 rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restrict__ out,
-               uint64_t* __restrict__ out_size, ScanCtx* ctx, uint64_t* status, uint32_t nc,
-               uint32_t total_waves) {
+               uint8_t* __restrict__ scratch, uint32_t* __restrict__ sizes, uint32_t nc) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int l = lane_id();
     const int wv = threadIdx.x >> 6;
     uint8_t* st = smem + wv * LDS_PER_WAVE;  // assembled chunk stream
     uint8_t* bm1 = st + LDS_STREAM;
 
-    const TileTicket tk = take_ticket(ctx);
-    if (tk.tile < nc) {
-        const uint32_t c = tk.tile;
+    const uint32_t c = blockIdx.x * WAVES + wv;
+    if (c < nc) {
         const uint64_t in_off = (uint64_t)c * CS;
         const int in_size = (int)((nbytes - in_off) < (uint64_t)CS ? (nbytes - in_off) : CS);
         const bool full = in_size == CS;
@@ -235,28 +235,101 @@ rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restr
             }
         }
         wave_lds_sync();
-        const uint32_t hdr = 8u + 4u * nc;
-        const uint32_t off = lookback_exclusive(status, c, tk.epoch, clean);
-        if (off == kLookbackFailed) {
-            if (l == 0 && c == nc - 1) *out_size = 0;  // signals failure to the host
-        } else {
+        if (!raw && clean > 0) {
+            // compressed bytes -> this chunk's scratch slot (dword stores; the
+            // slot is CS bytes and a non-raw chunk is shorter)
+            const uint32_t nd = (clean + 3) / 4;
+            const uint32_t* sw = reinterpret_cast<const uint32_t*>(st);
+            uint32_t* dw = reinterpret_cast<uint32_t*>(scratch + (uint64_t)c * CS);
+            for (uint32_t i = l; i < nd; i += 64) dw[i] = sw[i];
+        }
         if (l == 0) {
+            sizes[c] = size;
             reinterpret_cast<uint32_t*>(out + 8)[c] = size;
             if (c == 0) {
                 reinterpret_cast<uint32_t*>(out)[0] = (uint32_t)nbytes;
                 reinterpret_cast<uint32_t*>(out)[1] = nc;
             }
         }
-        lds_to_global(st, out + hdr + off, clean);
+    }
+}
+
+// Copy `size` bytes from `src` (4-byte aligned; `limit` bytes readable) to
+// `dst` (any alignment), touching no byte outside [dst, dst+size).  Whole
+// wave calls.
+__device__ __forceinline__ void copy_to_unaligned(const uint8_t* src, uint64_t limit, uint8_t* dst, uint32_t size) {
+    if (size == 0) return;
+    const int l = lane_id();
+    const uint32_t head = (uint32_t)((4 - ((uintptr_t)dst & 3)) & 3);
+    const uint32_t hb = head < size ? head : size;
+    if ((uint32_t)l < hb) dst[l] = src[l];
+    if (size <= hb) return;
+    uint32_t* dw = reinterpret_cast<uint32_t*>(dst + hb);
+    const uint32_t nd = (size - hb) / 4;
+    const uint32_t* sw = reinterpret_cast<const uint32_t*>(src);
+    auto word = [&](uint32_t q) -> uint32_t {
+        if (4ull * q + 4 <= limit) return sw[q];
+        uint32_t v = 0;
+        for (uint32_t k = 0; k < 4; k++)
+            if (4ull * q + k < limit) v |= (uint32_t)src[4 * q + k] << (8 * k);
+        return v;
+    };
+    for (uint32_t i = l; i < nd; i += 64) {
+        const uint32_t p = hb + 4 * i;  // source byte of this dword
+        const uint32_t q = p >> 2;
+        dw[i] = hb == 0 ? sw[q] : __builtin_amdgcn_alignbyte(word(q + 1), sw[q], hb);
+    }
+    const uint32_t done = hb + 4 * nd;
+    if ((uint32_t)l < size - done) dst[done + l] = src[done + l];
+}
+
+// Places every chunk at [8 + 4 nc + offset): workgroup g owns chunks
+// [PACK * g, PACK * g + PACK) and sums all earlier sizes itself (the encode
+// kernel has finished, so they are all in memory).
+constexpr uint32_t PACK = 16;
+
+__global__ void __launch_bounds__(256)
+rze_pack_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, const uint8_t* __restrict__ scratch,
+                const uint32_t* __restrict__ sizes, uint8_t* __restrict__ out, uint64_t* __restrict__ out_size,
+                uint32_t nc) {
+    __shared__ uint32_t s_red[4];
+    __shared__ uint32_t s_off[PACK];
+    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    const uint32_t c0 = blockIdx.x * PACK;
+    uint32_t part = 0;
+    for (uint32_t i = tid; i < c0; i += 256) part += sizes[i] & 0x7FFFFFFFu;
+    part = wave_sum(part);
+    if (l == 0) s_red[w] = part;
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < PACK; k++) {
+            s_off[k] = run;
+            if (c0 + k < nc) run += sizes[c0 + k] & 0x7FFFFFFFu;
+        }
+    }
+    __syncthreads();
+    const uint64_t base = (uint64_t)s_red[0] + s_red[1] + s_red[2] + s_red[3];
+    const uint64_t hdr = 8ull + 4ull * nc;
+    for (uint32_t k = w; k < PACK; k += 4) {
+        const uint32_t c = c0 + k;
+        if (c >= nc) break;
+        const uint32_t size = sizes[c];
+        const uint32_t clean = size & 0x7FFFFFFFu;
+        const bool raw = (size & 0x80000000u) != 0u;
+        const uint64_t dst = hdr + base + s_off[k];
+        if (raw) {
+            const uint64_t src = (uint64_t)c * CS;
+            copy_to_unaligned(in + src, nbytes - src, out + dst, clean);
+        } else {
+            copy_to_unaligned(scratch + (uint64_t)c * CS, CS, out + dst, clean);
+        }
         if (c == nc - 1) {
-            const uint64_t end = (uint64_t)hdr + off + clean;
+            const uint64_t end = dst + clean;
             const uint64_t padded = (end + 3) & ~3ull;
             if ((uint64_t)l < padded - end) out[end + l] = 0;
             if (l == 0) *out_size = padded;
         }
-        }
     }
-    release_ticket(ctx, tk.epoch, total_waves);
 }
 
 // ---------------------------------------------------------------------------
@@ -265,19 +338,43 @@ rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restr
 // Repeat-elimination decode of `n` bytes: out[i] = data[rank(i)-1], or 0 when
 // no bit of bm[0..i] is set (rze_stage.cu:283-298, done with prefix popcounts).
 
-__global__ void __launch_bounds__(256)
+// One workgroup: exclusive scan of the header's chunk sizes into offsets.
+__global__ void __launch_bounds__(1024)
+rze_offsets_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint32_t* __restrict__ offs, uint32_t nc) {
+    __shared__ uint32_t s_wsum[16];
+    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    const uint64_t hdr = 8ull + 4ull * nc;
+    const uint32_t per = (nc + 1023) / 1024;
+    const uint32_t a = tid * per;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t c = a + k;
+        if (c < nc && hdr <= in_bytes) sum += reinterpret_cast<const uint32_t*>(in + 8)[c] & 0x7FFFFFFFu;
+    }
+    const uint32_t incl = wave_incl_scan(sum);
+    if (l == 63) s_wsum[w] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (int j = 0; j < w; j++) run += s_wsum[j];
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t c = a + k;
+        if (c >= nc) break;
+        offs[c] = run;
+        if (hdr <= in_bytes) run += reinterpret_cast<const uint32_t*>(in + 8)[c] & 0x7FFFFFFFu;
+    }
+}
+
+__global__ void __launch_bounds__(64 * WAVES)
 rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __restrict__ out,
-               uint64_t orig_bytes, int* __restrict__ d_status, ScanCtx* ctx, uint64_t* status, uint32_t nc,
-               uint32_t total_waves) {
+               uint64_t orig_bytes, int* __restrict__ d_status, const uint32_t* __restrict__ offs, uint32_t nc) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int l = lane_id();
     const int wv = threadIdx.x >> 6;
     uint8_t* st = smem + wv * LDS_PER_WAVE;
     uint8_t* bm1 = st + LDS_STREAM;
 
-    const TileTicket tk = take_ticket(ctx);
-    if (tk.tile < nc) {
-        const uint32_t c = tk.tile;
+    const uint32_t c = blockIdx.x * WAVES + wv;
+    if (c < nc) {
         const uint32_t hdr = 8u + 4u * nc;
         bool bad = in_bytes < hdr;
         uint32_t entry = 0;
@@ -288,8 +385,7 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
             entry = reinterpret_cast<const uint32_t*>(in + 8)[c];
         }
         const uint32_t stored = bad ? 0u : (entry & 0x7FFFFFFFu);
-        const uint32_t off = lookback_exclusive(status, c, tk.epoch, stored);
-        if (off == kLookbackFailed) bad = true;
+        const uint32_t off = offs[c];
         const uint64_t src = (uint64_t)hdr + off;
         const uint32_t o = (c + 1 < nc) ? (uint32_t)CS : (uint32_t)(orig_bytes - (uint64_t)c * CS);
         const bool israw = (entry & 0x80000000u) != 0u;
@@ -437,7 +533,6 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
             }
         }
     }
-    release_ticket(ctx, tk.epoch, total_waves);
 }
 
 constexpr size_t kLds = (size_t)WAVES * LDS_PER_WAVE;
@@ -456,7 +551,11 @@ size_t fzh_rze_enc_bound(size_t nbytes) {
     return nbytes + 8 + 4 * nc + (nbytes % CS) / 7 + 64;
 }
 
-size_t fzh_rze_enc_ws_bytes(size_t nbytes) { return align_up(scan_state_bytes(cdiv(nbytes, CS)), 256); }
+// [sizes u32 x nc][chunk scratch, CS bytes per chunk]
+size_t fzh_rze_enc_ws_bytes(size_t nbytes) {
+    const size_t nc = cdiv(nbytes, CS);
+    return align_up(4 * std::max<size_t>(nc, 1), 256) + nc * CS;
+}
 
 int fzh_rze_enc(void* stream, const void* d_in, size_t nbytes, void* d_out, size_t out_cap, uint64_t* d_out_size,
                 void* d_ws, size_t ws_bytes) {
@@ -469,15 +568,17 @@ int fzh_rze_enc(void* stream, const void* d_in, size_t nbytes, void* d_out, size
     if (((uintptr_t)d_in | (uintptr_t)d_out) % 16 != 0) return fail(FZH_E_INVALID, "RZE: buffers must be 16-byte aligned");
     const uint32_t nc = (uint32_t)cdiv(nbytes, CS);
     const unsigned blocks = (unsigned)cdiv(nc, WAVES);
-    ScanCtx* ctx = (ScanCtx*)d_ws;
-    uint64_t* status = (uint64_t*)((uint8_t*)d_ws + sizeof(ScanCtx));
-    rze_enc_kernel<<<blocks, 64 * WAVES, kLds, S(stream)>>>((const uint8_t*)d_in, nbytes, (uint8_t*)d_out, d_out_size,
-                                                            ctx, status, nc, blocks * WAVES);
+    uint32_t* sizes = (uint32_t*)d_ws;
+    uint8_t* scratch = (uint8_t*)d_ws + align_up(4 * (size_t)nc, 256);
+    rze_enc_kernel<<<blocks, 64 * WAVES, kLds, S(stream)>>>((const uint8_t*)d_in, nbytes, (uint8_t*)d_out, scratch,
+                                                            sizes, nc);
+    rze_pack_kernel<<<(unsigned)cdiv(nc, PACK), 256, 0, S(stream)>>>((const uint8_t*)d_in, nbytes, scratch, sizes,
+                                                                     (uint8_t*)d_out, d_out_size, nc);
     FZH_CHECK_LAUNCH("rze_enc_kernel");
     return FZH_OK;
 }
 
-size_t fzh_rze_dec_ws_bytes(size_t orig_bytes) { return align_up(scan_state_bytes(cdiv(orig_bytes, CS)), 256); }
+size_t fzh_rze_dec_ws_bytes(size_t orig_bytes) { return align_up(4 * std::max<size_t>(cdiv(orig_bytes, CS), 1), 256); }
 
 int fzh_rze_dec(void* stream, const void* d_in, size_t in_bytes, void* d_out, size_t orig_bytes, int* d_status,
                 void* d_ws, size_t ws_bytes) {
@@ -489,10 +590,10 @@ int fzh_rze_dec(void* stream, const void* d_in, size_t in_bytes, void* d_out, si
     if (!d_status) return fail(FZH_E_INVALID, "RZE: null status pointer");
     const uint32_t nc = (uint32_t)cdiv(orig_bytes, CS);
     const unsigned blocks = (unsigned)cdiv(nc, WAVES);
-    ScanCtx* ctx = (ScanCtx*)d_ws;
-    uint64_t* status = (uint64_t*)((uint8_t*)d_ws + sizeof(ScanCtx));
+    uint32_t* offs = (uint32_t*)d_ws;
+    rze_offsets_kernel<<<1, 1024, 0, S(stream)>>>((const uint8_t*)d_in, in_bytes, offs, nc);
     rze_dec_kernel<<<blocks, 64 * WAVES, kLds, S(stream)>>>((const uint8_t*)d_in, in_bytes, (uint8_t*)d_out,
-                                                            orig_bytes, d_status, ctx, status, nc, blocks * WAVES);
+                                                            orig_bytes, d_status, offs, nc);
     FZH_CHECK_LAUNCH("rze_dec_kernel");
     return FZH_OK;
 }

@@ -92,6 +92,7 @@ int fzh_lorenzo_quant_fwd_f32(void* stream, const float* d_in, size_t n,
  * ebx2 = 2*eb are read from d_eb_params[1] / [3] (fzh_value_base_f32's
  * layout) when it is non-NULL.  Guaranteed for finite inputs; a tile with
  * |x/(2 eb)| >= 2^21 anywhere is patched whole.                             */
+size_t fzh_lorenzo_quant_fwd_strict_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz);
 int fzh_lorenzo_quant_fwd_strict_f32(void* stream, const float* d_in, size_t n,
                                      size_t nx, size_t ny, size_t nz,
                                      float ebx2_r, const float* d_ebx2_r,
