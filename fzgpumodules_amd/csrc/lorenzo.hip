@@ -15,7 +15,7 @@
 // tile scan, so the outlier list is deterministic: tile-major, and sorted by
 // element index inside each tile (the reference appends them with atomics in
 // arbitrary order).
-#include "fzh_internal.h"
+#include "fzh_compact.h"
 
 namespace fzh {
 namespace {
@@ -390,103 +390,6 @@ lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t
         const uint32_t pc = emit(pbal, pm, true, tl.pval + lbase, tl.pidx + lbase);
         if (l == 0) tl.pcnt[tile] = pc;
     }
-}
-
-// ===========================================================================
-// Compaction of the per-tile lists: workgroup g owns tiles [1024g, 1024g+1024),
-// scans their counts, takes its base from the aggregates of the earlier
-// workgroups and copies the entries.  Up to two lists (outliers, patches).
-// ===========================================================================
-struct CompactList {
-    const float* s_val;
-    const uint32_t* s_idx;
-    const uint32_t* cnt;
-    float* val;
-    uint32_t* idx;
-    uint32_t* total;
-    uint64_t cap;
-    uint64_t* status;
-};
-
-template <int NL>
-__global__ void __launch_bounds__(256)
-compact_kernel(CompactList L0, CompactList L1, uint32_t T, uint32_t E, ScanCtx* ctx, uint32_t G) {
-    __shared__ uint32_t s_off[NL][1024];
-    __shared__ uint32_t s_cnt[NL][1024];
-    __shared__ uint32_t s_wsum[NL][4];
-    __shared__ uint32_t s_base[NL];
-    __shared__ uint32_t s_unit, s_epoch;
-    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
-    if (w == 0) {
-        const TileTicket tk = take_ticket(ctx);
-        if (l == 0) {
-            s_unit = tk.tile;
-            s_epoch = tk.epoch;
-        }
-    }
-    __syncthreads();
-    const uint32_t g = s_unit, ep = s_epoch;
-    if (g < G) {
-#pragma unroll
-        for (int li = 0; li < NL; li++) {
-            const CompactList& L = li == 0 ? L0 : L1;
-            uint32_t c[4], sum = 0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint64_t tile = (uint64_t)g * 1024 + 4 * tid + k;
-                c[k] = tile < T ? L.cnt[tile] : 0u;
-                sum += c[k];
-            }
-            const uint32_t incl = wave_incl_scan(sum);
-            if (l == 63) s_wsum[li][w] = incl;
-            __syncthreads();
-            uint32_t wbase = 0;
-            for (int j = 0; j < w; j++) wbase += s_wsum[li][j];
-            uint32_t run = wbase + incl - sum;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                s_off[li][4 * tid + k] = run;
-                s_cnt[li][4 * tid + k] = c[k];
-                run += c[k];
-            }
-        }
-        __syncthreads();
-        if (w == 0) {
-#pragma unroll
-            for (int li = 0; li < NL; li++) {
-                const CompactList& L = li == 0 ? L0 : L1;
-                const uint32_t agg = s_wsum[li][0] + s_wsum[li][1] + s_wsum[li][2] + s_wsum[li][3];
-                const uint32_t base = aggregate_prefix(L.status, g, ep, agg);
-                if (l == 0) {
-                    s_base[li] = base;
-                    if (g == G - 1) *L.total = base == kLookbackFailed ? 0xFFFFFFFFu : base + agg;
-                }
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int li = 0; li < NL; li++) {
-            const CompactList& L = li == 0 ? L0 : L1;
-            const uint32_t base = s_base[li];
-            if (base == kLookbackFailed) continue;
-            for (int t = w; t < 1024; t += 4) {
-                const uint64_t tile = (uint64_t)g * 1024 + t;
-                if (tile >= T) break;
-                const uint32_t c = s_cnt[li][t];
-                if (c == 0) continue;
-                const uint64_t dst0 = (uint64_t)base + s_off[li][t];
-                const uint64_t src0 = tile * E;
-                for (uint32_t j = l; j < c; j += 64) {
-                    const uint64_t pos = dst0 + j;
-                    if (pos < L.cap) {
-                        L.val[pos] = L.s_val[src0 + j];
-                        L.idx[pos] = L.s_idx[src0 + j];
-                    }
-                }
-            }
-        }
-    }
-    if (w == 0) release_ticket(ctx, ep, G);
 }
 
 // ===========================================================================
@@ -919,7 +822,7 @@ int check_dims(size_t n, size_t nx, size_t ny, size_t nz, const char* who) {
 // list; written only where outliers/patches occur).
 struct FwdWs {
     size_t T, E, G;
-    size_t o_ctx, o_st0, o_st1, o_cnt, o_pcnt, o_err, o_idx, o_pval, o_pidx, total;
+    size_t o_ctx, o_cnt, o_pcnt, o_err, o_idx, o_pval, o_pidx, total;
 };
 
 FwdWs fwd_layout(size_t n, size_t nx, size_t ny, size_t nz, bool strict) {
@@ -928,9 +831,7 @@ FwdWs fwd_layout(size_t n, size_t nx, size_t ny, size_t nz, bool strict) {
     w.E = ndim_of(ny, nz) == 3 ? 512 : 1024;
     w.G = std::max<size_t>(1, cdiv(w.T, 1024));
     size_t o = 0;
-    w.o_ctx = o;  o = align_up(o + sizeof(ScanCtx), 256);
-    w.o_st0 = o;  o = align_up(o + 8 * (w.G + 1), 256);
-    w.o_st1 = o;  o = align_up(o + 8 * (w.G + 1), 256);
+    w.o_ctx = o;  o = align_up(o + compact_state_bytes(w.T), 256);
     w.o_cnt = o;  o = align_up(o + 4 * std::max<size_t>(w.T, 1), 256);
     w.o_pcnt = o; if (strict) o = align_up(o + 4 * std::max<size_t>(w.T, 1), 256);
     w.o_err = o;  o = align_up(o + 4 * w.T * w.E, 256);
@@ -958,7 +859,6 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
     if (ws_bytes < L.total) return fail(FZH_E_CAPACITY, "lorenzo fwd: workspace too small");
     const size_t T = L.T;
     uint8_t* w = (uint8_t*)ws;
-    ScanCtx* ctx = (ScanCtx*)(w + L.o_ctx);
     TileLists tl{};
     tl.err = (float*)(w + L.o_err);
     tl.idx = (uint32_t*)(w + L.o_idx);
@@ -993,11 +893,9 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
         FZH_LZ_FWD(false)
     }
 #undef FZH_LZ_FWD
-    CompactList L0{tl.err, tl.idx, tl.cnt, oerr, oidx, ocount, max_out, (uint64_t*)(w + L.o_st0)};
-    CompactList L1{};
-    if (STRICT) L1 = CompactList{tl.pval, tl.pidx, tl.pcnt, po.val, po.idx, po.count, po.cap, (uint64_t*)(w + L.o_st1)};
-    compact_kernel<STRICT ? 2 : 1><<<(unsigned)L.G, 256, 0, s>>>(L0, L1, (uint32_t)T, (uint32_t)L.E, ctx,
-                                                                  (uint32_t)L.G);
+    CompactList L0{tl.err, tl.idx, tl.cnt, oerr, oidx, ocount, max_out, nullptr};
+    CompactList L1{tl.pval, tl.pidx, tl.pcnt, po.val, po.idx, po.count, po.cap, nullptr};
+    launch_compact(s, w + L.o_ctx, T, (uint32_t)L.E, L0, STRICT ? &L1 : nullptr);
     FZH_CHECK_LAUNCH("lorenzo fwd");
     return FZH_OK;
 }

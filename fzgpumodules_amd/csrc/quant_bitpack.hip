@@ -6,13 +6,14 @@
 //   |q| < radius and |x| < threshold -> code = q (two's complement in TCode)
 //   or zigzag(q); otherwise code 0 and an outlier (x, i).
 //   inverse: out = (float)q * ebx2, then outliers assign their value.
-// Outliers are placed in index order with the ordered tile scan (the
-// reference appends with atomics).
+// Outliers come out in index order: each 1024-element tile writes its list
+// to a scratch slot and a compaction kernel places the lists (the reference
+// appends with atomics).
 //
 // Bitpack (modules/coders/bitpack/bitpack_stage.cu:30-124): the low `nbits`
 // bits of each element, nbits in {1,2,4,8,16,32}; sub-byte: element k of a
 // byte at bit k*nbits; multi-byte: little endian.
-#include "fzh_internal.h"
+#include "fzh_compact.h"
 
 namespace fzh {
 namespace {
@@ -35,15 +36,14 @@ constexpr int QTILE = 1024;  // elements per wave (16 per lane)
 template <typename TCode, bool ZZ>
 __global__ void __launch_bounds__(256)
 quant_fwd_kernel(const float* __restrict__ in, uint64_t n, float r, float threshold, int rad,
-                 TCode* __restrict__ codes, float* __restrict__ oval, uint32_t* __restrict__ oidx,
-                 uint32_t* __restrict__ ocount, uint64_t max_out, ScanCtx* ctx, uint64_t* status, uint32_t T,
-                 uint32_t total_waves) {
+                 TCode* __restrict__ codes, float* __restrict__ sval, uint32_t* __restrict__ sidx,
+                 uint32_t* __restrict__ scnt, uint32_t T) {
     using SC = typename SignedOf<TCode>::type;
     constexpr int W = 8 * sizeof(TCode);
-    const TileTicket tk = take_ticket(ctx);
-    if (tk.tile < T) {
+    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile < T) {
         const int l = lane_id();
-        const uint64_t base = (uint64_t)tk.tile * QTILE;
+        const uint64_t base = (uint64_t)tile * QTILE;
         float x[16];
         uint32_t mask = 0;
 #pragma unroll
@@ -77,9 +77,8 @@ quant_fwd_kernel(const float* __restrict__ in, uint64_t n, float r, float thresh
             }
         }
         const uint32_t agg = wave_sum(__popc(mask));
-        const uint32_t ob = lookback_exclusive(status, tk.tile, tk.epoch, agg);
-        if (agg && ob != kLookbackFailed) {
-            uint32_t run = ob;
+        if (agg) {
+            uint32_t run = 0;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const uint32_t mq = (mask >> (4 * q)) & 0xFu;
@@ -89,18 +88,15 @@ quant_fwd_kernel(const float* __restrict__ in, uint64_t n, float r, float thresh
 #pragma unroll
                 for (int k = 0; k < 4; k++)
                     if (mq & (1u << k)) {
-                        if (pos < max_out) {
-                            oval[pos] = x[4 * q + k];
-                            oidx[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
-                        }
+                        sval[base + pos] = x[4 * q + k];
+                        sidx[base + pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
                         pos++;
                     }
                 run += __shfl(incl, 63, 64);
             }
         }
-        if (tk.tile == T - 1 && l == 0) *ocount = (ob == kLookbackFailed) ? 0xFFFFFFFFu : ob + agg;
+        if (l == 0) scnt[tile] = agg;
     }
-    release_ticket(ctx, tk.epoch, total_waves);
 }
 
 template <typename TCode, bool ZZ>
@@ -218,8 +214,10 @@ using namespace fzh;
 
 extern "C" {
 
+// [compaction state][counts u32 x T][scratch values f32 x T*QTILE][scratch indices u32 x T*QTILE]
 size_t fzh_quant_abs_fwd_ws_bytes(size_t n) {
-    return align_up(scan_state_bytes(cdiv(std::max<size_t>(n, 1), QTILE)), 256) + 4096 + 256;
+    const size_t T = cdiv(std::max<size_t>(n, 1), QTILE);
+    return compact_state_bytes(T) + align_up(4 * T, 256) + 2 * align_up(4 * T * QTILE, 256);
 }
 
 int fzh_quant_abs_fwd_f32(void* stream, const float* d_in, size_t n, float ebx2_r, float threshold, uint32_t radius,
@@ -229,15 +227,20 @@ int fzh_quant_abs_fwd_f32(void* stream, const float* d_in, size_t n, float ebx2_
     if (n == 0) return FZH_OK;
     if (n > 0xFFFFFFFFull) return fail(FZH_E_INVALID, "quantizer: more than 2^32 elements");
     if ((uintptr_t)d_in % 16) return fail(FZH_E_INVALID, "quantizer: input must be 16-byte aligned");
+    if (max_outliers > 0 && (!d_oval || !d_oidx)) return fail(FZH_E_INVALID, "quantizer: null outlier buffers");
     const uint32_t T = (uint32_t)cdiv(n, QTILE);
     const unsigned blocks = (unsigned)cdiv(T, 4);
-    ScanCtx* ctx = (ScanCtx*)d_ws;
-    uint64_t* status = (uint64_t*)((uint8_t*)d_ws + sizeof(ScanCtx));
+    uint8_t* w = (uint8_t*)d_ws;
+    const size_t o_cnt = compact_state_bytes(T);
+    const size_t o_val = o_cnt + align_up(4 * (size_t)T, 256);
+    const size_t o_idx = o_val + align_up(4 * (size_t)T * QTILE, 256);
+    uint32_t* scnt = (uint32_t*)(w + o_cnt);
+    float* sval = (float*)(w + o_val);
+    uint32_t* sidx = (uint32_t*)(w + o_idx);
     hipStream_t s = S(stream);
 #define FZH_Q(TC, ZZV)                                                                                              \
     quant_fwd_kernel<TC, ZZV><<<blocks, 256, 0, s>>>(d_in, n, ebx2_r, threshold, (int)(TC)radius, (TC*)d_codes,   \
-                                                     d_oval, d_oidx, d_ocount, max_outliers, ctx, status, T,      \
-                                                     blocks * 4)
+                                                     sval, sidx, scnt, T)
     switch (code_bytes) {
     case 1: if (zigzag) FZH_Q(uint8_t, true); else FZH_Q(uint8_t, false); break;
     case 2: if (zigzag) FZH_Q(uint16_t, true); else FZH_Q(uint16_t, false); break;
@@ -245,6 +248,8 @@ int fzh_quant_abs_fwd_f32(void* stream, const float* d_in, size_t n, float ebx2_
     default: return fail(FZH_E_INVALID, "quantizer: code_bytes must be 1, 2 or 4");
     }
 #undef FZH_Q
+    CompactList L0{sval, sidx, scnt, d_oval, d_oidx, d_ocount, max_outliers, nullptr};
+    launch_compact(s, w, T, QTILE, L0, nullptr);
     FZH_CHECK_LAUNCH("quant_fwd_kernel");
     return FZH_OK;
 }
