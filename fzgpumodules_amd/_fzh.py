@@ -42,6 +42,8 @@ SIGNATURES = {
     "fzh_rze_enc_bound": (_Z, [_Z]),
     "fzh_rze_enc_ws_bytes": (_Z, [_Z]),
     "fzh_rze_enc": (_I, [_P, _P, _Z, _P, _Z, _P, _P, _Z]),
+    "fzh_rze_enc_shuffled": (_I, [_P, _P, _Z, _I, _P, _Z, _P, _P, _Z]),
+    "fzh_rze_dec_shuffled": (_I, [_P, _P, _Z, _P, _Z, _I, _P, _P, _Z]),
     "fzh_rze_dec_ws_bytes": (_Z, [_Z]),
     "fzh_rze_dec": (_I, [_P, _P, _Z, _P, _Z, _P, _P, _Z]),
     "fzh_rle_enc_bound": (_Z, [_Z, _I]),
@@ -85,6 +87,7 @@ SIGNATURES = {
     "fzp_finalize": (_I, [_P]),
     "fzp_enable_graph_mode": (_I, [_P, _I]),
     "fzp_enable_profiling": (_I, [_P, _I]),
+    "fzp_set_stage_fusion": (_I, [_P, _I]),
     "fzp_set_pool_managed_decomp_output": (_I, [_P, _I]),
     "fzp_max_compressed_size": (_I, [_P, _Z, C.POINTER(_Z)]),
     "fzp_compress": (_I, [_P, _P, _P, _Z, C.POINTER(_P), C.POINTER(_Z)]),

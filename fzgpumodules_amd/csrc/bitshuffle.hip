@@ -14,26 +14,10 @@
 // loads, transposes the bit matrix in registers (log2 block-swap steps) and
 // writes one word per plane; lanes of a wave own consecutive groups, so each
 // plane store is a contiguous 256-byte segment.
-#include "fzh_internal.h"
+#include "fzh_bitplane.h"
 
 namespace fzh {
 namespace {
-
-// In-register transpose step on rows A[k], A[k+s] (LSB-first columns):
-// swaps bits (row k, col i+s) <-> (row k+s, col i) for i with bit s clear.
-template <int S, int N>
-__device__ __forceinline__ void transpose_step(uint32_t (&A)[N]) {
-    constexpr uint32_t M = S == 16 ? 0x0000FFFFu : S == 8 ? 0x00FF00FFu : S == 4 ? 0x0F0F0F0Fu
-                         : S == 2 ? 0x33333333u : 0x55555555u;
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-        if ((k & S) == 0 && k + S < N) {
-            const uint32_t t = ((A[k] >> S) ^ A[k + S]) & M;
-            A[k] ^= t << S;
-            A[k + S] ^= t;
-        }
-    }
-}
 
 // Full 32x32 transpose: A[j] bit b  <->  A[b] bit j.
 __device__ __forceinline__ void transpose32(uint32_t (&A)[32]) {
@@ -44,14 +28,7 @@ __device__ __forceinline__ void transpose32(uint32_t (&A)[32]) {
     transpose_step<1>(A);
 }
 
-__device__ __forceinline__ uint4 ld16(const uint8_t* p) { return *reinterpret_cast<const uint4*>(p); }
-__device__ __forceinline__ void st16(uint8_t* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
-
-// ---------------------------------------------------------------------------
-// 2-byte elements: group = 32 u16 = 64 bytes.  Packed rows P[k] = E_k | E_{k+16}<<16
-// (the 16-step of the 32x32 transpose with the upper 16 columns zero), then
-// steps 8..1 on 16 rows give W_b (bit j = bit b of E_j) for b = 0..15.
-// ---------------------------------------------------------------------------
+// 2-byte elements: group = 32 u16 = 64 bytes.
 __device__ __forceinline__ void enc_group_u16(const uint8_t* src, uint32_t* dst_words, uint32_t npp) {
     uint32_t w[16];
 #pragma unroll
@@ -60,15 +37,7 @@ __device__ __forceinline__ void enc_group_u16(const uint8_t* src, uint32_t* dst_
         w[4 * i] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
     }
     uint32_t A[16];
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        A[2 * i] = (w[i] & 0xFFFFu) | (w[8 + i] << 16);
-        A[2 * i + 1] = (w[i] >> 16) | (w[8 + i] & 0xFFFF0000u);
-    }
-    transpose_step<8>(A);
-    transpose_step<4>(A);
-    transpose_step<2>(A);
-    transpose_step<1>(A);
+    planes16(w, A);
 #pragma unroll
     for (int b = 0; b < 16; b++) dst_words[(size_t)(15 - b) * npp] = A[b];
 }
@@ -77,37 +46,19 @@ __device__ __forceinline__ void dec_group_u16(const uint32_t* src_words, uint8_t
     uint32_t A[16];
 #pragma unroll
     for (int b = 0; b < 16; b++) A[b] = src_words[(size_t)(15 - b) * npp];
-    transpose_step<8>(A);
-    transpose_step<4>(A);
-    transpose_step<2>(A);
-    transpose_step<1>(A);
     uint32_t w[16];
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        w[i] = (A[2 * i] & 0xFFFFu) | (A[2 * i + 1] << 16);
-        w[8 + i] = (A[2 * i] >> 16) | (A[2 * i + 1] & 0xFFFF0000u);
-    }
+    unplanes16(A, w);
 #pragma unroll
     for (int i = 0; i < 4; i++) st16(dst + 16 * i, make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]));
 }
 
-// 1-byte elements: group = 32 bytes; P[k] = E_k | E_{k+8}<<8 | E_{k+16}<<16 | E_{k+24}<<24.
+// 1-byte elements: group = 32 bytes.
 __device__ __forceinline__ void enc_group_u8(const uint8_t* src, uint32_t* dst_words, uint32_t npp) {
     uint32_t w[8];
     const uint4 v0 = ld16(src), v1 = ld16(src + 16);
     w[0] = v0.x; w[1] = v0.y; w[2] = v0.z; w[3] = v0.w; w[4] = v1.x; w[5] = v1.y; w[6] = v1.z; w[7] = v1.w;
-    // byte j of the group lives in w[j/4], bits 8*(j%4)
     uint32_t A[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int sh = 8 * (k & 3);
-        const int wi = k >> 2;  // word holding byte k (k < 8)
-        A[k] = ((w[wi] >> sh) & 0xFFu) | (((w[wi + 2] >> sh) & 0xFFu) << 8) |
-               (((w[wi + 4] >> sh) & 0xFFu) << 16) | (((w[wi + 6] >> sh) & 0xFFu) << 24);
-    }
-    transpose_step<4>(A);
-    transpose_step<2>(A);
-    transpose_step<1>(A);
+    planes8(w, A);
 #pragma unroll
     for (int b = 0; b < 8; b++) dst_words[(size_t)(7 - b) * npp] = A[b];
 }
@@ -116,19 +67,8 @@ __device__ __forceinline__ void dec_group_u8(const uint32_t* src_words, uint8_t*
     uint32_t A[8];
 #pragma unroll
     for (int b = 0; b < 8; b++) A[b] = src_words[(size_t)(7 - b) * npp];
-    transpose_step<4>(A);
-    transpose_step<2>(A);
-    transpose_step<1>(A);
-    uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int sh = 8 * (k & 3);
-        const int wi = k >> 2;
-        w[wi] |= (A[k] & 0xFFu) << sh;
-        w[wi + 2] |= ((A[k] >> 8) & 0xFFu) << sh;
-        w[wi + 4] |= ((A[k] >> 16) & 0xFFu) << sh;
-        w[wi + 6] |= ((A[k] >> 24) & 0xFFu) << sh;
-    }
+    uint32_t w[8];
+    unplanes8(A, w);
     st16(dst, make_uint4(w[0], w[1], w[2], w[3]));
     st16(dst + 16, make_uint4(w[4], w[5], w[6], w[7]));
 }

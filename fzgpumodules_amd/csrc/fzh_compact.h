@@ -13,7 +13,8 @@
 namespace fzh {
 namespace {
 
-// Up to two lists (e.g. outliers and patches) per launch.
+// Up to two lists (e.g. outliers and patches) per launch; a list with a null
+// s_val moves indices only.
 struct CompactList {
     const float* s_val;
     const uint32_t* s_idx;
@@ -96,7 +97,7 @@ compact_kernel(CompactList L0, CompactList L1, uint32_t T, uint32_t E, ScanCtx* 
                 for (uint32_t j = l; j < c; j += 64) {
                     const uint64_t pos = dst0 + j;
                     if (pos < L.cap) {
-                        L.val[pos] = L.s_val[src0 + j];
+                        if (L.s_val) L.val[pos] = L.s_val[src0 + j];
                         L.idx[pos] = L.s_idx[src0 + j];
                     }
                 }

@@ -237,6 +237,13 @@ int fzp_enable_graph_mode(fzp_pipeline* p, int on) {
     });
 }
 
+int fzp_set_stage_fusion(fzp_pipeline* p, int on) {
+    return guard([&] {
+        need(p);
+        p->p->setStageFusion(on != 0);
+    });
+}
+
 int fzp_enable_profiling(fzp_pipeline* p, int on) {
     return guard([&] {
         need(p);

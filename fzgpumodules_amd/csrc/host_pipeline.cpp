@@ -205,6 +205,33 @@ void Pipeline::finalize() {
     if (warmup_on_finalize_ && input_hint_ > 0) warmup(nullptr);
 }
 
+// Fold each BitshuffleStage that feeds an RZEStage with matching 16 KiB
+// blocks/chunks into the RZE kernels.  Returns true if anything changed.
+bool Pipeline::updateFusion() {
+    bool changed = false;
+    for (auto& n : nodes_) {
+        auto* rze = dynamic_cast<RZEStage*>(n.stage);
+        if (!rze) continue;
+        int from = -1, ew = 0;
+        if (fusion_ && n.in_buf > 0) {
+            const int k = buffers_[n.in_buf].producer;
+            auto* bs = k >= 0 ? dynamic_cast<BitshuffleStage*>(nodes_[k].stage) : nullptr;
+            if (bs && bs->getBlockSize() == 16384 && rze->getChunkSize() == 16384 &&
+                (bs->getElementWidth() == 1 || bs->getElementWidth() == 2)) {
+                from = k;
+                ew = (int)bs->getElementWidth();
+            }
+        }
+        if (n.fused_from != from || rze->getPreShuffle() != ew) changed = true;
+        if (n.fused_from >= 0) nodes_[n.fused_from].elided = false;
+        n.fused_from = from;
+        rze->setPreShuffle(ew);
+        if (from >= 0) nodes_[from].elided = true;
+    }
+    if (changed) destroyGraph();
+    return changed;
+}
+
 void Pipeline::ensureBuffers(size_t padded) {
     if (buffers_for_ == padded) return;
     destroyGraph();
@@ -266,7 +293,12 @@ void Pipeline::launchForward(const void* d_in, size_t padded, bool zero_view, hi
     buffers_[0].size = padded;
     stage_events_.clear();
     for (auto& n : nodes_) {
-        const Buffer& in = buffers_[n.in_buf];
+        if (n.elided) {  // runs inside the consuming RZE node
+            buffers_[n.out_buf[0]].size = buffers_[n.in_buf].size;
+            stage_events_.push_back({nullptr, nullptr});
+            continue;
+        }
+        const Buffer& in = buffers_[n.fused_from >= 0 ? nodes_[n.fused_from].in_buf : n.in_buf];
         std::vector<void*> outs;
         for (int b : n.out_buf) outs.push_back(buffers_[b].ptr);
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -276,10 +308,8 @@ void Pipeline::launchForward(const void* d_in, size_t padded, bool zero_view, hi
             hipck(hipEventRecord(e0, stream), "hipEventRecord");
         }
         n.stage->execute(stream, pool_.get(), {in.ptr}, outs, {in.size});
-        if (e1) {
-            hipck(hipEventRecord(e1, stream), "hipEventRecord");
-            stage_events_.push_back({e0, e1});
-        }
+        if (e1) hipck(hipEventRecord(e1, stream), "hipEventRecord");
+        stage_events_.push_back({e0, e1});
         for (size_t i = 0; i < n.out_buf.size(); i++) buffers_[n.out_buf[i]].size = n.stage->getActualOutputSize((int)i);
     }
     // device-side concat from device-resident sizes
@@ -328,8 +358,10 @@ void Pipeline::finishCompress(hipStream_t stream, void** d_output, size_t* outpu
     hipck(hipStreamSynchronize(stream), "hipStreamSynchronize");
     collectReadbacks();
     for (auto& n : nodes_) n.stage->postStreamSync(stream);
-    for (auto& n : nodes_)
+    for (auto& n : nodes_) {
+        if (n.elided) continue;
         for (size_t i = 0; i < n.out_buf.size(); i++) buffers_[n.out_buf[i]].size = n.stage->getActualOutputSize((int)i);
+    }
     if (outputs_.size() == 1) {
         const Buffer& B = buffers_[outputs_[0]];
         *d_output = B.ptr;
@@ -359,6 +391,7 @@ void Pipeline::compress(const void* d_input, size_t input_size, void** d_output,
     Timer wall;
     const size_t padded = round_up(input_size, input_alignment_);
     ensureBuffers(padded);
+    updateFusion();
     input_size_ = padded;
     orig_input_size_ = input_size;
     Stage* src = nodes_[0].stage;
@@ -433,9 +466,10 @@ void Pipeline::compress(const void* d_input, size_t input_size, void** d_output,
         (void)hipEventDestroy(g1);
         for (size_t k = 0; k < stage_events_.size() && k < nodes_.size(); k++) {
             StageTimingResult st;
-            st.name = nodes_[k].stage->getName();
+            st.name = nodes_[k].stage->getName() + (nodes_[k].elided ? " (fused into RZE)" : "");
             st.level = nodes_[k].level;
-            (void)hipEventElapsedTime(&st.elapsed_ms, stage_events_[k].first, stage_events_[k].second);
+            if (stage_events_[k].first)
+                (void)hipEventElapsedTime(&st.elapsed_ms, stage_events_[k].first, stage_events_[k].second);
             st.input_bytes = buffers_[nodes_[k].in_buf].size;
             for (int b : nodes_[k].out_buf) st.output_bytes += buffers_[b].size;
             r.stages.push_back(st);
@@ -446,8 +480,8 @@ void Pipeline::compress(const void* d_input, size_t input_size, void** d_output,
         last_perf_ = r;
     }
     for (auto& e : stage_events_) {
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
+        if (e.first) (void)hipEventDestroy(e.first);
+        if (e.second) (void)hipEventDestroy(e.second);
     }
     stage_events_.clear();
 }
@@ -528,6 +562,7 @@ void Pipeline::runInverse(const std::vector<std::pair<const void*, size_t>>& slo
         inv_buf_.resize(buffers_.size(), nullptr);
         inv_cap_.resize(buffers_.size(), 0);
     }
+    updateFusion();
     std::vector<const void*> content(buffers_.size(), nullptr);
     std::vector<size_t> csize(buffers_.size(), 0);
     for (size_t i = 0; i < outputs_.size(); i++) {
@@ -543,6 +578,10 @@ void Pipeline::runInverse(const std::vector<std::pair<const void*, size_t>>& slo
     try {
         for (size_t k = nodes_.size(); k-- > 0;) {
             Node& n = nodes_[k];
+            if (n.elided) continue;  // its inverse ran inside the RZE decoder
+            // a fused RZE node reconstructs the Bitshuffle node's input
+            const int tgt_node = n.fused_from >= 0 ? n.fused_from : (int)k;
+            const int tgt_buf = nodes_[tgt_node].in_buf;
             std::vector<void*> ins;
             std::vector<size_t> sizes;
             for (int b : n.out_buf) {
@@ -551,10 +590,10 @@ void Pipeline::runInverse(const std::vector<std::pair<const void*, size_t>>& slo
             }
             const size_t need = n.stage->estimateOutputSizes(sizes).at(0);
             void* out;
-            if (k == 0 && d_final && final_cap >= need) {
+            if (tgt_node == 0 && d_final && final_cap >= need) {
                 out = d_final;
             } else {
-                const int b = n.in_buf;
+                const int b = tgt_buf;
                 const size_t cap = round_up(std::max<size_t>(need, 16), 256);
                 if (inv_cap_[b] < cap) {
                     if (inv_buf_[b]) pool_->freePersistentDevice(inv_buf_[b]);
@@ -565,11 +604,11 @@ void Pipeline::runInverse(const std::vector<std::pair<const void*, size_t>>& slo
                 out = inv_buf_[b];
             }
             n.stage->execute(stream, pool_.get(), ins, {out}, sizes);
-            content[n.in_buf] = out;
-            csize[n.in_buf] = n.stage->getActualOutputSize(0);
-            if (k == 0) {
+            content[tgt_buf] = out;
+            csize[tgt_buf] = n.stage->getActualOutputSize(0);
+            if (tgt_node == 0) {
                 result = out;
-                rsize = csize[n.in_buf];
+                rsize = csize[tgt_buf];
             }
         }
         std::vector<fzh_readback_entry> rb;

@@ -326,9 +326,14 @@ void RZEStage::execute(hipStream_t stream, MemoryPool* pool, const std::vector<v
         if (in_bytes == 0) { actual_output_size_ = 0; return; }
         cached_orig_bytes_ = (uint32_t)in_bytes;
         ws_.ensure(fzh_rze_enc_ws_bytes(in_bytes), pool, stream);
-        throwIfFzh(fzh_rze_enc(stream, inputs[0], in_bytes, outputs[0], fzh_rze_enc_bound(in_bytes), (uint64_t*)small_.ptr,
-                               ws_.ptr, ws_.bytes),
-                   "RZEStage");
+        if (pre_shuffle_)
+            throwIfFzh(fzh_rze_enc_shuffled(stream, inputs[0], in_bytes, pre_shuffle_, outputs[0],
+                                            fzh_rze_enc_bound(in_bytes), (uint64_t*)small_.ptr, ws_.ptr, ws_.bytes),
+                       "RZEStage (fused Bitshuffle)");
+        else
+            throwIfFzh(fzh_rze_enc(stream, inputs[0], in_bytes, outputs[0], fzh_rze_enc_bound(in_bytes),
+                                   (uint64_t*)small_.ptr, ws_.ptr, ws_.bytes),
+                       "RZEStage");
         actual_output_size_ = fzh_rze_enc_bound(in_bytes);
         size_pending_ = true;
         addReadback(small_.ptr, &h_size_, 8);
@@ -336,9 +341,14 @@ void RZEStage::execute(hipStream_t stream, MemoryPool* pool, const std::vector<v
         const size_t orig = cached_orig_bytes_;
         if (orig == 0) { actual_output_size_ = 0; return; }
         inv_ws_.ensure(fzh_rze_dec_ws_bytes(orig), pool, stream);
-        throwIfFzh(fzh_rze_dec(stream, inputs[0], sizes[0], outputs[0], orig, (int*)((uint8_t*)small_.ptr + 8),
-                               inv_ws_.ptr, inv_ws_.bytes),
-                   "RZEStage (inverse)");
+        if (pre_shuffle_)
+            throwIfFzh(fzh_rze_dec_shuffled(stream, inputs[0], sizes[0], outputs[0], orig, pre_shuffle_,
+                                            (int*)((uint8_t*)small_.ptr + 8), inv_ws_.ptr, inv_ws_.bytes),
+                       "RZEStage (inverse, fused Bitshuffle)");
+        else
+            throwIfFzh(fzh_rze_dec(stream, inputs[0], sizes[0], outputs[0], orig, (int*)((uint8_t*)small_.ptr + 8),
+                                   inv_ws_.ptr, inv_ws_.bytes),
+                       "RZEStage (inverse)");
         actual_output_size_ = orig;
         status_pending_ = true;
         addReadback((uint8_t*)small_.ptr + 8, &h_status_, 4);

@@ -14,14 +14,15 @@
 //     merged in-lane before the LDS atomic (the dominant code otherwise
 //     serialises a 64-lane conflict).
 //   encode: one wave per partition packs its codewords in parallel (bit
-//     offsets from a wave prefix sum, LDS OR into cells), takes its cell
-//     offset from the ordered tile scan and writes the cells, par_nbit and
-//     par_entry straight into the final stream: no gapped scratch, no host
-//     prefix sum, no host sync (reference phases 1-4 + two syncs).
+//     offsets from a wave prefix sum, LDS OR into cells) and writes them to
+//     the partition's scratch slot plus par_nbit; a segmented device scan of
+//     the cell counts writes par_entry into the stream; a pack kernel moves
+//     the cells and a one-workgroup kernel writes the header: no host prefix
+//     sum, no host sync (reference phases 1-4 + two syncs).
 //   decode: one lane per partition with a 1024-entry prefix lookup table in
 //     LDS (codes up to 10 bits resolve in one probe), canonical bit loop only
 //     for longer codes; outputs leave in 16-byte stores.
-#include "fzh_internal.h"
+#include "fzh_scan.h"
 
 namespace fzh {
 namespace {
@@ -89,20 +90,14 @@ hist_global_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, uint32_
 // ---------------------------------------------------------------------------
 // Encode
 // ---------------------------------------------------------------------------
-struct EncState {
-    unsigned long long nbit_total;
-    uint32_t finished;
-    uint32_t pad;
-};
+constexpr int CELLS = (SUBLEN * 28 + 31) / 32 + 2;  // cells per partition slot
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint32_t* __restrict__ bk4,
                const uint8_t* __restrict__ revbook, uint32_t rev_bytes, uint8_t* __restrict__ out,
-               uint64_t* __restrict__ out_size, EncState* est, ScanCtx* ctx, uint64_t* status, uint32_t pardeg,
-               uint32_t total_waves, int book_in_lds) {
+               uint32_t* __restrict__ scells, uint32_t* __restrict__ ncell_out, uint32_t pardeg, int book_in_lds) {
     extern __shared__ uint32_t smem[];
-    constexpr int CELLS = (SUBLEN * 28 + 31) / 32 + 2;
     const int wv = threadIdx.x >> 6;
     uint32_t* cells = smem + wv * CELLS;
     uint32_t* s_bk = smem + 4 * CELLS;
@@ -111,116 +106,104 @@ huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint3
         __syncthreads();
     }
     const uint32_t* book = book_in_lds ? s_bk : bk4;
-    const uint32_t ent1 = HDR_BYTES, ent2 = ent1 + rev_bytes, ent3 = ent2 + 4 * pardeg, ent4 = ent3 + 4 * pardeg;
-
-    const TileTicket tk = take_ticket(ctx);
+    const uint32_t ent1 = HDR_BYTES, ent2 = ent1 + rev_bytes;
+    const uint32_t p = blockIdx.x * 4 + wv;
     const int l = lane_id();
-    if (tk.tile < pardeg) {
-        const uint32_t p = tk.tile;
-        const uint64_t s0 = (uint64_t)p * SUBLEN;
-        const uint32_t cnt = (uint32_t)((n - s0) < (uint64_t)SUBLEN ? (n - s0) : SUBLEN);
-        for (int i = l; i < CELLS; i += 64) cells[i] = 0u;
-        // symbol s0 + l + 64k, bit offsets in (k, lane) order = symbol order
-        uint32_t code[12], len[12];
+    if (p >= pardeg) return;
+    const uint64_t s0 = (uint64_t)p * SUBLEN;
+    const uint32_t cnt = (uint32_t)((n - s0) < (uint64_t)SUBLEN ? (n - s0) : SUBLEN);
+    for (int i = l; i < CELLS; i += 64) cells[i] = 0u;
+    // symbol s0 + l + 64k, bit offsets in (k, lane) order = symbol order
+    uint32_t code[12], len[12];
 #pragma unroll
-        for (int k = 0; k < 12; k++) {
-            const uint32_t j = l + 64 * k;
-            uint32_t w = 0;
-            if (j < cnt) w = book[ld_sym(in, s0 + j)];
-            len[k] = j < cnt ? (w >> 27) : 0u;
-            code[k] = w & 0x07FFFFFFu;
-        }
-        wave_lds_sync();
-        uint32_t run = 0;
-#pragma unroll
-        for (int k = 0; k < 12; k++) {
-            const uint32_t incl = wave_incl_scan(len[k]);
-            const uint32_t b = run + incl - len[k];
-            const uint32_t L = len[k];
-            if (L) {
-                const uint32_t c = b >> 5, sh = b & 31u;
-                if (sh + L <= 32) {
-                    atomicOr(&cells[c], code[k] << (32 - sh - L));
-                } else {
-                    atomicOr(&cells[c], code[k] >> (sh + L - 32));
-                    atomicOr(&cells[c + 1], code[k] << (64 - sh - L));
-                }
-            }
-            run += __shfl(incl, 63, 64);
-        }
-        const uint32_t nbit = run;
-        const uint32_t ncell = (nbit + 31) / 32;
-        wave_lds_sync();
-        const uint32_t e = lookback_exclusive(status, p, tk.epoch, ncell);
-        if (e != kLookbackFailed) {
-            uint32_t* dst = reinterpret_cast<uint32_t*>(out + ent4) + e;
-            for (uint32_t i = l; i < ncell; i += 64) dst[i] = cells[i];
-            if (l == 0) {
-                reinterpret_cast<uint32_t*>(out + ent2)[p] = nbit;
-                reinterpret_cast<uint32_t*>(out + ent3)[p] = e;
-            }
-        }
-        if (p == 0) {
-            for (uint32_t i = l; i < rev_bytes; i += 64) out[ent1 + i] = revbook[i];
-            for (uint32_t i = 64 + l; i < HDR_BYTES; i += 64) out[i] = 0;
-        }
-        if (l == 0) {
-            atomicAdd(&est->nbit_total, (unsigned long long)nbit);
-            __threadfence();
-            const uint32_t done = atomicAdd(&est->finished, 1u);
-            if (p == pardeg - 1) {
-                // the last partition knows the total cell count; publish it
-                // through the state so whichever wave finishes last can write
-                // the header
-                __hip_atomic_store(&est->pad, (e == kLookbackFailed) ? 0xFFFFFFFFu : e + ncell,
-                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            if (done == pardeg - 1) {
-                __threadfence();
-                uint32_t total_ncell;
-                uint32_t spins = 0;
-                do {
-                    total_ncell = __hip_atomic_load(&est->pad, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                    if (total_ncell != 0xFFFFFFFEu) break;
-                    __builtin_amdgcn_s_sleep(1);
-                } while (++spins < kSpinLimit);
-                if (total_ncell == 0xFFFFFFFEu) total_ncell = 0xFFFFFFFFu;
-                const unsigned long long tnb = __hip_atomic_load(&est->nbit_total, __ATOMIC_RELAXED,
-                                                                 __HIP_MEMORY_SCOPE_AGENT);
-                uint32_t* h32 = reinterpret_cast<uint32_t*>(out);
-                if (total_ncell == 0xFFFFFFFFu) {
-                    *out_size = 0;
-                } else {
-                    const uint32_t ent5 = ent4 + 4 * total_ncell;
-                    h32[0] = bklen & 0xFFFFu;
-                    h32[1] = SUBLEN;
-                    h32[2] = pardeg;
-                    h32[3] = 0;
-                    reinterpret_cast<uint64_t*>(out)[2] = n;
-                    reinterpret_cast<uint64_t*>(out)[3] = tnb;
-                    reinterpret_cast<uint64_t*>(out)[4] = total_ncell;
-                    h32[10] = 0;
-                    h32[11] = ent1;
-                    h32[12] = ent2;
-                    h32[13] = ent3;
-                    h32[14] = ent4;
-                    h32[15] = ent5;
-                    *out_size = ent5;
-                }
-                est->nbit_total = 0;
-                est->finished = 0;
-                __hip_atomic_store(&est->pad, 0xFFFFFFFEu, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
+    for (int k = 0; k < 12; k++) {
+        const uint32_t j = l + 64 * k;
+        uint32_t w = 0;
+        if (j < cnt) w = book[ld_sym(in, s0 + j)];
+        len[k] = j < cnt ? (w >> 27) : 0u;
+        code[k] = w & 0x07FFFFFFu;
     }
-    release_ticket(ctx, tk.epoch, total_waves);
+    wave_lds_sync();
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const uint32_t incl = wave_incl_scan(len[k]);
+        const uint32_t b = run + incl - len[k];
+        const uint32_t L = len[k];
+        if (L) {
+            const uint32_t c = b >> 5, sh = b & 31u;
+            if (sh + L <= 32) {
+                atomicOr(&cells[c], code[k] << (32 - sh - L));
+            } else {
+                atomicOr(&cells[c], code[k] >> (sh + L - 32));
+                atomicOr(&cells[c + 1], code[k] << (64 - sh - L));
+            }
+        }
+        run += __shfl(incl, 63, 64);
+    }
+    const uint32_t nbit = run;
+    const uint32_t ncell = (nbit + 31) / 32;
+    wave_lds_sync();
+    uint32_t* dst = scells + (uint64_t)p * CELLS;
+    for (uint32_t i = l; i < ncell; i += 64) dst[i] = cells[i];
+    if (l == 0) {
+        reinterpret_cast<uint32_t*>(out + ent2)[p] = nbit;
+        ncell_out[p] = ncell;
+    }
+    if (p == 0) {
+        for (uint32_t i = l; i < rev_bytes; i += 64) out[ent1 + i] = revbook[i];
+        for (uint32_t i = 64 + l; i < HDR_BYTES; i += 64) out[i] = 0;
+    }
 }
 
-// Initialises the encoder state once (pad = "total not yet known").
-__global__ void huf_state_init_kernel(EncState* est) {
-    est->nbit_total = 0;
-    est->finished = 0;
-    est->pad = 0xFFFFFFFEu;
+// Cells of partition p -> stream cell par_entry[p] (one wave per partition).
+__global__ void __launch_bounds__(256)
+huf_pack_kernel(const uint32_t* __restrict__ scells, const uint32_t* __restrict__ ncell,
+                uint8_t* __restrict__ out, uint32_t ent3, uint32_t ent4, uint32_t pardeg) {
+    const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= pardeg) return;
+    const int l = lane_id();
+    const uint32_t e = reinterpret_cast<const uint32_t*>(out + ent3)[p];
+    const uint32_t c = ncell[p];
+    const uint32_t* src = scells + (uint64_t)p * CELLS;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(out + ent4) + e;
+    for (uint32_t i = l; i < c; i += 64) dst[i] = src[i];
+}
+
+// phf_header (hf.h:38-44) once every partition is placed.
+__global__ void __launch_bounds__(1024)
+huf_header_kernel(uint8_t* __restrict__ out, uint64_t n, uint32_t bklen, uint32_t ent1, uint32_t ent2,
+                  uint32_t ent3, uint32_t ent4, uint32_t pardeg, const uint32_t* __restrict__ total_ncell,
+                  uint64_t* __restrict__ out_size) {
+    __shared__ unsigned long long s_w[16];
+    const uint32_t* nb = reinterpret_cast<const uint32_t*>(out + ent2);
+    unsigned long long t = 0;
+    for (uint32_t p = threadIdx.x; p < pardeg; p += 1024) t += nb[p];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if (lane_id() == 0) s_w[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tnb = 0;
+        for (int i = 0; i < 16; i++) tnb += s_w[i];
+        const uint32_t tc = *total_ncell;
+        const uint32_t ent5 = ent4 + 4 * tc;
+        uint32_t* h32 = reinterpret_cast<uint32_t*>(out);
+        h32[0] = bklen & 0xFFFFu;
+        h32[1] = SUBLEN;
+        h32[2] = pardeg;
+        h32[3] = 0;
+        reinterpret_cast<uint64_t*>(out)[2] = n;
+        reinterpret_cast<uint64_t*>(out)[3] = tnb;
+        reinterpret_cast<uint64_t*>(out)[4] = tc;
+        h32[10] = 0;
+        h32[11] = ent1;
+        h32[12] = ent2;
+        h32[13] = ent3;
+        h32[14] = ent4;
+        h32[15] = ent5;
+        *out_size = ent5;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -353,14 +336,16 @@ huf_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, T* __restrict_
 }
 
 struct EncWs {
-    size_t o_state, o_ctx, total;
+    size_t pardeg, o_ncell, o_tot, o_scan, o_cells, total;
 };
 EncWs enc_layout(size_t n) {
     EncWs w;
-    const size_t pardeg = cdiv(std::max<size_t>(n, 1), SUBLEN);
+    w.pardeg = cdiv(std::max<size_t>(n, 1), SUBLEN);
     size_t o = 0;
-    w.o_state = o; o = align_up(o + sizeof(EncState), 256);
-    w.o_ctx = o;   o = align_up(o + scan_state_bytes(pardeg), 256);
+    w.o_ncell = o; o = align_up(o + 4 * w.pardeg, 256);
+    w.o_tot = o;   o = align_up(o + 16, 256);
+    w.o_scan = o;  o = align_up(o + scan_ws_bytes(w.pardeg), 256);
+    w.o_cells = o; o = align_up(o + 4 * (size_t)CELLS * w.pardeg, 256);
     w.total = o;
     return w;
 }
@@ -376,20 +361,23 @@ int enc_impl(hipStream_t s, const void* in, size_t n, uint32_t bklen, const uint
     const uint32_t rev_bytes = (uint32_t)fzh_huffman_revbook_bytes(bklen, sizeof(T));
     if (rev_bytes % 4) return fail(FZH_E_INVALID, "huffman enc: bklen * sizeof(T) must be a multiple of 4");
     if ((uintptr_t)out % 16) return fail(FZH_E_INVALID, "huffman enc: output must be 16-byte aligned");
-    const uint32_t pardeg = (uint32_t)cdiv(n, SUBLEN);
+    const uint32_t pardeg = (uint32_t)L.pardeg;
+    const uint32_t ent1 = HDR_BYTES, ent2 = ent1 + rev_bytes, ent3 = ent2 + 4 * pardeg, ent4 = ent3 + 4 * pardeg;
     uint8_t* w = (uint8_t*)ws;
-    EncState* est = (EncState*)(w + L.o_state);
-    ScanCtx* ctx = (ScanCtx*)(w + L.o_ctx);
-    uint64_t* status = (uint64_t*)(w + L.o_ctx + sizeof(ScanCtx));
-    // The state word `pad` must read "total unknown" (0xFFFFFFFE) before the
-    // first launch; a zeroed workspace is turned into that state here, once.
-    constexpr int CELLS = (SUBLEN * 28 + 31) / 32 + 2;
+    uint32_t* ncell = (uint32_t*)(w + L.o_ncell);
+    uint32_t* tot = (uint32_t*)(w + L.o_tot);
+    uint32_t* scells = (uint32_t*)(w + L.o_cells);
+    uint8_t* o8 = (uint8_t*)out;
     const int book_in_lds = bklen <= 8192 ? 1 : 0;
     const size_t lds = (size_t)4 * CELLS * 4 + (book_in_lds ? 4 * (size_t)bklen : 0);
     const unsigned blocks = (unsigned)cdiv(pardeg, 4);
-    huf_enc_kernel<T><<<blocks, 256, lds, s>>>((const T*)in, n, bklen, bk4, rev, rev_bytes, (uint8_t*)out, out_size,
-                                               est, ctx, status, pardeg, blocks * 4, book_in_lds);
-    FZH_CHECK_LAUNCH("huf_enc_kernel");
+    huf_enc_kernel<T><<<blocks, 256, lds, s>>>((const T*)in, n, bklen, bk4, rev, rev_bytes, o8, scells, ncell, pardeg,
+                                               book_in_lds);
+    if (int rc = scan_u32(s, ncell, reinterpret_cast<uint32_t*>(o8 + ent3), pardeg, nullptr, tot, w + L.o_scan))
+        return rc;
+    huf_pack_kernel<<<blocks, 256, 0, s>>>(scells, ncell, o8, ent3, ent4, pardeg);
+    huf_header_kernel<<<1, 1024, 0, s>>>(o8, n, bklen, ent1, ent2, ent3, ent4, pardeg, tot, out_size);
+    FZH_CHECK_LAUNCH("huffman encode");
     return FZH_OK;
 }
 
@@ -435,10 +423,7 @@ size_t fzh_huffman_enc_ws_bytes(size_t n) { return enc_layout(n).total; }
 int fzh_huffman_encode(void* stream, const void* d_in, size_t n, int elem_bytes, uint32_t bklen,
                        const uint32_t* d_bk4, const uint8_t* d_revbook, void* d_out, size_t out_cap,
                        uint64_t* d_out_size, void* d_ws, size_t ws_bytes) {
-    // The encoder state must start in the "total unknown" state; workspaces
-    // arrive zeroed, so initialise on every call (a one-thread kernel).
     if (ws_bytes < fzh_huffman_enc_ws_bytes(n)) return fail(FZH_E_CAPACITY, "huffman enc: workspace too small");
-    huf_state_init_kernel<<<1, 1, 0, S(stream)>>>((EncState*)d_ws);
     switch (elem_bytes) {
     case 1: return enc_impl<uint8_t>(S(stream), d_in, n, bklen, d_bk4, d_revbook, d_out, out_cap, d_out_size, d_ws, ws_bytes);
     case 2: return enc_impl<uint16_t>(S(stream), d_in, n, bklen, d_bk4, d_revbook, d_out, out_cap, d_out_size, d_ws, ws_bytes);

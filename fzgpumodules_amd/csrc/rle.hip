@@ -6,18 +6,21 @@
 // x[i] != x[i-1] (rle.cu:45-55).
 //
 // MI355X design
-//   encode: K1 — one wave per 8 x 64 x (16/sizeof T) elements; boundary
-//     flags from 16-byte loads, run offsets from the ordered tile scan; it
-//     writes run values straight into the output and run start positions into
-//     scratch.  K2 — run lengths = next start - start.  (The reference does
-//     5 passes plus a CUB scan through ~15 B/element of scratch.)
-//   decode: K1 — ordered scan of the run lengths into run starts, plus a map
-//     "output tile -> first run overlapping it" (every run writes the tile
-//     boundaries it covers).  K2 — one wave per 4096-element output tile
-//     stages that tile's runs in LDS; each lane binary-searches its first
-//     run and fills 64 consecutive outputs.  No host round trip (the
-//     reference syncs twice and fills with one thread per run).
-#include "fzh_internal.h"
+//   encode: K1 — one wave per 8 x 64 x (16/sizeof T) elements finds the run
+//     starts (boundary flags from 16-byte loads) and writes their positions to
+//     the tile's scratch slot; K2 — the shared compaction kernel packs the
+//     positions in order (R = their number); K3 — values = input at each
+//     start, lengths = next start - start, header and padding.  (The
+//     reference does 5 passes plus a CUB scan through ~15 B/element of
+//     scratch.)
+//   decode: a three-launch segmented scan of the run lengths gives the run
+//     starts and a map "output tile -> first run overlapping it" (every run
+//     marks the tile boundaries it covers); then one wave per 4096-element
+//     output tile stages that tile's runs in LDS, each lane binary-searches
+//     its first run and fills 64 consecutive outputs.  No host round trip
+//     (the reference syncs twice and fills with one thread per run).
+#include "fzh_compact.h"
+#include "fzh_scan.h"
 
 namespace fzh {
 namespace {
@@ -27,79 +30,66 @@ constexpr uint32_t DTILE = 4096;  // decode output tile (elements)
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-rle_enc_runs_kernel(const T* __restrict__ in, uint64_t n, uint8_t* __restrict__ out, uint32_t* __restrict__ pos,
-                    uint32_t* __restrict__ d_R, ScanCtx* ctx, uint64_t* status, uint32_t ntiles,
-                    uint32_t total_waves) {
+rle_enc_runs_kernel(const T* __restrict__ in, uint64_t n, uint32_t* __restrict__ spos, uint32_t* __restrict__ scnt,
+                    uint32_t ntiles) {
     constexpr int E = 16 / sizeof(T);       // elements per 16-byte load
     constexpr uint32_t TILE = 64 * E * STEPS;
-    const TileTicket tk = take_ticket(ctx);
-    if (tk.tile < ntiles) {
-        const int l = lane_id();
-        const uint64_t base = (uint64_t)tk.tile * TILE;
-        T v[STEPS][E];
+    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= ntiles) return;
+    const int l = lane_id();
+    const uint64_t base = (uint64_t)tile * TILE;
+    T v[STEPS][E];
 #pragma unroll
-        for (int j = 0; j < STEPS; j++) {
-            const uint64_t e0 = base + (uint64_t)j * 64 * E + (uint64_t)l * E;
-            if (e0 + E <= n) {
-                const uint4 w = *reinterpret_cast<const uint4*>(in + e0);
-                const T* p = reinterpret_cast<const T*>(&w);
+    for (int j = 0; j < STEPS; j++) {
+        const uint64_t e0 = base + (uint64_t)j * 64 * E + (uint64_t)l * E;
+        if (e0 + E <= n) {
+            const uint4 w = *reinterpret_cast<const uint4*>(in + e0);
+            const T* p = reinterpret_cast<const T*>(&w);
 #pragma unroll
-                for (int k = 0; k < E; k++) v[j][k] = p[k];
-            } else {
+            for (int k = 0; k < E; k++) v[j][k] = p[k];
+        } else {
 #pragma unroll
-                for (int k = 0; k < E; k++) v[j][k] = (e0 + k < n) ? in[e0 + k] : (T)0;
-            }
+            for (int k = 0; k < E; k++) v[j][k] = (e0 + k < n) ? in[e0 + k] : (T)0;
         }
-        const T before = (base > 0) ? in[base - 1] : (T)0;
-        uint32_t mask[STEPS];
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int j = 0; j < STEPS; j++) {
-            T prev = __shfl_up(v[j][E - 1], 1, 64);
-            const T prevj = __shfl(v[j > 0 ? j - 1 : 0][E - 1], 63, 64);
-            if (l == 0) prev = (j == 0) ? before : prevj;
-            uint32_t m = 0;
-            const uint64_t e0 = base + (uint64_t)j * 64 * E + (uint64_t)l * E;
-#pragma unroll
-            for (int k = 0; k < E; k++) {
-                const T p = (k == 0) ? prev : v[j][k - 1];
-                const uint64_t e = e0 + k;
-                const bool b = e < n && (e == 0 || v[j][k] != p);
-                m |= (b ? 1u : 0u) << k;
-            }
-            mask[j] = m;
-            cnt += __popc(m);
-        }
-        const uint32_t agg = wave_sum(cnt);
-        const uint32_t roff = lookback_exclusive(status, tk.tile, tk.epoch, agg);
-        if (roff != kLookbackFailed) {
-            T* vals = reinterpret_cast<T*>(out + 4);
-            uint32_t run = roff;
-#pragma unroll
-            for (int j = 0; j < STEPS; j++) {
-                const uint32_t c = __popc(mask[j]);
-                const uint32_t incl = wave_incl_scan(c);
-                uint32_t r = run + incl - c;
-                const uint64_t e0 = base + (uint64_t)j * 64 * E + (uint64_t)l * E;
-#pragma unroll
-                for (int k = 0; k < E; k++)
-                    if (mask[j] & (1u << k)) {
-                        vals[r] = v[j][k];
-                        pos[r] = (uint32_t)(e0 + k);
-                        r++;
-                    }
-                run += __shfl(incl, 63, 64);
-            }
-        }
-        if (tk.tile == ntiles - 1 && l == 0) *d_R = (roff == kLookbackFailed) ? 0xFFFFFFFFu : roff + agg;
     }
-    release_ticket(ctx, tk.epoch, total_waves);
+    const T before = (base > 0) ? in[base - 1] : (T)0;
+    uint32_t mask[STEPS];
+#pragma unroll
+    for (int j = 0; j < STEPS; j++) {
+        T prev = __shfl_up(v[j][E - 1], 1, 64);
+        const T prevj = __shfl(v[j > 0 ? j - 1 : 0][E - 1], 63, 64);
+        if (l == 0) prev = (j == 0) ? before : prevj;
+        uint32_t m = 0;
+        const uint64_t e0 = base + (uint64_t)j * 64 * E + (uint64_t)l * E;
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            const T p = (k == 0) ? prev : v[j][k - 1];
+            const uint64_t e = e0 + k;
+            const bool b = e < n && (e == 0 || v[j][k] != p);
+            m |= (b ? 1u : 0u) << k;
+        }
+        mask[j] = m;
+    }
+    uint32_t run = 0;
+    uint32_t* dst = spos + base;
+#pragma unroll
+    for (int j = 0; j < STEPS; j++) {
+        const uint32_t c = __popc(mask[j]);
+        const uint32_t incl = wave_incl_scan(c);
+        uint32_t r = run + incl - c;
+        const uint64_t e0 = base + (uint64_t)j * 64 * E + (uint64_t)l * E;
+#pragma unroll
+        for (int k = 0; k < E; k++)
+            if (mask[j] & (1u << k)) dst[r++] = (uint32_t)(e0 + k);
+        run += __shfl(incl, 63, 64);
+    }
+    if (l == 0) scnt[tile] = run;
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-rle_enc_lengths_kernel(uint8_t* __restrict__ out, const uint32_t* __restrict__ pos, const uint32_t* __restrict__ d_R,
-                       uint64_t n, uint64_t* __restrict__ out_size) {
+rle_enc_lengths_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const uint32_t* __restrict__ pos,
+                       const uint32_t* __restrict__ d_R, uint64_t n, uint64_t* __restrict__ out_size) {
     const uint32_t R = *d_R;
     if (R == 0xFFFFFFFFu) {
         if (blockIdx.x == 0 && threadIdx.x == 0) *out_size = 0;
@@ -107,10 +97,12 @@ rle_enc_lengths_kernel(uint8_t* __restrict__ out, const uint32_t* __restrict__ p
     }
     const uint64_t vb = (uint64_t)R * sizeof(T);
     const uint64_t va = (vb + 3) & ~3ull;
+    T* vals = reinterpret_cast<T*>(out + 4);
     uint32_t* lens = reinterpret_cast<uint32_t*>(out + 4 + va);
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R; r += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t s = pos[r];
         const uint32_t e = (r + 1 < R) ? pos[r + 1] : (uint32_t)n;
+        vals[r] = in[s];
         lens[r] = e - s;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -135,62 +127,89 @@ struct RleDecState {
     uint32_t total;
 };
 
-// Scan of run lengths -> start[r] (exclusive), start[R] = total.  One wave per
-// 1024 runs; waves past R only take/release tickets.  Each run also records
-// itself as the first run of every output tile whose first element it covers.
+// Validated run count (0 and status = 1 if the header is inconsistent).
+template <typename T>
+__global__ void rle_dec_prep_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint64_t n,
+                                    uint32_t* __restrict__ Rv, int* __restrict__ d_status) {
+    if (threadIdx.x != 0) return;
+    uint32_t R = 0;
+    bool ok = in_bytes >= 4;
+    if (ok) R = *reinterpret_cast<const uint32_t*>(in);
+    const uint64_t va = ((uint64_t)R * sizeof(T) + 3) & ~3ull;
+    if (ok) ok = R <= n && 4 + va + 4ull * R <= in_bytes && !(R == 0 && n != 0);
+    *Rv = ok ? R : 0u;
+    if (!ok) *d_status = 1;
+}
+
+__device__ __forceinline__ const uint32_t* rle_lengths(const uint8_t* in, uint32_t R, int eb) {
+    return reinterpret_cast<const uint32_t*>(in + 4 + (((uint64_t)R * eb + 3) & ~3ull));
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
-rle_dec_scan_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint64_t n, uint32_t* __restrict__ start,
-                    uint32_t* __restrict__ tile_first, int* __restrict__ d_status, ScanCtx* ctx, uint64_t* status,
-                    uint32_t nseg_max, uint32_t total_waves) {
-    const TileTicket tk = take_ticket(ctx);
-    const int l = lane_id();
-    uint32_t R = 0;
-    bool hdr_ok = in_bytes >= 4;
-    if (hdr_ok) R = *reinterpret_cast<const uint32_t*>(in);
-    const uint64_t va = ((uint64_t)R * sizeof(T) + 3) & ~3ull;
-    if (hdr_ok) hdr_ok = R <= n && 4 + va + 4ull * R <= in_bytes;
-    const uint32_t nseg = hdr_ok ? (uint32_t)((R + 1023) / 1024) : 0u;
-    if (tk.tile == 0 && l == 0 && !hdr_ok) *d_status = 1;
-    if (tk.tile < nseg && tk.tile < nseg_max) {
-        const uint32_t* lens = reinterpret_cast<const uint32_t*>(in + 4 + va);
-        const uint32_t b = tk.tile * 1024 + 16 * l;
-        uint32_t v[16];
-        uint64_t sum = 0;
+rle_dec_reduce_kernel(const uint8_t* __restrict__ in, const uint32_t* __restrict__ Rv,
+                      uint32_t* __restrict__ seg_sum) {
+    __shared__ uint64_t s_w[4];
+    const uint32_t R = *Rv;
+    const uint32_t* lens = rle_lengths(in, R, sizeof(T));
+    const uint64_t b = (uint64_t)blockIdx.x * SCAN_SEG;
+    uint64_t sm = 0;
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            v[i] = (b + i < R) ? lens[b + i] : 0u;
-            sum += v[i];
-        }
-        const uint32_t s32 = sum > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)sum;
-        const uint32_t incl = wave_incl_scan(s32);
-        const uint32_t agg = __shfl(incl, 63, 64);
-        uint32_t pre = lookback_exclusive(status, tk.tile, tk.epoch, agg);
-        const bool ok = pre != kLookbackFailed;
-        if (!ok) pre = 0;
-        uint64_t run = (uint64_t)pre + incl - s32;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t r = b + i;
-            if (r < R) {
-                const uint64_t s = run, e = run + v[i];
-                start[r] = (uint32_t)(s < n ? s : n);
-                // tiles whose first element lies in [s, e)
-                if (s < n && v[i] > 0) {
-                    const uint64_t e2 = e < n ? e : n;
-                    for (uint64_t t = (s + DTILE - 1) / DTILE; t * DTILE < e2; t++) tile_first[t] = r;
-                }
-            }
-            run += v[i];
-        }
-        if (tk.tile == nseg - 1 && l == 0) {
-            const uint64_t total = (uint64_t)pre + agg;
-            start[R] = (uint32_t)(total < n ? total : n);
-            if (!ok || total != n) *d_status = 1;
-        }
+    for (int k = 0; k < 16; k++) {
+        const uint64_t i = b + threadIdx.x + 256ull * k;
+        if (i < R) sm += lens[i];
     }
-    if (tk.tile == 0 && l == 0 && hdr_ok && R == 0 && n != 0) *d_status = 1;
-    release_ticket(ctx, tk.epoch, total_waves);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sm += __shfl_xor(sm, off, 64);
+    if (lane_id() == 0) s_w[threadIdx.x >> 6] = sm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t t = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        seg_sum[blockIdx.x] = t > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
+    }
+}
+
+// start[r] = exclusive prefix of the lengths (clamped to n), start[R] = the
+// total; each run marks the output tiles whose first element it covers.
+template <typename T>
+__global__ void __launch_bounds__(256)
+rle_dec_apply_kernel(const uint8_t* __restrict__ in, uint64_t n, const uint32_t* __restrict__ Rv,
+                     const uint32_t* __restrict__ seg_base, const uint32_t* __restrict__ total,
+                     uint32_t* __restrict__ start, uint32_t* __restrict__ tile_first, int* __restrict__ d_status) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t R = *Rv;
+    const uint32_t* lens = rle_lengths(in, R, sizeof(T));
+    const int tid = threadIdx.x, w = tid >> 6;
+    const uint64_t b = (uint64_t)blockIdx.x * SCAN_SEG + 16ull * tid;
+    uint32_t v[16], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        v[k] = (b + k < R) ? lens[b + k] : 0u;
+        sum += v[k];
+    }
+    const uint32_t incl = wave_incl_scan(sum);
+    if (lane_id() == 63) s_w[w] = incl;
+    __syncthreads();
+    uint64_t run = (uint64_t)seg_base[blockIdx.x] + incl - sum;
+    for (int j = 0; j < w; j++) run += s_w[j];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint64_t r = b + k;
+        if (r < R) {
+            const uint64_t s0 = run, e = run + v[k];
+            start[r] = (uint32_t)(s0 < n ? s0 : n);
+            if (s0 < n && v[k] > 0) {
+                const uint64_t e2 = e < n ? e : n;
+                for (uint64_t t = (s0 + DTILE - 1) / DTILE; t * DTILE < e2; t++) tile_first[t] = (uint32_t)r;
+            }
+        }
+        run += v[k];
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        const uint32_t tot = *total;
+        start[R] = tot < n ? tot : (uint32_t)n;
+        if (R != 0 && tot != n) *d_status = 1;
+    }
 }
 
 template <typename T>
@@ -260,16 +279,19 @@ rle_dec_expand_kernel(const uint8_t* __restrict__ in, uint64_t n, const uint32_t
 }
 
 struct EncWs {
-    size_t o_ctx, o_pos, o_R, total;
+    size_t ntiles, o_cst, o_cnt, o_spos, o_pos, o_R, total;
 };
 template <typename T>
 EncWs enc_layout(size_t n) {
     constexpr size_t TILE = 64 * (16 / sizeof(T)) * STEPS;
     EncWs w;
+    w.ntiles = std::max<size_t>(cdiv(n, TILE), 1);
     size_t o = 0;
-    w.o_ctx = o; o = align_up(o + scan_state_bytes(cdiv(n, TILE)), 256);
-    w.o_pos = o; o = align_up(o + 4 * std::max<size_t>(n, 1), 256);
-    w.o_R = o;   o = align_up(o + 16, 256);
+    w.o_cst = o;  o = align_up(o + compact_state_bytes(w.ntiles), 256);
+    w.o_cnt = o;  o = align_up(o + 4 * w.ntiles, 256);
+    w.o_spos = o; o = align_up(o + 4 * w.ntiles * TILE, 256);
+    w.o_pos = o;  o = align_up(o + 4 * std::max<size_t>(n, 1), 256);
+    w.o_R = o;    o = align_up(o + 16, 256);
     w.total = o;
     return w;
 }
@@ -289,27 +311,30 @@ int enc_impl(hipStream_t s, const void* in, size_t n, void* out, size_t cap, uin
     constexpr size_t TILE = 64 * (16 / sizeof(T)) * STEPS;
     const uint32_t ntiles = (uint32_t)cdiv(n, TILE);
     uint8_t* w = (uint8_t*)ws;
-    ScanCtx* ctx = (ScanCtx*)(w + L.o_ctx);
-    uint64_t* status = (uint64_t*)(w + L.o_ctx + sizeof(ScanCtx));
+    uint32_t* scnt = (uint32_t*)(w + L.o_cnt);
+    uint32_t* spos = (uint32_t*)(w + L.o_spos);
     uint32_t* pos = (uint32_t*)(w + L.o_pos);
     uint32_t* dR = (uint32_t*)(w + L.o_R);
     const unsigned blocks = (unsigned)cdiv(ntiles, 4);
-    rle_enc_runs_kernel<T><<<blocks, 256, 0, s>>>((const T*)in, n, (uint8_t*)out, pos, dR, ctx, status, ntiles, blocks * 4);
+    rle_enc_runs_kernel<T><<<blocks, 256, 0, s>>>((const T*)in, n, spos, scnt, ntiles);
+    CompactList L0{nullptr, spos, scnt, nullptr, pos, dR, n, nullptr};
+    launch_compact(s, w + L.o_cst, ntiles, (uint32_t)TILE, L0, nullptr);
     const unsigned lb = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256), 2048));
-    rle_enc_lengths_kernel<T><<<lb, 256, 0, s>>>((uint8_t*)out, pos, dR, n, out_size);
+    rle_enc_lengths_kernel<T><<<lb, 256, 0, s>>>((const T*)in, (uint8_t*)out, pos, dR, n, out_size);
     FZH_CHECK_LAUNCH("rle encode");
     return FZH_OK;
 }
 
 struct DecWs {
-    size_t nseg, ntiles, o_ctx, o_start, o_tf, total;
+    size_t nseg, ntiles, o_R, o_seg, o_start, o_tf, total;
 };
 DecWs dec_layout(size_t n) {
     DecWs w;
-    w.nseg = cdiv(std::max<size_t>(n, 1), 1024);
+    w.nseg = std::max<size_t>(cdiv(std::max<size_t>(n, 1), SCAN_SEG), 1);
     w.ntiles = cdiv(std::max<size_t>(n, 1), DTILE);
     size_t o = 0;
-    w.o_ctx = o;   o = align_up(o + scan_state_bytes(w.nseg), 256);
+    w.o_R = o;     o = align_up(o + 16, 256);
+    w.o_seg = o;   o = align_up(o + 4 * w.nseg, 256);
     w.o_start = o; o = align_up(o + 4 * (n + 2), 256);
     w.o_tf = o;    o = align_up(o + 4 * (w.ntiles + 1), 256);
     w.total = o;
@@ -325,14 +350,18 @@ int dec_impl(hipStream_t s, const void* in, size_t in_bytes, void* out, size_t n
     if (n == 0) return FZH_OK;
     if (((uintptr_t)in % 4) || ((uintptr_t)out % 16)) return fail(FZH_E_INVALID, "RLE dec: input must be 4-byte, output 16-byte aligned");
     if (n > 0xFFFFFFF0ull) return fail(FZH_E_INVALID, "RLE dec: more than 2^32 elements");
+    if (L.nseg > SCAN_MAX_SEGS) return fail(FZH_E_INVALID, "RLE dec: too many elements");
     uint8_t* w = (uint8_t*)ws;
-    ScanCtx* ctx = (ScanCtx*)(w + L.o_ctx);
-    uint64_t* status = (uint64_t*)(w + L.o_ctx + sizeof(ScanCtx));
+    uint32_t* Rv = (uint32_t*)(w + L.o_R);
+    uint32_t* tot = Rv + 1;
+    uint32_t* seg = (uint32_t*)(w + L.o_seg);
     uint32_t* start = (uint32_t*)(w + L.o_start);
     uint32_t* tf = (uint32_t*)(w + L.o_tf);
-    const unsigned sb = (unsigned)cdiv(L.nseg, 4);
-    rle_dec_scan_kernel<T><<<sb, 256, 0, s>>>((const uint8_t*)in, in_bytes, n, start, tf, d_status, ctx, status,
-                                              (uint32_t)L.nseg, sb * 4);
+    const uint8_t* bin = (const uint8_t*)in;
+    rle_dec_prep_kernel<T><<<1, 64, 0, s>>>(bin, in_bytes, n, Rv, d_status);
+    rle_dec_reduce_kernel<T><<<(unsigned)L.nseg, 256, 0, s>>>(bin, Rv, seg);
+    scan_partials_kernel<<<1, 1024, 0, s>>>(seg, (uint32_t)L.nseg, tot);
+    rle_dec_apply_kernel<T><<<(unsigned)L.nseg, 256, 0, s>>>(bin, n, Rv, seg, tot, start, tf, d_status);
     const unsigned eb = (unsigned)cdiv(L.ntiles, 2);
     rle_dec_expand_kernel<T><<<eb, 128, 0, s>>>((const uint8_t*)in, n, start, tf, d_status, (T*)out, (uint32_t)L.ntiles);
     FZH_CHECK_LAUNCH("rle decode");

@@ -23,7 +23,7 @@
 //   offsets; then one wave64 per chunk decodes, with the repeat-elimination
 //   decode done as a parallel prefix-popcount gather instead of a
 //   single-thread loop.
-#include "fzh_internal.h"
+#include "fzh_bitplane.h"
 
 namespace fzh {
 namespace {
@@ -35,6 +35,82 @@ constexpr int WAVES = 2;
 constexpr int LDS_STREAM = CS + 64;
 constexpr int LDS_BM1 = CS / 8;
 constexpr int LDS_PER_WAVE = LDS_STREAM + LDS_BM1;
+
+// Fused Bitshuffle (16 KiB blocks, element width SEW = 1 or 2, ballot
+// layout): a full chunk's bit-plane image in the register layout the RZE
+// kernels use (lane l, step j: bytes [1024 j + 16 l, +16) of the shuffled
+// block).  Shuffled word index = plane * npp + group with npp = 8192/SEW/32;
+// lane l step j holds words 256 j + 4 l + i, i = 0..3, so each lane
+// transposes its own groups: SEW 2 -> groups 4l+i (plane j);
+// SEW 1 -> groups 256 (j&1) + 4l + i (plane j/2).
+template <int SEW>
+__device__ __forceinline__ void shuffled_load(const uint8_t* blk, uint4 (&v)[16]) {
+    const int l = lane_id();
+    uint32_t vv[16][4];
+    if constexpr (SEW == 2) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint8_t* g = blk + 64 * (4 * l + i);
+            uint32_t w[16];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint4 t = ld16(g + 16 * q);
+                w[4 * q] = t.x; w[4 * q + 1] = t.y; w[4 * q + 2] = t.z; w[4 * q + 3] = t.w;
+            }
+            uint32_t A[16];
+            planes16(w, A);
+#pragma unroll
+            for (int j = 0; j < 16; j++) vv[j][i] = A[15 - j];
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint8_t* g = blk + 32 * (256 * h + 4 * l + i);
+                const uint4 t0 = ld16(g), t1 = ld16(g + 16);
+                const uint32_t w[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+                uint32_t A[8];
+                planes8(w, A);
+#pragma unroll
+                for (int P = 0; P < 8; P++) vv[2 * P + h][i] = A[7 - P];
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) v[j] = make_uint4(vv[j][0], vv[j][1], vv[j][2], vv[j][3]);
+}
+
+// Inverse of shuffled_load: vv[j][i] = shuffled words in the register layout;
+// writes the chunk's unshuffled 16 KiB to blk.
+template <int SEW>
+__device__ __forceinline__ void unshuffled_store(const uint32_t (&vv)[16][4], uint8_t* blk) {
+    const int l = lane_id();
+    if constexpr (SEW == 2) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint32_t A[16], w[16];
+#pragma unroll
+            for (int b = 0; b < 16; b++) A[b] = vv[15 - b][i];
+            unplanes16(A, w);
+            uint8_t* g = blk + 64 * (4 * l + i);
+#pragma unroll
+            for (int q = 0; q < 4; q++) st16(g + 16 * q, make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]));
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint32_t A[8], w[8];
+#pragma unroll
+                for (int P = 0; P < 8; P++) A[7 - P] = vv[2 * P + h][i];
+                unplanes8(A, w);
+                uint8_t* g = blk + 32 * (256 * h + 4 * l + i);
+                st16(g, make_uint4(w[0], w[1], w[2], w[3]));
+                st16(g + 16, make_uint4(w[4], w[5], w[6], w[7]));
+            }
+    }
+}
 
 __device__ __forceinline__ uint32_t byte_of(const uint4& v, int k) {
     const uint32_t w = k < 4 ? v.x : k < 8 ? v.y : k < 12 ? v.z : v.w;
@@ -74,9 +150,13 @@ __device__ __forceinline__ void lds_to_global(const uint8_t* lds, uint8_t* dst, 
 // ---------------------------------------------------------------------------
 // Encode
 // ---------------------------------------------------------------------------
+// SEW = 0: plain RZE; SEW = 1/2: the input is first bit-shuffled (16 KiB
+// blocks of SEW-byte elements; the first nfull chunks are full blocks, the
+// tail is taken verbatim, as BitshuffleStage leaves it).
+template <int SEW>
 __global__ void __launch_bounds__(64 * WAVES)
 rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restrict__ out,
-               uint8_t* __restrict__ scratch, uint32_t* __restrict__ sizes, uint32_t nc) {
+               uint8_t* __restrict__ scratch, uint32_t* __restrict__ sizes, uint32_t nc, uint64_t nfull) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int l = lane_id();
     const int wv = threadIdx.x >> 6;
@@ -91,6 +171,9 @@ rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restr
 
         // -- load the chunk: lane l, step j holds bytes [1024 j + 16 l, +16)
         uint4 v[16];
+        if (SEW != 0 && c < nfull) {
+            shuffled_load<SEW == 0 ? 2 : SEW>(in + in_off, v);
+        } else {
 #pragma unroll
         for (int j = 0; j < 16; j++) {
             const int p = 1024 * j + 16 * l;
@@ -102,6 +185,7 @@ rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restr
                     if (p + k < in_size) wds[k >> 2] |= (uint32_t)in[in_off + p + k] << (8 * (k & 3));
                 v[j] = make_uint4(wds[0], wds[1], wds[2], wds[3]);
             }
+        }
         }
         // -- level 1: nonzero masks (16 bits per lane per step) -> bm1 in LDS
         uint32_t m[16];
@@ -364,9 +448,12 @@ rze_offsets_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint32_t* 
     }
 }
 
+template <int SEW>
 __global__ void __launch_bounds__(64 * WAVES)
 rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __restrict__ out,
-               uint64_t orig_bytes, int* __restrict__ d_status, const uint32_t* __restrict__ offs, uint32_t nc) {
+               uint64_t orig_bytes, int* __restrict__ d_status, const uint32_t* __restrict__ offs, uint32_t nc,
+               uint64_t nfull) {
+    constexpr int SE = SEW == 0 ? 2 : SEW;
     extern __shared__ __align__(16) uint8_t smem[];
     const int l = lane_id();
     const int wv = threadIdx.x >> 6;
@@ -420,7 +507,18 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
             }
             wave_lds_sync();
             const uint8_t* cs = st + shift;  // chunk byte 0
-            if (israw) {
+            if (israw && SEW != 0 && c < nfull) {
+                uint32_t vv[16][4];
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t p = 1024 * j + 16 * l + 4 * q + shift;
+                        const uint32_t lo = stw[p >> 2], hi = stw[(p >> 2) + 1];
+                        vv[j][q] = (p & 3) ? __builtin_amdgcn_alignbyte(hi, lo, p & 3) : lo;
+                    }
+                unshuffled_store<SE>(vv, dst);
+            } else if (israw) {
                 // raw copy: cs[0 .. o) -> dst (dst 16-byte aligned)
                 for (uint32_t i = 16 * l; i < o; i += 1024) {
                     uint32_t w4[4];
@@ -508,7 +606,10 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
                 if ((int)data_nz > bm1_start) bad = true;
                 wave_lds_sync();
                 // zero-elimination decode: lane l, step j -> output bytes [1024 j + 16 l, +16)
+                const bool unshuf = SEW != 0 && c < nfull;
+                uint32_t vv[16][4];
                 uint32_t run = 0;
+#pragma unroll
                 for (int j = 0; j < 16; j++) {
                     const uint32_t mm = *reinterpret_cast<const uint16_t*>(bm1 + 128 * j + 2 * l);
                     const uint32_t cnt = __popc(mm);
@@ -525,10 +626,16 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
                     }
                     run += __shfl(incl, 63, 64);
                     const uint32_t p = 1024 * j + 16 * l;
-                    if (p + 16 <= o) *reinterpret_cast<uint4*>(dst + p) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
-                    else
+                    if (unshuf) {
+#pragma unroll
+                        for (int q = 0; q < 4; q++) vv[j][q] = w4[q];
+                    } else if (p + 16 <= o) {
+                        *reinterpret_cast<uint4*>(dst + p) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+                    } else {
                         for (uint32_t k = p; k < o; k++) dst[k] = (uint8_t)(w4[(k - p) >> 2] >> (8 * ((k - p) & 3)));
+                    }
                 }
+                if (unshuf) unshuffled_store<SE>(vv, dst);
                 if (__ballot(bad) != 0ull && l == 0) *d_status = 1;
             }
         }
@@ -557,31 +664,35 @@ size_t fzh_rze_enc_ws_bytes(size_t nbytes) {
     return align_up(4 * std::max<size_t>(nc, 1), 256) + nc * CS;
 }
 
-int fzh_rze_enc(void* stream, const void* d_in, size_t nbytes, void* d_out, size_t out_cap, uint64_t* d_out_size,
-                void* d_ws, size_t ws_bytes) {
-    if (nbytes == 0) {
-        return fail(FZH_E_INVALID, "RZE: empty input");
-    }
+}  // extern "C"
+
+namespace fzh {
+namespace {
+
+int rze_enc_impl(hipStream_t s, const void* d_in, size_t nbytes, int sew, void* d_out, size_t out_cap,
+                 uint64_t* d_out_size, void* d_ws, size_t ws_bytes) {
+    if (nbytes == 0) return fail(FZH_E_INVALID, "RZE: empty input");
     if (nbytes > 0xFFFFFFFFull) return fail(FZH_E_INVALID, "RZE: input larger than 4 GiB");
     if (out_cap < fzh_rze_enc_bound(nbytes)) return fail(FZH_E_CAPACITY, "RZE: output buffer below fzh_rze_enc_bound()");
     if (ws_bytes < fzh_rze_enc_ws_bytes(nbytes)) return fail(FZH_E_CAPACITY, "RZE: workspace too small");
     if (((uintptr_t)d_in | (uintptr_t)d_out) % 16 != 0) return fail(FZH_E_INVALID, "RZE: buffers must be 16-byte aligned");
     const uint32_t nc = (uint32_t)cdiv(nbytes, CS);
+    const uint64_t nfull = nbytes / CS;
     const unsigned blocks = (unsigned)cdiv(nc, WAVES);
     uint32_t* sizes = (uint32_t*)d_ws;
     uint8_t* scratch = (uint8_t*)d_ws + align_up(4 * (size_t)nc, 256);
-    rze_enc_kernel<<<blocks, 64 * WAVES, kLds, S(stream)>>>((const uint8_t*)d_in, nbytes, (uint8_t*)d_out, scratch,
-                                                            sizes, nc);
-    rze_pack_kernel<<<(unsigned)cdiv(nc, PACK), 256, 0, S(stream)>>>((const uint8_t*)d_in, nbytes, scratch, sizes,
-                                                                     (uint8_t*)d_out, d_out_size, nc);
+    const uint8_t* in = (const uint8_t*)d_in;
+    uint8_t* out = (uint8_t*)d_out;
+    if (sew == 1) rze_enc_kernel<1><<<blocks, 64 * WAVES, kLds, s>>>(in, nbytes, out, scratch, sizes, nc, nfull);
+    else if (sew == 2) rze_enc_kernel<2><<<blocks, 64 * WAVES, kLds, s>>>(in, nbytes, out, scratch, sizes, nc, nfull);
+    else rze_enc_kernel<0><<<blocks, 64 * WAVES, kLds, s>>>(in, nbytes, out, scratch, sizes, nc, 0);
+    rze_pack_kernel<<<(unsigned)cdiv(nc, PACK), 256, 0, s>>>(in, nbytes, scratch, sizes, out, d_out_size, nc);
     FZH_CHECK_LAUNCH("rze_enc_kernel");
     return FZH_OK;
 }
 
-size_t fzh_rze_dec_ws_bytes(size_t orig_bytes) { return align_up(4 * std::max<size_t>(cdiv(orig_bytes, CS), 1), 256); }
-
-int fzh_rze_dec(void* stream, const void* d_in, size_t in_bytes, void* d_out, size_t orig_bytes, int* d_status,
-                void* d_ws, size_t ws_bytes) {
+int rze_dec_impl(hipStream_t s, const void* d_in, size_t in_bytes, void* d_out, size_t orig_bytes, int sew,
+                 int* d_status, void* d_ws, size_t ws_bytes) {
     if (orig_bytes == 0) return FZH_OK;
     if (orig_bytes > 0xFFFFFFFFull) return fail(FZH_E_INVALID, "RZE: output larger than 4 GiB");
     if (ws_bytes < fzh_rze_dec_ws_bytes(orig_bytes)) return fail(FZH_E_CAPACITY, "RZE: workspace too small");
@@ -589,13 +700,51 @@ int fzh_rze_dec(void* stream, const void* d_in, size_t in_bytes, void* d_out, si
         return fail(FZH_E_INVALID, "RZE: input must be 4-byte and output 16-byte aligned");
     if (!d_status) return fail(FZH_E_INVALID, "RZE: null status pointer");
     const uint32_t nc = (uint32_t)cdiv(orig_bytes, CS);
+    const uint64_t nfull = orig_bytes / CS;
     const unsigned blocks = (unsigned)cdiv(nc, WAVES);
     uint32_t* offs = (uint32_t*)d_ws;
-    rze_offsets_kernel<<<1, 1024, 0, S(stream)>>>((const uint8_t*)d_in, in_bytes, offs, nc);
-    rze_dec_kernel<<<blocks, 64 * WAVES, kLds, S(stream)>>>((const uint8_t*)d_in, in_bytes, (uint8_t*)d_out,
-                                                            orig_bytes, d_status, offs, nc);
+    const uint8_t* in = (const uint8_t*)d_in;
+    uint8_t* out = (uint8_t*)d_out;
+    rze_offsets_kernel<<<1, 1024, 0, s>>>(in, in_bytes, offs, nc);
+    if (sew == 1)
+        rze_dec_kernel<1><<<blocks, 64 * WAVES, kLds, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, nfull);
+    else if (sew == 2)
+        rze_dec_kernel<2><<<blocks, 64 * WAVES, kLds, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, nfull);
+    else
+        rze_dec_kernel<0><<<blocks, 64 * WAVES, kLds, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, 0);
     FZH_CHECK_LAUNCH("rze_dec_kernel");
     return FZH_OK;
+}
+
+}  // namespace
+}  // namespace fzh
+
+extern "C" {
+
+int fzh_rze_enc(void* stream, const void* d_in, size_t nbytes, void* d_out, size_t out_cap, uint64_t* d_out_size,
+                void* d_ws, size_t ws_bytes) {
+    return rze_enc_impl(S(stream), d_in, nbytes, 0, d_out, out_cap, d_out_size, d_ws, ws_bytes);
+}
+
+int fzh_rze_enc_shuffled(void* stream, const void* d_in, size_t nbytes, int elem_width, void* d_out,
+                         size_t out_cap, uint64_t* d_out_size, void* d_ws, size_t ws_bytes) {
+    if (elem_width != 1 && elem_width != 2)
+        return fail(FZH_E_INVALID, "RZE (shuffled): element width must be 1 or 2");
+    return rze_enc_impl(S(stream), d_in, nbytes, elem_width, d_out, out_cap, d_out_size, d_ws, ws_bytes);
+}
+
+size_t fzh_rze_dec_ws_bytes(size_t orig_bytes) { return align_up(4 * std::max<size_t>(cdiv(orig_bytes, CS), 1), 256); }
+
+int fzh_rze_dec(void* stream, const void* d_in, size_t in_bytes, void* d_out, size_t orig_bytes, int* d_status,
+                void* d_ws, size_t ws_bytes) {
+    return rze_dec_impl(S(stream), d_in, in_bytes, d_out, orig_bytes, 0, d_status, d_ws, ws_bytes);
+}
+
+int fzh_rze_dec_shuffled(void* stream, const void* d_in, size_t in_bytes, void* d_out, size_t orig_bytes,
+                         int elem_width, int* d_status, void* d_ws, size_t ws_bytes) {
+    if (elem_width != 1 && elem_width != 2)
+        return fail(FZH_E_INVALID, "RZE (shuffled): element width must be 1 or 2");
+    return rze_dec_impl(S(stream), d_in, in_bytes, d_out, orig_bytes, elem_width, d_status, d_ws, ws_bytes);
 }
 
 }  // extern "C"

@@ -75,25 +75,33 @@ def bitshuffle(b, block_size=16384, elem_width=2, inverse=False):
     return dout.download(np.uint8, b.size)
 
 
-def rze_enc(b):
+def rze_enc(b, shuffle_width=0):
+    """RZE stream of b; with shuffle_width 1/2 the fused Bitshuffle(16 KiB)+RZE."""
     b = np.ascontiguousarray(b).view(np.uint8).ravel()
     din = DeviceBuffer.from_array(b)
     cap = lib().fzh_rze_enc_bound(b.size)
     dout = DeviceBuffer(cap)
     dsz = DeviceBuffer(8)
     ws = workspace(lib().fzh_rze_enc_ws_bytes(b.size))
-    call("fzh_rze_enc", None, din.ptr, b.size, dout.ptr, cap, dsz.ptr, ws.ptr, ws.nbytes)
+    if shuffle_width:
+        call("fzh_rze_enc_shuffled", None, din.ptr, b.size, shuffle_width, dout.ptr, cap, dsz.ptr, ws.ptr, ws.nbytes)
+    else:
+        call("fzh_rze_enc", None, din.ptr, b.size, dout.ptr, cap, dsz.ptr, ws.ptr, ws.nbytes)
     sz = int(dsz.download(np.uint64, 1)[0])
     return dout.download(np.uint8, sz)
 
 
-def rze_dec(s, orig_bytes):
+def rze_dec(s, orig_bytes, shuffle_width=0):
     s = np.ascontiguousarray(s).view(np.uint8).ravel()
     din = DeviceBuffer.from_array(s)
     dout = DeviceBuffer(orig_bytes)
     st = DeviceBuffer.zeros(4)
     ws = workspace(lib().fzh_rze_dec_ws_bytes(orig_bytes))
-    call("fzh_rze_dec", None, din.ptr, s.size, dout.ptr, orig_bytes, st.ptr, ws.ptr, ws.nbytes)
+    if shuffle_width:
+        call("fzh_rze_dec_shuffled", None, din.ptr, s.size, dout.ptr, orig_bytes, shuffle_width, st.ptr, ws.ptr,
+             ws.nbytes)
+    else:
+        call("fzh_rze_dec", None, din.ptr, s.size, dout.ptr, orig_bytes, st.ptr, ws.ptr, ws.nbytes)
     if int(st.download(np.int32, 1)[0]) != 0:
         raise ValueError("malformed RZE stream")
     return dout.download(np.uint8, orig_bytes)

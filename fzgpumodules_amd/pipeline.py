@@ -99,6 +99,9 @@ class Pipeline:
     def enable_graph_mode(self, on=True) -> None:
         call("fzp_enable_graph_mode", self.h, int(bool(on)))
 
+    def set_stage_fusion(self, on=True) -> None:
+        call("fzp_set_stage_fusion", self.h, int(bool(on)))
+
     def enable_profiling(self, on=True) -> None:
         call("fzp_enable_profiling", self.h, int(bool(on)))
 

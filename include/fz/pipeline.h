@@ -113,6 +113,12 @@ class Pipeline {
     // Graph mode: compress() is captured into a hipGraph on its first call
     // for a given (input pointer, size) and replayed afterwards.  Needs every
     // stage isGraphCompatible().
+    // Stage fusion: a BitshuffleStage (16 KiB blocks, element width 1/2)
+    // feeding an RZEStage runs as one kernel pass (identical bytes, no
+    // shuffled intermediate).  On by default.
+    void setStageFusion(bool on) { fusion_ = on; }
+    bool isStageFusionEnabled() const { return fusion_; }
+
     void enableGraphMode(bool on);
     bool isGraphModeEnabled() const { return graph_mode_; }
     void captureGraph(hipStream_t stream = nullptr);
@@ -148,6 +154,8 @@ class Pipeline {
         std::vector<int> out_buf;  // buffer id per output
         int in_buf = -1;           // buffer id of the single input
         int level = 0;
+        int fused_from = -1;       // RZE node: the Bitshuffle node folded into it
+        bool elided = false;       // Bitshuffle node run inside the next RZE node
     };
     struct Buffer {
         int producer = -1;  // node index, -1 = pipeline input
@@ -169,6 +177,7 @@ class Pipeline {
                                                             hipStream_t stream);
     void collectReadbacks();
     void destroyGraph();
+    bool updateFusion();
 
     std::unique_ptr<MemoryPool> pool_;
     MemoryStrategy strategy_ = MemoryStrategy::MINIMAL;
@@ -181,7 +190,7 @@ class Pipeline {
     std::vector<Buffer> buffers_;   // 0 = pipeline input
     std::vector<int> outputs_;      // buffer ids of pipeline outputs, in order
     bool finalized_ = false, warmup_on_finalize_ = false, pool_managed_decomp_ = false;
-    bool profiling_ = false, graph_mode_ = false, compressed_ = false;
+    bool profiling_ = false, graph_mode_ = false, compressed_ = false, fusion_ = true;
     size_t input_hint_ = 0, input_alignment_ = 1;
     size_t input_size_ = 0;       // padded input bytes of the last compress
     size_t orig_input_size_ = 0;  // caller's input bytes of the last compress

@@ -269,6 +269,11 @@ class RZEStage : public Stage {
     int getLevels() const { return levels_; }
     uint32_t getCachedOrigBytes() const { return cached_orig_bytes_; }
     size_t getRequiredInputAlignment() const override { return chunk_size_; }
+    // Set by the pipeline when a BitshuffleStage (16 KiB blocks, element
+    // width 1 or 2) feeds this stage: the input is then the un-shuffled data
+    // and the shuffle runs inside the RZE kernels (same stream bytes).
+    void setPreShuffle(int elem_width) { pre_shuffle_ = elem_width; }
+    int getPreShuffle() const { return pre_shuffle_; }
 
     void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
                  const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
@@ -306,6 +311,7 @@ class RZEStage : public Stage {
 
   private:
     bool is_inverse_ = false;
+    int pre_shuffle_ = 0;
     uint32_t chunk_size_ = 16384;
     uint8_t levels_ = 4;
     uint32_t cached_orig_bytes_ = 0;

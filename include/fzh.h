@@ -9,10 +9,14 @@
  *
  * Workspaces: entry points that need scratch take (d_ws, ws_bytes).  Query the
  * size with the matching *_ws_bytes() function.  A workspace must be zeroed
- * once with fzh_workspace_init() before first use; after that it can be reused
- * by any number of calls on one stream (its ordered-scan state persists from
- * call to call, so no per-call clearing is needed), but never by two calls
- * that may run concurrently.
+ * with fzh_workspace_init() before its first use and again whenever the
+ * problem size changes; between those, calls of the same size on one stream
+ * reuse it with no clearing (the small scan state it keeps is tagged per
+ * call), but two calls that may run concurrently never share one.  Stages
+ * that emit variable-length lists (outliers, runs, compressed chunks,
+ * Huffman cells) stage them in the workspace per tile/chunk and place them
+ * with a separate kernel, so the workspace holds about as many bytes as the
+ * stage's input.
  *
  * Reference interfaces replaced (file:line in szcompressor/FZGPUModules):
  *   fzh_lorenzo_quant_fwd_f32   launchLorenzoKernel / launchLorenzoKernel2D /
@@ -151,6 +155,19 @@ int fzh_rze_enc(void* stream, const void* d_in, size_t nbytes, void* d_out,
 size_t fzh_rze_dec_ws_bytes(size_t orig_bytes);
 int fzh_rze_dec(void* stream, const void* d_in, size_t in_bytes, void* d_out,
                 size_t orig_bytes, int* d_status, void* d_ws, size_t ws_bytes);
+
+/* Fused Bitshuffle(block 16384, element width 1 or 2) + RZE: the same stream
+ * fzh_bitshuffle_enc followed by fzh_rze_enc produce, in one pass over the
+ * input (no shuffled intermediate in HBM); the decoder likewise returns the
+ * unshuffled bytes.  Workspaces as for fzh_rze_enc / fzh_rze_dec.  This pair
+ * is what the pipeline runs for a BitshuffleStage feeding an RZEStage
+ * (bitshuffle_stage.cu:355-475 + rze_stage.cu:774-994).                   */
+int fzh_rze_enc_shuffled(void* stream, const void* d_in, size_t nbytes, int elem_width,
+                         void* d_out, size_t out_cap, uint64_t* d_out_size, void* d_ws,
+                         size_t ws_bytes);
+int fzh_rze_dec_shuffled(void* stream, const void* d_in, size_t in_bytes, void* d_out,
+                         size_t orig_bytes, int elem_width, int* d_status, void* d_ws,
+                         size_t ws_bytes);
 
 /* ---- RLE: [R u32][values T x R, padded to 4][lengths u32 x R] ------------ */
 size_t fzh_rle_enc_bound(size_t n, int elem_bytes);

@@ -66,6 +66,8 @@ int fzp_connect(fzp_pipeline* p, int dependent_id, int producer_id, const char* 
 int fzp_set_dims(fzp_pipeline* p, size_t x, size_t y, size_t z);
 int fzp_finalize(fzp_pipeline* p);
 int fzp_enable_graph_mode(fzp_pipeline* p, int on);
+/* Stage fusion (Bitshuffle folded into RZE); on by default. */
+int fzp_set_stage_fusion(fzp_pipeline* p, int on);
 int fzp_enable_profiling(fzp_pipeline* p, int on);
 int fzp_set_pool_managed_decomp_output(fzp_pipeline* p, int on);
 int fzp_max_compressed_size(fzp_pipeline* p, size_t input_bytes, size_t* out);

@@ -90,3 +90,15 @@ def test_bitshuffle_rze_chain_on_codes(gpu, orc):
     np.testing.assert_array_equal(s, orc.rze_enc(orc.bitshuffle_enc(z.view(np.uint8), 16384, 2)))
     back = gpu.bitshuffle(gpu.rze_dec(s, sh.size), 16384, 2, inverse=True)
     np.testing.assert_array_equal(back.view(np.uint16), z)
+
+
+@pytest.mark.parametrize("ew", [1, 2])
+@pytest.mark.parametrize("name", ["zeros_multi", "sparse", "runs", "random_raw", "partial_chunk", "mixed"])
+def test_fused_bitshuffle_rze_matches_two_stages(gpu, orc, ew, name):
+    """fzh_rze_enc_shuffled == RZE(Bitshuffle(x)) byte for byte, and the fused
+    decoder returns x (bitshuffle_stage.cu + rze_stage.cu composed)."""
+    b = _rze_cases()[name]
+    want = orc.rze_enc(orc.bitshuffle_enc(b, 16384, ew))
+    got = gpu.rze_enc(b, shuffle_width=ew)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(gpu.rze_dec(got, b.size, shuffle_width=ew), b)

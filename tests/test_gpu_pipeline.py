@@ -293,3 +293,32 @@ def test_strict_error_bound(case):
         expect = recon[: x.size].copy()
         expect[pidx] = pval
         np.testing.assert_array_equal(y, expect)
+
+
+@pytest.mark.parametrize("fusion", [True, False])
+@pytest.mark.parametrize("ew", [1, 2])
+def test_stage_fusion_same_bytes(fusion, ew):
+    """Bitshuffle folded into RZE (default) and the two-stage path give the
+    oracle's bytes; element width 1 runs on u8 codes (radius 64)."""
+    ny, nx = 150, 260
+    x = cloud(ny, nx, _seed(f"fuse{ew}"))
+    code_bytes, radius = (2, 32768) if ew == 2 else (1, 64)
+    xp, codes, err, idx, cnt, abs_eb, ebx2 = chains.lorenzo_part(x, (nx, ny, 1), 1, 1e-3, radius, 0.1, True,
+                                                                 code_bytes, 16384)
+    rze = chains.orc.rze_enc(chains.orc.bitshuffle_enc(codes.view(np.uint8), 16384, ew))
+    want = chains.concat([err, idx, np.array([cnt], np.uint32), rze])
+    with Pipeline(x.nbytes) as p:
+        lz = p.add_lorenzo(1e-3, mode="REL", quant_radius=radius, outlier_capacity=0.1, zigzag=True,
+                           code_bytes=code_bytes)
+        bs = p.add_bitshuffle(16384, ew)
+        rz = p.add_rze()
+        p.connect(bs, lz, "codes")
+        p.connect(rz, bs)
+        p.set_dims(nx, ny, 1)
+        p.set_stage_fusion(fusion)
+        p.finalize()
+        blob = p.compress_array(x)
+        np.testing.assert_array_equal(blob, want)
+        y = p.decompress_array(blob)
+        ref = chains.orc.lorenzo_inv(codes, (nx, ny, 1), err, idx, cnt, ebx2, True)
+        np.testing.assert_array_equal(y, ref[: x.size])
