@@ -122,6 +122,14 @@ concat_copy_kernel(ConcatArgs a, uint8_t* __restrict__ out, const uint64_t* __re
     if (w0 >= nw) return;
     const uint8_t* src = reinterpret_cast<const uint8_t*>(a.slot[sidx].src);
     uint4* dst = reinterpret_cast<uint4*>(out + ws[sidx]);
+    if (reinterpret_cast<const void*>(dst) == reinterpret_cast<const void*>(src)) {
+        // produced in place: only the pad of the last 16-byte word is due
+        if (w0 == 0 && threadIdx.x == 0 && (sz & 15)) {
+            uint8_t* d8 = out + ws[sidx];
+            for (uint64_t b = sz; b < nw * 16; b++) d8[b] = 0;
+        }
+        return;
+    }
     const uint64_t full = sz / 16;
 #pragma unroll
     for (int k = 0; k < 4; k++) {

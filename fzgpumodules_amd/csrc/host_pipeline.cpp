@@ -71,7 +71,7 @@ Pipeline::~Pipeline() {
     destroyGraph();
     if (graph_stream_) (void)hipStreamDestroy(graph_stream_);
     for (auto& b : buffers_)
-        if (b.producer >= 0 && b.ptr) pool_->freePersistentDevice(b.ptr);
+        if (b.producer >= 0 && b.ptr && !b.aliased) pool_->freePersistentDevice(b.ptr);
     for (void* p : inv_buf_)
         if (p) pool_->freePersistentDevice(p);
     if (pad_buf_) pool_->freePersistentDevice(pad_buf_);
@@ -177,9 +177,15 @@ void Pipeline::finalize() {
                                      dataTypeToString((DataType)pt) + " but '" + e.dep->getName() + "' expects " +
                                      dataTypeToString((DataType)ct));
     }
+    // Pipeline outputs, deepest stage first (then stage and port order): the
+    // first one -- the coder stream of a chain -- is written straight into
+    // the concat buffer, the rest are gathered behind it.
     outputs_.clear();
     for (size_t b = 1; b < buffers_.size(); b++)
         if (buffers_[b].consumer < 0) outputs_.push_back((int)b);
+    std::stable_sort(outputs_.begin(), outputs_.end(), [&](int a, int b) {
+        return nodes_[buffers_[a].producer].level > nodes_[buffers_[b].producer].level;
+    });
     if (outputs_.empty()) throw std::runtime_error("Pipeline has no outputs");
     if (outputs_.size() > FZH_CONCAT_MAX_SLOTS || outputs_.size() > FZM_MAX_BUFFERS)
         throw std::runtime_error("Pipeline has more than 32 outputs");
@@ -242,11 +248,33 @@ void Pipeline::ensureBuffers(size_t padded) {
         const auto e = n.stage->estimateOutputSizes({est[n.in_buf]});
         for (size_t i = 0; i < n.out_buf.size(); i++) est[n.out_buf[i]] = i < e.size() ? e[i] : 0;
     }
+    const int primary = outputs_.size() > 1 ? outputs_[0] : -1;
+    if (outputs_.size() > 1) {
+        std::vector<fzh_concat_slot> slots(outputs_.size());
+        for (size_t i = 0; i < outputs_.size(); i++) slots[i].cap = est[outputs_[i]];
+        const size_t need = round_up(fzh_concat_bound(slots.data(), (int)slots.size()), 256);
+        if (concat_cap_ < need) {
+            if (buffers_[primary].ptr && !buffers_[primary].aliased) pool_->freePersistentDevice(buffers_[primary].ptr);
+            buffers_[primary].ptr = nullptr;
+            if (concat_buf_) pool_->freePersistentDevice(concat_buf_);
+            concat_buf_ = pool_->allocatePersistentDevice(need, "concat");
+            if (!concat_buf_) throw std::runtime_error("Pipeline: concat allocation failed");
+            concat_cap_ = need;
+        }
+        // the primary output lives in its concat slot (the slot right after
+        // the header, 16-byte aligned)
+        Buffer& P = buffers_[primary];
+        P.ptr = (uint8_t*)concat_buf_ + ((4 + 8 * outputs_.size() + 15) & ~size_t(15));
+        P.cap = round_up(std::max<size_t>(est[primary], 16), 16);
+        P.aliased = true;
+    }
     for (size_t b = 1; b < buffers_.size(); b++) {
+        if ((int)b == primary) continue;
         Buffer& B = buffers_[b];
         const size_t need = round_up(std::max<size_t>(est[b], 16), 256);
-        if (B.ptr && B.cap >= need) continue;
-        if (B.ptr) pool_->freePersistentDevice(B.ptr);
+        if (B.ptr && B.cap >= need && !B.aliased) continue;
+        if (B.ptr && !B.aliased) pool_->freePersistentDevice(B.ptr);
+        B.aliased = false;
         B.ptr = pool_->allocatePersistentDevice(need, B.name);
         if (!B.ptr) throw std::runtime_error("Pipeline: buffer allocation failed (" + std::to_string(need) + " bytes)");
         B.cap = need;
@@ -255,17 +283,6 @@ void Pipeline::ensureBuffers(size_t padded) {
     last_sizes_.assign(buffers_.size(), 0);
     for (size_t b = 0; b < buffers_.size(); b++) buffers_[b].size = est[b];
     est_ = est;
-    if (outputs_.size() > 1) {
-        std::vector<fzh_concat_slot> slots(outputs_.size());
-        for (size_t i = 0; i < outputs_.size(); i++) slots[i].cap = est[outputs_[i]];
-        const size_t need = round_up(fzh_concat_bound(slots.data(), (int)slots.size()), 256);
-        if (concat_cap_ < need) {
-            if (concat_buf_) pool_->freePersistentDevice(concat_buf_);
-            concat_buf_ = pool_->allocatePersistentDevice(need, "concat");
-            if (!concat_buf_) throw std::runtime_error("Pipeline: concat allocation failed");
-            concat_cap_ = need;
-        }
-    }
     buffers_for_ = padded;
 }
 

@@ -165,6 +165,7 @@ class Pipeline {
         size_t cap = 0;
         size_t size = 0;
         std::string name;
+        bool aliased = false;  // points into the concat buffer (not owned)
     };
 
     int nodeOf(const Stage* s) const;

@@ -241,6 +241,8 @@ int fzh_copy(void* stream, void* d_dst, const void* d_src, size_t bytes);
  * kind 2 = *(const uint32_t*)size_ptr * elem; every size is clamped to `cap`
  * (the slot's source capacity).  slots is a host array (passed by value to
  * the kernels, so the call is capture-safe); src must be 16-byte aligned.
+ * A slot whose src already is its place in d_out (a stage that wrote its
+ * output in place) is not copied; only its pad bytes are zeroed.
  * d_total (u64) receives the concat length.  d_ws: fzh_concat_ws_bytes().  */
 #define FZH_CONCAT_MAX_SLOTS 32
 typedef struct fzh_concat_slot {

@@ -5,9 +5,9 @@ chain: the input zero-padded to the stages' alignment
 (src/pipeline/compressor_exec.cpp:35-54), the error bound resolved as in
 lorenzo_quant.cu:612-644, the stages applied in order, and the pipeline
 outputs concatenated as [u32 n][u64 size x n] + 16-byte-aligned slots
-(src/pipeline/pipeline_utils.h:36-46).  Slot order is the order in which
-the stages and their ports appear (Lorenzo's outlier_errors,
-outlier_indices, outlier_count, then the coder stream); pads are zero.
+(src/pipeline/pipeline_utils.h:36-46).  Slot order: the deepest stage's
+output first (the coder stream), then the others in stage and port order
+(Lorenzo's outlier_errors, outlier_indices, outlier_count); pads are zero.
 """
 from __future__ import annotations
 
@@ -83,7 +83,7 @@ def lorenzo_bitshuffle_rze(x, dims, mode=1, user_eb=1e-3, radius=32768, cap=0.1,
     xp, codes, err, idx, cnt, abs_eb, ebx2 = lorenzo_part(x, dims, mode, user_eb, radius, cap, zigzag, 2, 16384)
     bs = orc.bitshuffle_enc(codes.view(np.uint8), 16384, 2)
     rze = orc.rze_enc(bs)
-    blob = concat([err, idx, np.array([cnt], np.uint32), rze])
+    blob = concat([rze, err, idx, np.array([cnt], np.uint32)])
     return blob, dict(xp=xp, codes=codes, err=err, idx=idx, cnt=cnt, abs_eb=abs_eb, ebx2=ebx2)
 
 
@@ -91,7 +91,7 @@ def lorenzo_rle(x, user_eb=1e-4, radius=32768, cap=0.2, zigzag=False):
     """Config-1 chain: 1D LorenzoQuant (ABS) -> RLE<uint16_t>."""
     xp, codes, err, idx, cnt, abs_eb, ebx2 = lorenzo_part(x, (0, 1, 1), 0, user_eb, radius, cap, zigzag, 2, 1)
     rle = orc.rle_enc(codes)
-    blob = concat([err, idx, np.array([cnt], np.uint32), rle])
+    blob = concat([rle, err, idx, np.array([cnt], np.uint32)])
     return blob, dict(xp=xp, codes=codes, err=err, idx=idx, cnt=cnt, abs_eb=abs_eb, ebx2=ebx2)
 
 
@@ -99,7 +99,7 @@ def lorenzo_huffman(x, dims, mode=1, user_eb=1e-4, radius=512, cap=0.2, zigzag=T
     """Config-3 chain: LorenzoQuant -> Huffman (coarse, bklen 2*radius)."""
     xp, codes, err, idx, cnt, abs_eb, ebx2 = lorenzo_part(x, dims, mode, user_eb, radius, cap, zigzag, 2, 1)
     huf = orc.huffman_encode(codes, bklen)
-    blob = concat([err, idx, np.array([cnt], np.uint32), huf])
+    blob = concat([huf, err, idx, np.array([cnt], np.uint32)])
     return blob, dict(xp=xp, codes=codes, err=err, idx=idx, cnt=cnt, abs_eb=abs_eb, ebx2=ebx2)
 
 

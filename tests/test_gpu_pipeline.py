@@ -57,7 +57,7 @@ def test_cfg2_chain_bit_exact(shape):
         blob = p.compress_array(x)
         np.testing.assert_array_equal(blob, want)
         names = [o[0] for o in p.outputs()]
-        assert names == ["outlier_errors", "outlier_indices", "outlier_count", "output"]
+        assert names == ["output", "outlier_errors", "outlier_indices", "outlier_count"]
         y = p.decompress_array(blob)
         assert y.size == x.size
         ref = chains.reconstruct(info, (nx, ny, 1))
@@ -216,9 +216,9 @@ def test_quantizer_bitpack_chain(orc):
         parts = chains.split(blob)
         r, ebx2 = np.float32(1) / np.float32(1.0), np.float32(1.0)
         codes, vals, idx, cnt = orc.quant_abs_fwd(x, r, 32768, True, 2, max_outliers=5000)
-        assert int(parts[2].view(np.uint32)[0]) == cnt
+        assert int(parts[3].view(np.uint32)[0]) == cnt
         nb = orc.bitpack_nbits_for_max(int(codes.max()), 2)
-        np.testing.assert_array_equal(parts[3], orc.bitpack_enc(codes, nb))
+        np.testing.assert_array_equal(parts[0], orc.bitpack_enc(codes, nb))
         y = p.decompress_array(blob)
         np.testing.assert_array_equal(y, orc.quant_abs_inv(codes, ebx2, True, vals, idx, cnt))
 
@@ -282,11 +282,11 @@ def test_strict_error_bound(case):
         ref = chains.split(want)
         assert len(got) == 7
         assert pidx.size > 0  # the data exercise the patch path
-        for i, j in ((0, 0), (1, 1), (2, 2), (6, 3)):
-            np.testing.assert_array_equal(got[i], ref[j], err_msg=f"part {i}")
-        np.testing.assert_array_equal(got[4].view(np.uint32), pidx)
-        np.testing.assert_array_equal(got[3].view(np.float32), pval)
-        assert got[5].view(np.uint32)[0] == pidx.size
+        for i in range(4):
+            np.testing.assert_array_equal(got[i], ref[i], err_msg=f"part {i}")
+        np.testing.assert_array_equal(got[5].view(np.uint32), pidx)
+        np.testing.assert_array_equal(got[4].view(np.float32), pval)
+        assert got[6].view(np.uint32)[0] == pidx.size
         y = p.decompress_array(blob)[: x.size]
         err = np.abs(y.astype(np.float64) - x.astype(np.float64))
         assert err.max() <= float(info["abs_eb"])  # exhaustive, no slack
@@ -306,7 +306,7 @@ def test_stage_fusion_same_bytes(fusion, ew):
     xp, codes, err, idx, cnt, abs_eb, ebx2 = chains.lorenzo_part(x, (nx, ny, 1), 1, 1e-3, radius, 0.1, True,
                                                                  code_bytes, 16384)
     rze = chains.orc.rze_enc(chains.orc.bitshuffle_enc(codes.view(np.uint8), 16384, ew))
-    want = chains.concat([err, idx, np.array([cnt], np.uint32), rze])
+    want = chains.concat([rze, err, idx, np.array([cnt], np.uint32)])
     with Pipeline(x.nbytes) as p:
         lz = p.add_lorenzo(1e-3, mode="REL", quant_radius=radius, outlier_capacity=0.1, zigzag=True,
                            code_bytes=code_bytes)
