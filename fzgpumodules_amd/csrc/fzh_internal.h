@@ -88,19 +88,32 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  This maps the hardware index to a logical one so that every XCD
+// works through one contiguous range: neighbouring tiles, which share cache
+// lines at their edges, then meet in the same L2.  A bijection on
+// [0, nblocks).
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t nblocks) {
+    constexpr uint32_t X = 8;
+    const uint32_t per = nblocks / X, rem = nblocks % X;
+    const uint32_t xcd = b % X, k = b / X;
+    const uint32_t base = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+    return base + k;
+}
+
 // ---------------------------------------------------------------------------
-// Ordered tile scan (single-pass decoupled look-back), one wave per tile.
+// Cross-workgroup prefixes over a few hundred coarse units.
 //
 // Layout of the scan state inside a workspace (zeroed once):
 //   ScanCtx  (ticket, done, epoch)          16 bytes
-//   status[max_tiles]  u64: [63:34] epoch (30 bits) [33:32] flag [31:0] value
-// flag 1 = tile aggregate published, 2 = inclusive prefix published.
-//
-// Tiles are handed out in order through `ticket`, so every predecessor of a
-// waiting tile is already running and the look-back always terminates.  The
-// last wave of a launch (counted through `done`) resets ticket/done and bumps
-// `epoch`; entries left by older launches carry an older epoch and read as
-// "not ready", so no per-launch clearing is needed (also under graph replay).
+//   status[units]  u64: [63:34] epoch (30 bits) [33:32] flag [31:0] value
+// Units are handed out in order through `ticket`, so every predecessor of a
+// waiting unit is already running and every wait ends.  The last wave of a
+// launch (counted through `done`) resets ticket/done and bumps `epoch`;
+// entries left by older launches carry an older epoch and read as "not
+// ready", so no per-launch clearing is needed (also under graph replay).
+// Only the compaction kernels use this; per-tile work never waits on
+// another workgroup.
 // ---------------------------------------------------------------------------
 struct ScanCtx {
     uint32_t ticket;
@@ -154,65 +167,19 @@ __device__ __forceinline__ void release_ticket(ScanCtx* ctx, uint32_t epoch, uin
     }
 }
 
-// Returned by lookback_exclusive when a predecessor never published (only
+// Returned by aggregate_prefix when a predecessor never published (only
 // possible with a workspace that was not initialised or is shared by two
 // concurrent launches).  Callers must then write nothing and flag an error.
 constexpr uint32_t kLookbackFailed = 0xFFFFFFFFu;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
-// Publishes `aggregate` for `tile` and returns the exclusive prefix of all
-// earlier tiles (same value in every lane), or kLookbackFailed.  The spin on a
-// predecessor is bounded, so every wave always finishes.  Whole wave must call.
-__device__ __forceinline__ uint32_t lookback_exclusive(uint64_t* status, uint32_t tile,
-                                                       uint32_t epoch, uint32_t aggregate) {
-    const int l = lane_id();
-    const uint32_t ep = epoch & 0x3FFFFFFFu;
-    if (tile == 0) {
-        if (l == 0) st_status(&status[0], pack_status(epoch, 2u, aggregate));
-        return 0u;
-    }
-    if (l == 0) st_status(&status[tile], pack_status(epoch, 1u, aggregate));
-    uint32_t prefix = 0;
-    int64_t base = (int64_t)tile - 1;  // window covers tiles base, base-1, ..., base-63
-    while (true) {
-        const int64_t j = base - l;
-        uint32_t flag = 2u, val = 0u;  // lanes before tile 0 act as "inclusive 0"
-        bool timed_out = false;
-        if (j >= 0) {
-            uint64_t s;
-            uint32_t spins = 0;
-            while (true) {
-                s = ld_status(&status[j]);
-                if ((uint32_t)(s >> 34) == ep && ((s >> 32) & 3u) != 0u) break;
-                if (++spins > kSpinLimit) { timed_out = true; break; }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            flag = (uint32_t)(s >> 32) & 3u;
-            val = (uint32_t)s;
-        }
-        if (__ballot(timed_out) != 0ull) return kLookbackFailed;
-        // Closest lane (smallest l) holding an inclusive prefix ends the walk.
-        const uint64_t inc = __ballot(flag == 2u);
-        if (inc) {
-            const int stop = __builtin_ctzll(inc);
-            uint32_t contrib = (l <= stop) ? val : 0u;
-            prefix += wave_sum(contrib);
-            break;
-        }
-        prefix += wave_sum(val);
-        base -= 64;
-    }
-    if (l == 0) st_status(&status[tile], pack_status(epoch, 2u, prefix + aggregate));
-    return prefix;
-}
-
 // Exclusive prefix for `unit` from the aggregates of ALL earlier units
-// (published with flag 1, read 64 at a time).  Unlike lookback_exclusive
-// there is no chain of inclusive prefixes: every unit publishes at once and
-// waits only for its predecessors' aggregates, so the latency is
+// (published with flag 1, read 64 at a time).  There is no chain of
+// inclusive prefixes (as in a decoupled look-back): every unit publishes at
+// once and waits only for its predecessors' aggregates, so the latency is
 // ceil(unit/64) rounds of device-scope loads.  Meant for a few hundred
 // coarse units (O(units^2) loads overall).  Whole wave must call; bounded
-// spin, returns kLookbackFailed like lookback_exclusive.
+// spin; returns kLookbackFailed if a predecessor never publishes.
 __device__ __forceinline__ uint32_t aggregate_prefix(uint64_t* status, uint32_t unit, uint32_t epoch,
                                                      uint32_t aggregate) {
     const int l = lane_id();

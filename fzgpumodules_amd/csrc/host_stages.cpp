@@ -201,7 +201,7 @@ void LorenzoQuantStage<TI, TC>::postStreamSync(hipStream_t) {
         computed_abs_eb_ = h_params_[1];
     }
     uint32_t cnt = h_count_;
-    if (cnt == 0xFFFFFFFFu) throw std::runtime_error("LorenzoQuantStage: ordered scan failed (workspace state)");
+    if (cnt == 0xFFFFFFFFu) throw std::runtime_error("LorenzoQuantStage: device prefix failed (workspace state)");
     const size_t max_out = maxOutliers(num_elements_);
     if (cnt > max_out) {
         if (config_.outlier_capacity != 0.0f)
@@ -215,7 +215,7 @@ void LorenzoQuantStage<TI, TC>::postStreamSync(hipStream_t) {
     actual_output_sizes_[2] = (size_t)cnt * sizeof(uint32_t);
     if (config_.strict_error_bound && d_pcount_ptr_) {
         const uint32_t pc = h_pcount_;
-        if (pc == 0xFFFFFFFFu) throw std::runtime_error("LorenzoQuantStage: ordered scan failed (workspace state)");
+        if (pc == 0xFFFFFFFFu) throw std::runtime_error("LorenzoQuantStage: device prefix failed (workspace state)");
         if (pc > maxPatches(num_elements_))
             throw std::runtime_error("LorenzoQuantStage: strict error bound needs " + std::to_string(pc) +
                                      " patched values but has room for " + std::to_string(maxPatches(num_elements_)) +
@@ -359,7 +359,7 @@ void RZEStage::postStreamSync(hipStream_t) {
     if (!size_pending_ && !status_pending_) return;
     completeReadbacks();
     if (size_pending_) {
-        if (h_size_ == 0) throw std::runtime_error("RZEStage: ordered scan failed (workspace state)");
+        if (h_size_ == 0) throw std::runtime_error("RZEStage: device prefix failed (workspace state)");
         actual_output_size_ = h_size_;
     }
     if (status_pending_) {
@@ -410,7 +410,7 @@ void RLEStage<T>::postStreamSync(hipStream_t) {
     if (!size_pending_ && !status_pending_) return;
     completeReadbacks();
     if (size_pending_) {
-        if (h_size_ == 0) throw std::runtime_error("RLEStage: ordered scan failed (workspace state)");
+        if (h_size_ == 0) throw std::runtime_error("RLEStage: device prefix failed (workspace state)");
         actual_output_size_ = h_size_;
     }
     if (status_pending_) {
@@ -497,7 +497,7 @@ void HuffmanStage<T>::postStreamSync(hipStream_t) {
     if (!size_pending_ && !status_pending_) return;
     completeReadbacks();
     if (size_pending_) {
-        if (h_size_ == 0) throw std::runtime_error("HuffmanStage: ordered scan failed (workspace state)");
+        if (h_size_ == 0) throw std::runtime_error("HuffmanStage: device prefix failed (workspace state)");
         actual_output_size_ = h_size_;
     }
     if (status_pending_) {
@@ -592,7 +592,7 @@ void QuantizerStage<TI, TC>::postStreamSync(hipStream_t) {
     if (is_inverse_ || !d_count_ptr_) return;
     completeReadbacks();
     uint32_t cnt = h_count_;
-    if (cnt == 0xFFFFFFFFu) throw std::runtime_error("QuantizerStage: ordered scan failed (workspace state)");
+    if (cnt == 0xFFFFFFFFu) throw std::runtime_error("QuantizerStage: device prefix failed (workspace state)");
     cnt = (uint32_t)std::min<size_t>(cnt, maxOutliers(num_elements_));
     outlier_count_ = cnt;
     actual_output_sizes_[1] = (size_t)cnt * sizeof(TI);

@@ -96,7 +96,7 @@ template <typename TCode, bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256)
 lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, uint64_t nv, float r_host, const float* __restrict__ d_r,
                 float rad, TCode* __restrict__ codes, TileLists tl, uint32_t T) {
-    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
     const int l = lane_id();
@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(256)
 lz2d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n, float r_host,
                 const float* __restrict__ d_r, float rad, TCode* __restrict__ codes, TileLists tl, uint32_t gx,
                 uint32_t T) {
-    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
     const int l = lane_id();
@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(256)
 lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, uint64_t n,
                 float r_host, const float* __restrict__ d_r, float rad, TCode* __restrict__ codes, TileLists tl,
                 Tiles tg, uint32_t T) {
-    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
     const int l = lane_id();
@@ -597,7 +597,7 @@ __global__ void __launch_bounds__(256)
 lz1d_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, const uint32_t* __restrict__ off,
                 const float* __restrict__ serr, const uint32_t* __restrict__ sloc, float* __restrict__ out,
                 uint32_t T) {
-    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const int l = lane_id();
     const int h = l >> 5, lam = l & 31;
@@ -690,7 +690,7 @@ __global__ void __launch_bounds__(256)
 lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float ebx2,
                 const uint32_t* __restrict__ off, const float* __restrict__ serr,
                 const uint32_t* __restrict__ sloc, float* __restrict__ out, uint32_t gx, uint32_t T) {
-    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const int l = lane_id();
     const int x = l & 31, h = l >> 5;
@@ -745,7 +745,7 @@ __global__ void __launch_bounds__(256)
 lz3d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, uint32_t nz, float ebx2,
                 const uint32_t* __restrict__ off, const float* __restrict__ serr,
                 const uint32_t* __restrict__ sloc, float* __restrict__ out, Tiles tg, uint32_t T) {
-    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const int l = lane_id();
     const int x = l & 7, z = l >> 3;

@@ -271,8 +271,7 @@ rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restr
         }
         const uint32_t clean = size & 0x7FFFFFFFu;
 
-        // -- assemble the chunk stream in LDS (before the look-back so the
-        //    wait overlaps useful work)
+        // -- assemble the chunk stream in LDS
         if (size == 2) {
             if (l == 0) { st[0] = (uint8_t)(in_size & 0xFF); st[1] = (uint8_t)((in_size >> 8) & 0xFF); }
         } else if (raw) {
