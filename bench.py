@@ -141,7 +141,7 @@ def main():
 
     # ---- per-kernel device time, HIP events on the launch stream ------------
     kern = kernel_times(args.kernel_reps, x, d_in, abs_eb)
-    dom = max((k for k in kern if k != "copy_reference"), key=lambda k: kern[k]["ms"])
+    dom = max((k for k in kern if kern[k]["pipeline"]), key=lambda k: kern[k]["ms"])
     d = kern[dom]
     roof = {
         "kernel": dom,
@@ -152,8 +152,9 @@ def main():
         "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
         "traffic": None,
     }
-    chain_bytes = sum(kern[k]["bytes"] for k in ("value_base", "lorenzo_fwd_2d", "bitshuffle_enc", "rze_enc"))
-    chain_ms = sum(kern[k]["ms"] for k in ("value_base", "lorenzo_fwd_2d", "bitshuffle_enc", "rze_enc"))
+    chain = ("value_base", "lorenzo_fwd_2d", "bitshuffle_rze_enc")
+    chain_bytes = sum(kern[k]["bytes"] for k in chain)
+    chain_ms = sum(kern[k]["ms"] for k in chain)
 
     # ---- CPU oracle baseline --------------------------------------------------
     cpu = None if args.no_cpu else cpu_baseline(x, args.cpu_runs)
@@ -218,7 +219,11 @@ def kernel_times(reps, x, d_in, abs_eb):
     err = DeviceBuffer(4 * m)
     idx = DeviceBuffer(4 * m)
     cnt = DeviceBuffer(16)
-    lz_ws = workspace(L.fzh_lorenzo_quant_fwd_ws_bytes(n, NX, NY, 1))
+    lz_ws = workspace(L.fzh_lorenzo_quant_fwd_strict_ws_bytes(n, NX, NY, 1))
+    mp = max(256, math.ceil(n * 0.01))
+    pval = DeviceBuffer(4 * mp)
+    pidx = DeviceBuffer(4 * mp)
+    pcnt = DeviceBuffer(16)
     shuf = DeviceBuffer(2 * n)
     rze_cap = L.fzh_rze_enc_bound(2 * n)
     rze_out = DeviceBuffer(rze_cap)
@@ -237,8 +242,10 @@ def kernel_times(reps, x, d_in, abs_eb):
         call("fzh_value_base_f32", st, d_in.ptr, nv, 1, np.float32(USER_EB), params.ptr, vb_ws.ptr, vb_ws.nbytes)
 
     def lzf():
-        call("fzh_lorenzo_quant_fwd_f32", st, d_in.ptr, n, NX, NY, 1, np.float32(0), params.ptr + 8, 32768, 1, 2,
-             codes.ptr, err.ptr, idx.ptr, cnt.ptr, m, lz_ws.ptr, lz_ws.nbytes)
+        # the strict variant, as the benchmarked pipeline runs it
+        call("fzh_lorenzo_quant_fwd_strict_f32", st, d_in.ptr, n, NX, NY, 1, np.float32(0), params.ptr + 8, 32768,
+             1, 2, codes.ptr, err.ptr, idx.ptr, cnt.ptr, m, np.float32(0), np.float32(0), params.ptr, pval.ptr,
+             pidx.ptr, pcnt.ptr, mp, lz_ws.ptr, lz_ws.nbytes)
 
     def bse():
         call("fzh_bitshuffle_enc", st, codes.ptr, shuf.ptr, 2 * n, 16384, 2)
@@ -251,6 +258,18 @@ def kernel_times(reps, x, d_in, abs_eb):
 
     def bsd():
         call("fzh_bitshuffle_dec", st, shuf.ptr, unshuf.ptr, 2 * n, 16384, 2)
+
+    # the fused pair the pipeline runs for Bitshuffle -> RZE
+    def frze():
+        call("fzh_rze_enc_shuffled", st, codes.ptr, 2 * n, 2, rze_out.ptr, rze_cap, rze_sz.ptr, rze_ws.ptr,
+             rze_ws.nbytes)
+
+    def frzd():
+        call("fzh_rze_dec_shuffled", st, rze_out.ptr, int(csz[0]), unshuf.ptr, 2 * n, 2, status.ptr, rzd_ws.ptr,
+             rzd_ws.nbytes)
+
+    def patch():
+        call("fzh_lorenzo_apply_patches", st, y.ptr, pval.ptr, pidx.ptr, pcnt.ptr, mp, nv)
 
     def lzi():
         call("fzh_lorenzo_quant_inv_f32", st, unshuf.ptr, n, NX, NY, 1, err.ptr, idx.ptr, cnt.ptr, m, ebx2, 1, 2,
@@ -280,21 +299,27 @@ def kernel_times(reps, x, d_in, abs_eb):
     rze()
     call("fzh_stream_sync", st)
     k = int(cnt.download(np.uint32, 1)[0])
+    kp = min(int(pcnt.download(np.uint32, 1)[0]), mp)
     csz = rze_sz.download(np.uint64, 1)
     C_rze = int(csz[0])
+    # (name, launch, algorithmic bytes per launch, run by the pipeline)
     specs = [
-        ("value_base", vb, 4 * nv),
-        ("lorenzo_fwd_2d", lzf, 4 * NX * NY + 2 * n + 8 * k + 4),
-        ("bitshuffle_enc", bse, 4 * n),
-        ("rze_enc", rze, 2 * n + C_rze),
-        ("rze_dec", rzd, C_rze + 2 * n),
-        ("bitshuffle_dec", bsd, 4 * n),
-        ("lorenzo_inv_2d", lzi, 2 * n + 8 * k + 4 + 4 * NX * NY),
-        ("copy_reference", cpy, 8 * nv),
+        ("value_base", vb, 4 * nv, True),
+        ("lorenzo_fwd_2d", lzf, 4 * NX * NY + 2 * n + 8 * k + 8 * kp + 8, True),
+        ("bitshuffle_rze_enc", frze, 2 * n + C_rze, True),
+        ("rze_bitshuffle_dec", frzd, C_rze + 2 * n, True),
+        ("lorenzo_inv_2d", lzi, 2 * n + 8 * k + 4 + 4 * NX * NY, True),
+        ("apply_patches", patch, 8 * kp + 4 * kp, True),
+        ("bitshuffle_enc", bse, 4 * n, False),
+        ("rze_enc", rze, 2 * n + C_rze, False),
+        ("rze_dec", rzd, C_rze + 2 * n, False),
+        ("bitshuffle_dec", bsd, 4 * n, False),
+        ("copy_reference", cpy, 8 * nv, False),
     ]
-    for name, fn, nb in specs:
+    for name, fn, nb, used in specs:
         ms = timed(fn)
-        out[name] = {"ms": ms, "bytes": int(nb), "gbs": nb / (ms * 1e-3) / 1e9, "frac": nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        out[name] = {"ms": ms, "bytes": int(nb), "gbs": nb / (ms * 1e-3) / 1e9,
+                     "frac": nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "pipeline": used}
         log(f"[bench] kernel {name:16s} {ms * 1e3:8.1f} us  {out[name]['gbs']:7.1f} GB/s  "
             f"{100 * out[name]['frac']:5.1f}% of HBM peak")
     st_ = int(status.download(np.int32, 1)[0])

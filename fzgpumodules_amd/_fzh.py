@@ -101,6 +101,19 @@ SIGNATURES = {
     "fzp_num_outputs": (_I, [_P, C.POINTER(_I)]),
     "fzp_output_info": (_I, [_P, _I, C.c_char_p, _Z, C.POINTER(_Z)]),
     "fzp_stage_abs_error_bound": (_I, [_P, _I, C.POINTER(_F)]),
+    "fzb_create": (_I, [_P, C.POINTER(_P)]),
+    "fzb_destroy": (_I, [_P]),
+    "fzb_set_fields": (_I, [_P, _P, _Z]),
+    "fzb_compress": (_I, [_P, _P]),
+    "fzb_output": (_I, [_P, C.POINTER(_P), C.POINTER(C.c_uint64)]),
+    "fzb_field_result": (_I, [_P, _Z, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "fzb_totals": (_I, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "fzb_set_launch": (_I, [_P, _P]),
+    "fzb_get_launch": (_I, [_P, _P]),
+    "fzb_autotune": (_I, [_P, _P, _I, _P]),
+    "fzb_enable_graph_mode": (_I, [_P, _I]),
+    "fzb_enable_profiling": (_I, [_P, _I]),
+    "fzb_stage_ms": (_I, [_P, _I, C.c_char_p, _Z, C.POINTER(_F)]),
     "fzh_device_alloc": (_I, [C.POINTER(_P), _Z]),
     "fzh_device_free": (_I, [_P]),
     "fzh_memcpy_htod": (_I, [_P, _P, _Z]),

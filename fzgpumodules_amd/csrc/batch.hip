@@ -1,0 +1,436 @@
+// batch.hip — persistent, table-driven kernels of the batched compressor
+// (LorenzoQuant -> Bitshuffle -> RZE over many fields at once), gfx950.
+//
+// No counterpart in the reference (one field per Pipeline::compress,
+// src/pipeline/compressor_exec.cpp).  Every kernel here runs a fixed grid
+// that walks a global index space (tiles, chunks or value-base units of all
+// fields, fzh_batch.h) and does the per-tile / per-chunk work with the SAME
+// device functions as the single-field kernels (fzh_lorenzo_tile.h,
+// fzh_rze_chunk.h), so each field's bytes equal a single-field run's.
+//
+// Launch sequence of one compress (all on one stream, captured as a graph):
+//   value_base  per-unit min/max, then one wave per field resolves eb
+//   lorenzo     codes + per-tile lists (kListSlots entries) + per-tile counts
+//   rze         fused Bitshuffle+RZE per chunk into a chunk scratch slot
+//   scan        one exclusive scan (mod 2^32) over [tile outliers | tile
+//               patches | chunk bytes]; per-field offsets are differences
+//   finalize    one thread per field: result layout, concat header, pads
+//   pack        chunks to their place in the field's RZE stream
+//   scatter     tile lists to their place in the field's outlier / patch
+//               arrays; a tile whose list outgrew kListSlots is recomputed
+#include "fzh_batch.h"
+#include "fzh_lorenzo_tile.h"
+#include "fzh_rze_chunk.h"
+#include "fzh_scan.h"
+
+namespace fzh {
+namespace batch {
+namespace {
+
+// Largest f with start[f] <= g (start strictly increasing, start[F] = total).
+__device__ __forceinline__ uint32_t find_field(const uint32_t* __restrict__ start, uint32_t F, uint32_t g) {
+    uint32_t lo = 0, hi = F;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (start[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// value base: per-unit min/max (first element and the zero padding folded in
+// exactly as the single-field reduction does), then one wave per field
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) vb_partial_kernel(Plan p) {
+    __shared__ float smn[4], smx[4];
+    const int w = threadIdx.x >> 6;
+    for (uint32_t u = blockIdx.x; u < p.U; u += gridDim.x) {
+        const uint32_t f = find_field(p.unit0, p.F, u);
+        const Field& fd = p.fields[f];
+        const float* in = fd.in;
+        const uint64_t b = (uint64_t)(u - fd.unit0) * kVbUnit;
+        const uint64_t e = min(fd.nv, b + kVbUnit);
+        const float first = in[0];
+        float mn = first, mx = first;
+        if (fd.nv < fd.n) {
+            if (0.0f < mn) mn = 0.0f;
+            if (0.0f > mx) mx = 0.0f;
+        }
+        const uint64_t e4 = b + ((e - b) & ~3ull);
+        for (uint64_t i = b + 4 * (uint64_t)threadIdx.x; i < e4; i += 1024) {
+            const float4 v = *reinterpret_cast<const float4*>(in + i);
+            if (v.x < mn) mn = v.x;
+            if (v.x > mx) mx = v.x;
+            if (v.y < mn) mn = v.y;
+            if (v.y > mx) mx = v.y;
+            if (v.z < mn) mn = v.z;
+            if (v.z > mx) mx = v.z;
+            if (v.w < mn) mn = v.w;
+            if (v.w > mx) mx = v.w;
+        }
+        for (uint64_t i = e4 + threadIdx.x; i < e; i += 256) {
+            const float v = in[i];
+            if (v < mn) mn = v;
+            if (v > mx) mx = v;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const float a = __shfl_xor(mn, off, 64), c = __shfl_xor(mx, off, 64);
+            if (a < mn) mn = a;
+            if (c > mx) mx = c;
+        }
+        if (lane_id() == 0) { smn[w] = mn; smx[w] = mx; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < 4; i++) {
+                if (smn[i] < mn) mn = smn[i];
+                if (smx[i] > mx) mx = smx[i];
+            }
+            p.vb_part[u] = make_float2(mn, mx);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) vb_finalize_kernel(Plan p) {
+    const uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= p.F) return;
+    float gmn = INFINITY, gmx = -INFINITY;
+    if (p.mode != 0) {
+        const Field& fd = p.fields[f];
+        for (uint32_t u = lane_id(); u < fd.nunits; u += 64) {
+            const float2 v = p.vb_part[fd.unit0 + u];
+            if (v.x < gmn) gmn = v.x;
+            if (v.y > gmx) gmx = v.y;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const float a = __shfl_xor(gmn, off, 64), c = __shfl_xor(gmx, off, 64);
+            if (a < gmn) gmn = a;
+            if (c > gmx) gmx = c;
+        }
+    } else {
+        gmn = gmx = 0.0f;  // ABS: value base 0 -> abs_eb = user_eb
+    }
+    if (lane_id() == 0) resolve_eb(gmn, gmx, p.mode == 0 ? 1 : p.mode, p.user_eb, p.params + 4 * (size_t)f);
+}
+
+// ---------------------------------------------------------------------------
+// Lorenzo forward over all tiles of all fields
+// ---------------------------------------------------------------------------
+template <bool ZZ, bool STRICT>
+__device__ __forceinline__ TileCounts run_tile(const Plan& p, const Field& fd, uint32_t f, uint32_t lt,
+                                               uint16_t* codes, ListOut lo, ListOut lp) {
+    const float* prm = p.params + 4 * (size_t)f;
+    const float r = prm[2];
+    const StrictEb se{prm[1], prm[3]};
+    const bool last = lt + 1 == fd.ntiles;
+    if (fd.ndim == 1)
+        return lz1d_tile<uint16_t, ZZ, STRICT>(fd.in, fd.n, fd.nv, lt, r, p.rad, codes, lo, lp, se);
+    if (fd.ndim == 2)
+        return lz2d_tile<uint16_t, ZZ, STRICT>(fd.in, fd.nx, fd.ny, fd.n, lt, fd.gx, last, r, p.rad, codes, lo, lp,
+                                               se);
+    return lz3d_tile<uint16_t, ZZ, STRICT>(fd.in, fd.nx, fd.ny, fd.nz, fd.n, lt, Tiles{fd.gx, fd.gy, 0}, last, r,
+                                           p.rad, codes, lo, lp, se);
+}
+
+template <bool ZZ, bool STRICT>
+__global__ void __launch_bounds__(256) lz_batch_kernel(Plan p, uint32_t G) {
+    const uint32_t waves = gridDim.x * 4;
+    const uint32_t nu = (p.T + G - 1) / G;
+    for (uint32_t u = blockIdx.x * 4 + (threadIdx.x >> 6); u < nu; u += waves) {
+        const uint32_t g0 = u * G, g1 = min(p.T, g0 + G);
+        uint32_t f = find_field(p.tile0, p.F, g0);
+        for (uint32_t g = g0; g < g1; g++) {
+            while (g >= p.tile0[f + 1]) f++;
+            const Field& fd = p.fields[f];
+            const uint64_t sl = (uint64_t)g * kListSlots;
+            const ListOut lo{p.lerr + sl, p.lidx + sl, kListSlots};
+            const ListOut lp = STRICT ? ListOut{p.lpval + sl, p.lpidx + sl, kListSlots} : ListOut{nullptr, nullptr, 0};
+            const TileCounts tc = run_tile<ZZ, STRICT>(p, fd, f, g - fd.tile0, p.codes + fd.codes_off, lo, lp);
+            if (lane_id() == 0) {
+                p.counts[g] = tc.cnt;
+                if constexpr (STRICT) p.counts[p.T + g] = tc.pcnt;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused Bitshuffle + RZE over all chunks of all fields
+// ---------------------------------------------------------------------------
+constexpr int kRzeWaves = 2;
+constexpr size_t kRzeLds = (size_t)kRzeWaves * LDS_PER_WAVE;
+
+__global__ void __launch_bounds__(64 * kRzeWaves) rze_batch_kernel(Plan p) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wv = threadIdx.x >> 6;
+    uint8_t* st = smem + wv * LDS_PER_WAVE;
+    uint8_t* bm1 = st + LDS_STREAM;
+    const uint32_t waves = gridDim.x * kRzeWaves;
+    const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
+    for (uint32_t gc = blockIdx.x * kRzeWaves + wv; gc < p.NC; gc += waves) {
+        const uint32_t f = find_field(p.chunk0, p.F, gc);
+        const Field& fd = p.fields[f];
+        const uint32_t lc = gc - fd.chunk0;
+        const uint64_t cb = 2 * fd.n;  // code bytes
+        const uint64_t off = (uint64_t)lc * kChunk;
+        const int in_size = (int)min((uint64_t)kChunk, cb - off);
+        const uint8_t* src = reinterpret_cast<const uint8_t*>(p.codes + fd.codes_off) + off;
+        const uint32_t size = rze_encode_chunk<2>(src, in_size, off + kChunk <= cb, st, bm1,
+                                                  p.chunk_scratch + (uint64_t)gc * kChunk);
+        if (lane_id() == 0) {
+            p.sizes[gc] = size;
+            p.counts[cbase + gc] = size & 0x7FFFFFFFu;
+            uint32_t* rz = reinterpret_cast<uint32_t*>(p.out + fd.out_off + p.hdr);
+            rz[2 + lc] = size;
+            if (lc == 0) {
+                rz[0] = (uint32_t)cb;
+                rz[1] = fd.nchunks;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-field layout of the result (fz::Pipeline's concat, fzh_core.hip)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t pad16(uint64_t v) { return (v + 15) & ~15ull; }
+
+__device__ __forceinline__ void zero_range(uint8_t* b, uint64_t from, uint64_t to) {
+    for (uint64_t i = from; i < to; i++) b[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) finalize_kernel(Plan p) {
+    const uint32_t f = blockIdx.x * 256 + threadIdx.x;
+    if (f >= p.F) return;
+    const Field& fd = p.fields[f];
+    const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
+    const uint32_t ex_t = p.excl[fd.tile0];
+    const uint32_t k = p.excl[fd.tile0 + fd.ntiles] - ex_t;
+    uint32_t ex_p = 0, kp = 0;
+    if (p.strict) {
+        ex_p = p.excl[p.T + fd.tile0];
+        kp = p.excl[p.T + fd.tile0 + fd.ntiles] - ex_p;
+    }
+    const uint32_t ex_c = p.excl[cbase + fd.chunk0];
+    const uint64_t csum = (uint32_t)(p.excl[cbase + fd.chunk0 + fd.nchunks] - ex_c);
+    const uint64_t rze_len = 8ull + 4ull * fd.nchunks + csum;
+    const uint64_t rze_bytes = (rze_len + 3) & ~3ull;
+    const uint32_t ko = min(k, fd.max_out), kpo = min(kp, fd.max_patch);
+    uint64_t sz[7] = {rze_bytes, 4ull * ko, 4ull * ko, 4, 4ull * kpo, 4ull * kpo, 4};
+    uint64_t off[7];
+    uint64_t o = p.hdr;
+    for (uint32_t i = 0; i < p.nout; i++) {
+        off[i] = o;
+        o += pad16(sz[i]);
+    }
+    Result res;
+    res.bytes = o;
+    res.k = k;
+    res.kp = kp;
+    res.flags = (k > fd.max_out ? kOverflowOutliers : 0u) | (kp > fd.max_patch ? kOverflowPatches : 0u);
+    res.pad[0] = res.pad[1] = res.pad[2] = 0;
+    Derived d;
+    d.o_err = off[1];
+    d.o_idx = off[2];
+    d.o_pval = p.strict ? off[4] : 0;
+    d.o_pidx = p.strict ? off[5] : 0;
+    d.ko = ko;
+    d.kpo = kpo;
+    d.ex_t = ex_t;
+    d.ex_p = ex_p;
+    d.ex_c = ex_c;
+    d.pad = 0;
+    if (o > fd.out_cap) {
+        res.flags |= kOverflowCapacity;
+    } else {
+        uint8_t* b = p.out + fd.out_off;
+        uint32_t* b32 = reinterpret_cast<uint32_t*>(b);
+        b32[0] = p.nout;
+        for (uint32_t i = 0; i < p.nout; i++) {
+            b32[1 + 2 * i] = (uint32_t)sz[i];
+            b32[2 + 2 * i] = (uint32_t)(sz[i] >> 32);
+        }
+        zero_range(b, 4 + 8ull * p.nout, p.hdr);
+        zero_range(b, off[0] + rze_len, off[0] + pad16(rze_bytes));
+        for (uint32_t i = 1; i < p.nout; i++) zero_range(b, off[i] + sz[i], off[i] + pad16(sz[i]));
+        b32[off[3] / 4] = k;
+        if (p.strict) b32[off[6] / 4] = kp;
+    }
+    p.derived[f] = d;
+    p.results[f] = res;
+}
+
+// ---------------------------------------------------------------------------
+// Chunks to their place in the field's RZE stream
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) pack_batch_kernel(Plan p) {
+    const uint32_t waves = gridDim.x * 4;
+    const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
+    for (uint32_t gc = blockIdx.x * 4 + (threadIdx.x >> 6); gc < p.NC; gc += waves) {
+        const uint32_t f = find_field(p.chunk0, p.F, gc);
+        if (p.results[f].flags & kOverflowCapacity) continue;
+        const Field& fd = p.fields[f];
+        const uint32_t lc = gc - fd.chunk0;
+        const uint32_t size = p.sizes[gc];
+        const uint32_t clean = size & 0x7FFFFFFFu;
+        const bool raw = (size & 0x80000000u) != 0u;
+        const uint64_t cb = 2 * fd.n;
+        const uint64_t off = (uint64_t)lc * kChunk;
+        const bool shuffled = off + kChunk <= cb;
+        uint8_t* dst = p.out + fd.out_off + p.hdr + 8ull + 4ull * fd.nchunks +
+                       (uint32_t)(p.excl[cbase + gc] - p.derived[f].ex_c);
+        if (raw && !shuffled)
+            copy_to_unaligned16(reinterpret_cast<const uint8_t*>(p.codes + fd.codes_off) + off, cb - off, dst, clean);
+        else
+            copy_to_unaligned16(p.chunk_scratch + (uint64_t)gc * kChunk, kChunk, dst, clean);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile lists to the field's outlier / patch arrays.  A wave takes 64 tiles,
+// finds those with a non-empty list by ballot and places each; a list longer
+// than kListSlots is produced again by recomputing the tile (lists only).
+// ---------------------------------------------------------------------------
+template <bool ZZ, bool STRICT>
+__global__ void __launch_bounds__(256) scatter_batch_kernel(Plan p) {
+    const uint32_t waves = gridDim.x * 4;
+    const int l = lane_id();
+    const uint32_t nblk = (p.T + 63) / 64;
+    for (uint32_t blk = blockIdx.x * 4 + (threadIdx.x >> 6); blk < nblk; blk += waves) {
+        const uint32_t gl = blk * 64 + (uint32_t)l;
+        uint32_t c = 0, pc = 0;
+        if (gl < p.T) {
+            c = p.counts[gl];
+            if constexpr (STRICT) pc = p.counts[p.T + gl];
+        }
+        uint64_t todo = __ballot((c | pc) != 0u);
+        while (todo) {
+            const int j = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint32_t g = blk * 64 + (uint32_t)j;
+            const uint32_t tc = __shfl(c, j, 64), tp = __shfl(pc, j, 64);
+            const uint32_t f = find_field(p.tile0, p.F, g);
+            if (p.results[f].flags & kOverflowCapacity) continue;
+            const Field& fd = p.fields[f];
+            const Derived d = p.derived[f];
+            uint8_t* b = p.out + fd.out_off;
+            float* err = reinterpret_cast<float*>(b + d.o_err);
+            uint32_t* idx = reinterpret_cast<uint32_t*>(b + d.o_idx);
+            const uint64_t sl = (uint64_t)g * kListSlots;
+            const uint32_t po = p.excl[g] - d.ex_t;
+            bool redo_o = false, redo_p = false;
+            if (tc <= kListSlots) {
+                if ((uint32_t)l < tc && po + l < d.ko) {
+                    err[po + l] = p.lerr[sl + l];
+                    idx[po + l] = p.lidx[sl + l];
+                }
+            } else {
+                redo_o = po < d.ko;
+            }
+            uint32_t pp = 0;
+            float* pval = nullptr;
+            uint32_t* pidx = nullptr;
+            if constexpr (STRICT) {
+                pval = reinterpret_cast<float*>(b + d.o_pval);
+                pidx = reinterpret_cast<uint32_t*>(b + d.o_pidx);
+                pp = p.excl[p.T + g] - d.ex_p;
+                if (tp <= kListSlots) {
+                    if ((uint32_t)l < tp && pp + l < d.kpo) {
+                        pval[pp + l] = p.lpval[sl + l];
+                        pidx[pp + l] = p.lpidx[sl + l];
+                    }
+                } else {
+                    redo_p = pp < d.kpo;
+                }
+            }
+            if (redo_o || redo_p) {
+                const ListOut lo = redo_o ? ListOut{err + po, idx + po, d.ko - po} : ListOut{nullptr, nullptr, 0};
+                const ListOut lp = redo_p ? ListOut{pval + pp, pidx + pp, d.kpo - pp} : ListOut{nullptr, nullptr, 0};
+                (void)run_tile<ZZ, STRICT>(p, fd, f, g - fd.tile0, nullptr, lo, lp);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) readback_batch_kernel(const uint32_t* __restrict__ src, uint32_t* dst,
+                                                             uint64_t words) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (uint64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+}  // namespace
+
+size_t scan_ws_bytes_for(size_t n_counts) { return fzh::scan_ws_bytes(n_counts); }
+
+int launch_value_base(hipStream_t s, const Plan& p, uint32_t grid) {
+    if (p.mode != 0 && p.U) vb_partial_kernel<<<std::max<uint32_t>(1, std::min(grid, p.U)), 256, 0, s>>>(p);
+    vb_finalize_kernel<<<(unsigned)cdiv(p.F, 4), 256, 0, s>>>(p);
+    FZH_CHECK_LAUNCH("batch value base");
+    return FZH_OK;
+}
+
+int launch_lorenzo(hipStream_t s, const Plan& p, uint32_t grid, uint32_t G) {
+    G = std::max<uint32_t>(1, G);
+    const uint32_t nu = (uint32_t)cdiv(p.T, G);
+    const unsigned g = std::max<uint32_t>(1, std::min<uint32_t>(grid, (uint32_t)cdiv(nu, 4)));
+    if (p.zigzag) {
+        if (p.strict) lz_batch_kernel<true, true><<<g, 256, 0, s>>>(p, G);
+        else lz_batch_kernel<true, false><<<g, 256, 0, s>>>(p, G);
+    } else {
+        if (p.strict) lz_batch_kernel<false, true><<<g, 256, 0, s>>>(p, G);
+        else lz_batch_kernel<false, false><<<g, 256, 0, s>>>(p, G);
+    }
+    FZH_CHECK_LAUNCH("batch lorenzo");
+    return FZH_OK;
+}
+
+int launch_rze(hipStream_t s, const Plan& p, uint32_t grid) {
+    const unsigned g = std::max<uint32_t>(1, std::min<uint32_t>(grid, (uint32_t)cdiv(p.NC, kRzeWaves)));
+    rze_batch_kernel<<<g, 64 * kRzeWaves, kRzeLds, s>>>(p);
+    FZH_CHECK_LAUNCH("batch rze");
+    return FZH_OK;
+}
+
+int launch_scan(hipStream_t s, const Plan& p) {
+    return fzh::scan_u32<true>(s, p.counts, p.excl, p.n_counts, nullptr, p.excl + p.n_counts, p.scan_ws);
+}
+
+int launch_finalize(hipStream_t s, const Plan& p) {
+    finalize_kernel<<<(unsigned)cdiv(p.F, 256), 256, 0, s>>>(p);
+    FZH_CHECK_LAUNCH("batch finalize");
+    return FZH_OK;
+}
+
+int launch_pack(hipStream_t s, const Plan& p, uint32_t grid) {
+    const unsigned g = std::max<uint32_t>(1, std::min<uint32_t>(grid, (uint32_t)cdiv(p.NC, 4)));
+    pack_batch_kernel<<<g, 256, 0, s>>>(p);
+    FZH_CHECK_LAUNCH("batch pack");
+    return FZH_OK;
+}
+
+int launch_scatter(hipStream_t s, const Plan& p, uint32_t grid) {
+    const unsigned g = std::max<uint32_t>(1, std::min<uint32_t>(grid, (uint32_t)cdiv(cdiv(p.T, 64), 4)));
+    if (p.zigzag) {
+        if (p.strict) scatter_batch_kernel<true, true><<<g, 256, 0, s>>>(p);
+        else scatter_batch_kernel<true, false><<<g, 256, 0, s>>>(p);
+    } else {
+        if (p.strict) scatter_batch_kernel<false, true><<<g, 256, 0, s>>>(p);
+        else scatter_batch_kernel<false, false><<<g, 256, 0, s>>>(p);
+    }
+    FZH_CHECK_LAUNCH("batch scatter");
+    return FZH_OK;
+}
+
+int launch_readback(hipStream_t s, const Plan& p, void* dst) {
+    const uint64_t words = (uint64_t)p.F * sizeof(Result) / 4;
+    readback_batch_kernel<<<(unsigned)std::max<uint64_t>(1, std::min<uint64_t>(cdiv(words, 256), 64)), 256, 0, s>>>(
+        reinterpret_cast<const uint32_t*>(p.results), reinterpret_cast<uint32_t*>(dst), words);
+    FZH_CHECK_LAUNCH("batch readback");
+    return FZH_OK;
+}
+
+}  // namespace batch
+}  // namespace fzh

@@ -1,0 +1,413 @@
+// fzh_lorenzo_tile.h — one Lorenzo predict+quantize tile on one wave64.
+//
+// Shared by the single-field kernels (lorenzo.hip) and the batched
+// persistent kernels (batch.hip), so a field's codes and outlier lists are
+// the same whichever launch produced them.
+//
+// Format geometry follows the reference exactly: 1D tiles of 1024 elements
+// (modules/fused/lorenzo_quant/lorenzo_quant.cu:262-364), 2D tiles of 32x32
+// (lorenzo_quant_nd.cu:29-105), 3D tiles of 8x8x8 (lorenzo_quant_nd.cu:199-274).
+// Lane layouts are chosen for wave64:
+//   1D  lane l, step q: elements 4(l+64q) .. 4(l+64q)+3 of the tile
+//   2D  lane l: column x = l&31, rows 16*(l>>5) .. +15 of the tile
+//   3D  lane l: x = l&7, z = l>>3, rows y = 0..7 of the tile
+// The arithmetic per element is the reference's (same FP32 operation order).
+// A tile's outliers (and strict-mode patches) are listed in element-index
+// order inside the tile.
+#pragma once
+
+#include "fzh_internal.h"
+
+namespace fzh {
+namespace {
+
+template <typename TCode> struct SignedOf;
+template <> struct SignedOf<uint8_t> { using type = int8_t; };
+template <> struct SignedOf<uint16_t> { using type = int16_t; };
+
+// (TCode)(int)delta, or zigzag((SCode)(int)delta)   (lorenzo_quant.cu:328-334)
+template <typename TCode, bool ZZ>
+__device__ __forceinline__ TCode enc_code(float d) {
+    using SC = typename SignedOf<TCode>::type;
+    constexpr int W = 8 * sizeof(TCode);
+    const int q = (int)d;
+    if constexpr (ZZ) {
+        const SC s = (SC)q;
+        return (TCode)(((TCode)s << 1) ^ (TCode)(s >> (W - 1)));
+    } else {
+        return (TCode)q;
+    }
+}
+
+// (TIn)(SCode)code, or zigzag-decode  (lorenzo_quant.cu:195-202)
+template <typename TCode, bool ZZ>
+__device__ __forceinline__ float dec_code(TCode c) {
+    using SC = typename SignedOf<TCode>::type;
+    if constexpr (ZZ) {
+        return (float)(SC)((TCode)(c >> 1) ^ (TCode)(-(SC)(c & 1u)));
+    } else {
+        return (float)(SC)c;
+    }
+}
+
+__device__ __forceinline__ float quant_round(float x, float r) { return __builtin_roundf(x * r); }
+
+// Strict error bound (an extension; no counterpart in the reference).  The
+// inverse reconstructs element i as fl(S_i * ebx2), S_i the exact integer
+// prefix of the tile's deltas, whenever every partial sum of the inverse's
+// addition tree is exact.  Partial sums are sums of deltas over sub-boxes of
+// the tile, i.e. signed sums of at most 8 quantised values, so |s| < 2^21
+// across the tile keeps them below 2^24 (exact in FP32).  An element is
+// patched (its original value stored) when |fl(s*ebx2) - x| > eb, or when
+// its tile breaks the exactness condition (then every element of the tile).
+constexpr float kExactS = 2097152.0f;  // 2^21
+
+__device__ __forceinline__ bool needs_patch(float x, float s, float eb, float ebx2) {
+    const float xh = s * ebx2;
+    const double e = fabs((double)xh - (double)x);
+    return !(e <= (double)eb);  // NaN compares false: patched
+}
+
+// Destination of one tile's list: entry j of the tile's list goes to
+// val/idx[j] when j < cap (entries past cap are counted, not stored).
+struct ListOut {
+    float* val;
+    uint32_t* idx;
+    uint32_t cap;
+};
+
+struct TileCounts {
+    uint32_t cnt;   // outliers of the tile
+    uint32_t pcnt;  // strict-mode patches of the tile
+};
+
+// Strict-mode bound of a tile (eb, 2*eb as the inverse uses them).
+struct StrictEb {
+    float eb, ebx2;
+};
+
+// Zero codes[from, to) (tail beyond nx*ny[*nz] when the input was padded).
+template <typename TCode>
+__device__ __forceinline__ void zero_code_tail(TCode* codes, uint64_t from, uint64_t to) {
+    for (uint64_t i = from + lane_id(); i < to; i += 64) codes[i] = (TCode)0;
+}
+
+// ---------------------------------------------------------------------------
+// 1D: tile `tile` of a field of n (padded) elements, nv of them in memory
+// (elements in [nv, n) read as zero).  codes == nullptr: lists only.
+// ---------------------------------------------------------------------------
+template <typename TCode, bool ZZ, bool STRICT>
+__device__ __forceinline__ TileCounts lz1d_tile(const float* __restrict__ in, uint64_t n, uint64_t nv,
+                                                uint32_t tile, float r, float rad, TCode* __restrict__ codes,
+                                                ListOut lo, ListOut lp, StrictEb se) {
+    const int l = lane_id();
+    const uint64_t base = (uint64_t)tile * 1024;
+    const bool full = base + 1024 <= nv;
+    float s[4][4], xv[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
+        if (full) {
+            const float4 v = *reinterpret_cast<const float4*>(in + e0);
+            xv[q][0] = v.x;
+            xv[q][1] = v.y;
+            xv[q][2] = v.z;
+            xv[q][3] = v.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) xv[q][k] = (e0 + k < nv) ? in[e0 + k] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[q][k] = quant_round(xv[q][k], r);
+    }
+    // delta_i = s_i - s_{i-1}, chain restarts at the tile start.
+    float d[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float prev = __shfl_up(s[q][3], 1, 64);
+        const float last_prev_q = __shfl(q > 0 ? s[q > 0 ? q - 1 : 0][3] : 0.0f, 63, 64);
+        if (l == 0) prev = (q > 0) ? last_prev_q : 0.0f;
+        d[q][0] = s[q][0] - prev;
+#pragma unroll
+        for (int k = 1; k < 4; k++) d[q][k] = s[q][k] - s[q][k - 1];
+    }
+    TCode c[4][4];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
+            const bool ok = fabsf(d[q][k]) < rad;
+            c[q][k] = ok ? enc_code<TCode, ZZ>(d[q][k]) : (TCode)0;
+            if (!ok && e < n) mask |= 1u << (q * 4 + k);
+        }
+    if (codes) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
+            if (base + 1024 <= n) {
+                if constexpr (sizeof(TCode) == 2) {
+                    uint2 w;
+                    w.x = (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 16);
+                    w.y = (uint32_t)c[q][2] | ((uint32_t)c[q][3] << 16);
+                    *reinterpret_cast<uint2*>(codes + e0) = w;
+                } else {
+                    *reinterpret_cast<uint32_t*>(codes + e0) =
+                        (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 8) | ((uint32_t)c[q][2] << 16) |
+                        ((uint32_t)c[q][3] << 24);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (e0 + k < n) codes[e0 + k] = c[q][k];
+            }
+        }
+    }
+    // Tile-local lists in index order: key (q, lane, k).
+    auto emit = [&](uint32_t m, bool patch, const ListOut& o) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t mq = (m >> (4 * q)) & 0xFu;
+            const uint32_t cq = __popc(mq);
+            const uint32_t incl = wave_incl_scan(cq);
+            uint32_t pos = run + incl - cq;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (mq & (1u << k)) {
+                    if (pos < o.cap) {
+                        o.val[pos] = patch ? xv[q][k] : d[q][k];
+                        o.idx[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
+                    }
+                    pos++;
+                }
+            run += __shfl(incl, 63, 64);
+        }
+        return run;
+    };
+    TileCounts tc{emit(mask, false, lo), 0u};
+    if constexpr (STRICT) {
+        bool huge = false;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) huge |= !(fabsf(s[q][k]) < kExactS);
+        const bool tile_huge = __ballot(huge) != 0ull;
+        uint32_t pm = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
+                if (e < nv && (tile_huge || needs_patch(xv[q][k], s[q][k], se.eb, se.ebx2))) pm |= 1u << (q * 4 + k);
+            }
+        tc.pcnt = emit(pm, true, lp);
+    }
+    return tc;
+}
+
+// ---------------------------------------------------------------------------
+// 2D: tile (tile % gx, tile / gx) of an nx x ny field; `last` zeroes the code
+// tail [nx*ny, n).  codes == nullptr: lists only.
+// ---------------------------------------------------------------------------
+template <typename TCode, bool ZZ, bool STRICT>
+__device__ __forceinline__ TileCounts lz2d_tile(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n,
+                                                uint32_t tile, uint32_t gx, bool last, float r, float rad,
+                                                TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
+    const int l = lane_id();
+    const int x = l & 31, h = l >> 5;
+    const uint32_t bx = tile % gx, by = tile / gx;
+    const uint32_t X = bx * 32 + x;
+    const uint32_t Y0 = by * 32 + 16 * h;
+    const bool xin = X < nx;
+    float s[16], xv[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t Y = Y0 + i;
+        xv[i] = (xin && Y < ny) ? in[(uint64_t)Y * nx + X] : 0.0f;
+        s[i] = quant_round(xv[i], r);
+    }
+    // t = s(y) - s(y-1) inside the tile (row 0 of the tile subtracts 0).
+    const float north = __shfl(s[15], x, 64);  // row 15 of the tile, for h = 1
+    float t[16];
+    t[0] = s[0] - (h ? north : 0.0f);
+#pragma unroll
+    for (int i = 1; i < 16; i++) t[i] = s[i] - s[i - 1];
+    uint32_t mask = 0;
+    uint64_t bal[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const float w = __shfl_up(t[i], 1, 32);
+        const float d = (x > 0) ? t[i] - w : t[i];
+        const bool valid = xin && (Y0 + i) < ny;
+        const bool ok = fabsf(d) < rad;
+        if (valid && codes) codes[(uint64_t)(Y0 + i) * nx + X] = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
+        const bool is_out = valid && !ok;
+        if (is_out) mask |= 1u << i;
+        bal[i] = __ballot(is_out);
+        t[i] = d;  // keep the delta for the outlier pass
+    }
+    // Row-major order inside the tile: rows 0..15 (h=0) then 16..31 (h=1).
+    auto emit = [&](const uint64_t (&bb)[16], uint32_t m, bool patch, const ListOut& o) {
+        uint32_t tot = 0, lw = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            tot += __popcll(bb[i]);
+            lw += __popc((uint32_t)bb[i]);
+        }
+        if (tot) {
+            uint32_t run = h ? lw : 0u;
+            const uint32_t below = (1u << x) - 1u;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t half = h ? (uint32_t)(bb[i] >> 32) : (uint32_t)bb[i];
+                if (m & (1u << i)) {
+                    const uint32_t pos = run + __popc(half & below);
+                    if (pos < o.cap) {
+                        o.val[pos] = patch ? xv[i] : t[i];
+                        o.idx[pos] = (Y0 + i) * nx + X;
+                    }
+                }
+                run += __popc(half);
+            }
+        }
+        return tot;
+    };
+    TileCounts tc{emit(bal, mask, false, lo), 0u};
+    if (last && codes) zero_code_tail(codes, (uint64_t)nx * ny, n);
+    if constexpr (STRICT) {
+        bool huge = false;
+#pragma unroll
+        for (int i = 0; i < 16; i++) huge |= !(fabsf(s[i]) < kExactS);
+        const bool tile_huge = __ballot(huge) != 0ull;
+        uint32_t pm = 0;
+        uint64_t pbal[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const bool valid = xin && (Y0 + i) < ny;
+            const bool p = valid && (tile_huge || needs_patch(xv[i], s[i], se.eb, se.ebx2));
+            if (p) pm |= 1u << i;
+            pbal[i] = __ballot(p);
+        }
+        tc.pcnt = emit(pbal, pm, true, lp);
+    }
+    return tc;
+}
+
+struct Tiles {
+    uint32_t gx, gy, gz;  // tile grid
+};
+
+// ---------------------------------------------------------------------------
+// 3D: tile (bx, by, bz) = (tile % gx, (tile / gx) % gy, tile / (gx gy)) of an
+// nx x ny x nz field; `last` zeroes the code tail.  codes == nullptr: lists only.
+// ---------------------------------------------------------------------------
+template <typename TCode, bool ZZ, bool STRICT>
+__device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+                                                uint64_t n, uint32_t tile, Tiles tg, bool last, float r, float rad,
+                                                TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
+    const int l = lane_id();
+    const int x = l & 7, z = l >> 3;
+    const uint32_t bx = tile % tg.gx, by = (tile / tg.gx) % tg.gy, bz = tile / (tg.gx * tg.gy);
+    const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
+    const bool xz = X < nx && Z < nz;
+    const uint64_t plane = (uint64_t)nx * ny;
+    float s[8], xv[8];
+#pragma unroll
+    for (int y = 0; y < 8; y++) {
+        xv[y] = (xz && Y0 + y < ny) ? in[Z * plane + (uint64_t)(Y0 + y) * nx + X] : 0.0f;
+        s[y] = quant_round(xv[y], r);
+    }
+    // a = s - s(z-1); b = a - a(x-1); d = b - b(y-1)   (lorenzo_quant_nd.cu:234-246)
+    float b[8];
+#pragma unroll
+    for (int y = 0; y < 8; y++) {
+        const float sz = __shfl_up(s[y], 8, 64);
+        const float a = s[y] - (z > 0 ? sz : 0.0f);
+        const float aw = __shfl_up(a, 1, 8);
+        b[y] = (x > 0) ? a - aw : a;
+    }
+    float d[8];
+    uint32_t mask = 0;
+    uint64_t bal[8];
+#pragma unroll
+    for (int y = 0; y < 8; y++) {
+        d[y] = (y > 0) ? b[y] - b[y - 1] : b[y];
+        const bool valid = xz && (Y0 + y) < ny;
+        const bool ok = fabsf(d[y]) < rad;
+        if (valid && codes) codes[Z * plane + (uint64_t)(Y0 + y) * nx + X] = ok ? enc_code<TCode, ZZ>(d[y]) : (TCode)0;
+        const bool is_out = valid && !ok;
+        if (is_out) mask |= 1u << y;
+        bal[y] = __ballot(is_out);
+    }
+    // Order inside the tile: (z, y, x).
+    auto emit = [&](const uint64_t (&bb)[8], uint32_t m, bool patch, const ListOut& o) {
+        uint32_t tot = 0;
+#pragma unroll
+        for (int y = 0; y < 8; y++) tot += __popcll(bb[y]);
+        if (tot) {
+            const uint64_t zlow = (z == 0) ? 0ull : ((1ull << (8 * z)) - 1ull);
+            uint32_t run = 0;
+#pragma unroll
+            for (int y = 0; y < 8; y++) run += __popcll(bb[y] & zlow);
+            const uint32_t below = (1u << x) - 1u;
+#pragma unroll
+            for (int y = 0; y < 8; y++) {
+                const uint32_t row = (uint32_t)(bb[y] >> (8 * z)) & 0xFFu;
+                if (m & (1u << y)) {
+                    const uint32_t pos = run + __popc(row & below);
+                    if (pos < o.cap) {
+                        o.val[pos] = patch ? xv[y] : d[y];
+                        o.idx[pos] = (uint32_t)(Z * plane + (uint64_t)(Y0 + y) * nx + X);
+                    }
+                }
+                run += __popc(row);
+            }
+        }
+        return tot;
+    };
+    TileCounts tc{emit(bal, mask, false, lo), 0u};
+    if (last && codes) zero_code_tail(codes, (uint64_t)nx * ny * nz, n);
+    if constexpr (STRICT) {
+        bool huge = false;
+#pragma unroll
+        for (int y = 0; y < 8; y++) huge |= !(fabsf(s[y]) < kExactS);
+        const bool tile_huge = __ballot(huge) != 0ull;
+        uint32_t pm = 0;
+        uint64_t pbal[8];
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            const bool valid = xz && (Y0 + y) < ny;
+            const bool p = valid && (tile_huge || needs_patch(xv[y], s[y], se.eb, se.ebx2));
+            if (p) pm |= 1u << y;
+            pbal[y] = __ballot(p);
+        }
+        tc.pcnt = emit(pbal, pm, true, lp);
+    }
+    return tc;
+}
+
+// Device-side resolution of the error bound from the value base, shared by
+// the single-field and batched reductions (lorenzo_quant.cu:612-644):
+//   params = [value_base, abs_eb, 1/(2 abs_eb), 2 abs_eb]
+// with value_base = max(|min|, |max|) (REL, mode 1) or max - min (NOA, 2),
+// and abs_eb = user_eb when value_base <= 0.
+__device__ __forceinline__ void resolve_eb(float gmn, float gmx, int mode, float user_eb, float* params) {
+    float vb;
+    if (mode == 2) {
+        vb = gmx - gmn;
+    } else {
+        const float amin = gmn < 0.0f ? -gmn : gmn;
+        const float amax = gmx < 0.0f ? -gmx : gmx;
+        vb = amin > amax ? amin : amax;
+    }
+    const float abs_eb = (vb <= 0.0f) ? user_eb : user_eb * vb;
+    params[0] = vb;
+    params[1] = abs_eb;
+    params[2] = 1.0f / (2.0f * abs_eb);
+    params[3] = 2.0f * abs_eb;
+}
+
+}  // namespace
+}  // namespace fzh

@@ -11,61 +11,16 @@
 // The arithmetic per element is the reference's (same FP32 operation order),
 // and the inverse scans reproduce the reference's addition trees exactly.
 //
-// Outliers are compacted per tile and placed with an ordered single-pass
-// tile scan, so the outlier list is deterministic: tile-major, and sorted by
-// element index inside each tile (the reference appends them with atomics in
-// arbitrary order).
+// The per-tile forward work lives in fzh_lorenzo_tile.h (shared with the
+// batched launch).  Each tile lists its outliers in scratch and a compaction
+// kernel places the lists, so the outlier list is deterministic: tile-major,
+// and sorted by element index inside each tile (the reference appends them
+// with atomics in arbitrary order).
 #include "fzh_compact.h"
+#include "fzh_lorenzo_tile.h"
 
 namespace fzh {
 namespace {
-
-template <typename TCode> struct SignedOf;
-template <> struct SignedOf<uint8_t> { using type = int8_t; };
-template <> struct SignedOf<uint16_t> { using type = int16_t; };
-
-// (TCode)(int)delta, or zigzag((SCode)(int)delta)   (lorenzo_quant.cu:328-334)
-template <typename TCode, bool ZZ>
-__device__ __forceinline__ TCode enc_code(float d) {
-    using SC = typename SignedOf<TCode>::type;
-    constexpr int W = 8 * sizeof(TCode);
-    const int q = (int)d;
-    if constexpr (ZZ) {
-        const SC s = (SC)q;
-        return (TCode)(((TCode)s << 1) ^ (TCode)(s >> (W - 1)));
-    } else {
-        return (TCode)q;
-    }
-}
-
-// (TIn)(SCode)code, or zigzag-decode  (lorenzo_quant.cu:195-202)
-template <typename TCode, bool ZZ>
-__device__ __forceinline__ float dec_code(TCode c) {
-    using SC = typename SignedOf<TCode>::type;
-    if constexpr (ZZ) {
-        return (float)(SC)((TCode)(c >> 1) ^ (TCode)(-(SC)(c & 1u)));
-    } else {
-        return (float)(SC)c;
-    }
-}
-
-__device__ __forceinline__ float quant_round(float x, float r) { return __builtin_roundf(x * r); }
-
-// Strict error bound (an extension; no counterpart in the reference).  The
-// inverse reconstructs element i as fl(S_i * ebx2), S_i the exact integer
-// prefix of the tile's deltas, whenever every partial sum of the inverse's
-// addition tree is exact.  Partial sums are sums of deltas over sub-boxes of
-// the tile, i.e. signed sums of at most 8 quantised values, so |s| < 2^21
-// across the tile keeps them below 2^24 (exact in FP32).  An element is
-// patched (its original value stored) when |fl(s*ebx2) - x| > eb, or when
-// its tile breaks the exactness condition (then every element of the tile).
-constexpr float kExactS = 2097152.0f;  // 2^21
-
-__device__ __forceinline__ bool needs_patch(float x, float s, float eb, float ebx2) {
-    const float xh = s * ebx2;
-    const double e = fabs((double)xh - (double)x);
-    return !(e <= (double)eb);  // NaN compares false: patched
-}
 
 // Per-tile lists written by the forward kernels: tile t's outliers occupy
 // err/idx[t*E, t*E + cnt[t]) in the tile's order (E = elements per tile), and
@@ -84,9 +39,17 @@ struct TileLists {
     float eb, ebx2;         // used when d_params is null
 };
 
-struct Tiles {
-    uint32_t gx, gy, gz;  // tile grid
-};
+__device__ __forceinline__ StrictEb strict_eb(const TileLists& tl) {
+    return tl.d_params ? StrictEb{tl.d_params[1], tl.d_params[3]} : StrictEb{tl.eb, tl.ebx2};
+}
+
+template <bool STRICT>
+__device__ __forceinline__ void store_counts(const TileLists& tl, uint32_t tile, TileCounts tc) {
+    if (lane_id() == 0) {
+        tl.cnt[tile] = tc.cnt;
+        if constexpr (STRICT) tl.pcnt[tile] = tc.pcnt;
+    }
+}
 
 // ===========================================================================
 // Forward kernels: one wave64 per tile, four tiles per 256-thread workgroup
@@ -99,116 +62,12 @@ lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, uint64_t nv, float r_h
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
-    const int l = lane_id();
-    const uint64_t base = (uint64_t)tile * 1024;
-    const bool full = base + 1024 <= nv;  // elements in [nv, n) read as zero (padding)
-    float s[4][4], xv[4][4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
-        if (full) {
-            const float4 v = *reinterpret_cast<const float4*>(in + e0);
-            xv[q][0] = v.x;
-            xv[q][1] = v.y;
-            xv[q][2] = v.z;
-            xv[q][3] = v.w;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; k++) xv[q][k] = (e0 + k < nv) ? in[e0 + k] : 0.0f;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) s[q][k] = quant_round(xv[q][k], r);
-    }
-    // delta_i = s_i - s_{i-1}, chain restarts at the tile start.
-    float d[4][4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        float prev = __shfl_up(s[q][3], 1, 64);
-        const float last_prev_q = __shfl(q > 0 ? s[q > 0 ? q - 1 : 0][3] : 0.0f, 63, 64);
-        if (l == 0) prev = (q > 0) ? last_prev_q : 0.0f;
-        d[q][0] = s[q][0] - prev;
-#pragma unroll
-        for (int k = 1; k < 4; k++) d[q][k] = s[q][k] - s[q][k - 1];
-    }
-    TCode c[4][4];
-    uint32_t mask = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
-            const bool ok = fabsf(d[q][k]) < rad;
-            c[q][k] = ok ? enc_code<TCode, ZZ>(d[q][k]) : (TCode)0;
-            if (!ok && e < n) mask |= 1u << (q * 4 + k);
-        }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
-        if (base + 1024 <= n) {
-            if constexpr (sizeof(TCode) == 2) {
-                uint2 w;
-                w.x = (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 16);
-                w.y = (uint32_t)c[q][2] | ((uint32_t)c[q][3] << 16);
-                *reinterpret_cast<uint2*>(codes + e0) = w;
-            } else {
-                *reinterpret_cast<uint32_t*>(codes + e0) =
-                    (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 8) | ((uint32_t)c[q][2] << 16) |
-                    ((uint32_t)c[q][3] << 24);
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (e0 + k < n) codes[e0 + k] = c[q][k];
-        }
-    }
-    // Tile-local lists in index order: key (q, lane, k).
-    auto emit = [&](uint32_t m, bool patch, float* vdst, uint32_t* idst) {
-        uint32_t run = 0;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint32_t mq = (m >> (4 * q)) & 0xFu;
-            const uint32_t cq = __popc(mq);
-            const uint32_t incl = wave_incl_scan(cq);
-            uint32_t pos = run + incl - cq;
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (mq & (1u << k)) {
-                    vdst[pos] = patch ? xv[q][k] : d[q][k];
-                    idst[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
-                    pos++;
-                }
-            run += __shfl(incl, 63, 64);
-        }
-        return run;
-    };
-    const uint32_t cnt = emit(mask, false, tl.err + base, tl.idx + base);
-    if (l == 0) tl.cnt[tile] = cnt;
-    if constexpr (STRICT) {
-        const float eb = tl.d_params ? tl.d_params[1] : tl.eb;
-        const float e2 = tl.d_params ? tl.d_params[3] : tl.ebx2;
-        bool huge = false;
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int k = 0; k < 4; k++) huge |= !(fabsf(s[q][k]) < kExactS);
-        const bool tile_huge = __ballot(huge) != 0ull;
-        uint32_t pm = 0;
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
-                if (e < nv && (tile_huge || needs_patch(xv[q][k], s[q][k], eb, e2))) pm |= 1u << (q * 4 + k);
-            }
-        const uint32_t pc = emit(pm, true, tl.pval + base, tl.pidx + base);
-        if (l == 0) tl.pcnt[tile] = pc;
-    }
-}
-
-// Zero codes[from, to) (tail beyond nx*ny[*nz] when the input was padded).
-template <typename TCode>
-__device__ __forceinline__ void zero_code_tail(TCode* codes, uint64_t from, uint64_t to) {
-    for (uint64_t i = from + lane_id(); i < to; i += 64) codes[i] = (TCode)0;
+    const uint64_t b = (uint64_t)tile * 1024;
+    const TileCounts tc = lz1d_tile<TCode, ZZ, STRICT>(in, n, nv, tile, r, rad, codes,
+                                                       ListOut{tl.err + b, tl.idx + b, 1024u},
+                                                       ListOut{tl.pval + b, tl.pidx + b, 1024u},
+                                                       STRICT ? strict_eb(tl) : StrictEb{0.0f, 0.0f});
+    store_counts<STRICT>(tl, tile, tc);
 }
 
 template <typename TCode, bool ZZ, bool STRICT>
@@ -219,86 +78,12 @@ lz2d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
-    const int l = lane_id();
-    const int x = l & 31, h = l >> 5;
-    const uint32_t bx = tile % gx, by = tile / gx;
-    const uint32_t X = bx * 32 + x;
-    const uint32_t Y0 = by * 32 + 16 * h;
-    const bool xin = X < nx;
-    float s[16], xv[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const uint32_t Y = Y0 + i;
-        xv[i] = (xin && Y < ny) ? in[(uint64_t)Y * nx + X] : 0.0f;
-        s[i] = quant_round(xv[i], r);
-    }
-    // t = s(y) - s(y-1) inside the tile (row 0 of the tile subtracts 0).
-    const float north = __shfl(s[15], x, 64);  // row 15 of the tile, for h = 1
-    float t[16];
-    t[0] = s[0] - (h ? north : 0.0f);
-#pragma unroll
-    for (int i = 1; i < 16; i++) t[i] = s[i] - s[i - 1];
-    uint32_t mask = 0;
-    uint64_t bal[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const float w = __shfl_up(t[i], 1, 32);
-        const float d = (x > 0) ? t[i] - w : t[i];
-        const bool valid = xin && (Y0 + i) < ny;
-        const bool ok = fabsf(d) < rad;
-        if (valid) codes[(uint64_t)(Y0 + i) * nx + X] = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
-        const bool is_out = valid && !ok;
-        if (is_out) mask |= 1u << i;
-        bal[i] = __ballot(is_out);
-        t[i] = d;  // keep the delta for the outlier pass
-    }
-    const uint64_t lbase = (uint64_t)tile * 1024;
-    // Row-major order inside the tile: rows 0..15 (h=0) then 16..31 (h=1).
-    auto emit = [&](const uint64_t (&bb)[16], uint32_t m, bool patch, float* vdst, uint32_t* idst) {
-        uint32_t tot = 0, lo = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            tot += __popcll(bb[i]);
-            lo += __popc((uint32_t)bb[i]);
-        }
-        if (tot) {
-            uint32_t run = h ? lo : 0u;
-            const uint32_t below = (1u << x) - 1u;
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const uint32_t half = h ? (uint32_t)(bb[i] >> 32) : (uint32_t)bb[i];
-                if (m & (1u << i)) {
-                    const uint32_t pos = run + __popc(half & below);
-                    vdst[pos] = patch ? xv[i] : t[i];
-                    idst[pos] = (Y0 + i) * nx + X;
-                }
-                run += __popc(half);
-            }
-        }
-        return tot;
-    };
-    const uint32_t cnt = emit(bal, mask, false, tl.err + lbase, tl.idx + lbase);
-    if (l == 0) tl.cnt[tile] = cnt;
-    if (tile == T - 1) zero_code_tail(codes, (uint64_t)nx * ny, n);
-    if constexpr (STRICT) {
-        const float eb = tl.d_params ? tl.d_params[1] : tl.eb;
-        const float e2 = tl.d_params ? tl.d_params[3] : tl.ebx2;
-        bool huge = false;
-#pragma unroll
-        for (int i = 0; i < 16; i++) huge |= !(fabsf(s[i]) < kExactS);
-        const bool tile_huge = __ballot(huge) != 0ull;
-        uint32_t pm = 0;
-        uint64_t pbal[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const bool valid = xin && (Y0 + i) < ny;
-            const bool p = valid && (tile_huge || needs_patch(xv[i], s[i], eb, e2));
-            if (p) pm |= 1u << i;
-            pbal[i] = __ballot(p);
-        }
-        const uint32_t pc = emit(pbal, pm, true, tl.pval + lbase, tl.pidx + lbase);
-        if (l == 0) tl.pcnt[tile] = pc;
-    }
+    const uint64_t b = (uint64_t)tile * 1024;
+    const TileCounts tc = lz2d_tile<TCode, ZZ, STRICT>(in, nx, ny, n, tile, gx, tile == T - 1, r, rad, codes,
+                                                       ListOut{tl.err + b, tl.idx + b, 1024u},
+                                                       ListOut{tl.pval + b, tl.pidx + b, 1024u},
+                                                       STRICT ? strict_eb(tl) : StrictEb{0.0f, 0.0f});
+    store_counts<STRICT>(tl, tile, tc);
 }
 
 template <typename TCode, bool ZZ, bool STRICT>
@@ -309,87 +94,12 @@ lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
-    const int l = lane_id();
-    const int x = l & 7, z = l >> 3;
-    const uint32_t bx = tile % tg.gx, by = (tile / tg.gx) % tg.gy, bz = tile / (tg.gx * tg.gy);
-    const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
-    const bool xz = X < nx && Z < nz;
-    const uint64_t plane = (uint64_t)nx * ny;
-    float s[8], xv[8];
-#pragma unroll
-    for (int y = 0; y < 8; y++) {
-        xv[y] = (xz && Y0 + y < ny) ? in[Z * plane + (uint64_t)(Y0 + y) * nx + X] : 0.0f;
-        s[y] = quant_round(xv[y], r);
-    }
-    // a = s - s(z-1); b = a - a(x-1); d = b - b(y-1)   (lorenzo_quant_nd.cu:234-246)
-    float b[8];
-#pragma unroll
-    for (int y = 0; y < 8; y++) {
-        const float sz = __shfl_up(s[y], 8, 64);
-        const float a = s[y] - (z > 0 ? sz : 0.0f);
-        const float aw = __shfl_up(a, 1, 8);
-        b[y] = (x > 0) ? a - aw : a;
-    }
-    float d[8];
-    uint32_t mask = 0;
-    uint64_t bal[8];
-#pragma unroll
-    for (int y = 0; y < 8; y++) {
-        d[y] = (y > 0) ? b[y] - b[y - 1] : b[y];
-        const bool valid = xz && (Y0 + y) < ny;
-        const bool ok = fabsf(d[y]) < rad;
-        if (valid) codes[Z * plane + (uint64_t)(Y0 + y) * nx + X] = ok ? enc_code<TCode, ZZ>(d[y]) : (TCode)0;
-        const bool is_out = valid && !ok;
-        if (is_out) mask |= 1u << y;
-        bal[y] = __ballot(is_out);
-    }
-    const uint64_t lbase = (uint64_t)tile * 512;
-    // Order inside the tile: (z, y, x).
-    auto emit = [&](const uint64_t (&bb)[8], uint32_t m, bool patch, float* vdst, uint32_t* idst) {
-        uint32_t tot = 0;
-#pragma unroll
-        for (int y = 0; y < 8; y++) tot += __popcll(bb[y]);
-        if (tot) {
-            const uint64_t zlow = (z == 0) ? 0ull : ((1ull << (8 * z)) - 1ull);
-            uint32_t run = 0;
-#pragma unroll
-            for (int y = 0; y < 8; y++) run += __popcll(bb[y] & zlow);
-            const uint32_t below = (1u << x) - 1u;
-#pragma unroll
-            for (int y = 0; y < 8; y++) {
-                const uint32_t row = (uint32_t)(bb[y] >> (8 * z)) & 0xFFu;
-                if (m & (1u << y)) {
-                    const uint32_t pos = run + __popc(row & below);
-                    vdst[pos] = patch ? xv[y] : d[y];
-                    idst[pos] = (uint32_t)(Z * plane + (uint64_t)(Y0 + y) * nx + X);
-                }
-                run += __popc(row);
-            }
-        }
-        return tot;
-    };
-    const uint32_t cnt = emit(bal, mask, false, tl.err + lbase, tl.idx + lbase);
-    if (l == 0) tl.cnt[tile] = cnt;
-    if (tile == T - 1) zero_code_tail(codes, (uint64_t)nx * ny * nz, n);
-    if constexpr (STRICT) {
-        const float eb = tl.d_params ? tl.d_params[1] : tl.eb;
-        const float e2 = tl.d_params ? tl.d_params[3] : tl.ebx2;
-        bool huge = false;
-#pragma unroll
-        for (int y = 0; y < 8; y++) huge |= !(fabsf(s[y]) < kExactS);
-        const bool tile_huge = __ballot(huge) != 0ull;
-        uint32_t pm = 0;
-        uint64_t pbal[8];
-#pragma unroll
-        for (int y = 0; y < 8; y++) {
-            const bool valid = xz && (Y0 + y) < ny;
-            const bool p = valid && (tile_huge || needs_patch(xv[y], s[y], eb, e2));
-            if (p) pm |= 1u << y;
-            pbal[y] = __ballot(p);
-        }
-        const uint32_t pc = emit(pbal, pm, true, tl.pval + lbase, tl.pidx + lbase);
-        if (l == 0) tl.pcnt[tile] = pc;
-    }
+    const uint64_t b = (uint64_t)tile * 512;
+    const TileCounts tc = lz3d_tile<TCode, ZZ, STRICT>(in, nx, ny, nz, n, tile, tg, tile == T - 1, r, rad, codes,
+                                                       ListOut{tl.err + b, tl.idx + b, 512u},
+                                                       ListOut{tl.pval + b, tl.pidx + b, 512u},
+                                                       STRICT ? strict_eb(tl) : StrictEb{0.0f, 0.0f});
+    store_counts<STRICT>(tl, tile, tc);
 }
 
 // ===========================================================================
@@ -408,8 +118,8 @@ minmax_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ par
     }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t n4 = n / 4;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        const float4 v = reinterpret_cast<const float4*>(in)[i];
+    const float4* in4 = reinterpret_cast<const float4*>(in);
+    auto fold = [&](const float4& v) {
         if (v.x < mn) mn = v.x;
         if (v.x > mx) mx = v.x;
         if (v.y < mn) mn = v.y;
@@ -418,7 +128,17 @@ minmax_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ par
         if (v.z > mx) mx = v.z;
         if (v.w < mn) mn = v.w;
         if (v.w > mx) mx = v.w;
+    };
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // four independent loads in flight per thread
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const float4 a = in4[i], b = in4[i + stride], c = in4[i + 2 * stride], d = in4[i + 3 * stride];
+        fold(a);
+        fold(b);
+        fold(c);
+        fold(d);
     }
+    for (; i < n4; i += stride) fold(in4[i]);
     for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const float v = in[i];
         if (v < mn) mn = v;
@@ -467,19 +187,7 @@ minmax_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ par
                 if (smn[i] < gmn) gmn = smn[i];
                 if (smx[i] > gmx) gmx = smx[i];
             }
-            float vb;
-            if (mode == 2) {
-                vb = gmx - gmn;
-            } else {
-                const float amin = gmn < 0.0f ? -gmn : gmn;
-                const float amax = gmx < 0.0f ? -gmx : gmx;
-                vb = amin > amax ? amin : amax;
-            }
-            const float abs_eb = (vb <= 0.0f) ? user_eb : user_eb * vb;
-            params[0] = vb;
-            params[1] = abs_eb;
-            params[2] = 1.0f / (2.0f * abs_eb);
-            params[3] = 2.0f * abs_eb;
+            resolve_eb(gmn, gmx, mode, user_eb, params);
             *counter = 0u;  // ready for the next call
         }
     }
@@ -521,40 +229,6 @@ bin_count_kernel(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ 
         uint32_t t, loc;
         if (tile_of(m, idx[i], t, loc)) slot[i] = atomicAdd(&tile_cnt[t], 1u);
     }
-}
-
-// Exclusive scan of tile_cnt[0..T) into off[0..T] (off[T] = total), zeroing
-// tile_cnt as it goes so the array is clear for the next call.  One wave per
-// 1024 entries.
-__global__ void __launch_bounds__(256)
-bin_scan_kernel(uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ off, uint32_t T, ScanCtx* ctx,
-                uint64_t* status, uint32_t nseg, uint32_t total_waves) {
-    const TileTicket tk = take_ticket(ctx);
-    if (tk.tile < nseg) {
-        const int l = lane_id();
-        const uint32_t b = tk.tile * 1024 + 16 * l;
-        uint32_t v[16], sum = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            v[i] = (b + i < T) ? tile_cnt[b + i] : 0u;
-            sum += v[i];
-        }
-        const uint32_t incl = wave_incl_scan(sum);
-        const uint32_t agg = __shfl(incl, 63, 64);
-        uint32_t pre = aggregate_prefix(status, tk.tile, tk.epoch, agg);
-        if (pre == kLookbackFailed) pre = 0u;  // offsets stay in range; counts are bounded by the list
-        uint32_t run = pre + incl - sum;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            if (b + i < T) {
-                off[b + i] = run;
-                tile_cnt[b + i] = 0u;
-            }
-            run += v[i];
-        }
-        if (tk.tile == nseg - 1 && l == 0) off[T] = pre + agg;
-    }
-    release_ticket(ctx, tk.epoch, total_waves);
 }
 
 __global__ void __launch_bounds__(256)
@@ -817,9 +491,9 @@ int check_dims(size_t n, size_t nx, size_t ny, size_t nz, const char* who) {
     return FZH_OK;
 }
 
-// Forward workspace: compaction scan state (ScanCtx + one status array per
-// list), per-tile counts, and the per-tile list scratch (T x E entries per
-// list; written only where outliers/patches occur).
+// Forward workspace: compaction state (prefixes + scan workspace), per-tile
+// counts, and the per-tile list scratch (T x E entries per list; written
+// only where outliers/patches occur).
 struct FwdWs {
     size_t T, E, G;
     size_t o_ctx, o_cnt, o_pcnt, o_err, o_idx, o_pval, o_pidx, total;
@@ -832,8 +506,9 @@ FwdWs fwd_layout(size_t n, size_t nx, size_t ny, size_t nz, bool strict) {
     w.G = std::max<size_t>(1, cdiv(w.T, 1024));
     size_t o = 0;
     w.o_ctx = o;  o = align_up(o + compact_state_bytes(w.T), 256);
-    w.o_cnt = o;  o = align_up(o + 4 * std::max<size_t>(w.T, 1), 256);
-    w.o_pcnt = o; if (strict) o = align_up(o + 4 * std::max<size_t>(w.T, 1), 256);
+    // patch counts directly follow the outlier counts (one scan covers both)
+    w.o_cnt = o;  w.o_pcnt = o + 4 * w.T;
+    o = align_up(o + 4 * std::max<size_t>(w.T, 1) * (strict ? 2 : 1), 256);
     w.o_err = o;  o = align_up(o + 4 * w.T * w.E, 256);
     w.o_idx = o;  o = align_up(o + 4 * w.T * w.E, 256);
     w.o_pval = o; if (strict) o = align_up(o + 4 * w.T * w.E, 256);
@@ -893,25 +568,27 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
         FZH_LZ_FWD(false)
     }
 #undef FZH_LZ_FWD
-    CompactList L0{tl.err, tl.idx, tl.cnt, oerr, oidx, ocount, max_out, nullptr};
-    CompactList L1{tl.pval, tl.pidx, tl.pcnt, po.val, po.idx, po.count, po.cap, nullptr};
-    launch_compact(s, w + L.o_ctx, T, (uint32_t)L.E, L0, STRICT ? &L1 : nullptr);
     FZH_CHECK_LAUNCH("lorenzo fwd");
+    CompactList L0{tl.err, tl.idx, tl.cnt, oerr, oidx, ocount, max_out};
+    CompactList L1{tl.pval, tl.pidx, tl.pcnt, po.val, po.idx, po.count, po.cap};
+    if (int rc = launch_compact(s, w + L.o_ctx, T, (uint32_t)L.E, L0, STRICT ? &L1 : nullptr)) return rc;
     return FZH_OK;
 }
 
+// Inverse workspace: per-tile outlier counts (zero between calls: the scan
+// clears them as it reads), their exclusive prefix, the scan workspace, and
+// the outliers binned by tile.
 struct InvWs {
-    size_t T, nseg, max_out;
-    size_t o_ctx, o_cnt, o_off, o_slot, o_err, o_loc, total;
+    size_t T, max_out;
+    size_t o_scan, o_cnt, o_off, o_slot, o_err, o_loc, total;
 };
 
 InvWs inv_layout(size_t n, size_t nx, size_t ny, size_t nz, size_t max_out) {
     InvWs w;
     w.T = tile_count(n, nx, ny, nz);
-    w.nseg = cdiv(w.T, 1024);
     w.max_out = max_out;
     size_t o = 0;
-    w.o_ctx = o;  o = align_up(o + scan_state_bytes(w.nseg), 256);
+    w.o_scan = o; o = align_up(o + scan_ws_bytes(std::max<size_t>(w.T, 1)), 256);
     w.o_cnt = o;  o = align_up(o + 4 * w.T, 256);
     w.o_off = o;  o = align_up(o + 4 * (w.T + 1), 256);
     w.o_slot = o; o = align_up(o + 4 * std::max<size_t>(max_out, 1), 256);
@@ -928,8 +605,6 @@ int launch_inv(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny,
     const InvWs L = inv_layout(n, nx, ny, nz, max_out);
     if (ws_bytes < L.total) return fail(FZH_E_CAPACITY, "lorenzo inv: workspace too small");
     uint8_t* w = (uint8_t*)ws;
-    ScanCtx* ctx = (ScanCtx*)(w + L.o_ctx);
-    uint64_t* status = (uint64_t*)(w + L.o_ctx + sizeof(ScanCtx));
     uint32_t* cnt = (uint32_t*)(w + L.o_cnt);
     uint32_t* off = (uint32_t*)(w + L.o_off);
     uint32_t* slot = (uint32_t*)(w + L.o_slot);
@@ -949,10 +624,7 @@ int launch_inv(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny,
     if (has_out) {
         bin_count_kernel<<<ob, 256, 0, s>>>(idx, count, max_out, m, cnt, slot);
     }
-    {
-        const unsigned sb = (unsigned)cdiv(L.nseg, 4);
-        bin_scan_kernel<<<sb, 256, 0, s>>>(cnt, off, (uint32_t)L.T, ctx, status, (uint32_t)L.nseg, sb * 4);
-    }
+    if (int rc = scan_u32<false, true>(s, cnt, off, L.T, nullptr, off + L.T, w + L.o_scan)) return rc;
     if (has_out) {
         bin_place_kernel<<<ob, 256, 0, s>>>(err, idx, count, max_out, m, off, slot, serr, sloc);
     }
@@ -1110,7 +782,9 @@ int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode, floa
     if (mode != 1 && mode != 2) return fail(FZH_E_INVALID, "value base: mode must be 1 (REL) or 2 (NOA)");
     if (n == 0) return fail(FZH_E_INVALID, "value base: empty input");
     if (reinterpret_cast<uintptr_t>(d_in) % 16 != 0) return fail(FZH_E_INVALID, "value base: input must be 16-byte aligned");
-    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256 * 16), 1024));
+    // about 16 float4 per thread, at most two workgroups per CU (each
+    // workgroup ends with one atomic on the shared counter)
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256 * 64), 512));
     uint32_t* counter = (uint32_t*)d_ws;
     float2* partial = (float2*)((uint8_t*)d_ws + 256);
     minmax_kernel<<<blocks, 256, 0, S(stream)>>>(d_in, n, partial, counter, mode, zero_pad, user_eb, d_params);

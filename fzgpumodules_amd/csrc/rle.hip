@@ -317,8 +317,8 @@ int enc_impl(hipStream_t s, const void* in, size_t n, void* out, size_t cap, uin
     uint32_t* dR = (uint32_t*)(w + L.o_R);
     const unsigned blocks = (unsigned)cdiv(ntiles, 4);
     rle_enc_runs_kernel<T><<<blocks, 256, 0, s>>>((const T*)in, n, spos, scnt, ntiles);
-    CompactList L0{nullptr, spos, scnt, nullptr, pos, dR, n, nullptr};
-    launch_compact(s, w + L.o_cst, ntiles, (uint32_t)TILE, L0, nullptr);
+    CompactList L0{nullptr, spos, scnt, nullptr, pos, dR, n};
+    if (int rc = launch_compact(s, w + L.o_cst, ntiles, (uint32_t)TILE, L0, nullptr)) return rc;
     const unsigned lb = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256), 2048));
     rle_enc_lengths_kernel<T><<<lb, 256, 0, s>>>((const T*)in, (uint8_t*)out, pos, dR, n, out_size);
     FZH_CHECK_LAUNCH("rle encode");
@@ -360,7 +360,7 @@ int dec_impl(hipStream_t s, const void* in, size_t in_bytes, void* out, size_t n
     const uint8_t* bin = (const uint8_t*)in;
     rle_dec_prep_kernel<T><<<1, 64, 0, s>>>(bin, in_bytes, n, Rv, d_status);
     rle_dec_reduce_kernel<T><<<(unsigned)L.nseg, 256, 0, s>>>(bin, Rv, seg);
-    scan_partials_kernel<<<1, 1024, 0, s>>>(seg, (uint32_t)L.nseg, tot);
+    scan_partials_kernel<false><<<1, 1024, 0, s>>>(seg, (uint32_t)L.nseg, tot);
     rle_dec_apply_kernel<T><<<(unsigned)L.nseg, 256, 0, s>>>(bin, n, Rv, seg, tot, start, tf, d_status);
     const unsigned eb = (unsigned)cdiv(L.ntiles, 2);
     rle_dec_expand_kernel<T><<<eb, 128, 0, s>>>((const uint8_t*)in, n, start, tf, d_status, (T*)out, (uint32_t)L.ntiles);
