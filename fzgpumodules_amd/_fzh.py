@@ -114,6 +114,15 @@ SIGNATURES = {
     "fzb_enable_graph_mode": (_I, [_P, _I]),
     "fzb_enable_profiling": (_I, [_P, _I]),
     "fzb_stage_ms": (_I, [_P, _I, C.c_char_p, _Z, C.POINTER(_F)]),
+    "fzs_features_f32": (_I, [_P, _P, _Z, _U32, _U32, _U32, _P, _P]),
+    "fzs_mlp_param_count": (_Z, []),
+    "fzs_mlp_forward": (_I, [_P, _P, _P, _P, _Z, _P]),
+    "fzs_mlp_train_ws_bytes": (_Z, [_Z]),
+    "fzs_mlp_train_step": (_I, [_P, _P, _P, _P, _P, _Z, _F, _P, _P, _P, _Z]),
+    "fzs_mlp_adam": (_I, [_P, _P, _P, _P, _P, _U32, _F, _F, _F, _F, _F]),
+    "fzs_label_ws_bytes": (_Z, [_Z, _U32, _U32, _U32]),
+    "fzs_label_sizes": (_I, [_P, _P, _Z, _U32, _U32, _U32, _P, _P, _P, _Z, _I]),
+    "fzs_select": (_I, [_P, _P, _Z, _P]),
     "fzh_device_alloc": (_I, [C.POINTER(_P), _Z]),
     "fzh_device_free": (_I, [_P]),
     "fzh_memcpy_htod": (_I, [_P, _P, _Z]),

@@ -108,8 +108,9 @@ __device__ __forceinline__ uint32_t byte_of(const uint4& v, int k) {
 // bit-shuffled first when SEW != 0 and `shuffled`).  Returns the size word
 // (bit 31 = raw).  The chunk's stored bytes are written to `slot` (CS bytes,
 // dword aligned) when they cannot be taken from the input as they are: a
-// compressed chunk, or a raw bit-shuffled block.  `st` and `bm1` are this
-// wave's LDS (LDS_STREAM and LDS_BM1 bytes).  Whole wave calls.
+// compressed chunk, or a raw bit-shuffled block (slot == nullptr: size
+// only).  `st` and `bm1` are this wave's LDS (LDS_STREAM and LDS_BM1
+// bytes).  Whole wave calls.
 template <int SEW>
 __device__ __forceinline__ uint32_t rze_encode_chunk(const uint8_t* __restrict__ src, int in_size, bool shuffled,
                                                      uint8_t* st, uint8_t* bm1, uint8_t* __restrict__ slot) {
@@ -268,7 +269,7 @@ __device__ __forceinline__ uint32_t rze_encode_chunk(const uint8_t* __restrict__
     // A raw chunk is copied from the input by the pack kernel, except a
     // bit-shuffled block, whose stored bytes are the shuffled image.
     const bool from_scratch = !raw || (SEW != 0 && shuffled);
-    if (from_scratch && clean > 0) {
+    if (slot && from_scratch && clean > 0) {
         // chunk bytes -> this chunk's scratch slot (dword stores; the slot
         // is CS bytes)
         const uint32_t nd = (clean + 3) / 4;

@@ -88,3 +88,70 @@ def smooth_1d(n: int) -> np.ndarray:
     (tests/helpers/fz_test_utils.h:184-196)."""
     i = np.arange(n, dtype=np.float64)
     return (50.0 * np.sin(i * 0.01) + 20.0 * np.cos(i * 0.003)).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# Chain-selector workload: fields of five regimes cut into 256 KiB chunks
+# (2D 256 x 256, 3D 16 x 64 x 64 as (nz, ny, nx)); every chunk's bound is
+# 1e-3 of its parent field's max |x|.
+# ---------------------------------------------------------------------------
+REGIMES = ("smooth_k4", "white_noise", "sparse80", "piecewise", "poisson20")
+CHUNK_2D = (256, 256)
+CHUNK_3D = (16, 64, 64)
+
+
+def regime_field(regime: int, shape, seed: int) -> np.ndarray:
+    """One parent field of regime `regime` (index into REGIMES)."""
+    rng = np.random.default_rng(seed)
+    if regime == 0:  # smooth Gaussian random field, power ~ k^-4
+        g = power_law_field(shape, -4.0, seed)
+    elif regime == 1:  # white noise, sigma 1
+        g = rng.standard_normal(shape)
+    elif regime == 2:  # 80% exact zeros, the rest N(0, 1)
+        g = rng.standard_normal(shape) * (rng.random(shape) >= 0.8)
+    elif regime == 3:  # piecewise constant: random axis-aligned boxes
+        g = np.zeros(shape)
+        for _ in range(48):
+            sl = []
+            for n in shape:
+                a, b = sorted(rng.integers(0, n + 1, 2))
+                sl.append(slice(a, max(b, a + 1)))
+            g[tuple(sl)] = rng.standard_normal()
+    elif regime == 4:  # integer-valued Poisson, lambda 20
+        g = rng.poisson(20.0, shape).astype(np.float64)
+    else:
+        raise ValueError("regime must be 0..4")
+    return g.astype(np.float32)
+
+
+def cut_chunks(field: np.ndarray) -> np.ndarray:
+    """Chunks of a 2D (ny, nx) or 3D (nz, ny, nx) field, shape (n, *chunk)."""
+    cs = CHUNK_2D if field.ndim == 2 else CHUNK_3D
+    grid = [s // c for s, c in zip(field.shape, cs)]
+    f = field[tuple(slice(0, g * c) for g, c in zip(grid, cs))]
+    if field.ndim == 2:
+        f = f.reshape(grid[0], cs[0], grid[1], cs[1]).transpose(0, 2, 1, 3)
+    else:
+        f = f.reshape(grid[0], cs[0], grid[1], cs[1], grid[2], cs[2]).transpose(0, 2, 4, 1, 3, 5)
+    return np.ascontiguousarray(f.reshape(-1, *cs))
+
+
+def selector_set(n_fields: int, seed: int, dims=(2, 3), fields_shape_2d=(512, 512), fields_shape_3d=(32, 128, 128)):
+    """(chunks_2d, eb_2d, regime_2d, chunks_3d, eb_3d, regime_3d): parent
+    fields drawn round-robin over the regimes, cut into chunks, each chunk
+    with bound 1e-3 * max |parent|."""
+    out = {}
+    for nd in dims:
+        cs, ebs, rg = [], [], []
+        for i in range(n_fields):
+            r = i % len(REGIMES)
+            shape = fields_shape_2d if nd == 2 else fields_shape_3d
+            f = regime_field(r, shape, seed * 7919 + 31 * i + nd)
+            vb = float(np.max(np.abs(f)))
+            eb = np.float32(np.float32(1e-3) * np.float32(vb)) if vb > 0 else np.float32(1e-3)
+            ch = cut_chunks(f)
+            cs.append(ch)
+            ebs.append(np.full(ch.shape[0], eb, np.float32))
+            rg.append(np.full(ch.shape[0], r, np.int32))
+        out[nd] = (np.concatenate(cs), np.concatenate(ebs), np.concatenate(rg))
+    return out

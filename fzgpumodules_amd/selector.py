@@ -1,0 +1,180 @@
+"""Python view of the learned chain selector (include/fzs.h).
+
+``features``/``label_sizes`` wrap the device kernels; ``Selector`` holds the
+MLP (FP32 master weights on the device, BF16 where a layer reads them) and
+trains it with the device forward/backward/Adam kernels; ``chain_pipeline``
+builds the fz::Pipeline of a candidate chain with the labeller's settings,
+which is how a chunk compressed under a recorded choice is decoded.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._fzh import DeviceBuffer, call, lib
+from .pipeline import Pipeline
+
+NFEAT, NCHAINS = 32, 4
+CHAINS = ("lorenzo-rle", "lorenzo-bitshuffle-rze", "lorenzo-huffman", "quantizer-bitpack")
+LZ_RADIUS, HUF_BKLEN, Q_RADIUS, OUTLIER_CAP = 512, 1024, 32768, 0.2
+
+
+def _dims(shape):
+    """(nx, ny, nz) of a chunk shape given as (ny, nx) or (nz, ny, nx)."""
+    s = tuple(shape)
+    return (s[-1], s[-2] if len(s) > 1 else 1, s[-3] if len(s) > 2 else 1)
+
+
+def chain_pipeline(k: int, abs_eb: float, shape) -> Pipeline:
+    """fz::Pipeline of candidate chain k for chunks of `shape` (fzs.h)."""
+    nx, ny, nz = _dims(shape)
+    nbytes = 4 * nx * ny * nz
+    p = Pipeline(nbytes)
+    if k == 3:
+        q = p.add_quantizer(abs_eb, mode="ABS", quant_radius=Q_RADIUS, outlier_capacity=OUTLIER_CAP, zigzag=True)
+        bp = p.add_bitpack(2, 0)
+        p.connect(bp, q, "codes")
+    else:
+        lz = p.add_lorenzo(abs_eb, mode="ABS", quant_radius=LZ_RADIUS, outlier_capacity=OUTLIER_CAP, zigzag=True)
+        if k == 0:
+            c = p.add_rle(2)
+            p.connect(c, lz, "codes")
+        elif k == 1:
+            bs = p.add_bitshuffle(16384, 2)
+            rz = p.add_rze(16384, 4)
+            p.connect(bs, lz, "codes")
+            p.connect(rz, bs)
+        elif k == 2:
+            c = p.add_huffman(2, HUF_BKLEN)
+            p.connect(c, lz, "codes")
+        else:
+            raise ValueError("chain must be 0..3")
+    p.set_dims(nx, ny, nz)
+    p.finalize()
+    return p
+
+
+def features(d_chunks, n: int, shape, d_eb, stream=None) -> DeviceBuffer:
+    nx, ny, nz = _dims(shape)
+    out = DeviceBuffer(max(4 * NFEAT * n, 16))
+    call("fzs_features_f32", stream, int(d_chunks), n, nx, ny, nz, int(d_eb), out.ptr)
+    return out
+
+
+def label_sizes(d_chunks, n: int, shape, d_eb, stream=None, host_threads=0) -> np.ndarray:
+    """Exact fz::Pipeline::compress sizes, shape (n, 4) uint64."""
+    nx, ny, nz = _dims(shape)
+    ws = DeviceBuffer(max(lib().fzs_label_ws_bytes(n, nx, ny, nz), 256))
+    out = np.zeros((n, NCHAINS), np.uint64)
+    call("fzs_label_sizes", stream, int(d_chunks), n, nx, ny, nz, int(d_eb), out.ctypes.data, ws.ptr, ws.nbytes,
+         int(host_threads))
+    ws.free()
+    return out
+
+
+def regret_costs(sizes: np.ndarray) -> np.ndarray:
+    s = sizes.astype(np.float64)
+    return (s / s.min(axis=1, keepdims=True) - 1.0).astype(np.float32)
+
+
+class Selector:
+    """The MLP; parameters live on the device (FP32 master copy)."""
+
+    def __init__(self, seed: int = 0):
+        self.P = int(lib().fzs_mlp_param_count())
+        rng = np.random.default_rng(seed)
+        p = np.zeros(self.P, np.float32)
+        o = 0
+        for fan_in, fan_out in ((NFEAT, 64), (64, 64), (64, NCHAINS)):
+            lim = np.sqrt(6.0 / fan_in)
+            p[o:o + fan_in * fan_out] = rng.uniform(-lim, lim, fan_in * fan_out)
+            o += fan_in * fan_out + fan_out
+        self.params = DeviceBuffer.from_array(p)
+        self.norm = DeviceBuffer.from_array(np.concatenate([np.zeros(NFEAT, np.float32), np.ones(NFEAT, np.float32)]))
+        self.m = DeviceBuffer.zeros(4 * self.P)
+        self.v = DeviceBuffer.zeros(4 * self.P)
+        self.grad = DeviceBuffer(4 * self.P)
+        self.loss = DeviceBuffer(16)
+        self.step = 0
+        self._ws = None
+
+    def set_params(self, p: np.ndarray) -> None:
+        self.params.upload(np.ascontiguousarray(p, np.float32))
+
+    def get_params(self) -> np.ndarray:
+        return self.params.download(np.float32, self.P)
+
+    def set_norm_from(self, feat: np.ndarray) -> None:
+        mu = feat.mean(axis=0).astype(np.float32)
+        sd = feat.std(axis=0).astype(np.float32)
+        sc = np.where(sd > 1e-6, 1.0 / np.maximum(sd, 1e-6), 1.0).astype(np.float32)
+        self.norm.upload(np.concatenate([mu, sc]).astype(np.float32))
+
+    def get_norm(self) -> np.ndarray:
+        return self.norm.download(np.float32, 2 * NFEAT)
+
+    def forward(self, d_feat, n: int, stream=None) -> np.ndarray:
+        out = DeviceBuffer(max(16 * n, 16))
+        call("fzs_mlp_forward", stream, self.params.ptr, self.norm.ptr, int(d_feat), n, out.ptr)
+        z = out.download(np.float32, n * NCHAINS).reshape(n, NCHAINS)
+        out.free()
+        return z
+
+    def select(self, d_feat, n: int, stream=None) -> np.ndarray:
+        out = DeviceBuffer(max(16 * n, 16))
+        ch = DeviceBuffer(max(n, 16))
+        call("fzs_mlp_forward", stream, self.params.ptr, self.norm.ptr, int(d_feat), n, out.ptr)
+        call("fzs_select", stream, out.ptr, n, ch.ptr)
+        r = ch.download(np.uint8, n)
+        out.free()
+        ch.free()
+        return r
+
+    def train_step(self, d_feat, d_cost, n: int, tau: float, stream=None) -> float:
+        """Gradient of one batch (into self.grad); returns the mean loss."""
+        need = lib().fzs_mlp_train_ws_bytes(n)
+        if self._ws is None or self._ws.nbytes < need:
+            if self._ws is not None:
+                self._ws.free()
+            self._ws = DeviceBuffer(need)
+        call("fzs_mlp_train_step", stream, self.params.ptr, self.norm.ptr, int(d_feat), int(d_cost), n,
+             np.float32(tau), self.grad.ptr, self.loss.ptr, self._ws.ptr, self._ws.nbytes)
+        return float(self.loss.download(np.float32, 1)[0])
+
+    def adam(self, lr: float, b1=0.9, b2=0.999, eps=1e-8, wd=0.0, stream=None) -> None:
+        self.step += 1
+        call("fzs_mlp_adam", stream, self.params.ptr, self.grad.ptr, self.m.ptr, self.v.ptr, self.step,
+             np.float32(lr), np.float32(b1), np.float32(b2), np.float32(eps), np.float32(wd))
+
+    def fit(self, feat: np.ndarray, cost: np.ndarray, epochs=20, batch=1024, lr=3e-3, tau=0.02, seed=0,
+            wd=1e-4) -> list:
+        """Mini-batch training on the device; returns the mean loss per epoch."""
+        n = feat.shape[0]
+        self.set_norm_from(feat)
+        rng = np.random.default_rng(seed)
+        d_feat = DeviceBuffer(4 * NFEAT * batch)
+        d_cost = DeviceBuffer(4 * NCHAINS * batch)
+        hist = []
+        for _ in range(epochs):
+            perm = rng.permutation(n)
+            tot, cnt = 0.0, 0
+            for s in range(0, n - batch + 1 if n >= batch else 1, batch):
+                idx = perm[s:s + batch]
+                d_feat.upload(np.ascontiguousarray(feat[idx], np.float32))
+                d_cost.upload(np.ascontiguousarray(cost[idx], np.float32))
+                tot += self.train_step(d_feat.ptr, d_cost.ptr, len(idx), tau) * len(idx)
+                cnt += len(idx)
+                self.adam(lr, wd=wd)
+            hist.append(tot / max(cnt, 1))
+        d_feat.free()
+        d_cost.free()
+        return hist
+
+    def close(self):
+        for b in (self.params, self.norm, self.m, self.v, self.grad, self.loss, self._ws):
+            if b is not None:
+                b.free()
+
+
+__all__ = ["Selector", "features", "label_sizes", "chain_pipeline", "regret_costs", "CHAINS"]

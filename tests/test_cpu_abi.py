@@ -1,6 +1,6 @@
 """The drop-in boundary on the CPU: libfzh.so loads, exports every function
-include/fzh.h, include/fzp.h and include/fzb.h declare, the ctypes table
-covers them, and
+include/fzh.h, include/fzp.h, include/fzb.h and include/fzs.h declare, the
+ctypes table covers them, and
 the host-only entry points (bounds, codebook build, error strings) work
 without a GPU.  No kernel is launched here."""
 import ctypes as C
@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("fzh.h", "fzp.h", "fzb.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("fzh.h", "fzp.h", "fzb.h", "fzs.h")]
 
 
 def declared():
@@ -19,7 +19,7 @@ def declared():
     for h in HEADERS:
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(fz[hpb]_\w+)\s*\(", text, flags=re.M)
+        names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(fz[hpbs]_\w+)\s*\(", text, flags=re.M)
     return sorted(set(names))
 
 
@@ -29,7 +29,8 @@ def test_headers_declare_the_boundary():
                  "fzh_rze_dec", "fzh_rle_enc", "fzh_rle_dec", "fzh_histogram", "fzh_huffman_encode",
                  "fzh_huffman_decode", "fzh_huffman_build_book", "fzh_quant_abs_fwd_f32", "fzh_bitpack_enc",
                  "fzh_gather", "fzh_concat", "fzp_create", "fzp_compress", "fzp_decompress", "fzp_write_file",
-                 "fzp_decompress_file", "fzb_create", "fzb_set_fields", "fzb_compress", "fzb_autotune"):
+                 "fzp_decompress_file", "fzb_create", "fzb_set_fields", "fzb_compress", "fzb_autotune", "fzs_features_f32",
+                 "fzs_mlp_forward", "fzs_mlp_train_step", "fzs_mlp_adam", "fzs_label_sizes", "fzs_select"):
         assert must in names, must
 
 

@@ -1,0 +1,147 @@
+"""Learned chain selector on the GPU: features and MLP kernels against the
+CPU reference (oracle/selector_ref.py), the exhaustive labeller against the
+real fz::Pipeline compress of every chunk under every chain, and a small
+training run that must beat the selection rules without the network."""
+import numpy as np
+import pytest
+
+from fzgpumodules_amd import fields
+from fzgpumodules_amd._fzh import DeviceBuffer, FzhError, call
+from fzgpumodules_amd import selector as S
+from oracle import selector_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_set():
+    return fields.selector_set(5, 101)  # 5 parent fields per dimensionality, one per regime
+
+
+def _dev(a):
+    return DeviceBuffer.from_array(np.ascontiguousarray(a))
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_features_kernel_matches_reference(small_set, nd):
+    chunks, eb, regime = small_set[nd]
+    pick = np.concatenate([np.where(regime == r)[0][:2] for r in range(5)])
+    c = chunks[pick]
+    d_c, d_eb = _dev(c), _dev(eb[pick])
+    d_f = S.features(d_c.ptr, len(pick), c.shape[1:], d_eb.ptr)
+    got = d_f.download(np.float32, 32 * len(pick)).reshape(-1, 32)
+    for i, j in enumerate(pick):
+        want = R.features(chunks[j], eb[j])
+        np.testing.assert_allclose(got[i], want, rtol=2e-6, atol=2e-6, err_msg=f"chunk {j} regime {regime[j]}")
+
+
+def _rand_params(seed):
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal(R.NP) * 0.25).astype(np.float32)
+    norm = np.concatenate([rng.standard_normal(32) * 0.2, 0.5 + rng.random(32)]).astype(np.float32)
+    return p, norm
+
+
+def test_mlp_forward_is_bit_exact_with_the_bf16_reference():
+    p, norm = _rand_params(1)
+    rng = np.random.default_rng(2)
+    feat = (rng.standard_normal((333, 32)) * 3).astype(np.float32)
+    sel = S.Selector()
+    try:
+        sel.set_params(p)
+        sel.norm.upload(norm)
+        d_f = _dev(feat)
+        z = sel.forward(d_f.ptr, feat.shape[0])
+        want, _ = R.forward(p, norm, feat, "exact32")
+        np.testing.assert_array_equal(z, want)
+        ch = sel.select(d_f.ptr, feat.shape[0])
+        np.testing.assert_array_equal(ch, np.argmax(want, axis=1))
+    finally:
+        sel.close()
+
+
+def test_mlp_gradients_match_the_reference():
+    p, norm = _rand_params(3)
+    rng = np.random.default_rng(4)
+    n = 200
+    feat = (rng.standard_normal((n, 32)) * 2).astype(np.float32)
+    cost = (rng.random((n, 4)) * 0.6).astype(np.float32)
+    cost[np.arange(n), rng.integers(0, 4, n)] = 0
+    sel = S.Selector()
+    try:
+        sel.set_params(p)
+        sel.norm.upload(norm)
+        d_f, d_c = _dev(feat), _dev(cost)
+        loss = sel.train_step(d_f.ptr, d_c.ptr, n, 0.05)
+        g = sel.grad.download(np.float32, R.NP)
+        l32, g32 = R.loss_and_grad(p, norm, feat, cost, 0.05, "exact32")
+        l64, g64 = R.loss_and_grad(p.astype(np.float64), norm.astype(np.float64), feat, cost, 0.05, "f64")
+        assert abs(loss - l32) <= 1e-5 * abs(l32) + 1e-6
+        # same rounding points: agreement to FP32 noise, except where exp()
+        # differs by an ulp before a BF16 rounding of a delta
+        assert np.max(np.abs(g - g32)) <= 2e-3 * np.max(np.abs(g32))
+        assert np.linalg.norm(g - g32) <= 1e-3 * np.linalg.norm(g32)
+        # and the BF16 network's gradient tracks the float64 one
+        assert np.linalg.norm(g - g64) <= 0.05 * np.linalg.norm(g64)
+        # Adam step
+        m0, v0 = np.zeros(R.NP, np.float32), np.zeros(R.NP, np.float32)
+        sel.adam(1e-3, wd=1e-4)
+        p1 = sel.get_params()
+        want, _, _ = R.adam(p, g, m0, v0, 1, 1e-3, wd=1e-4)
+        np.testing.assert_allclose(p1, want, rtol=1e-6, atol=1e-7)
+    finally:
+        sel.close()
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_labeller_equals_pipeline_sizes(small_set, nd):
+    chunks, eb, regime = small_set[nd]
+    pick = np.concatenate([np.where(regime == r)[0][:2] for r in range(5)])
+    c, e = chunks[pick], eb[pick]
+    d_c, d_eb = _dev(c), _dev(e)
+    sizes = S.label_sizes(d_c.ptr, len(pick), c.shape[1:], d_eb.ptr)
+    for k in range(4):
+        for i, j in enumerate(pick):
+            with S.chain_pipeline(k, float(e[i]), c.shape[1:]) as p:
+                blob = p.compress_array(chunks[j])
+                assert sizes[i, k] == blob.size, (nd, S.CHAINS[k], int(regime[j]), int(sizes[i, k]), blob.size)
+                if i % 3 == 0:  # decode from the recorded chain stays within eb
+                    y = p.decompress_array(blob)
+                    err = np.max(np.abs(y[: c[i].size].astype(np.float64) - c[i].ravel().astype(np.float64)))
+                    assert err <= float(e[i]) * 1.01
+
+
+def test_selector_training_beats_rules(small_set):
+    train = fields.selector_set(30, 202)
+    test = small_set
+    sets = {}
+    for name, data in (("train", train), ("test", test)):
+        feats, costs, sizes = [], [], []
+        for nd in (2, 3):
+            ch, eb, _ = data[nd]
+            d_c, d_eb = _dev(ch), _dev(eb)
+            f = S.features(d_c.ptr, len(ch), ch.shape[1:], d_eb.ptr).download(np.float32, 32 * len(ch))
+            s = S.label_sizes(d_c.ptr, len(ch), ch.shape[1:], d_eb.ptr)
+            feats.append(f.reshape(-1, 32))
+            sizes.append(s)
+            d_c.free()
+            d_eb.free()
+        sz = np.concatenate(sizes)
+        sets[name] = (np.concatenate(feats), S.regret_costs(sz), sz)
+    f_tr, c_tr, _ = sets["train"]
+    f_te, c_te, s_te = sets["test"]
+    sel = S.Selector(seed=5)
+    try:
+        hist = sel.fit(f_tr, c_tr, epochs=60, batch=64, lr=3e-3, tau=0.02)
+        assert hist[-1] < hist[0]
+        d_f = _dev(f_te)
+        choice = sel.select(d_f.ptr, f_te.shape[0])
+    finally:
+        sel.close()
+    best = s_te.min(axis=1).sum()
+    learned = s_te[np.arange(len(choice)), choice].sum()
+    regret = learned / best - 1
+    # rules without the network: the best single chain for every chunk
+    single = min(s_te[:, k].sum() for k in range(4)) / best - 1
+    assert regret <= 0.02, regret
+    assert regret < single, (regret, single)
