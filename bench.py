@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-runs", type=int, default=5)
+    ap.add_argument("--batch-fields", type=int, default=4096,
+                    help="fields of the batched workload (0 skips it)")
+    ap.add_argument("--selector-train", type=int, default=8192,
+                    help="training chunks of the learned-selector workload (0 skips it)")
     return ap.parse_args()
 
 
@@ -196,8 +200,13 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(res), flush=True)
     p.close()
+    d_in.free()
+    if args.batch_fields > 0:
+        res["batched"] = batched_workload(args)
+    if args.selector_train > 0:
+        res["selector"] = selector_workload(args)
+    print(json.dumps(res), flush=True)
 
 
 def kernel_times(reps, x, d_in, abs_eb):
@@ -328,6 +337,221 @@ def kernel_times(reps, x, d_in, abs_eb):
     call("fzh_stream_destroy", st)
     out["_outliers"] = k  # noqa: kept for the record
     return {kk: vv for kk, vv in out.items() if not kk.startswith("_")}
+
+
+def batched_workload(args):
+    """Thousands of uneven fields through one persistent launch per stage:
+    sizes log-uniform 64 KiB .. 64 MiB, dimensionality 1D/2D/3D in equal
+    thirds with random aspect ratios, each a window of a periodic Gaussian
+    random field with power ~ k^e (e drawn from U[-5,-1], rounded to one of
+    five base fields per dimensionality); REL 1e-3, radius 32768, zigzag,
+    cap 0.1, strict bound -> Bitshuffle -> RZE; autotuned launch geometry."""
+    from fzgpumodules_amd import fields
+    from fzgpumodules_amd._fzh import DeviceBuffer, call
+    from fzgpumodules_amd.batch import Batch
+    from fzgpumodules_amd.pipeline import Pipeline
+
+    t = time.time()
+    F = args.batch_fields
+    rng = np.random.default_rng(fields.BASE_SEED + 4)
+    expos = (-5.0, -4.0, -3.0, -2.0, -1.0)
+    base_shapes = {1: (1 << 22,), 2: (2048, 2048), 3: (128, 128, 128)}
+    bases = {}
+    for nd, shape in base_shapes.items():
+        for j, e in enumerate(expos):
+            bases[nd, j] = DeviceBuffer.from_array(
+                fields.power_law_field(shape, e, fields.BASE_SEED + 400 + 10 * nd + j).astype(np.float32))
+    nbytes = np.exp(rng.uniform(np.log(64 << 10), np.log(64 << 20), F))
+    nds = rng.permutation(np.arange(F) % 3 + 1)
+    ex = rng.uniform(-5.0, -1.0, F)
+    bufs, dims = [], []
+    for i in range(F):
+        n = max(int(nbytes[i]) // 4, 4096)
+        nd = int(nds[i])
+        if nd == 1:
+            d = (n, 1, 1)
+        elif nd == 2:
+            a = float(np.exp(rng.uniform(-np.log(4), np.log(4))))
+            nx = max(32, int(round(np.sqrt(n * a))))
+            d = (nx, max(2, n // nx), 1)
+        else:
+            a, b = np.exp(rng.uniform(-np.log(2), np.log(2), 2))
+            side = n ** (1.0 / 3.0)
+            nx, ny = max(8, int(round(side * a))), max(8, int(round(side * b)))
+            d = (nx, ny, max(2, n // (nx * ny)))
+        j = int(np.argmin([abs(ex[i] - e) for e in expos]))
+        bs = base_shapes[nd][::-1] + (1,) * (3 - nd)  # (bx, by, bz)
+        buf = DeviceBuffer(4 * d[0] * d[1] * d[2])
+        off = [int(rng.integers(0, b)) for b in bs]
+        call("fzh_periodic_window_f32", None, bases[nd, j].ptr, bs[0], bs[1], bs[2], off[0], off[1], off[2], buf.ptr,
+             d[0], d[1], d[2])
+        bufs.append(buf)
+        dims.append(d)
+    call("fzh_synchronize")
+    for b in bases.values():
+        b.free()
+    total_in = sum(4 * d[0] * d[1] * d[2] for d in dims)
+    log(f"[bench] batched: {F} fields, {total_in / 2**30:.2f} GiB generated in {time.time() - t:.1f}s")
+    b = Batch(1e-3, "REL", 32768, 0.1, True, strict=True, patch_capacity=0.01)
+    b.set_fields([(buf.ptr, d) for buf, d in zip(bufs, dims)])
+    t = time.time()
+    tuned = b.autotune(reps=2)
+    log(f"[bench] batched: autotune {time.time() - t:.1f}s -> {tuned}")
+    b.enable_profiling(True)
+    b.compress()
+    stages = b.stage_ms()
+    b.enable_profiling(False)
+    for _ in range(2):  # eager, then capture
+        b.compress()
+    steps = max(3, min(args.steps, 10))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.compress()
+    t1 = time.perf_counter()
+    sec = (t1 - t0) / steps
+    tin, tout = b.totals()
+    # two fields against the single-field pipeline, byte for byte
+    same = True
+    for i in (0, F // 2):
+        d = dims[i]
+        with Pipeline(4 * d[0] * d[1] * d[2]) as p:
+            lz = p.add_lorenzo(1e-3, mode="REL", quant_radius=32768, outlier_capacity=0.1, zigzag=True)
+            p.set_strict(lz, True, 0.01)
+            bs_ = p.add_bitshuffle(16384, 2)
+            rz = p.add_rze(16384, 4)
+            p.connect(bs_, lz, "codes")
+            p.connect(rz, bs_)
+            p.set_dims(*d)
+            p.finalize()
+            ptr, n = p.compress(bufs[i], 4 * d[0] * d[1] * d[2])
+            want = np.empty(n, np.uint8)
+            call("fzh_memcpy_dtoh", want.ctypes.data, ptr, n)
+        same &= bool(np.array_equal(b.field_bytes(i), want))
+    b.close()
+    for buf in bufs:
+        buf.free()
+    gbs = tin / sec / 1e9
+    log(f"[bench] batched: {sec * 1e3:.2f} ms/compress  {gbs:.1f} GB/s  ratio {tin / tout:.2f}  "
+        f"stages {[(n, round(m, 3)) for n, m in stages]}  fields equal to single-field runs: {same}")
+    stage_ms = {n: round(m, 4) for n, m in stages}
+    lz_gbs = (6 * tin / 4) / (stage_ms["lorenzo"] * 1e-3) / 1e9 if stage_ms.get("lorenzo") else None
+    return {
+        "workload": f"{F} fields, 64 KiB..64 MiB log-uniform, 1D/2D/3D thirds, GRF k^[-5,-1]; REL 1e-3 strict "
+                    "Lorenzo -> Bitshuffle -> RZE, one persistent launch per stage, hipGraph",
+        "fields": F,
+        "input_bytes": int(tin),
+        "compressed_bytes": int(tout),
+        "ratio": round(tin / tout, 3),
+        "compress_gbs": round(gbs, 2),
+        "ms_per_compress": round(sec * 1e3, 3),
+        "stage_ms": stage_ms,
+        "lorenzo_stage_gbs": round(lz_gbs, 1) if lz_gbs else None,
+        "autotuned_launch": tuned,
+        "fields_equal_single_field_pipeline": same,
+    }
+
+
+def selector_workload(args):
+    """Learned chain selection on 256 KiB chunks of five regimes (smooth
+    k^-4 GRF, white noise, 80% zeros, piecewise constant, Poisson 20), 2D
+    256 x 256 and 3D 16 x 64 x 64, each chunk bounded by 1e-3 of its parent
+    field's max |x|: exhaustive labels -> device training -> held-out regret
+    and selection throughput (learned: features + MLP; exhaustive: the
+    labeller, i.e. every chain's exact size)."""
+    from fzgpumodules_amd import fields
+    from fzgpumodules_amd._fzh import DeviceBuffer, Event, call
+    from fzgpumodules_amd import selector as S
+
+    t = time.time()
+    per_dim = max(args.selector_train // 32, 5)  # 16 chunks per parent field, two dimensionalities
+
+    def make(nfields, seed):
+        return fields.selector_set(nfields, seed, fields_shape_2d=(1024, 1024), fields_shape_3d=(64, 128, 128))
+
+    def featurize(data):
+        feats, sizes, dev = [], [], []
+        for nd in (2, 3):
+            ch, eb, _ = data[nd]
+            d_c, d_eb = DeviceBuffer.from_array(ch), DeviceBuffer.from_array(eb)
+            f = S.features(d_c.ptr, len(ch), ch.shape[1:], d_eb.ptr)
+            feats.append(f.download(np.float32, 32 * len(ch)).reshape(-1, 32))
+            f.free()
+            sizes.append(S.label_sizes(d_c.ptr, len(ch), ch.shape[1:], d_eb.ptr))
+            dev.append((d_c, d_eb, len(ch), ch.shape[1:]))
+        return np.concatenate(feats), np.concatenate(sizes), dev
+
+    train = make(per_dim, fields.BASE_SEED + 5)
+    f_tr, s_tr, dev_tr = featurize(train)
+    for d_c, d_eb, _, _ in dev_tr:
+        d_c.free()
+        d_eb.free()
+    del train
+    test = make(max(per_dim // 4, 5), fields.BASE_SEED + 500)
+    f_te, s_te, dev_te = featurize(test)
+    log(f"[bench] selector: {len(f_tr)} training / {len(f_te)} held-out chunks generated and labelled in "
+        f"{time.time() - t:.1f}s")
+    sel = S.Selector(seed=11)
+    t = time.time()
+    hist = sel.fit(f_tr, S.regret_costs(s_tr), epochs=40, batch=256, lr=3e-3, tau=0.02)
+    train_s = time.time() - t
+    d_f = DeviceBuffer.from_array(f_te)
+    choice = sel.select(d_f.ptr, len(f_te))
+    best = s_te.min(axis=1).astype(np.float64)
+    learned = s_te[np.arange(len(choice)), choice].astype(np.float64)
+    in_bytes = 262144 * len(f_te)
+    single = [float(s_te[:, k].astype(np.float64).sum()) for k in range(4)]
+    kfix = int(np.argmin([s_tr[:, k].astype(np.float64).sum() for k in range(4)]))  # best chain on the training set
+    # selection throughput on the device-resident held-out chunks
+    st = C.c_void_p()
+    call("fzh_stream_create", C.byref(st))
+
+    def timed(fn, reps=3):
+        fn()
+        call("fzh_synchronize")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        call("fzh_synchronize")
+        return (time.perf_counter() - t0) / reps
+
+    def learned_select():
+        for d_c, d_eb, n, shape in dev_te:
+            f = S.features(d_c.ptr, n, shape, d_eb.ptr, stream=st.value)
+            sel.select(f.ptr, n, stream=st.value)
+            f.free()
+
+    def exhaustive_select():
+        for d_c, d_eb, n, shape in dev_te:
+            S.label_sizes(d_c.ptr, n, shape, d_eb.ptr, stream=st.value)
+
+    t_learn = timed(learned_select)
+    t_exh = timed(exhaustive_select)
+    call("fzh_stream_destroy", st.value)
+    for d_c, d_eb, _, _ in dev_te:
+        d_c.free()
+        d_eb.free()
+    d_f.free()
+    sel.close()
+    res = {
+        "workload": "256 KiB chunks (2D 256x256, 3D 16x64x64) of 5 regimes, eb = 1e-3 max|parent|; chains "
+                    + ", ".join(S.CHAINS),
+        "train_chunks": int(len(f_tr)),
+        "heldout_chunks": int(len(f_te)),
+        "train_seconds": round(train_s, 2),
+        "train_loss_first_last": [round(hist[0], 4), round(hist[-1], 4)],
+        "ratio_exhaustive": round(in_bytes / best.sum(), 3),
+        "ratio_learned": round(in_bytes / learned.sum(), 3),
+        "regret_learned": round(learned.sum() / best.sum() - 1, 5),
+        "ratio_single_chain_baseline": round(in_bytes / single[kfix], 3),
+        "single_chain_baseline": S.CHAINS[kfix],
+        "regret_single_chain": round(single[kfix] / best.sum() - 1, 5),
+        "choice_accuracy": round(float(np.mean(choice == np.argmin(s_te, axis=1))), 4),
+        "select_gbs_learned": round(in_bytes / t_learn / 1e9, 2),
+        "select_gbs_exhaustive": round(in_bytes / t_exh / 1e9, 2),
+        "select_speedup": round(t_exh / t_learn, 2),
+    }
+    log(f"[bench] selector: {res}")
+    return res
 
 
 def cpu_baseline(x, runs):

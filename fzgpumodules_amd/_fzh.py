@@ -123,6 +123,17 @@ SIGNATURES = {
     "fzs_label_ws_bytes": (_Z, [_Z, _U32, _U32, _U32]),
     "fzs_label_sizes": (_I, [_P, _P, _Z, _U32, _U32, _U32, _P, _P, _P, _Z, _I]),
     "fzs_select": (_I, [_P, _P, _Z, _P]),
+    "fzs_gather_chunks": (_I, [_P, _P, C.c_uint64, C.c_uint64, C.c_uint64, _U32, _U32, _U32, _P]),
+    "fzs_scatter_chunks": (_I, [_P, _P, C.c_uint64, C.c_uint64, C.c_uint64, _U32, _U32, _U32, _P]),
+    "fzs_create": (_I, [_U32, _U32, _U32, C.POINTER(_P)]),
+    "fzs_destroy": (_I, [_P]),
+    "fzs_set_model": (_I, [_P, _P, _P]),
+    "fzs_compress_field": (_I, [_P, _P, _P, C.c_uint64, C.c_uint64, C.c_uint64, _F, _I, _I, C.POINTER(_P),
+                                C.POINTER(_Z)]),
+    "fzs_decompress_field": (_I, [_P, _P, _P, _Z, _P, _Z]),
+    "fzs_last_choices": (_I, [_P, C.POINTER(_P), C.POINTER(_Z)]),
+    "fzs_last_timing": (_I, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_F)]),
+    "fzh_periodic_window_f32": (_I, [_P, _P] + [C.c_uint64] * 6 + [_P] + [C.c_uint64] * 3),
     "fzh_device_alloc": (_I, [C.POINTER(_P), _Z]),
     "fzh_device_free": (_I, [_P]),
     "fzh_memcpy_htod": (_I, [_P, _P, _Z]),

@@ -144,6 +144,16 @@ concat_copy_kernel(ConcatArgs a, uint8_t* __restrict__ out, const uint64_t* __re
     }
 }
 
+// out[z][y][x] = base[(z0 + z) % bz][(y0 + y) % by][(x0 + x) % bx]
+__global__ void __launch_bounds__(256)
+periodic_window_kernel(const float* __restrict__ base, uint64_t bx, uint64_t by, uint64_t bz, uint64_t x0,
+                       uint64_t y0, uint64_t z0, float* __restrict__ out, uint64_t nx, uint64_t ny, uint64_t total) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256) {
+        const uint64_t x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+        out[i] = base[(((z0 + z) % bz) * by + (y0 + y) % by) * bx + (x0 + x) % bx];
+    }
+}
+
 __global__ void __launch_bounds__(64) readback_kernel(ReadbackArgs a, int n, uint8_t* __restrict__ dst) {
     for (int i = 0; i < n; i++) {
         const uint32_t words = a.e[i].bytes / 4;
@@ -186,6 +196,18 @@ int fzh_copy(void* stream, void* d_dst, const void* d_src, size_t bytes) {
     if (bytes % 16)
         copy_tail_kernel<<<1, 256, 0, S(stream)>>>((const uint8_t*)d_src, (uint8_t*)d_dst, nw * 16, bytes);
     FZH_CHECK_LAUNCH("copy_kernel");
+    return FZH_OK;
+}
+
+int fzh_periodic_window_f32(void* stream, const float* d_base, uint64_t bx, uint64_t by, uint64_t bz, uint64_t x0,
+                            uint64_t y0, uint64_t z0, float* d_out, uint64_t nx, uint64_t ny, uint64_t nz) {
+    if (!d_base || !d_out) return fail(FZH_E_INVALID, "periodic window: null pointer");
+    if (!bx || !by || !bz) return fail(FZH_E_INVALID, "periodic window: empty base");
+    const uint64_t total = nx * ny * nz;
+    if (total == 0) return FZH_OK;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(cdiv(total, 256), 65536);
+    periodic_window_kernel<<<blocks, 256, 0, S(stream)>>>(d_base, bx, by, bz, x0, y0, z0, d_out, nx, ny, total);
+    FZH_CHECK_LAUNCH("periodic_window_kernel");
     return FZH_OK;
 }
 

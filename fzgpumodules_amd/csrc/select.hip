@@ -25,6 +25,7 @@
 // master weights are FP32.  Backward rounds the propagated deltas to BF16
 // before they enter a product, and weight gradients are FP32 sums over the
 // batch in sample order.
+#include <algorithm>
 #include <cmath>
 
 #include "fzh_internal.h"
@@ -408,6 +409,24 @@ adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restric
     p[q] = p[q] - lr * (mh / (sqrtf(vh) + eps) + wd * p[q]);
 }
 
+// Chunk-major copy of a field: chunk c = (bz, by, bx) in row-major order of
+// the chunk grid, element (z, y, x) of the chunk at field (bz cz + z, by cy +
+// y, bx cx + x).  TO_CHUNKS = false scatters back.  One thread per element.
+template <bool TO_CHUNKS>
+__global__ void __launch_bounds__(256)
+chunk_copy_kernel(const float* __restrict__ src, float* __restrict__ dst, uint64_t total, uint32_t nx, uint32_t ny,
+                  uint32_t cx, uint32_t cy, uint32_t cz, uint32_t gx, uint32_t gy) {
+    const uint64_t E = (uint64_t)cx * cy * cz;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256) {
+        const uint64_t c = i / E, e = i % E;
+        const uint32_t x = (uint32_t)(e % cx), y = (uint32_t)((e / cx) % cy), z = (uint32_t)(e / ((uint64_t)cx * cy));
+        const uint32_t bx = (uint32_t)(c % gx), by = (uint32_t)((c / gx) % gy), bz = (uint32_t)(c / ((uint64_t)gx * gy));
+        const uint64_t f = ((uint64_t)(bz * cz + z) * ny + (by * cy + y)) * nx + (bx * cx + x);
+        if constexpr (TO_CHUNKS) dst[i] = src[f];
+        else dst[f] = src[i];
+    }
+}
+
 __global__ void __launch_bounds__(256) select_kernel(const float* __restrict__ logits, uint64_t n,
                                                      uint8_t* __restrict__ choice) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -480,6 +499,35 @@ int fzs_mlp_adam(void* stream, float* d_params, const float* d_grad, float* d_m,
                                                                weight_decay, c1, c2);
     FZH_CHECK_LAUNCH("adam_kernel");
     return FZH_OK;
+}
+
+static int chunk_copy(void* stream, const float* src, float* dst, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t cx,
+                      uint32_t cy, uint32_t cz, bool to_chunks) {
+    if (!src || !dst) return fail(FZH_E_INVALID, "chunk copy: null pointer");
+    if (!cx || !cy || !cz || nx % cx || ny % cy || nz % cz)
+        return fail(FZH_E_INVALID, "chunk copy: the field must be a whole number of chunks");
+    const uint64_t total = nx * ny * nz;
+    if (total == 0) return FZH_OK;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(cdiv(total, 256), 65536);
+    const uint32_t gx = (uint32_t)(nx / cx), gy = (uint32_t)(ny / cy);
+    if (to_chunks)
+        chunk_copy_kernel<true><<<blocks, 256, 0, S(stream)>>>(src, dst, total, (uint32_t)nx, (uint32_t)ny, cx, cy, cz,
+                                                               gx, gy);
+    else
+        chunk_copy_kernel<false><<<blocks, 256, 0, S(stream)>>>(src, dst, total, (uint32_t)nx, (uint32_t)ny, cx, cy,
+                                                                cz, gx, gy);
+    FZH_CHECK_LAUNCH("chunk_copy_kernel");
+    return FZH_OK;
+}
+
+int fzs_gather_chunks(void* stream, const float* d_field, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t cx,
+                      uint32_t cy, uint32_t cz, float* d_chunks) {
+    return chunk_copy(stream, d_field, d_chunks, nx, ny, nz, cx, cy, cz, true);
+}
+
+int fzs_scatter_chunks(void* stream, const float* d_chunks, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t cx,
+                       uint32_t cy, uint32_t cz, float* d_field) {
+    return chunk_copy(stream, d_chunks, d_field, nx, ny, nz, cx, cy, cz, false);
 }
 
 int fzs_select(void* stream, const float* d_logits, size_t n, uint8_t* d_choice) {

@@ -177,4 +177,60 @@ class Selector:
                 b.free()
 
 
-__all__ = ["Selector", "features", "label_sizes", "chain_pipeline", "regret_costs", "CHAINS"]
+LEARNED, EXHAUSTIVE, FIXED = 0, 1, 2
+
+
+class SelectingCompressor:
+    """fzs_compress_field / fzs_decompress_field: a field cut into chunks of
+    `chunk` ((ny, nx) or (nz, ny, nx)), each compressed with its chain."""
+
+    def __init__(self, chunk):
+        self.chunk = tuple(chunk)
+        nx, ny, nz = _dims(chunk)
+        h = C.c_void_p()
+        call("fzs_create", nx, ny, nz, C.byref(h))
+        self.h = h.value
+
+    def set_model(self, params: np.ndarray, norm: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        q = np.ascontiguousarray(norm, np.float32)
+        call("fzs_set_model", self.h, p.ctypes.data, q.ctypes.data)
+
+    def compress(self, d_field, shape, rel_eb=1e-3, mode=LEARNED, chain=0, stream=None) -> np.ndarray:
+        nx, ny, nz = _dims(shape)
+        p, n = C.c_void_p(), C.c_size_t()
+        call("fzs_compress_field", self.h, stream, int(d_field), nx, ny, nz, np.float32(rel_eb), int(mode),
+             int(chain), C.byref(p), C.byref(n))
+        return np.ctypeslib.as_array((C.c_uint8 * n.value).from_address(p.value)).copy() if n.value else \
+            np.zeros(0, np.uint8)
+
+    def decompress(self, blob: np.ndarray, d_out, n_elems: int, stream=None) -> None:
+        b = np.ascontiguousarray(blob, np.uint8)
+        call("fzs_decompress_field", self.h, stream, b.ctypes.data, b.size, int(d_out), int(n_elems))
+
+    def choices(self) -> np.ndarray:
+        p, n = C.c_void_p(), C.c_size_t()
+        call("fzs_last_choices", self.h, C.byref(p), C.byref(n))
+        if not n.value:
+            return np.zeros(0, np.uint8)
+        return np.ctypeslib.as_array((C.c_uint8 * n.value).from_address(p.value)).copy()
+
+    def timing(self):
+        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        call("fzs_last_timing", self.h, C.byref(a), C.byref(b), C.byref(c))
+        return {"select_ms": a.value, "compress_ms": b.value, "total_ms": c.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fzs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["Selector", "SelectingCompressor", "features", "label_sizes", "chain_pipeline", "regret_costs", "CHAINS",
+           "LEARNED", "EXHAUSTIVE", "FIXED"]

@@ -228,6 +228,12 @@ typedef struct fzh_copy_desc {
 } fzh_copy_desc;
 int fzh_gather(void* stream, const fzh_copy_desc* d_descs, int n_segs);
 
+/* Synthetic-data helper: an nx*ny*nz window of a periodic base field
+ * (bx*by*bz, x fastest) starting at (x0, y0, z0), wrapping at the edges.   */
+int fzh_periodic_window_f32(void* stream, const float* d_base, uint64_t bx, uint64_t by, uint64_t bz,
+                            uint64_t x0, uint64_t y0, uint64_t z0, float* d_out, uint64_t nx, uint64_t ny,
+                            uint64_t nz);
+
 /* Plain device copy (grid-stride 16-byte accesses; 16-byte-aligned
  * pointers), capture-safe.  Also the bandwidth reference for benchmarks.   */
 int fzh_copy(void* stream, void* d_dst, const void* d_src, size_t bytes);

@@ -91,6 +91,42 @@ int fzs_label_sizes(void* stream, const float* d_chunks, size_t n_chunks, uint32
 /* d_choice[i] = argmax of logits row i (lowest index on ties). */
 int fzs_select(void* stream, const float* d_logits, size_t n, uint8_t* d_choice);
 
+/* Chunk-major copy of an nx*ny*nz field (x fastest) that is a whole number
+ * of cx*cy*cz chunks: chunk (bz, by, bx) in row-major order of the chunk
+ * grid, and back.                                                          */
+int fzs_gather_chunks(void* stream, const float* d_field, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t cx,
+                      uint32_t cy, uint32_t cz, float* d_chunks);
+int fzs_scatter_chunks(void* stream, const float* d_chunks, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t cx,
+                       uint32_t cy, uint32_t cz, float* d_field);
+
+/* ---- selecting compressor -------------------------------------------------
+ * Compresses a field chunk by chunk, each chunk with the chain the selector
+ * picks (FZS_MODE_LEARNED: the MLP; FZS_MODE_EXHAUSTIVE: the smallest size
+ * from the labeller; FZS_MODE_FIXED: `fixed_chain` everywhere).  The bound
+ * is REL: rel_eb * max|field| for every chunk.  The result is one host
+ * container (FZS1, layout in host_select.cpp) that records every chunk's
+ * chain and the chains' stage records (FZMStageInfo), followed by each
+ * chunk's fz::Pipeline result; fzs_decompress_field rebuilds the recorded
+ * chains and decodes every chunk exactly as fz::Pipeline::decompress.     */
+#define FZS_MODE_LEARNED 0
+#define FZS_MODE_EXHAUSTIVE 1
+#define FZS_MODE_FIXED 2
+typedef struct fzs_compressor fzs_compressor;
+int fzs_create(uint32_t cx, uint32_t cy, uint32_t cz, fzs_compressor** out);
+int fzs_destroy(fzs_compressor* c);
+/* Host arrays: fzs_mlp_param_count() parameters and 2 * FZS_NFEAT norm. */
+int fzs_set_model(fzs_compressor* c, const float* h_params, const float* h_norm);
+/* *h_out is owned by the handle, valid until its next compress.           */
+int fzs_compress_field(fzs_compressor* c, void* stream, const float* d_field, uint64_t nx, uint64_t ny,
+                       uint64_t nz, float rel_eb, int mode, int fixed_chain, const uint8_t** h_out,
+                       size_t* out_bytes);
+int fzs_decompress_field(fzs_compressor* c, void* stream, const uint8_t* h_in, size_t in_bytes, float* d_out,
+                         size_t out_elems);
+/* Chains of the last compress (one per chunk) and its timing split (ms):
+ * choosing the chains, compressing the chunks, the whole call.            */
+int fzs_last_choices(fzs_compressor* c, const uint8_t** h_choices, size_t* n);
+int fzs_last_timing(fzs_compressor* c, float* select_ms, float* compress_ms, float* total_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,37 +111,87 @@ def test_labeller_equals_pipeline_sizes(small_set, nd):
                     assert err <= float(e[i]) * 1.01
 
 
-def test_selector_training_beats_rules(small_set):
-    train = fields.selector_set(30, 202)
-    test = small_set
-    sets = {}
-    for name, data in (("train", train), ("test", test)):
-        feats, costs, sizes = [], [], []
-        for nd in (2, 3):
-            ch, eb, _ = data[nd]
-            d_c, d_eb = _dev(ch), _dev(eb)
-            f = S.features(d_c.ptr, len(ch), ch.shape[1:], d_eb.ptr).download(np.float32, 32 * len(ch))
-            s = S.label_sizes(d_c.ptr, len(ch), ch.shape[1:], d_eb.ptr)
-            feats.append(f.reshape(-1, 32))
-            sizes.append(s)
-            d_c.free()
-            d_eb.free()
-        sz = np.concatenate(sizes)
-        sets[name] = (np.concatenate(feats), S.regret_costs(sz), sz)
-    f_tr, c_tr, _ = sets["train"]
-    f_te, c_te, s_te = sets["test"]
+def _featurize(data):
+    feats, sizes = [], []
+    for nd in (2, 3):
+        ch, eb, _ = data[nd]
+        d_c, d_eb = _dev(ch), _dev(eb)
+        f = S.features(d_c.ptr, len(ch), ch.shape[1:], d_eb.ptr).download(np.float32, 32 * len(ch))
+        feats.append(f.reshape(-1, 32))
+        sizes.append(S.label_sizes(d_c.ptr, len(ch), ch.shape[1:], d_eb.ptr))
+        d_c.free()
+        d_eb.free()
+    sz = np.concatenate(sizes)
+    return np.concatenate(feats), S.regret_costs(sz), sz
+
+
+@pytest.fixture(scope="module")
+def trained():
+    f_tr, c_tr, _ = _featurize(fields.selector_set(30, 202))
     sel = S.Selector(seed=5)
     try:
         hist = sel.fit(f_tr, c_tr, epochs=60, batch=64, lr=3e-3, tau=0.02)
-        assert hist[-1] < hist[0]
-        d_f = _dev(f_te)
-        choice = sel.select(d_f.ptr, f_te.shape[0])
+        return sel.get_params(), sel.get_norm(), hist
+    finally:
+        sel.close()
+
+
+def test_selector_training_beats_rules(small_set, trained):
+    params, norm, hist = trained
+    assert hist[-1] < hist[0]
+    f_te, _, s_te = _featurize(small_set)
+    sel = S.Selector()
+    try:
+        sel.set_params(params)
+        sel.norm.upload(norm)
+        choice = sel.select(_dev(f_te).ptr, f_te.shape[0])
     finally:
         sel.close()
     best = s_te.min(axis=1).sum()
     learned = s_te[np.arange(len(choice)), choice].sum()
     regret = learned / best - 1
-    # rules without the network: the best single chain for every chunk
+    # the method without the network: the best single chain for every chunk
     single = min(s_te[:, k].sum() for k in range(4)) / best - 1
     assert regret <= 0.02, regret
     assert regret < single, (regret, single)
+
+
+def _composite_field():
+    """A 512 x 1024 field whose eight 256 x 256 chunks come from all regimes."""
+    tiles = [fields.regime_field(r % 5, (256, 256), 900 + r) for r in range(8)]
+    rows = [np.concatenate(tiles[4 * i:4 * i + 4], axis=1) for i in range(2)]
+    return np.ascontiguousarray(np.concatenate(rows, axis=0))
+
+
+def test_selecting_compressor_round_trip(trained):
+    params, norm, _ = trained
+    x = _composite_field()
+    d = _dev(x)
+    out = DeviceBuffer(x.nbytes)
+    sc = S.SelectingCompressor(fields.CHUNK_2D)
+    try:
+        sc.set_model(params, norm)
+        eb = float(np.float32(np.float32(1e-3) * np.float32(np.abs(x).max())))
+        sizes = {}
+        for mode in (S.LEARNED, S.EXHAUSTIVE, S.FIXED):
+            blob = sc.compress(d.ptr, x.shape, 1e-3, mode=mode, chain=1)
+            ch = sc.choices()
+            assert ch.size == 8
+            if mode == S.FIXED:
+                assert np.all(ch == 1)
+            sizes[mode] = blob.size
+            out.upload(np.zeros(x.size, np.float32))
+            sc.decompress(blob, out.ptr, x.size)
+            y = out.download(np.float32, x.size).reshape(x.shape)
+            err = np.max(np.abs(y.astype(np.float64) - x.astype(np.float64)))
+            assert err <= eb * 1.01, (mode, err, eb)
+        assert sizes[S.EXHAUSTIVE] <= sizes[S.LEARNED] <= sizes[S.FIXED] * 1.02
+        blob = sc.compress(d.ptr, x.shape, 1e-3, mode=S.LEARNED)
+        bad = blob.copy()
+        bad[-5] ^= 0xFF
+        with pytest.raises(FzhError, match="checksum"):
+            sc.decompress(bad, out.ptr, x.size)
+    finally:
+        sc.close()
+        d.free()
+        out.free()
