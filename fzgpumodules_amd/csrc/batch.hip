@@ -11,7 +11,8 @@
 // Launch sequence of one compress (all on one stream, captured as a graph):
 //   value_base  per-unit min/max, then one wave per field resolves eb
 //   lorenzo     codes + per-tile lists (kListSlots entries) + per-tile counts
-//   rze         fused Bitshuffle+RZE per chunk into a chunk scratch slot
+//   rze         fused Bitshuffle+RZE, one wave per chunk-table entry, into
+//               a chunk scratch slot
 //   scan        one exclusive scan (mod 2^32) over [tile outliers | tile
 //               patches | chunk bytes]; per-field offsets are differences
 //   finalize    one thread per field: result layout, concat header, pads
@@ -42,7 +43,8 @@ __device__ __forceinline__ uint32_t find_field(const uint32_t* __restrict__ star
 // value base: per-unit min/max (first element and the zero padding folded in
 // exactly as the single-field reduction does), then one wave per field
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) vb_partial_kernel(Plan p) {
+__global__ void __launch_bounds__(256) vb_partial_kernel(const Plan* __restrict__ pp) {
+    const Plan& p = *pp;
     __shared__ float smn[4], smx[4];
     const int w = threadIdx.x >> 6;
     for (uint32_t u = blockIdx.x; u < p.U; u += gridDim.x) {
@@ -93,7 +95,8 @@ __global__ void __launch_bounds__(256) vb_partial_kernel(Plan p) {
     }
 }
 
-__global__ void __launch_bounds__(256) vb_finalize_kernel(Plan p) {
+__global__ void __launch_bounds__(256) vb_finalize_kernel(const Plan* __restrict__ pp) {
+    const Plan& p = *pp;
     const uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (f >= p.F) return;
     float gmn = INFINITY, gmx = -INFINITY;
@@ -136,7 +139,8 @@ __device__ __forceinline__ TileCounts run_tile(const Plan& p, const Field& fd, u
 }
 
 template <bool ZZ, bool STRICT>
-__global__ void __launch_bounds__(256) lz_batch_kernel(Plan p, uint32_t G) {
+__global__ void __launch_bounds__(256) lz_batch_kernel(const Plan* __restrict__ pp, uint32_t G) {
+    const Plan& p = *pp;
     const uint32_t waves = gridDim.x * 4;
     const uint32_t nu = (p.T + G - 1) / G;
     for (uint32_t u = blockIdx.x * 4 + (threadIdx.x >> 6); u < nu; u += waves) {
@@ -158,37 +162,38 @@ __global__ void __launch_bounds__(256) lz_batch_kernel(Plan p, uint32_t G) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused Bitshuffle + RZE over all chunks of all fields
+// Fused Bitshuffle + RZE over all chunks of all fields: one wave per entry of
+// the chunk table (the encoder keeps a whole 16 KiB chunk in registers, about
+// 236 VGPRs, so a persistent loop around it only adds spills); WV waves per
+// workgroup is the tuned parameter.
 // ---------------------------------------------------------------------------
-constexpr int kRzeWaves = 2;
-constexpr size_t kRzeLds = (size_t)kRzeWaves * LDS_PER_WAVE;
-
-__global__ void __launch_bounds__(64 * kRzeWaves) rze_batch_kernel(Plan p) {
+template <int WV>
+__global__ void __launch_bounds__(64 * WV) rze_batch_kernel(const Plan* __restrict__ pp) {
+    const Plan& p = *pp;
     extern __shared__ __align__(16) uint8_t smem[];
     const int wv = threadIdx.x >> 6;
     uint8_t* st = smem + wv * LDS_PER_WAVE;
     uint8_t* bm1 = st + LDS_STREAM;
-    const uint32_t waves = gridDim.x * kRzeWaves;
+    const uint32_t gc = blockIdx.x * WV + wv;
+    if (gc >= p.NC) return;
     const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
-    for (uint32_t gc = blockIdx.x * kRzeWaves + wv; gc < p.NC; gc += waves) {
-        const uint32_t f = find_field(p.chunk0, p.F, gc);
-        const Field& fd = p.fields[f];
-        const uint32_t lc = gc - fd.chunk0;
-        const uint64_t cb = 2 * fd.n;  // code bytes
-        const uint64_t off = (uint64_t)lc * kChunk;
-        const int in_size = (int)min((uint64_t)kChunk, cb - off);
-        const uint8_t* src = reinterpret_cast<const uint8_t*>(p.codes + fd.codes_off) + off;
-        const uint32_t size = rze_encode_chunk<2>(src, in_size, off + kChunk <= cb, st, bm1,
-                                                  p.chunk_scratch + (uint64_t)gc * kChunk);
-        if (lane_id() == 0) {
-            p.sizes[gc] = size;
-            p.counts[cbase + gc] = size & 0x7FFFFFFFu;
-            uint32_t* rz = reinterpret_cast<uint32_t*>(p.out + fd.out_off + p.hdr);
-            rz[2 + lc] = size;
-            if (lc == 0) {
-                rz[0] = (uint32_t)cb;
-                rz[1] = fd.nchunks;
-            }
+    const uint32_t f = find_field(p.chunk0, p.F, gc);
+    const Field& fd = p.fields[f];
+    const uint32_t lc = gc - fd.chunk0;
+    const uint64_t cb = 2 * fd.n;  // code bytes
+    const uint64_t off = (uint64_t)lc * kChunk;
+    const int in_size = (int)min((uint64_t)kChunk, cb - off);
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(p.codes + fd.codes_off) + off;
+    const uint32_t size =
+        rze_encode_chunk<2>(src, in_size, off + kChunk <= cb, st, bm1, p.chunk_scratch + (uint64_t)gc * kChunk);
+    if (lane_id() == 0) {
+        p.sizes[gc] = size;
+        p.counts[cbase + gc] = size & 0x7FFFFFFFu;
+        uint32_t* rz = reinterpret_cast<uint32_t*>(p.out + fd.out_off + p.hdr);
+        rz[2 + lc] = size;
+        if (lc == 0) {
+            rz[0] = (uint32_t)cb;
+            rz[1] = fd.nchunks;
         }
     }
 }
@@ -202,7 +207,8 @@ __device__ __forceinline__ void zero_range(uint8_t* b, uint64_t from, uint64_t t
     for (uint64_t i = from; i < to; i++) b[i] = 0;
 }
 
-__global__ void __launch_bounds__(256) finalize_kernel(Plan p) {
+__global__ void __launch_bounds__(256) finalize_kernel(const Plan* __restrict__ pp) {
+    const Plan& p = *pp;
     const uint32_t f = blockIdx.x * 256 + threadIdx.x;
     if (f >= p.F) return;
     const Field& fd = p.fields[f];
@@ -266,7 +272,8 @@ __global__ void __launch_bounds__(256) finalize_kernel(Plan p) {
 // ---------------------------------------------------------------------------
 // Chunks to their place in the field's RZE stream
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) pack_batch_kernel(Plan p) {
+__global__ void __launch_bounds__(256) pack_batch_kernel(const Plan* __restrict__ pp) {
+    const Plan& p = *pp;
     const uint32_t waves = gridDim.x * 4;
     const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
     for (uint32_t gc = blockIdx.x * 4 + (threadIdx.x >> 6); gc < p.NC; gc += waves) {
@@ -295,7 +302,8 @@ __global__ void __launch_bounds__(256) pack_batch_kernel(Plan p) {
 // than kListSlots is produced again by recomputing the tile (lists only).
 // ---------------------------------------------------------------------------
 template <bool ZZ, bool STRICT>
-__global__ void __launch_bounds__(256) scatter_batch_kernel(Plan p) {
+__global__ void __launch_bounds__(256) scatter_batch_kernel(const Plan* __restrict__ pp) {
+    const Plan& p = *pp;
     const uint32_t waves = gridDim.x * 4;
     const int l = lane_id();
     const uint32_t nblk = (p.T + 63) / 64;
@@ -365,31 +373,36 @@ __global__ void __launch_bounds__(256) readback_batch_kernel(const uint32_t* __r
 
 size_t scan_ws_bytes_for(size_t n_counts) { return fzh::scan_ws_bytes(n_counts); }
 
-int launch_value_base(hipStream_t s, const Plan& p, uint32_t grid) {
-    if (p.mode != 0 && p.U) vb_partial_kernel<<<std::max<uint32_t>(1, std::min(grid, p.U)), 256, 0, s>>>(p);
-    vb_finalize_kernel<<<(unsigned)cdiv(p.F, 4), 256, 0, s>>>(p);
+int launch_value_base(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid) {
+    if (p.mode != 0 && p.U) vb_partial_kernel<<<std::max<uint32_t>(1, std::min(grid, p.U)), 256, 0, s>>>(dp);
+    vb_finalize_kernel<<<(unsigned)cdiv(p.F, 4), 256, 0, s>>>(dp);
     FZH_CHECK_LAUNCH("batch value base");
     return FZH_OK;
 }
 
-int launch_lorenzo(hipStream_t s, const Plan& p, uint32_t grid, uint32_t G) {
+int launch_lorenzo(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid, uint32_t G) {
     G = std::max<uint32_t>(1, G);
     const uint32_t nu = (uint32_t)cdiv(p.T, G);
     const unsigned g = std::max<uint32_t>(1, std::min<uint32_t>(grid, (uint32_t)cdiv(nu, 4)));
     if (p.zigzag) {
-        if (p.strict) lz_batch_kernel<true, true><<<g, 256, 0, s>>>(p, G);
-        else lz_batch_kernel<true, false><<<g, 256, 0, s>>>(p, G);
+        if (p.strict) lz_batch_kernel<true, true><<<g, 256, 0, s>>>(dp, G);
+        else lz_batch_kernel<true, false><<<g, 256, 0, s>>>(dp, G);
     } else {
-        if (p.strict) lz_batch_kernel<false, true><<<g, 256, 0, s>>>(p, G);
-        else lz_batch_kernel<false, false><<<g, 256, 0, s>>>(p, G);
+        if (p.strict) lz_batch_kernel<false, true><<<g, 256, 0, s>>>(dp, G);
+        else lz_batch_kernel<false, false><<<g, 256, 0, s>>>(dp, G);
     }
     FZH_CHECK_LAUNCH("batch lorenzo");
     return FZH_OK;
 }
 
-int launch_rze(hipStream_t s, const Plan& p, uint32_t grid) {
-    const unsigned g = std::max<uint32_t>(1, std::min<uint32_t>(grid, (uint32_t)cdiv(p.NC, kRzeWaves)));
-    rze_batch_kernel<<<g, 64 * kRzeWaves, kRzeLds, s>>>(p);
+int launch_rze(hipStream_t s, const Plan& p, const Plan* dp, uint32_t waves_per_wg) {
+    const size_t lds = (size_t)LDS_PER_WAVE;
+    if (waves_per_wg >= 4)
+        rze_batch_kernel<4><<<(unsigned)cdiv(p.NC, 4), 256, 4 * lds, s>>>(dp);
+    else if (waves_per_wg >= 2)
+        rze_batch_kernel<2><<<(unsigned)cdiv(p.NC, 2), 128, 2 * lds, s>>>(dp);
+    else
+        rze_batch_kernel<1><<<(unsigned)p.NC, 64, lds, s>>>(dp);
     FZH_CHECK_LAUNCH("batch rze");
     return FZH_OK;
 }
@@ -398,27 +411,27 @@ int launch_scan(hipStream_t s, const Plan& p) {
     return fzh::scan_u32<true>(s, p.counts, p.excl, p.n_counts, nullptr, p.excl + p.n_counts, p.scan_ws);
 }
 
-int launch_finalize(hipStream_t s, const Plan& p) {
-    finalize_kernel<<<(unsigned)cdiv(p.F, 256), 256, 0, s>>>(p);
+int launch_finalize(hipStream_t s, const Plan& p, const Plan* dp) {
+    finalize_kernel<<<(unsigned)cdiv(p.F, 256), 256, 0, s>>>(dp);
     FZH_CHECK_LAUNCH("batch finalize");
     return FZH_OK;
 }
 
-int launch_pack(hipStream_t s, const Plan& p, uint32_t grid) {
+int launch_pack(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid) {
     const unsigned g = std::max<uint32_t>(1, std::min<uint32_t>(grid, (uint32_t)cdiv(p.NC, 4)));
-    pack_batch_kernel<<<g, 256, 0, s>>>(p);
+    pack_batch_kernel<<<g, 256, 0, s>>>(dp);
     FZH_CHECK_LAUNCH("batch pack");
     return FZH_OK;
 }
 
-int launch_scatter(hipStream_t s, const Plan& p, uint32_t grid) {
+int launch_scatter(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid) {
     const unsigned g = std::max<uint32_t>(1, std::min<uint32_t>(grid, (uint32_t)cdiv(cdiv(p.T, 64), 4)));
     if (p.zigzag) {
-        if (p.strict) scatter_batch_kernel<true, true><<<g, 256, 0, s>>>(p);
-        else scatter_batch_kernel<true, false><<<g, 256, 0, s>>>(p);
+        if (p.strict) scatter_batch_kernel<true, true><<<g, 256, 0, s>>>(dp);
+        else scatter_batch_kernel<true, false><<<g, 256, 0, s>>>(dp);
     } else {
-        if (p.strict) scatter_batch_kernel<false, true><<<g, 256, 0, s>>>(p);
-        else scatter_batch_kernel<false, false><<<g, 256, 0, s>>>(p);
+        if (p.strict) scatter_batch_kernel<false, true><<<g, 256, 0, s>>>(dp);
+        else scatter_batch_kernel<false, false><<<g, 256, 0, s>>>(dp);
     }
     FZH_CHECK_LAUNCH("batch scatter");
     return FZH_OK;

@@ -35,6 +35,7 @@ uint64_t align_to(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
 struct BatchCompressor::Impl {
     fzh::batch::Plan plan{};
+    fzh::batch::Plan* d_plan = nullptr;  // device copy read by the kernels
     std::vector<void*> allocs;
     fzh::batch::Result* h_results = nullptr;  // host-mapped
     void* d_results_map = nullptr;
@@ -224,6 +225,8 @@ void BatchCompressor::setFields(const std::vector<BatchField>& fields) {
     impl_->h_results = (Result*)mapped;
     std::memset(mapped, 0, sizeof(Result) * F);
     hipck(hipHostGetDevicePointer(&impl_->d_results_map, mapped, 0), "hipHostGetDevicePointer");
+    impl_->d_plan = (Plan*)impl_->dev(sizeof(Plan));
+    hipck(hipMemcpy(impl_->d_plan, &p, sizeof(Plan), hipMemcpyHostToDevice), "hipMemcpy (plan)");
     for (auto& e : impl_->ev) hipck(hipEventCreate(&e), "hipEventCreate");
 }
 
@@ -233,7 +236,7 @@ void BatchCompressor::setLaunch(const BatchLaunch& l) {
     launch_.vb_wg_per_cu = fix(l.vb_wg_per_cu, d.vb_wg_per_cu);
     launch_.lz_wg_per_cu = fix(l.lz_wg_per_cu, d.lz_wg_per_cu);
     launch_.lz_tiles_per_unit = fix(l.lz_tiles_per_unit, d.lz_tiles_per_unit);
-    launch_.rze_wg_per_cu = fix(l.rze_wg_per_cu, d.rze_wg_per_cu);
+    launch_.rze_waves_per_wg = fix(l.rze_waves_per_wg, d.rze_waves_per_wg);
     launch_.pack_wg_per_cu = fix(l.pack_wg_per_cu, d.pack_wg_per_cu);
     launch_.scatter_wg_per_cu = fix(l.scatter_wg_per_cu, d.scatter_wg_per_cu);
     destroyGraph();
@@ -247,23 +250,24 @@ void BatchCompressor::enableGraphMode(bool on) {
 void BatchCompressor::enqueue(hipStream_t s) {
     using namespace fzh::batch;
     const Plan& p = impl_->plan;
+    const Plan* dp = impl_->d_plan;
     const uint32_t cu = (uint32_t)cus_;
     const bool prof = profiling_;
     hipEvent_t* ev = impl_->ev;
     if (prof) hipck(hipEventRecord(ev[0], s), "hipEventRecord");
-    fzck(launch_value_base(s, p, cu * launch_.vb_wg_per_cu), "batch value base");
+    fzck(launch_value_base(s, p, dp, cu * launch_.vb_wg_per_cu), "batch value base");
     if (prof) hipck(hipEventRecord(ev[1], s), "hipEventRecord");
-    fzck(launch_lorenzo(s, p, cu * launch_.lz_wg_per_cu, launch_.lz_tiles_per_unit), "batch lorenzo");
+    fzck(launch_lorenzo(s, p, dp, cu * launch_.lz_wg_per_cu, launch_.lz_tiles_per_unit), "batch lorenzo");
     if (prof) hipck(hipEventRecord(ev[2], s), "hipEventRecord");
-    fzck(launch_rze(s, p, cu * launch_.rze_wg_per_cu), "batch rze");
+    fzck(launch_rze(s, p, dp, launch_.rze_waves_per_wg), "batch rze");
     if (prof) hipck(hipEventRecord(ev[3], s), "hipEventRecord");
     fzck(launch_scan(s, p), "batch scan");
     if (prof) hipck(hipEventRecord(ev[4], s), "hipEventRecord");
-    fzck(launch_finalize(s, p), "batch finalize");
+    fzck(launch_finalize(s, p, dp), "batch finalize");
     if (prof) hipck(hipEventRecord(ev[5], s), "hipEventRecord");
-    fzck(launch_pack(s, p, cu * launch_.pack_wg_per_cu), "batch pack");
+    fzck(launch_pack(s, p, dp, cu * launch_.pack_wg_per_cu), "batch pack");
     if (prof) hipck(hipEventRecord(ev[6], s), "hipEventRecord");
-    fzck(launch_scatter(s, p, cu * launch_.scatter_wg_per_cu), "batch scatter");
+    fzck(launch_scatter(s, p, dp, cu * launch_.scatter_wg_per_cu), "batch scatter");
     if (prof) hipck(hipEventRecord(ev[7], s), "hipEventRecord");
     fzck(launch_readback(s, p, impl_->d_results_map), "batch readback");
 }
@@ -348,6 +352,7 @@ BatchLaunch BatchCompressor::autotune(hipStream_t s, int reps) {
     destroyGraph();
     compress(s);  // every buffer the kernels read now holds this batch's data
     const Plan& p = impl_->plan;
+    const Plan* dp = impl_->d_plan;
     const uint32_t cu = (uint32_t)cus_;
     hipEvent_t e0 = impl_->ev[0], e1 = impl_->ev[1];
     auto time = [&](auto&& launch) {
@@ -363,28 +368,28 @@ BatchLaunch BatchCompressor::autotune(hipStream_t s, int reps) {
     BatchLaunch best = launch_;
     float bt = std::numeric_limits<float>::max();
     for (uint32_t w : {1u, 2u, 4u, 8u}) {
-        const float t = time([&] { fzck(launch_value_base(s, p, cu * w), "tune value base"); });
+        const float t = time([&] { fzck(launch_value_base(s, p, dp, cu * w), "tune value base"); });
         if (t < bt) { bt = t; best.vb_wg_per_cu = w; }
     }
     bt = std::numeric_limits<float>::max();
     for (uint32_t w : {2u, 4u, 8u, 16u})
         for (uint32_t g : {1u, 4u, 16u}) {
-            const float t = time([&] { fzck(launch_lorenzo(s, p, cu * w, g), "tune lorenzo"); });
+            const float t = time([&] { fzck(launch_lorenzo(s, p, dp, cu * w, g), "tune lorenzo"); });
             if (t < bt) { bt = t; best.lz_wg_per_cu = w; best.lz_tiles_per_unit = g; }
         }
     bt = std::numeric_limits<float>::max();
-    for (uint32_t w : {1u, 2u, 3u, 4u, 8u}) {
-        const float t = time([&] { fzck(launch_rze(s, p, cu * w), "tune rze"); });
-        if (t < bt) { bt = t; best.rze_wg_per_cu = w; }
+    for (uint32_t w : {1u, 2u, 4u}) {
+        const float t = time([&] { fzck(launch_rze(s, p, dp, w), "tune rze"); });
+        if (t < bt) { bt = t; best.rze_waves_per_wg = w; }
     }
     bt = std::numeric_limits<float>::max();
     for (uint32_t w : {1u, 2u, 4u, 8u}) {
-        const float t = time([&] { fzck(launch_pack(s, p, cu * w), "tune pack"); });
+        const float t = time([&] { fzck(launch_pack(s, p, dp, cu * w), "tune pack"); });
         if (t < bt) { bt = t; best.pack_wg_per_cu = w; }
     }
     bt = std::numeric_limits<float>::max();
     for (uint32_t w : {1u, 2u, 4u, 8u}) {
-        const float t = time([&] { fzck(launch_scatter(s, p, cu * w), "tune scatter"); });
+        const float t = time([&] { fzck(launch_scatter(s, p, dp, cu * w), "tune scatter"); });
         if (t < bt) { bt = t; best.scatter_wg_per_cu = w; }
     }
     hipck(hipStreamSynchronize(s), "hipStreamSynchronize (tune)");
@@ -503,7 +508,7 @@ static fz::BatchLaunch from_c(const fzb_launch& l) {
     r.vb_wg_per_cu = l.vb_wg_per_cu;
     r.lz_wg_per_cu = l.lz_wg_per_cu;
     r.lz_tiles_per_unit = l.lz_tiles_per_unit;
-    r.rze_wg_per_cu = l.rze_wg_per_cu;
+    r.rze_waves_per_wg = l.rze_waves_per_wg;
     r.pack_wg_per_cu = l.pack_wg_per_cu;
     r.scatter_wg_per_cu = l.scatter_wg_per_cu;
     return r;
@@ -513,7 +518,7 @@ static void to_c(const fz::BatchLaunch& l, fzb_launch* o) {
     o->vb_wg_per_cu = l.vb_wg_per_cu;
     o->lz_wg_per_cu = l.lz_wg_per_cu;
     o->lz_tiles_per_unit = l.lz_tiles_per_unit;
-    o->rze_wg_per_cu = l.rze_wg_per_cu;
+    o->rze_waves_per_wg = l.rze_waves_per_wg;
     o->pack_wg_per_cu = l.pack_wg_per_cu;
     o->scatter_wg_per_cu = l.scatter_wg_per_cu;
 }

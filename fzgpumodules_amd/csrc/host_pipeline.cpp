@@ -658,6 +658,17 @@ void Pipeline::runInverse(const std::vector<std::pair<const void*, size_t>>& slo
     *result_size = rsize;
 }
 
+void Pipeline::prepareDecompress(size_t uncompressed_bytes) {
+    if (!finalized_) throw std::runtime_error("Pipeline must be finalized before execution");
+    if (uncompressed_bytes == 0) throw std::invalid_argument("prepareDecompress: zero input size");
+    const size_t padded = round_up(uncompressed_bytes, input_alignment_);
+    ensureBuffers(padded);
+    updateFusion();
+    input_size_ = padded;
+    orig_input_size_ = uncompressed_bytes;
+    compressed_ = true;
+}
+
 void Pipeline::decompress(const void* d_input, size_t input_size, void** d_output, size_t* output_size,
                           hipStream_t stream) {
     if (!finalized_) throw std::runtime_error("Pipeline must be finalized before execution");

@@ -324,6 +324,7 @@ class ChainSelectCompressor {
                     ch[k].stages[j]->deserializeHeader(r.stage_config, r.config_size);
                 }
                 o += (size_t)ns * sizeof(FZMStageInfo);
+                ch[k].p->prepareDecompress(4 * E);
             }
         const size_t o_choice = o, o_bytes = o_choice + ((n + 7) & ~7ull);
         if (o_bytes + 8ull * n > bytes) throw std::runtime_error("selector: truncated chunk table");

@@ -105,11 +105,14 @@ lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t
 // ===========================================================================
 // REL / NOA value base (predictor_utils.cuh:27-117), resolved on the device
 // ===========================================================================
+// Pass 1: per-workgroup (min, max) of a grid-stride slice, the first element
+// (and 0.0 when the input is zero-padded) folded in as the reference's
+// reduction starts from them.  Pass 2: one workgroup reduces the partials
+// and resolves the bound -- two launches instead of a last-workgroup atomic,
+// whose device-scope fences are costly across the XCDs' separate L2s.
 __global__ void __launch_bounds__(256)
-minmax_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ partial,
-              uint32_t* __restrict__ counter, int mode, bool zero_pad, float user_eb, float* __restrict__ params) {
+minmax_partial_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ partial, bool zero_pad) {
     __shared__ float smn[4], smx[4];
-    __shared__ bool s_last;
     const float first = in[0];
     float mn = first, mx = first;
     if (zero_pad) {  // the buffer continues with zeros up to the padded length
@@ -139,8 +142,8 @@ minmax_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ par
         fold(d);
     }
     for (; i < n4; i += stride) fold(in4[i]);
-    for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const float v = in[i];
+    for (uint64_t j = n4 * 4 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const float v = in[j];
         if (v < mn) mn = v;
         if (v > mx) mx = v;
     }
@@ -154,42 +157,39 @@ minmax_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ par
     if (lane_id() == 0) { smn[w] = mn; smx[w] = mx; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int i = 1; i < 4; i++) {
-            if (smn[i] < mn) mn = smn[i];
-            if (smx[i] > mx) mx = smx[i];
+        for (int k = 1; k < 4; k++) {
+            if (smn[k] < mn) mn = smn[k];
+            if (smx[k] > mx) mx = smx[k];
         }
         partial[blockIdx.x] = make_float2(mn, mx);
-        __threadfence();
-        s_last = atomicAdd(counter, 1u) == gridDim.x - 1;
     }
-    __syncthreads();
-    if (s_last) {
-        // The last workgroup reduces the partials in parallel.
-        __threadfence();
-        float gmn = INFINITY, gmx = -INFINITY;
-        for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-            const uint64_t raw = __hip_atomic_load(reinterpret_cast<const uint64_t*>(&partial[b]),
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const float px = __uint_as_float((uint32_t)raw), py = __uint_as_float((uint32_t)(raw >> 32));
-            if (px < gmn) gmn = px;
-            if (py > gmx) gmx = py;
-        }
+}
+
+__global__ void __launch_bounds__(256)
+minmax_final_kernel(const float2* __restrict__ partial, uint32_t np, int mode, float user_eb,
+                    float* __restrict__ params) {
+    __shared__ float smn[4], smx[4];
+    float gmn = INFINITY, gmx = -INFINITY;
+    for (uint32_t b = threadIdx.x; b < np; b += 256) {
+        const float2 v = partial[b];
+        if (v.x < gmn) gmn = v.x;
+        if (v.y > gmx) gmx = v.y;
+    }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const float a = __shfl_xor(gmn, off, 64), b = __shfl_xor(gmx, off, 64);
-            if (a < gmn) gmn = a;
-            if (b > gmx) gmx = b;
+    for (int off = 32; off > 0; off >>= 1) {
+        const float a = __shfl_xor(gmn, off, 64), b = __shfl_xor(gmx, off, 64);
+        if (a < gmn) gmn = a;
+        if (b > gmx) gmx = b;
+    }
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) { smn[w] = gmn; smx[w] = gmx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 4; k++) {
+            if (smn[k] < gmn) gmn = smn[k];
+            if (smx[k] > gmx) gmx = smx[k];
         }
-        if (lane_id() == 0) { smn[w] = gmn; smx[w] = gmx; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int i = 0; i < 4; i++) {
-                if (smn[i] < gmn) gmn = smn[i];
-                if (smx[i] > gmx) gmx = smx[i];
-            }
-            resolve_eb(gmn, gmx, mode, user_eb, params);
-            *counter = 0u;  // ready for the next call
-        }
+        resolve_eb(gmn, gmx, mode, user_eb, params);
     }
 }
 
@@ -782,13 +782,12 @@ int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode, floa
     if (mode != 1 && mode != 2) return fail(FZH_E_INVALID, "value base: mode must be 1 (REL) or 2 (NOA)");
     if (n == 0) return fail(FZH_E_INVALID, "value base: empty input");
     if (reinterpret_cast<uintptr_t>(d_in) % 16 != 0) return fail(FZH_E_INVALID, "value base: input must be 16-byte aligned");
-    // about 16 float4 per thread, at most two workgroups per CU (each
-    // workgroup ends with one atomic on the shared counter)
+    // about 16 float4 per thread, at most two workgroups per CU
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256 * 64), 512));
-    uint32_t* counter = (uint32_t*)d_ws;
     float2* partial = (float2*)((uint8_t*)d_ws + 256);
-    minmax_kernel<<<blocks, 256, 0, S(stream)>>>(d_in, n, partial, counter, mode, zero_pad, user_eb, d_params);
-    FZH_CHECK_LAUNCH("minmax_kernel");
+    minmax_partial_kernel<<<blocks, 256, 0, S(stream)>>>(d_in, n, partial, zero_pad);
+    minmax_final_kernel<<<1, 256, 0, S(stream)>>>(partial, blocks, mode, user_eb, d_params);
+    FZH_CHECK_LAUNCH("minmax");
     return FZH_OK;
 }
 

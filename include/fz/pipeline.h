@@ -100,6 +100,12 @@ class Pipeline {
     void decompress(const void* d_input, size_t input_size, void* d_output_buf, size_t output_buf_capacity,
                     size_t* actual_output_size, hipStream_t stream = nullptr);
     void reset(hipStream_t stream = nullptr);
+    // Extension (no counterpart in the reference): lets decompress() decode
+    // a result that another Pipeline with the same stages produced from
+    // `uncompressed_bytes` of input.  The stages' state must first be
+    // restored from their recorded headers (Stage::deserializeHeader), as
+    // decompressFromFile does from FZM stage records.
+    void prepareDecompress(size_t uncompressed_bytes);
 
     void enableProfiling(bool on) { profiling_ = on; }
     bool isProfilingEnabled() const { return profiling_; }
