@@ -35,7 +35,6 @@ class LaunchC(C.Structure):
         ("vb_wg_per_cu", C.c_uint32),
         ("lz_wg_per_cu", C.c_uint32),
         ("lz_tiles_per_unit", C.c_uint32),
-        ("rze_waves_per_wg", C.c_uint32),
         ("pack_wg_per_cu", C.c_uint32),
         ("scatter_wg_per_cu", C.c_uint32),
     ]

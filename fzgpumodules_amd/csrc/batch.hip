@@ -11,8 +11,8 @@
 // Launch sequence of one compress (all on one stream, captured as a graph):
 //   value_base  per-unit min/max, then one wave per field resolves eb
 //   lorenzo     codes + per-tile lists (kListSlots entries) + per-tile counts
-//   rze         fused Bitshuffle+RZE, one wave per chunk-table entry, into
-//               a chunk scratch slot
+//   rze         fused Bitshuffle+RZE, one workgroup per chunk, into a chunk
+//               scratch slot
 //   scan        one exclusive scan (mod 2^32) over [tile outliers | tile
 //               patches | chunk bytes]; per-field offsets are differences
 //   finalize    one thread per field: result layout, concat header, pads
@@ -162,19 +162,15 @@ __global__ void __launch_bounds__(256) lz_batch_kernel(const Plan* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Fused Bitshuffle + RZE over all chunks of all fields: one wave per entry of
-// the chunk table (the encoder keeps a whole 16 KiB chunk in registers, about
-// 236 VGPRs, so a persistent loop around it only adds spills); WV waves per
-// workgroup is the tuned parameter.
+// Fused Bitshuffle + RZE over all chunks of all fields: one 256-thread
+// workgroup per entry of the chunk table.  (Wrapped in a persistent loop the
+// encoder's register allocation triples -- 74 -> 213 VGPRs -- and occupancy
+// drops to 2 waves/SIMD, so this launch covers the table directly.)
 // ---------------------------------------------------------------------------
-template <int WV>
-__global__ void __launch_bounds__(64 * WV) rze_batch_kernel(const Plan* __restrict__ pp) {
+__global__ void __launch_bounds__(256) rze_batch_kernel(const Plan* __restrict__ pp) {
     const Plan& p = *pp;
-    extern __shared__ __align__(16) uint8_t smem[];
-    const int wv = threadIdx.x >> 6;
-    uint8_t* st = smem + wv * LDS_PER_WAVE;
-    uint8_t* bm1 = st + LDS_STREAM;
-    const uint32_t gc = blockIdx.x * WV + wv;
+    __shared__ RzeWgLds L;
+    const uint32_t gc = xcd_swizzle(blockIdx.x, gridDim.x);
     if (gc >= p.NC) return;
     const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
     const uint32_t f = find_field(p.chunk0, p.F, gc);
@@ -185,8 +181,8 @@ __global__ void __launch_bounds__(64 * WV) rze_batch_kernel(const Plan* __restri
     const int in_size = (int)min((uint64_t)kChunk, cb - off);
     const uint8_t* src = reinterpret_cast<const uint8_t*>(p.codes + fd.codes_off) + off;
     const uint32_t size =
-        rze_encode_chunk<2>(src, in_size, off + kChunk <= cb, st, bm1, p.chunk_scratch + (uint64_t)gc * kChunk);
-    if (lane_id() == 0) {
+        rze_encode_chunk_wg<2>(src, in_size, off + kChunk <= cb, L, p.chunk_scratch + (uint64_t)gc * kChunk);
+    if (threadIdx.x == 0) {
         p.sizes[gc] = size;
         p.counts[cbase + gc] = size & 0x7FFFFFFFu;
         uint32_t* rz = reinterpret_cast<uint32_t*>(p.out + fd.out_off + p.hdr);
@@ -395,14 +391,8 @@ int launch_lorenzo(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid, 
     return FZH_OK;
 }
 
-int launch_rze(hipStream_t s, const Plan& p, const Plan* dp, uint32_t waves_per_wg) {
-    const size_t lds = (size_t)LDS_PER_WAVE;
-    if (waves_per_wg >= 4)
-        rze_batch_kernel<4><<<(unsigned)cdiv(p.NC, 4), 256, 4 * lds, s>>>(dp);
-    else if (waves_per_wg >= 2)
-        rze_batch_kernel<2><<<(unsigned)cdiv(p.NC, 2), 128, 2 * lds, s>>>(dp);
-    else
-        rze_batch_kernel<1><<<(unsigned)p.NC, 64, lds, s>>>(dp);
+int launch_rze(hipStream_t s, const Plan& p, const Plan* dp) {
+    rze_batch_kernel<<<p.NC, 256, 0, s>>>(dp);
     FZH_CHECK_LAUNCH("batch rze");
     return FZH_OK;
 }

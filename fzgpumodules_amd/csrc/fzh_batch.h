@@ -6,10 +6,10 @@
 //   tiles   Lorenzo tiles of every field (1024 / 32x32 / 8x8x8 elements)
 //   chunks  16 KiB chunks of every field's codes (Bitshuffle block = RZE chunk)
 //   units   VB_UNIT-element slices of every field (value-base reduction)
-// Each kernel walks one space (a fixed persistent grid, except the RZE
-// encoder, which takes one wave per chunk), mapping a global index to its
-// field by binary search, and calls the same per-tile / per-chunk device
-// functions as the single-field kernels.
+// Each kernel walks one space -- with a fixed persistent grid, except the
+// RZE encoder, which takes one workgroup per chunk -- mapping a global index
+// to its field by binary search, and calls the same per-tile / per-chunk
+// device functions as the single-field kernels.
 #pragma once
 
 #include "fzh_internal.h"
@@ -96,7 +96,7 @@ size_t scan_ws_bytes_for(size_t n_counts);
 // scalar registers across the persistent loops).
 int launch_value_base(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid);
 int launch_lorenzo(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid, uint32_t tiles_per_unit);
-int launch_rze(hipStream_t s, const Plan& p, const Plan* dp, uint32_t waves_per_wg);
+int launch_rze(hipStream_t s, const Plan& p, const Plan* dp);
 int launch_scan(hipStream_t s, const Plan& p);
 int launch_finalize(hipStream_t s, const Plan& p, const Plan* dp);
 int launch_pack(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid);

@@ -1,9 +1,10 @@
-// fzh_rze_chunk.h — encoding of one 16 KiB RZE chunk on one wave64, with
-// the optional fused Bitshuffle of its block (gfx950).
+// fzh_rze_chunk.h — encoding of one 16 KiB RZE chunk by one 256-thread
+// workgroup, with the optional fused Bitshuffle of its block, plus the
+// decode-side un-shuffle and the unaligned copy helpers (gfx950).
 //
-// Shared by the single-field kernels (rze.hip) and the batched persistent
-// kernels (batch.hip), so a chunk's bytes are the same whichever launch
-// produced them.  Chunk format (modules/coders/rze/rze_stage.cu:324-445):
+// Shared by the single-field kernels (rze.hip), the batched kernels
+// (batch.hip) and the labeller (label.hip), so a chunk's bytes are the same
+// whichever launch produced them.  Chunk format (modules/coders/rze/rze_stage.cu:324-445):
 // [nonzero bytes][bm1 non-repeated][bm2 non-repeated][bm3 non-repeated]
 // [bm4 raw 4 B][in_size u16 LE]; an all-zero chunk is the 2-byte tag; size
 // bit 31 = chunk stored raw.  The raw-fallback thresholds are the
@@ -24,51 +25,11 @@ constexpr int LDS_BM1 = CS / 8;
 constexpr int LDS_PER_WAVE = LDS_STREAM + LDS_BM1;
 
 // Fused Bitshuffle (16 KiB blocks, element width SEW = 1 or 2, ballot
-// layout): a full chunk's bit-plane image in the register layout the RZE
-// kernels use (lane l, step j: bytes [1024 j + 16 l, +16) of the shuffled
-// block).  Shuffled word index = plane * npp + group with npp = 8192/SEW/32;
-// lane l step j holds words 256 j + 4 l + i, i = 0..3, so each lane
-// transposes its own groups: SEW 2 -> groups 4l+i (plane j);
-// SEW 1 -> groups 256 (j&1) + 4l + i (plane j/2).
-template <int SEW>
-__device__ __forceinline__ void shuffled_load(const uint8_t* blk, uint4 (&v)[16]) {
-    const int l = lane_id();
-    uint32_t vv[16][4];
-    if constexpr (SEW == 2) {
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const uint8_t* g = blk + 64 * (4 * l + i);
-            uint32_t w[16];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint4 t = ld16(g + 16 * q);
-                w[4 * q] = t.x; w[4 * q + 1] = t.y; w[4 * q + 2] = t.z; w[4 * q + 3] = t.w;
-            }
-            uint32_t A[16];
-            planes16(w, A);
-#pragma unroll
-            for (int j = 0; j < 16; j++) vv[j][i] = A[15 - j];
-        }
-    } else {
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint8_t* g = blk + 32 * (256 * h + 4 * l + i);
-                const uint4 t0 = ld16(g), t1 = ld16(g + 16);
-                const uint32_t w[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
-                uint32_t A[8];
-                planes8(w, A);
-#pragma unroll
-                for (int P = 0; P < 8; P++) vv[2 * P + h][i] = A[7 - P];
-            }
-    }
-#pragma unroll
-    for (int j = 0; j < 16; j++) v[j] = make_uint4(vv[j][0], vv[j][1], vv[j][2], vv[j][3]);
-}
-
-// Inverse of shuffled_load: vv[j][i] = shuffled words in the register layout;
-// writes the chunk's unshuffled 16 KiB to blk.
+// layout) on the decode side: vv[j][i] = the shuffled words 256 j + 4 l + i
+// of the block (lane l); writes the block's unshuffled 16 KiB to blk.
+// Shuffled word index = plane * npp + group with npp = 8192/SEW/32, so lane
+// l transposes groups 4l+i (SEW 2, plane j) or 256 (j&1) + 4l + i (SEW 1,
+// plane j/2) back.
 template <int SEW>
 __device__ __forceinline__ void unshuffled_store(const uint32_t (&vv)[16][4], uint8_t* blk) {
     const int l = lane_id();
@@ -99,56 +60,100 @@ __device__ __forceinline__ void unshuffled_store(const uint32_t (&vv)[16][4], ui
     }
 }
 
-__device__ __forceinline__ uint32_t byte_of(const uint4& v, int k) {
-    const uint32_t w = k < 4 ? v.x : k < 8 ? v.y : k < 12 ? v.z : v.w;
-    return (w >> (8 * (k & 3))) & 0xFFu;
+
+// ---------------------------------------------------------------------------
+// The same encoder for a 256-thread workgroup (four waves share one chunk):
+// thread t, step p holds the chunk's dword at byte 1024 p + 4 t, so every
+// load and store of a step covers 1 KiB contiguously and a thread keeps 16
+// dwords instead of 64.  For bit-shuffled blocks thread t transposes the
+// element group(s) whose plane words land at exactly those positions.
+// Level 2 gives thread t bm1 bytes 8t..8t+7 (so bm2 byte t is its 8-bit
+// repeat mask), level 3 one bm2 byte per thread (bm3 bytes from per-wave
+// ballots), level 4 runs in wave 0.  Identical bytes to rze_encode_chunk.
+// ---------------------------------------------------------------------------
+struct RzeWgLds {
+    uint8_t st[LDS_STREAM];   // assembled stream
+    uint8_t bm1[LDS_BM1];     // level-1 bitmap
+    uint32_t wsum[4][16];     // per-wave partial sums (scratch for block scans)
+    uint32_t tot[16];         // broadcast totals
+    uint8_t bm2[256];
+    uint64_t bal[4];          // level-3 ballots (bm3 bytes 8w..8w+7)
+};
+
+// Block-wide exclusive scan of one u32 per thread (256 threads); returns
+// the thread's exclusive prefix, *total gets the sum.  Uses L.wsum[..][slot].
+__device__ __forceinline__ uint32_t wg_excl_scan(RzeWgLds& L, uint32_t v, int slot, uint32_t& total) {
+    const int w = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane_id() == 63) L.wsum[w][slot] = incl;
+    __syncthreads();
+    uint32_t base = 0;
+    for (int j = 0; j < w; j++) base += L.wsum[j][slot];
+    total = L.wsum[0][slot] + L.wsum[1][slot] + L.wsum[2][slot] + L.wsum[3][slot];
+    return base + incl - v;
 }
 
-// Encodes the chunk at `src` (in_size <= CS bytes; a full block that is
-// bit-shuffled first when SEW != 0 and `shuffled`).  Returns the size word
-// (bit 31 = raw).  The chunk's stored bytes are written to `slot` (CS bytes,
-// dword aligned) when they cannot be taken from the input as they are: a
-// compressed chunk, or a raw bit-shuffled block (slot == nullptr: size
-// only).  `st` and `bm1` are this wave's LDS (LDS_STREAM and LDS_BM1
-// bytes).  Whole wave calls.
 template <int SEW>
-__device__ __forceinline__ uint32_t rze_encode_chunk(const uint8_t* __restrict__ src, int in_size, bool shuffled,
-                                                     uint8_t* st, uint8_t* bm1, uint8_t* __restrict__ slot) {
+__device__ __forceinline__ uint32_t rze_encode_chunk_wg(const uint8_t* __restrict__ src, int in_size, bool shuffled,
+                                                        RzeWgLds& L, uint8_t* __restrict__ slot) {
+    const int t = threadIdx.x;
+    const int w = t >> 6;
     const int l = lane_id();
     const bool full = in_size == CS;
-
-    // -- load the chunk: lane l, step j holds bytes [1024 j + 16 l, +16)
-    uint4 v[16];
-    if (SEW != 0 && shuffled) {
-        shuffled_load<SEW == 0 ? 2 : SEW>(src, v);
+    // -- load: dword d[p] = chunk bytes [1024 p + 4 t, +4)
+    uint32_t d[16];
+    if (SEW == 2 && shuffled) {
+        const uint8_t* g = src + 64 * t;  // element group t (32 x u16)
+        uint32_t wd[16];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 x = ld16(g + 16 * q);
+            wd[4 * q] = x.x; wd[4 * q + 1] = x.y; wd[4 * q + 2] = x.z; wd[4 * q + 3] = x.w;
+        }
+        uint32_t A[16];
+        planes16(wd, A);
+#pragma unroll
+        for (int p = 0; p < 16; p++) d[p] = A[15 - p];
+    } else if (SEW == 1 && shuffled) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint8_t* g = src + 32 * (256 * h + t);  // element group 256 h + t (32 x u8)
+            const uint4 x0 = ld16(g), x1 = ld16(g + 16);
+            const uint32_t wd[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            uint32_t A[8];
+            planes8(wd, A);
+#pragma unroll
+            for (int P = 0; P < 8; P++) d[2 * P + h] = A[7 - P];
+        }
     } else {
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const int p = 1024 * j + 16 * l;
-            if (full || p + 16 <= in_size) {
-                v[j] = *reinterpret_cast<const uint4*>(src + p);
+        for (int p = 0; p < 16; p++) {
+            const int b = 1024 * p + 4 * t;
+            if (full || b + 4 <= in_size) {
+                d[p] = *reinterpret_cast<const uint32_t*>(src + b);
             } else {
-                uint32_t wds[4] = {0, 0, 0, 0};
-                for (int k = 0; k < 16; k++)
-                    if (p + k < in_size) wds[k >> 2] |= (uint32_t)src[p + k] << (8 * (k & 3));
-                v[j] = make_uint4(wds[0], wds[1], wds[2], wds[3]);
+                uint32_t x = 0;
+                for (int k = 0; k < 4; k++)
+                    if (b + k < in_size) x |= (uint32_t)src[b + k] << (8 * k);
+                d[p] = x;
             }
         }
     }
-    // -- level 1: nonzero masks (16 bits per lane per step) -> bm1 in LDS
-    uint32_t m[16];
+    // -- level 1: nibble of nonzero bytes per step; bm1 byte 128 p + t/2
+    uint64_t nib = 0;  // 4 bits per step
     uint32_t my_nz = 0;
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        uint32_t mm = 0;
+    for (int p = 0; p < 16; p++) {
+        uint32_t m = 0;
 #pragma unroll
-        for (int k = 0; k < 16; k++) mm |= (byte_of(v[j], k) != 0u ? 1u : 0u) << k;
-        m[j] = mm;
-        my_nz += __popc(mm);
-        *reinterpret_cast<uint16_t*>(bm1 + 128 * j + 2 * l) = (uint16_t)mm;
+        for (int k = 0; k < 4; k++) m |= (((d[p] >> (8 * k)) & 0xFFu) != 0u ? 1u : 0u) << k;
+        nib |= (uint64_t)m << (4 * p);
+        my_nz += __popc(m);
+        const uint32_t other = __shfl_xor(m, 1, 64);
+        if ((t & 1) == 0) L.bm1[128 * p + (t >> 1)] = (uint8_t)(m | (other << 4));
     }
-    const uint32_t data_nz = wave_sum(my_nz);
-    wave_lds_sync();
+    uint32_t data_nz;
+    (void)wg_excl_scan(L, my_nz, 0, data_nz);  // barrier: bm1 complete
 
     const int avail = CS - 2;
     const int bm1_bytes = (in_size + 7) / 8;
@@ -157,60 +162,53 @@ __device__ __forceinline__ uint32_t rze_encode_chunk(const uint8_t* __restrict__
     uint32_t size = 0;
     bool raw = false;
     uint32_t bm2_nr = 0, bm3_nr = 0, bm4_nr = 0;
-    uint32_t rmask1 = 0;   // bm1 bytes 32l..32l+31 non-repeated
-    uint32_t rmask2 = 0;   // bm2 bytes 4l..4l+3 non-repeated (4 bits)
-    uint32_t bm3_byte = 0; // lanes 0..31: bm3 byte l
-    uint32_t rmask3 = 0;   // ballot over bm3 bytes (= bm4 as u32)
-    uint32_t bm2w = 0;     // bm2 bytes 4l..4l+3 as a word
+    uint32_t rmask1 = 0;  // bm1 bytes 8t..8t+7 non-repeated (= bm2 byte t)
+    uint32_t nr2 = 0;     // bm2 byte t non-repeated
+    uint32_t rmask3 = 0;  // bm4 (wave 0 computes it; broadcast through LDS)
     if (data_nz == 0) {
         size = 2;
     } else {
         int wpos = (int)data_nz;
         if (wpos >= avail || wpos > avail - bm1_bytes) raw = true;
         if (!raw) {
-            // -- level 2: RE over bm1 (N = bm1_bytes); lane l owns bytes 32l..32l+31
-            const uint4 a = *reinterpret_cast<const uint4*>(bm1 + 32 * l);
-            const uint4 b = *reinterpret_cast<const uint4*>(bm1 + 32 * l + 16);
-            const uint32_t last_prev = __shfl_up(b.w >> 24, 1, 64);
-            uint32_t prev = (l == 0) ? 0u : last_prev;
-            const uint32_t wds[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            // -- level 2: RE over bm1; thread t owns bytes 8t..8t+7
+            const uint32_t lo = *reinterpret_cast<const uint32_t*>(L.bm1 + 8 * t);
+            const uint32_t hi = *reinterpret_cast<const uint32_t*>(L.bm1 + 8 * t + 4);
+            uint32_t prev = t > 0 ? L.bm1[8 * t - 1] : 0u;
 #pragma unroll
-            for (int k = 0; k < 32; k++) {
-                const uint32_t cur = (wds[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-                if (cur != prev && 32 * l + k < bm1_bytes) rmask1 |= 1u << k;
+            for (int k = 0; k < 8; k++) {
+                const uint32_t cur = ((k < 4 ? lo : hi) >> (8 * (k & 3))) & 0xFFu;
+                if (cur != prev && 8 * t + k < bm1_bytes) rmask1 |= 1u << k;
                 prev = cur;
             }
-            bm2w = rmask1;  // bm2 bytes 4l..4l+3 (bit k <-> bm1 byte 32l+k)
-            bm2_nr = wave_sum(__popc(rmask1));
+            L.bm2[t] = (uint8_t)rmask1;
+            (void)wg_excl_scan(L, __popc(rmask1), 1, bm2_nr);  // barrier: bm2 complete
             wpos += (int)bm2_nr;
             if (wpos >= avail) raw = true;
             if (!raw && (wpos >= avail || wpos > avail - bm2_bytes)) raw = true;
         }
         if (!raw) {
-            // -- level 3: RE over bm2 (N = bm2_bytes); lane l owns bytes 4l..4l+3
-            uint32_t prev = __shfl_up(bm2w >> 24, 1, 64);
-            if (l == 0) prev = 0u;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t cur = (bm2w >> (8 * k)) & 0xFFu;
-                if (cur != prev && 4 * l + k < bm2_bytes) rmask2 |= 1u << k;
-                prev = cur;
-            }
-            bm3_nr = wave_sum(__popc(rmask2));
-            // bm3 byte m = nibble(2m) | nibble(2m+1) << 4
-            const uint32_t nlo = __shfl(rmask2, (2 * l) & 63, 64);
-            const uint32_t nhi = __shfl(rmask2, (2 * l + 1) & 63, 64);
-            bm3_byte = (l < 32) ? (nlo | (nhi << 4)) : 0u;
+            // -- level 3: RE over bm2; thread t owns bm2 byte t
+            const uint32_t prev = t > 0 ? L.bm2[t - 1] : 0u;
+            nr2 = (t < bm2_bytes && rmask1 != prev) ? 1u : 0u;
+            const uint64_t b = __ballot(nr2 != 0u);
+            if (l == 0) L.bal[w] = b;
+            (void)wg_excl_scan(L, nr2, 2, bm3_nr);  // barrier: ballots visible
             wpos += (int)bm3_nr;
             if (wpos >= avail) raw = true;
             if (!raw && (wpos >= avail - 4 || wpos > (avail - 4) - bm3_bytes)) raw = true;
         }
         if (!raw) {
-            // -- level 4: RE over bm3 (N = bm3_bytes); lanes 0..31 own bm3 bytes
-            uint32_t prev = __shfl_up(bm3_byte, 1, 64);
-            if (l == 0) prev = 0u;
-            const bool nr = (l < bm3_bytes) && bm3_byte != prev;
-            rmask3 = (uint32_t)__ballot(nr);
+            // -- level 4: RE over bm3 (32 bytes: ballot bytes), in wave 0
+            if (w == 0) {
+                const uint32_t cur = l < 32 ? (uint32_t)((L.bal[l >> 3] >> (8 * (l & 7))) & 0xFFu) : 0u;
+                uint32_t prev = __shfl_up(cur, 1, 64);
+                if (l == 0) prev = 0u;
+                const uint32_t m3 = (uint32_t)__ballot(l < bm3_bytes && cur != prev);
+                if (l == 0) L.tot[0] = m3;
+            }
+            __syncthreads();
+            rmask3 = L.tot[0];
             bm4_nr = __popc(rmask3);
             wpos += (int)bm4_nr;
             if (wpos >= avail - 4) raw = true;
@@ -218,66 +216,75 @@ __device__ __forceinline__ uint32_t rze_encode_chunk(const uint8_t* __restrict__
         size = raw ? ((1u << 31) | (uint32_t)in_size) : (uint32_t)(wpos + 4 + 2);
     }
     const uint32_t clean = size & 0x7FFFFFFFu;
-
-    // -- assemble the chunk stream in LDS
-    if (size == 2) {
-        if (l == 0) { st[0] = (uint8_t)(in_size & 0xFF); st[1] = (uint8_t)((in_size >> 8) & 0xFF); }
-    } else if (raw) {
+    const bool from_scratch = !raw || (SEW != 0 && shuffled);
+    if (raw) {
+        // stored bytes = the (shuffled) chunk image, straight from registers
+        if (slot && from_scratch) {
 #pragma unroll
-        for (int j = 0; j < 16; j++) *reinterpret_cast<uint4*>(st + 1024 * j + 16 * l) = v[j];
+            for (int p = 0; p < 16; p++) reinterpret_cast<uint32_t*>(slot)[256 * p + t] = d[p];
+        }
+        __syncthreads();
+        return size;
+    }
+    if (size == 2) {
+        if (t == 0) { L.st[0] = (uint8_t)(in_size & 0xFF); L.st[1] = (uint8_t)((in_size >> 8) & 0xFF); }
     } else {
-        // nonzero bytes in order (j, lane, k)
+        // nonzero bytes in chunk order: step-major, then thread, then byte
+        uint32_t incl[16];
+#pragma unroll
+        for (int p = 0; p < 16; p++) {
+            incl[p] = wave_incl_scan((uint32_t)__popcll((nib >> (4 * p)) & 0xFull));
+            if (l == 63) L.wsum[w][p] = incl[p];
+        }
+        __syncthreads();
         uint32_t run = 0;
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const uint32_t cnt = __popc(m[j]);
-            const uint32_t incl = wave_incl_scan(cnt);
-            uint32_t pos = run + incl - cnt;
+        for (int p = 0; p < 16; p++) {
+            uint32_t before = 0, all = 0;
 #pragma unroll
-            for (int k = 0; k < 16; k++)
-                if (m[j] & (1u << k)) st[pos++] = (uint8_t)byte_of(v[j], k);
-            run += __shfl(incl, 63, 64);
-        }
-        // bm1 non-repeated bytes
-        {
-            const uint32_t cnt = __popc(rmask1);
-            const uint32_t incl = wave_incl_scan(cnt);
-            uint32_t pos = data_nz + incl - cnt;
-            for (int k = 0; k < 32; k++)
-                if (rmask1 & (1u << k)) st[pos++] = bm1[32 * l + k];
-        }
-        // bm2 non-repeated bytes
-        {
-            const uint32_t cnt = __popc(rmask2);
-            const uint32_t incl = wave_incl_scan(cnt);
-            uint32_t pos = data_nz + bm2_nr + incl - cnt;
+            for (int j = 0; j < 4; j++) {
+                const uint32_t v = L.wsum[j][p];
+                before += j < w ? v : 0u;
+                all += v;
+            }
+            const uint32_t m = (uint32_t)((nib >> (4 * p)) & 0xFull);
+            uint32_t pos = run + before + incl[p] - __popc(m);
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (rmask2 & (1u << k)) st[pos++] = (uint8_t)((bm2w >> (8 * k)) & 0xFFu);
+                if (m & (1u << k)) L.st[pos++] = (uint8_t)((d[p] >> (8 * k)) & 0xFFu);
+            run += all;
         }
+        __syncthreads();  // wsum reused below
+        // bm1 non-repeated bytes (thread t: bm1 bytes 8t..8t+7)
+        uint32_t dummy;
+        uint32_t pos1 = data_nz + wg_excl_scan(L, __popc(rmask1), 3, dummy);
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (rmask1 & (1u << k)) L.st[pos1++] = L.bm1[8 * t + k];
+        // bm2 non-repeated bytes (one per thread at most)
+        const uint32_t pos2 = data_nz + bm2_nr + wg_excl_scan(L, nr2, 4, dummy);
+        if (nr2) L.st[pos2] = (uint8_t)rmask1;
         // bm3 non-repeated bytes, bm4, tag
         const uint32_t base3 = data_nz + bm2_nr + bm3_nr;
-        if (rmask3 & (1u << (l & 31)) && l < 32) st[base3 + __popc(rmask3 & ((1u << l) - 1u))] = (uint8_t)bm3_byte;
+        if (w == 0 && l < 32 && (rmask3 & (1u << l))) {
+            const uint32_t b3 = (uint32_t)((L.bal[l >> 3] >> (8 * (l & 7))) & 0xFFu);
+            L.st[base3 + __popc(rmask3 & ((1u << l) - 1u))] = (uint8_t)b3;
+        }
         const uint32_t base4 = base3 + bm4_nr;
-        if (l < 4) st[base4 + l] = (uint8_t)((rmask3 >> (8 * l)) & 0xFFu);
-        if (l == 0) {
-            st[base4 + 4] = (uint8_t)(in_size & 0xFF);
-            st[base4 + 5] = (uint8_t)((in_size >> 8) & 0xFF);
+        if (t < 4) L.st[base4 + t] = (uint8_t)((rmask3 >> (8 * t)) & 0xFFu);
+        if (t == 0) {
+            L.st[base4 + 4] = (uint8_t)(in_size & 0xFF);
+            L.st[base4 + 5] = (uint8_t)((in_size >> 8) & 0xFF);
         }
     }
-    wave_lds_sync();
-    // A raw chunk is copied from the input by the pack kernel, except a
-    // bit-shuffled block, whose stored bytes are the shuffled image.
-    const bool from_scratch = !raw || (SEW != 0 && shuffled);
-    if (slot && from_scratch && clean > 0) {
-        // chunk bytes -> this chunk's scratch slot (dword stores; the slot
-        // is CS bytes)
+    __syncthreads();
+    if (slot && clean > 0) {
         const uint32_t nd = (clean + 3) / 4;
-        const uint32_t* sw = reinterpret_cast<const uint32_t*>(st);
+        const uint32_t* sw = reinterpret_cast<const uint32_t*>(L.st);
         uint32_t* dw = reinterpret_cast<uint32_t*>(slot);
-        for (uint32_t i = l; i < nd; i += 64) dw[i] = sw[i];
+        for (uint32_t i = t; i < nd; i += 256) dw[i] = sw[i];
     }
-    wave_lds_sync();  // LDS free for the wave's next chunk
+    __syncthreads();  // LDS free for the workgroup's next chunk
     return size;
 }
 

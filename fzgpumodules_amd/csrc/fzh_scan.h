@@ -109,6 +109,43 @@ scan_apply_kernel(uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_
     }
 }
 
+// Whole scan in one 1024-thread workgroup for n <= SCAN_SMALL: thread t owns
+// elements [t*per, (t+1)*per).  Saves two launches on the short arrays
+// (per-tile counts, chunk sizes of one field) that most calls scan.
+constexpr uint32_t SCAN_SMALL = 65536;
+
+template <bool WRAP, bool CLEAR>
+__global__ void __launch_bounds__(1024)
+scan_small_kernel(uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n_max,
+                  const uint32_t* __restrict__ d_n, uint32_t* __restrict__ total) {
+    __shared__ uint64_t s_w[16];
+    const uint32_t n = (uint32_t)eff_count(n_max, d_n);
+    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    const uint32_t per = (n_max + 1023) / 1024;
+    const uint32_t a = tid * per;
+    uint64_t sum = 0;
+    for (uint32_t k = 0; k < per; k++)
+        if (a + k < n) sum += in[a + k];
+    uint64_t incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t t = __shfl_up(incl, off, 64);
+        if (l >= off) incl += t;
+    }
+    if (l == 63) s_w[w] = incl;
+    __syncthreads();
+    uint64_t run = incl - sum;
+    for (int j = 0; j < w; j++) run += s_w[j];
+    for (uint32_t k = 0; k < per; k++) {
+        if (a + k >= n) break;
+        const uint32_t v = in[a + k];
+        if constexpr (CLEAR) in[a + k] = 0u;
+        out[a + k] = (!WRAP && run > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)run;
+        run += v;
+    }
+    if (tid == 1023 && total) *total = (!WRAP && run > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)run;
+}
+
 inline size_t scan_ws_bytes(size_t n) { return align_up(4 * std::max<size_t>(cdiv(n, SCAN_SEG), 1), 256); }
 
 // Exclusive scan of in[0..n) into out (may alias in); *d_total = sum.  With
@@ -118,6 +155,10 @@ inline size_t scan_ws_bytes(size_t n) { return align_up(4 * std::max<size_t>(cdi
 template <bool WRAP = false, bool CLEAR = false>
 inline int scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, size_t n, const uint32_t* d_n,
                     uint32_t* d_total, void* ws) {
+    if (n <= SCAN_SMALL) {
+        scan_small_kernel<WRAP, CLEAR><<<1, 1024, 0, s>>>(const_cast<uint32_t*>(in), out, (uint32_t)n, d_n, d_total);
+        return FZH_OK;
+    }
     const size_t nseg = std::max<size_t>(cdiv(n, SCAN_SEG), 1);
     if (nseg > SCAN_MAX_SEGS) return fail(FZH_E_INVALID, "scan: more than 2^28 elements");
     uint32_t* seg = (uint32_t*)ws;

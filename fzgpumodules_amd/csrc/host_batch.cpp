@@ -236,7 +236,6 @@ void BatchCompressor::setLaunch(const BatchLaunch& l) {
     launch_.vb_wg_per_cu = fix(l.vb_wg_per_cu, d.vb_wg_per_cu);
     launch_.lz_wg_per_cu = fix(l.lz_wg_per_cu, d.lz_wg_per_cu);
     launch_.lz_tiles_per_unit = fix(l.lz_tiles_per_unit, d.lz_tiles_per_unit);
-    launch_.rze_waves_per_wg = fix(l.rze_waves_per_wg, d.rze_waves_per_wg);
     launch_.pack_wg_per_cu = fix(l.pack_wg_per_cu, d.pack_wg_per_cu);
     launch_.scatter_wg_per_cu = fix(l.scatter_wg_per_cu, d.scatter_wg_per_cu);
     destroyGraph();
@@ -259,7 +258,7 @@ void BatchCompressor::enqueue(hipStream_t s) {
     if (prof) hipck(hipEventRecord(ev[1], s), "hipEventRecord");
     fzck(launch_lorenzo(s, p, dp, cu * launch_.lz_wg_per_cu, launch_.lz_tiles_per_unit), "batch lorenzo");
     if (prof) hipck(hipEventRecord(ev[2], s), "hipEventRecord");
-    fzck(launch_rze(s, p, dp, launch_.rze_waves_per_wg), "batch rze");
+    fzck(launch_rze(s, p, dp), "batch rze");
     if (prof) hipck(hipEventRecord(ev[3], s), "hipEventRecord");
     fzck(launch_scan(s, p), "batch scan");
     if (prof) hipck(hipEventRecord(ev[4], s), "hipEventRecord");
@@ -377,11 +376,6 @@ BatchLaunch BatchCompressor::autotune(hipStream_t s, int reps) {
             const float t = time([&] { fzck(launch_lorenzo(s, p, dp, cu * w, g), "tune lorenzo"); });
             if (t < bt) { bt = t; best.lz_wg_per_cu = w; best.lz_tiles_per_unit = g; }
         }
-    bt = std::numeric_limits<float>::max();
-    for (uint32_t w : {1u, 2u, 4u}) {
-        const float t = time([&] { fzck(launch_rze(s, p, dp, w), "tune rze"); });
-        if (t < bt) { bt = t; best.rze_waves_per_wg = w; }
-    }
     bt = std::numeric_limits<float>::max();
     for (uint32_t w : {1u, 2u, 4u, 8u}) {
         const float t = time([&] { fzck(launch_pack(s, p, dp, cu * w), "tune pack"); });
@@ -508,7 +502,6 @@ static fz::BatchLaunch from_c(const fzb_launch& l) {
     r.vb_wg_per_cu = l.vb_wg_per_cu;
     r.lz_wg_per_cu = l.lz_wg_per_cu;
     r.lz_tiles_per_unit = l.lz_tiles_per_unit;
-    r.rze_waves_per_wg = l.rze_waves_per_wg;
     r.pack_wg_per_cu = l.pack_wg_per_cu;
     r.scatter_wg_per_cu = l.scatter_wg_per_cu;
     return r;
@@ -518,7 +511,6 @@ static void to_c(const fz::BatchLaunch& l, fzb_launch* o) {
     o->vb_wg_per_cu = l.vb_wg_per_cu;
     o->lz_wg_per_cu = l.lz_wg_per_cu;
     o->lz_tiles_per_unit = l.lz_tiles_per_unit;
-    o->rze_waves_per_wg = l.rze_waves_per_wg;
     o->pack_wg_per_cu = l.pack_wg_per_cu;
     o->scatter_wg_per_cu = l.scatter_wg_per_cu;
 }

@@ -105,19 +105,16 @@ codes_kernel(const uint16_t* __restrict__ codes, uint64_t E, uint32_t* __restric
         if (s_h[i]) atomicAdd(&h[i], s_h[i]);
 }
 
-constexpr int kRzeWaves = 2;
-
-// Bitshuffle + RZE size of every 16 KiB block of codes.
-__global__ void __launch_bounds__(64 * kRzeWaves)
+// Bitshuffle + RZE size of every 16 KiB block of codes (one workgroup per
+// block, the encoder's own device function, size only).
+__global__ void __launch_bounds__(256)
 rze_kernel(const uint16_t* __restrict__ codes, uint64_t nblocks, uint32_t bpc, Stats st) {
-    extern __shared__ __align__(16) uint8_t smem[];
-    const int wv = threadIdx.x >> 6;
-    uint8_t* s = smem + wv * LDS_PER_WAVE;
-    const uint64_t b = (uint64_t)blockIdx.x * kRzeWaves + wv;
+    __shared__ RzeWgLds L;
+    const uint64_t b = blockIdx.x;
     if (b >= nblocks) return;
     const uint8_t* src = reinterpret_cast<const uint8_t*>(codes) + b * (uint64_t)CS;
-    const uint32_t size = rze_encode_chunk<2>(src, CS, true, s, s + LDS_STREAM, nullptr);
-    if (lane_id() == 0) atomicAdd(&st.rze[b / bpc], size & 0x7FFFFFFFu);
+    const uint32_t size = rze_encode_chunk_wg<2>(src, CS, true, L, nullptr);
+    if (threadIdx.x == 0) atomicAdd(&st.rze[b / bpc], size & 0x7FFFFFFFu);
 }
 
 __device__ __forceinline__ int f2i_rn(float f) {
@@ -230,7 +227,7 @@ int label_launch_pass1(hipStream_t s, const float* chunks, size_t n, uint32_t nx
     codes_kernel<<<(unsigned)segs, 256, 0, s>>>(codes, E, (uint32_t*)(w + L.o_hist), st);
     const uint32_t bpc = (uint32_t)(2 * E / CS);
     const uint64_t nb = (uint64_t)bpc * n;
-    rze_kernel<<<(unsigned)cdiv(nb, kRzeWaves), 64 * kRzeWaves, kRzeWaves * LDS_PER_WAVE, s>>>(codes, nb, bpc, st);
+    rze_kernel<<<(unsigned)nb, 256, 0, s>>>(codes, nb, bpc, st);
     quant_kernel<<<(unsigned)segs, 256, 0, s>>>(chunks, E, d_eb, st);
     FZH_CHECK_LAUNCH("labeller pass 1");
     return FZH_OK;

@@ -11,8 +11,8 @@
 // The raw-fallback thresholds are the reference's, verbatim (:369-431).
 //
 // MI355X design
-//   encode: one wave64 per chunk (fzh_rze_chunk.h).  The chunk is held in
-//   registers (16 x 16 B per lane), bitmaps are built with per-lane masks in
+//   encode: one 256-thread workgroup per chunk (fzh_rze_chunk.h).  The
+//   chunk is held in registers (16 dwords per thread), bitmaps are built in
 //   LDS and the compressed chunk is assembled in LDS, then written to its
 //   slot of a chunk-sized scratch (raw chunks are taken from the input by
 //   the pack kernel, except bit-shuffled blocks).  A pack kernel
@@ -37,21 +37,18 @@ namespace {
 // blocks of SEW-byte elements; the first nfull chunks are full blocks, the
 // tail is taken verbatim, as BitshuffleStage leaves it).
 template <int SEW>
-__global__ void __launch_bounds__(64 * WAVES)
+__global__ void __launch_bounds__(256)
 rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restrict__ out,
                uint8_t* __restrict__ scratch, uint32_t* __restrict__ sizes, uint32_t* __restrict__ clean,
                uint32_t nc, uint64_t nfull) {
-    extern __shared__ __align__(16) uint8_t smem[];
-    const int wv = threadIdx.x >> 6;
-    uint8_t* st = smem + wv * LDS_PER_WAVE;  // assembled chunk stream
-    uint8_t* bm1 = st + LDS_STREAM;
-    const uint32_t c = blockIdx.x * WAVES + wv;
+    __shared__ RzeWgLds L;
+    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x);
     if (c >= nc) return;
     const uint64_t in_off = (uint64_t)c * CS;
     const int in_size = (int)((nbytes - in_off) < (uint64_t)CS ? (nbytes - in_off) : CS);
     const uint32_t size =
-        rze_encode_chunk<SEW>(in + in_off, in_size, SEW != 0 && c < nfull, st, bm1, scratch + (uint64_t)c * CS);
-    if (lane_id() == 0) {
+        rze_encode_chunk_wg<SEW>(in + in_off, in_size, SEW != 0 && c < nfull, L, scratch + (uint64_t)c * CS);
+    if (threadIdx.x == 0) {
         sizes[c] = size;
         clean[c] = size & 0x7FFFFFFFu;
         reinterpret_cast<uint32_t*>(out + 8)[c] = size;
@@ -369,7 +366,6 @@ int rze_enc_impl(hipStream_t s, const void* d_in, size_t nbytes, int sew, void* 
     if (((uintptr_t)d_in | (uintptr_t)d_out) % 16 != 0) return fail(FZH_E_INVALID, "RZE: buffers must be 16-byte aligned");
     const uint32_t nc = (uint32_t)cdiv(nbytes, CS);
     const uint64_t nfull = nbytes / CS;
-    const unsigned blocks = (unsigned)cdiv(nc, WAVES);
     const EncWs L = enc_layout(nbytes);
     uint8_t* w = (uint8_t*)d_ws;
     uint32_t* sizes = (uint32_t*)(w + L.o_sizes);
@@ -379,11 +375,11 @@ int rze_enc_impl(hipStream_t s, const void* d_in, size_t nbytes, int sew, void* 
     const uint8_t* in = (const uint8_t*)d_in;
     uint8_t* out = (uint8_t*)d_out;
     if (sew == 1)
-        rze_enc_kernel<1><<<blocks, 64 * WAVES, kLds, s>>>(in, nbytes, out, scratch, sizes, clean, nc, nfull);
+        rze_enc_kernel<1><<<nc, 256, 0, s>>>(in, nbytes, out, scratch, sizes, clean, nc, nfull);
     else if (sew == 2)
-        rze_enc_kernel<2><<<blocks, 64 * WAVES, kLds, s>>>(in, nbytes, out, scratch, sizes, clean, nc, nfull);
+        rze_enc_kernel<2><<<nc, 256, 0, s>>>(in, nbytes, out, scratch, sizes, clean, nc, nfull);
     else
-        rze_enc_kernel<0><<<blocks, 64 * WAVES, kLds, s>>>(in, nbytes, out, scratch, sizes, clean, nc, 0);
+        rze_enc_kernel<0><<<nc, 256, 0, s>>>(in, nbytes, out, scratch, sizes, clean, nc, 0);
     FZH_CHECK_LAUNCH("rze_enc_kernel");
     if (int rc = scan_u32(s, clean, offs, nc, nullptr, offs + nc, w + L.o_scan)) return rc;
     rze_pack_kernel<<<(unsigned)cdiv(nc, 4), 256, 0, s>>>(in, nbytes, scratch, sizes, offs, out, d_out_size, nc,

@@ -34,14 +34,13 @@ struct BatchConfig {
     float patch_capacity = 0.01f;
 };
 
-// Launch geometry: workgroups per CU of each persistent grid, Lorenzo tiles
-// per wave step, and waves per workgroup of the RZE encoder (one wave per
-// chunk).  Changes timing only.
+// Persistent-grid geometry (workgroups per CU per kernel; Lorenzo tiles per
+// wave step; the RZE encoder takes one workgroup per chunk).  Changes timing
+// only.
 struct BatchLaunch {
     uint32_t vb_wg_per_cu = 4;
     uint32_t lz_wg_per_cu = 8;
     uint32_t lz_tiles_per_unit = 4;
-    uint32_t rze_waves_per_wg = 2;
     uint32_t pack_wg_per_cu = 4;
     uint32_t scatter_wg_per_cu = 2;
 };

@@ -51,15 +51,14 @@ typedef struct fzb_config {
     float patch_capacity;   /* strict mode: fraction, at least 256 slots */
 } fzb_config;
 
-/* Launch geometry (changes timing only, never bytes).  0 = library
- * default.  wg_per_cu: workgroups per compute unit of each persistent grid;
- * tiles_per_unit: Lorenzo tiles taken per wave step; rze_waves_per_wg:
- * waves per workgroup of the RZE encoder (one wave per chunk; 1, 2 or 4). */
+/* Launch geometry of the persistent kernels (changes timing only, never
+ * bytes).  0 = library default.  wg_per_cu: workgroups per compute unit of
+ * each persistent grid; tiles_per_unit: Lorenzo tiles taken per wave step.
+ * (The RZE encoder takes one workgroup per chunk and has no parameter.)    */
 typedef struct fzb_launch {
     uint32_t vb_wg_per_cu;
     uint32_t lz_wg_per_cu;
     uint32_t lz_tiles_per_unit;
-    uint32_t rze_waves_per_wg;
     uint32_t pack_wg_per_cu;
     uint32_t scatter_wg_per_cu;
 } fzb_launch;

@@ -127,11 +127,9 @@ def test_batch_fields_match_the_oracle_chain(mixed, mixed_reference):
 
 
 @pytest.mark.parametrize("launch", [
-    dict(vb_wg_per_cu=1, lz_wg_per_cu=1, lz_tiles_per_unit=1, rze_waves_per_wg=1, pack_wg_per_cu=1,
-         scatter_wg_per_cu=1),
-    dict(vb_wg_per_cu=8, lz_wg_per_cu=16, lz_tiles_per_unit=16, rze_waves_per_wg=4, pack_wg_per_cu=8,
-         scatter_wg_per_cu=8),
-    dict(lz_tiles_per_unit=3, rze_waves_per_wg=2),
+    dict(vb_wg_per_cu=1, lz_wg_per_cu=1, lz_tiles_per_unit=1, pack_wg_per_cu=1, scatter_wg_per_cu=1),
+    dict(vb_wg_per_cu=8, lz_wg_per_cu=16, lz_tiles_per_unit=16, pack_wg_per_cu=8, scatter_wg_per_cu=8),
+    dict(lz_tiles_per_unit=3, pack_wg_per_cu=3),
 ])
 def test_batch_bytes_do_not_depend_on_launch_geometry(mixed, mixed_reference, launch):
     bt = _Batch(mixed)
