@@ -17,59 +17,51 @@ namespace fzh {
 namespace {
 
 constexpr int CS = 16384;
-constexpr int WAVES = 2;
-// LDS per wave: stream/staging buffer (+ slack for the aligned dword reads
-// that straddle its end) and bm1.
+// Stream/staging buffer (+ slack for the aligned dword reads that straddle
+// its end) and level-1 bitmap sizes in LDS.
 constexpr int LDS_STREAM = CS + 64;
 constexpr int LDS_BM1 = CS / 8;
-constexpr int LDS_PER_WAVE = LDS_STREAM + LDS_BM1;
 
-// Fused Bitshuffle (16 KiB blocks, element width SEW = 1 or 2, ballot
-// layout) on the decode side: vv[j][i] = the shuffled words 256 j + 4 l + i
-// of the block (lane l); writes the block's unshuffled 16 KiB to blk.
-// Shuffled word index = plane * npp + group with npp = 8192/SEW/32, so lane
-// l transposes groups 4l+i (SEW 2, plane j) or 256 (j&1) + 4l + i (SEW 1,
-// plane j/2) back.
+// Fused Bitshuffle (16 KiB blocks, element width SEW = 1 or 2) on the decode
+// side, in the workgroup layout: d[p] is the shuffled block's dword at byte
+// 1024 p + 4 t (thread t).  Shuffled word index = plane * npp + group with
+// npp = 8192 / SEW / 32, so thread t holds every plane of element group t
+// (SEW 2: plane 15 - p in d[p]) or of groups t and 256 + t (SEW 1: plane
+// 7 - P of group 256 h + t in d[2P + h]) and writes them back transposed.
 template <int SEW>
-__device__ __forceinline__ void unshuffled_store(const uint32_t (&vv)[16][4], uint8_t* blk) {
-    const int l = lane_id();
+__device__ __forceinline__ void unshuffled_store(const uint32_t (&d)[16], uint8_t* blk) {
+    const int t = threadIdx.x;
     if constexpr (SEW == 2) {
+        uint32_t A[16], w[16];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            uint32_t A[16], w[16];
+        for (int b = 0; b < 16; b++) A[b] = d[15 - b];
+        unplanes16(A, w);
+        uint8_t* g = blk + 64 * t;
 #pragma unroll
-            for (int b = 0; b < 16; b++) A[b] = vv[15 - b][i];
-            unplanes16(A, w);
-            uint8_t* g = blk + 64 * (4 * l + i);
-#pragma unroll
-            for (int q = 0; q < 4; q++) st16(g + 16 * q, make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]));
-        }
+        for (int q = 0; q < 4; q++) st16(g + 16 * q, make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]));
     } else {
 #pragma unroll
-        for (int h = 0; h < 2; h++)
+        for (int h = 0; h < 2; h++) {
+            uint32_t A[8], w[8];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                uint32_t A[8], w[8];
-#pragma unroll
-                for (int P = 0; P < 8; P++) A[7 - P] = vv[2 * P + h][i];
-                unplanes8(A, w);
-                uint8_t* g = blk + 32 * (256 * h + 4 * l + i);
-                st16(g, make_uint4(w[0], w[1], w[2], w[3]));
-                st16(g + 16, make_uint4(w[4], w[5], w[6], w[7]));
-            }
+            for (int P = 0; P < 8; P++) A[7 - P] = d[2 * P + h];
+            unplanes8(A, w);
+            uint8_t* g = blk + 32 * (256 * h + t);
+            st16(g, make_uint4(w[0], w[1], w[2], w[3]));
+            st16(g + 16, make_uint4(w[4], w[5], w[6], w[7]));
+        }
     }
 }
 
-
 // ---------------------------------------------------------------------------
-// The same encoder for a 256-thread workgroup (four waves share one chunk):
-// thread t, step p holds the chunk's dword at byte 1024 p + 4 t, so every
+// The encoder: one 256-thread workgroup (four waves) per chunk; thread t,
+// step p holds the chunk's dword at byte 1024 p + 4 t, so every
 // load and store of a step covers 1 KiB contiguously and a thread keeps 16
 // dwords instead of 64.  For bit-shuffled blocks thread t transposes the
 // element group(s) whose plane words land at exactly those positions.
 // Level 2 gives thread t bm1 bytes 8t..8t+7 (so bm2 byte t is its 8-bit
 // repeat mask), level 3 one bm2 byte per thread (bm3 bytes from per-wave
-// ballots), level 4 runs in wave 0.  Identical bytes to rze_encode_chunk.
+// ballots), level 4 runs in wave 0.
 // ---------------------------------------------------------------------------
 struct RzeWgLds {
     uint8_t st[LDS_STREAM];   // assembled stream

@@ -21,9 +21,9 @@
 //   waiting on another (the reference stages per-chunk scratch too, then
 //   scans on the host stream with CUB and copies).
 //   decode: a one-workgroup kernel scans the header's sizes into chunk
-//   offsets; then one wave64 per chunk decodes, with the repeat-elimination
-//   decode done as a parallel prefix-popcount gather instead of a
-//   single-thread loop.
+//   offsets; then one 256-thread workgroup per chunk decodes, with the
+//   repeat-elimination decode done as a parallel prefix-popcount gather
+//   instead of a single-thread loop.
 #include "fzh_rze_chunk.h"
 #include "fzh_scan.h"
 
@@ -121,201 +121,224 @@ rze_offsets_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint32_t* 
     }
 }
 
+// Decoder, one 256-thread workgroup per chunk.  Thread t, step p produces
+// the chunk's dword at byte 1024 p + 4 t (the encoder's layout): bm3 in wave
+// 0, bm2 byte t and bm1 bytes 8t..8t+7 in thread t (repeat-elimination
+// decode by prefix popcounts: byte i = the last stored byte at or before
+// i), then the zero-eliminated data by one scan in chunk order.  Bit-shuffled
+// blocks are un-shuffled in registers (thread t owns element group t, or
+// groups t and 256 + t for 1-byte elements).
+struct RzeDecLds {
+    uint32_t stw[(CS + 64) / 4 + 4];  // the chunk's stored bytes, dword aligned
+    uint8_t bm1[LDS_BM1];
+    uint8_t bm2[256];
+    uint8_t bm3[32];
+    uint32_t hsum[4][3];   // per-wave sums of the bitmap stages
+    uint32_t wsum[4][16];  // per-wave sums of the 16 data steps
+};
+
+__device__ __forceinline__ uint32_t dec_block_sum(RzeDecLds& L, uint32_t v, int slot) {
+    const uint32_t ws = wave_sum(v);
+    if (lane_id() == 0) L.hsum[threadIdx.x >> 6][slot] = ws;
+    __syncthreads();
+    return L.hsum[0][slot] + L.hsum[1][slot] + L.hsum[2][slot] + L.hsum[3][slot];
+}
+
+// stream dword starting at chunk byte q (any alignment) from the staged words
+__device__ __forceinline__ uint32_t staged_dword(const uint32_t* stw, uint32_t q) {
+    const uint32_t lo = stw[q >> 2], hi = stw[(q >> 2) + 1];
+    return (q & 3) ? __builtin_amdgcn_alignbyte(hi, lo, q & 3) : lo;
+}
+
 template <int SEW>
-__global__ void __launch_bounds__(64 * WAVES)
+__global__ void __launch_bounds__(256)
 rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __restrict__ out,
                uint64_t orig_bytes, int* __restrict__ d_status, const uint32_t* __restrict__ offs, uint32_t nc,
                uint64_t nfull) {
     constexpr int SE = SEW == 0 ? 2 : SEW;
-    extern __shared__ __align__(16) uint8_t smem[];
-    const int l = lane_id();
-    const int wv = threadIdx.x >> 6;
-    uint8_t* st = smem + wv * LDS_PER_WAVE;
-    uint8_t* bm1 = st + LDS_STREAM;
-
-    const uint32_t c = blockIdx.x * WAVES + wv;
-    if (c < nc) {
-        const uint32_t hdr = 8u + 4u * nc;
-        bool bad = in_bytes < hdr;
-        uint32_t entry = 0;
-        if (!bad) {
-            const uint32_t h0 = reinterpret_cast<const uint32_t*>(in)[0];
-            const uint32_t h1 = reinterpret_cast<const uint32_t*>(in)[1];
-            bad = (uint64_t)h0 != orig_bytes || h1 != nc;
-            entry = reinterpret_cast<const uint32_t*>(in + 8)[c];
+    __shared__ RzeDecLds L;
+    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x);
+    if (c >= nc) return;
+    const int t = threadIdx.x, w = t >> 6, l = lane_id();
+    const uint32_t hdr = 8u + 4u * nc;
+    bool bad = in_bytes < hdr;
+    uint32_t entry = 0;
+    if (!bad) {
+        const uint32_t h0 = reinterpret_cast<const uint32_t*>(in)[0];
+        const uint32_t h1 = reinterpret_cast<const uint32_t*>(in)[1];
+        bad = (uint64_t)h0 != orig_bytes || h1 != nc;
+        entry = reinterpret_cast<const uint32_t*>(in + 8)[c];
+    }
+    const uint32_t stored = bad ? 0u : (entry & 0x7FFFFFFFu);
+    const uint64_t src = (uint64_t)hdr + offs[c];
+    const uint32_t o = (c + 1 < nc) ? (uint32_t)CS : (uint32_t)(orig_bytes - (uint64_t)c * CS);
+    const bool israw = (entry & 0x80000000u) != 0u;
+    if (!bad) bad = src + stored > in_bytes || stored > (uint32_t)CS || (israw && stored < o) ||
+                    (!israw && stored != 2 && stored < 7);
+    uint8_t* dst = out + (uint64_t)c * CS;
+    if (bad) {  // uniform over the workgroup
+        if (t == 0) *d_status = 1;
+        return;
+    }
+    if (!israw && stored == 2) {
+        for (uint32_t i = 16 * t; i < o; i += 4096) {
+            if (i + 16 <= o) *reinterpret_cast<uint4*>(dst + i) = make_uint4(0, 0, 0, 0);
+            else for (uint32_t k = i; k < o; k++) dst[k] = 0;
         }
-        const uint32_t stored = bad ? 0u : (entry & 0x7FFFFFFFu);
-        const uint32_t off = offs[c];
-        const uint64_t src = (uint64_t)hdr + off;
-        const uint32_t o = (c + 1 < nc) ? (uint32_t)CS : (uint32_t)(orig_bytes - (uint64_t)c * CS);
-        const bool israw = (entry & 0x80000000u) != 0u;
-        if (!bad) bad = src + stored > in_bytes || stored > (uint32_t)CS || (israw && stored < o) ||
-                        (!israw && stored != 2 && stored < 7);
-        uint8_t* dst = out + (uint64_t)c * CS;
-        if (bad) {
-            if (l == 0) *d_status = 1;
-        } else if (!israw && stored == 2) {
-            for (uint32_t i = 16 * l; i < o; i += 1024) {
-                if (i + 16 <= o) *reinterpret_cast<uint4*>(dst + i) = make_uint4(0, 0, 0, 0);
-                else for (uint32_t k = i; k < o; k++) dst[k] = 0;
-            }
+        return;
+    }
+    // -- stage the chunk's bytes into LDS (dword aligned start)
+    const uint64_t a0 = src & ~3ull;
+    const uint32_t shift = (uint32_t)(src - a0);
+    const uint32_t nwd = (uint32_t)((((src + stored + 3) & ~3ull) - a0) / 4);
+    for (uint32_t i = t; i < nwd; i += 256) {
+        const uint64_t ga = a0 + 4ull * i;
+        uint32_t v;
+        if (ga + 4 <= in_bytes) {
+            v = *reinterpret_cast<const uint32_t*>(in + ga);
         } else {
-            // -- stage the chunk's bytes into LDS (dword aligned start)
-            const uint64_t a0 = src & ~3ull;
-            const uint32_t shift = (uint32_t)(src - a0);
-            const uint32_t span = (uint32_t)(((src + stored + 3) & ~3ull) - a0);  // bytes to fetch
-            const uint32_t nwd = span / 4;
-            uint32_t* stw = reinterpret_cast<uint32_t*>(st);
-            for (uint32_t i = l; i < nwd; i += 64) {
-                const uint64_t ga = a0 + 4ull * i;
-                uint32_t w;
-                if (ga + 4 <= in_bytes) {
-                    w = *reinterpret_cast<const uint32_t*>(in + ga);
+            v = 0;
+            for (uint32_t k = 0; k < 4; k++)
+                if (ga + k < in_bytes) v |= (uint32_t)in[ga + k] << (8 * k);
+        }
+        L.stw[i] = v;
+    }
+    if (t < 4) L.stw[nwd + t] = 0u;  // the dword after the last one read by staged_dword
+    __syncthreads();
+    const uint8_t* cs = reinterpret_cast<const uint8_t*>(L.stw) + shift;  // chunk byte 0
+    const bool unshuf = SEW != 0 && c < nfull;
+    uint32_t d[16];
+    if (israw) {
+        if (unshuf) {
+#pragma unroll
+            for (int p = 0; p < 16; p++) d[p] = staged_dword(L.stw, 1024 * p + 4 * t + shift);
+            unshuffled_store<SE>(d, dst);
+        } else {
+            for (uint32_t i = 16 * t; i < o; i += 4096) {
+                if (i + 16 <= o) {
+                    *reinterpret_cast<uint4*>(dst + i) =
+                        make_uint4(staged_dword(L.stw, i + shift), staged_dword(L.stw, i + 4 + shift),
+                                   staged_dword(L.stw, i + 8 + shift), staged_dword(L.stw, i + 12 + shift));
                 } else {
-                    w = 0;
-                    for (uint32_t k = 0; k < 4; k++)
-                        if (ga + k < in_bytes) w |= (uint32_t)in[ga + k] << (8 * k);
+                    for (uint32_t k = i; k < o; k++) dst[k] = cs[k];
                 }
-                stw[i] = w;
             }
-            wave_lds_sync();
-            const uint8_t* cs = st + shift;  // chunk byte 0
-            if (israw && SEW != 0 && c < nfull) {
-                uint32_t vv[16][4];
+        }
+        return;
+    }
+    const int bm1_bytes = ((int)o + 7) / 8;
+    const int bm2_bytes = (bm1_bytes + 7) / 8;
+    const int bm3_bytes = (bm2_bytes + 7) / 8;
+    const int bm4_start = (int)stored - 6;
+    const uint32_t bm4 = (uint32_t)cs[bm4_start] | ((uint32_t)cs[bm4_start + 1] << 8) |
+                         ((uint32_t)cs[bm4_start + 2] << 16) | ((uint32_t)cs[bm4_start + 3] << 24);
+    const int bm3_start = bm4_start - __popc(bm4);
+    // -- bm3: thread t < 32 owns byte t
+    if (t < 32) {
+        const uint32_t rank = __popc(bm4 & (t >= 31 ? 0xFFFFFFFFu : ((2u << t) - 1u)));
+        const int p = bm3_start + (int)rank - 1;
+        uint32_t b3 = 0;
+        if (t < bm3_bytes && rank > 0) {
+            if (p < 0) bad = true;
+            else b3 = cs[p];
+        }
+        L.bm3[t] = (uint8_t)b3;
+    }
+    __syncthreads();
+    const uint64_t m3[4] = {*reinterpret_cast<const uint64_t*>(L.bm3), *reinterpret_cast<const uint64_t*>(L.bm3 + 8),
+                            *reinterpret_cast<const uint64_t*>(L.bm3 + 16),
+                            *reinterpret_cast<const uint64_t*>(L.bm3 + 24)};
+    const uint32_t bm3_nr = __popcll(m3[0]) + __popcll(m3[1]) + __popcll(m3[2]) + __popcll(m3[3]);
+    const int bm2_start = bm3_start - (int)bm3_nr;
+    // -- bm2: thread t owns byte t; its flag is bm3 bit t
+    uint32_t b2 = 0;
+    {
+        uint32_t rank = 0;
 #pragma unroll
-                for (int j = 0; j < 16; j++)
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const uint32_t p = 1024 * j + 16 * l + 4 * q + shift;
-                        const uint32_t lo = stw[p >> 2], hi = stw[(p >> 2) + 1];
-                        vv[j][q] = (p & 3) ? __builtin_amdgcn_alignbyte(hi, lo, p & 3) : lo;
-                    }
-                unshuffled_store<SE>(vv, dst);
-            } else if (israw) {
-                // raw copy: cs[0 .. o) -> dst (dst 16-byte aligned)
-                for (uint32_t i = 16 * l; i < o; i += 1024) {
-                    uint32_t w4[4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const uint32_t p = i + 4 * q + shift;
-                        const uint32_t lo = stw[p >> 2], hi = stw[(p >> 2) + 1];
-                        w4[q] = (p & 3) ? __builtin_amdgcn_alignbyte(hi, lo, p & 3) : lo;
-                    }
-                    if (i + 16 <= o) *reinterpret_cast<uint4*>(dst + i) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
-                    else for (uint32_t k = i; k < o; k++) dst[k] = cs[k];
-                }
-            } else {
-                const int bm1_bytes = ((int)o + 7) / 8;
-                const int bm2_bytes = (bm1_bytes + 7) / 8;
-                const int bm3_bytes = (bm2_bytes + 7) / 8;
-                const int bm4_start = (int)stored - 6;
-                const uint32_t bm4 = (uint32_t)cs[bm4_start] | ((uint32_t)cs[bm4_start + 1] << 8) |
-                                     ((uint32_t)cs[bm4_start + 2] << 16) | ((uint32_t)cs[bm4_start + 3] << 24);
-                const int bm3_start = bm4_start - __popc(bm4);
-                // bm3: lanes 0..31, byte l
-                uint32_t b3 = 0;
-                {
-                    const uint32_t rank = __popc(bm4 & (l >= 31 ? 0xFFFFFFFFu : ((2u << l) - 1u)));
-                    const int p = bm3_start + (int)rank - 1;
-                    b3 = (l < bm3_bytes && rank > 0 && p >= 0) ? cs[p] : 0u;
-                    if (l < bm3_bytes && rank > 0 && p < 0) bad = true;
-                }
-                const uint32_t bm3_nr = wave_sum(l < 32 ? __popc(b3) : 0u);
-                const int bm2_start = bm3_start - (int)bm3_nr;
-                // bm2: lane l owns bytes 4l..4l+3; bits 4l..4l+3 of bm3 are its nibble
-                uint32_t b2w = 0;
-                {
-                    const uint32_t b3byte = __shfl(b3, (l >> 1) & 63, 64);
-                    const uint32_t nib = (b3byte >> (4 * (l & 1))) & 0xFu;
-                    const uint32_t cnt = __popc(nib);
-                    const uint32_t excl = wave_incl_scan(cnt) - cnt;
-                    uint32_t r = excl;
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        if (nib & (1u << k)) r++;
-                        const int p = bm2_start + (int)r - 1;
-                        uint32_t bv = 0;
-                        if (4 * l + k < bm2_bytes && r > 0) {
-                            if (p < 0) bad = true; else bv = cs[p];
-                        }
-                        b2w |= bv << (8 * k);
-                    }
-                }
-                const uint32_t bm2_nr = wave_sum(__popc(b2w));
-                const int bm1_start = bm2_start - (int)bm2_nr;
-                // bm1: lane l owns bytes 32l..32l+31; bits = bm2 bytes 4l..4l+3 = b2w
-                {
-                    const uint32_t cnt = __popc(b2w);
-                    const uint32_t excl = wave_incl_scan(cnt) - cnt;
-                    uint32_t r = excl;
-                    uint32_t wds[8];
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        uint32_t wq = 0;
-#pragma unroll
-                        for (int k = 0; k < 4; k++) {
-                            const int bi = 4 * q + k;
-                            if (b2w & (1u << bi)) r++;
-                            const int p = bm1_start + (int)r - 1;
-                            uint32_t bv = 0;
-                            if (32 * l + bi < bm1_bytes && r > 0) {
-                                if (p < 0) bad = true; else bv = cs[p];
-                            }
-                            wq |= bv << (8 * k);
-                        }
-                        wds[q] = wq;
-                    }
-                    *reinterpret_cast<uint4*>(bm1 + 32 * l) = make_uint4(wds[0], wds[1], wds[2], wds[3]);
-                    *reinterpret_cast<uint4*>(bm1 + 32 * l + 16) = make_uint4(wds[4], wds[5], wds[6], wds[7]);
-                }
-                const uint32_t data_nz = wave_sum(__popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l)) +
-                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 4)) +
-                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 8)) +
-                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 12)) +
-                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 16)) +
-                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 20)) +
-                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 24)) +
-                                                  __popc(*reinterpret_cast<const uint32_t*>(bm1 + 32 * l + 28)));
-                if ((int)data_nz > bm1_start) bad = true;
-                wave_lds_sync();
-                // zero-elimination decode: lane l, step j -> output bytes [1024 j + 16 l, +16)
-                const bool unshuf = SEW != 0 && c < nfull;
-                uint32_t vv[16][4];
-                uint32_t run = 0;
-#pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    const uint32_t mm = *reinterpret_cast<const uint16_t*>(bm1 + 128 * j + 2 * l);
-                    const uint32_t cnt = __popc(mm);
-                    const uint32_t incl = wave_incl_scan(cnt);
-                    uint32_t r = run + incl - cnt;
-                    uint32_t w4[4] = {0, 0, 0, 0};
-#pragma unroll
-                    for (int k = 0; k < 16; k++) {
-                        if (mm & (1u << k)) {
-                            const uint32_t bv = (r < data_nz && (int)r < bm1_start) ? cs[r] : 0u;
-                            w4[k >> 2] |= bv << (8 * (k & 3));
-                            r++;
-                        }
-                    }
-                    run += __shfl(incl, 63, 64);
-                    const uint32_t p = 1024 * j + 16 * l;
-                    if (unshuf) {
-#pragma unroll
-                        for (int q = 0; q < 4; q++) vv[j][q] = w4[q];
-                    } else if (p + 16 <= o) {
-                        *reinterpret_cast<uint4*>(dst + p) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
-                    } else {
-                        for (uint32_t k = p; k < o; k++) dst[k] = (uint8_t)(w4[(k - p) >> 2] >> (8 * ((k - p) & 3)));
-                    }
-                }
-                if (unshuf) unshuffled_store<SE>(vv, dst);
-                if (__ballot(bad) != 0ull && l == 0) *d_status = 1;
-            }
+        for (int j = 0; j < 4; j++) {
+            const uint64_t m = m3[j];
+            const int lo = 64 * j;
+            if (t >= lo + 63) rank += __popcll(m);
+            else if (t >= lo) rank += __popcll(m & ((2ull << (t - lo)) - 1ull));
+        }
+        const int p = bm2_start + (int)rank - 1;
+        if (t < bm2_bytes && rank > 0) {
+            if (p < 0) bad = true;
+            else b2 = cs[p];
         }
     }
+    const uint32_t bm2_nr = dec_block_sum(L, __popc(b2), 0);
+    const int bm1_start = bm2_start - (int)bm2_nr;
+    // -- bm1: thread t owns bytes 8t..8t+7; their flags are the bits of bm2 byte t
+    {
+        const uint32_t cnt = __popc(b2);
+        const uint32_t incl = wave_incl_scan(cnt);
+        if (l == 63) L.hsum[w][1] = incl;
+        __syncthreads();
+        uint32_t r = incl - cnt;
+        for (int j = 0; j < w; j++) r += L.hsum[j][1];
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (b2 & (1u << k)) r++;
+            const int p = bm1_start + (int)r - 1;
+            uint32_t bv = 0;
+            if (8 * t + k < bm1_bytes && r > 0) {
+                if (p < 0) bad = true;
+                else bv = cs[p];
+            }
+            if (k < 4) lo |= bv << (8 * k);
+            else hi |= bv << (8 * (k - 4));
+        }
+        *reinterpret_cast<uint32_t*>(L.bm1 + 8 * t) = lo;
+        *reinterpret_cast<uint32_t*>(L.bm1 + 8 * t + 4) = hi;
+        const uint32_t data_nz = dec_block_sum(L, __popc(lo) + __popc(hi), 2);  // barrier: bm1 complete
+        if ((int)data_nz > bm1_start) bad = true;
+    }
+    // -- zero elimination: dword d[p] = chunk bytes [1024 p + 4 t, +4)
+    uint32_t incl[16];
+#pragma unroll
+    for (int p = 0; p < 16; p++) {
+        const uint32_t m = (L.bm1[128 * p + (t >> 1)] >> (4 * (t & 1))) & 0xFu;
+        incl[p] = wave_incl_scan(__popc(m));
+        if (l == 63) L.wsum[w][p] = incl[p];
+    }
+    __syncthreads();
+    uint32_t run = 0;
+#pragma unroll
+    for (int p = 0; p < 16; p++) {
+        uint32_t before = 0, all = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t v = L.wsum[j][p];
+            before += j < w ? v : 0u;
+            all += v;
+        }
+        const uint32_t m = (L.bm1[128 * p + (t >> 1)] >> (4 * (t & 1))) & 0xFu;
+        uint32_t r = run + before + incl[p] - __popc(m);
+        uint32_t x = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (m & (1u << k)) {
+                const uint32_t bv = ((int)r < bm1_start) ? cs[r] : 0u;
+                x |= bv << (8 * k);
+                r++;
+            }
+        d[p] = x;
+        run += all;
+    }
+    if (unshuf) {
+        unshuffled_store<SE>(d, dst);
+    } else {
+#pragma unroll
+        for (int p = 0; p < 16; p++) {
+            const uint32_t b = 1024 * p + 4 * t;
+            if (b + 4 <= o) *reinterpret_cast<uint32_t*>(dst + b) = d[p];
+            else for (uint32_t k = b; k < o; k++) dst[k] = (uint8_t)(d[p] >> (8 * (k - b)));
+        }
+    }
+    if (bad) *d_status = 1;
 }
-
-constexpr size_t kLds = (size_t)WAVES * LDS_PER_WAVE;
 
 // Encode workspace: flagged sizes, clean sizes, offsets (nc + 1), the scan
 // workspace, then one CS-byte scratch slot per chunk.
@@ -398,17 +421,16 @@ int rze_dec_impl(hipStream_t s, const void* d_in, size_t in_bytes, void* d_out, 
     if (!d_status) return fail(FZH_E_INVALID, "RZE: null status pointer");
     const uint32_t nc = (uint32_t)cdiv(orig_bytes, CS);
     const uint64_t nfull = orig_bytes / CS;
-    const unsigned blocks = (unsigned)cdiv(nc, WAVES);
     uint32_t* offs = (uint32_t*)d_ws;
     const uint8_t* in = (const uint8_t*)d_in;
     uint8_t* out = (uint8_t*)d_out;
     rze_offsets_kernel<<<1, 1024, 0, s>>>(in, in_bytes, offs, nc);
     if (sew == 1)
-        rze_dec_kernel<1><<<blocks, 64 * WAVES, kLds, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, nfull);
+        rze_dec_kernel<1><<<nc, 256, 0, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, nfull);
     else if (sew == 2)
-        rze_dec_kernel<2><<<blocks, 64 * WAVES, kLds, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, nfull);
+        rze_dec_kernel<2><<<nc, 256, 0, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, nfull);
     else
-        rze_dec_kernel<0><<<blocks, 64 * WAVES, kLds, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, 0);
+        rze_dec_kernel<0><<<nc, 256, 0, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, 0);
     FZH_CHECK_LAUNCH("rze_dec_kernel");
     return FZH_OK;
 }
