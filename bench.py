@@ -34,6 +34,12 @@ HBM_PEAK_GBS = 8000.0
 NX, NY = 3600, 1800
 USER_EB = 1e-3
 CAP = 0.10
+# HBM bytes per launch of the chain's main kernels on this workload, from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the
+# same counters read for a plain copy of known size (FETCH x 2.0, WRITE x
+# 1.0); the counter runs and the scaling are kept with the profiles.  None
+# until measured for the current kernels.
+MEASURED_TRAFFIC = {}
 # Reference CLI example, benchmark mean of 10, host time (docs/cli.md:66-70)
 REF_COMPRESS_HOST_GBS = 18.85
 
@@ -144,9 +150,10 @@ def main():
     p.enable_profiling(False)
 
     # ---- per-kernel device time, HIP events on the launch stream ------------
-    kern = kernel_times(args.kernel_reps, x, d_in, abs_eb)
-    dom = max((k for k in kern if kern[k]["pipeline"]), key=lambda k: kern[k]["ms"])
-    d = kern[dom]
+    kern, single = kernel_times(args.kernel_reps, x, d_in, abs_eb)
+    # dominant kernel of the compress chain (the main kernel of each stage)
+    dom = max(("minmax_partial_kernel", "lz2d_fwd_kernel", "rze_enc_kernel"), key=lambda k: single[k]["ms"])
+    d = single[dom]
     roof = {
         "kernel": dom,
         "bound": "hbm",
@@ -154,7 +161,9 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "avg_launch_us": round(d["ms"] * 1e3, 3),
+        "algorithmic_bytes": d["bytes"],
+        "traffic": MEASURED_TRAFFIC.get(dom),
     }
     chain = ("value_base", "lorenzo_fwd_2d", "bitshuffle_rze_enc")
     chain_bytes = sum(kern[k]["bytes"] for k in chain)
@@ -197,6 +206,8 @@ def main():
         "compress_chain_roofline_frac": round(chain_bytes / (chain_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": {k: {kk: round(vv, 4) if isinstance(vv, float) else vv for kk, vv in v.items()}
                     for k, v in kern.items()},
+        "single_kernels": {k: {kk: round(vv, 5) if isinstance(vv, float) else vv for kk, vv in v.items()}
+                           for k, v in single.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -331,12 +342,44 @@ def kernel_times(reps, x, d_in, abs_eb):
                      "frac": nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "pipeline": used}
         log(f"[bench] kernel {name:16s} {ms * 1e3:8.1f} us  {out[name]['gbs']:7.1f} GB/s  "
             f"{100 * out[name]['frac']:5.1f}% of HBM peak")
+    # ---- single kernels: library HIP-event pairs around each launch --------
+    # (name, algorithmic bytes per launch or None for helper kernels, whose
+    # bytes are not compulsory) per entry point the pipeline runs
+    per_entry = [
+        (vb, [("minmax_partial_kernel", 4 * nv)]),
+        (lzf, [("lz2d_fwd_kernel", 4 * NX * NY + 2 * n + 8 * k + 8 * kp + 8), ("scan_small_kernel", None),
+               ("place_kernel", None)]),
+        (frze, [("rze_enc_kernel", 2 * n + C_rze), ("scan_small_kernel", None), ("rze_pack_kernel", None)]),
+        (frzd, [("rze_offsets_kernel", None), ("rze_dec_kernel", C_rze + 2 * n)]),
+        (lzi, [("bin_count_kernel", None), ("scan_small_kernel", None), ("bin_place_kernel", None),
+               ("lz2d_inv_kernel", 2 * n + 8 * k + 4 + 4 * NX * NY)]),
+        (cpy, [("copy_kernel", 8 * nv)]),
+    ]
+    single = {}
+    tot, cnt_ = C.c_double(), C.c_uint64()
+    for fn, names in per_entry:
+        call("fzh_kernel_timing_enable", 1)  # clears the totals
+        for _ in range(reps):
+            fn()
+        call("fzh_stream_sync", st)
+        for kname, nb in names:
+            call("fzh_kernel_timing_get", kname.encode(), C.byref(tot), C.byref(cnt_))
+            if cnt_.value == 0:
+                raise RuntimeError(f"kernel timing recorded no launch of {kname}")
+            ms = tot.value / cnt_.value
+            key = kname if kname not in single else f"{kname}@{names[0][0]}"
+            single[key] = {"ms": ms, "launches": int(cnt_.value), "bytes": nb}
+            if nb:
+                single[key]["gbs"] = nb / (ms * 1e-3) / 1e9
+                single[key]["frac"] = single[key]["gbs"] / HBM_PEAK_GBS
+            log(f"[bench] single kernel {key:32s} {ms * 1e3:8.2f} us"
+                + (f"  {single[key]['gbs']:7.1f} GB/s  {100 * single[key]['frac']:5.1f}% of HBM peak" if nb else ""))
+    call("fzh_kernel_timing_enable", 0)
     st_ = int(status.download(np.int32, 1)[0])
     if st_ != 0:
         raise RuntimeError("rze_dec reported a malformed stream in the kernel timing loop")
     call("fzh_stream_destroy", st)
-    out["_outliers"] = k  # noqa: kept for the record
-    return {kk: vv for kk, vv in out.items() if not kk.startswith("_")}
+    return {kk: vv for kk, vv in out.items() if not kk.startswith("_")}, single
 
 
 def batched_workload(args):

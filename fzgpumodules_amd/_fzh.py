@@ -133,6 +133,10 @@ SIGNATURES = {
     "fzs_decompress_field": (_I, [_P, _P, _P, _Z, _P, _Z]),
     "fzs_last_choices": (_I, [_P, C.POINTER(_P), C.POINTER(_Z)]),
     "fzs_last_timing": (_I, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_F)]),
+    "fzh_kernel_timing_enable": (_I, [C.c_int]),
+    "fzh_kernel_timing_get": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "fzh_scan_ws_bytes": (C.c_size_t, [C.c_uint64]),
+    "fzh_scan_u32": (_I, [_P, _P, _P, C.c_uint64, _P, C.c_int, _P, C.c_size_t]),
     "fzh_periodic_window_f32": (_I, [_P, _P] + [C.c_uint64] * 6 + [_P] + [C.c_uint64] * 3),
     "fzh_device_alloc": (_I, [C.POINTER(_P), _Z]),
     "fzh_device_free": (_I, [_P]),

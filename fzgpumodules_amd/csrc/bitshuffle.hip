@@ -190,6 +190,7 @@ int launch(hipStream_t s, const void* in, void* out, size_t nbytes, uint32_t bs,
     const unsigned grid = (unsigned)(nblk + (tail ? 1 : 0));
     const uint8_t* i8 = (const uint8_t*)in;
     uint8_t* o8 = (uint8_t*)out;
+    KernelTimer kt_(s, "bitshuffle_kernel");
     switch (ew) {
     case 1: bitshuffle_kernel<1, ENC><<<grid, 256, 0, s>>>(i8, o8, nbytes, bs, (uint32_t)nblk); break;
     case 2: bitshuffle_kernel<2, ENC><<<grid, 256, 0, s>>>(i8, o8, nbytes, bs, (uint32_t)nblk); break;

@@ -1,8 +1,13 @@
 // fzh_core.hip — library plumbing: error strings, workspace init, device
 // helpers used by the bindings, and the multi-output gather kernel.
 #include "fzh_internal.h"
+#include "fzh_scan.h"
 
+#include <atomic>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
 
 namespace fzh {
 
@@ -13,6 +18,80 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 int fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
+}
+
+// ---------------------------------------------------------------------------
+// Per-kernel timing (opt-in; see KernelTimer)
+// ---------------------------------------------------------------------------
+namespace {
+struct KtPending {
+    std::string name;
+    hipEvent_t a, b;
+};
+std::atomic<bool> g_kt_on{false};
+std::mutex g_kt_mu;
+std::vector<KtPending> g_kt_pending;
+std::vector<hipEvent_t> g_kt_pool;
+std::map<std::string, std::pair<double, uint64_t>> g_kt_acc;
+
+hipEvent_t kt_event() {
+    if (!g_kt_pool.empty()) {
+        hipEvent_t e = g_kt_pool.back();
+        g_kt_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Resolves every pending pair (waits for them); caller holds g_kt_mu.
+int kt_drain() {
+    int rc = FZH_OK;
+    for (auto& p : g_kt_pending) {
+        if (p.name.empty()) {  // its stop event was never recorded
+            g_kt_pool.push_back(p.a);
+            g_kt_pool.push_back(p.b);
+            continue;
+        }
+        float ms = 0.f;
+        hipError_t e = hipEventSynchronize(p.b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.a, p.b);
+        if (e == hipSuccess) {
+            auto& acc = g_kt_acc[p.name];
+            acc.first += ms;
+            acc.second += 1;
+        } else if (rc == FZH_OK) {
+            rc = fail(FZH_E_RUNTIME, std::string("kernel timing: ") + hipGetErrorString(e));
+        }
+        g_kt_pool.push_back(p.a);
+        g_kt_pool.push_back(p.b);
+    }
+    g_kt_pending.clear();
+    return rc;
+}
+}  // namespace
+
+int ktime_begin(hipStream_t s, const char* name) {
+    if (!g_kt_on.load(std::memory_order_relaxed)) return -1;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return -1;
+    std::lock_guard<std::mutex> g(g_kt_mu);
+    hipEvent_t a = kt_event(), b = kt_event();
+    if (!a || !b || hipEventRecord(a, s) != hipSuccess) {
+        if (a) g_kt_pool.push_back(a);
+        if (b) g_kt_pool.push_back(b);
+        return -1;
+    }
+    g_kt_pending.push_back({name, a, b});
+    return (int)g_kt_pending.size() - 1;
+}
+
+void ktime_end(hipStream_t s, int slot) {
+    std::lock_guard<std::mutex> g(g_kt_mu);
+    if (slot >= 0 && slot < (int)g_kt_pending.size()) {
+        if (hipEventRecord(g_kt_pending[slot].b, s) != hipSuccess) g_kt_pending[slot].name.clear();
+    }
 }
 
 // Fill kernel: zeroes a byte range with 16-byte stores (no memset node, so it
@@ -191,6 +270,7 @@ int fzh_copy(void* stream, void* d_dst, const void* d_src, size_t bytes) {
     const size_t nw = bytes / 16;
     if (nw) {
         const size_t blocks = std::min<size_t>(cdiv(nw, 256 * 4), 8192);
+        KernelTimer kt_(S(stream), "copy_kernel");
         copy_kernel<<<(unsigned)blocks, 256, 0, S(stream)>>>((const uint4*)d_src, (uint4*)d_dst, nw);
     }
     if (bytes % 16)
@@ -208,6 +288,47 @@ int fzh_periodic_window_f32(void* stream, const float* d_base, uint64_t bx, uint
     const unsigned blocks = (unsigned)std::min<uint64_t>(cdiv(total, 256), 65536);
     periodic_window_kernel<<<blocks, 256, 0, S(stream)>>>(d_base, bx, by, bz, x0, y0, z0, d_out, nx, ny, total);
     FZH_CHECK_LAUNCH("periodic_window_kernel");
+    return FZH_OK;
+}
+
+int fzh_kernel_timing_enable(int on) {
+    std::lock_guard<std::mutex> g(fzh::g_kt_mu);
+    int rc = fzh::kt_drain();
+    fzh::g_kt_acc.clear();
+    fzh::g_kt_on.store(on != 0);
+    return rc;
+}
+
+int fzh_kernel_timing_get(const char* name, double* total_ms, uint64_t* count) {
+    if (!name || !total_ms || !count) return fail(FZH_E_INVALID, "kernel timing: null argument");
+    std::lock_guard<std::mutex> g(fzh::g_kt_mu);
+    if (int rc = fzh::kt_drain()) return rc;
+    auto it = fzh::g_kt_acc.find(name);
+    *total_ms = it == fzh::g_kt_acc.end() ? 0.0 : it->second.first;
+    *count = it == fzh::g_kt_acc.end() ? 0 : it->second.second;
+    return FZH_OK;
+}
+
+size_t fzh_scan_ws_bytes(uint64_t n) { return scan_ws_bytes(n); }
+
+int fzh_scan_u32(void* stream, uint32_t* d_in, uint32_t* d_out, uint64_t n, uint32_t* d_total, int flags,
+                 void* d_ws, size_t ws_bytes) {
+    if ((!d_in || !d_out) && n) return fail(FZH_E_INVALID, "scan: null pointer");
+    if (!d_total) return fail(FZH_E_INVALID, "scan: null total");
+    if (flags & ~(FZH_SCAN_WRAP | FZH_SCAN_CLEAR)) return fail(FZH_E_INVALID, "scan: unknown flags");
+    if ((flags & FZH_SCAN_CLEAR) && d_in == d_out && n) return fail(FZH_E_INVALID, "scan: clear needs d_in != d_out");
+    if (n > (uint64_t)SCAN_SEG * SCAN_MAX_SEGS) return fail(FZH_E_INVALID, "scan: more than 2^28 elements");
+    if (n > SCAN_SMALL && (!d_ws || ws_bytes < scan_ws_bytes(n))) return fail(FZH_E_CAPACITY, "scan: workspace too small");
+    hipStream_t s = S(stream);
+    int rc;
+    switch (flags) {
+        case 0: rc = scan_u32<false, false>(s, d_in, d_out, n, nullptr, d_total, d_ws); break;
+        case FZH_SCAN_WRAP: rc = scan_u32<true, false>(s, d_in, d_out, n, nullptr, d_total, d_ws); break;
+        case FZH_SCAN_CLEAR: rc = scan_u32<false, true>(s, d_in, d_out, n, nullptr, d_total, d_ws); break;
+        default: rc = scan_u32<true, true>(s, d_in, d_out, n, nullptr, d_total, d_ws); break;
+    }
+    if (rc) return rc;
+    FZH_CHECK_LAUNCH("scan");
     return FZH_OK;
 }
 

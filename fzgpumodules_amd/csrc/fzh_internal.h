@@ -21,6 +21,21 @@ int fail(int code, const std::string& msg);
 // Zeroes [p, p+n) with a fill kernel on stream s.
 int zero_bytes(hipStream_t s, void* p, size_t n);
 
+// Optional per-kernel device timing: while enabled (fzh_kernel_timing_enable)
+// and the stream is not being captured, a KernelTimer records a HIP event
+// pair on the launch stream around the one launch in its scope; the pairs
+// are resolved into per-name totals by fzh_kernel_timing_get.
+int ktime_begin(hipStream_t s, const char* name);  // slot, or -1 when off
+void ktime_end(hipStream_t s, int slot);
+struct KernelTimer {
+    hipStream_t s;
+    int slot;
+    KernelTimer(hipStream_t st, const char* name) : s(st), slot(ktime_begin(st, name)) {}
+    ~KernelTimer() {
+        if (slot >= 0) ktime_end(s, slot);
+    }
+};
+
 #define FZH_TRY_HIP(expr)                                                        \
     do {                                                                         \
         hipError_t e_ = (expr);                                                  \

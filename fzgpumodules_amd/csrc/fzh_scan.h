@@ -44,106 +44,165 @@ scan_reduce_kernel(const uint32_t* __restrict__ in, uint64_t n, const uint32_t* 
     }
 }
 
+// 64-bit inclusive scan across the 64 lanes of a wave.
+__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t t = __shfl_up(v, off, 64);
+        if (l >= off) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t sat_u32(uint64_t v, bool wrap) {
+    return (!wrap && v > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)v;
+}
+
+// Exclusive scan of in[0..n) into out (may alias in) by one 1024-thread
+// workgroup in passes of 8192 elements: each pass is read and written with
+// coalesced dword accesses through LDS, and thread t scans the pass's
+// elements [8t, 8t+8) in registers.  Returns the sum (64-bit).  CLEAR zeroes
+// in[] as it is read (in and out distinct).
+template <bool CLEAR>
+__device__ __forceinline__ uint64_t block_scan_1024(uint32_t* in, uint32_t* out, uint32_t n, bool wrap) {
+    constexpr uint32_t PER = 8, PASS = 1024 * PER;
+    __shared__ __align__(16) uint32_t s_v[PASS];
+    __shared__ uint64_t s_w[16];
+    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < n; base += PASS) {
+        const uint32_t m = n - base < PASS ? n - base : PASS;
+#pragma unroll
+        for (uint32_t r = 0; r < PER; r++) {
+            const uint32_t i = 1024 * r + tid;
+            uint32_t v = 0;
+            if (i < m) {
+                v = in[base + i];
+                if constexpr (CLEAR) in[base + i] = 0u;
+            }
+            s_v[i] = v;
+        }
+        __syncthreads();
+        const uint4 x0 = *reinterpret_cast<const uint4*>(s_v + PER * tid);
+        const uint4 x1 = *reinterpret_cast<const uint4*>(s_v + PER * tid + 4);
+        const uint32_t x[PER] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+        uint64_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) sum += x[k];
+        const uint64_t incl = wave_incl_scan64(sum);
+        if (l == 63) s_w[w] = incl;
+        __syncthreads();
+        uint64_t run = carry + incl - sum, tot = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint64_t t = s_w[j];
+            run += j < w ? t : 0ull;
+            tot += t;
+        }
+        uint32_t y[PER];
+#pragma unroll
+        for (uint32_t k = 0; k < PER; k++) {
+            y[k] = sat_u32(run, wrap);
+            run += x[k];
+        }
+        *reinterpret_cast<uint4*>(s_v + PER * tid) = make_uint4(y[0], y[1], y[2], y[3]);
+        *reinterpret_cast<uint4*>(s_v + PER * tid + 4) = make_uint4(y[4], y[5], y[6], y[7]);
+        carry += tot;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t r = 0; r < PER; r++) {
+            const uint32_t i = 1024 * r + tid;
+            if (i < m) out[base + i] = s_v[i];
+        }
+        __syncthreads();  // s_v and s_w are rewritten by the next pass
+    }
+    return carry;
+}
+
 // In-place exclusive scan of seg[0..nseg) by one 1024-thread workgroup;
 // *total = sum (saturated to 0xFFFFFFFF, or modulo 2^32 with WRAP).
 template <bool WRAP>
 __global__ void __launch_bounds__(1024)
 scan_partials_kernel(uint32_t* __restrict__ seg, uint32_t nseg, uint32_t* __restrict__ total) {
-    __shared__ uint64_t s_w[16];
-    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
-    const uint32_t per = (nseg + 1023) / 1024;
-    const uint32_t a = tid * per;
-    uint64_t sum = 0;
-    for (uint32_t k = 0; k < per; k++)
-        if (a + k < nseg) sum += seg[a + k];
-    uint64_t incl = sum;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint64_t t = __shfl_up(incl, off, 64);
-        if (l >= off) incl += t;
-    }
-    if (l == 63) s_w[w] = incl;
-    __syncthreads();
-    uint64_t run = incl - sum;
-    for (int j = 0; j < w; j++) run += s_w[j];
-    for (uint32_t k = 0; k < per; k++) {
-        if (a + k >= nseg) break;
-        const uint32_t v = seg[a + k];
-        seg[a + k] = (!WRAP && run > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)run;
-        run += v;
-    }
-    if (tid == 1023 && total) *total = (!WRAP && run > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)run;
+    const uint64_t t = block_scan_1024<false>(seg, seg, nseg, WRAP);
+    if (threadIdx.x == 0 && total) *total = sat_u32(t, WRAP);
 }
 
-// out[i] = seg_base[s] + exclusive prefix of in inside segment s.  Thread t
-// owns elements [16t, 16t+16) of its segment.  CLEAR: in[i] is zeroed after
-// it is read (in and out distinct), ready for counters accumulated next time.
-template <bool CLEAR>
+// out[i] = seg_base[s] + exclusive prefix of in inside segment s; the
+// segment's 4096 elements go through LDS (coalesced dword loads and stores)
+// and thread t scans elements [16t, 16t+16) of it in registers.  CLEAR: in[i]
+// is zeroed after it is read (in and out distinct), ready for counters
+// accumulated next time.
+template <bool WRAP, bool CLEAR>
 __global__ void __launch_bounds__(256)
 scan_apply_kernel(uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n,
                   const uint32_t* __restrict__ d_n, const uint32_t* __restrict__ seg_base) {
-    __shared__ uint32_t s_w[4];
+    __shared__ __align__(16) uint32_t s_v[SCAN_SEG];
+    __shared__ uint64_t s_w[4];
     n = eff_count(n, d_n);
     const int tid = threadIdx.x, w = tid >> 6;
-    const uint64_t b = (uint64_t)blockIdx.x * SCAN_SEG + 16ull * tid;
-    uint32_t v[16], sum = 0;
+    const uint64_t b = (uint64_t)blockIdx.x * SCAN_SEG;
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        v[k] = (b + k < n) ? in[b + k] : 0u;
-        sum += v[k];
+    for (int r = 0; r < 16; r++) {
+        const uint64_t i = b + 256u * r + tid;
+        uint32_t v = 0;
+        if (i < n) {
+            v = in[i];
+            if constexpr (CLEAR) in[i] = 0u;
+        }
+        s_v[256 * r + tid] = v;
     }
-    if constexpr (CLEAR) {
+    __syncthreads();
+    uint32_t x[16];
 #pragma unroll
-        for (int k = 0; k < 16; k++)
-            if (b + k < n) in[b + k] = 0u;
+    for (int q = 0; q < 4; q++) {
+        const uint4 u = *reinterpret_cast<const uint4*>(s_v + 16 * tid + 4 * q);
+        x[4 * q] = u.x;
+        x[4 * q + 1] = u.y;
+        x[4 * q + 2] = u.z;
+        x[4 * q + 3] = u.w;
     }
-    const uint32_t incl = wave_incl_scan(sum);
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) sum += x[k];
+    const uint64_t incl = wave_incl_scan64(sum);
     if (lane_id() == 63) s_w[w] = incl;
     __syncthreads();
-    uint32_t run = seg_base[blockIdx.x] + incl - sum;
+    const uint32_t sb = seg_base[blockIdx.x];
+    // a saturated segment base stays saturated
+    uint64_t run = ((!WRAP && sb == 0xFFFFFFFFu) ? 0x100000000ull : (uint64_t)sb) + incl - sum;
     for (int j = 0; j < w; j++) run += s_w[j];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (b + k < n) out[b + k] = run;
-        run += v[k];
+    for (int q = 0; q < 4; q++) {
+        uint32_t y[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            y[k] = sat_u32(run, WRAP);
+            run += x[4 * q + k];
+        }
+        *reinterpret_cast<uint4*>(s_v + 16 * tid + 4 * q) = make_uint4(y[0], y[1], y[2], y[3]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint64_t i = b + 256u * r + tid;
+        if (i < n) out[i] = s_v[256 * r + tid];
     }
 }
 
-// Whole scan in one 1024-thread workgroup for n <= SCAN_SMALL: thread t owns
-// elements [t*per, (t+1)*per).  Saves two launches on the short arrays
-// (per-tile counts, chunk sizes of one field) that most calls scan.
+// Whole scan in one 1024-thread workgroup for n <= SCAN_SMALL (saves two
+// launches on the short arrays -- per-tile counts, chunk sizes of one field
+// -- that most calls scan).
 constexpr uint32_t SCAN_SMALL = 65536;
 
 template <bool WRAP, bool CLEAR>
 __global__ void __launch_bounds__(1024)
 scan_small_kernel(uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n_max,
                   const uint32_t* __restrict__ d_n, uint32_t* __restrict__ total) {
-    __shared__ uint64_t s_w[16];
     const uint32_t n = (uint32_t)eff_count(n_max, d_n);
-    const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
-    const uint32_t per = (n_max + 1023) / 1024;
-    const uint32_t a = tid * per;
-    uint64_t sum = 0;
-    for (uint32_t k = 0; k < per; k++)
-        if (a + k < n) sum += in[a + k];
-    uint64_t incl = sum;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint64_t t = __shfl_up(incl, off, 64);
-        if (l >= off) incl += t;
-    }
-    if (l == 63) s_w[w] = incl;
-    __syncthreads();
-    uint64_t run = incl - sum;
-    for (int j = 0; j < w; j++) run += s_w[j];
-    for (uint32_t k = 0; k < per; k++) {
-        if (a + k >= n) break;
-        const uint32_t v = in[a + k];
-        if constexpr (CLEAR) in[a + k] = 0u;
-        out[a + k] = (!WRAP && run > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)run;
-        run += v;
-    }
-    if (tid == 1023 && total) *total = (!WRAP && run > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)run;
+    const uint64_t t = block_scan_1024<CLEAR>(in, out, n, WRAP);
+    if (threadIdx.x == 0 && total) *total = sat_u32(t, WRAP);
 }
 
 inline size_t scan_ws_bytes(size_t n) { return align_up(4 * std::max<size_t>(cdiv(n, SCAN_SEG), 1), 256); }
@@ -156,6 +215,7 @@ template <bool WRAP = false, bool CLEAR = false>
 inline int scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, size_t n, const uint32_t* d_n,
                     uint32_t* d_total, void* ws) {
     if (n <= SCAN_SMALL) {
+        KernelTimer kt_(s, "scan_small_kernel");
         scan_small_kernel<WRAP, CLEAR><<<1, 1024, 0, s>>>(const_cast<uint32_t*>(in), out, (uint32_t)n, d_n, d_total);
         return FZH_OK;
     }
@@ -164,7 +224,7 @@ inline int scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, size_t n, 
     uint32_t* seg = (uint32_t*)ws;
     scan_reduce_kernel<WRAP><<<(unsigned)nseg, 256, 0, s>>>(in, n, d_n, seg);
     scan_partials_kernel<WRAP><<<1, 1024, 0, s>>>(seg, (uint32_t)nseg, d_total);
-    if (n) scan_apply_kernel<CLEAR><<<(unsigned)nseg, 256, 0, s>>>(const_cast<uint32_t*>(in), out, n, d_n, seg);
+    if (n) scan_apply_kernel<WRAP, CLEAR><<<(unsigned)nseg, 256, 0, s>>>(const_cast<uint32_t*>(in), out, n, d_n, seg);
     return FZH_OK;
 }
 

@@ -562,10 +562,13 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
         lz3d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(                                         \
             in, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, n, r, d_r, rad, c, tl,                            \
             Tiles{(uint32_t)cdiv(nx, 8), (uint32_t)cdiv(ny, 8), 0}, (uint32_t)T);
-    if (zz) {
-        FZH_LZ_FWD(true)
-    } else {
-        FZH_LZ_FWD(false)
+    {
+        KernelTimer kt_(s, nd == 1 ? "lz1d_fwd_kernel" : nd == 2 ? "lz2d_fwd_kernel" : "lz3d_fwd_kernel");
+        if (zz) {
+            FZH_LZ_FWD(true)
+        } else {
+            FZH_LZ_FWD(false)
+        }
     }
 #undef FZH_LZ_FWD
     FZH_CHECK_LAUNCH("lorenzo fwd");
@@ -622,10 +625,12 @@ int launch_inv(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny,
     const bool has_out = count != nullptr && max_out > 0;
     const unsigned ob = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(max_out, 256), 1024));
     if (has_out) {
+        KernelTimer kt_(s, "bin_count_kernel");
         bin_count_kernel<<<ob, 256, 0, s>>>(idx, count, max_out, m, cnt, slot);
     }
     if (int rc = scan_u32<false, true>(s, cnt, off, L.T, nullptr, off + L.T, w + L.o_scan)) return rc;
     if (has_out) {
+        KernelTimer kt_(s, "bin_place_kernel");
         bin_place_kernel<<<ob, 256, 0, s>>>(err, idx, count, max_out, m, off, slot, serr, sloc);
     }
     FZH_CHECK_LAUNCH("lorenzo inv binning");
@@ -641,10 +646,13 @@ int launch_inv(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny,
         lz3d_inv_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(c, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, \
                                                            ebx2, off, serr, sloc, out,                 \
                                                            Tiles{m.gx, m.gy, 0}, (uint32_t)L.T);
-    if (zz) {
-        FZH_LZ_INV(true)
-    } else {
-        FZH_LZ_INV(false)
+    {
+        KernelTimer kt_(s, nd == 1 ? "lz1d_inv_kernel" : nd == 2 ? "lz2d_inv_kernel" : "lz3d_inv_kernel");
+        if (zz) {
+            FZH_LZ_INV(true)
+        } else {
+            FZH_LZ_INV(false)
+        }
     }
 #undef FZH_LZ_INV
     FZH_CHECK_LAUNCH("lorenzo inv");
@@ -735,6 +743,7 @@ int fzh_lorenzo_apply_patches(void* stream, float* d_out, const float* d_patch_v
     if (!d_out || !d_patch_val || !d_patch_idx || !d_patch_count)
         return fail(FZH_E_INVALID, "lorenzo patches: null pointer");
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(max_patches, 256), 1024));
+    KernelTimer kt_(S(stream), "apply_patches_kernel");
     apply_patches_kernel<<<blocks, 256, 0, S(stream)>>>(d_out, d_patch_val, d_patch_idx, d_patch_count, max_patches,
                                                          n_out);
     FZH_CHECK_LAUNCH("apply_patches_kernel");
@@ -785,7 +794,10 @@ int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode, floa
     // about 16 float4 per thread, at most two workgroups per CU
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256 * 64), 512));
     float2* partial = (float2*)((uint8_t*)d_ws + 256);
-    minmax_partial_kernel<<<blocks, 256, 0, S(stream)>>>(d_in, n, partial, zero_pad);
+    {
+        KernelTimer kt_(S(stream), "minmax_partial_kernel");
+        minmax_partial_kernel<<<blocks, 256, 0, S(stream)>>>(d_in, n, partial, zero_pad);
+    }
     minmax_final_kernel<<<1, 256, 0, S(stream)>>>(partial, blocks, mode, user_eb, d_params);
     FZH_CHECK_LAUNCH("minmax");
     return FZH_OK;

@@ -208,6 +208,18 @@ def quant_abs_inv(codes, ebx2, zigzag, vals, idx, count):
     return out.download(np.float32, codes.size)
 
 
+def scan_u32(v, wrap=False, clear=False, in_place=False):
+    """Exclusive scan through fzh_scan_u32: (prefixes, total, input after the call)."""
+    v = np.ascontiguousarray(v, np.uint32)
+    din = DeviceBuffer.from_array(v if v.size else np.zeros(1, np.uint32))
+    dout = din if in_place else DeviceBuffer(max(v.nbytes, 16))
+    tot = DeviceBuffer(16)
+    ws = workspace(lib().fzh_scan_ws_bytes(v.size))
+    flags = (1 if wrap else 0) | (2 if clear else 0)
+    call("fzh_scan_u32", None, din.ptr, dout.ptr, v.size, tot.ptr, flags, ws.ptr, ws.nbytes)
+    return dout.download(np.uint32, v.size), int(tot.download(np.uint32, 1)[0]), din.download(np.uint32, v.size)
+
+
 def max_u32(v):
     v = np.ascontiguousarray(v)
     din = DeviceBuffer.from_array(v)

@@ -234,6 +234,34 @@ int fzh_periodic_window_f32(void* stream, const float* d_base, uint64_t bx, uint
                             uint64_t x0, uint64_t y0, uint64_t z0, float* d_out, uint64_t nx, uint64_t ny,
                             uint64_t nz);
 
+/* Per-kernel device time (no counterpart in the reference, whose
+ * benchmarks time whole stages).  While enabled, each launch of the chain's
+ * main kernels outside stream capture is bracketed by a HIP event pair on
+ * its own stream; fzh_kernel_timing_get waits for the recorded pairs and
+ * returns the summed milliseconds and launch count under `name`:
+ * "minmax_partial_kernel", "lz1d_fwd_kernel" / "lz2d_fwd_kernel" /
+ * "lz3d_fwd_kernel", "place_kernel", "scan_small_kernel", "rze_enc_kernel",
+ * "rze_pack_kernel", "rze_offsets_kernel", "rze_dec_kernel",
+ * "bin_count_kernel", "bin_place_kernel", "lz1d_inv_kernel" /
+ * "lz2d_inv_kernel" / "lz3d_inv_kernel", "apply_patches_kernel",
+ * "bitshuffle_kernel", "copy_kernel".  Enabling or disabling clears the
+ * totals.  The event records add a small gap between launches.            */
+int fzh_kernel_timing_enable(int on);
+int fzh_kernel_timing_get(const char* name, double* total_ms, uint64_t* count);
+
+/* Exclusive scan of n u32 values: the offsets primitive under every stage
+ * whose unit sizes are only known on the device (RZE chunk offsets, outlier
+ * compaction, outlier binning, Huffman cell offsets); the reference takes
+ * these offsets from CUB scans or host loops (rze_stage.cu:616-640).
+ * d_out may alias d_in; *d_total = the sum.  Sums saturate at 0xFFFFFFFF
+ * unless flags has FZH_SCAN_WRAP (modulo 2^32); FZH_SCAN_CLEAR zeroes d_in
+ * as it is read (d_in != d_out).  Workspace: fzh_scan_ws_bytes(n).        */
+#define FZH_SCAN_WRAP 1
+#define FZH_SCAN_CLEAR 2
+size_t fzh_scan_ws_bytes(uint64_t n);
+int fzh_scan_u32(void* stream, uint32_t* d_in, uint32_t* d_out, uint64_t n, uint32_t* d_total, int flags,
+                 void* d_ws, size_t ws_bytes);
+
 /* Plain device copy (grid-stride 16-byte accesses; 16-byte-aligned
  * pointers), capture-safe.  Also the bandwidth reference for benchmarks.   */
 int fzh_copy(void* stream, void* d_dst, const void* d_src, size_t bytes);
