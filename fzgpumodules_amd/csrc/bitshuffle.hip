@@ -190,13 +190,9 @@ int launch(hipStream_t s, const void* in, void* out, size_t nbytes, uint32_t bs,
     const unsigned grid = (unsigned)(nblk + (tail ? 1 : 0));
     const uint8_t* i8 = (const uint8_t*)in;
     uint8_t* o8 = (uint8_t*)out;
-    KernelTimer kt_(s, "bitshuffle_kernel");
-    switch (ew) {
-    case 1: bitshuffle_kernel<1, ENC><<<grid, 256, 0, s>>>(i8, o8, nbytes, bs, (uint32_t)nblk); break;
-    case 2: bitshuffle_kernel<2, ENC><<<grid, 256, 0, s>>>(i8, o8, nbytes, bs, (uint32_t)nblk); break;
-    case 4: bitshuffle_kernel<4, ENC><<<grid, 256, 0, s>>>(i8, o8, nbytes, bs, (uint32_t)nblk); break;
-    default: bitshuffle_kernel<8, ENC><<<grid, 256, 0, s>>>(i8, o8, nbytes, bs, (uint32_t)nblk); break;
-    }
+    auto* k = ew == 1 ? bitshuffle_kernel<1, ENC> : ew == 2 ? bitshuffle_kernel<2, ENC>
+            : ew == 4 ? bitshuffle_kernel<4, ENC> : bitshuffle_kernel<8, ENC>;
+    launch_k("bitshuffle_kernel", k, dim3(grid), dim3(256), 0, s, i8, o8, nbytes, bs, (uint32_t)nblk);
     FZH_CHECK_LAUNCH("bitshuffle_kernel");
     return FZH_OK;
 }

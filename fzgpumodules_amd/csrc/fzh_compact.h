@@ -80,9 +80,8 @@ inline int launch_compact(hipStream_t s, void* state, size_t T, uint32_t E, Comp
     void* ws = (uint8_t*)state + align_up(4 * (2 * T + 1), 256);
     if (int rc = scan_u32<true>(s, L0.cnt, excl, nl * T, nullptr, excl + nl * T, ws)) return rc;
     const unsigned blocks = (unsigned)cdiv(cdiv(T, 32), 4);
-    KernelTimer kt_(s, "place_kernel");
-    if (L1) place_kernel<2><<<blocks, 256, 0, s>>>(L0, *L1, (uint32_t)T, E, excl);
-    else place_kernel<1><<<blocks, 256, 0, s>>>(L0, CompactList{}, (uint32_t)T, E, excl);
+    launch_k("place_kernel", L1 ? place_kernel<2> : place_kernel<1>, dim3(blocks), dim3(256), 0, s, L0,
+             L1 ? *L1 : CompactList{}, (uint32_t)T, E, excl);
     FZH_CHECK_LAUNCH("compact");
     return FZH_OK;
 }

@@ -21,7 +21,7 @@ int fail(int code, const std::string& msg) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-kernel timing (opt-in; see KernelTimer)
+// Per-kernel timing (opt-in; see launch_k)
 // ---------------------------------------------------------------------------
 namespace {
 struct KtPending {
@@ -49,11 +49,6 @@ hipEvent_t kt_event() {
 int kt_drain() {
     int rc = FZH_OK;
     for (auto& p : g_kt_pending) {
-        if (p.name.empty()) {  // its stop event was never recorded
-            g_kt_pool.push_back(p.a);
-            g_kt_pool.push_back(p.b);
-            continue;
-        }
         float ms = 0.f;
         hipError_t e = hipEventSynchronize(p.b);
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.a, p.b);
@@ -72,26 +67,22 @@ int kt_drain() {
 }
 }  // namespace
 
-int ktime_begin(hipStream_t s, const char* name) {
-    if (!g_kt_on.load(std::memory_order_relaxed)) return -1;
+KtEvents ktime_events(hipStream_t s, const char* name) {
+    KtEvents e;
+    if (!g_kt_on.load(std::memory_order_relaxed)) return e;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return -1;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return e;
     std::lock_guard<std::mutex> g(g_kt_mu);
     hipEvent_t a = kt_event(), b = kt_event();
-    if (!a || !b || hipEventRecord(a, s) != hipSuccess) {
+    if (!a || !b) {
         if (a) g_kt_pool.push_back(a);
         if (b) g_kt_pool.push_back(b);
-        return -1;
+        return e;
     }
     g_kt_pending.push_back({name, a, b});
-    return (int)g_kt_pending.size() - 1;
-}
-
-void ktime_end(hipStream_t s, int slot) {
-    std::lock_guard<std::mutex> g(g_kt_mu);
-    if (slot >= 0 && slot < (int)g_kt_pending.size()) {
-        if (hipEventRecord(g_kt_pending[slot].b, s) != hipSuccess) g_kt_pending[slot].name.clear();
-    }
+    e.a = a;
+    e.b = b;
+    return e;
 }
 
 // Fill kernel: zeroes a byte range with 16-byte stores (no memset node, so it
@@ -270,8 +261,8 @@ int fzh_copy(void* stream, void* d_dst, const void* d_src, size_t bytes) {
     const size_t nw = bytes / 16;
     if (nw) {
         const size_t blocks = std::min<size_t>(cdiv(nw, 256 * 4), 8192);
-        KernelTimer kt_(S(stream), "copy_kernel");
-        copy_kernel<<<(unsigned)blocks, 256, 0, S(stream)>>>((const uint4*)d_src, (uint4*)d_dst, nw);
+        launch_k("copy_kernel", copy_kernel, dim3((unsigned)blocks), dim3(256), 0, S(stream), (const uint4*)d_src,
+                 (uint4*)d_dst, nw);
     }
     if (bytes % 16)
         copy_tail_kernel<<<1, 256, 0, S(stream)>>>((const uint8_t*)d_src, (uint8_t*)d_dst, nw * 16, bytes);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstddef>
 #include <cstdint>
@@ -21,20 +22,24 @@ int fail(int code, const std::string& msg);
 // Zeroes [p, p+n) with a fill kernel on stream s.
 int zero_bytes(hipStream_t s, void* p, size_t n);
 
-// Optional per-kernel device timing: while enabled (fzh_kernel_timing_enable)
-// and the stream is not being captured, a KernelTimer records a HIP event
-// pair on the launch stream around the one launch in its scope; the pairs
-// are resolved into per-name totals by fzh_kernel_timing_get.
-int ktime_begin(hipStream_t s, const char* name);  // slot, or -1 when off
-void ktime_end(hipStream_t s, int slot);
-struct KernelTimer {
-    hipStream_t s;
-    int slot;
-    KernelTimer(hipStream_t st, const char* name) : s(st), slot(ktime_begin(st, name)) {}
-    ~KernelTimer() {
-        if (slot >= 0) ktime_end(s, slot);
-    }
+// Optional per-kernel device timing.  launch_k launches like
+// kernel<<<grid, block, shm, s>>>(args...); while timing is on
+// (fzh_kernel_timing_enable) and the stream is not being captured, the
+// launch carries a start/stop event pair that HIP fills with the dispatch's
+// own begin/end timestamps, so the time is the kernel's execution alone.
+// The pairs are resolved into per-name totals by fzh_kernel_timing_get.
+struct KtEvents {
+    hipEvent_t a = nullptr, b = nullptr;
 };
+KtEvents ktime_events(hipStream_t s, const char* name);  // null events when off
+
+template <typename... P, typename... A>
+inline void launch_k(const char* name, void (*kernel)(P...), dim3 grid, dim3 block, uint32_t shm, hipStream_t s,
+                     A... args) {
+    static_assert(sizeof...(P) == sizeof...(A), "launch_k: argument count");
+    const KtEvents e = ktime_events(s, name);
+    hipExtLaunchKernelGGL(kernel, grid, block, shm, s, e.a, e.b, 0u, static_cast<P>(args)...);
+}
 
 #define FZH_TRY_HIP(expr)                                                        \
     do {                                                                         \
@@ -77,21 +82,41 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Inclusive scan of a u32 across the 64 lanes of a wave.
+// Lane movement inside the VALU with DPP (data-parallel primitives) instead
+// of the LDS crossbar (ds_bpermute, one LDS round trip per step).  Every
+// caller runs them with all 64 lanes active.  dpp_ctrl encodings (GFX9):
+// quad_perm 0x00-0xFF, row_shr:n 0x110+n, wave_shr:1 0x138, row_bcast:15
+// 0x142, row_bcast:31 0x143; lanes without a source keep `old` (0 here).
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Value of lane l - 1 (lane 0: 0).
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v) { return dpp_u32<0x138>(v); }
+__device__ __forceinline__ float wave_shr1(float v) { return __uint_as_float(dpp_u32<0x138>(__float_as_uint(v))); }
+// Value of lane l ^ 1.
+__device__ __forceinline__ uint32_t lane_xor1(uint32_t v) { return dpp_u32<0xB1>(v); }
+
+// Inclusive scan of a u32 across the 64 lanes of a wave: Hillis-Steele
+// inside each row of 16 lanes, then the row totals carried by broadcasts.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const int l = lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t t = __shfl_up(v, off, 64);
-        if (l >= off) v += t;
-    }
+    v += dpp_u32<0x111>(v);
+    v += dpp_u32<0x112>(v);
+    v += dpp_u32<0x114>(v);
+    v += dpp_u32<0x118>(v);
+    v += dpp_u32<0x142, 0xA>(v);
+    v += dpp_u32<0x143, 0xC>(v);
     return v;
 }
 
+// Lane 63's value, wave-uniform (the total of an inclusive scan).
+__device__ __forceinline__ uint32_t lane63(uint32_t v) { return (uint32_t)__builtin_amdgcn_readlane((int)v, 63); }
+__device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+
+// Wave-uniform sum and maximum.
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    return lane63(wave_incl_scan(v));
 }
 
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {

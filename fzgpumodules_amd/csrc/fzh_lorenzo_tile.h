@@ -124,8 +124,8 @@ __device__ __forceinline__ TileCounts lz1d_tile(const float* __restrict__ in, ui
     float d[4][4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        float prev = __shfl_up(s[q][3], 1, 64);
-        const float last_prev_q = __shfl(q > 0 ? s[q > 0 ? q - 1 : 0][3] : 0.0f, 63, 64);
+        float prev = wave_shr1(s[q][3]);
+        const float last_prev_q = lane63(q > 0 ? s[q > 0 ? q - 1 : 0][3] : 0.0f);
         if (l == 0) prev = (q > 0) ? last_prev_q : 0.0f;
         d[q][0] = s[q][0] - prev;
 #pragma unroll
@@ -182,7 +182,7 @@ __device__ __forceinline__ TileCounts lz1d_tile(const float* __restrict__ in, ui
                     }
                     pos++;
                 }
-            run += __shfl(incl, 63, 64);
+            run += lane63(incl);
         }
         return run;
     };
@@ -238,7 +238,7 @@ __device__ __forceinline__ TileCounts lz2d_tile(const float* __restrict__ in, ui
     uint64_t bal[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const float w = __shfl_up(t[i], 1, 32);
+        const float w = wave_shr1(t[i]);  // lane x - 1 (x > 0; lane 32's is unused)
         const float d = (x > 0) ? t[i] - w : t[i];
         const bool valid = xin && (Y0 + i) < ny;
         const bool ok = fabsf(d) < rad;
@@ -325,7 +325,7 @@ __device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, ui
     for (int y = 0; y < 8; y++) {
         const float sz = __shfl_up(s[y], 8, 64);
         const float a = s[y] - (z > 0 ? sz : 0.0f);
-        const float aw = __shfl_up(a, 1, 8);
+        const float aw = wave_shr1(a);  // lane x - 1 of the group (x > 0)
         b[y] = (x > 0) ? a - aw : a;
     }
     float d[8];

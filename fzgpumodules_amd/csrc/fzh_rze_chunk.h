@@ -141,7 +141,7 @@ __device__ __forceinline__ uint32_t rze_encode_chunk_wg(const uint8_t* __restric
         for (int k = 0; k < 4; k++) m |= (((d[p] >> (8 * k)) & 0xFFu) != 0u ? 1u : 0u) << k;
         nib |= (uint64_t)m << (4 * p);
         my_nz += __popc(m);
-        const uint32_t other = __shfl_xor(m, 1, 64);
+        const uint32_t other = lane_xor1(m);
         if ((t & 1) == 0) L.bm1[128 * p + (t >> 1)] = (uint8_t)(m | (other << 4));
     }
     uint32_t data_nz;
@@ -194,8 +194,7 @@ __device__ __forceinline__ uint32_t rze_encode_chunk_wg(const uint8_t* __restric
             // -- level 4: RE over bm3 (32 bytes: ballot bytes), in wave 0
             if (w == 0) {
                 const uint32_t cur = l < 32 ? (uint32_t)((L.bal[l >> 3] >> (8 * (l & 7))) & 0xFFu) : 0u;
-                uint32_t prev = __shfl_up(cur, 1, 64);
-                if (l == 0) prev = 0u;
+                const uint32_t prev = wave_shr1(cur);  // lane 0: 0
                 const uint32_t m3 = (uint32_t)__ballot(l < bm3_bytes && cur != prev);
                 if (l == 0) L.tot[0] = m3;
             }

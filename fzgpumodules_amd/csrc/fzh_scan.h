@@ -215,8 +215,8 @@ template <bool WRAP = false, bool CLEAR = false>
 inline int scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, size_t n, const uint32_t* d_n,
                     uint32_t* d_total, void* ws) {
     if (n <= SCAN_SMALL) {
-        KernelTimer kt_(s, "scan_small_kernel");
-        scan_small_kernel<WRAP, CLEAR><<<1, 1024, 0, s>>>(const_cast<uint32_t*>(in), out, (uint32_t)n, d_n, d_total);
+        launch_k("scan_small_kernel", scan_small_kernel<WRAP, CLEAR>, dim3(1), dim3(1024), 0, s,
+                 const_cast<uint32_t*>(in), out, (uint32_t)n, d_n, d_total);
         return FZH_OK;
     }
     const size_t nseg = std::max<size_t>(cdiv(n, SCAN_SEG), 1);

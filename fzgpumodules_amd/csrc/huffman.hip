@@ -139,7 +139,7 @@ huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint3
                 atomicOr(&cells[c + 1], code[k] << (64 - sh - L));
             }
         }
-        run += __shfl(incl, 63, 64);
+        run += lane63(incl);
     }
     const uint32_t nbit = run;
     const uint32_t ncell = (nbit + 31) / 32;

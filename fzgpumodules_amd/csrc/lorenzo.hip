@@ -551,24 +551,21 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
     TCode* c = (TCode*)codes;
     const int nd = ndim_of(ny, nz);
     const size_t nv = (nd == 1 && nx > 0 && nx < n) ? nx : n;
-#define FZH_LZ_FWD(ZZV)                                                                                     \
-    if (nd == 1)                                                                                            \
-        lz1d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(in, n, nv, r, d_r, rad, c, tl, (uint32_t)T); \
-    else if (nd == 2)                                                                                       \
-        lz2d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(in, (uint32_t)nx, (uint32_t)ny, n, r, d_r, \
-                                                                   rad, c, tl, (uint32_t)cdiv(nx, 32),     \
-                                                                   (uint32_t)T);                           \
-    else                                                                                                    \
-        lz3d_fwd_kernel<TCode, ZZV, STRICT><<<blocks, 256, 0, s>>>(                                         \
-            in, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, n, r, d_r, rad, c, tl,                            \
-            Tiles{(uint32_t)cdiv(nx, 8), (uint32_t)cdiv(ny, 8), 0}, (uint32_t)T);
-    {
-        KernelTimer kt_(s, nd == 1 ? "lz1d_fwd_kernel" : nd == 2 ? "lz2d_fwd_kernel" : "lz3d_fwd_kernel");
-        if (zz) {
-            FZH_LZ_FWD(true)
-        } else {
-            FZH_LZ_FWD(false)
-        }
+#define FZH_LZ_FWD(ZZV)                                                                                 \
+    if (nd == 1)                                                                                        \
+        launch_k("lz1d_fwd_kernel", lz1d_fwd_kernel<TCode, ZZV, STRICT>, dim3(blocks), dim3(256), 0, s, in, n, \
+                 nv, r, d_r, rad, c, tl, (uint32_t)T);                                                  \
+    else if (nd == 2)                                                                                   \
+        launch_k("lz2d_fwd_kernel", lz2d_fwd_kernel<TCode, ZZV, STRICT>, dim3(blocks), dim3(256), 0, s, in, \
+                 (uint32_t)nx, (uint32_t)ny, n, r, d_r, rad, c, tl, (uint32_t)cdiv(nx, 32), (uint32_t)T);  \
+    else                                                                                                \
+        launch_k("lz3d_fwd_kernel", lz3d_fwd_kernel<TCode, ZZV, STRICT>, dim3(blocks), dim3(256), 0, s, in, \
+                 (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, n, r, d_r, rad, c, tl,                       \
+                 Tiles{(uint32_t)cdiv(nx, 8), (uint32_t)cdiv(ny, 8), 0}, (uint32_t)T);
+    if (zz) {
+        FZH_LZ_FWD(true)
+    } else {
+        FZH_LZ_FWD(false)
     }
 #undef FZH_LZ_FWD
     FZH_CHECK_LAUNCH("lorenzo fwd");
@@ -625,34 +622,31 @@ int launch_inv(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny,
     const bool has_out = count != nullptr && max_out > 0;
     const unsigned ob = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(max_out, 256), 1024));
     if (has_out) {
-        KernelTimer kt_(s, "bin_count_kernel");
-        bin_count_kernel<<<ob, 256, 0, s>>>(idx, count, max_out, m, cnt, slot);
+        launch_k("bin_count_kernel", bin_count_kernel, dim3(ob), dim3(256), 0, s, idx, count, max_out, m, cnt, slot);
     }
     if (int rc = scan_u32<false, true>(s, cnt, off, L.T, nullptr, off + L.T, w + L.o_scan)) return rc;
     if (has_out) {
-        KernelTimer kt_(s, "bin_place_kernel");
-        bin_place_kernel<<<ob, 256, 0, s>>>(err, idx, count, max_out, m, off, slot, serr, sloc);
+        launch_k("bin_place_kernel", bin_place_kernel, dim3(ob), dim3(256), 0, s, err, idx, count, max_out, m, off,
+                 slot, serr, sloc);
     }
     FZH_CHECK_LAUNCH("lorenzo inv binning");
     const unsigned blocks = (unsigned)cdiv(L.T, 4);
     const TCode* c = (const TCode*)codes;
-#define FZH_LZ_INV(ZZV)                                                                                \
-    if (nd == 1)                                                                                       \
-        lz1d_inv_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(c, n, ebx2, off, serr, sloc, out, (uint32_t)L.T); \
-    else if (nd == 2)                                                                                  \
-        lz2d_inv_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(c, (uint32_t)nx, (uint32_t)ny, ebx2, off,   \
-                                                           serr, sloc, out, m.gx, (uint32_t)L.T);       \
-    else                                                                                               \
-        lz3d_inv_kernel<TCode, ZZV><<<blocks, 256, 0, s>>>(c, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, \
-                                                           ebx2, off, serr, sloc, out,                 \
-                                                           Tiles{m.gx, m.gy, 0}, (uint32_t)L.T);
-    {
-        KernelTimer kt_(s, nd == 1 ? "lz1d_inv_kernel" : nd == 2 ? "lz2d_inv_kernel" : "lz3d_inv_kernel");
-        if (zz) {
-            FZH_LZ_INV(true)
-        } else {
-            FZH_LZ_INV(false)
-        }
+#define FZH_LZ_INV(ZZV)                                                                                 \
+    if (nd == 1)                                                                                        \
+        launch_k("lz1d_inv_kernel", lz1d_inv_kernel<TCode, ZZV>, dim3(blocks), dim3(256), 0, s, c, n, ebx2, off, \
+                 serr, sloc, out, (uint32_t)L.T);                                                       \
+    else if (nd == 2)                                                                                   \
+        launch_k("lz2d_inv_kernel", lz2d_inv_kernel<TCode, ZZV>, dim3(blocks), dim3(256), 0, s, c,      \
+                 (uint32_t)nx, (uint32_t)ny, ebx2, off, serr, sloc, out, m.gx, (uint32_t)L.T);          \
+    else                                                                                                \
+        launch_k("lz3d_inv_kernel", lz3d_inv_kernel<TCode, ZZV>, dim3(blocks), dim3(256), 0, s, c,      \
+                 (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, ebx2, off, serr, sloc, out, Tiles{m.gx, m.gy, 0}, \
+                 (uint32_t)L.T);
+    if (zz) {
+        FZH_LZ_INV(true)
+    } else {
+        FZH_LZ_INV(false)
     }
 #undef FZH_LZ_INV
     FZH_CHECK_LAUNCH("lorenzo inv");
@@ -743,9 +737,8 @@ int fzh_lorenzo_apply_patches(void* stream, float* d_out, const float* d_patch_v
     if (!d_out || !d_patch_val || !d_patch_idx || !d_patch_count)
         return fail(FZH_E_INVALID, "lorenzo patches: null pointer");
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(max_patches, 256), 1024));
-    KernelTimer kt_(S(stream), "apply_patches_kernel");
-    apply_patches_kernel<<<blocks, 256, 0, S(stream)>>>(d_out, d_patch_val, d_patch_idx, d_patch_count, max_patches,
-                                                         n_out);
+    launch_k("apply_patches_kernel", apply_patches_kernel, dim3(blocks), dim3(256), 0, S(stream), d_out, d_patch_val,
+             d_patch_idx, d_patch_count, max_patches, n_out);
     FZH_CHECK_LAUNCH("apply_patches_kernel");
     return FZH_OK;
 }
@@ -794,10 +787,8 @@ int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode, floa
     // about 16 float4 per thread, at most two workgroups per CU
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256 * 64), 512));
     float2* partial = (float2*)((uint8_t*)d_ws + 256);
-    {
-        KernelTimer kt_(S(stream), "minmax_partial_kernel");
-        minmax_partial_kernel<<<blocks, 256, 0, S(stream)>>>(d_in, n, partial, zero_pad);
-    }
+    launch_k("minmax_partial_kernel", minmax_partial_kernel, dim3(blocks), dim3(256), 0, S(stream), d_in, n, partial,
+             zero_pad);
     minmax_final_kernel<<<1, 256, 0, S(stream)>>>(partial, blocks, mode, user_eb, d_params);
     FZH_CHECK_LAUNCH("minmax");
     return FZH_OK;

@@ -92,7 +92,7 @@ quant_fwd_kernel(const float* __restrict__ in, uint64_t n, float r, float thresh
                         sidx[base + pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
                         pos++;
                     }
-                run += __shfl(incl, 63, 64);
+                run += lane63(incl);
             }
         }
         if (l == 0) scnt[tile] = agg;

@@ -81,7 +81,7 @@ rle_enc_runs_kernel(const T* __restrict__ in, uint64_t n, uint32_t* __restrict__
 #pragma unroll
         for (int k = 0; k < E; k++)
             if (mask[j] & (1u << k)) dst[r++] = (uint32_t)(e0 + k);
-        run += __shfl(incl, 63, 64);
+        run += lane63(incl);
     }
     if (l == 0) scnt[tile] = run;
 }

@@ -397,22 +397,13 @@ int rze_enc_impl(hipStream_t s, const void* d_in, size_t nbytes, int sew, void* 
     uint8_t* scratch = w + L.o_scratch;
     const uint8_t* in = (const uint8_t*)d_in;
     uint8_t* out = (uint8_t*)d_out;
-    {
-        KernelTimer kt_(s, "rze_enc_kernel");
-        if (sew == 1)
-            rze_enc_kernel<1><<<nc, 256, 0, s>>>(in, nbytes, out, scratch, sizes, clean, nc, nfull);
-        else if (sew == 2)
-            rze_enc_kernel<2><<<nc, 256, 0, s>>>(in, nbytes, out, scratch, sizes, clean, nc, nfull);
-        else
-            rze_enc_kernel<0><<<nc, 256, 0, s>>>(in, nbytes, out, scratch, sizes, clean, nc, 0);
-    }
+    auto* ek = sew == 1 ? rze_enc_kernel<1> : sew == 2 ? rze_enc_kernel<2> : rze_enc_kernel<0>;
+    launch_k("rze_enc_kernel", ek, dim3(nc), dim3(256), 0, s, in, nbytes, out, scratch, sizes, clean, nc,
+             sew ? nfull : (uint64_t)0);
     FZH_CHECK_LAUNCH("rze_enc_kernel");
     if (int rc = scan_u32(s, clean, offs, nc, nullptr, offs + nc, w + L.o_scan)) return rc;
-    {
-        KernelTimer kt_(s, "rze_pack_kernel");
-        rze_pack_kernel<<<(unsigned)cdiv(nc, 4), 256, 0, s>>>(in, nbytes, scratch, sizes, offs, out, d_out_size,
-                                                              nc, sew ? nfull : 0);
-    }
+    launch_k("rze_pack_kernel", rze_pack_kernel, dim3((unsigned)cdiv(nc, 4)), dim3(256), 0, s, in, nbytes, scratch,
+             sizes, offs, out, d_out_size, nc, sew ? nfull : (uint64_t)0);
     FZH_CHECK_LAUNCH("rze_pack_kernel");
     return FZH_OK;
 }
@@ -430,19 +421,10 @@ int rze_dec_impl(hipStream_t s, const void* d_in, size_t in_bytes, void* d_out, 
     uint32_t* offs = (uint32_t*)d_ws;
     const uint8_t* in = (const uint8_t*)d_in;
     uint8_t* out = (uint8_t*)d_out;
-    {
-        KernelTimer kt_(s, "rze_offsets_kernel");
-        rze_offsets_kernel<<<1, 1024, 0, s>>>(in, in_bytes, offs, nc);
-    }
-    {
-        KernelTimer kt_(s, "rze_dec_kernel");
-        if (sew == 1)
-            rze_dec_kernel<1><<<nc, 256, 0, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, nfull);
-        else if (sew == 2)
-            rze_dec_kernel<2><<<nc, 256, 0, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, nfull);
-        else
-            rze_dec_kernel<0><<<nc, 256, 0, s>>>(in, in_bytes, out, orig_bytes, d_status, offs, nc, 0);
-    }
+    launch_k("rze_offsets_kernel", rze_offsets_kernel, dim3(1), dim3(1024), 0, s, in, in_bytes, offs, nc);
+    auto* dk = sew == 1 ? rze_dec_kernel<1> : sew == 2 ? rze_dec_kernel<2> : rze_dec_kernel<0>;
+    launch_k("rze_dec_kernel", dk, dim3(nc), dim3(256), 0, s, in, in_bytes, out, orig_bytes, d_status, offs, nc,
+             sew ? nfull : (uint64_t)0);
     FZH_CHECK_LAUNCH("rze_dec_kernel");
     return FZH_OK;
 }
