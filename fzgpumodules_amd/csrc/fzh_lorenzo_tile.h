@@ -221,13 +221,27 @@ __device__ __forceinline__ TileCounts lz2d_tile(const float* __restrict__ in, ui
     const uint32_t X = bx * 32 + x;
     const uint32_t Y0 = by * 32 + 16 * h;
     const bool xin = X < nx;
+    // interior tiles (wave-uniform) read with a running row pointer and no
+    // bounds tests; edge tiles test every row
+    const bool interior = bx * 32 + 32 <= nx && by * 32 + 32 <= ny;
+    const uint64_t o0 = (uint64_t)Y0 * nx + X;  // this lane's first element
     float s[16], xv[16];
+    if (interior) {
+        const float* p = in + o0;
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const uint32_t Y = Y0 + i;
-        xv[i] = (xin && Y < ny) ? in[(uint64_t)Y * nx + X] : 0.0f;
-        s[i] = quant_round(xv[i], r);
+        for (int i = 0; i < 16; i++) {
+            xv[i] = *p;
+            p += nx;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t Y = Y0 + i;
+            xv[i] = (xin && Y < ny) ? in[(uint64_t)Y * nx + X] : 0.0f;
+        }
     }
+#pragma unroll
+    for (int i = 0; i < 16; i++) s[i] = quant_round(xv[i], r);
     // t = s(y) - s(y-1) inside the tile (row 0 of the tile subtracts 0).
     const float north = __shfl(s[15], x, 64);  // row 15 of the tile, for h = 1
     float t[16];
@@ -236,13 +250,16 @@ __device__ __forceinline__ TileCounts lz2d_tile(const float* __restrict__ in, ui
     for (int i = 1; i < 16; i++) t[i] = s[i] - s[i - 1];
     uint32_t mask = 0;
     uint64_t bal[16];
+    uintptr_t cp = codes ? reinterpret_cast<uintptr_t>(codes + o0) : 0;  // running row pointer
+    const uintptr_t cstride = (uintptr_t)nx * sizeof(TCode);
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         const float w = wave_shr1(t[i]);  // lane x - 1 (x > 0; lane 32's is unused)
         const float d = (x > 0) ? t[i] - w : t[i];
-        const bool valid = xin && (Y0 + i) < ny;
+        const bool valid = interior || (xin && (Y0 + i) < ny);
         const bool ok = fabsf(d) < rad;
-        if (valid && codes) codes[(uint64_t)(Y0 + i) * nx + X] = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
+        if (valid && codes) *reinterpret_cast<TCode*>(cp) = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
+        cp += cstride;
         const bool is_out = valid && !ok;
         if (is_out) mask |= 1u << i;
         bal[i] = __ballot(is_out);

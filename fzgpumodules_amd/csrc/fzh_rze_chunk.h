@@ -64,7 +64,8 @@ __device__ __forceinline__ void unshuffled_store(const uint32_t (&d)[16], uint8_
 // ballots), level 4 runs in wave 0.
 // ---------------------------------------------------------------------------
 struct RzeWgLds {
-    uint8_t st[LDS_STREAM];   // assembled stream
+    uint8_t st[LDS_STREAM];   // assembled stream (dword aligned: offset 0)
+    uint32_t sel[16];         // v_perm selectors packing the nonzero bytes of a dword, by nibble
     uint8_t bm1[LDS_BM1];     // level-1 bitmap
     uint32_t wsum[4][16];     // per-wave partial sums (scratch for block scans)
     uint32_t tot[16];         // broadcast totals
@@ -92,6 +93,16 @@ __device__ __forceinline__ uint32_t rze_encode_chunk_wg(const uint8_t* __restric
     const int w = t >> 6;
     const int l = lane_id();
     const bool full = in_size == CS;
+    if (t < 16) {  // selector for nibble t: byte slot c <- c-th nonzero byte, zeros after
+        uint32_t sel = 0x0C0C0C0Cu;
+        int c = 0;
+        for (int k = 0; k < 4; k++)
+            if (t & (1 << k)) {
+                sel = (sel & ~(0xFFu << (8 * c))) | ((uint32_t)k << (8 * c));
+                c++;
+            }
+        L.sel[t] = sel;
+    }
     // -- load: dword d[p] = chunk bytes [1024 p + 4 t, +4)
     uint32_t d[16];
     if (SEW == 2 && shuffled) {
@@ -146,6 +157,8 @@ __device__ __forceinline__ uint32_t rze_encode_chunk_wg(const uint8_t* __restric
     }
     uint32_t data_nz;
     (void)wg_excl_scan(L, my_nz, 0, data_nz);  // barrier: bm1 complete
+    // zero the data part of the stream: the data bytes are ORed in below
+    for (uint32_t i = t; i < (data_nz + 3) / 4; i += 256) reinterpret_cast<uint32_t*>(L.st)[i] = 0u;
 
     const int avail = CS - 2;
     const int bm1_bytes = (in_size + 7) / 8;
@@ -228,22 +241,37 @@ __device__ __forceinline__ uint32_t rze_encode_chunk_wg(const uint8_t* __restric
             if (l == 63) L.wsum[w][p] = incl[p];
         }
         __syncthreads();
-        uint32_t run = 0;
+        // lane q < 16 of each wave: where this wave's bytes of step q start =
+        // bytes of all steps before q + bytes of step q in earlier waves
+        uint32_t qb = 0;
+        {
+            uint32_t before = 0, all = 0;
+            if (l < 16) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t v = L.wsum[j][l];
+                    before += j < w ? v : 0u;
+                    all += v;
+                }
+            }
+            qb = wave_incl_scan(all) - all + before;  // lanes 0..15 form one DPP row
+        }
 #pragma unroll
         for (int p = 0; p < 16; p++) {
-            uint32_t before = 0, all = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t v = L.wsum[j][p];
-                before += j < w ? v : 0u;
-                all += v;
-            }
             const uint32_t m = (uint32_t)((nib >> (4 * p)) & 0xFull);
-            uint32_t pos = run + before + incl[p] - __popc(m);
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (m & (1u << k)) L.st[pos++] = (uint8_t)((d[p] >> (8 * k)) & 0xFFu);
-            run += all;
+            const uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)qb, p);
+            const uint32_t pos = base + incl[p] - __popc(m);
+            if (m) {
+                // the dword's nonzero bytes packed to its low end, ORed into
+                // the zeroed stream at [pos, pos + popc(m)) with two aligned
+                // dword ORs (neighbouring threads' bytes share the boundary
+                // dwords, never a byte)
+                const uint32_t packed = __builtin_amdgcn_perm(0u, d[p], L.sel[m]);
+                const uint64_t sh = (uint64_t)packed << (8 * (pos & 3));
+                uint32_t* sw = reinterpret_cast<uint32_t*>(L.st) + (pos >> 2);
+                atomicOr(sw, (uint32_t)sh);
+                if ((uint32_t)(sh >> 32)) atomicOr(sw + 1, (uint32_t)(sh >> 32));
+            }
         }
         __syncthreads();  // wsum reused below
         // bm1 non-repeated bytes (thread t: bm1 bytes 8t..8t+7)

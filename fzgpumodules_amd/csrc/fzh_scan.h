@@ -60,13 +60,14 @@ __device__ __forceinline__ uint32_t sat_u32(uint64_t v, bool wrap) {
 }
 
 // Exclusive scan of in[0..n) into out (may alias in) by one 1024-thread
-// workgroup in passes of 8192 elements: each pass is read and written with
-// coalesced dword accesses through LDS, and thread t scans the pass's
-// elements [8t, 8t+8) in registers.  Returns the sum (64-bit).  CLEAR zeroes
-// in[] as it is read (in and out distinct).
+// workgroup in passes of 16384 elements (64 KiB of LDS; each pass costs one
+// dependent HBM round trip, so short arrays take one): each pass is read and
+// written with coalesced dword accesses through LDS, and thread t scans the
+// pass's elements [16t, 16t+16) in registers.  Returns the sum (64-bit).
+// CLEAR zeroes in[] as it is read (in and out distinct).
 template <bool CLEAR>
 __device__ __forceinline__ uint64_t block_scan_1024(uint32_t* in, uint32_t* out, uint32_t n, bool wrap) {
-    constexpr uint32_t PER = 8, PASS = 1024 * PER;
+    constexpr uint32_t PER = 16, PASS = 1024 * PER;
     __shared__ __align__(16) uint32_t s_v[PASS];
     __shared__ uint64_t s_w[16];
     const int tid = threadIdx.x, w = tid >> 6, l = lane_id();
@@ -84,9 +85,15 @@ __device__ __forceinline__ uint64_t block_scan_1024(uint32_t* in, uint32_t* out,
             s_v[i] = v;
         }
         __syncthreads();
-        const uint4 x0 = *reinterpret_cast<const uint4*>(s_v + PER * tid);
-        const uint4 x1 = *reinterpret_cast<const uint4*>(s_v + PER * tid + 4);
-        const uint32_t x[PER] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+        uint32_t x[PER];
+#pragma unroll
+        for (uint32_t q = 0; q < PER / 4; q++) {
+            const uint4 u = *reinterpret_cast<const uint4*>(s_v + PER * tid + 4 * q);
+            x[4 * q] = u.x;
+            x[4 * q + 1] = u.y;
+            x[4 * q + 2] = u.z;
+            x[4 * q + 3] = u.w;
+        }
         uint64_t sum = 0;
 #pragma unroll
         for (uint32_t k = 0; k < PER; k++) sum += x[k];
@@ -100,14 +107,16 @@ __device__ __forceinline__ uint64_t block_scan_1024(uint32_t* in, uint32_t* out,
             run += j < w ? t : 0ull;
             tot += t;
         }
-        uint32_t y[PER];
 #pragma unroll
-        for (uint32_t k = 0; k < PER; k++) {
-            y[k] = sat_u32(run, wrap);
-            run += x[k];
+        for (uint32_t q = 0; q < PER / 4; q++) {
+            uint32_t y[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++) {
+                y[k] = sat_u32(run, wrap);
+                run += x[4 * q + k];
+            }
+            *reinterpret_cast<uint4*>(s_v + PER * tid + 4 * q) = make_uint4(y[0], y[1], y[2], y[3]);
         }
-        *reinterpret_cast<uint4*>(s_v + PER * tid) = make_uint4(y[0], y[1], y[2], y[3]);
-        *reinterpret_cast<uint4*>(s_v + PER * tid + 4) = make_uint4(y[4], y[5], y[6], y[7]);
         carry += tot;
         __syncthreads();
 #pragma unroll

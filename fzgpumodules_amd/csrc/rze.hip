@@ -135,6 +135,7 @@ struct RzeDecLds {
     uint8_t bm3[32];
     uint32_t hsum[4][3];   // per-wave sums of the bitmap stages
     uint32_t wsum[4][16];  // per-wave sums of the 16 data steps
+    uint32_t xsel[16];     // v_perm selectors spreading packed bytes over a dword, by nibble
 };
 
 __device__ __forceinline__ uint32_t dec_block_sum(RzeDecLds& L, uint32_t v, int slot) {
@@ -186,6 +187,13 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
             else for (uint32_t k = i; k < o; k++) dst[k] = 0;
         }
         return;
+    }
+    if (t < 16) {  // selector for nibble t: byte k <- packed byte (rank of k) if bit k set, else 0
+        uint32_t sel = 0x0C0C0C0Cu;
+        int c = 0;
+        for (int k = 0; k < 4; k++)
+            if (t & (1 << k)) sel = (sel & ~(0xFFu << (8 * k))) | ((uint32_t)(c++) << (8 * k));
+        L.xsel[t] = sel;
     }
     // -- stage the chunk's bytes into LDS (dword aligned start)
     const uint64_t a0 = src & ~3ull;
@@ -304,28 +312,28 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
         if (l == 63) L.wsum[w][p] = incl[p];
     }
     __syncthreads();
-    uint32_t run = 0;
+    // lane q < 16 of each wave: first stored byte of this wave's step q
+    uint32_t qb = 0;
+    {
+        uint32_t before = 0, all = 0;
+        if (l < 16) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t v = L.wsum[j][l];
+                before += j < w ? v : 0u;
+                all += v;
+            }
+        }
+        qb = wave_incl_scan(all) - all + before;  // lanes 0..15 form one DPP row
+    }
 #pragma unroll
     for (int p = 0; p < 16; p++) {
-        uint32_t before = 0, all = 0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t v = L.wsum[j][p];
-            before += j < w ? v : 0u;
-            all += v;
-        }
         const uint32_t m = (L.bm1[128 * p + (t >> 1)] >> (4 * (t & 1))) & 0xFu;
-        uint32_t r = run + before + incl[p] - __popc(m);
-        uint32_t x = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-            if (m & (1u << k)) {
-                const uint32_t bv = ((int)r < bm1_start) ? cs[r] : 0u;
-                x |= bv << (8 * k);
-                r++;
-            }
-        d[p] = x;
-        run += all;
+        const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)qb, p) + incl[p] - __popc(m);
+        // the popc(m) stored bytes at r (one unaligned dword from the staged
+        // stream, inside its slack) spread to the set positions of m; r stays
+        // below data_nz <= bm1_start unless the stream is malformed (flagged)
+        d[p] = m ? __builtin_amdgcn_perm(0u, staged_dword(L.stw, r + shift), L.xsel[m]) : 0u;
     }
     if (unshuf) {
         unshuffled_store<SE>(d, dst);

@@ -8,7 +8,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [0, 1, 7, 1023, 1024, 1025, 8191, 8192, 8193, 12656, 65535, 65536, 65537, 100003, (1 << 20) + 5]
+SIZES = [0, 1, 7, 1023, 1024, 1025, 8191, 8192, 8193, 12656, 16383, 16384, 16385, 32769, 65535, 65536, 65537,
+         100003, (1 << 20) + 5]
 
 
 def expected(v, wrap):
