@@ -35,6 +35,8 @@ SIGNATURES = {
     "fzh_lorenzo_apply_patches": (_I, [_P, _P, _P, _P, _P, _Z, _Z]),
     "fzh_lorenzo_quant_inv_ws_bytes": (_Z, [_Z, _Z, _Z, _Z, _Z]),
     "fzh_lorenzo_quant_inv_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _F, _I, _I, _P, _P, _Z]),
+    "fzh_lorenzo_quant_inv_bin": (_I, [_P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _P, _Z]),
+    "fzh_lorenzo_quant_inv_apply_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _F, _I, _I, _P, _P, _Z]),
     "fzh_value_base_ws_bytes": (_Z, [_Z]),
     "fzh_value_base_f32": (_I, [_P, _P, _Z, _I, _F, _P, _P, _Z]),
     "fzh_bitshuffle_enc": (_I, [_P, _P, _P, _Z, _U32, _I]),
@@ -57,6 +59,7 @@ SIGNATURES = {
     "fzh_huffman_enc_bound": (_Z, [_Z, _U32, _I]),
     "fzh_huffman_enc_ws_bytes": (_Z, [_Z]),
     "fzh_huffman_encode": (_I, [_P, _P, _Z, _I, _U32, _P, _P, _P, _Z, _P, _P, _Z]),
+    "fzh_huffman_encode_fine": (_I, [_P, _P, _Z, _I, _U32, _P, _P, _P, _Z, _P, _P, _Z]),
     "fzh_huffman_decode": (_I, [_P, _P, _Z, _I, _P, _Z, _P]),
     "fzh_quant_abs_fwd_f32": (_I, [_P, _P, _Z, _F, _F, _U32, _I, _I, _P, _P, _P, _P, _Z, _P, _Z]),
     "fzh_quant_abs_fwd_ws_bytes": (_Z, [_Z]),
@@ -135,6 +138,19 @@ SIGNATURES = {
     "fzs_last_timing": (_I, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_F)]),
     "fzh_kernel_timing_enable": (_I, [C.c_int]),
     "fzh_kernel_timing_get": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "fzp_add_difference": (_I, [_P, _I, _I, _Z, C.POINTER(_I)]),
+    "fzp_set_huffman_fine": (_I, [_P, _I, _I]),
+    "fzp_add_zigzag": (_I, [_P, _I, C.POINTER(_I)]),
+    "fzp_add_negabinary": (_I, [_P, _I, C.POINTER(_I)]),
+    "fzp_add_lorenzo_int": (_I, [_P, _I, C.POINTER(_I)]),
+    "fzh_zigzag": (_I, [_P, _P, _P, _Z, _I, _I]),
+    "fzh_negabinary": (_I, [_P, _P, _P, _Z, _I, _I]),
+    "fzh_diff_fwd": (_I, [_P, _P, _P, _Z, _I, _Z, _I]),
+    "fzh_diff_inv_ws_bytes": (_Z, [_Z, _I, _Z]),
+    "fzh_diff_inv": (_I, [_P, _P, _P, _Z, _I, _Z, _I, _P, _Z]),
+    "fzh_lorenzo_int_fwd": (_I, [_P, _P, _P, _Z, _Z, _Z, _I]),
+    "fzh_lorenzo_int_inv_ws_bytes": (_Z, [_Z, _Z, _Z]),
+    "fzh_lorenzo_int_inv": (_I, [_P, _P, _P, _Z, _Z, _Z, _I, _P, _Z]),
     "fzh_scan_ws_bytes": (C.c_size_t, [C.c_uint64]),
     "fzh_scan_u32": (_I, [_P, _P, _P, C.c_uint64, _P, C.c_int, _P, C.c_size_t]),
     "fzh_periodic_window_f32": (_I, [_P, _P] + [C.c_uint64] * 6 + [_P] + [C.c_uint64] * 3),

@@ -53,6 +53,8 @@ void need(const void* p) {
 
 }  // namespace
 
+static int add_code(fzp_pipeline* p, int eb, bool nb, int* id);
+
 extern "C" {
 
 int fzp_create(size_t hint, float mult, fzp_pipeline** out) {
@@ -204,6 +206,68 @@ int fzp_add_bitpack(fzp_pipeline* p, int eb, int nbits, int* id) {
         default: throw std::invalid_argument("Bitpack elem_bytes must be 1, 2 or 4");
         }
         add(p, s, id);
+    });
+}
+
+int fzp_add_difference(fzp_pipeline* p, int dtype, int negabinary, size_t chunk_bytes, int* id) {
+    return guard([&] {
+        need(p);
+        fz::Stage* s = nullptr;
+        auto mk = [&](auto* d) {
+            d->setChunkSize(chunk_bytes);
+            s = d;
+        };
+        if (negabinary) {
+            switch (dtype) {
+            case 4: mk(new fz::DifferenceStage<int8_t, uint8_t>()); break;
+            case 5: mk(new fz::DifferenceStage<int16_t, uint16_t>()); break;
+            case 6: mk(new fz::DifferenceStage<int32_t, uint32_t>()); break;
+            case 7: mk(new fz::DifferenceStage<int64_t, uint64_t>()); break;
+            default: throw std::invalid_argument("Difference with negabinary needs a signed integer type");
+            }
+        } else {
+            switch (dtype) {
+            case 0: mk(new fz::DifferenceStage<uint8_t>()); break;
+            case 1: mk(new fz::DifferenceStage<uint16_t>()); break;
+            case 2: mk(new fz::DifferenceStage<uint32_t>()); break;
+            case 6: mk(new fz::DifferenceStage<int32_t>()); break;
+            case 7: mk(new fz::DifferenceStage<int64_t>()); break;
+            case 8: mk(new fz::DifferenceStage<float>()); break;
+            case 9: mk(new fz::DifferenceStage<double>()); break;
+            default: throw std::invalid_argument("Difference: unsupported data type");
+            }
+        }
+        add(p, s, id);
+    });
+}
+
+int fzp_add_zigzag(fzp_pipeline* p, int eb, int* id) { return add_code(p, eb, false, id); }
+int fzp_add_negabinary(fzp_pipeline* p, int eb, int* id) { return add_code(p, eb, true, id); }
+
+int fzp_add_lorenzo_int(fzp_pipeline* p, int eb, int* id) {
+    return guard([&] {
+        need(p);
+        fz::Stage* s;
+        switch (eb) {
+        case 1: s = new fz::LorenzoStage<int8_t>(); break;
+        case 2: s = new fz::LorenzoStage<int16_t>(); break;
+        case 4: s = new fz::LorenzoStage<int32_t>(); break;
+        case 8: s = new fz::LorenzoStage<int64_t>(); break;
+        default: throw std::invalid_argument("elem_bytes must be 1, 2, 4 or 8");
+        }
+        add(p, s, id);
+    });
+}
+
+int fzp_set_huffman_fine(fzp_pipeline* p, int sid, int fine) {
+    return guard([&] {
+        need(p);
+        fz::Stage* s = stage_at(p, sid);
+        const auto m = fine ? fz::HuffmanEncodeMode::Fine : fz::HuffmanEncodeMode::Coarse;
+        if (auto* h = dynamic_cast<fz::HuffmanStage<uint8_t>*>(s)) h->setEncodeMode(m);
+        else if (auto* h2 = dynamic_cast<fz::HuffmanStage<uint16_t>*>(s)) h2->setEncodeMode(m);
+        else if (auto* h4 = dynamic_cast<fz::HuffmanStage<uint32_t>*>(s)) h4->setEncodeMode(m);
+        else throw std::invalid_argument("stage is not a Huffman stage");
     });
 }
 
@@ -368,3 +432,18 @@ int fzp_stage_abs_error_bound(fzp_pipeline* p, int id, float* abs_eb) {
 }
 
 }  // extern "C"
+
+static int add_code(fzp_pipeline* p, int eb, bool nb, int* id) {
+    return guard([&] {
+        need(p);
+        fz::Stage* s;
+        switch (eb) {
+        case 1: s = nb ? (fz::Stage*)new fz::NegabinaryStage<int8_t, uint8_t>() : new fz::ZigzagStage<int8_t, uint8_t>(); break;
+        case 2: s = nb ? (fz::Stage*)new fz::NegabinaryStage<int16_t, uint16_t>() : new fz::ZigzagStage<int16_t, uint16_t>(); break;
+        case 4: s = nb ? (fz::Stage*)new fz::NegabinaryStage<int32_t, uint32_t>() : new fz::ZigzagStage<int32_t, uint32_t>(); break;
+        case 8: s = nb ? (fz::Stage*)new fz::NegabinaryStage<int64_t, uint64_t>() : new fz::ZigzagStage<int64_t, uint64_t>(); break;
+        default: throw std::invalid_argument("elem_bytes must be 1, 2, 4 or 8");
+        }
+        add(p, s, id);
+    });
+}

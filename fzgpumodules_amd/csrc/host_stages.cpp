@@ -469,10 +469,15 @@ void HuffmanStage<T>::execute(hipStream_t stream, MemoryPool* pool, const std::v
         book_.ensure(book.size(), pool, stream);
         FZ_HIP(hipMemcpyAsync(book_.ptr, book.data(), book.size(), hipMemcpyHostToDevice, stream));
         ws_.ensure(fzh_huffman_enc_ws_bytes(n), pool, stream);
-        throwIfFzh(fzh_huffman_encode(stream, inputs[0], n, (int)sizeof(T), bklen_, (const uint32_t*)book_.ptr,
-                                      (const uint8_t*)book_.ptr + 4 * (size_t)bklen_, outputs[0],
-                                      fzh_huffman_enc_bound(n, bklen_, sizeof(T)), (uint64_t*)small_.ptr, ws_.ptr,
-                                      ws_.bytes),
+        // Fine mode: 1024-symbol partitions when no codeword exceeds 8 bits
+        // (hf_hl.cc:69-80), else the coarse stream
+        uint32_t max_cl = 0;
+        for (uint32_t i = 0; i < bklen_; i++) max_cl = std::max(max_cl, ((const uint32_t*)book.data())[i] >> 27);
+        const bool fine = encode_mode_ == HuffmanEncodeMode::Fine && max_cl <= 8;
+        throwIfFzh((fine ? fzh_huffman_encode_fine : fzh_huffman_encode)(
+                       stream, inputs[0], n, (int)sizeof(T), bklen_, (const uint32_t*)book_.ptr,
+                       (const uint8_t*)book_.ptr + 4 * (size_t)bklen_, outputs[0],
+                       fzh_huffman_enc_bound(n, bklen_, sizeof(T)), (uint64_t*)small_.ptr, ws_.ptr, ws_.bytes),
                    "HuffmanStage encode");
         // keep the host copy alive until the copy has been consumed
         FZ_HIP(hipStreamSynchronize(stream));
@@ -726,6 +731,47 @@ Stage* createStage(StageType type, const uint8_t* config, size_t config_size) {
         if (dt == DataType::UINT8) s = new BitpackStage<uint8_t>();
         else if (dt == DataType::UINT32) s = new BitpackStage<uint32_t>();
         else s = new BitpackStage<uint16_t>();
+        break;
+    }
+    case StageType::DIFFERENCE: {
+        // [0] T, [1] TOut, [2..5] chunk size; a 1-byte legacy header names T only
+        const DataType ti = config_size >= 1 ? (DataType)config[0] : DataType::FLOAT32;
+        const DataType to = config_size >= 2 ? (DataType)config[1] : ti;
+        if (ti == DataType::INT8 && to == DataType::UINT8) s = new DifferenceStage<int8_t, uint8_t>();
+        else if (ti == DataType::INT16 && to == DataType::UINT16) s = new DifferenceStage<int16_t, uint16_t>();
+        else if (ti == DataType::INT32 && to == DataType::UINT32) s = new DifferenceStage<int32_t, uint32_t>();
+        else if (ti == DataType::INT64 && to == DataType::UINT64) s = new DifferenceStage<int64_t, uint64_t>();
+        else if (ti == DataType::FLOAT32) s = new DifferenceStage<float>();
+        else if (ti == DataType::FLOAT64) s = new DifferenceStage<double>();
+        else if (ti == DataType::UINT8) s = new DifferenceStage<uint8_t>();
+        else if (ti == DataType::UINT16) s = new DifferenceStage<uint16_t>();
+        else if (ti == DataType::UINT32) s = new DifferenceStage<uint32_t>();
+        else if (ti == DataType::INT32) s = new DifferenceStage<int32_t>();
+        else if (ti == DataType::INT64) s = new DifferenceStage<int64_t>();
+        else throw std::runtime_error("Unsupported Difference data type: " + std::to_string((int)ti));
+        break;
+    }
+    case StageType::ZIGZAG:
+    case StageType::NEGABINARY: {
+        const bool nb = type == StageType::NEGABINARY;
+        const DataType ti = config_size >= 2 ? (DataType)config[0] : DataType::INT32;
+        const DataType to = config_size >= 2 ? (DataType)config[1] : DataType::UINT32;
+        if (ti == DataType::INT8 && to == DataType::UINT8) s = nb ? (Stage*)new NegabinaryStage<int8_t, uint8_t>() : new ZigzagStage<int8_t, uint8_t>();
+        else if (ti == DataType::INT16 && to == DataType::UINT16) s = nb ? (Stage*)new NegabinaryStage<int16_t, uint16_t>() : new ZigzagStage<int16_t, uint16_t>();
+        else if (ti == DataType::INT32 && to == DataType::UINT32) s = nb ? (Stage*)new NegabinaryStage<int32_t, uint32_t>() : new ZigzagStage<int32_t, uint32_t>();
+        else if (ti == DataType::INT64 && to == DataType::UINT64) s = nb ? (Stage*)new NegabinaryStage<int64_t, uint64_t>() : new ZigzagStage<int64_t, uint64_t>();
+        else throw std::runtime_error(std::string(nb ? "Unsupported NegabinaryStage" : "Unsupported ZigzagStage") +
+                                      " type pair: TIn=" + std::to_string((int)ti) + " TOut=" + std::to_string((int)to));
+        break;
+    }
+    case StageType::LORENZO: {
+        const DataType dt = config_size >= sizeof(LorenzoConfig) ? (DataType)config[0] : DataType::INT32;
+        if (dt == DataType::INT8) s = new LorenzoStage<int8_t>();
+        else if (dt == DataType::INT16) s = new LorenzoStage<int16_t>();
+        else if (dt == DataType::INT32) s = new LorenzoStage<int32_t>();
+        else if (dt == DataType::INT64) s = new LorenzoStage<int64_t>();
+        else throw std::runtime_error("Unsupported LorenzoStage DataType: " + std::to_string((int)dt));
+        if (config_size < sizeof(LorenzoConfig)) return s;  // no header: default dims
         break;
     }
     default:

@@ -5,9 +5,11 @@
 //   [par_entry u32 x pardeg][bitstream u32 cells]
 // phf_header: bklen (int:16) @0, sublen @4, pardeg @8, original_len u64 @16,
 // total_nbit u64 @24, total_ncell u64 @32, entry[6] u32 @40 (entry[5] = total
-// stream bytes).  Partitions of sublen = 768 symbols; each partition's
-// codewords are packed MSB-first into 32-bit cells (hf_kernels.cu:73-122);
-// par_entry = exclusive prefix of the per-partition cell counts.
+// stream bytes).  Partitions of sublen = 768 symbols (coarse,
+// hf_bk.cc:28-31) or 1024 (fine, the ReVISIT-lite path, hf_buf.cc:48-49);
+// each partition's codewords are packed MSB-first into 32-bit cells
+// (hf_kernels.cu:73-122, :145-250 build the same cells); par_entry =
+// exclusive prefix of the per-partition cell counts.
 //
 // MI355X design
 //   histogram: privatized per-wave LDS histograms, consecutive equal symbols
@@ -27,7 +29,8 @@
 namespace fzh {
 namespace {
 
-constexpr int SUBLEN = 768;
+constexpr int SUBLEN = 768;       // coarse partitions
+constexpr int SUBLEN_FINE = 1024; // fine partitions
 constexpr int HDR_BYTES = 128;
 constexpr int LUT_BITS = 10;
 
@@ -90,13 +93,16 @@ hist_global_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, uint32_
 // ---------------------------------------------------------------------------
 // Encode
 // ---------------------------------------------------------------------------
-constexpr int CELLS = (SUBLEN * 28 + 31) / 32 + 2;  // cells per partition slot
+// cells per partition slot (every codeword is at most 28 bits)
+constexpr int cells_of(int sl) { return (sl * 28 + 31) / 32 + 2; }
 
-template <typename T>
+template <typename T, int SL>
 __global__ void __launch_bounds__(256)
 huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint32_t* __restrict__ bk4,
                const uint8_t* __restrict__ revbook, uint32_t rev_bytes, uint8_t* __restrict__ out,
                uint32_t* __restrict__ scells, uint32_t* __restrict__ ncell_out, uint32_t pardeg, int book_in_lds) {
+    constexpr int CELLS = cells_of(SL);
+    constexpr int K = SL / 64;  // symbols per lane
     extern __shared__ uint32_t smem[];
     const int wv = threadIdx.x >> 6;
     uint32_t* cells = smem + wv * CELLS;
@@ -110,13 +116,13 @@ huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint3
     const uint32_t p = blockIdx.x * 4 + wv;
     const int l = lane_id();
     if (p >= pardeg) return;
-    const uint64_t s0 = (uint64_t)p * SUBLEN;
-    const uint32_t cnt = (uint32_t)((n - s0) < (uint64_t)SUBLEN ? (n - s0) : SUBLEN);
+    const uint64_t s0 = (uint64_t)p * SL;
+    const uint32_t cnt = (uint32_t)((n - s0) < (uint64_t)SL ? (n - s0) : SL);
     for (int i = l; i < CELLS; i += 64) cells[i] = 0u;
     // symbol s0 + l + 64k, bit offsets in (k, lane) order = symbol order
-    uint32_t code[12], len[12];
+    uint32_t code[K], len[K];
 #pragma unroll
-    for (int k = 0; k < 12; k++) {
+    for (int k = 0; k < K; k++) {
         const uint32_t j = l + 64 * k;
         uint32_t w = 0;
         if (j < cnt) w = book[ld_sym(in, s0 + j)];
@@ -126,7 +132,7 @@ huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint3
     wave_lds_sync();
     uint32_t run = 0;
 #pragma unroll
-    for (int k = 0; k < 12; k++) {
+    for (int k = 0; k < K; k++) {
         const uint32_t incl = wave_incl_scan(len[k]);
         const uint32_t b = run + incl - len[k];
         const uint32_t L = len[k];
@@ -159,20 +165,20 @@ huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint3
 // Cells of partition p -> stream cell par_entry[p] (one wave per partition).
 __global__ void __launch_bounds__(256)
 huf_pack_kernel(const uint32_t* __restrict__ scells, const uint32_t* __restrict__ ncell,
-                uint8_t* __restrict__ out, uint32_t ent3, uint32_t ent4, uint32_t pardeg) {
+                uint8_t* __restrict__ out, uint32_t ent3, uint32_t ent4, uint32_t pardeg, uint32_t cells) {
     const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= pardeg) return;
     const int l = lane_id();
     const uint32_t e = reinterpret_cast<const uint32_t*>(out + ent3)[p];
     const uint32_t c = ncell[p];
-    const uint32_t* src = scells + (uint64_t)p * CELLS;
+    const uint32_t* src = scells + (uint64_t)p * cells;
     uint32_t* dst = reinterpret_cast<uint32_t*>(out + ent4) + e;
     for (uint32_t i = l; i < c; i += 64) dst[i] = src[i];
 }
 
 // phf_header (hf.h:38-44) once every partition is placed.
 __global__ void __launch_bounds__(1024)
-huf_header_kernel(uint8_t* __restrict__ out, uint64_t n, uint32_t bklen, uint32_t ent1, uint32_t ent2,
+huf_header_kernel(uint8_t* __restrict__ out, uint64_t n, uint32_t bklen, uint32_t sublen, uint32_t ent1, uint32_t ent2,
                   uint32_t ent3, uint32_t ent4, uint32_t pardeg, const uint32_t* __restrict__ total_ncell,
                   uint64_t* __restrict__ out_size) {
     __shared__ unsigned long long s_w[16];
@@ -190,7 +196,7 @@ huf_header_kernel(uint8_t* __restrict__ out, uint64_t n, uint32_t bklen, uint32_
         const uint32_t ent5 = ent4 + 4 * tc;
         uint32_t* h32 = reinterpret_cast<uint32_t*>(out);
         h32[0] = bklen & 0xFFFFu;
-        h32[1] = SUBLEN;
+        h32[1] = sublen;
         h32[2] = pardeg;
         h32[3] = 0;
         reinterpret_cast<uint64_t*>(out)[2] = n;
@@ -338,25 +344,26 @@ huf_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, T* __restrict_
 struct EncWs {
     size_t pardeg, o_ncell, o_tot, o_scan, o_cells, total;
 };
-EncWs enc_layout(size_t n) {
+EncWs enc_layout(size_t n, int sl) {
     EncWs w;
-    w.pardeg = cdiv(std::max<size_t>(n, 1), SUBLEN);
+    w.pardeg = cdiv(std::max<size_t>(n, 1), (size_t)sl);
     size_t o = 0;
     w.o_ncell = o; o = align_up(o + 4 * w.pardeg, 256);
     w.o_tot = o;   o = align_up(o + 16, 256);
     w.o_scan = o;  o = align_up(o + scan_ws_bytes(w.pardeg), 256);
-    w.o_cells = o; o = align_up(o + 4 * (size_t)CELLS * w.pardeg, 256);
+    w.o_cells = o; o = align_up(o + 4 * (size_t)cells_of(sl) * w.pardeg, 256);
     w.total = o;
     return w;
 }
 
-template <typename T>
+template <typename T, int SL>
 int enc_impl(hipStream_t s, const void* in, size_t n, uint32_t bklen, const uint32_t* bk4, const uint8_t* rev,
              void* out, size_t cap, uint64_t* out_size, void* ws, size_t ws_bytes) {
-    const EncWs L = enc_layout(n);
+    constexpr int CELLS = cells_of(SL);
+    const EncWs L = enc_layout(n, SL);
     if (ws_bytes < L.total) return fail(FZH_E_CAPACITY, "huffman enc: workspace too small");
     if (n == 0) return fail(FZH_E_INVALID, "huffman enc: empty input");
-    if (n > 0x7FFFFFFFull * SUBLEN) return fail(FZH_E_INVALID, "huffman enc: input too large");
+    if (n > 0x7FFFFFFFull * SL) return fail(FZH_E_INVALID, "huffman enc: input too large");
     if (cap < fzh_huffman_enc_bound(n, bklen, sizeof(T))) return fail(FZH_E_CAPACITY, "huffman enc: output below bound");
     const uint32_t rev_bytes = (uint32_t)fzh_huffman_revbook_bytes(bklen, sizeof(T));
     if (rev_bytes % 4) return fail(FZH_E_INVALID, "huffman enc: bklen * sizeof(T) must be a multiple of 4");
@@ -371,14 +378,27 @@ int enc_impl(hipStream_t s, const void* in, size_t n, uint32_t bklen, const uint
     const int book_in_lds = bklen <= 8192 ? 1 : 0;
     const size_t lds = (size_t)4 * CELLS * 4 + (book_in_lds ? 4 * (size_t)bklen : 0);
     const unsigned blocks = (unsigned)cdiv(pardeg, 4);
-    huf_enc_kernel<T><<<blocks, 256, lds, s>>>((const T*)in, n, bklen, bk4, rev, rev_bytes, o8, scells, ncell, pardeg,
-                                               book_in_lds);
+    huf_enc_kernel<T, SL><<<blocks, 256, lds, s>>>((const T*)in, n, bklen, bk4, rev, rev_bytes, o8, scells, ncell,
+                                                   pardeg, book_in_lds);
     if (int rc = scan_u32(s, ncell, reinterpret_cast<uint32_t*>(o8 + ent3), pardeg, nullptr, tot, w + L.o_scan))
         return rc;
-    huf_pack_kernel<<<blocks, 256, 0, s>>>(scells, ncell, o8, ent3, ent4, pardeg);
-    huf_header_kernel<<<1, 1024, 0, s>>>(o8, n, bklen, ent1, ent2, ent3, ent4, pardeg, tot, out_size);
+    huf_pack_kernel<<<blocks, 256, 0, s>>>(scells, ncell, o8, ent3, ent4, pardeg, (uint32_t)CELLS);
+    huf_header_kernel<<<1, 1024, 0, s>>>(o8, n, bklen, (uint32_t)SL, ent1, ent2, ent3, ent4, pardeg, tot, out_size);
     FZH_CHECK_LAUNCH("huffman encode");
     return FZH_OK;
+}
+
+template <int SL>
+int encode_sl(void* stream, const void* d_in, size_t n, int elem_bytes, uint32_t bklen, const uint32_t* d_bk4,
+                     const uint8_t* d_revbook, void* d_out, size_t out_cap, uint64_t* d_out_size, void* d_ws,
+                     size_t ws_bytes) {
+    if (ws_bytes < std::max(enc_layout(n, SUBLEN).total, enc_layout(n, SUBLEN_FINE).total)) return fail(FZH_E_CAPACITY, "huffman enc: workspace too small");
+    switch (elem_bytes) {
+    case 1: return enc_impl<uint8_t, SL>(S(stream), d_in, n, bklen, d_bk4, d_revbook, d_out, out_cap, d_out_size, d_ws, ws_bytes);
+    case 2: return enc_impl<uint16_t, SL>(S(stream), d_in, n, bklen, d_bk4, d_revbook, d_out, out_cap, d_out_size, d_ws, ws_bytes);
+    case 4: return enc_impl<uint32_t, SL>(S(stream), d_in, n, bklen, d_bk4, d_revbook, d_out, out_cap, d_out_size, d_ws, ws_bytes);
+    default: return fail(FZH_E_INVALID, "huffman enc: elem_bytes must be 1, 2 or 4");
+    }
 }
 
 }  // namespace
@@ -418,18 +438,22 @@ size_t fzh_huffman_enc_bound(size_t n, uint32_t bklen, int elem_bytes) {
     return HDR_BYTES + fzh_huffman_revbook_bytes(bklen, elem_bytes) + 8 * pardeg + 4 * cdiv(n * 28, 32) + 4 * pardeg + 64;
 }
 
-size_t fzh_huffman_enc_ws_bytes(size_t n) { return enc_layout(n).total; }
+size_t fzh_huffman_enc_ws_bytes(size_t n) {
+    return std::max(enc_layout(n, SUBLEN).total, enc_layout(n, SUBLEN_FINE).total);
+}
 
 int fzh_huffman_encode(void* stream, const void* d_in, size_t n, int elem_bytes, uint32_t bklen,
                        const uint32_t* d_bk4, const uint8_t* d_revbook, void* d_out, size_t out_cap,
                        uint64_t* d_out_size, void* d_ws, size_t ws_bytes) {
-    if (ws_bytes < fzh_huffman_enc_ws_bytes(n)) return fail(FZH_E_CAPACITY, "huffman enc: workspace too small");
-    switch (elem_bytes) {
-    case 1: return enc_impl<uint8_t>(S(stream), d_in, n, bklen, d_bk4, d_revbook, d_out, out_cap, d_out_size, d_ws, ws_bytes);
-    case 2: return enc_impl<uint16_t>(S(stream), d_in, n, bklen, d_bk4, d_revbook, d_out, out_cap, d_out_size, d_ws, ws_bytes);
-    case 4: return enc_impl<uint32_t>(S(stream), d_in, n, bklen, d_bk4, d_revbook, d_out, out_cap, d_out_size, d_ws, ws_bytes);
-    default: return fail(FZH_E_INVALID, "huffman enc: elem_bytes must be 1, 2 or 4");
-    }
+    return encode_sl<SUBLEN>(stream, d_in, n, elem_bytes, bklen, d_bk4, d_revbook, d_out, out_cap, d_out_size, d_ws,
+                             ws_bytes);
+}
+
+int fzh_huffman_encode_fine(void* stream, const void* d_in, size_t n, int elem_bytes, uint32_t bklen,
+                            const uint32_t* d_bk4, const uint8_t* d_revbook, void* d_out, size_t out_cap,
+                            uint64_t* d_out_size, void* d_ws, size_t ws_bytes) {
+    return encode_sl<SUBLEN_FINE>(stream, d_in, n, elem_bytes, bklen, d_bk4, d_revbook, d_out, out_cap, d_out_size,
+                                  d_ws, ws_bytes);
 }
 
 int fzh_huffman_decode(void* stream, const void* d_in, size_t in_bytes, int elem_bytes, void* d_out, size_t n,

@@ -265,12 +265,61 @@ bin_place_kernel(const float* __restrict__ err, const uint32_t* __restrict__ idx
         }                                                                    \
     } while (0)
 
+// With few outliers (at most kDirectOutliers, known on the host from the
+// outlier array's size) every tile's wave reads the whole list itself and
+// applies the entries that fall in its tile: no binning launches.
+constexpr uint32_t kDirectOutliers = 256;
+
+struct DirectOut {
+    const float* err;
+    const uint32_t* idx;
+    const uint32_t* count;
+    uint32_t max_out;
+    TileMap m;
+};
+
+#define FZH_APPLY_DIRECT(NREG, DVAR, LANE_OF, REG_OF)                                         \
+    do {                                                                                      \
+        const uint32_t kk = dout.count ? min(*dout.count, dout.max_out) : 0u;                 \
+        for (uint32_t j0 = 0; j0 < kk; j0 += 64) {                                            \
+            const uint32_t jj = j0 + (uint32_t)l;                                             \
+            uint32_t t_ = 0xFFFFFFFFu, loc_ = 0;                                              \
+            float e_ = 0.0f;                                                                  \
+            if (jj < kk) {                                                                    \
+                if (tile_of(dout.m, dout.idx[jj], t_, loc_)) e_ = dout.err[jj];               \
+                else t_ = 0xFFFFFFFFu;                                                        \
+            }                                                                                 \
+            uint64_t hit = __ballot(t_ == tile);                                              \
+            while (hit) {                                                                     \
+                const int src_ = __builtin_ctzll(hit);                                        \
+                hit &= hit - 1;                                                               \
+                const uint32_t loc = (uint32_t)__shfl((int)loc_, src_, 64);                   \
+                const float e = __shfl(e_, src_, 64);                                         \
+                const int tl = (LANE_OF);                                                     \
+                const int tr = (REG_OF);                                                      \
+                if (l == tl) {                                                                \
+                    _Pragma("unroll") for (int rr = 0; rr < NREG; rr++)                       \
+                        if (rr == tr) DVAR[rr] = e + DVAR[rr];                                \
+                }                                                                             \
+            }                                                                                 \
+        }                                                                                     \
+    } while (0)
+
+#define FZH_APPLY_ANY(NREG, DVAR, LANE_OF, REG_OF)                 \
+    do {                                                           \
+        if constexpr (DIRECT) {                                    \
+            FZH_APPLY_DIRECT(NREG, DVAR, LANE_OF, REG_OF);         \
+        } else {                                                   \
+            FZH_APPLY_OUTLIERS(NREG, DVAR, LANE_OF, REG_OF);       \
+        }                                                          \
+    } while (0)
+
 // 1D inverse, scan tree of lorenzo_quant.cu:17-87 (see header comment).
-template <typename TCode, bool ZZ>
+template <typename TCode, bool ZZ, bool DIRECT>
 __global__ void __launch_bounds__(256)
 lz1d_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, const uint32_t* __restrict__ off,
                 const float* __restrict__ serr, const uint32_t* __restrict__ sloc, float* __restrict__ out,
-                uint32_t T) {
+                uint32_t T, DirectOut dout) {
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const int l = lane_id();
@@ -298,7 +347,7 @@ lz1d_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, const u
         }
     }
     // local index li -> reference thread t = li/4 = l + 64q, k = li%4
-    FZH_APPLY_OUTLIERS(16, d, (int)((loc >> 2) & 63u), (int)(((loc >> 8) << 2) | (loc & 3u)));
+    FZH_APPLY_ANY(16, d, (int)((loc >> 2) & 63u), (int)(((loc >> 8) << 2) | (loc & 3u)));
     // per reference thread: sequential prefix of 4
 #pragma unroll
     for (int q = 0; q < 4; q++)
@@ -359,11 +408,12 @@ lz1d_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, const u
 }
 
 // 2D inverse, scan tree of lorenzo_quant_nd.cu:112-190.
-template <typename TCode, bool ZZ>
+template <typename TCode, bool ZZ, bool DIRECT>
 __global__ void __launch_bounds__(256)
 lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float ebx2,
                 const uint32_t* __restrict__ off, const float* __restrict__ serr,
-                const uint32_t* __restrict__ sloc, float* __restrict__ out, uint32_t gx, uint32_t T) {
+                const uint32_t* __restrict__ sloc, float* __restrict__ out, uint32_t gx, uint32_t T,
+                DirectOut dout) {
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const int l = lane_id();
@@ -376,7 +426,7 @@ lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float
     for (int i = 0; i < 16; i++)
         d[i] = (xin && Y0 + i < ny) ? dec_code<TCode, ZZ>(codes[(uint64_t)(Y0 + i) * nx + X]) : 0.0f;
     // local = (ty << 5) | tx ; lane = tx + 32*(ty >= 16), register = ty & 15
-    FZH_APPLY_OUTLIERS(16, d, (int)((loc & 31u) | ((loc >> 4) & 32u)), (int)((loc >> 5) & 15u));
+    FZH_APPLY_ANY(16, d, (int)((loc & 31u) | ((loc >> 4) & 32u)), (int)((loc >> 5) & 15u));
     // y: sequential inside each 8-row group
 #pragma unroll
     for (int i = 1; i < 8; i++) d[i] += d[i - 1];
@@ -414,11 +464,12 @@ lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float
 
 // 3D inverse, scan tree of lorenzo_quant_nd.cu:281-364 (y sequential, x
 // Hillis-Steele over 8, z Hillis-Steele over 8).
-template <typename TCode, bool ZZ>
+template <typename TCode, bool ZZ, bool DIRECT>
 __global__ void __launch_bounds__(256)
 lz3d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, uint32_t nz, float ebx2,
                 const uint32_t* __restrict__ off, const float* __restrict__ serr,
-                const uint32_t* __restrict__ sloc, float* __restrict__ out, Tiles tg, uint32_t T) {
+                const uint32_t* __restrict__ sloc, float* __restrict__ out, Tiles tg, uint32_t T,
+                DirectOut dout) {
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (tile >= T) return;
     const int l = lane_id();
@@ -432,7 +483,7 @@ lz3d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, uint3
     for (int y = 0; y < 8; y++)
         d[y] = (xz && Y0 + y < ny) ? dec_code<TCode, ZZ>(codes[Z * plane + (uint64_t)(Y0 + y) * nx + X]) : 0.0f;
     // local = (z << 6) | (y << 3) | x ; lane = x + 8z, register = y
-    FZH_APPLY_OUTLIERS(8, d, (int)((loc & 7u) | ((loc >> 3) & 56u)), (int)((loc >> 3) & 7u));
+    FZH_APPLY_ANY(8, d, (int)((loc & 7u) | ((loc >> 3) & 56u)), (int)((loc >> 3) & 7u));
 #pragma unroll
     for (int y = 1; y < 8; y++) d[y] += d[y - 1];
 #pragma unroll
@@ -598,10 +649,21 @@ InvWs inv_layout(size_t n, size_t nx, size_t ny, size_t nz, size_t max_out) {
     return w;
 }
 
-template <typename TCode>
-int launch_inv(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny, size_t nz,
-               const float* err, const uint32_t* idx, const uint32_t* count, size_t max_out,
-               float ebx2, int zz, float* out, void* ws, size_t ws_bytes) {
+TileMap tile_map(size_t n, size_t nx, size_t ny, size_t nz) {
+    const int nd = ndim_of(ny, nz);
+    TileMap m;
+    m.ndim = nd;
+    m.nx = (uint32_t)nx; m.ny = (uint32_t)ny; m.nz = (uint32_t)nz;
+    m.gx = (uint32_t)(nd == 2 ? cdiv(nx, 32) : cdiv(nx, 8));
+    m.gy = (uint32_t)cdiv(ny, 8);
+    m.total = nd == 3 ? (uint64_t)nx * ny * nz : nd == 2 ? (uint64_t)nx * ny : n;
+    return m;
+}
+
+// Inverse, part 1: the outliers binned by tile into the workspace (needs
+// only the outlier arrays, so it may run before the codes exist).
+int inv_bin(hipStream_t s, size_t n, size_t nx, size_t ny, size_t nz, const float* err, const uint32_t* idx,
+            const uint32_t* count, size_t max_out, void* ws, size_t ws_bytes) {
     const InvWs L = inv_layout(n, nx, ny, nz, max_out);
     if (ws_bytes < L.total) return fail(FZH_E_CAPACITY, "lorenzo inv: workspace too small");
     uint8_t* w = (uint8_t*)ws;
@@ -610,15 +672,8 @@ int launch_inv(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny,
     uint32_t* slot = (uint32_t*)(w + L.o_slot);
     float* serr = (float*)(w + L.o_err);
     uint32_t* sloc = (uint32_t*)(w + L.o_loc);
-
-    const int nd = ndim_of(ny, nz);
-    TileMap m;
-    m.ndim = nd;
-    m.nx = (uint32_t)nx; m.ny = (uint32_t)ny; m.nz = (uint32_t)nz;
-    m.gx = (uint32_t)(nd == 2 ? cdiv(nx, 32) : cdiv(nx, 8));
-    m.gy = (uint32_t)cdiv(ny, 8);
-    m.total = nd == 3 ? (uint64_t)nx * ny * nz : nd == 2 ? (uint64_t)nx * ny : n;
-
+    if (max_out <= kDirectOutliers) return FZH_OK;  // the inverse reads the list itself
+    const TileMap m = tile_map(n, nx, ny, nz);
     const bool has_out = count != nullptr && max_out > 0;
     const unsigned ob = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(max_out, 256), 1024));
     if (has_out) {
@@ -630,23 +685,41 @@ int launch_inv(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny,
                  slot, serr, sloc);
     }
     FZH_CHECK_LAUNCH("lorenzo inv binning");
+    return FZH_OK;
+}
+
+// Inverse, part 2: the tiles, with the binned outliers of part 1.
+template <typename TCode>
+int inv_apply(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny, size_t nz, const float* err,
+              const uint32_t* idx, const uint32_t* count, size_t max_out, float ebx2, int zz, float* out, void* ws,
+              size_t ws_bytes) {
+    const InvWs L = inv_layout(n, nx, ny, nz, max_out);
+    if (ws_bytes < L.total) return fail(FZH_E_CAPACITY, "lorenzo inv: workspace too small");
+    uint8_t* w = (uint8_t*)ws;
+    const uint32_t* off = (const uint32_t*)(w + L.o_off);
+    const float* serr = (const float*)(w + L.o_err);
+    const uint32_t* sloc = (const uint32_t*)(w + L.o_loc);
+    const int nd = ndim_of(ny, nz);
+    const TileMap m = tile_map(n, nx, ny, nz);
     const unsigned blocks = (unsigned)cdiv(L.T, 4);
     const TCode* c = (const TCode*)codes;
-#define FZH_LZ_INV(ZZV)                                                                                 \
+    const bool direct = max_out <= kDirectOutliers;
+    const DirectOut dout{err, idx, (err && idx) ? count : nullptr, (uint32_t)max_out, m};
+#define FZH_LZ_INV(ZZV, DV)                                                                             \
     if (nd == 1)                                                                                        \
-        launch_k("lz1d_inv_kernel", lz1d_inv_kernel<TCode, ZZV>, dim3(blocks), dim3(256), 0, s, c, n, ebx2, off, \
-                 serr, sloc, out, (uint32_t)L.T);                                                       \
+        launch_k("lz1d_inv_kernel", lz1d_inv_kernel<TCode, ZZV, DV>, dim3(blocks), dim3(256), 0, s, c, n, ebx2, \
+                 off, serr, sloc, out, (uint32_t)L.T, dout);                                            \
     else if (nd == 2)                                                                                   \
-        launch_k("lz2d_inv_kernel", lz2d_inv_kernel<TCode, ZZV>, dim3(blocks), dim3(256), 0, s, c,      \
-                 (uint32_t)nx, (uint32_t)ny, ebx2, off, serr, sloc, out, m.gx, (uint32_t)L.T);          \
+        launch_k("lz2d_inv_kernel", lz2d_inv_kernel<TCode, ZZV, DV>, dim3(blocks), dim3(256), 0, s, c,  \
+                 (uint32_t)nx, (uint32_t)ny, ebx2, off, serr, sloc, out, m.gx, (uint32_t)L.T, dout);    \
     else                                                                                                \
-        launch_k("lz3d_inv_kernel", lz3d_inv_kernel<TCode, ZZV>, dim3(blocks), dim3(256), 0, s, c,      \
+        launch_k("lz3d_inv_kernel", lz3d_inv_kernel<TCode, ZZV, DV>, dim3(blocks), dim3(256), 0, s, c,  \
                  (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, ebx2, off, serr, sloc, out, Tiles{m.gx, m.gy, 0}, \
-                 (uint32_t)L.T);
+                 (uint32_t)L.T, dout);
     if (zz) {
-        FZH_LZ_INV(true)
+        if (direct) { FZH_LZ_INV(true, true) } else { FZH_LZ_INV(true, false) }
     } else {
-        FZH_LZ_INV(false)
+        if (direct) { FZH_LZ_INV(false, true) } else { FZH_LZ_INV(false, false) }
     }
 #undef FZH_LZ_INV
     FZH_CHECK_LAUNCH("lorenzo inv");
@@ -747,28 +820,55 @@ size_t fzh_lorenzo_quant_inv_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz,
     return inv_layout(n, nx, ny, nz, max_outliers).total;
 }
 
-int fzh_lorenzo_quant_inv_f32(void* stream, const void* d_codes, size_t n, size_t nx, size_t ny, size_t nz,
-                              const float* d_outlier_err, const uint32_t* d_outlier_idx,
-                              const uint32_t* d_outlier_count, size_t max_outliers, float ebx2, int zigzag,
-                              int code_bytes, float* d_out, void* d_ws, size_t ws_bytes) {
+int fzh_lorenzo_quant_inv_bin(void* stream, size_t n, size_t nx, size_t ny, size_t nz, const float* d_outlier_err,
+                              const uint32_t* d_outlier_idx, const uint32_t* d_outlier_count, size_t max_outliers,
+                              void* d_ws, size_t ws_bytes) {
+    if (int rc = check_dims(n, nx, ny, nz, "lorenzo inv")) return rc;
+    if (n == 0) return FZH_OK;
+    if (max_outliers > 0 && d_outlier_count && (!d_outlier_err || !d_outlier_idx))
+        return fail(FZH_E_INVALID, "lorenzo inv: null outlier buffers");
+    return inv_bin(S(stream), n, nx, ny, nz, d_outlier_err, d_outlier_idx, d_outlier_count, max_outliers, d_ws,
+                   ws_bytes);
+}
+
+int fzh_lorenzo_quant_inv_apply_f32(void* stream, const void* d_codes, size_t n, size_t nx, size_t ny, size_t nz,
+                                    const float* d_outlier_err, const uint32_t* d_outlier_idx,
+                                    const uint32_t* d_outlier_count, size_t max_outliers, float ebx2, int zigzag,
+                                    int code_bytes, float* d_out, void* d_ws, size_t ws_bytes) {
     if (int rc = check_dims(n, nx, ny, nz, "lorenzo inv")) return rc;
     if (n == 0) return FZH_OK;
     if (!d_codes || !d_out) return fail(FZH_E_INVALID, "lorenzo inv: null pointer");
     if (ndim_of(ny, nz) == 1 &&
         (reinterpret_cast<uintptr_t>(d_out) % 16 != 0 || reinterpret_cast<uintptr_t>(d_codes) % 8 != 0))
         return fail(FZH_E_INVALID, "lorenzo inv: 1D buffers must be 16/8-byte aligned");
-    if (max_outliers > 0 && d_outlier_count && (!d_outlier_err || !d_outlier_idx))
-        return fail(FZH_E_INVALID, "lorenzo inv: null outlier buffers");
     switch (code_bytes) {
     case 1:
-        return launch_inv<uint8_t>(S(stream), d_codes, n, nx, ny, nz, d_outlier_err, d_outlier_idx, d_outlier_count,
-                                   max_outliers, ebx2, zigzag, d_out, d_ws, ws_bytes);
+        return inv_apply<uint8_t>(S(stream), d_codes, n, nx, ny, nz, d_outlier_err, d_outlier_idx, d_outlier_count,
+                                  max_outliers, ebx2, zigzag, d_out, d_ws, ws_bytes);
     case 2:
-        return launch_inv<uint16_t>(S(stream), d_codes, n, nx, ny, nz, d_outlier_err, d_outlier_idx,
-                                    d_outlier_count, max_outliers, ebx2, zigzag, d_out, d_ws, ws_bytes);
+        return inv_apply<uint16_t>(S(stream), d_codes, n, nx, ny, nz, d_outlier_err, d_outlier_idx,
+                                   d_outlier_count, max_outliers, ebx2, zigzag, d_out, d_ws, ws_bytes);
     default:
         return fail(FZH_E_INVALID, "lorenzo inv: code_bytes must be 1 or 2");
     }
+}
+
+int fzh_lorenzo_quant_inv_f32(void* stream, const void* d_codes, size_t n, size_t nx, size_t ny, size_t nz,
+                              const float* d_outlier_err, const uint32_t* d_outlier_idx,
+                              const uint32_t* d_outlier_count, size_t max_outliers, float ebx2, int zigzag,
+                              int code_bytes, float* d_out, void* d_ws, size_t ws_bytes) {
+    if (code_bytes != 1 && code_bytes != 2) return fail(FZH_E_INVALID, "lorenzo inv: code_bytes must be 1 or 2");
+    if (!d_codes || !d_out) {
+        if (int rc = check_dims(n, nx, ny, nz, "lorenzo inv")) return rc;
+        if (n == 0) return FZH_OK;
+        return fail(FZH_E_INVALID, "lorenzo inv: null pointer");
+    }
+    if (int rc = fzh_lorenzo_quant_inv_bin(stream, n, nx, ny, nz, d_outlier_err, d_outlier_idx, d_outlier_count,
+                                           max_outliers, d_ws, ws_bytes))
+        return rc;
+    return fzh_lorenzo_quant_inv_apply_f32(stream, d_codes, n, nx, ny, nz, d_outlier_err, d_outlier_idx,
+                                           d_outlier_count, max_outliers, ebx2, zigzag, code_bytes, d_out, d_ws,
+                                           ws_bytes);
 }
 
 size_t fzh_value_base_ws_bytes(size_t n) {

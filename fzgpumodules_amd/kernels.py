@@ -148,8 +148,10 @@ def huffman_book(freq, elem_bytes=2):
     return bk4, rev
 
 
-def huffman_encode(v, bklen):
-    """Histogram on the GPU, codebook on the host, encode on the GPU."""
+def huffman_encode(v, bklen, fine=False):
+    """Histogram on the GPU, codebook on the host, encode on the GPU (fine:
+    1024-symbol partitions when no codeword exceeds 8 bits, as HuffmanStage's
+    Fine mode)."""
     v = np.ascontiguousarray(v)
     eb = v.dtype.itemsize
     freq = histogram(v, bklen)
@@ -163,8 +165,8 @@ def huffman_encode(v, bklen):
     dout = DeviceBuffer(cap)
     dsz = DeviceBuffer(8)
     ws = workspace(lib().fzh_huffman_enc_ws_bytes(v.size))
-    call("fzh_huffman_encode", None, din.ptr, v.size, eb, bklen, dbk.ptr, drev.ptr, dout.ptr, cap, dsz.ptr,
-         ws.ptr, ws.nbytes)
+    fn = "fzh_huffman_encode_fine" if fine and int((bk4 >> 27).max()) <= 8 else "fzh_huffman_encode"
+    call(fn, None, din.ptr, v.size, eb, bklen, dbk.ptr, drev.ptr, dout.ptr, cap, dsz.ptr, ws.ptr, ws.nbytes)
     return dout.download(np.uint8, int(dsz.download(np.uint64, 1)[0]))
 
 

@@ -17,6 +17,19 @@ ABS, REL, NOA = 0, 1, 2
 _MODES = {"ABS": ABS, "REL": REL, "NOA": NOA}
 
 
+_DTYPES = {"u8": 0, "u16": 1, "u32": 2, "u64": 3, "i8": 4, "i16": 5, "i32": 6, "i64": 7, "f32": 8, "f64": 9}
+
+
+def dtype_code(d) -> int:
+    """FZM DataType code of a type name ('u8' .. 'f64') or numpy dtype."""
+    if isinstance(d, int):
+        return d
+    if isinstance(d, str) and d in _DTYPES:
+        return _DTYPES[d]
+    k = np.dtype(d)
+    return _DTYPES[{"b": "i", "i": "i", "u": "u", "f": "f"}[k.kind] + str(8 * k.itemsize)]
+
+
 def _mode(m) -> int:
     return _MODES[m.upper()] if isinstance(m, str) else int(m)
 
@@ -86,6 +99,21 @@ class Pipeline:
 
     def add_bitpack(self, elem_bytes=2, nbits=0) -> int:
         return self._id("fzp_add_bitpack", int(elem_bytes), int(nbits))
+
+    def set_huffman_fine(self, stage_id: int, on: bool = True) -> None:
+        call("fzp_set_huffman_fine", self.h, int(stage_id), int(bool(on)))
+
+    def add_difference(self, dtype="f32", negabinary=False, chunk_bytes=0) -> int:
+        return self._id("fzp_add_difference", dtype_code(dtype), int(bool(negabinary)), int(chunk_bytes))
+
+    def add_zigzag(self, elem_bytes=4) -> int:
+        return self._id("fzp_add_zigzag", int(elem_bytes))
+
+    def add_negabinary(self, elem_bytes=4) -> int:
+        return self._id("fzp_add_negabinary", int(elem_bytes))
+
+    def add_lorenzo_int(self, elem_bytes=4) -> int:
+        return self._id("fzp_add_lorenzo_int", int(elem_bytes))
 
     def connect(self, dependent: int, producer: int, output_name: str | None = None) -> None:
         call("fzp_connect", self.h, dependent, producer, output_name.encode() if output_name else None)

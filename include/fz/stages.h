@@ -9,6 +9,10 @@
 //   HuffmanStage        modules/coders/huffman/huffman_stage.h:61-256
 //   QuantizerStage      modules/quantizers/quantizer/quantizer.h:25-260 (ABS/NOA)
 //   BitpackStage        modules/coders/bitpack/bitpack_stage.h (nbits / auto)
+//   DifferenceStage     modules/predictors/diff/diff.h:16-200
+//   ZigzagStage         modules/transforms/zigzag/zigzag_stage.h
+//   NegabinaryStage     modules/transforms/negabinary/negabinary_stage.h
+//   LorenzoStage        modules/predictors/lorenzo/lorenzo_stage.h (integer)
 // Differences in behaviour are confined to where the reference is
 // nondeterministic or waits on the host: outlier lists come out in a fixed
 // order, REL/NOA error bounds and all inverse paths resolve on the device
@@ -398,8 +402,9 @@ class HuffmanStage : public Stage {
   public:
     void setBklen(uint32_t b) { bklen_ = b; }
     uint32_t getBklen() const { return bklen_; }
-    // Both modes emit the coarse stream here (768-symbol partitions); the
-    // encode is device-side either way.
+    // Coarse: 768-symbol partitions.  Fine: 1024-symbol partitions (the
+    // ReVISIT-lite layout) when no codeword exceeds 8 bits, else coarse
+    // (hf_hl.cc:69-80).  The encode is device-side either way.
     void setEncodeMode(HuffmanEncodeMode m) { encode_mode_ = m; }
     HuffmanEncodeMode getEncodeMode() const { return encode_mode_; }
     void setInverse(bool inv) override { is_inverse_ = inv; }
@@ -630,6 +635,187 @@ class BitpackStage : public Stage {
     uint8_t nbits_ = 8 * sizeof(T);
     size_t num_elements_ = 0, actual_output_size_ = 0;
     struct Saved { uint8_t nb; size_t n; size_t out; } saved_{8 * sizeof(T), 0, 0};
+    DeviceScratch ws_;
+};
+
+// ---------------------------------------------------------------------------
+// Difference (modules/predictors/diff/diff.h:16-200): first-order
+// differences, optionally reset at every chunk_size-byte boundary, optionally
+// negabinary-coded (T signed, TOut its unsigned counterpart).  Inverse: the
+// cumulative sum; floating-point sums in the reference's association.
+// ---------------------------------------------------------------------------
+template <typename T = float, typename TOut = T>
+class DifferenceStage : public Stage {
+    static_assert(std::is_same_v<T, TOut> ||
+                      (std::is_integral_v<T> && std::is_signed_v<T> && std::is_integral_v<TOut> &&
+                       std::is_unsigned_v<TOut> && sizeof(T) == sizeof(TOut)),
+                  "DifferenceStage: TOut must equal T, or be the unsigned counterpart of a signed T");
+
+  public:
+    void setInverse(bool inv) override { is_inverse_ = inv; }
+    bool isInverse() const override { return is_inverse_; }
+    void setChunkSize(size_t bytes) { chunk_size_ = bytes; }
+    size_t getChunkSize() const { return chunk_size_; }
+    size_t getRequiredInputAlignment() const override { return chunk_size_ > 0 ? chunk_size_ : 1; }
+    void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                 const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
+    std::string getName() const override { return "Difference"; }
+    size_t getNumInputs() const override { return 1; }
+    size_t getNumOutputs() const override { return 1; }
+    std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& in) const override { return {in[0]}; }
+    std::unordered_map<std::string, size_t> getActualOutputSizesByName() const override {
+        return {{"output", actual_output_size_}};
+    }
+    size_t getActualOutputSize(int i) const override { return i == 0 ? actual_output_size_ : 0; }
+    uint16_t getStageTypeId() const override { return (uint16_t)StageType::DIFFERENCE; }
+    uint8_t getOutputDataType(size_t) const override { return (uint8_t)dataTypeOf<TOut>(); }
+    uint8_t getInputDataType(size_t) const override { return (uint8_t)dataTypeOf<T>(); }
+    // [0] DataType T, [1] DataType TOut, [2..5] chunk_size u32 (0 = none)
+    size_t serializeHeader(size_t, uint8_t* buf, size_t max_size) const override {
+        if (max_size < 6) return 0;
+        buf[0] = (uint8_t)dataTypeOf<T>();
+        buf[1] = (uint8_t)dataTypeOf<TOut>();
+        const uint32_t cs = (uint32_t)chunk_size_;
+        std::memcpy(buf + 2, &cs, 4);
+        return 6;
+    }
+    void deserializeHeader(const uint8_t* buf, size_t size) override {
+        if (size >= 6) {
+            uint32_t cs;
+            std::memcpy(&cs, buf + 2, 4);
+            chunk_size_ = cs;
+        }
+    }
+    size_t getMaxHeaderSize(size_t) const override { return 6; }
+    void saveState() override { saved_ = {chunk_size_, actual_output_size_}; }
+    void restoreState() override {
+        chunk_size_ = saved_.cs;
+        actual_output_size_ = saved_.out;
+    }
+
+  private:
+    bool is_inverse_ = false;
+    size_t chunk_size_ = 0, actual_output_size_ = 0;
+    struct Saved { size_t cs, out; } saved_{0, 0};
+    DeviceScratch ws_;
+};
+
+// ---------------------------------------------------------------------------
+// Zigzag (modules/transforms/zigzag/zigzag_stage.h) and Negabinary
+// (modules/transforms/negabinary/negabinary_stage.h): signed TIn <-> its
+// unsigned TOut, element-wise; 2-byte header [TIn DataType, TOut DataType].
+// ---------------------------------------------------------------------------
+template <bool NEGABINARY, typename TIn, typename TOut>
+class IntCodeStage : public Stage {
+    static_assert(std::is_integral_v<TIn> && std::is_signed_v<TIn> && std::is_unsigned_v<TOut> &&
+                      sizeof(TIn) == sizeof(TOut),
+                  "signed TIn and its unsigned counterpart TOut");
+
+  public:
+    void setInverse(bool inv) override { is_inverse_ = inv; }
+    bool isInverse() const override { return is_inverse_; }
+    void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                 const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
+    std::string getName() const override { return NEGABINARY ? "Negabinary" : "Zigzag"; }
+    size_t getNumInputs() const override { return 1; }
+    size_t getNumOutputs() const override { return 1; }
+    std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& in) const override { return {in[0]}; }
+    std::unordered_map<std::string, size_t> getActualOutputSizesByName() const override {
+        return {{"output", actual_output_size_}};
+    }
+    size_t getActualOutputSize(int i) const override { return i == 0 ? actual_output_size_ : 0; }
+    uint16_t getStageTypeId() const override {
+        return (uint16_t)(NEGABINARY ? StageType::NEGABINARY : StageType::ZIGZAG);
+    }
+    uint8_t getOutputDataType(size_t) const override {
+        return (uint8_t)(is_inverse_ ? dataTypeOf<TIn>() : dataTypeOf<TOut>());
+    }
+    uint8_t getInputDataType(size_t) const override {
+        return (uint8_t)(is_inverse_ ? dataTypeOf<TOut>() : dataTypeOf<TIn>());
+    }
+    size_t serializeHeader(size_t, uint8_t* buf, size_t max_size) const override {
+        if (max_size < 2) return 0;
+        buf[0] = (uint8_t)dataTypeOf<TIn>();
+        buf[1] = (uint8_t)dataTypeOf<TOut>();
+        return 2;
+    }
+    size_t getMaxHeaderSize(size_t) const override { return 2; }
+    void saveState() override { saved_out_ = actual_output_size_; }
+    void restoreState() override { actual_output_size_ = saved_out_; }
+
+  private:
+    bool is_inverse_ = false;
+    size_t actual_output_size_ = 0, saved_out_ = 0;
+};
+template <typename TIn = int32_t, typename TOut = uint32_t> using ZigzagStage = IntCodeStage<false, TIn, TOut>;
+template <typename TIn = int32_t, typename TOut = uint32_t> using NegabinaryStage = IntCodeStage<true, TIn, TOut>;
+
+// ---------------------------------------------------------------------------
+// Integer Lorenzo (modules/predictors/lorenzo/lorenzo_stage.h): 1D deltas
+// inside 256-element blocks; 2D/3D inclusion-exclusion deltas over the whole
+// field; inverse prefix sums along x, y, z.  16-byte header (LorenzoConfig).
+// ---------------------------------------------------------------------------
+struct LorenzoConfig {
+    DataType data_type;
+    uint8_t ndim;
+    uint8_t reserved[2];
+    uint32_t dim_x, dim_y, dim_z;
+    LorenzoConfig() : data_type(DataType::INT32), ndim(1), reserved{0, 0}, dim_x(0), dim_y(1), dim_z(1) {}
+};
+static_assert(sizeof(LorenzoConfig) == 16, "LorenzoConfig is 16 bytes");
+
+template <typename T = int32_t>
+class LorenzoStage : public Stage {
+    static_assert(std::is_integral_v<T> && std::is_signed_v<T>, "LorenzoStage requires a signed integer type");
+
+  public:
+    void setInverse(bool inv) override { is_inverse_ = inv; }
+    bool isInverse() const override { return is_inverse_; }
+    void setDims(const std::array<size_t, 3>& dims) override { dims_ = dims; }
+    void setDims(size_t x, size_t y = 1, size_t z = 1) { dims_ = {x, y, z}; }
+    std::array<size_t, 3> getDims() const { return dims_; }
+    int ndim() const { return dims_[2] > 1 ? 3 : (dims_[1] > 1 ? 2 : 1); }
+    void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                 const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
+    std::string getName() const override { return "Lorenzo"; }
+    size_t getNumInputs() const override { return 1; }
+    size_t getNumOutputs() const override { return 1; }
+    std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& in) const override {
+        return {in.empty() ? 0 : in[0]};
+    }
+    std::unordered_map<std::string, size_t> getActualOutputSizesByName() const override {
+        return {{"output", actual_output_size_}};
+    }
+    size_t getActualOutputSize(int i) const override { return i == 0 ? actual_output_size_ : 0; }
+    uint16_t getStageTypeId() const override { return (uint16_t)StageType::LORENZO; }
+    uint8_t getOutputDataType(size_t) const override { return (uint8_t)dataTypeOf<T>(); }
+    uint8_t getInputDataType(size_t) const override { return (uint8_t)dataTypeOf<T>(); }
+    size_t serializeHeader(size_t, uint8_t* buf, size_t max_size) const override {
+        if (max_size < sizeof(LorenzoConfig)) return 0;
+        LorenzoConfig c;
+        c.data_type = dataTypeOf<T>();
+        c.ndim = (uint8_t)ndim();
+        c.dim_x = (uint32_t)dims_[0];
+        c.dim_y = (uint32_t)dims_[1];
+        c.dim_z = (uint32_t)dims_[2];
+        std::memcpy(buf, &c, sizeof(c));
+        return sizeof(c);
+    }
+    void deserializeHeader(const uint8_t* buf, size_t size) override {
+        if (size < sizeof(LorenzoConfig)) throw std::runtime_error("LorenzoStage: header too small");
+        LorenzoConfig c;
+        std::memcpy(&c, buf, sizeof(c));
+        const int nd = c.ndim == 0 ? 1 : c.ndim;
+        dims_ = {c.dim_x, nd >= 2 ? c.dim_y : 1u, nd >= 3 ? c.dim_z : 1u};
+    }
+    size_t getMaxHeaderSize(size_t) const override { return sizeof(LorenzoConfig); }
+    void saveState() override { saved_out_ = actual_output_size_; }
+    void restoreState() override { actual_output_size_ = saved_out_; }
+
+  private:
+    bool is_inverse_ = false;
+    std::array<size_t, 3> dims_ = {0, 1, 1};
+    size_t actual_output_size_ = 0, saved_out_ = 0;
     DeviceScratch ws_;
 };
 

@@ -126,6 +126,25 @@ int fzh_lorenzo_quant_inv_f32(void* stream, const void* d_codes, size_t n,
                               int code_bytes, float* d_out, void* d_ws,
                               size_t ws_bytes);
 
+/* The same inverse in two parts (together exactly fzh_lorenzo_quant_inv_f32
+ * on one workspace): _bin sorts the outliers by tile into the workspace and
+ * needs only the outlier arrays, so a caller can queue it before the codes
+ * exist; _apply then reconstructs from the codes and the binned outliers
+ * (same n, dims, outlier arrays, max_outliers and workspace).  With
+ * max_outliers <= 256, _bin does nothing and every tile reads the outlier
+ * list directly in _apply (no binning launches).                          */
+int fzh_lorenzo_quant_inv_bin(void* stream, size_t n, size_t nx, size_t ny, size_t nz,
+                              const float* d_outlier_err, const uint32_t* d_outlier_idx,
+                              const uint32_t* d_outlier_count, size_t max_outliers,
+                              void* d_ws, size_t ws_bytes);
+int fzh_lorenzo_quant_inv_apply_f32(void* stream, const void* d_codes, size_t n,
+                                    size_t nx, size_t ny, size_t nz,
+                                    const float* d_outlier_err,
+                                    const uint32_t* d_outlier_idx,
+                                    const uint32_t* d_outlier_count, size_t max_outliers,
+                                    float ebx2, int zigzag, int code_bytes, float* d_out,
+                                    void* d_ws, size_t ws_bytes);
+
 /* REL (mode 1: max|x|) / NOA (mode 2: max-min) value base, resolved on the
  * device.  d_params (4 floats) receives
  *   [0] value_base, [1] abs_eb, [2] ebx2_r = 1/(2*abs_eb), [3] ebx2 = 2*abs_eb
@@ -180,7 +199,7 @@ int fzh_rle_dec(void* stream, const void* d_in, size_t in_bytes,
                 int elem_bytes, void* d_out, size_t n, int* d_status,
                 void* d_ws, size_t ws_bytes);
 
-/* ---- Histogram / Huffman (coarse, 768-symbol partitions) ----------------- */
+/* ---- Histogram / Huffman (coarse 768- or fine 1024-symbol partitions) ------ */
 int fzh_histogram(void* stream, const void* d_in, size_t n, int elem_bytes,
                   uint32_t bklen, uint32_t* d_freq);
 /* Host routine: canonical codebook from a host histogram. */
@@ -193,6 +212,16 @@ int fzh_huffman_encode(void* stream, const void* d_in, size_t n, int elem_bytes,
                        uint32_t bklen, const uint32_t* d_bk4,
                        const uint8_t* d_revbook, void* d_out, size_t out_cap,
                        uint64_t* d_out_size, void* d_ws, size_t ws_bytes);
+/* Fine mode (HuffmanStage Fine, huffman_stage.cu:41; hf_buf.cc:48-49): the
+ * same stream with 1024-symbol partitions, as the reference's ReVISIT-lite
+ * kernel lays it out (hf_kernels.cu:145-250).  The reference takes it only
+ * when no codeword of the book is longer than 8 bits and falls back to the
+ * coarse stream otherwise (hf_hl.cc:69-80); callers check that on the host
+ * book (HuffmanStage does).  Same bound and workspace as fzh_huffman_encode. */
+int fzh_huffman_encode_fine(void* stream, const void* d_in, size_t n, int elem_bytes,
+                            uint32_t bklen, const uint32_t* d_bk4,
+                            const uint8_t* d_revbook, void* d_out, size_t out_cap,
+                            uint64_t* d_out_size, void* d_ws, size_t ws_bytes);
 int fzh_huffman_decode(void* stream, const void* d_in, size_t in_bytes,
                        int elem_bytes, void* d_out, size_t n, int* d_status);
 
@@ -248,6 +277,46 @@ int fzh_periodic_window_f32(void* stream, const float* d_base, uint64_t bx, uint
  * totals.  The event records add a small gap between launches.            */
 int fzh_kernel_timing_enable(int on);
 int fzh_kernel_timing_get(const char* name, double* total_ms, uint64_t* count);
+
+/* ---- Transforms (modules/transforms, modules/predictors/diff,
+ *      modules/predictors/lorenzo) ------------------------------------------
+ * Element types are given as the FZM DataType code: 0 u8, 1 u16, 2 u32,
+ * 3 u64, 4 i8, 5 i16, 6 i32, 7 i64, 8 f32, 9 f64.  Buffers 16-byte aligned.
+ *
+ * Zigzag (zigzag.h:70-80: (x << 1) ^ (x >> (W-1))) and negabinary
+ * (negabinary.h: (x + 0xAA..) ^ 0xAA..) codes of signed integers of
+ * elem_bytes 1/2/4/8, element-wise, in place allowed; inverse = 1 decodes.  */
+int fzh_zigzag(void* stream, const void* d_in, void* d_out, size_t n, int elem_bytes, int inverse);
+int fzh_negabinary(void* stream, const void* d_in, void* d_out, size_t n, int elem_bytes, int inverse);
+
+/* Difference coding (DifferenceStage, diff.cu:22-240): d_out[i] = in[i] -
+ * in[i-1], the first element of every chunk of chunk_elems (a power of two;
+ * 0 = one chunk) stored as is, in the element type's arithmetic (integers
+ * wrap); negabinary = 1 (signed integer types) stores the negabinary codes.
+ * Inverse: the cumulative sum per chunk (negabinary decoded first), d_in !=
+ * d_out; floating-point sums follow the reference's association: 512-element
+ * tiles of a chunk, each as cub::BlockScan<T, 512> (raking) sums it -- 32
+ * segments of 16 elements added left to right, a Kogge-Stone scan of the
+ * segment sums over 32 lanes, each segment's elements added left to right
+ * onto the previous segments' sum -- plus the left-to-right sum of the
+ * chunk's earlier tile totals.  Workspace: fzh_diff_inv_ws_bytes.           */
+int fzh_diff_fwd(void* stream, const void* d_in, void* d_out, size_t n, int dtype, size_t chunk_elems,
+                 int negabinary);
+size_t fzh_diff_inv_ws_bytes(size_t n, int dtype, size_t chunk_elems);
+int fzh_diff_inv(void* stream, const void* d_in, void* d_out, size_t n, int dtype, size_t chunk_elems,
+                 int negabinary, void* d_ws, size_t ws_bytes);
+
+/* Integer Lorenzo (LorenzoStage, lorenzo_stage.cu): signed elements of
+ * elem_bytes 1/2/4/8, wrapping arithmetic, d_in != d_out.  1D (ny = nz = 1):
+ * d_out[i] = in[i] - in[i-1] inside blocks of 256 elements (the first of a
+ * block as is); 2D/3D: the inclusion-exclusion delta of every element over
+ * the whole field, neighbours outside it 0.  Inverse: inclusive prefix sums
+ * per 256-element block (1D), or along x, then y, then z.                   */
+int fzh_lorenzo_int_fwd(void* stream, const void* d_in, void* d_out, size_t nx, size_t ny, size_t nz,
+                        int elem_bytes);
+size_t fzh_lorenzo_int_inv_ws_bytes(size_t nx, size_t ny, size_t nz);
+int fzh_lorenzo_int_inv(void* stream, const void* d_in, void* d_out, size_t nx, size_t ny, size_t nz,
+                        int elem_bytes, void* d_ws, size_t ws_bytes);
 
 /* Exclusive scan of n u32 values: the offsets primitive under every stage
  * whose unit sizes are only known on the device (RZE chunk offsets, outlier

@@ -56,11 +56,23 @@ int fzp_add_bitshuffle(fzp_pipeline* p, uint32_t block_size, int elem_width, int
 int fzp_add_rze(fzp_pipeline* p, uint32_t chunk_size, int levels, int* stage_id);
 int fzp_add_rle(fzp_pipeline* p, int elem_bytes, int* stage_id);
 int fzp_add_huffman(fzp_pipeline* p, int elem_bytes, uint32_t bklen, int* stage_id);
+/* HuffmanStage::setEncodeMode: fine = 1 for HuffmanEncodeMode::Fine. */
+int fzp_set_huffman_fine(fzp_pipeline* p, int stage_id, int fine);
 int fzp_add_quantizer(fzp_pipeline* p, float error_bound, int eb_mode, uint32_t quant_radius,
                       float outlier_capacity, int zigzag, int code_bytes, float value_base,
                       int* stage_id);
 /* nbits 0 = auto-detect from the maximum */
 int fzp_add_bitpack(fzp_pipeline* p, int elem_bytes, int nbits, int* stage_id);
+/* DifferenceStage<T, TOut> (diff.h:48-200): dtype = FZM DataType code of T
+ * (0 u8 1 u16 2 u32 4 i8 5 i16 6 i32 7 i64 8 f32 9 f64); negabinary = 1
+ * picks the fused signed->unsigned variant; chunk_bytes 0 = no chunking.  */
+int fzp_add_difference(fzp_pipeline* p, int dtype, int negabinary, size_t chunk_bytes, int* stage_id);
+/* ZigzagStage / NegabinaryStage<intN, uintN> (zigzag_stage.h, negabinary_stage.h). */
+int fzp_add_zigzag(fzp_pipeline* p, int elem_bytes, int* stage_id);
+int fzp_add_negabinary(fzp_pipeline* p, int elem_bytes, int* stage_id);
+/* LorenzoStage<intN> (lorenzo_stage.h): the integer Lorenzo predictor; its
+ * dims come from fzp_set_dims. */
+int fzp_add_lorenzo_int(fzp_pipeline* p, int elem_bytes, int* stage_id);
 
 int fzp_connect(fzp_pipeline* p, int dependent_id, int producer_id, const char* output_name);
 int fzp_set_dims(fzp_pipeline* p, size_t x, size_t y, size_t z);

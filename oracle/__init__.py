@@ -67,6 +67,7 @@ def _declare(L):
         "orc_huffman_book": (None, [_P, C.c_uint32, C.c_int, _P, _P]),
         "orc_huffman_revbook_bytes": (_Z, [C.c_uint32, C.c_int]),
         "orc_huffman_encode": (_Z, [_P, _Z, C.c_int, C.c_uint32, _P, _Z]),
+        "orc_huffman_encode_mode": (_Z, [_P, _Z, C.c_int, C.c_uint32, C.c_int, _P, _Z]),
         "orc_huffman_encode_bound": (_Z, [_Z, C.c_uint32, C.c_int]),
         "orc_huffman_decode": (C.c_int, [_P, _Z, C.c_int, _P, _Z]),
         "orc_quant_abs_fwd_f32": (C.c_uint32, [_P, _Z, C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_int, _P, _P, _P, _Z]),
@@ -221,12 +222,12 @@ def ref_huffman_book(freq: np.ndarray, elem_bytes: int = 2):
     return bk4, rev
 
 
-def huffman_encode(v: np.ndarray, bklen: int) -> np.ndarray:
+def huffman_encode(v: np.ndarray, bklen: int, fine: bool = False) -> np.ndarray:
     v = np.ascontiguousarray(v)
     eb = v.dtype.itemsize
     cap = lib().orc_huffman_encode_bound(v.size, bklen, eb)
     out = np.zeros(cap, dtype=np.uint8)
-    sz = lib().orc_huffman_encode(_ptr(v), v.size, eb, bklen, _ptr(out), cap)
+    sz = lib().orc_huffman_encode_mode(_ptr(v), v.size, eb, bklen, int(bool(fine)), _ptr(out), cap)
     if sz == 0 and v.size:
         raise ValueError("symbol out of range")
     return out[:sz].copy()

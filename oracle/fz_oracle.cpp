@@ -940,6 +940,16 @@ size_t orc_huffman_encode_bound(size_t n, uint32_t bklen, int eb) {
 }
 
 size_t orc_huffman_encode(const void* in, size_t n, int eb, uint32_t bklen, uint8_t* out, size_t cap) {
+    return orc_huffman_encode_mode(in, n, eb, bklen, 0, out, cap);
+}
+
+// fine = 1: the Fine mode of HuffmanStage (huffman_stage.cu:41): 1024-symbol
+// partitions (hf_buf.cc:48-49), taken only when no codeword is longer than
+// 8 bits, else the coarse 768 (hf_hl.cc:69-80).  The ReVISIT-lite kernel
+// (hf_kernels.cu:145-250) builds each partition's cells by the same MSB-first
+// concatenation as the coarse deflate below.
+size_t orc_huffman_encode_mode(const void* in, size_t n, int eb, uint32_t bklen, int fine, uint8_t* out,
+                               size_t cap) {
     if (n == 0) return 0;
     std::vector<uint32_t> freq(bklen);
     orc_histogram(in, n, eb, bklen, freq.data());
@@ -951,7 +961,9 @@ size_t orc_huffman_encode(const void* in, size_t n, int eb, uint32_t bklen, uint
     std::vector<uint8_t> rev(rbytes);
     orc_huffman_book(freq.data(), bklen, eb, bk4.data(), rev.data());
 
-    const size_t sublen = 768, pardeg = (n - 1) / sublen + 1;
+    uint32_t max_cl = 0;
+    for (uint32_t i = 0; i < bklen; i++) max_cl = std::max(max_cl, bk4[i] >> 27);
+    const size_t sublen = (fine && max_cl <= 8) ? 1024 : 768, pardeg = (n - 1) / sublen + 1;
     std::vector<uint32_t> nbit(pardeg), ncell(pardeg), entry(pardeg);
     std::vector<uint32_t> gapped(n + 1, 0);
     // Phase 2 deflate (hf_kernels.cu:73-122), one partition at a time.

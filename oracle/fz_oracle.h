@@ -95,6 +95,8 @@ size_t orc_huffman_revbook_bytes(uint32_t bklen, int elem_bytes);
  * PHF stream).  Returns the stream size, 0 if a symbol is >= bklen. */
 size_t orc_huffman_encode(const void* in, size_t n, int elem_bytes,
                           uint32_t bklen, uint8_t* out, size_t out_cap);
+size_t orc_huffman_encode_mode(const void* in, size_t n, int elem_bytes, uint32_t bklen, int fine,
+                               uint8_t* out, size_t cap);
 size_t orc_huffman_encode_bound(size_t n, uint32_t bklen, int elem_bytes);
 /* Decode a PHF stream into n symbols.  Returns 0 on success. */
 int orc_huffman_decode(const uint8_t* in, size_t in_bytes, int elem_bytes,

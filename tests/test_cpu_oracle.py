@@ -372,3 +372,26 @@ def test_concat_layout_helper():
     assert b.size == 32 + 16 + 16 + 16
     assert [p.size for p in chains.split(b)] == [12, 4, 5]
     assert not b[28:32].any() and not b[32 + 12:48].any()
+
+
+def test_huffman_fine_mode_layout(orc):
+    """Fine mode (huffman_stage.cu:41, hf_hl.cc:69-80, hf_buf.cc:48-49):
+    1024-symbol partitions when every codeword of the book is at most 8 bits
+    -- unused symbols keep the all-ones word (hf_bk.cc:96-109), so only a
+    book using every symbol qualifies -- else the coarse 768; the stream
+    decodes back and its partition bit counts equal the codeword lengths."""
+    rng = np.random.default_rng(21)
+    v = rng.integers(0, 256, 1024 * 50 + 77).astype(np.uint8)
+    s = orc.huffman_encode(v, 256, fine=True)
+    hdr = s[:64]
+    sublen, pardeg = int(hdr[4:8].view(np.int32)[0]), int(hdr[8:12].view(np.int32)[0])
+    assert (sublen, pardeg) == (1024, -(-v.size // 1024))
+    assert np.array_equal(orc.huffman_decode(s, np.uint8, v.size), v)
+    bk4, _ = orc.huffman_book(np.bincount(v, minlength=256).astype(np.uint32), 1)
+    lens = (bk4 >> 27)[v].astype(np.int64)
+    ent = hdr[40:64].view(np.uint32)
+    nbit = s[ent[2]:ent[3]].view(np.uint32)
+    assert np.array_equal(nbit, np.add.reduceat(lens, np.arange(0, v.size, 1024)))
+    # one unused symbol: the book's all-ones word forces the coarse stream
+    w = np.where(v == 7, 8, v).astype(np.uint8)
+    assert int(orc.huffman_encode(w, 256, fine=True)[4:8].view(np.int32)[0]) == 768

@@ -105,6 +105,43 @@ def test_huffman_long_codes(gpu, orc):
     np.testing.assert_array_equal(gpu.huffman_decode(s, np.uint16, v.size), v)
 
 
+@pytest.mark.parametrize("name", ["gauss", "skewed", "tiny", "constant", "partial_partition", "u8", "small_alpha",
+                                  "u8_all_symbols", "u8_all_symbols_partial"])
+def test_huffman_fine_matches_oracle(gpu, orc, name):
+    """Fine mode: 1024-symbol partitions when every codeword of the book fits
+    8 bits (hf_hl.cc:69-80), else the coarse stream.  Unused symbols keep
+    the all-ones word (hf_bk.cc:96-109), whose length field reads 31, so the
+    fine stream needs every symbol of [0, bklen) to occur."""
+    rng = np.random.default_rng(3)
+    inputs = _huffman_inputs()
+    inputs["small_alpha"] = rng.integers(0, 40, 1024 * 37 + 5).astype(np.uint16)
+    inputs["u8_all_symbols"] = rng.integers(0, 256, 1024 * 300).astype(np.uint8)
+    inputs["u8_all_symbols_partial"] = rng.integers(0, 256, 1024 * 77 + 333).astype(np.uint8)
+    v = inputs[name]
+    bklen = 256 if v.dtype == np.uint8 else 1024
+    s = gpu.huffman_encode(v, bklen, fine=True)
+    o = orc.huffman_encode(v, bklen, fine=True)
+    np.testing.assert_array_equal(s, o)
+    np.testing.assert_array_equal(gpu.huffman_decode(s, v.dtype, v.size), v)
+    sublen = int(o[4:8].view(np.int32)[0])
+    bk4, _ = orc.huffman_book(np.bincount(v, minlength=bklen).astype(np.uint32), v.dtype.itemsize)
+    assert sublen == (1024 if int((bk4 >> 27).max()) <= 8 else 768)
+
+
+@pytest.mark.parametrize("fine", [False, True])
+def test_huffman_fine_in_pipeline(gpu, orc, fine):
+    from fzgpumodules_amd.pipeline import Pipeline
+    v = np.random.default_rng(4).integers(0, 256, 200_000).astype(np.uint8)
+    with Pipeline(v.nbytes) as p:
+        h = p.add_huffman(1, 256)
+        p.set_huffman_fine(h, fine)
+        p.finalize()
+        blob = p.compress_array(v)
+        np.testing.assert_array_equal(p.decompress_array(blob, np.uint8)[: v.size], v)
+    np.testing.assert_array_equal(blob, orc.huffman_encode(v, 256, fine=fine))
+    assert int(blob[4:8].view(np.int32)[0]) == (1024 if fine else 768)  # phf_header.sublen
+
+
 def test_huffman_rejects_out_of_range(gpu):
     with pytest.raises(ValueError):
         gpu.huffman_encode(np.array([1, 2, 2000], np.uint16), 1024)

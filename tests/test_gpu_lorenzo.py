@@ -130,3 +130,40 @@ def test_forward_reads_device_eb(gpu, orc):
     oc, oe, oi, ocnt = orc.lorenzo_fwd(x, (200, 150, 1), p[2], 32768, True, 2, x.size)
     np.testing.assert_array_equal(gc, oc)
     assert gcnt == ocnt
+
+
+@pytest.mark.parametrize("dims", [(5000, 1, 1), (100, 70, 1), (37, 29, 21)])
+def test_inverse_in_two_parts_on_two_streams(gpu, orc, dims):
+    """fzh_lorenzo_quant_inv_bin on a second stream, then _apply after an
+    event join: bit-exact with the oracle, as the one-call inverse."""
+    import ctypes as C
+    from fzgpumodules_amd._fzh import DeviceBuffer, call, lib, workspace
+    rng = np.random.default_rng(11)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    x = make_field(rng, n, n)
+    ebx2_r, ebx2 = eb_params(1e-3)
+    c, e, i, cnt = orc.lorenzo_fwd(x, dims, ebx2_r, 512, True, 2, n)
+    assert cnt > 0
+    y_ref = orc.lorenzo_inv(c, dims, e, i, cnt, ebx2, True, n)
+    dc = DeviceBuffer.from_array(c)
+    de = DeviceBuffer.from_array(e)
+    di = DeviceBuffer.from_array(i)
+    dn = DeviceBuffer.from_array(np.array([cnt], np.uint32))
+    out = DeviceBuffer(4 * n)
+    ws = workspace(lib().fzh_lorenzo_quant_inv_ws_bytes(n, nx, ny, nz, n))
+    s1, s2 = C.c_void_p(), C.c_void_p()
+    call("fzh_stream_create", C.byref(s1))
+    call("fzh_stream_create", C.byref(s2))
+    try:
+        for rep in range(2):  # the scan clears the per-tile counters for the next call
+            call("fzh_lorenzo_quant_inv_bin", s2.value, n, nx, ny, nz, de.ptr, di.ptr, dn.ptr, n, ws.ptr, ws.nbytes)
+            call("fzh_stream_sync", s2.value)
+            call("fzh_lorenzo_quant_inv_apply_f32", s1.value, dc.ptr, n, nx, ny, nz, de.ptr, di.ptr, dn.ptr, n,
+                 np.float32(ebx2), 1, 2, out.ptr, ws.ptr, ws.nbytes)
+            call("fzh_stream_sync", s1.value)
+            y = out.download(np.float32, n)
+            np.testing.assert_array_equal(y.view(np.uint32), y_ref.view(np.uint32))
+    finally:
+        call("fzh_stream_destroy", s1.value)
+        call("fzh_stream_destroy", s2.value)
