@@ -74,6 +74,7 @@ SIGNATURES = {
     "fzh_concat_bound": (C.c_uint64, [_P, _I]),
     "fzh_concat": (_I, [_P, _P, _I, _P, C.c_uint64, _P, _P, _Z]),
     "fzh_readback": (_I, [_P, _P, _I, _P]),
+    "fzh_concat_readback": (_I, [_P, _P, _I, _P, C.c_uint64, _P, _P, _I, _P]),
     # pipeline C ABI (include/fzp.h)
     "fzp_create": (_I, [_Z, _F, C.POINTER(_P)]),
     "fzp_destroy": (_I, [_P]),

@@ -3,10 +3,12 @@
 // Forward kernels that emit variable-length lists (outliers, patches) write
 // each tile's entries to a fixed per-tile scratch slot, in the tile's order,
 // plus a per-tile count.  launch_compact then places every tile's list at
-// its exclusive prefix: one device-wide scan of the counts (fzh_scan.h, no
-// workgroup waits on another) and a place kernel in which a wave takes 32
-// tiles of each list, finds the non-empty ones by ballot and copies them
-// with all 64 lanes.
+// its exclusive prefix with a place kernel in which a wave takes 32 tiles
+// of each list, finds the non-empty ones by ballot and copies them with all
+// 64 lanes.  Up to COMPACT_INLINE tiles each workgroup sums the counts
+// before its 128 tiles itself (one launch in all); beyond that a
+// device-wide scan of the counts (fzh_scan.h) runs first.  No workgroup
+// waits on another.
 #pragma once
 
 #include "fzh_scan.h"
@@ -27,22 +29,56 @@ struct CompactList {
     uint64_t cap;
 };
 
-// excl: exclusive prefix (mod 2^32) of the NL*T counts, excl[NL*T] = sum.
-template <int NL>
+constexpr uint32_t COMPACT_INLINE = 16384;
+
+// excl: exclusive prefix (mod 2^32) of the NL*T counts, excl[NL*T] = sum
+// (INL: unused, the prefixes are summed in the kernel).
+template <int NL, bool INL>
 __global__ void __launch_bounds__(256)
 place_kernel(CompactList L0, CompactList L1, uint32_t T, uint32_t E, const uint32_t* __restrict__ excl) {
-    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int w = threadIdx.x >> 6;
+    const uint32_t wave = blockIdx.x * 4 + (uint32_t)w;
     const int l = lane_id();
     const uint32_t tb = wave * 32;
-    if (tb >= T) return;
-    if (wave == 0 && l == 0) {
-        *L0.total = excl[T];
-        if constexpr (NL == 2) *L1.total = excl[2 * T] - excl[T];
-    }
     const uint32_t li = (uint32_t)l >> 5;
     const uint32_t t = tb + (uint32_t)(l & 31);
     uint32_t c = 0;
     if (t < T && li < (uint32_t)NL) c = L0.cnt[(uint64_t)li * T + t];
+    uint32_t ex = 0;  // INL: this lane's tile's exclusive prefix within its list
+    if constexpr (INL) {
+        __shared__ uint32_t s_sum[4 * NL];
+        __shared__ uint32_t s_wave[4][2];
+        uint32_t before[NL];
+        wg256_sums_before<NL>(min(blockIdx.x * 128u, T),
+                              [&](int k, uint32_t i) { return L0.cnt[(uint64_t)k * T + i]; }, s_sum, before);
+        const uint32_t inc = wave_incl_scan(c);
+        const uint32_t h0 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 31);  // list 0 part of the wave
+        const uint32_t h1 = lane63(inc) - h0;                                    // list 1 part
+        if (l == 0) {
+            s_wave[w][0] = h0;
+            s_wave[w][1] = h1;
+        }
+        __syncthreads();
+        uint32_t p0 = before[0], p1 = NL == 2 ? before[NL - 1] : 0u;
+        for (int k = 0; k < w; k++) {
+            p0 += s_wave[k][0];
+            p1 += s_wave[k][1];
+        }
+        ex = inc - c - (li ? h0 : 0u) + (li ? p1 : p0);
+        if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {  // the last workgroup ends at or past T
+            uint32_t q0 = before[0], q1 = NL == 2 ? before[NL - 1] : 0u;
+            for (int k = 0; k < 4; k++) {
+                q0 += s_wave[k][0];
+                q1 += s_wave[k][1];
+            }
+            *L0.total = q0;
+            if constexpr (NL == 2) *L1.total = q1;
+        }
+    } else if (wave == 0 && l == 0) {
+        *L0.total = excl[T];
+        if constexpr (NL == 2) *L1.total = excl[2 * T] - excl[T];
+    }
+    if (tb >= T) return;
     uint64_t todo = __ballot(c != 0u);
     while (todo) {
         const int j = __builtin_ctzll(todo);
@@ -50,7 +86,9 @@ place_kernel(CompactList L0, CompactList L1, uint32_t T, uint32_t E, const uint3
         const uint32_t cj = __shfl(c, j, 64);
         const uint32_t lj = (uint32_t)j >> 5;
         const uint32_t tile = tb + (uint32_t)(j & 31);
-        const uint32_t dst0 = excl[(uint64_t)lj * T + tile] - (lj ? excl[T] : 0u);
+        uint32_t dst0;
+        if constexpr (INL) dst0 = (uint32_t)__shfl((int)ex, j, 64);
+        else dst0 = excl[(uint64_t)lj * T + tile] - (lj ? excl[T] : 0u);
         const uint64_t src0 = (uint64_t)tile * E;
         const float* sv = lj ? L1.s_val : L0.s_val;
         const uint32_t* si = lj ? L1.s_idx : L0.s_idx;
@@ -78,10 +116,13 @@ inline int launch_compact(hipStream_t s, void* state, size_t T, uint32_t E, Comp
     if (L1 && L1->cnt != L0.cnt + T) return fail(FZH_E_INVALID, "compact: second count list must follow the first");
     uint32_t* excl = (uint32_t*)state;
     void* ws = (uint8_t*)state + align_up(4 * (2 * T + 1), 256);
-    if (int rc = scan_u32<true>(s, L0.cnt, excl, nl * T, nullptr, excl + nl * T, ws)) return rc;
+    const bool inl = T <= COMPACT_INLINE;
+    if (!inl)
+        if (int rc = scan_u32<true>(s, L0.cnt, excl, nl * T, nullptr, excl + nl * T, ws)) return rc;
     const unsigned blocks = (unsigned)cdiv(cdiv(T, 32), 4);
-    launch_k("place_kernel", L1 ? place_kernel<2> : place_kernel<1>, dim3(blocks), dim3(256), 0, s, L0,
-             L1 ? *L1 : CompactList{}, (uint32_t)T, E, excl);
+    auto* k = inl ? (L1 ? place_kernel<2, true> : place_kernel<1, true>)
+                  : (L1 ? place_kernel<2, false> : place_kernel<1, false>);
+    launch_k("place_kernel", k, dim3(blocks), dim3(256), 0, s, L0, L1 ? *L1 : CompactList{}, (uint32_t)T, E, excl);
     FZH_CHECK_LAUNCH("compact");
     return FZH_OK;
 }

@@ -311,11 +311,12 @@ __device__ __forceinline__ uint32_t rze_encode_chunk_wg(const uint8_t* __restric
 // `dst` (any alignment), touching no byte outside [dst, dst+size): bytes up
 // to the first 16-byte boundary of dst, then 16-byte stores (each lane
 // assembles its word from five source dwords with alignbyte), then the
-// tail.  Whole wave calls.
+// tail.  Called by a whole wave (NT = 64) or a whole workgroup of NT threads.
+template <int NT = 64>
 __device__ __forceinline__ void copy_to_unaligned16(const uint8_t* src, uint64_t limit, uint8_t* dst,
                                                     uint32_t size) {
     if (size == 0) return;
-    const int l = lane_id();
+    const int l = NT == 64 ? lane_id() : (int)threadIdx.x;
     const uint32_t head = (uint32_t)((16 - ((uintptr_t)dst & 15)) & 15);
     const uint32_t hb = head < size ? head : size;
     if ((uint32_t)l < hb) dst[l] = src[l];
@@ -331,7 +332,7 @@ __device__ __forceinline__ void copy_to_unaligned16(const uint8_t* src, uint64_t
     };
     uint4* dq = reinterpret_cast<uint4*>(dst + hb);
     const uint32_t n16 = (size - hb) / 16;
-    for (uint32_t i = l; i < n16; i += 64) {
+    for (uint32_t i = l; i < n16; i += NT) {
         const uint32_t q = (hb + 16 * i) >> 2;
         const uint32_t w0 = sw[q], w1 = sw[q + 1], w2 = sw[q + 2], w3 = sw[q + 3];
         uint4 v;

@@ -59,6 +59,34 @@ __device__ __forceinline__ uint32_t sat_u32(uint64_t v, bool wrap) {
     return (!wrap && v > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)v;
 }
 
+// Sums (mod 2^32) of NL arrays over [0, end), computed by one 256-thread
+// workgroup and returned to every thread: a kernel whose workgroup needs
+// the exclusive prefix at its own position reads the counts before it
+// instead of waiting for a separate scan launch (worth it while the arrays
+// are short: every workgroup reads up to `end` elements per array, from L2).
+// Every thread of the workgroup must call it (it holds a barrier); s holds
+// 4 * NL u32 of LDS.
+template <int NL, typename Load>
+__device__ __forceinline__ void wg256_sums_before(uint32_t end, Load load, uint32_t* s, uint32_t (&sum)[NL]) {
+    uint32_t a[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) a[k] = 0;
+#pragma unroll 4
+    for (uint32_t i = threadIdx.x; i < end; i += 256) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) a[k] += load(k, i);
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        a[k] = wave_sum(a[k]);
+        if (lane_id() == 0) s[4 * k + w] = a[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NL; k++) sum[k] = s[4 * k] + s[4 * k + 1] + s[4 * k + 2] + s[4 * k + 3];
+}
+
 // Exclusive scan of in[0..n) into out (may alias in) by one 1024-thread
 // workgroup in passes of 16384 elements (64 KiB of LDS; each pass costs one
 // dependent HBM round trip, so short arrays take one): each pass is read and

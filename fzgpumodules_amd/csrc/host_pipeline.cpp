@@ -354,12 +354,18 @@ void Pipeline::launchForward(const void* d_in, size_t padded, bool zero_view, hi
             s.elem = ds.elem;
             s.size = ds.kind == 0 ? std::min<size_t>(B.size, s.cap) : 0;
         }
-        throwIfFzh(fzh_concat(stream, slots.data(), (int)slots.size(), concat_buf_, concat_cap_,
-                              (uint64_t*)((uint8_t*)concat_ws_ + 512), concat_ws_, 512),
-                   "Pipeline concat");
         concat_hdr_.assign(4 + 8 * outputs_.size(), 0);
         add_rb(concat_buf_, concat_hdr_.data(), (uint32_t)round_up(concat_hdr_.size(), 4));
         add_rb((uint8_t*)concat_ws_ + 512, &concat_total_, 8);
+        for (auto& n : nodes_)
+            for (const auto& r : n.stage->pendingReadbacks()) add_rb(r.d_src, r.h_dst, r.bytes);
+        // concat and every readback in one launch (the header and total
+        // entries see the values that launch writes)
+        throwIfFzh(fzh_concat_readback(stream, slots.data(), (int)slots.size(), concat_buf_, concat_cap_,
+                                       (uint64_t*)((uint8_t*)concat_ws_ + 512), rb.data(), (int)rb.size(),
+                                       rb_dev_),
+                   "Pipeline concat");
+        return;
     }
     for (auto& n : nodes_)
         for (const auto& r : n.stage->pendingReadbacks()) add_rb(r.d_src, r.h_dst, r.bytes);

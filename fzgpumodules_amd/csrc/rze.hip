@@ -17,11 +17,13 @@
 //   slot of a chunk-sized scratch (raw chunks are taken from the input by
 //   the pack kernel, except bit-shuffled blocks).  A pack kernel
 //   places every chunk at its offset, which each workgroup sums directly from
-//   the sizes the encode kernel wrote -- no host round trip and no wave
-//   waiting on another (the reference stages per-chunk scratch too, then
-//   scans on the host stream with CUB and copies).
-//   decode: a one-workgroup kernel scans the header's sizes into chunk
-//   offsets; then one 256-thread workgroup per chunk decodes, with the
+//   the sizes the encode kernel wrote (beyond 4096 chunks a device-wide scan
+//   runs first) -- no host round trip and no wave waiting on another (the
+//   reference stages per-chunk scratch too, then scans on the host stream
+//   with CUB and copies).
+//   decode: one 256-thread workgroup per chunk sums the header's sizes
+//   before its chunk (beyond 4096 chunks a one-workgroup kernel scans them
+//   first) and decodes, with the
 //   repeat-elimination decode done as a parallel prefix-popcount gather
 //   instead of a single-thread loop.
 #include "fzh_rze_chunk.h"
@@ -29,6 +31,12 @@
 
 namespace fzh {
 namespace {
+
+// Streams of up to PACK_INLINE chunks (64 MiB) take their chunk offsets
+// inside the pack and decode kernels (each workgroup reads the sizes before
+// its chunk, at most 16 KiB, from L2); longer ones scan them in separate
+// launches first.
+constexpr uint32_t PACK_INLINE = 4096;
 
 // ---------------------------------------------------------------------------
 // Encode
@@ -59,33 +67,44 @@ rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restr
     }
 }
 
-// Places chunk c at [8 + 4 nc + offs[c]) of the stream, one wave per chunk
-// (offs = exclusive scan of the clean sizes, offs[nc] = their sum).  Raw
-// chunks below `nshuf` (bit-shuffled blocks) come from the scratch slot,
-// other raw chunks straight from the input.
+// Places chunk c at [8 + 4 nc + offs[c]) of the stream, one workgroup per
+// chunk (offs = exclusive scan of the clean sizes, offs[nc] = their sum;
+// with offs null, up to PACK_INLINE chunks, each workgroup sums the clean
+// sizes before its chunk itself).  Raw chunks below `nshuf` (bit-shuffled
+// blocks) come from the scratch slot, other raw chunks straight from the
+// input.
 __global__ void __launch_bounds__(256)
 rze_pack_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, const uint8_t* __restrict__ scratch,
-                const uint32_t* __restrict__ sizes, const uint32_t* __restrict__ offs, uint8_t* __restrict__ out,
-                uint64_t* __restrict__ out_size, uint32_t nc, uint64_t nshuf) {
-    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    if (c >= nc) return;
-    const int l = lane_id();
+                const uint32_t* __restrict__ sizes, const uint32_t* __restrict__ cleans,
+                const uint32_t* __restrict__ offs, uint8_t* __restrict__ out, uint64_t* __restrict__ out_size,
+                uint32_t nc, uint64_t nshuf) {
+    __shared__ uint32_t s_sum[4];
+    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x);
+    if (c >= nc) return;  // uniform over the workgroup
+    uint32_t off;
+    if (offs) {
+        off = offs[c];
+    } else {
+        uint32_t before[1];
+        wg256_sums_before<1>(c, [&](int, uint32_t i) { return cleans[i]; }, s_sum, before);
+        off = before[0];
+    }
     const uint64_t hdr = 8ull + 4ull * nc;
     const uint32_t size = sizes[c];
     const uint32_t clean = size & 0x7FFFFFFFu;
     const bool raw = (size & 0x80000000u) != 0u;
-    const uint64_t dst = hdr + offs[c];
+    const uint64_t dst = hdr + off;
     if (raw && c >= nshuf) {
         const uint64_t src = (uint64_t)c * CS;
-        copy_to_unaligned16(in + src, nbytes - src, out + dst, clean);
+        copy_to_unaligned16<256>(in + src, nbytes - src, out + dst, clean);
     } else {
-        copy_to_unaligned16(scratch + (uint64_t)c * CS, CS, out + dst, clean);
+        copy_to_unaligned16<256>(scratch + (uint64_t)c * CS, CS, out + dst, clean);
     }
     if (c == nc - 1) {
-        const uint64_t end = hdr + offs[nc];
+        const uint64_t end = hdr + (uint32_t)(off + clean);
         const uint64_t padded = (end + 3) & ~3ull;
-        if ((uint64_t)l < padded - end) out[end + l] = 0;
-        if (l == 0) *out_size = padded;
+        if ((uint64_t)threadIdx.x < padded - end) out[end + threadIdx.x] = 0;
+        if (threadIdx.x == 0) *out_size = padded;
     }
 }
 
@@ -135,6 +154,7 @@ struct RzeDecLds {
     uint8_t bm3[32];
     uint32_t hsum[4][3];   // per-wave sums of the bitmap stages
     uint32_t wsum[4][16];  // per-wave sums of the 16 data steps
+    uint32_t osum[4];      // per-wave sums of the sizes before the chunk
     uint32_t xsel[16];     // v_perm selectors spreading packed bytes over a dword, by nibble
 };
 
@@ -163,6 +183,15 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
     const int t = threadIdx.x, w = t >> 6, l = lane_id();
     const uint32_t hdr = 8u + 4u * nc;
     bool bad = in_bytes < hdr;
+    uint32_t off_c = 0;
+    if (offs) {
+        off_c = offs[c];
+    } else {  // short header: the workgroup sums the sizes before chunk c itself
+        const uint32_t* ent = reinterpret_cast<const uint32_t*>(in + 8);
+        uint32_t before[1];
+        wg256_sums_before<1>(bad ? 0u : c, [&](int, uint32_t i) { return ent[i] & 0x7FFFFFFFu; }, L.osum, before);
+        off_c = before[0];
+    }
     uint32_t entry = 0;
     if (!bad) {
         const uint32_t h0 = reinterpret_cast<const uint32_t*>(in)[0];
@@ -171,7 +200,7 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
         entry = reinterpret_cast<const uint32_t*>(in + 8)[c];
     }
     const uint32_t stored = bad ? 0u : (entry & 0x7FFFFFFFu);
-    const uint64_t src = (uint64_t)hdr + offs[c];
+    const uint64_t src = (uint64_t)hdr + off_c;
     const uint32_t o = (c + 1 < nc) ? (uint32_t)CS : (uint32_t)(orig_bytes - (uint64_t)c * CS);
     const bool israw = (entry & 0x80000000u) != 0u;
     if (!bad) bad = src + stored > in_bytes || stored > (uint32_t)CS || (israw && stored < o) ||
@@ -409,9 +438,11 @@ int rze_enc_impl(hipStream_t s, const void* d_in, size_t nbytes, int sew, void* 
     launch_k("rze_enc_kernel", ek, dim3(nc), dim3(256), 0, s, in, nbytes, out, scratch, sizes, clean, nc,
              sew ? nfull : (uint64_t)0);
     FZH_CHECK_LAUNCH("rze_enc_kernel");
-    if (int rc = scan_u32(s, clean, offs, nc, nullptr, offs + nc, w + L.o_scan)) return rc;
-    launch_k("rze_pack_kernel", rze_pack_kernel, dim3((unsigned)cdiv(nc, 4)), dim3(256), 0, s, in, nbytes, scratch,
-             sizes, offs, out, d_out_size, nc, sew ? nfull : (uint64_t)0);
+    const bool inline_offs = nc <= PACK_INLINE;
+    if (!inline_offs)
+        if (int rc = scan_u32(s, clean, offs, nc, nullptr, offs + nc, w + L.o_scan)) return rc;
+    launch_k("rze_pack_kernel", rze_pack_kernel, dim3(nc), dim3(256), 0, s, in, nbytes, scratch,
+             sizes, clean, inline_offs ? nullptr : offs, out, d_out_size, nc, sew ? nfull : (uint64_t)0);
     FZH_CHECK_LAUNCH("rze_pack_kernel");
     return FZH_OK;
 }
@@ -426,10 +457,10 @@ int rze_dec_impl(hipStream_t s, const void* d_in, size_t in_bytes, void* d_out, 
     if (!d_status) return fail(FZH_E_INVALID, "RZE: null status pointer");
     const uint32_t nc = (uint32_t)cdiv(orig_bytes, CS);
     const uint64_t nfull = orig_bytes / CS;
-    uint32_t* offs = (uint32_t*)d_ws;
+    uint32_t* offs = nc <= PACK_INLINE ? nullptr : (uint32_t*)d_ws;
     const uint8_t* in = (const uint8_t*)d_in;
     uint8_t* out = (uint8_t*)d_out;
-    launch_k("rze_offsets_kernel", rze_offsets_kernel, dim3(1), dim3(1024), 0, s, in, in_bytes, offs, nc);
+    if (offs) launch_k("rze_offsets_kernel", rze_offsets_kernel, dim3(1), dim3(1024), 0, s, in, in_bytes, offs, nc);
     auto* dk = sew == 1 ? rze_dec_kernel<1> : sew == 2 ? rze_dec_kernel<2> : rze_dec_kernel<0>;
     launch_k("rze_dec_kernel", dk, dim3(nc), dim3(256), 0, s, in, in_bytes, out, orig_bytes, d_status, offs, nc,
              sew ? nfull : (uint64_t)0);

@@ -373,6 +373,16 @@ typedef struct fzh_readback_entry {
 } fzh_readback_entry;
 int fzh_readback(void* stream, const fzh_readback_entry* entries, int n, void* d_mapped_dst);
 
+/* fzh_concat followed by fzh_readback, in one launch (the pipeline's last
+ * step of compress: Pipeline::writeConcatBuffer, compressor.cpp:805-851,
+ * plus the size readbacks the reference does with cudaMemcpy).  Entries may
+ * read the concat header in d_out and the total; they see the values this
+ * call writes.  No workspace.                                              */
+int fzh_concat_readback(void* stream, const fzh_concat_slot* slots, int n, void* d_out,
+                        uint64_t out_cap, uint64_t* d_total,
+                        const fzh_readback_entry* entries, int n_entries,
+                        void* d_mapped_dst);
+
 /* ---- Device helpers for bindings (not on the hot path) ------------------- */
 int fzh_device_alloc(void** d_ptr, size_t bytes);
 int fzh_device_free(void* d_ptr);

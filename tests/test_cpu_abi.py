@@ -78,3 +78,4 @@ def test_invalid_arguments_fail_before_any_launch():
     assert L.fzh_last_error()
     slots = (C.c_byte * 40)()
     assert L.fzh_concat(None, slots, 33, 16, 1 << 20, 16, 16, 4096) == -1
+    assert L.fzh_concat_readback(None, slots, 33, 16, 1 << 20, 16, None, 0, None) == -1

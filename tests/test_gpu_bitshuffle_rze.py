@@ -66,6 +66,34 @@ def test_rze_matches_oracle(gpu, orc, name):
     np.testing.assert_array_equal(gpu.rze_dec(s_orc, data.size), data)
 
 
+def _many_chunks(nbytes, seed):
+    """Sparse chunks with every fifth chunk random (stored raw) and every
+    seventh all zero."""
+    rng = np.random.default_rng(seed)
+    b = rng.integers(0, 256, nbytes, dtype=np.uint8)
+    r = rng.integers(0, 256, nbytes, dtype=np.uint8)
+    b[(r & 0xF0) != 0] = 0
+    full = nbytes // 16384
+    rows = b[: full * 16384].reshape(full, 16384)
+    rows[::5] = rng.integers(0, 256, (rows[::5].shape[0], 16384), dtype=np.uint8)
+    rows[::7] = 0
+    return b
+
+
+@pytest.mark.parametrize("nbytes", [16384 * 4096, 16384 * 4096 + 1000, 16384 * 4200])
+def test_rze_chunk_offsets_both_ways(gpu, orc, nbytes):
+    """Up to 4096 chunks every workgroup sums the sizes before its chunk; from
+    4097 on the offsets come from a separate scan.  Same bytes as the oracle
+    either way, plain and bit-shuffled."""
+    data = _many_chunks(nbytes, nbytes)
+    s_gpu = gpu.rze_enc(data)
+    np.testing.assert_array_equal(s_gpu, orc.rze_enc(data))
+    np.testing.assert_array_equal(gpu.rze_dec(s_gpu, data.size), data)
+    got = gpu.rze_enc(data, shuffle_width=2)
+    np.testing.assert_array_equal(got, orc.rze_enc(orc.bitshuffle_enc(data, 16384, 2)))
+    np.testing.assert_array_equal(gpu.rze_dec(got, data.size, shuffle_width=2), data)
+
+
 def test_rze_all_zero_chunk_known_answer(gpu):
     s = gpu.rze_enc(np.zeros(16384, np.uint8))
     assert s.size <= 17  # test_rze_stage.cpp:197-214

@@ -76,6 +76,47 @@ def test_inverse_accepts_any_outlier_order(gpu, orc, dims):
     np.testing.assert_array_equal(y_gpu.view(np.uint32), y_ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("name,dims,n", CASES + [("2d_large", (1000, 700, 1), None), ("1d_large", (300000, 1, 1), None)])
+def test_inverse_short_outlier_lists(gpu, orc, name, dims, n):
+    """Lists of at most 256 outliers are read by every tile directly (no
+    binning launches); lists beyond that are binned.  Both give the oracle's
+    bits, for a capacity equal to the count (as a stored stream has it), for
+    a larger capacity, and in any outlier order."""
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    nx, ny, nz = dims
+    cover = nx * ny * nz
+    n = cover if n is None else n
+    x = make_field(rng, cover, n)
+    ebx2_r, ebx2 = eb_params(1e-3)
+    c, e, i, cnt = orc.lorenzo_fwd(x, dims, ebx2_r, 512, True, 2, n, n)
+    for k in sorted({min(cnt, 1), min(cnt, 255), min(cnt, 256), min(cnt, 257), cnt}):
+        ek, ik = e[:k], i[:k]
+        for cap in sorted({k, max(k, 256), max(k, 257)}):
+            want = orc.lorenzo_inv(c, dims, ek, ik, k, ebx2, True, cap)
+            got = gpu.lorenzo_inv(c, dims, ek, ik, k, ebx2, True, cap)
+            np.testing.assert_array_equal(got[:cover].view(np.uint32), want[:cover].view(np.uint32))
+        perm = rng.permutation(k)
+        got = gpu.lorenzo_inv(c, dims, ek[perm], ik[perm], k, ebx2, True, k)
+        np.testing.assert_array_equal(got[:cover].view(np.uint32), want[:cover].view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1024 * 16384, 1024 * 16384 + 1])
+def test_outlier_compaction_both_ways(gpu, orc, n):
+    """Up to 16384 tiles the place kernel sums the outlier counts before its
+    tiles itself; one tile more and the counts are scanned first.  Both give
+    the oracle's outlier list, in the same order."""
+    rng = np.random.default_rng(n)
+    x = make_field(rng, n, n)
+    ebx2_r, ebx2 = eb_params(1e-3)
+    cap = n // 5
+    oc, oe, oi, ocnt = orc.lorenzo_fwd(x, (n, 1, 1), ebx2_r, 512, True, 2, cap, n)
+    gc, ge, gi, gcnt = gpu.lorenzo_fwd(x, (n, 1, 1), ebx2_r, 512, True, 2, cap, n)
+    assert gcnt == ocnt > 256
+    np.testing.assert_array_equal(gc, oc)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(ge.view(np.uint32), oe.view(np.uint32))
+
+
 @pytest.mark.parametrize("dims", [(1 << 20, 1, 1), (1024, 768, 1), (96, 80, 64)])
 def test_error_bound_round_trip(gpu, dims):
     """Values stay where float32 resolves the bound (ulp(max|x|) << eb)."""

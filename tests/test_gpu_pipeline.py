@@ -238,13 +238,14 @@ def test_empty_and_errors():
             p.decompress(0x1000, 64)
 
 
-@pytest.mark.parametrize("case", ["1d_rle", "2d_rze", "3d_rze"])
+@pytest.mark.parametrize("case", ["1d_rle", "1d_rle_many_tiles", "2d_rze", "3d_rze"])
 def test_strict_error_bound(case):
     """Strict mode: codes/outliers/coder streams identical to the plain chain,
     plus the patch list = exactly the elements the reference arithmetic
-    reconstructs outside eb; every decompressed value is within eb."""
-    if case == "1d_rle":
-        x = fields.sinusoid_1d(1 << 20, seed=_seed("strict1"))
+    reconstructs outside eb; every decompressed value is within eb.  The
+    many-tiles case (16400 tiles) takes the scanned compaction path."""
+    if case.startswith("1d_rle"):
+        x = fields.sinusoid_1d(1 << 20 if case == "1d_rle" else 1024 * 16400, seed=_seed("strict1"))
         dims = (x.size, 1, 1)
         want, info = chains.lorenzo_rle(x)
     elif case == "2d_rze":
@@ -257,10 +258,11 @@ def test_strict_error_bound(case):
         x = fields.power_law_field(shape, -11.0 / 3.0, _seed("strict3")).astype(np.float32).ravel()
         dims = (56, 40, 24)
         want, info = chains.lorenzo_bitshuffle_rze(x, dims, user_eb=1e-5, cap=0.2)
-    recon = chains.reconstruct(info, dims if case != "1d_rle" else (0, 1, 1), zigzag=case != "1d_rle")
+    is_1d = case.startswith("1d_rle")
+    recon = chains.reconstruct(info, dims if not is_1d else (0, 1, 1), zigzag=not is_1d)
     pidx, pval = chains.strict_patches(x, recon, info["abs_eb"], dims)
     with Pipeline(x.nbytes) as p:
-        if case == "1d_rle":
+        if is_1d:
             lz = p.add_lorenzo(1e-4, mode="ABS", outlier_capacity=0.2)
             co = p.add_rle(2)
         else:
@@ -269,12 +271,12 @@ def test_strict_error_bound(case):
             bs = p.add_bitshuffle(16384, 2)
             co = p.add_rze()
         p.set_strict(lz, True)
-        if case != "1d_rle":
+        if not is_1d:
             p.connect(bs, lz, "codes")
             p.connect(co, bs)
         else:
             p.connect(co, lz, "codes")
-        if case != "1d_rle":
+        if not is_1d:
             p.set_dims(*dims)
         p.finalize()
         blob = p.compress_array(x)
