@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) vb_partial_kernel(const Plan* __restrict_
 
 __global__ void __launch_bounds__(256) vb_finalize_kernel(const Plan* __restrict__ pp) {
     const Plan& p = *pp;
-    const uint32_t f = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t f = blockIdx.x * 4 + wave_in_wg();
     if (f >= p.F) return;
     float gmn = INFINITY, gmx = -INFINITY;
     if (p.mode != 0) {
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) lz_batch_kernel(const Plan* __restrict__ 
     const Plan& p = *pp;
     const uint32_t waves = gridDim.x * 4;
     const uint32_t nu = (p.T + G - 1) / G;
-    for (uint32_t u = blockIdx.x * 4 + (threadIdx.x >> 6); u < nu; u += waves) {
+    for (uint32_t u = blockIdx.x * 4 + wave_in_wg(); u < nu; u += waves) {
         const uint32_t g0 = u * G, g1 = min(p.T, g0 + G);
         uint32_t f = find_field(p.tile0, p.F, g0);
         for (uint32_t g = g0; g < g1; g++) {
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(256) pack_batch_kernel(const Plan* __restrict_
     const Plan& p = *pp;
     const uint32_t waves = gridDim.x * 4;
     const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
-    for (uint32_t gc = blockIdx.x * 4 + (threadIdx.x >> 6); gc < p.NC; gc += waves) {
+    for (uint32_t gc = blockIdx.x * 4 + wave_in_wg(); gc < p.NC; gc += waves) {
         const uint32_t f = find_field(p.chunk0, p.F, gc);
         if (p.results[f].flags & kOverflowCapacity) continue;
         const Field& fd = p.fields[f];
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(256) scatter_batch_kernel(const Plan* __restri
     const uint32_t waves = gridDim.x * 4;
     const int l = lane_id();
     const uint32_t nblk = (p.T + 63) / 64;
-    for (uint32_t blk = blockIdx.x * 4 + (threadIdx.x >> 6); blk < nblk; blk += waves) {
+    for (uint32_t blk = blockIdx.x * 4 + wave_in_wg(); blk < nblk; blk += waves) {
         const uint32_t gl = blk * 64 + (uint32_t)l;
         uint32_t c = 0, pc = 0;
         if (gl < p.T) {

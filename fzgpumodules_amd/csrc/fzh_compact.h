@@ -36,7 +36,7 @@ constexpr uint32_t COMPACT_INLINE = 16384;
 template <int NL, bool INL>
 __global__ void __launch_bounds__(256)
 place_kernel(CompactList L0, CompactList L1, uint32_t T, uint32_t E, const uint32_t* __restrict__ excl) {
-    const int w = threadIdx.x >> 6;
+    const int w = (int)wave_in_wg();
     const uint32_t wave = blockIdx.x * 4 + (uint32_t)w;
     const int l = lane_id();
     const uint32_t tb = wave * 32;

@@ -68,6 +68,13 @@ constexpr int kWave = 64;
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// This thread's wave within its workgroup, as a wave-uniform value: work
+// indices derived from it (tiles, chunks) then live in scalar registers, so
+// their divisions, bounds tests and base addresses are scalar work.
+__device__ __forceinline__ uint32_t wave_in_wg() {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // Number of set bits of `mask` in lanes below this one.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),

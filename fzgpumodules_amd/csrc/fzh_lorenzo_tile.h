@@ -86,6 +86,26 @@ struct StrictEb {
     float eb, ebx2;
 };
 
+// Margins for deciding |fl(s*ebx2) - x| <= eb from the FP32 difference:
+// the rounded difference is within a relative 2^-24 of the exact one, so
+// below lo (<= eb (1 - 2^-23)) the exact difference is within eb and above
+// hi (>= eb (1 + 2^-23)) it is not; between them (and for NaN) the exact
+// FP64 test decides.
+struct StrictBand {
+    float lo, hi;
+};
+
+__device__ __forceinline__ StrictBand strict_band(StrictEb se) {
+    return StrictBand{se.eb * 0.99999976f, se.eb * 1.00000024f};  // eb (1 -+ 2^-22), rounded
+}
+
+__device__ __forceinline__ bool needs_patch_fast(float x, float s, StrictEb se, StrictBand b) {
+    const float ef = fabsf(s * se.ebx2 - x);
+    if (ef < b.lo) return false;
+    if (ef > b.hi) return true;
+    return needs_patch(x, s, se.eb, se.ebx2);
+}
+
 // Zero codes[from, to) (tail beyond nx*ny[*nz] when the input was padded).
 template <typename TCode>
 __device__ __forceinline__ void zero_code_tail(TCode* codes, uint64_t from, uint64_t to) {
@@ -140,7 +160,7 @@ __device__ __forceinline__ TileCounts lz1d_tile(const float* __restrict__ in, ui
             const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
             const bool ok = fabsf(d[q][k]) < rad;
             c[q][k] = ok ? enc_code<TCode, ZZ>(d[q][k]) : (TCode)0;
-            if (!ok && e < n) mask |= 1u << (q * 4 + k);
+            mask |= (uint32_t)(!ok && e < n) << (q * 4 + k);
         }
     if (codes) {
 #pragma unroll
@@ -194,13 +214,14 @@ __device__ __forceinline__ TileCounts lz1d_tile(const float* __restrict__ in, ui
 #pragma unroll
             for (int k = 0; k < 4; k++) huge |= !(fabsf(s[q][k]) < kExactS);
         const bool tile_huge = __ballot(huge) != 0ull;
+        const StrictBand band = strict_band(se);
         uint32_t pm = 0;
 #pragma unroll
         for (int q = 0; q < 4; q++)
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const uint64_t e = base + 4 * (uint64_t)(l + 64 * q) + k;
-                if (e < nv && (tile_huge || needs_patch(xv[q][k], s[q][k], se.eb, se.ebx2))) pm |= 1u << (q * 4 + k);
+                if (e < nv && (tile_huge || needs_patch_fast(xv[q][k], s[q][k], se, band))) pm |= 1u << (q * 4 + k);
             }
         tc.pcnt = emit(pm, true, lp);
     }
@@ -210,29 +231,29 @@ __device__ __forceinline__ TileCounts lz1d_tile(const float* __restrict__ in, ui
 // ---------------------------------------------------------------------------
 // 2D: tile (tile % gx, tile / gx) of an nx x ny field; `last` zeroes the code
 // tail [nx*ny, n).  codes == nullptr: lists only.
+//
+// Interior tiles (the tile lies inside the field and its rows span less
+// than 2^32 bytes) run a variant with no bounds tests whose loads and code
+// stores address from the tile's scalar base with 32-bit lane offsets.
 // ---------------------------------------------------------------------------
-template <typename TCode, bool ZZ, bool STRICT>
-__device__ __forceinline__ TileCounts lz2d_tile(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n,
-                                                uint32_t tile, uint32_t gx, bool last, float r, float rad,
-                                                TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
+template <typename TCode, bool ZZ, bool STRICT, bool INTERIOR, bool CODES>
+__device__ __forceinline__ TileCounts lz2d_tile_v(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n,
+                                                  uint32_t tile, uint32_t gx, bool last, float r, float rad,
+                                                  TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
     const int l = lane_id();
     const int x = l & 31, h = l >> 5;
     const uint32_t bx = tile % gx, by = tile / gx;
     const uint32_t X = bx * 32 + x;
     const uint32_t Y0 = by * 32 + 16 * h;
     const bool xin = X < nx;
-    // interior tiles (wave-uniform) read with a running row pointer and no
-    // bounds tests; edge tiles test every row
-    const bool interior = bx * 32 + 32 <= nx && by * 32 + 32 <= ny;
-    const uint64_t o0 = (uint64_t)Y0 * nx + X;  // this lane's first element
+    const uint64_t tbase = (uint64_t)(by * 32) * nx + bx * 32;  // the tile's first element (scalar)
+    const uint32_t loff = (uint32_t)(16 * h) * nx + (uint32_t)x;  // this lane's first element in the tile
     float s[16], xv[16];
-    if (interior) {
-        const float* p = in + o0;
+    if constexpr (INTERIOR) {
+        const char* tp = reinterpret_cast<const char*>(in + tbase);
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            xv[i] = *p;
-            p += nx;
-        }
+        for (int i = 0; i < 16; i++)
+            xv[i] = *reinterpret_cast<const float*>(tp + 4u * (loff + (uint32_t)i * nx));
     } else {
 #pragma unroll
         for (int i = 0; i < 16; i++) {
@@ -250,18 +271,23 @@ __device__ __forceinline__ TileCounts lz2d_tile(const float* __restrict__ in, ui
     for (int i = 1; i < 16; i++) t[i] = s[i] - s[i - 1];
     uint32_t mask = 0;
     uint64_t bal[16];
-    uintptr_t cp = codes ? reinterpret_cast<uintptr_t>(codes + o0) : 0;  // running row pointer
-    const uintptr_t cstride = (uintptr_t)nx * sizeof(TCode);
+    char* ctp = CODES ? reinterpret_cast<char*>(codes + tbase) : nullptr;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         const float w = wave_shr1(t[i]);  // lane x - 1 (x > 0; lane 32's is unused)
         const float d = (x > 0) ? t[i] - w : t[i];
-        const bool valid = interior || (xin && (Y0 + i) < ny);
+        const bool valid = INTERIOR || (xin && (Y0 + i) < ny);
         const bool ok = fabsf(d) < rad;
-        if (valid && codes) *reinterpret_cast<TCode*>(cp) = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
-        cp += cstride;
+        if constexpr (CODES) {
+            const TCode c = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
+            if constexpr (INTERIOR) {
+                *reinterpret_cast<TCode*>(ctp + (uint32_t)sizeof(TCode) * (loff + (uint32_t)i * nx)) = c;
+            } else if (valid) {
+                codes[(uint64_t)(Y0 + i) * nx + X] = c;
+            }
+        }
         const bool is_out = valid && !ok;
-        if (is_out) mask |= 1u << i;
+        mask |= (uint32_t)is_out << i;
         bal[i] = __ballot(is_out);
         t[i] = d;  // keep the delta for the outlier pass
     }
@@ -292,7 +318,7 @@ __device__ __forceinline__ TileCounts lz2d_tile(const float* __restrict__ in, ui
         return tot;
     };
     TileCounts tc{emit(bal, mask, false, lo), 0u};
-    if (last && codes) zero_code_tail(codes, (uint64_t)nx * ny, n);
+    if (CODES && last) zero_code_tail(codes, (uint64_t)nx * ny, n);
     if constexpr (STRICT) {
         bool huge = false;
 #pragma unroll
@@ -300,16 +326,32 @@ __device__ __forceinline__ TileCounts lz2d_tile(const float* __restrict__ in, ui
         const bool tile_huge = __ballot(huge) != 0ull;
         uint32_t pm = 0;
         uint64_t pbal[16];
+        const StrictBand band = strict_band(se);
 #pragma unroll
         for (int i = 0; i < 16; i++) {
-            const bool valid = xin && (Y0 + i) < ny;
-            const bool p = valid && (tile_huge || needs_patch(xv[i], s[i], se.eb, se.ebx2));
-            if (p) pm |= 1u << i;
+            const bool valid = INTERIOR || (xin && (Y0 + i) < ny);
+            const bool p = valid && (tile_huge || needs_patch_fast(xv[i], s[i], se, band));
+            pm |= (uint32_t)p << i;
             pbal[i] = __ballot(p);
         }
         tc.pcnt = emit(pbal, pm, true, lp);
     }
     return tc;
+}
+
+template <typename TCode, bool ZZ, bool STRICT>
+__device__ __forceinline__ TileCounts lz2d_tile(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n,
+                                                uint32_t tile, uint32_t gx, bool last, float r, float rad,
+                                                TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
+    const uint32_t bx = tile % gx, by = tile / gx;
+    // 32 rows of the tile must span less than 2^32 bytes for 32-bit offsets
+    const bool interior = bx * 32 + 32 <= nx && by * 32 + 32 <= ny && nx < (1u << 25);
+    if (interior) {
+        if (codes) return lz2d_tile_v<TCode, ZZ, STRICT, true, true>(in, nx, ny, n, tile, gx, last, r, rad, codes, lo, lp, se);
+        return lz2d_tile_v<TCode, ZZ, STRICT, true, false>(in, nx, ny, n, tile, gx, last, r, rad, codes, lo, lp, se);
+    }
+    if (codes) return lz2d_tile_v<TCode, ZZ, STRICT, false, true>(in, nx, ny, n, tile, gx, last, r, rad, codes, lo, lp, se);
+    return lz2d_tile_v<TCode, ZZ, STRICT, false, false>(in, nx, ny, n, tile, gx, last, r, rad, codes, lo, lp, se);
 }
 
 struct Tiles {
@@ -355,7 +397,7 @@ __device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, ui
         const bool ok = fabsf(d[y]) < rad;
         if (valid && codes) codes[Z * plane + (uint64_t)(Y0 + y) * nx + X] = ok ? enc_code<TCode, ZZ>(d[y]) : (TCode)0;
         const bool is_out = valid && !ok;
-        if (is_out) mask |= 1u << y;
+        mask |= (uint32_t)is_out << y;
         bal[y] = __ballot(is_out);
     }
     // Order inside the tile: (z, y, x).
@@ -391,13 +433,14 @@ __device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, ui
 #pragma unroll
         for (int y = 0; y < 8; y++) huge |= !(fabsf(s[y]) < kExactS);
         const bool tile_huge = __ballot(huge) != 0ull;
+        const StrictBand band = strict_band(se);
         uint32_t pm = 0;
         uint64_t pbal[8];
 #pragma unroll
         for (int y = 0; y < 8; y++) {
             const bool valid = xz && (Y0 + y) < ny;
-            const bool p = valid && (tile_huge || needs_patch(xv[y], s[y], se.eb, se.ebx2));
-            if (p) pm |= 1u << y;
+            const bool p = valid && (tile_huge || needs_patch_fast(xv[y], s[y], se, band));
+            pm |= (uint32_t)p << y;
             pbal[y] = __ballot(p);
         }
         tc.pcnt = emit(pbal, pm, true, lp);

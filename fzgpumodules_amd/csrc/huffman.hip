@@ -166,7 +166,7 @@ huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint3
 __global__ void __launch_bounds__(256)
 huf_pack_kernel(const uint32_t* __restrict__ scells, const uint32_t* __restrict__ ncell,
                 uint8_t* __restrict__ out, uint32_t ent3, uint32_t ent4, uint32_t pardeg, uint32_t cells) {
-    const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t p = blockIdx.x * 4 + wave_in_wg();
     if (p >= pardeg) return;
     const int l = lane_id();
     const uint32_t e = reinterpret_cast<const uint32_t*>(out + ent3)[p];

@@ -42,7 +42,7 @@ template <int ND>
 __global__ void __launch_bounds__(256)
 lorenzo_kernel(const float* __restrict__ chunks, uint32_t nx, uint32_t ny, uint32_t nz, uint64_t T,
                uint32_t tpc, const float* __restrict__ d_eb, uint16_t* __restrict__ codes, Stats st) {
-    const uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t g = (uint64_t)blockIdx.x * 4 + wave_in_wg();
     if (g >= T) return;
     const uint64_t E = (uint64_t)nx * ny * nz;
     const uint32_t c = (uint32_t)(g / tpc), lt = (uint32_t)(g % tpc);
@@ -165,7 +165,7 @@ quant_kernel(const float* __restrict__ chunks, uint64_t E, const float* __restri
 __global__ void __launch_bounds__(256)
 huff_kernel(const uint16_t* __restrict__ codes, uint64_t E, uint32_t pardeg, uint64_t nparts,
             const uint8_t* __restrict__ lens, Stats st) {
-    const uint64_t g = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t g = (uint64_t)blockIdx.x * 4 + wave_in_wg();
     if (g >= nparts) return;
     const uint32_t c = (uint32_t)(g / pardeg), p = (uint32_t)(g % pardeg);
     const uint64_t s0 = (uint64_t)p * kSub;

@@ -59,7 +59,7 @@ template <typename TCode, bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256)
 lz1d_fwd_kernel(const float* __restrict__ in, uint64_t n, uint64_t nv, float r_host, const float* __restrict__ d_r,
                 float rad, TCode* __restrict__ codes, TileLists tl, uint32_t T) {
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
     const uint64_t b = (uint64_t)tile * 1024;
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256)
 lz2d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n, float r_host,
                 const float* __restrict__ d_r, float rad, TCode* __restrict__ codes, TileLists tl, uint32_t gx,
                 uint32_t T) {
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
     const uint64_t b = (uint64_t)tile * 1024;
@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(256)
 lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, uint64_t n,
                 float r_host, const float* __restrict__ d_r, float rad, TCode* __restrict__ codes, TileLists tl,
                 Tiles tg, uint32_t T) {
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (tile >= T) return;
     const float r = d_r ? *d_r : r_host;
     const uint64_t b = (uint64_t)tile * 512;
@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(256)
 lz1d_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, const uint32_t* __restrict__ off,
                 const float* __restrict__ serr, const uint32_t* __restrict__ sloc, float* __restrict__ out,
                 uint32_t T, DirectOut dout) {
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (tile >= T) return;
     const int l = lane_id();
     const int h = l >> 5, lam = l & 31;
@@ -414,7 +414,7 @@ lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float
                 const uint32_t* __restrict__ off, const float* __restrict__ serr,
                 const uint32_t* __restrict__ sloc, float* __restrict__ out, uint32_t gx, uint32_t T,
                 DirectOut dout) {
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (tile >= T) return;
     const int l = lane_id();
     const int x = l & 31, h = l >> 5;
@@ -470,7 +470,7 @@ lz3d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, uint3
                 const uint32_t* __restrict__ off, const float* __restrict__ serr,
                 const uint32_t* __restrict__ sloc, float* __restrict__ out, Tiles tg, uint32_t T,
                 DirectOut dout) {
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (tile >= T) return;
     const int l = lane_id();
     const int x = l & 7, z = l >> 3;

@@ -40,7 +40,7 @@ quant_fwd_kernel(const float* __restrict__ in, uint64_t n, float r, float thresh
                  uint32_t* __restrict__ scnt, uint32_t T) {
     using SC = typename SignedOf<TCode>::type;
     constexpr int W = 8 * sizeof(TCode);
-    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = blockIdx.x * 4 + wave_in_wg();
     if (tile < T) {
         const int l = lane_id();
         const uint64_t base = (uint64_t)tile * QTILE;

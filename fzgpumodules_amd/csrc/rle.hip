@@ -34,7 +34,7 @@ rle_enc_runs_kernel(const T* __restrict__ in, uint64_t n, uint32_t* __restrict__
                     uint32_t ntiles) {
     constexpr int E = 16 / sizeof(T);       // elements per 16-byte load
     constexpr uint32_t TILE = 64 * E * STEPS;
-    const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = blockIdx.x * 4 + wave_in_wg();
     if (tile >= ntiles) return;
     const int l = lane_id();
     const uint64_t base = (uint64_t)tile * TILE;

@@ -209,7 +209,7 @@ __device__ __forceinline__ T half_wave_incl(T v, int s) {
 template <typename T, bool NB>
 __global__ void __launch_bounds__(256) tile_sum_kernel(const void* __restrict__ in, TileGeom g, T* __restrict__ agg) {
     const int l = lane_id(), s = l & 31;
-    const uint64_t tile = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 2 + (l >> 5);
+    const uint64_t tile = ((uint64_t)blockIdx.x * 4 + wave_in_wg()) * 2 + (l >> 5);
     uint64_t base = 0;
     uint32_t len = 0;
     const bool live = tile < g.ntiles && tile_origin(g, tile, base, len);
@@ -248,7 +248,7 @@ template <typename T, bool NB>
 __global__ void __launch_bounds__(256) tile_apply_kernel(const void* __restrict__ in, TileGeom g,
                                                          const T* __restrict__ tprefix, T* __restrict__ out) {
     const int l = lane_id(), s = l & 31;
-    const uint64_t tile = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 2 + (l >> 5);
+    const uint64_t tile = ((uint64_t)blockIdx.x * 4 + wave_in_wg()) * 2 + (l >> 5);
     uint64_t base = 0;
     uint32_t len = 0;
     const bool live = tile < g.ntiles && tile_origin(g, tile, base, len);

@@ -238,7 +238,22 @@ def test_empty_and_errors():
             p.decompress(0x1000, 64)
 
 
-@pytest.mark.parametrize("case", ["1d_rle", "1d_rle_many_tiles", "2d_rze", "3d_rze"])
+def _boundary_field(ny, nx, eb, seed):
+    """Values within a few ulps of eb from a quantisation point: the strict
+    test |fl(s*ebx2) - x| <= eb is decided near its boundary everywhere."""
+    rng = np.random.default_rng(seed)
+    eb = np.float32(eb)
+    ebx2 = np.float32(2) * eb
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    k = np.round(200 * np.sin(xx / 37.0) * np.cos(yy / 23.0)).astype(np.float32)
+    xh = k * ebx2
+    u = rng.integers(-4, 5, (ny, nx)).astype(np.float32) * np.float32(2.0 ** -23)
+    delta = eb * (np.float32(1) + u)
+    sign = np.where(rng.random((ny, nx)) < 0.5, np.float32(-1), np.float32(1))
+    return (xh + sign * delta).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", ["1d_rle", "1d_rle_many_tiles", "2d_rze", "2d_boundary", "3d_rze"])
 def test_strict_error_bound(case):
     """Strict mode: codes/outliers/coder streams identical to the plain chain,
     plus the patch list = exactly the elements the reference arithmetic
@@ -253,6 +268,11 @@ def test_strict_error_bound(case):
         x = cloud(ny, nx, _seed("strict2")).ravel()
         dims = (nx, ny, 1)
         want, info = chains.lorenzo_bitshuffle_rze(x, dims)
+    elif case == "2d_boundary":
+        ny, nx = 300, 700
+        x = _boundary_field(ny, nx, 1e-3, _seed("strict4")).ravel()
+        dims = (nx, ny, 1)
+        want, info = chains.lorenzo_bitshuffle_rze(x, dims, mode=0, user_eb=1e-3)
     else:
         shape = (24, 40, 56)
         x = fields.power_law_field(shape, -11.0 / 3.0, _seed("strict3")).astype(np.float32).ravel()
@@ -265,12 +285,19 @@ def test_strict_error_bound(case):
         if is_1d:
             lz = p.add_lorenzo(1e-4, mode="ABS", outlier_capacity=0.2)
             co = p.add_rle(2)
+        elif case == "2d_boundary":
+            lz = p.add_lorenzo(1e-3, mode="ABS", outlier_capacity=0.1, zigzag=True)
+            bs = p.add_bitshuffle(16384, 2)
+            co = p.add_rze()
         else:
             lz = p.add_lorenzo(1e-3 if case == "2d_rze" else 1e-5, mode="REL",
                                outlier_capacity=0.1 if case == "2d_rze" else 0.2, zigzag=True)
             bs = p.add_bitshuffle(16384, 2)
             co = p.add_rze()
-        p.set_strict(lz, True)
+        if case == "2d_boundary":
+            p.set_strict(lz, True, 1.0)  # about half the elements are patched
+        else:
+            p.set_strict(lz, True)
         if not is_1d:
             p.connect(bs, lz, "codes")
             p.connect(co, bs)
