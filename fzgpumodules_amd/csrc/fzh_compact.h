@@ -49,8 +49,10 @@ place_kernel(CompactList L0, CompactList L1, uint32_t T, uint32_t E, const uint3
         __shared__ uint32_t s_sum[4 * NL];
         __shared__ uint32_t s_wave[4][2];
         uint32_t before[NL];
-        wg256_sums_before<NL>(min(blockIdx.x * 128u, T),
-                              [&](int k, uint32_t i) { return L0.cnt[(uint64_t)k * T + i]; }, s_sum, before);
+        const uint32_t* lists[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) lists[k] = L0.cnt + (uint64_t)k * T;
+        wg256_sums_before_u32<NL>(lists, min(blockIdx.x * 128u, T), s_sum, before);
         const uint32_t inc = wave_incl_scan(c);
         const uint32_t h0 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 31);  // list 0 part of the wave
         const uint32_t h1 = lane63(inc) - h0;                                    // list 1 part

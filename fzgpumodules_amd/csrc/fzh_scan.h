@@ -66,6 +66,42 @@ __device__ __forceinline__ uint32_t sat_u32(uint64_t v, bool wrap) {
 // are short: every workgroup reads up to `end` elements per array, from L2).
 // Every thread of the workgroup must call it (it holds a barrier); s holds
 // 4 * NL u32 of LDS.
+// The same over NL u32 arrays p[k][0, end) read with 16-byte loads (the
+// arrays need only 4-byte alignment): fewer, wider loads shorten the
+// latency chain of a workgroup that sums thousands of counts.
+template <int NL>
+__device__ __forceinline__ void wg256_sums_before_u32(const uint32_t* const (&p)[NL], uint32_t end, uint32_t* s,
+                                                      uint32_t (&sum)[NL]) {
+    const uint32_t t = threadIdx.x;
+    uint32_t a[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(p[k]) >> 2) & 3u);
+        uint32_t head = (4u - mis) & 3u;
+        if (head > end) head = end;
+        const uint32_t nq = (end - head) / 4;
+        const uint32_t tail0 = head + 4 * nq;
+        a[k] = 0;
+        if (t < head) a[k] += p[k][t];
+        if (tail0 + t < end) a[k] += p[k][tail0 + t];
+        const uint4* q = reinterpret_cast<const uint4*>(p[k] + head);
+#pragma unroll 4
+        for (uint32_t i = t; i < nq; i += 256) {
+            const uint4 v = q[i];
+            a[k] += (v.x + v.y) + (v.z + v.w);
+        }
+    }
+    const int w = (int)(threadIdx.x >> 6);
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        a[k] = wave_sum(a[k]);
+        if (lane_id() == 0) s[4 * k + w] = a[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NL; k++) sum[k] = s[4 * k] + s[4 * k + 1] + s[4 * k + 2] + s[4 * k + 3];
+}
+
 template <int NL, typename Load>
 __device__ __forceinline__ void wg256_sums_before(uint32_t end, Load load, uint32_t* s, uint32_t (&sum)[NL]) {
     uint32_t a[NL];

@@ -407,24 +407,34 @@ lz1d_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, const u
     }
 }
 
-// 2D inverse, scan tree of lorenzo_quant_nd.cu:112-190.
-template <typename TCode, bool ZZ, bool DIRECT>
-__global__ void __launch_bounds__(256)
-lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float ebx2,
-                const uint32_t* __restrict__ off, const float* __restrict__ serr,
-                const uint32_t* __restrict__ sloc, float* __restrict__ out, uint32_t gx, uint32_t T,
-                DirectOut dout) {
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
-    if (tile >= T) return;
+// 2D inverse, scan tree of lorenzo_quant_nd.cu:112-190.  Interior tiles
+// (inside the field, rows spanning less than 2^32 bytes) load codes and
+// store values with 32-bit lane offsets from the tile's scalar base and no
+// bounds tests.
+template <typename TCode, bool ZZ, bool DIRECT, bool INTERIOR>
+__device__ __forceinline__ void lz2d_inv_tile(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float ebx2,
+                                              const uint32_t* __restrict__ off, const float* __restrict__ serr,
+                                              const uint32_t* __restrict__ sloc, float* __restrict__ out,
+                                              uint32_t gx, uint32_t tile, const DirectOut& dout) {
     const int l = lane_id();
     const int x = l & 31, h = l >> 5;
     const uint32_t bx = tile % gx, by = tile / gx;
     const uint32_t X = bx * 32 + x, Y0 = by * 32 + 16 * h;
     const bool xin = X < nx;
+    const uint64_t tbase = (uint64_t)(by * 32) * nx + bx * 32;
+    const uint32_t loff = (uint32_t)(16 * h) * nx + (uint32_t)x;
     float d[16];
+    if constexpr (INTERIOR) {
+        const char* cp = reinterpret_cast<const char*>(codes + tbase);
 #pragma unroll
-    for (int i = 0; i < 16; i++)
-        d[i] = (xin && Y0 + i < ny) ? dec_code<TCode, ZZ>(codes[(uint64_t)(Y0 + i) * nx + X]) : 0.0f;
+        for (int i = 0; i < 16; i++)
+            d[i] = dec_code<TCode, ZZ>(
+                *reinterpret_cast<const TCode*>(cp + (uint32_t)sizeof(TCode) * (loff + (uint32_t)i * nx)));
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            d[i] = (xin && Y0 + i < ny) ? dec_code<TCode, ZZ>(codes[(uint64_t)(Y0 + i) * nx + X]) : 0.0f;
+    }
     // local = (ty << 5) | tx ; lane = tx + 32*(ty >= 16), register = ty & 15
     FZH_APPLY_ANY(16, d, (int)((loc & 31u) | ((loc >> 4) & 32u)), (int)((loc >> 5) & 15u));
     // y: sequential inside each 8-row group
@@ -457,9 +467,30 @@ lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float
         }
         d[i] = v * ebx2;
     }
+    if constexpr (INTERIOR) {
+        char* op = reinterpret_cast<char*>(out + tbase);
 #pragma unroll
-    for (int i = 0; i < 16; i++)
-        if (xin && Y0 + i < ny) out[(uint64_t)(Y0 + i) * nx + X] = d[i];
+        for (int i = 0; i < 16; i++) *reinterpret_cast<float*>(op + 4u * (loff + (uint32_t)i * nx)) = d[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (xin && Y0 + i < ny) out[(uint64_t)(Y0 + i) * nx + X] = d[i];
+    }
+}
+
+template <typename TCode, bool ZZ, bool DIRECT>
+__global__ void __launch_bounds__(256)
+lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float ebx2,
+                const uint32_t* __restrict__ off, const float* __restrict__ serr,
+                const uint32_t* __restrict__ sloc, float* __restrict__ out, uint32_t gx, uint32_t T,
+                DirectOut dout) {
+    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
+    if (tile >= T) return;
+    const uint32_t bx = tile % gx, by = tile / gx;
+    if (bx * 32 + 32 <= nx && by * 32 + 32 <= ny && nx < (1u << 25))
+        lz2d_inv_tile<TCode, ZZ, DIRECT, true>(codes, nx, ny, ebx2, off, serr, sloc, out, gx, tile, dout);
+    else
+        lz2d_inv_tile<TCode, ZZ, DIRECT, false>(codes, nx, ny, ebx2, off, serr, sloc, out, gx, tile, dout);
 }
 
 // 3D inverse, scan tree of lorenzo_quant_nd.cu:281-364 (y sequential, x
