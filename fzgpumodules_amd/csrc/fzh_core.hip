@@ -274,19 +274,22 @@ concat_readback_kernel(ConcatArgs a, int n, uint8_t* __restrict__ out, uint64_t*
     };
     for (uint64_t k = t; k < hdr / 4; k += 256) reinterpret_cast<uint32_t*>(out)[k] = hword(k);
     if (t == 0) *total = s_total;
+    // the entries' words flattened over the workgroup's threads: one load
+    // round trip for all of them instead of one per entry
     const uintptr_t o0 = reinterpret_cast<uintptr_t>(out), t0 = reinterpret_cast<uintptr_t>(total);
-    for (int i = 0; i < nr; i++) {
-        const uint32_t words = r.e[i].bytes / 4;
-        const uintptr_t s0 = reinterpret_cast<uintptr_t>(r.e[i].src);
-        uint32_t* d = reinterpret_cast<uint32_t*>(mapped + r.e[i].dst_offset);
-        for (uint32_t w = t; w < words; w += 256) {
-            const uintptr_t at = s0 + 4ull * w;
-            uint32_t v;
-            if (at >= o0 && at < o0 + hdr) v = hword((at - o0) / 4);
-            else if (at >= t0 && at < t0 + 8) v = at == t0 ? (uint32_t)s_total : (uint32_t)(s_total >> 32);
-            else v = *reinterpret_cast<const uint32_t*>(at);
-            d[w] = v;
-        }
+    uint32_t all = 0;
+    for (int i = 0; i < nr; i++) all += r.e[i].bytes / 4;
+    for (uint32_t g = t; g < all; g += 256) {
+        int i = 0;
+        uint32_t base = 0;
+        while (base + r.e[i].bytes / 4 <= g) base += r.e[i++].bytes / 4;
+        const uint32_t w = g - base;
+        const uintptr_t at = reinterpret_cast<uintptr_t>(r.e[i].src) + 4ull * w;
+        uint32_t v;
+        if (at >= o0 && at < o0 + hdr) v = hword((at - o0) / 4);
+        else if (at >= t0 && at < t0 + 8) v = at == t0 ? (uint32_t)s_total : (uint32_t)(s_total >> 32);
+        else v = *reinterpret_cast<const uint32_t*>(at);
+        reinterpret_cast<uint32_t*>(mapped + r.e[i].dst_offset)[w] = v;
     }
 }
 
@@ -300,12 +303,17 @@ periodic_window_kernel(const float* __restrict__ base, uint64_t bx, uint64_t by,
     }
 }
 
+// The entries' words flattened over the wave's lanes (one load round trip
+// for all of them).
 __global__ void __launch_bounds__(64) readback_kernel(ReadbackArgs a, int n, uint8_t* __restrict__ dst) {
-    for (int i = 0; i < n; i++) {
-        const uint32_t words = a.e[i].bytes / 4;
-        const uint32_t* s = reinterpret_cast<const uint32_t*>(a.e[i].src);
-        uint32_t* d = reinterpret_cast<uint32_t*>(dst + a.e[i].dst_offset);
-        for (uint32_t w = threadIdx.x; w < words; w += 64) d[w] = s[w];
+    uint32_t all = 0;
+    for (int i = 0; i < n; i++) all += a.e[i].bytes / 4;
+    for (uint32_t g = threadIdx.x; g < all; g += 64) {
+        int i = 0;
+        uint32_t base = 0;
+        while (base + a.e[i].bytes / 4 <= g) base += a.e[i++].bytes / 4;
+        const uint32_t w = g - base;
+        reinterpret_cast<uint32_t*>(dst + a.e[i].dst_offset)[w] = reinterpret_cast<const uint32_t*>(a.e[i].src)[w];
     }
 }
 
