@@ -62,6 +62,8 @@ SIGNATURES = {
     "fzh_huffman_encode_fine": (_I, [_P, _P, _Z, _I, _U32, _P, _P, _P, _Z, _P, _P, _Z]),
     "fzh_huffman_decode": (_I, [_P, _P, _Z, _I, _P, _Z, _P]),
     "fzh_quant_abs_fwd_f32": (_I, [_P, _P, _Z, _F, _F, _U32, _I, _I, _P, _P, _P, _P, _Z, _P, _Z]),
+    "fzh_quant_abs_fwd_inplace_f32": (_I, [_P, _P, _Z, _F, _F, _U32, _P]),
+    "fzh_quant_abs_inv_inplace_f32": (_I, [_P, _P, _Z, _F, _U32, _P]),
     "fzh_quant_abs_fwd_ws_bytes": (_Z, [_Z]),
     "fzh_quant_abs_inv_f32": (_I, [_P, _P, _Z, _F, _I, _I, _P, _P, _P, _Z, _P]),
     "fzh_max_u32_ws_bytes": (_Z, []),
@@ -85,6 +87,7 @@ SIGNATURES = {
     "fzp_add_rle": (_I, [_P, _I, C.POINTER(_I)]),
     "fzp_add_huffman": (_I, [_P, _I, _U32, C.POINTER(_I)]),
     "fzp_add_quantizer": (_I, [_P, _F, _I, _U32, _F, _I, _I, _F, C.POINTER(_I)]),
+    "fzp_set_quantizer_outliers": (_I, [_P, _I, _F, _I]),
     "fzp_add_bitpack": (_I, [_P, _I, _I, C.POINTER(_I)]),
     "fzp_connect": (_I, [_P, _I, _I, C.c_char_p]),
     "fzp_set_dims": (_I, [_P, _Z, _Z, _Z]),

@@ -191,6 +191,20 @@ int fzp_add_quantizer(fzp_pipeline* p, float eb, int eb_mode, uint32_t radius, f
     });
 }
 
+int fzp_set_quantizer_outliers(fzp_pipeline* p, int id, float threshold, int inplace) {
+    return guard([&] {
+        fz::Stage* s = stage_at(p, id);
+        auto set = [&](auto* q) {
+            q->setOutlierThreshold(threshold);
+            q->setInplaceOutliers(inplace != 0);
+        };
+        if (auto* a = dynamic_cast<fz::QuantizerStage<float, uint8_t>*>(s)) set(a);
+        else if (auto* b = dynamic_cast<fz::QuantizerStage<float, uint16_t>*>(s)) set(b);
+        else if (auto* c = dynamic_cast<fz::QuantizerStage<float, uint32_t>*>(s)) set(c);
+        else throw std::invalid_argument("stage is not a Quantizer stage");
+    });
+}
+
 int fzp_add_bitpack(fzp_pipeline* p, int eb, int nbits, int* id) {
     return guard([&] {
         need(p);

@@ -526,6 +526,7 @@ template <typename TI, typename TC>
 std::vector<size_t> QuantizerStage<TI, TC>::estimateOutputSizes(const std::vector<size_t>& in) const {
     if (is_inverse_) return {num_elements_ * sizeof(TI)};
     const size_t n = in.empty() ? 0 : in[0] / sizeof(TI);
+    if (isInplaceMode()) return {n * sizeof(TC)};
     const size_t m = maxOutliers(n);
     return {n * sizeof(TC), m * sizeof(TI), m * 4, 4};
 }
@@ -543,6 +544,23 @@ void QuantizerStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const
                                      const std::vector<void*>& outputs, const std::vector<size_t>& sizes) {
     if (config_.eb_mode == ErrorBoundMode::REL)
         throw std::runtime_error("QuantizerStage: REL (log-space) mode is not provided by this build");
+    if (isInplaceMode()) {  // quantizer.cu:396-452 / 492-615, one port each way
+        if (!config_.zigzag_codes)
+            throw std::runtime_error("QuantizerStage: setInplaceOutliers(true) requires setZigzagCodes(true)");
+        if (sizeof(TC) != sizeof(TI))
+            throw std::runtime_error("QuantizerStage: setInplaceOutliers(true) requires sizeof(TCode)==sizeof(TInput) "
+                                     "(use QuantizerStage<float, uint32_t>)");
+    }
+    if (is_inverse_ && isInplaceMode()) {
+        if (inputs.empty() || outputs.empty() || sizes.empty())
+            throw std::runtime_error("QuantizerStage (inverse, inplace): requires 1 input and 1 output");
+        const size_t n = sizes[0] / sizeof(TC);
+        actual_output_sizes_.assign(1, n * sizeof(TI));
+        throwIfFzh(fzh_quant_abs_inv_inplace_f32(stream, (const uint32_t*)inputs[0], n, 2.0f * computed_abs_eb_,
+                                                 (uint32_t)config_.quant_radius, (float*)outputs[0]),
+                   "QuantizerStage (inverse, inplace)");
+        return;
+    }
     if (is_inverse_) {
         if (inputs.size() < 4 || outputs.empty() || sizes.size() < 4)
             throw std::runtime_error("QuantizerStage (inverse): requires 4 inputs");
@@ -554,14 +572,15 @@ void QuantizerStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const
                    "QuantizerStage (inverse)");
         return;
     }
-    if (inputs.empty() || outputs.size() < 4 || sizes.empty())
-        throw std::runtime_error("QuantizerStage: requires 1 input and 4 outputs");
+    if (inputs.empty() || outputs.size() < (isInplaceMode() ? 1u : 4u) || sizes.empty())
+        throw std::runtime_error(isInplaceMode() ? "QuantizerStage (inplace): requires 1 input and 1 output"
+                                                 : "QuantizerStage: requires 1 input and 4 outputs");
     const size_t n = sizes[0] / sizeof(TI);
     num_elements_ = n;
     const size_t m = maxOutliers(n);
     clearReadbacks();
     d_count_ptr_ = nullptr;
-    if (n == 0) { actual_output_sizes_.assign(4, 0); return; }
+    if (n == 0) { actual_output_sizes_.assign(isInplaceMode() ? 1 : 4, 0); return; }
     if (config_.eb_mode == ErrorBoundMode::ABS) {
         computed_abs_eb_ = config_.error_bound;
         computed_value_base_ = 0.0f;
@@ -581,6 +600,14 @@ void QuantizerStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const
         computed_abs_eb_ = vb > 0.0f ? config_.error_bound * vb : config_.error_bound;
     }
     const float r = 1.0f / (2.0f * computed_abs_eb_);
+    if (isInplaceMode()) {
+        throwIfFzh(fzh_quant_abs_fwd_inplace_f32(stream, (const float*)inputs[0], n, r, config_.outlier_threshold,
+                                                 (uint32_t)config_.quant_radius, (uint32_t*)outputs[0]),
+                   "QuantizerStage (inplace)");
+        outlier_count_ = 0;
+        actual_output_sizes_ = {n * sizeof(TC)};
+        return;
+    }
     ws_.ensure(256 + fzh_quant_abs_fwd_ws_bytes(n), pool, stream);
     throwIfFzh(fzh_quant_abs_fwd_f32(stream, (const float*)inputs[0], n, r, config_.outlier_threshold,
                                      (uint32_t)config_.quant_radius, config_.zigzag_codes ? 1 : 0, (int)sizeof(TC),
@@ -619,6 +646,7 @@ size_t QuantizerStage<TI, TC>::serializeHeader(size_t, uint8_t* buf, size_t max_
     c.eb_mode = (uint8_t)config_.eb_mode;
     c.zigzag_codes = config_.zigzag_codes ? 1 : 0;
     c.outlier_threshold = config_.outlier_threshold;
+    c.inplace_outliers = config_.inplace_outliers ? 1 : 0;
     std::memcpy(buf, &c, sizeof(c));
     return sizeof(c);
 }
@@ -635,7 +663,14 @@ void QuantizerStage<TI, TC>::deserializeHeader(const uint8_t* buf, size_t size) 
     outlier_count_ = c.outlier_count;
     config_.eb_mode = (ErrorBoundMode)c.eb_mode;
     config_.zigzag_codes = c.zigzag_codes != 0;
-    config_.outlier_threshold = c.outlier_threshold;
+    // quantizer.cu:714-745: headers shorter than 36 bytes predate the last two fields
+    if (size >= sizeof(QuantizerConfig)) {
+        config_.outlier_threshold = c.outlier_threshold;
+        config_.inplace_outliers = c.inplace_outliers != 0;
+    } else {
+        config_.outlier_threshold = INFINITY;
+        config_.inplace_outliers = false;
+    }
 }
 
 template class QuantizerStage<float, uint8_t>;

@@ -9,6 +9,9 @@
 // Outliers come out in index order: each 1024-element tile writes its list
 // to a scratch slot and a compaction kernel places the lists (the reference
 // appends with atomics).
+// In-place outliers (quantizer.cu:83-104, 147-168): u32 codes, valid ones
+// zigzag(q) in [0, 2 radius), an outlier's code is the raw bits of x; the
+// inverse takes (code >> 1) >= radius as an outlier.
 //
 // Bitpack (modules/coders/bitpack/bitpack_stage.cu:30-124): the low `nbits`
 // bits of each element, nbits in {1,2,4,8,16,32}; sub-byte: element k of a
@@ -109,6 +112,35 @@ quant_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, float*
         if constexpr (ZZ) q = (int)(SC)((TCode)(c >> 1) ^ (TCode)(-(SC)(c & 1u)));
         else q = (int)(SC)c;
         out[i] = (float)q * ebx2;
+    }
+}
+
+// Grid-stride, one element per thread per step.
+__global__ void __launch_bounds__(256)
+quant_inplace_fwd_kernel(const float* __restrict__ in, uint64_t n, float r, float threshold, int rad,
+                         uint32_t* __restrict__ codes) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const float x = in[i];
+        const int q = f2i_rn(x * r);
+        const bool ok = q > -rad && q < rad && fabsf(x) < threshold;
+        codes[i] = ok ? (uint32_t)((q << 1) ^ (q >> 31)) : __float_as_uint(x);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+quant_inplace_inv_kernel(const uint32_t* __restrict__ codes, uint64_t n, float ebx2, uint32_t rad,
+                         float* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const uint32_t c = codes[i];
+        if ((c >> 1) >= rad) {
+            out[i] = __uint_as_float(c);
+        } else {
+            const int ci = (int)c;
+            const int q = (ci & 1) ? -((ci + 1) >> 1) : (ci >> 1);
+            out[i] = (float)q * ebx2;
+        }
     }
 }
 
@@ -273,6 +305,27 @@ int fzh_quant_abs_inv_f32(void* stream, const void* d_codes, size_t n, float ebx
         scatter_assign_kernel<<<ob, 256, 0, s>>>(d_oval, d_oidx, d_ocount, max_outliers, n, d_out);
     }
     FZH_CHECK_LAUNCH("quantizer inverse");
+    return FZH_OK;
+}
+
+int fzh_quant_abs_fwd_inplace_f32(void* stream, const float* d_in, size_t n, float ebx2_r, float threshold,
+                                  uint32_t radius, uint32_t* d_codes) {
+    if (n == 0) return FZH_OK;
+    if (!d_in || !d_codes) return fail(FZH_E_INVALID, "quantizer (in place): null pointer");
+    if (radius == 0 || radius > (1u << 30)) return fail(FZH_E_INVALID, "quantizer (in place): radius must be in [1, 2^30]");
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256), 8192));
+    quant_inplace_fwd_kernel<<<blocks, 256, 0, S(stream)>>>(d_in, n, ebx2_r, threshold, (int)radius, d_codes);
+    FZH_CHECK_LAUNCH("quant_inplace_fwd_kernel");
+    return FZH_OK;
+}
+
+int fzh_quant_abs_inv_inplace_f32(void* stream, const uint32_t* d_codes, size_t n, float ebx2, uint32_t radius,
+                                  float* d_out) {
+    if (n == 0) return FZH_OK;
+    if (!d_codes || !d_out) return fail(FZH_E_INVALID, "quantizer (in place): null pointer");
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256), 8192));
+    quant_inplace_inv_kernel<<<blocks, 256, 0, S(stream)>>>(d_codes, n, ebx2, radius, d_out);
+    FZH_CHECK_LAUNCH("quant_inplace_inv_kernel");
     return FZH_OK;
 }
 

@@ -85,6 +85,10 @@ class Pipeline:
         return self._id("fzp_add_quantizer", float(error_bound), _mode(mode), int(quant_radius),
                         float(outlier_capacity), int(bool(zigzag)), int(code_bytes), float(value_base))
 
+    def set_quantizer_outliers(self, stage_id: int, threshold: float = float("inf"), inplace: bool = False) -> None:
+        """QuantizerStage::setOutlierThreshold / setInplaceOutliers."""
+        call("fzp_set_quantizer_outliers", self.h, stage_id, float(threshold), int(bool(inplace)))
+
     def add_bitshuffle(self, block_size=16384, element_width=2) -> int:
         return self._id("fzp_add_bitshuffle", int(block_size), int(element_width))
 

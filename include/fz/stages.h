@@ -508,6 +508,8 @@ class QuantizerStage : public Stage {
         float precomputed_value_base = 0.0f;
         bool zigzag_codes = false;
         float outlier_threshold = INFINITY;
+        // ABS/NOA: outliers as raw float bits in the codes (u32 codes, zigzag on)
+        bool inplace_outliers = false;
         Config() = default;
         Config(float eb, int radius = 32768, float cap = 0.05f) : error_bound(eb), quant_radius(radius), outlier_capacity(cap) {}
     };
@@ -517,10 +519,12 @@ class QuantizerStage : public Stage {
                  const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
     void postStreamSync(hipStream_t stream) override;
     std::string getName() const override { return "Quantizer"; }
-    size_t getNumInputs() const override { return is_inverse_ ? 4 : 1; }
-    size_t getNumOutputs() const override { return is_inverse_ ? 1 : 4; }
+    // quantizer.h:134-146: one port each way in in-place mode
+    size_t getNumInputs() const override { return !is_inverse_ ? 1 : (isInplaceMode() ? 1 : 4); }
+    size_t getNumOutputs() const override { return is_inverse_ ? 1 : (isInplaceMode() ? 1 : 4); }
     std::vector<std::string> getOutputNames() const override {
         if (is_inverse_) return {"reconstructed"};
+        if (isInplaceMode()) return {"codes"};
         return {"codes", "outlier_vals", "outlier_idxs", "outlier_count"};
     }
     std::vector<size_t> estimateOutputSizes(const std::vector<size_t>& in) const override;
@@ -539,8 +543,19 @@ class QuantizerStage : public Stage {
     void setValueBase(float vb) { config_.precomputed_value_base = vb; }
     void setZigzagCodes(bool on) { config_.zigzag_codes = on; }
     void setOutlierThreshold(float t) { config_.outlier_threshold = t; }
+    void setInplaceOutliers(bool on) { config_.inplace_outliers = on; }
+    TInput getErrorBound() const { return (TInput)config_.error_bound; }
+    int getQuantRadius() const { return config_.quant_radius; }
+    ErrorBoundMode getErrorBoundMode() const { return config_.eb_mode; }
+    float getValueBase() const { return config_.precomputed_value_base; }
+    float getOutlierCapacity() const { return config_.outlier_capacity; }
+    bool getZigzagCodes() const { return config_.zigzag_codes; }
+    float getOutlierThreshold() const { return config_.outlier_threshold; }
+    bool getInplaceOutliers() const { return config_.inplace_outliers; }
     uint16_t getStageTypeId() const override { return (uint16_t)StageType::QUANTIZER; }
     uint8_t getOutputDataType(size_t i) const override {
+        if (is_inverse_) return (uint8_t)DataType::FLOAT32;
+        if (isInplaceMode()) return (uint8_t)dataTypeOf<TCode>();
         switch (i) {
         case 0: return (uint8_t)dataTypeOf<TCode>();
         case 1: return (uint8_t)DataType::FLOAT32;
@@ -553,6 +568,7 @@ class QuantizerStage : public Stage {
     size_t getMaxHeaderSize(size_t) const override { return sizeof(QuantizerConfig); }
 
   private:
+    bool isInplaceMode() const { return config_.inplace_outliers && config_.eb_mode != ErrorBoundMode::REL; }
     size_t maxOutliers(size_t n) const { return (size_t)std::ceil((double)n * config_.outlier_capacity); }
     Config config_;
     bool is_inverse_ = false;

@@ -239,6 +239,16 @@ int fzh_quant_abs_inv_f32(void* stream, const void* d_codes, size_t n,
                           const uint32_t* d_outlier_idx,
                           const uint32_t* d_outlier_count,
                           size_t max_outliers, float* d_out);
+/* In-place outliers (QuantizerStage::setInplaceOutliers,
+ * quantizer_abs_fwd_inplace_kernel / quantizer_abs_inv_inplace_kernel,
+ * quantizer.cu:83-104, 147-168): u32 codes; a quantisable x stores
+ * zigzag(q) = (q << 1) ^ (q >> 31), any other x its raw IEEE-754 bits; the
+ * inverse reads (code >> 1) >= radius as raw bits.  No outlier lists.       */
+int fzh_quant_abs_fwd_inplace_f32(void* stream, const float* d_in, size_t n,
+                                  float ebx2_r, float threshold, uint32_t radius,
+                                  uint32_t* d_codes);
+int fzh_quant_abs_inv_inplace_f32(void* stream, const uint32_t* d_codes, size_t n,
+                                  float ebx2, uint32_t radius, float* d_out);
 
 /* ---- Bitpack -------------------------------------------------------------- */
 size_t fzh_max_u32_ws_bytes(void);

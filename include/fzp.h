@@ -61,6 +61,12 @@ int fzp_set_huffman_fine(fzp_pipeline* p, int stage_id, int fine);
 int fzp_add_quantizer(fzp_pipeline* p, float error_bound, int eb_mode, uint32_t quant_radius,
                       float outlier_capacity, int zigzag, int code_bytes, float value_base,
                       int* stage_id);
+/* QuantizerStage::setOutlierThreshold and ::setInplaceOutliers
+ * (quantizer.h:216-219): |x| >= threshold is always an outlier (INFINITY =
+ * off); inplace = 1 stores outliers as raw float bits in u32 zigzag codes
+ * (code_bytes 4, zigzag 1) and leaves one output port.  Call before
+ * finalize.                                                                */
+int fzp_set_quantizer_outliers(fzp_pipeline* p, int stage_id, float threshold, int inplace);
 /* nbits 0 = auto-detect from the maximum */
 int fzp_add_bitpack(fzp_pipeline* p, int elem_bytes, int nbits, int* stage_id);
 /* DifferenceStage<T, TOut> (diff.h:48-200): dtype = FZM DataType code of T
