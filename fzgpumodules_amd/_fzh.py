@@ -36,6 +36,8 @@ SIGNATURES = {
     "fzh_lorenzo_quant_inv_ws_bytes": (_Z, [_Z, _Z, _Z, _Z, _Z]),
     "fzh_lorenzo_quant_inv_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _F, _I, _I, _P, _P, _Z]),
     "fzh_lorenzo_quant_inv_bin": (_I, [_P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _P, _Z]),
+    "fzh_lorenzo_quant_inv_strict_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _F, _I, _I, _P, _P, _P, _Z,
+                                              _P, _P, _Z]),
     "fzh_lorenzo_quant_inv_apply_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _F, _I, _I, _P, _P, _Z]),
     "fzh_value_base_ws_bytes": (_Z, [_Z]),
     "fzh_value_base_f32": (_I, [_P, _P, _Z, _I, _F, _P, _P, _Z]),

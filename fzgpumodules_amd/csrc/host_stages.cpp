@@ -104,16 +104,21 @@ void LorenzoQuantStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, co
         const size_t max_out = sizes[1] / sizeof(TI);
         const float ebx2 = 2.0f * computed_abs_eb_;
         inv_ws_.ensure(fzh_lorenzo_quant_inv_ws_bytes(n, nx, ny, nz, max_out), pool, stream);
+        if (config_.strict_error_bound) {
+            // the patches go in with the inverse (one launch for short lists)
+            throwIfFzh(fzh_lorenzo_quant_inv_strict_f32(
+                           stream, inputs[0], n, nx, ny, nz, (const float*)inputs[1], (const uint32_t*)inputs[2],
+                           (const uint32_t*)inputs[3], max_out, ebx2, config_.zigzag_codes ? 1 : 0, (int)sizeof(TC),
+                           (const float*)inputs[4], (const uint32_t*)inputs[5], (const uint32_t*)inputs[6],
+                           sizes[4] / sizeof(TI), (float*)outputs[0], inv_ws_.ptr, inv_ws_.bytes),
+                       "LorenzoQuantStage (inverse)");
+            return;
+        }
         throwIfFzh(fzh_lorenzo_quant_inv_f32(stream, inputs[0], n, nx, ny, nz, (const float*)inputs[1],
                                              (const uint32_t*)inputs[2], (const uint32_t*)inputs[3], max_out, ebx2,
                                              config_.zigzag_codes ? 1 : 0, (int)sizeof(TC), (float*)outputs[0],
                                              inv_ws_.ptr, inv_ws_.bytes),
                    "LorenzoQuantStage (inverse)");
-        if (config_.strict_error_bound)
-            throwIfFzh(fzh_lorenzo_apply_patches(stream, (float*)outputs[0], (const float*)inputs[4],
-                                                 (const uint32_t*)inputs[5], (const uint32_t*)inputs[6],
-                                                 sizes[4] / sizeof(TI), n),
-                       "LorenzoQuantStage (inverse) patches");
         return;
     }
     if (inputs.empty() || outputs.size() < numPorts() || sizes.empty())

@@ -270,23 +270,28 @@ bin_place_kernel(const float* __restrict__ err, const uint32_t* __restrict__ idx
 // applies the entries that fall in its tile: no binning launches.
 constexpr uint32_t kDirectOutliers = 256;
 
+// A short (value, index) list read whole by every tile's wave: the
+// outliers (added to the decoded deltas) or, in strict mode, the patches
+// (assigned to the reconstructed values).  n_out: patch indices past the
+// tiles but below n_out are stored by tile 0's wave.
 struct DirectOut {
     const float* err;
     const uint32_t* idx;
     const uint32_t* count;
     uint32_t max_out;
     TileMap m;
+    uint64_t n_out;
 };
 
-#define FZH_APPLY_DIRECT(NREG, DVAR, LANE_OF, REG_OF)                                         \
+#define FZH_LIST_DIRECT(LIST, ASSIGN, NREG, DVAR, LANE_OF, REG_OF)                            \
     do {                                                                                      \
-        const uint32_t kk = dout.count ? min(*dout.count, dout.max_out) : 0u;                 \
+        const uint32_t kk = LIST.count ? min(*LIST.count, LIST.max_out) : 0u;                 \
         for (uint32_t j0 = 0; j0 < kk; j0 += 64) {                                            \
             const uint32_t jj = j0 + (uint32_t)l;                                             \
             uint32_t t_ = 0xFFFFFFFFu, loc_ = 0;                                              \
             float e_ = 0.0f;                                                                  \
             if (jj < kk) {                                                                    \
-                if (tile_of(dout.m, dout.idx[jj], t_, loc_)) e_ = dout.err[jj];               \
+                if (tile_of(LIST.m, LIST.idx[jj], t_, loc_)) e_ = LIST.err[jj];               \
                 else t_ = 0xFFFFFFFFu;                                                        \
             }                                                                                 \
             uint64_t hit = __ballot(t_ == tile);                                              \
@@ -299,10 +304,28 @@ struct DirectOut {
                 const int tr = (REG_OF);                                                      \
                 if (l == tl) {                                                                \
                     _Pragma("unroll") for (int rr = 0; rr < NREG; rr++)                       \
-                        if (rr == tr) DVAR[rr] = e + DVAR[rr];                                \
+                        if (rr == tr) DVAR[rr] = (ASSIGN) ? e : e + DVAR[rr];                 \
                 }                                                                             \
             }                                                                                 \
         }                                                                                     \
+    } while (0)
+
+#define FZH_APPLY_DIRECT(NREG, DVAR, LANE_OF, REG_OF) FZH_LIST_DIRECT(dout, false, NREG, DVAR, LANE_OF, REG_OF)
+
+// Strict-mode patches inside the inverse kernel (PATCH), and tile 0's
+// store of any patch index past the tiles.
+#define FZH_PATCH_ANY(NREG, DVAR, LANE_OF, REG_OF)                                           \
+    do {                                                                                     \
+        if constexpr (PATCH) {                                                               \
+            FZH_LIST_DIRECT(pin, true, NREG, DVAR, LANE_OF, REG_OF);                          \
+            if (tile == 0) {                                                                 \
+                const uint32_t kp = pin.count ? min(*pin.count, pin.max_out) : 0u;           \
+                for (uint32_t j = (uint32_t)l; j < kp; j += 64) {                            \
+                    const uint32_t q = pin.idx[j];                                           \
+                    if ((uint64_t)q >= pin.m.total && (uint64_t)q < pin.n_out) out[q] = pin.err[j]; \
+                }                                                                            \
+            }                                                                                \
+        }                                                                                    \
     } while (0)
 
 #define FZH_APPLY_ANY(NREG, DVAR, LANE_OF, REG_OF)                 \
@@ -315,11 +338,11 @@ struct DirectOut {
     } while (0)
 
 // 1D inverse, scan tree of lorenzo_quant.cu:17-87 (see header comment).
-template <typename TCode, bool ZZ, bool DIRECT>
+template <typename TCode, bool ZZ, bool DIRECT, bool PATCH>
 __global__ void __launch_bounds__(256)
 lz1d_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, const uint32_t* __restrict__ off,
                 const float* __restrict__ serr, const uint32_t* __restrict__ sloc, float* __restrict__ out,
-                uint32_t T, DirectOut dout) {
+                uint32_t T, DirectOut dout, DirectOut pin) {
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (tile >= T) return;
     const int l = lane_id();
@@ -393,9 +416,12 @@ lz1d_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, const u
         }
     }
 #pragma unroll
+    for (int i = 0; i < 16; i++) d[i] = d[i] * ebx2;
+    FZH_PATCH_ANY(16, d, (int)((loc >> 2) & 63u), (int)(((loc >> 8) << 2) | (loc & 3u)));
+#pragma unroll
     for (int q = 0; q < 4; q++) {
         const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
-        float4 v = make_float4(d[4 * q] * ebx2, d[4 * q + 1] * ebx2, d[4 * q + 2] * ebx2, d[4 * q + 3] * ebx2);
+        float4 v = make_float4(d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]);
         if (full) {
             *reinterpret_cast<float4*>(out + e0) = v;
         } else {
@@ -411,11 +437,12 @@ lz1d_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, const u
 // (inside the field, rows spanning less than 2^32 bytes) load codes and
 // store values with 32-bit lane offsets from the tile's scalar base and no
 // bounds tests.
-template <typename TCode, bool ZZ, bool DIRECT, bool INTERIOR>
+template <typename TCode, bool ZZ, bool DIRECT, bool PATCH, bool INTERIOR>
 __device__ __forceinline__ void lz2d_inv_tile(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float ebx2,
                                               const uint32_t* __restrict__ off, const float* __restrict__ serr,
                                               const uint32_t* __restrict__ sloc, float* __restrict__ out,
-                                              uint32_t gx, uint32_t tile, const DirectOut& dout) {
+                                              uint32_t gx, uint32_t tile, const DirectOut& dout,
+                                              const DirectOut& pin) {
     const int l = lane_id();
     const int x = l & 31, h = l >> 5;
     const uint32_t bx = tile % gx, by = tile / gx;
@@ -467,6 +494,7 @@ __device__ __forceinline__ void lz2d_inv_tile(const TCode* __restrict__ codes, u
         }
         d[i] = v * ebx2;
     }
+    FZH_PATCH_ANY(16, d, (int)((loc & 31u) | ((loc >> 4) & 32u)), (int)((loc >> 5) & 15u));
     if constexpr (INTERIOR) {
         char* op = reinterpret_cast<char*>(out + tbase);
 #pragma unroll
@@ -478,29 +506,29 @@ __device__ __forceinline__ void lz2d_inv_tile(const TCode* __restrict__ codes, u
     }
 }
 
-template <typename TCode, bool ZZ, bool DIRECT>
+template <typename TCode, bool ZZ, bool DIRECT, bool PATCH>
 __global__ void __launch_bounds__(256)
 lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float ebx2,
                 const uint32_t* __restrict__ off, const float* __restrict__ serr,
                 const uint32_t* __restrict__ sloc, float* __restrict__ out, uint32_t gx, uint32_t T,
-                DirectOut dout) {
+                DirectOut dout, DirectOut pin) {
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (tile >= T) return;
     const uint32_t bx = tile % gx, by = tile / gx;
     if (bx * 32 + 32 <= nx && by * 32 + 32 <= ny && nx < (1u << 25))
-        lz2d_inv_tile<TCode, ZZ, DIRECT, true>(codes, nx, ny, ebx2, off, serr, sloc, out, gx, tile, dout);
+        lz2d_inv_tile<TCode, ZZ, DIRECT, PATCH, true>(codes, nx, ny, ebx2, off, serr, sloc, out, gx, tile, dout, pin);
     else
-        lz2d_inv_tile<TCode, ZZ, DIRECT, false>(codes, nx, ny, ebx2, off, serr, sloc, out, gx, tile, dout);
+        lz2d_inv_tile<TCode, ZZ, DIRECT, PATCH, false>(codes, nx, ny, ebx2, off, serr, sloc, out, gx, tile, dout, pin);
 }
 
 // 3D inverse, scan tree of lorenzo_quant_nd.cu:281-364 (y sequential, x
 // Hillis-Steele over 8, z Hillis-Steele over 8).
-template <typename TCode, bool ZZ, bool DIRECT>
+template <typename TCode, bool ZZ, bool DIRECT, bool PATCH>
 __global__ void __launch_bounds__(256)
 lz3d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, uint32_t nz, float ebx2,
                 const uint32_t* __restrict__ off, const float* __restrict__ serr,
                 const uint32_t* __restrict__ sloc, float* __restrict__ out, Tiles tg, uint32_t T,
-                DirectOut dout) {
+                DirectOut dout, DirectOut pin) {
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (tile >= T) return;
     const int l = lane_id();
@@ -532,6 +560,7 @@ lz3d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, uint3
         }
         d[y] = v * ebx2;
     }
+    FZH_PATCH_ANY(8, d, (int)((loc & 7u) | ((loc >> 3) & 56u)), (int)((loc >> 3) & 7u));
 #pragma unroll
     for (int y = 0; y < 8; y++)
         if (xz && Y0 + y < ny) out[Z * plane + (uint64_t)(Y0 + y) * nx + X] = d[y];
@@ -723,7 +752,7 @@ int inv_bin(hipStream_t s, size_t n, size_t nx, size_t ny, size_t nz, const floa
 template <typename TCode>
 int inv_apply(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny, size_t nz, const float* err,
               const uint32_t* idx, const uint32_t* count, size_t max_out, float ebx2, int zz, float* out, void* ws,
-              size_t ws_bytes) {
+              size_t ws_bytes, const DirectOut* patches = nullptr) {
     const InvWs L = inv_layout(n, nx, ny, nz, max_out);
     if (ws_bytes < L.total) return fail(FZH_E_CAPACITY, "lorenzo inv: workspace too small");
     uint8_t* w = (uint8_t*)ws;
@@ -735,25 +764,39 @@ int inv_apply(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny, 
     const unsigned blocks = (unsigned)cdiv(L.T, 4);
     const TCode* c = (const TCode*)codes;
     const bool direct = max_out <= kDirectOutliers;
-    const DirectOut dout{err, idx, (err && idx) ? count : nullptr, (uint32_t)max_out, m};
-#define FZH_LZ_INV(ZZV, DV)                                                                             \
+    const DirectOut dout{err, idx, (err && idx) ? count : nullptr, (uint32_t)max_out, m, 0};
+    // short patch lists ride along with short outlier lists (one launch)
+    const bool fold = patches && patches->max_out > 0 && direct && patches->max_out <= kDirectOutliers;
+    DirectOut pin{};
+    if (fold) pin = DirectOut{patches->err, patches->idx, patches->count, patches->max_out, m, patches->n_out};
+#define FZH_LZ_INV(ZZV, DV, PV)                                                                         \
     if (nd == 1)                                                                                        \
-        launch_k("lz1d_inv_kernel", lz1d_inv_kernel<TCode, ZZV, DV>, dim3(blocks), dim3(256), 0, s, c, n, ebx2, \
-                 off, serr, sloc, out, (uint32_t)L.T, dout);                                            \
+        launch_k("lz1d_inv_kernel", lz1d_inv_kernel<TCode, ZZV, DV, PV>, dim3(blocks), dim3(256), 0, s, c, n, \
+                 ebx2, off, serr, sloc, out, (uint32_t)L.T, dout, pin);                                 \
     else if (nd == 2)                                                                                   \
-        launch_k("lz2d_inv_kernel", lz2d_inv_kernel<TCode, ZZV, DV>, dim3(blocks), dim3(256), 0, s, c,  \
-                 (uint32_t)nx, (uint32_t)ny, ebx2, off, serr, sloc, out, m.gx, (uint32_t)L.T, dout);    \
+        launch_k("lz2d_inv_kernel", lz2d_inv_kernel<TCode, ZZV, DV, PV>, dim3(blocks), dim3(256), 0, s, c, \
+                 (uint32_t)nx, (uint32_t)ny, ebx2, off, serr, sloc, out, m.gx, (uint32_t)L.T, dout, pin); \
     else                                                                                                \
-        launch_k("lz3d_inv_kernel", lz3d_inv_kernel<TCode, ZZV, DV>, dim3(blocks), dim3(256), 0, s, c,  \
+        launch_k("lz3d_inv_kernel", lz3d_inv_kernel<TCode, ZZV, DV, PV>, dim3(blocks), dim3(256), 0, s, c, \
                  (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, ebx2, off, serr, sloc, out, Tiles{m.gx, m.gy, 0}, \
-                 (uint32_t)L.T, dout);
+                 (uint32_t)L.T, dout, pin);
     if (zz) {
-        if (direct) { FZH_LZ_INV(true, true) } else { FZH_LZ_INV(true, false) }
+        if (fold) { FZH_LZ_INV(true, true, true) }
+        else if (direct) { FZH_LZ_INV(true, true, false) }
+        else { FZH_LZ_INV(true, false, false) }
     } else {
-        if (direct) { FZH_LZ_INV(false, true) } else { FZH_LZ_INV(false, false) }
+        if (fold) { FZH_LZ_INV(false, true, true) }
+        else if (direct) { FZH_LZ_INV(false, true, false) }
+        else { FZH_LZ_INV(false, false, false) }
     }
 #undef FZH_LZ_INV
     FZH_CHECK_LAUNCH("lorenzo inv");
+    if (patches && patches->max_out > 0 && !fold) {
+        const unsigned pb = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(patches->max_out, 256), 1024));
+        launch_k("apply_patches_kernel", apply_patches_kernel, dim3(pb), dim3(256), 0, s, out, patches->err,
+                 patches->idx, patches->count, (uint64_t)patches->max_out, patches->n_out);
+        FZH_CHECK_LAUNCH("apply_patches_kernel");
+    }
     return FZH_OK;
 }
 
@@ -900,6 +943,33 @@ int fzh_lorenzo_quant_inv_f32(void* stream, const void* d_codes, size_t n, size_
     return fzh_lorenzo_quant_inv_apply_f32(stream, d_codes, n, nx, ny, nz, d_outlier_err, d_outlier_idx,
                                            d_outlier_count, max_outliers, ebx2, zigzag, code_bytes, d_out, d_ws,
                                            ws_bytes);
+}
+
+int fzh_lorenzo_quant_inv_strict_f32(void* stream, const void* d_codes, size_t n, size_t nx, size_t ny, size_t nz,
+                                     const float* d_outlier_err, const uint32_t* d_outlier_idx,
+                                     const uint32_t* d_outlier_count, size_t max_outliers, float ebx2, int zigzag,
+                                     int code_bytes, const float* d_patch_val, const uint32_t* d_patch_idx,
+                                     const uint32_t* d_patch_count, size_t max_patches, float* d_out, void* d_ws,
+                                     size_t ws_bytes) {
+    if (code_bytes != 1 && code_bytes != 2) return fail(FZH_E_INVALID, "lorenzo inv: code_bytes must be 1 or 2");
+    if (int rc = check_dims(n, nx, ny, nz, "lorenzo inv")) return rc;
+    if (n == 0) return FZH_OK;
+    if (!d_codes || !d_out) return fail(FZH_E_INVALID, "lorenzo inv: null pointer");
+    if (max_patches > 0 && (!d_patch_val || !d_patch_idx || !d_patch_count))
+        return fail(FZH_E_INVALID, "lorenzo patches: null pointer");
+    if (max_patches > 0xFFFFFFFFull) return fail(FZH_E_INVALID, "lorenzo patches: more than 2^32 entries");
+    if (ndim_of(ny, nz) == 1 &&
+        (reinterpret_cast<uintptr_t>(d_out) % 16 != 0 || reinterpret_cast<uintptr_t>(d_codes) % 8 != 0))
+        return fail(FZH_E_INVALID, "lorenzo inv: 1D buffers must be 16/8-byte aligned");
+    if (int rc = fzh_lorenzo_quant_inv_bin(stream, n, nx, ny, nz, d_outlier_err, d_outlier_idx, d_outlier_count,
+                                           max_outliers, d_ws, ws_bytes))
+        return rc;
+    const DirectOut pl{d_patch_val, d_patch_idx, d_patch_count, (uint32_t)max_patches, TileMap{}, (uint64_t)n};
+    return code_bytes == 1
+               ? inv_apply<uint8_t>(S(stream), d_codes, n, nx, ny, nz, d_outlier_err, d_outlier_idx, d_outlier_count,
+                                    max_outliers, ebx2, zigzag, d_out, d_ws, ws_bytes, &pl)
+               : inv_apply<uint16_t>(S(stream), d_codes, n, nx, ny, nz, d_outlier_err, d_outlier_idx,
+                                     d_outlier_count, max_outliers, ebx2, zigzag, d_out, d_ws, ws_bytes, &pl);
 }
 
 size_t fzh_value_base_ws_bytes(size_t n) {

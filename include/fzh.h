@@ -107,6 +107,20 @@ int fzh_lorenzo_quant_fwd_strict_f32(void* stream, const float* d_in, size_t n,
                                      const float* d_eb_params, float* d_patch_val,
                                      uint32_t* d_patch_idx, uint32_t* d_patch_count,
                                      size_t max_patches, void* d_ws, size_t ws_bytes);
+/* fzh_lorenzo_quant_inv_f32 followed by fzh_lorenzo_apply_patches (n_out =
+ * n), for strict-mode streams (an extension; no counterpart in the
+ * reference): with at most 256 outliers and 256 patches a single launch,
+ * every tile's wave applying both short lists itself.  Same workspace as
+ * fzh_lorenzo_quant_inv_f32.                                               */
+int fzh_lorenzo_quant_inv_strict_f32(void* stream, const void* d_codes, size_t n,
+                                     size_t nx, size_t ny, size_t nz,
+                                     const float* d_outlier_err,
+                                     const uint32_t* d_outlier_idx,
+                                     const uint32_t* d_outlier_count, size_t max_outliers,
+                                     float ebx2, int zigzag, int code_bytes,
+                                     const float* d_patch_val, const uint32_t* d_patch_idx,
+                                     const uint32_t* d_patch_count, size_t max_patches,
+                                     float* d_out, void* d_ws, size_t ws_bytes);
 /* After the inverse: d_out[idx[i]] = val[i] for i < min(*count, max_patches)
  * and idx[i] < n_out.                                                      */
 int fzh_lorenzo_apply_patches(void* stream, float* d_out, const float* d_patch_val,

@@ -117,6 +117,50 @@ def test_outlier_compaction_both_ways(gpu, orc, n):
     np.testing.assert_array_equal(ge.view(np.uint32), oe.view(np.uint32))
 
 
+@pytest.mark.parametrize("name,dims,n", CASES + [("2d_large", (1000, 700, 1), None)])
+@pytest.mark.parametrize("npatch", [0, 1, 200, 256, 257, 1000])
+def test_strict_inverse_matches_inverse_then_patches(gpu, orc, name, dims, n, npatch):
+    """fzh_lorenzo_quant_inv_strict_f32 == fzh_lorenzo_quant_inv_f32 then
+    fzh_lorenzo_apply_patches, bit for bit: patches folded into the inverse
+    launch (short lists) or applied after it (long ones), including patch
+    indices past the tiles of a padded field."""
+    from fzgpumodules_amd._fzh import DeviceBuffer, call, lib, workspace
+    rng = np.random.default_rng(zlib.crc32(f"{name}-{npatch}".encode()))
+    nx, ny, nz = dims
+    cover = nx * ny * nz
+    n = cover if n is None else n
+    x = make_field(rng, cover, n)
+    ebx2_r, ebx2 = eb_params(1e-3)
+    c, e, i, cnt = orc.lorenzo_fwd(x, dims, ebx2_r, 512, True, 2, n, n)
+    k = min(cnt, 200)
+    pidx = np.sort(rng.choice(n, size=min(npatch, n), replace=False)).astype(np.uint32)
+    pval = rng.standard_normal(pidx.size).astype(np.float32)
+    dc = DeviceBuffer.from_array(c)
+    de, di = _dev_or_dummy(e[:k], np.float32), _dev_or_dummy(i[:k], np.uint32)
+    dn = DeviceBuffer.from_array(np.array([k], np.uint32))
+    pv, pi = _dev_or_dummy(pval, np.float32), _dev_or_dummy(pidx, np.uint32)
+    pn = DeviceBuffer.from_array(np.array([pidx.size], np.uint32))
+    ws = workspace(lib().fzh_lorenzo_quant_inv_ws_bytes(n, nx, ny, nz, k))
+    a, b = DeviceBuffer(4 * n), DeviceBuffer(4 * n)
+    call("fzh_memset", a.ptr, 0, 4 * n)
+    call("fzh_memset", b.ptr, 0, 4 * n)
+    call("fzh_lorenzo_quant_inv_f32", None, dc.ptr, n, nx, ny, nz, de.ptr, di.ptr, dn.ptr, k, np.float32(ebx2), 1, 2,
+         a.ptr, ws.ptr, ws.nbytes)
+    call("fzh_lorenzo_apply_patches", None, a.ptr, pv.ptr, pi.ptr, pn.ptr, pidx.size, n)
+    call("fzh_lorenzo_quant_inv_strict_f32", None, dc.ptr, n, nx, ny, nz, de.ptr, di.ptr, dn.ptr, k,
+         np.float32(ebx2), 1, 2, pv.ptr, pi.ptr, pn.ptr, pidx.size, b.ptr, ws.ptr, ws.nbytes)
+    np.testing.assert_array_equal(b.download(np.uint32, n), a.download(np.uint32, n))
+    want = orc.lorenzo_inv(c, dims, e[:k], i[:k], k, ebx2, True, k)
+    want[pidx] = pval
+    np.testing.assert_array_equal(b.download(np.uint32, n)[:cover], want[:cover].view(np.uint32))
+
+
+def _dev_or_dummy(a, dt):
+    from fzgpumodules_amd._fzh import DeviceBuffer
+    a = np.ascontiguousarray(a, dt)
+    return DeviceBuffer.from_array(a if a.size else np.zeros(1, dt))
+
+
 @pytest.mark.parametrize("dims", [(1 << 20, 1, 1), (1024, 768, 1), (96, 80, 64)])
 def test_error_bound_round_trip(gpu, dims):
     """Values stay where float32 resolves the bound (ulp(max|x|) << eb)."""
