@@ -561,8 +561,17 @@ std::vector<std::pair<const void*, size_t>> Pipeline::parseConcat(const void* d_
     const size_t hdr = round_up(4 + 8 * n, 16);
     if (input_size < hdr) throw std::runtime_error("decompress(): input smaller than the concat header");
     std::vector<uint8_t> h(4 + 8 * n);
-    hipck(hipMemcpyAsync(h.data(), d_input, h.size(), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
-    hipck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (reinterpret_cast<uintptr_t>(d_input) % 4 == 0 && round_up(h.size(), 4) <= kReadbackBytes) {
+        // one small kernel into the host-mapped readback buffer: a shorter
+        // round trip than a device-to-host copy
+        const fzh_readback_entry e{d_input, (uint32_t)round_up(h.size(), 4), 0};
+        throwIfFzh(fzh_readback(stream, &e, 1, rb_dev_), "decompress(): header readback");
+        hipck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        std::memcpy(h.data(), rb_host_, h.size());
+    } else {
+        hipck(hipMemcpyAsync(h.data(), d_input, h.size(), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+        hipck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
     uint32_t cnt;
     std::memcpy(&cnt, h.data(), 4);
     if (cnt != n)
