@@ -485,6 +485,39 @@ int fzh_concat_readback(void* stream, const fzh_concat_slot* slots, int n, void*
     return FZH_OK;
 }
 
+}  // extern "C"
+
+namespace fzh {
+
+int read_small(hipStream_t s, const void* d_src, size_t bytes, void* h_dst) {
+    constexpr size_t kCap = 64 * 1024;
+    thread_local void* host = nullptr;  // this thread's staging buffer: allocated once, never released
+    if (bytes == 0) return FZH_OK;
+    const bool fast = bytes <= kCap && bytes % 4 == 0 && reinterpret_cast<uintptr_t>(d_src) % 4 == 0;
+    if (fast && !host &&
+        hipHostMalloc(&host, kCap, hipHostMallocMapped | hipHostMallocPortable | hipHostMallocCoherent) != hipSuccess)
+        host = nullptr;
+    if (!fast || !host) {
+        FZH_TRY_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, s));
+        FZH_TRY_HIP(hipStreamSynchronize(s));
+        return FZH_OK;
+    }
+    void* dev = nullptr;
+    FZH_TRY_HIP(hipHostGetDevicePointer(&dev, host, 0));
+    ReadbackArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.e[0] = fzh_readback_entry{d_src, (uint32_t)bytes, 0};
+    readback_kernel<<<1, 64, 0, s>>>(a, 1, (uint8_t*)dev);
+    FZH_CHECK_LAUNCH("readback_kernel");
+    FZH_TRY_HIP(hipStreamSynchronize(s));
+    std::memcpy(h_dst, host, bytes);
+    return FZH_OK;
+}
+
+}  // namespace fzh
+
+extern "C" {
+
 int fzh_readback(void* stream, const fzh_readback_entry* entries, int n, void* d_mapped_dst) {
     if (n <= 0) return FZH_OK;
     if (n > FZH_READBACK_MAX) return fail(FZH_E_INVALID, "readback: at most 32 entries");

@@ -21,6 +21,11 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 // Zeroes [p, p+n) with a fill kernel on stream s.
 int zero_bytes(hipStream_t s, void* p, size_t n);
+// Copies `bytes` device bytes to host memory and synchronises the stream.
+// Reads of 4-byte multiples up to 64 KiB go through a host-mapped staging
+// buffer (one small kernel; a shorter round trip than a device-to-host
+// copy); others fall back to hipMemcpyAsync.
+int read_small(hipStream_t s, const void* d_src, size_t bytes, void* h_dst);
 
 // Optional per-kernel device timing.  launch_k launches like
 // kernel<<<grid, block, shm, s>>>(args...); while timing is on

@@ -177,8 +177,7 @@ class ChainSelectCompressor {
                                 vb_ws_.bytes - 256),
              "value base");
         float prm[4];
-        hipck(hipMemcpyAsync(prm, vb_ws_.p, 16, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-        hipck(hipStreamSynchronize(s), "hipStreamSynchronize");
+        fzck(fzh::read_small(s, vb_ws_.p, 16, prm), "value base readback");
         const float abs_eb = prm[1];
         chunks_.ensure(4 * n * E);
         fzck(fzs_gather_chunks(s, d_field, nx, ny, nz, cx_, cy_, cz_, (float*)chunks_.p), "gather chunks");
@@ -198,8 +197,7 @@ class ChainSelectCompressor {
                                      (float*)logits_.p),
                      "mlp forward");
                 fzck(fzs_select(s, (const float*)logits_.p, n, (uint8_t*)dchoice_.p), "select");
-                hipck(hipMemcpyAsync(choice_.data(), dchoice_.p, n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-                hipck(hipStreamSynchronize(s), "hipStreamSynchronize");
+                fzck(fzh::read_small(s, dchoice_.p, n, choice_.data()), "choice readback");
             } else {
                 label_ws_.ensure(fzs_label_ws_bytes(n, cx_, cy_, cz_));
                 sizes_.assign(FZS_NCHAINS * n, 0);

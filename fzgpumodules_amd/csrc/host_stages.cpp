@@ -457,8 +457,7 @@ void HuffmanStage<T>::execute(hipStream_t stream, MemoryPool* pool, const std::v
         freq_.ensure(4 * (size_t)bklen_, pool, stream);
         throwIfFzh(fzh_histogram(stream, inputs[0], n, (int)sizeof(T), bklen_, (uint32_t*)freq_.ptr), "HuffmanStage histogram");
         std::vector<uint32_t> h(bklen_);
-        FZ_HIP(hipMemcpyAsync(h.data(), freq_.ptr, 4 * (size_t)bklen_, hipMemcpyDeviceToHost, stream));
-        FZ_HIP(hipStreamSynchronize(stream));
+        throwIfFzh(fzh::read_small(stream, freq_.ptr, 4 * (size_t)bklen_, h.data()), "HuffmanStage histogram readback");
         uint64_t sum = 0;
         for (uint32_t v : h) sum += v;
         if (sum != (uint64_t)n)
@@ -597,8 +596,7 @@ void QuantizerStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const
                                           (uint8_t*)vb_.ptr + 256, vb_.bytes - 256),
                        "QuantizerStage value range");
             float p[4];
-            FZ_HIP(hipMemcpyAsync(p, vb_.ptr, 16, hipMemcpyDeviceToHost, stream));
-            FZ_HIP(hipStreamSynchronize(stream));
+            throwIfFzh(fzh::read_small(stream, vb_.ptr, 16, p), "QuantizerStage value range readback");
             vb = p[0];
         }
         computed_value_base_ = vb;
@@ -700,8 +698,7 @@ void BitpackStage<T>::execute(hipStream_t stream, MemoryPool* pool, const std::v
                                    ws_.bytes - 256),
                        "BitpackStage max");
             uint32_t mx = 0;
-            FZ_HIP(hipMemcpyAsync(&mx, ws_.ptr, 4, hipMemcpyDeviceToHost, stream));
-            FZ_HIP(hipStreamSynchronize(stream));
+            throwIfFzh(fzh::read_small(stream, ws_.ptr, 4, &mx), "BitpackStage max readback");
             uint8_t nb = 8 * sizeof(T);
             for (uint8_t b = 1; b < 8 * sizeof(T); b = (uint8_t)(b * 2))
                 if ((uint64_t)mx < (1ull << b)) { nb = b; break; }
