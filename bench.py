@@ -42,9 +42,9 @@ CAP = 0.10
 # 1.000); profiles/cloud2d_1800x3600_chain_hbm_traffic.md holds the counter
 # runs, their commands and the scaling.
 MEASURED_TRAFFIC = {
-    "minmax_partial_kernel": 25951238,
-    "lz2d_fwd_kernel": 39265875,
-    "rze_enc_kernel": 16288208,
+    "minmax_partial_kernel": 25951380,
+    "lz2d_fwd_kernel": 39264659,
+    "rze_enc_kernel": 16288073,
 }
 # Reference CLI example, benchmark mean of 10, host time (docs/cli.md:66-70)
 REF_COMPRESS_HOST_GBS = 18.85
