@@ -362,8 +362,8 @@ struct Tiles {
 // 3D: tile (bx, by, bz) = (tile % gx, (tile / gx) % gy, tile / (gx gy)) of an
 // nx x ny x nz field; `last` zeroes the code tail.  codes == nullptr: lists only.
 // ---------------------------------------------------------------------------
-template <typename TCode, bool ZZ, bool STRICT>
-__device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+template <typename TCode, bool ZZ, bool STRICT, bool INTERIOR, bool CODES>
+__device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
                                                 uint64_t n, uint32_t tile, Tiles tg, bool last, float r, float rad,
                                                 TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
     const int l = lane_id();
@@ -372,10 +372,18 @@ __device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, ui
     const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
     const bool xz = X < nx && Z < nz;
     const uint64_t plane = (uint64_t)nx * ny;
+    // interior: the tile's scalar base plus 32-bit lane offsets, no tests
+    const uint64_t tbase = (uint64_t)(bz * 8) * plane + (uint64_t)(by * 8) * nx + bx * 8;
+    const uint32_t loff = (uint32_t)z * (uint32_t)plane + (uint32_t)x;
     float s[8], xv[8];
 #pragma unroll
     for (int y = 0; y < 8; y++) {
-        xv[y] = (xz && Y0 + y < ny) ? in[Z * plane + (uint64_t)(Y0 + y) * nx + X] : 0.0f;
+        if constexpr (INTERIOR) {
+            xv[y] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(in + tbase) +
+                                                     4u * (loff + (uint32_t)y * nx));
+        } else {
+            xv[y] = (xz && Y0 + y < ny) ? in[Z * plane + (uint64_t)(Y0 + y) * nx + X] : 0.0f;
+        }
         s[y] = quant_round(xv[y], r);
     }
     // a = s - s(z-1); b = a - a(x-1); d = b - b(y-1)   (lorenzo_quant_nd.cu:234-246)
@@ -393,9 +401,16 @@ __device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, ui
 #pragma unroll
     for (int y = 0; y < 8; y++) {
         d[y] = (y > 0) ? b[y] - b[y - 1] : b[y];
-        const bool valid = xz && (Y0 + y) < ny;
+        const bool valid = INTERIOR || (xz && (Y0 + y) < ny);
         const bool ok = fabsf(d[y]) < rad;
-        if (valid && codes) codes[Z * plane + (uint64_t)(Y0 + y) * nx + X] = ok ? enc_code<TCode, ZZ>(d[y]) : (TCode)0;
+        if constexpr (CODES) {
+            const TCode cv = ok ? enc_code<TCode, ZZ>(d[y]) : (TCode)0;
+            if constexpr (INTERIOR)
+                *reinterpret_cast<TCode*>(reinterpret_cast<char*>(codes + tbase) +
+                                          (uint32_t)sizeof(TCode) * (loff + (uint32_t)y * nx)) = cv;
+            else if (valid)
+                codes[Z * plane + (uint64_t)(Y0 + y) * nx + X] = cv;
+        }
         const bool is_out = valid && !ok;
         mask |= (uint32_t)is_out << y;
         bal[y] = __ballot(is_out);
@@ -427,7 +442,7 @@ __device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, ui
         return tot;
     };
     TileCounts tc{emit(bal, mask, false, lo), 0u};
-    if (last && codes) zero_code_tail(codes, (uint64_t)nx * ny * nz, n);
+    if (CODES && last) zero_code_tail(codes, (uint64_t)nx * ny * nz, n);
     if constexpr (STRICT) {
         bool huge = false;
 #pragma unroll
@@ -438,7 +453,7 @@ __device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, ui
         uint64_t pbal[8];
 #pragma unroll
         for (int y = 0; y < 8; y++) {
-            const bool valid = xz && (Y0 + y) < ny;
+            const bool valid = INTERIOR || (xz && (Y0 + y) < ny);
             const bool p = valid && (tile_huge || needs_patch_fast(xv[y], s[y], se, band));
             pm |= (uint32_t)p << y;
             pbal[y] = __ballot(p);
@@ -446,6 +461,20 @@ __device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, ui
         tc.pcnt = emit(pbal, pm, true, lp);
     }
     return tc;
+}
+
+template <typename TCode, bool ZZ, bool STRICT>
+__device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+                                                uint64_t n, uint32_t tile, Tiles tg, bool last, float r, float rad,
+                                                TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
+    const uint32_t bx = tile % tg.gx, by = (tile / tg.gx) % tg.gy, bz = tile / (tg.gx * tg.gy);
+    // inside the field, and 8 planes of the tile spanning less than 2^32 bytes
+    const bool interior = bx * 8 + 8 <= nx && by * 8 + 8 <= ny && bz * 8 + 8 <= nz &&
+                          (uint64_t)nx * ny < (1ull << 26);
+#define FZH_LZ3(I, C) lz3d_tile_v<TCode, ZZ, STRICT, I, C>(in, nx, ny, nz, n, tile, tg, last, r, rad, codes, lo, lp, se)
+    if (interior) return codes ? FZH_LZ3(true, true) : FZH_LZ3(true, false);
+    return codes ? FZH_LZ3(false, true) : FZH_LZ3(false, false);
+#undef FZH_LZ3
 }
 
 // Device-side resolution of the error bound from the value base, shared by
