@@ -379,7 +379,7 @@ int launch_value_base(hipStream_t s, const Plan& p, const Plan* dp, uint32_t gri
 int launch_lorenzo(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid, uint32_t G) {
     G = std::max<uint32_t>(1, G);
     const uint32_t nu = (uint32_t)cdiv(p.T, G);
-    const unsigned g = std::max<uint32_t>(1, std::min<uint32_t>(grid, (uint32_t)cdiv(nu, 4)));
+    const unsigned g = std::max<uint32_t>(1, std::min<uint64_t>(grid, cdiv(nu, 4)));
     if (p.zigzag) {
         if (p.strict) lz_batch_kernel<true, true><<<g, 256, 0, s>>>(dp, G);
         else lz_batch_kernel<true, false><<<g, 256, 0, s>>>(dp, G);

@@ -371,7 +371,9 @@ BatchLaunch BatchCompressor::autotune(hipStream_t s, int reps) {
         if (t < bt) { bt = t; best.vb_wg_per_cu = w; }
     }
     bt = std::numeric_limits<float>::max();
-    for (uint32_t w : {2u, 4u, 8u, 16u})
+    // 1 << 20 workgroups per CU exceeds every grid: one unit per wave (the
+    // launch clamps the grid to the units), i.e. not persistent at all
+    for (uint32_t w : {2u, 4u, 8u, 16u, 1u << 20})
         for (uint32_t g : {1u, 4u, 16u}) {
             const float t = time([&] { fzck(launch_lorenzo(s, p, dp, cu * w, g), "tune lorenzo"); });
             if (t < bt) { bt = t; best.lz_wg_per_cu = w; best.lz_tiles_per_unit = g; }
