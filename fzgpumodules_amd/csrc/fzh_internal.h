@@ -27,6 +27,28 @@ int zero_bytes(hipStream_t s, void* p, size_t n);
 // copy); others fall back to hipMemcpyAsync.
 int read_small(hipStream_t s, const void* d_src, size_t bytes, void* h_dst);
 
+// RZE encode without its pack launch: the encoder (and, past 4096 chunks,
+// the offset scan) run, and the pack step is described for a later launch
+// that also does the pipeline's concat and readback
+// (rze_pack_concat_readback), saving one launch per compress.
+struct RzePackDesc {
+    const uint8_t* in;
+    uint64_t nbytes;
+    const uint8_t* scratch;
+    const uint32_t* sizes;
+    const uint32_t* cleans;
+    const uint32_t* offs;  // null: each workgroup sums the clean sizes before its chunk
+    uint8_t* out;
+    uint64_t* out_size;
+    uint32_t nc;
+    uint64_t nshuf;
+};
+int rze_enc_prepare(hipStream_t s, const void* d_in, size_t nbytes, int sew, void* d_out, size_t out_cap,
+                    uint64_t* d_out_size, void* d_ws, size_t ws_bytes, RzePackDesc* desc);
+int rze_pack_concat_readback(hipStream_t s, const RzePackDesc& d, const fzh_concat_slot* slots, int n, void* d_out,
+                             uint64_t out_cap, uint64_t* d_total, const fzh_readback_entry* entries, int n_entries,
+                             void* d_mapped_dst);
+
 // Optional per-kernel device timing.  launch_k launches like
 // kernel<<<grid, block, shm, s>>>(args...); while timing is on
 // (fzh_kernel_timing_enable) and the stream is not being captured, the

@@ -11,6 +11,7 @@
 #include "fz/pipeline.h"
 #include "fz/stages.h"
 #include "fzh.h"
+#include "fzh_internal.h"
 
 namespace fz {
 
@@ -309,6 +310,16 @@ void Pipeline::launchForward(const void* d_in, size_t padded, bool zero_view, hi
     buffers_[0].ptr = const_cast<void*>(d_in);
     buffers_[0].size = padded;
     stage_events_.clear();
+    // an RZE stage writing the concat's first slot leaves its pack step to
+    // the final concat + readback launch
+    RZEStage* defer_rze = nullptr;
+    if (outputs_.size() > 1) {
+        const Buffer& P = buffers_[outputs_[0]];
+        if (P.aliased && P.producer >= 0 && !nodes_[P.producer].elided)
+            defer_rze = dynamic_cast<RZEStage*>(nodes_[P.producer].stage);
+    }
+    for (auto& n : nodes_)
+        if (auto* r = dynamic_cast<RZEStage*>(n.stage)) r->setDeferPack(r == defer_rze);
     for (auto& n : nodes_) {
         if (n.elided) {  // runs inside the consuming RZE node
             buffers_[n.out_buf[0]].size = buffers_[n.in_buf].size;
@@ -360,11 +371,21 @@ void Pipeline::launchForward(const void* d_in, size_t padded, bool zero_view, hi
         for (auto& n : nodes_)
             for (const auto& r : n.stage->pendingReadbacks()) add_rb(r.d_src, r.h_dst, r.bytes);
         // concat and every readback in one launch (the header and total
-        // entries see the values that launch writes)
-        throwIfFzh(fzh_concat_readback(stream, slots.data(), (int)slots.size(), concat_buf_, concat_cap_,
-                                       (uint64_t*)((uint8_t*)concat_ws_ + 512), rb.data(), (int)rb.size(),
-                                       rb_dev_),
-                   "Pipeline concat");
+        // entries see the values that launch writes), with the RZE pack step
+        // when that stage deferred it
+        const void* pd = defer_rze ? defer_rze->deferredPack() : nullptr;
+        if (defer_rze) defer_rze->setDeferPack(false);
+        if (pd)
+            throwIfFzh(fzh::rze_pack_concat_readback(stream, *static_cast<const fzh::RzePackDesc*>(pd), slots.data(),
+                                                     (int)slots.size(), concat_buf_, concat_cap_,
+                                                     (uint64_t*)((uint8_t*)concat_ws_ + 512), rb.data(),
+                                                     (int)rb.size(), rb_dev_),
+                       "Pipeline concat");
+        else
+            throwIfFzh(fzh_concat_readback(stream, slots.data(), (int)slots.size(), concat_buf_, concat_cap_,
+                                           (uint64_t*)((uint8_t*)concat_ws_ + 512), rb.data(), (int)rb.size(),
+                                           rb_dev_),
+                       "Pipeline concat");
         return;
     }
     for (auto& n : nodes_)

@@ -326,12 +326,20 @@ void RZEStage::execute(hipStream_t stream, MemoryPool* pool, const std::vector<v
     small_.ensure(256, pool, stream);
     clearReadbacks();
     size_pending_ = status_pending_ = false;
+    pack_deferred_ = false;
     if (!is_inverse_) {
         const size_t in_bytes = sizes[0];
         if (in_bytes == 0) { actual_output_size_ = 0; return; }
         cached_orig_bytes_ = (uint32_t)in_bytes;
         ws_.ensure(fzh_rze_enc_ws_bytes(in_bytes), pool, stream);
-        if (pre_shuffle_)
+        static_assert(sizeof(fzh::RzePackDesc) <= sizeof(pack_desc_), "pack description storage");
+        if (defer_pack_) {
+            throwIfFzh(fzh::rze_enc_prepare(stream, inputs[0], in_bytes, pre_shuffle_, outputs[0],
+                                            fzh_rze_enc_bound(in_bytes), (uint64_t*)small_.ptr, ws_.ptr, ws_.bytes,
+                                            reinterpret_cast<fzh::RzePackDesc*>(pack_desc_)),
+                       "RZEStage");
+            pack_deferred_ = true;
+        } else if (pre_shuffle_)
             throwIfFzh(fzh_rze_enc_shuffled(stream, inputs[0], in_bytes, pre_shuffle_, outputs[0],
                                             fzh_rze_enc_bound(in_bytes), (uint64_t*)small_.ptr, ws_.ptr, ws_.bytes),
                        "RZEStage (fused Bitshuffle)");

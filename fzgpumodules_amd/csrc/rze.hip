@@ -26,6 +26,7 @@
 //   first) and decodes, with the
 //   repeat-elimination decode done as a parallel prefix-popcount gather
 //   instead of a single-thread loop.
+#include "fzh_concat.h"
 #include "fzh_rze_chunk.h"
 #include "fzh_scan.h"
 
@@ -73,14 +74,16 @@ rze_enc_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, uint8_t* __restr
 // sizes before its chunk itself).  Raw chunks below `nshuf` (bit-shuffled
 // blocks) come from the scratch slot, other raw chunks straight from the
 // input.
-__global__ void __launch_bounds__(256)
-rze_pack_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, const uint8_t* __restrict__ scratch,
-                const uint32_t* __restrict__ sizes, const uint32_t* __restrict__ cleans,
-                const uint32_t* __restrict__ offs, uint8_t* __restrict__ out, uint64_t* __restrict__ out_size,
-                uint32_t nc, uint64_t nshuf) {
-    __shared__ uint32_t s_sum[4];
-    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x);
-    if (c >= nc) return;  // uniform over the workgroup
+__device__ __forceinline__ void rze_pack_chunk(const RzePackDesc& d, uint32_t c, uint32_t* s_sum) {
+    const uint8_t* __restrict__ in = d.in;
+    const uint64_t nbytes = d.nbytes;
+    const uint8_t* __restrict__ scratch = d.scratch;
+    const uint32_t* __restrict__ sizes = d.sizes;
+    const uint32_t* __restrict__ cleans = d.cleans;
+    const uint32_t* __restrict__ offs = d.offs;
+    uint8_t* __restrict__ out = d.out;
+    const uint32_t nc = d.nc;
+    const uint64_t nshuf = d.nshuf;
     uint32_t off;
     if (offs) {
         off = offs[c];
@@ -104,8 +107,43 @@ rze_pack_kernel(const uint8_t* __restrict__ in, uint64_t nbytes, const uint8_t* 
         const uint64_t end = hdr + (uint32_t)(off + clean);
         const uint64_t padded = (end + 3) & ~3ull;
         if ((uint64_t)threadIdx.x < padded - end) out[end + threadIdx.x] = 0;
-        if (threadIdx.x == 0) *out_size = padded;
+        if (threadIdx.x == 0) *d.out_size = padded;
     }
+}
+
+__global__ void __launch_bounds__(256) rze_pack_kernel(RzePackDesc d) {
+    __shared__ uint32_t s_sum[4];
+    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x);
+    if (c >= d.nc) return;  // uniform over the workgroup
+    rze_pack_chunk(d, c, s_sum);
+}
+
+// The pack step and the pipeline's concat + readback in one launch:
+// workgroups [0, nc) place the chunks (slot 0 of the concat, already in
+// place), the rest lay out the other slots and serve the readback, with the
+// RZE stream's size summed from the clean sizes rather than read from the
+// size word the pack workgroups are still writing.
+__global__ void __launch_bounds__(256)
+rze_pack_concat_kernel(RzePackDesc d, ConcatArgs a, int n, uint8_t* __restrict__ out, uint64_t* __restrict__ total,
+                       ReadbackArgs r, int nr, uint8_t* __restrict__ mapped) {
+    __shared__ uint32_t s_sum[4];
+    const uint32_t b = blockIdx.x;
+    if (b < d.nc) {
+        rze_pack_chunk(d, b, s_sum);
+        return;
+    }
+    uint32_t sum;
+    if (d.offs) {
+        sum = d.offs[d.nc];
+    } else {
+        uint32_t before[1];
+        wg256_sums_before<1>(d.nc, [&](int, uint32_t i) { return d.cleans[i]; }, s_sum, before);
+        sum = before[0];
+    }
+    const uint64_t end = 8ull + 4ull * d.nc + sum;
+    const uint64_t size0 = (end + 3) & ~3ull;
+    concat_readback_body(a, n, out, total, r, nr, mapped, b - d.nc, gridDim.x - d.nc,
+                         SizeOverride{true, size0, reinterpret_cast<uintptr_t>(d.out_size), size0});
 }
 
 // ---------------------------------------------------------------------------
@@ -417,8 +455,10 @@ size_t fzh_rze_enc_ws_bytes(size_t nbytes) { return enc_layout(nbytes).total; }
 namespace fzh {
 namespace {
 
-int rze_enc_impl(hipStream_t s, const void* d_in, size_t nbytes, int sew, void* d_out, size_t out_cap,
-                 uint64_t* d_out_size, void* d_ws, size_t ws_bytes) {
+}  // namespace
+
+int rze_enc_prepare(hipStream_t s, const void* d_in, size_t nbytes, int sew, void* d_out, size_t out_cap,
+                    uint64_t* d_out_size, void* d_ws, size_t ws_bytes, RzePackDesc* desc) {
     if (nbytes == 0) return fail(FZH_E_INVALID, "RZE: empty input");
     if (nbytes > 0xFFFFFFFFull) return fail(FZH_E_INVALID, "RZE: input larger than 4 GiB");
     if (out_cap < fzh_rze_enc_bound(nbytes)) return fail(FZH_E_CAPACITY, "RZE: output buffer below fzh_rze_enc_bound()");
@@ -441,8 +481,34 @@ int rze_enc_impl(hipStream_t s, const void* d_in, size_t nbytes, int sew, void* 
     const bool inline_offs = nc <= PACK_INLINE;
     if (!inline_offs)
         if (int rc = scan_u32(s, clean, offs, nc, nullptr, offs + nc, w + L.o_scan)) return rc;
-    launch_k("rze_pack_kernel", rze_pack_kernel, dim3(nc), dim3(256), 0, s, in, nbytes, scratch,
-             sizes, clean, inline_offs ? nullptr : offs, out, d_out_size, nc, sew ? nfull : (uint64_t)0);
+    *desc = RzePackDesc{in,    nbytes, scratch, sizes, clean, inline_offs ? nullptr : offs,
+                        out,   d_out_size, nc, sew ? nfull : (uint64_t)0};
+    return FZH_OK;
+}
+
+int rze_pack_concat_readback(hipStream_t s, const RzePackDesc& d, const fzh_concat_slot* slots, int n, void* d_out,
+                             uint64_t out_cap, uint64_t* d_total, const fzh_readback_entry* entries, int n_entries,
+                             void* d_mapped_dst) {
+    ConcatArgs a;
+    ReadbackArgs r;
+    unsigned grid = 1;
+    if (int rc = build_concat_args(slots, n, d_out, out_cap, d_total, entries, n_entries, d_mapped_dst, a, r, grid))
+        return rc;
+    if (n < 1 || slots[0].src != d.out)
+        return fail(FZH_E_INVALID, "RZE pack + concat: slot 0 must be the RZE stream");
+    launch_k("rze_pack_concat_kernel", rze_pack_concat_kernel, dim3(d.nc + grid), dim3(256), 0, s, d, a, n,
+             (uint8_t*)d_out, d_total, r, n_entries, (uint8_t*)d_mapped_dst);
+    FZH_CHECK_LAUNCH("rze_pack_concat_kernel");
+    return FZH_OK;
+}
+
+namespace {
+
+int rze_enc_impl(hipStream_t s, const void* d_in, size_t nbytes, int sew, void* d_out, size_t out_cap,
+                 uint64_t* d_out_size, void* d_ws, size_t ws_bytes) {
+    RzePackDesc d;
+    if (int rc = rze_enc_prepare(s, d_in, nbytes, sew, d_out, out_cap, d_out_size, d_ws, ws_bytes, &d)) return rc;
+    launch_k("rze_pack_kernel", rze_pack_kernel, dim3(d.nc), dim3(256), 0, s, d);
     FZH_CHECK_LAUNCH("rze_pack_kernel");
     return FZH_OK;
 }

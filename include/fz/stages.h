@@ -278,6 +278,12 @@ class RZEStage : public Stage {
     // and the shuffle runs inside the RZE kernels (same stream bytes).
     void setPreShuffle(int elem_width) { pre_shuffle_ = elem_width; }
     int getPreShuffle() const { return pre_shuffle_; }
+    // Set by the pipeline when this stage's output is the concat's first
+    // slot: execute() then leaves the pack step to the pipeline's final
+    // concat + readback launch, and deferredPack() describes it (until the
+    // next execute).
+    void setDeferPack(bool on) { defer_pack_ = on; }
+    const void* deferredPack() const { return pack_deferred_ ? pack_desc_ : nullptr; }
 
     void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
                  const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
@@ -316,6 +322,8 @@ class RZEStage : public Stage {
   private:
     bool is_inverse_ = false;
     int pre_shuffle_ = 0;
+    bool defer_pack_ = false, pack_deferred_ = false;
+    alignas(8) unsigned char pack_desc_[96] = {};
     uint32_t chunk_size_ = 16384;
     uint8_t levels_ = 4;
     uint32_t cached_orig_bytes_ = 0;

@@ -238,6 +238,27 @@ def test_empty_and_errors():
             p.decompress(0x1000, 64)
 
 
+@pytest.mark.parametrize("n", [1 << 20, 34_000_000])
+def test_rze_pack_in_the_concat_launch(n):
+    """The RZE stage writing the concat's first slot leaves its pack step to
+    the final concat + readback launch; the stream stays the oracle's byte
+    for byte, below and above 4096 chunks (the offset-scan path), and
+    decompresses."""
+    x = fields.sinusoid_1d(n, seed=_seed(f"packfuse{n}"))
+    want, info = chains.lorenzo_bitshuffle_rze(x, (0, 1, 1), mode=0, user_eb=1e-4)
+    with Pipeline(x.nbytes) as p:
+        lz = p.add_lorenzo(1e-4, mode="ABS", outlier_capacity=0.1, zigzag=True)
+        bs = p.add_bitshuffle(16384, 2)
+        co = p.add_rze()
+        p.connect(bs, lz, "codes")
+        p.connect(co, bs)
+        p.finalize()
+        blob = p.compress_array(x)
+        np.testing.assert_array_equal(blob, want)
+        y = p.decompress_array(blob)[: x.size]
+        assert np.abs(y.astype(np.float64) - x).max() <= 1e-4 * 1.01
+
+
 def _boundary_field(ny, nx, eb, seed):
     """Values within a few ulps of eb from a quantisation point: the strict
     test |fl(s*ebx2) - x| <= eb is decided near its boundary everywhere."""
