@@ -8,7 +8,7 @@ element width 2), RZE (16 KiB chunks, 4 levels) -- the reference CLI's
 `lorenzo->bitshuffle->rze -m rel -e 1e-3` setup.
 
 A step is one Pipeline::compress() of the whole field (input resident in HBM,
-output size read back, stream synchronised), its seven kernels launched
+output size read back, stream synchronised), its six kernels launched
 directly (replaying them as one hipGraph costs about 5 us more per call on
 this ROCm stack, so graph mode is left off).  The
 JSON line also carries decompress GB/s, the ratio, per-kernel device times
