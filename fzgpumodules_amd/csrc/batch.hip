@@ -145,7 +145,8 @@ __global__ void __launch_bounds__(256) lz_batch_kernel(const Plan* __restrict__ 
     const uint32_t nu = (p.T + G - 1) / G;
     for (uint32_t u = blockIdx.x * 4 + wave_in_wg(); u < nu; u += waves) {
         const uint32_t g0 = u * G, g1 = min(p.T, g0 + G);
-        uint32_t f = find_field(p.tile0, p.F, g0);
+        uint32_t f = p.tfield[g0 >> 6];
+        while (g0 >= p.tile0[f + 1]) f++;
         for (uint32_t g = g0; g < g1; g++) {
             while (g >= p.tile0[f + 1]) f++;
             const Field& fd = p.fields[f];

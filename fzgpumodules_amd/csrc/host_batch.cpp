@@ -196,6 +196,17 @@ void BatchCompressor::setFields(const std::vector<BatchField>& fields) {
     hipck(hipMemcpy(d_t0, tile0.data(), 4 * (F + 1), hipMemcpyHostToDevice), "hipMemcpy (tiles)");
     hipck(hipMemcpy(d_c0, chunk0.data(), 4 * (F + 1), hipMemcpyHostToDevice), "hipMemcpy (chunks)");
     hipck(hipMemcpy(d_u0, unit0.data(), 4 * (F + 1), hipMemcpyHostToDevice), "hipMemcpy (units)");
+    // field of every 64th tile: the Lorenzo kernel starts its walk there
+    // instead of a binary search over all fields
+    const size_t nb = std::max<size_t>((T + 63) / 64, 1);
+    std::vector<uint32_t> tfield(nb, 0);
+    for (size_t k = 0, f = 0; k < nb; k++) {
+        while (f + 1 < F && tile0[f + 1] <= 64 * k) f++;
+        tfield[k] = (uint32_t)f;
+    }
+    uint32_t* d_tf = (uint32_t*)impl_->dev(4 * nb);
+    hipck(hipMemcpy(d_tf, tfield.data(), 4 * nb, hipMemcpyHostToDevice), "hipMemcpy (tile fields)");
+    p.tfield = d_tf;
     p.fields = d_tab;
     p.tile0 = d_t0;
     p.chunk0 = d_c0;
