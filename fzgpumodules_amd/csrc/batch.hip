@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256) rze_batch_kernel(const Plan* __restrict__
     const uint32_t gc = xcd_swizzle(blockIdx.x, gridDim.x);
     if (gc >= p.NC) return;
     const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
-    const uint32_t f = find_field(p.chunk0, p.F, gc);
+    const uint32_t f = p.cfield[gc];
     const Field& fd = p.fields[f];
     const uint32_t lc = gc - fd.chunk0;
     const uint64_t cb = 2 * fd.n;  // code bytes
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(256) pack_batch_kernel(const Plan* __restrict_
     const uint32_t waves = gridDim.x * 4;
     const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
     for (uint32_t gc = blockIdx.x * 4 + wave_in_wg(); gc < p.NC; gc += waves) {
-        const uint32_t f = find_field(p.chunk0, p.F, gc);
+        const uint32_t f = p.cfield[gc];
         if (p.results[f].flags & kOverflowCapacity) continue;
         const Field& fd = p.fields[f];
         const uint32_t lc = gc - fd.chunk0;

@@ -207,6 +207,12 @@ void BatchCompressor::setFields(const std::vector<BatchField>& fields) {
     uint32_t* d_tf = (uint32_t*)impl_->dev(4 * nb);
     hipck(hipMemcpy(d_tf, tfield.data(), 4 * nb, hipMemcpyHostToDevice), "hipMemcpy (tile fields)");
     p.tfield = d_tf;
+    std::vector<uint32_t> cfield(std::max<uint64_t>(NC, 1), 0);
+    for (uint32_t f = 0; f < F; f++)
+        for (uint32_t c = chunk0[f]; c < chunk0[f + 1]; c++) cfield[c] = f;
+    uint32_t* d_cf = (uint32_t*)impl_->dev(4 * cfield.size());
+    hipck(hipMemcpy(d_cf, cfield.data(), 4 * cfield.size(), hipMemcpyHostToDevice), "hipMemcpy (chunk fields)");
+    p.cfield = d_cf;
     p.fields = d_tab;
     p.tile0 = d_t0;
     p.chunk0 = d_c0;
