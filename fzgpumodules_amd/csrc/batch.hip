@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256) vb_partial_kernel(const Plan* __restrict_
     __shared__ float smn[4], smx[4];
     const int w = threadIdx.x >> 6;
     for (uint32_t u = blockIdx.x; u < p.U; u += gridDim.x) {
-        const uint32_t f = find_field(p.unit0, p.F, u);
+        const uint32_t f = p.ufield[u];
         const Field& fd = p.fields[f];
         const float* in = fd.in;
         const uint64_t b = (uint64_t)(u - fd.unit0) * kVbUnit;

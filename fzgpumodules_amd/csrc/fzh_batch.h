@@ -63,6 +63,7 @@ struct Plan {
     const uint32_t* unit0;   // F + 1
     const uint32_t* tfield;  // ceil(T / 64): the field holding tile 64 k (a start for the tile -> field walk)
     const uint32_t* cfield;  // NC: the field of each chunk
+    const uint32_t* ufield;  // U: the field of each value-base unit
     uint32_t F, T, NC, U;
     // chain settings
     float user_eb;

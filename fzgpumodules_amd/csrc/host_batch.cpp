@@ -213,6 +213,12 @@ void BatchCompressor::setFields(const std::vector<BatchField>& fields) {
     uint32_t* d_cf = (uint32_t*)impl_->dev(4 * cfield.size());
     hipck(hipMemcpy(d_cf, cfield.data(), 4 * cfield.size(), hipMemcpyHostToDevice), "hipMemcpy (chunk fields)");
     p.cfield = d_cf;
+    std::vector<uint32_t> ufield(std::max<uint64_t>(U, 1), 0);
+    for (uint32_t f = 0; f < F; f++)
+        for (uint32_t u = unit0[f]; u < unit0[f + 1]; u++) ufield[u] = f;
+    uint32_t* d_uf = (uint32_t*)impl_->dev(4 * ufield.size());
+    hipck(hipMemcpy(d_uf, ufield.data(), 4 * ufield.size(), hipMemcpyHostToDevice), "hipMemcpy (unit fields)");
+    p.ufield = d_uf;
     p.fields = d_tab;
     p.tile0 = d_t0;
     p.chunk0 = d_c0;
