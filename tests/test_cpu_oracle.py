@@ -240,6 +240,215 @@ def test_rze_round_trips(orc):
     assert raw[8:12].view(np.uint32)[0] & 0x80000000  # incompressible chunk stored raw
 
 
+# RZE chunk and stream format restated with numpy primitives, independently
+# of the oracle's loops (rze_stage.cu:135-224 ZE/RE helpers, :324-445 the
+# levels, raw-fallback thresholds and size tag, :774-843 the stream):
+#   ZE:  nonzero bytes in order; bitmap bit i of byte i/8 = (byte i != 0)
+#   RE:  byte i kept iff it differs from byte i-1, byte -1 := 0; bitmap of
+#        the kept flags, bit i of byte i/8
+#   chunk: [ZE data][RE(bm1) data][RE(bm2) data][RE(bm3) data][bm4, 4 B]
+#          [in_size u16 LE]; all-zero chunk -> the 2-byte tag alone;
+#          incompressible -> the input verbatim, size | 0x8000_0000
+#   stream: [orig u32][nchunks u32][size word u32 x nchunks][chunks], zero
+#           padded to 4 bytes
+# The `mutant` switches restate plausible mistakes (MSB-first bitmaps, a
+# repeat-elimination seed of 0xFF, the bitmap levels written in reverse);
+# the tests check that each one changes the known answers.
+# ---------------------------------------------------------------------------
+RZE_CS = 16384
+
+
+def _bits_to_bytes(flags, mutant):
+    return np.packbits(flags, bitorder="big" if mutant == "msb_first" else "little")
+
+
+def _re_def(b, mutant=None):
+    seed = 0xFF if mutant == "seed_ff" else 0
+    prev = np.concatenate([np.array([seed], np.uint8), b[:-1]])
+    keep = b != prev
+    return b[keep], _bits_to_bytes(keep, mutant)
+
+
+def rze_chunk_def(x, mutant=None):
+    """(stored bytes, size word, branch taken)."""
+    x = np.asarray(x, np.uint8)
+    n = x.size
+    tag = np.array([n & 0xFF, (n >> 8) & 0xFF], np.uint8)
+    nz = x != 0
+    if not nz.any():
+        return tag, 2, "zero"
+
+    def raw(why):
+        return x.copy(), 0x80000000 | n, why
+
+    avail = RZE_CS - 2
+    data = x[nz]
+    bm1 = _bits_to_bytes(nz, mutant)
+    w = data.size
+    if w >= avail or w > avail - bm1.size:
+        return raw("ze")
+    r1, bm2 = _re_def(bm1, mutant)
+    w += r1.size
+    if w >= avail:
+        return raw("re1")
+    if w > avail - bm2.size:
+        return raw("bm2")
+    r2, bm3 = _re_def(bm2, mutant)
+    w += r2.size
+    if w >= avail:
+        return raw("re2")
+    if w >= avail - 4 or w > avail - 4 - bm3.size:
+        return raw("bm3")
+    r3, bm4 = _re_def(bm3, mutant)
+    w += r3.size
+    if w >= avail - 4:
+        return raw("re3")
+    bm4p = np.zeros(4, np.uint8)
+    bm4p[: bm4.size] = bm4
+    levels = [r1, r2, r3] if mutant != "levels_reversed" else [r3, r2, r1]
+    out = np.concatenate([data] + levels + [bm4p, tag])
+    return out, out.size, "packed"
+
+
+def rze_stream_def(b, mutant=None):
+    b = np.asarray(b, np.uint8)
+    nc = -(-b.size // RZE_CS)
+    parts = [rze_chunk_def(b[c * RZE_CS:(c + 1) * RZE_CS], mutant) for c in range(nc)]
+    head = np.array([b.size, nc] + [p[1] for p in parts], np.uint32).view(np.uint8)
+    s = np.concatenate([head] + [p[0] for p in parts])
+    return np.concatenate([s, np.zeros(-s.size % 4, np.uint8)])
+
+
+def _values_for(keep, pop_total):
+    """Bytes v with v[i] != v[i-1] (v[-1] = 0) exactly where keep[i], and
+    sum(popcount(v)) = pop_total where the run structure allows it (the
+    returned total says what was reached)."""
+    keep = np.asarray(keep, bool)
+    starts = np.nonzero(keep)[0]
+    forced_zero = not keep[0]  # a first run equal to the seed 0
+    if forced_zero:
+        starts = np.r_[0, starts]
+    lens = np.diff(np.r_[starts, keep.size])
+    c = np.full(starts.size, 7)
+    if forced_zero:
+        c[0] = 0
+    total = int((c * lens).sum())
+
+    def lowest(r):  # a run must differ from the run before it (run -1 = 0)
+        return 1 if (r == 0 or c[r - 1] == 0) else 0
+
+    # lower run popcounts (a run may drop to 0 unless a neighbour is 0) or
+    # raise them to 8 (0xFF, never on two adjacent runs); longest runs first
+    # so the final steps are fine grained
+    order = np.argsort(-lens, kind="stable")
+    for r in order:
+        if forced_zero and r == 0:
+            continue
+        while total > pop_total and c[r] > lowest(r) and total - lens[r] >= pop_total:
+            if c[r] == 1 and r + 1 < c.size and c[r + 1] == 0:
+                break
+            c[r] -= 1
+            total -= lens[r]
+        if total < pop_total and c[r] == 7 and total + lens[r] <= pop_total:
+            if (r == 0 or c[r - 1] != 8) and (r + 1 == c.size or c[r + 1] != 8):
+                c[r] = 8
+                total += lens[r]
+    pal = {k: [v for v in range(256) if bin(v).count("1") == k] for k in range(9)}
+    vals = np.zeros(starts.size, np.uint8)
+    prev = 0
+    for r in range(starts.size):
+        choices = [v for v in pal[int(c[r])] if v != prev] if not (forced_zero and r == 0) else [0]
+        vals[r] = choices[0]  # the smallest: set bits low, so lower levels start at byte 0
+        prev = int(vals[r])
+    return np.repeat(vals, lens), total
+
+
+def _chunk_with_counts(nr4_bits, nr3, nr2, nz, rng):
+    """A full 16 KiB chunk whose level counts are (nz, nr2, nr3, nr4): the
+    bitmaps are chosen top-down (bm3 from the bm4 flags, bm2 from bm3, bm1
+    from bm2) and the chunk's nonzero bytes follow bm1."""
+    bm4 = np.zeros(32, bool)
+    bm4[:nr4_bits] = True
+    bm3, p3 = _values_for(bm4, nr3)
+    bm2, p2 = _values_for(np.unpackbits(bm3, bitorder="little").astype(bool), nr2)
+    bm1, p1 = _values_for(np.unpackbits(bm2, bitorder="little").astype(bool), nz)
+    mask = np.unpackbits(bm1, bitorder="little").astype(bool)
+    x = np.where(mask, rng.integers(1, 256, RZE_CS), 0).astype(np.uint8)
+    return x, (p1, p2, p3, nr4_bits)
+
+
+def test_rze_known_answers(orc):
+    """Hand-derived chunks: ZE + every RE level, and a repeat-heavy bitmap."""
+    x = np.zeros(24, np.uint8)
+    x[[2, 3, 4]] = 5
+    x[16], x[23] = 7, 9
+    # data 05 05 05 07 09 | bm1 = 1C 00 81, all kept | bm2 = 07 | bm3 = 01 |
+    # bm4 = 01 00 00 00 | tag 24
+    chunk = [5, 5, 5, 7, 9, 0x1C, 0x00, 0x81, 0x07, 0x01, 1, 0, 0, 0, 24, 0]
+    want = np.array([24, 0, 0, 0, 1, 0, 0, 0, 16, 0, 0, 0] + chunk, np.uint8)
+    np.testing.assert_array_equal(rze_stream_def(x), want)
+    np.testing.assert_array_equal(orc.rze_enc(x), want)
+    y = np.full(64, 0x11, np.uint8)
+    # bm1 = FF x 8 -> RE keeps byte 0 only: FF | bm2 = 01 | bm3 = 01
+    chunk = [0x11] * 64 + [0xFF, 0x01, 0x01, 1, 0, 0, 0, 64, 0]
+    want = np.array([64, 0, 0, 0, 1, 0, 0, 0, 73, 0, 0, 0] + chunk + [0, 0, 0], np.uint8)
+    np.testing.assert_array_equal(rze_stream_def(y), want)
+    np.testing.assert_array_equal(orc.rze_enc(y), want)
+    # each plausible mistake changes one of the two known answers
+    for m in ("msb_first", "seed_ff", "levels_reversed"):
+        assert not (np.array_equal(rze_stream_def(x, m), orc.rze_enc(x))
+                    and np.array_equal(rze_stream_def(y, m), orc.rze_enc(y))), m
+
+
+def test_rze_threshold_boundaries(orc):
+    """Every raw-fallback branch of rze_stage.cu:369-431, each at the exact
+    count where it starts to apply, against the definition."""
+    rng = np.random.default_rng(77)
+    cases = []
+    # ZE level: nz = avail - bm1_bytes = 14334 passes, 14335 falls back
+    for w in (14333, 14334, 14335):
+        x = np.zeros(RZE_CS, np.uint8)
+        x[:w] = rng.integers(1, 256, w)
+        cases.append((x, "ze" if w > 14334 else "packed"))
+    # (bm4 flags, nr3, nr2, nz): the cumulative write position meets each
+    # later threshold exactly (16382 = avail, 16126 = avail - 256 bm2 bytes,
+    # 16378 = avail - 4, 16346 = avail - 4 - 32 bm3 bytes)
+    for spec, branch in (((2, 1, 2048, 14334), "re1"),           # 14334 + 2048 = 16382
+                         ((31, 220, 1792, 14334), "packed"),     # 16126, 16346, 16377: every test passes
+                         ((32, 224, 1793, 14334), "bm2"),        # 16127 > 16126
+                         ((32, 224, 1792, 14330), "re3"),        # 16122 + 224 + 32 = 16378
+                         ((31, 224, 1792, 14330), "packed"),     # 16377 + 6 = 16383 bytes
+                         ((32, 224, 1792, 14331), "bm3")):       # 16347 > 16346
+        x, got = _chunk_with_counts(*spec, rng)
+        assert got == (spec[3], spec[2], spec[1], spec[0]), (spec, got)
+        cases.append((x, branch))
+    for x, branch in cases:
+        s, size, why = rze_chunk_def(x)
+        assert why == branch, (why, branch)
+        enc = orc.rze_enc(x)
+        np.testing.assert_array_equal(enc, rze_stream_def(x))
+        assert enc[8:12].view(np.uint32)[0] == size
+        if branch == "packed":
+            np.testing.assert_array_equal(orc.rze_dec(enc, x.size), x)
+
+
+def test_rze_matches_definition(orc):
+    rng = np.random.default_rng(78)
+    n = 16384 * 6 + 5000
+    cases = [
+        np.zeros(n, np.uint8),
+        rng.integers(0, 256, n).astype(np.uint8),
+        ((rng.random(n) < 0.03) * rng.integers(1, 256, n)).astype(np.uint8),
+        ((rng.random(n) < 0.6) * rng.integers(1, 256, n)).astype(np.uint8),
+        np.repeat(rng.integers(0, 3, n // 100 + 1).astype(np.uint8), 100)[:n],
+        np.tile(np.array([0, 0, 1, 0], np.uint8), n // 4 + 1)[:n],
+        rng.integers(0, 256, 16384 + 15000).astype(np.uint8),   # raw partial last chunk
+        np.r_[np.zeros(16384, np.uint8), np.full(300, 3, np.uint8)],
+    ]
+    for b in cases:
+        np.testing.assert_array_equal(orc.rze_enc(b), rze_stream_def(b))
+
+
 # ---------------------------------------------------------------------------
 # RLE (rle.cu; known answer test_rle.cpp:120-145)
 # ---------------------------------------------------------------------------
@@ -332,6 +541,90 @@ def test_huffman_stream_round_trip(orc, bklen):
     s = orc.huffman_encode(v, bklen)
     np.testing.assert_array_equal(orc.huffman_decode(s, np.uint16, v.size), v)
     assert not s[64:128].any()  # header filler bytes zero in this build
+
+
+# PHF stream restated with numpy primitives, independently of the oracle's
+# loops (hf_kernels.cu:73-122 packing, hf_hl.cc:50-67 metadata, hf_buf.cc:
+# 151-176 the merge, hf.h:38-44 the header):
+#   codeword w = bk4[symbol]: length = w >> 27, code = low 27 bits
+#   partition p (sublen symbols): the codes' bits, most significant first,
+#   concatenated and cut into u32 cells whose first bit is bit 31;
+#   par_nbit[p] = its bit count; par_entry = exclusive prefix of the cell
+#   counts; stream = [header 64 B][64 zero B][revbook][par_nbit][par_entry]
+#   [cells].  `mutant="lsb_first"` packs each cell from bit 0 instead.
+# ---------------------------------------------------------------------------
+def huffman_stream_def(v, bk4, rev, bklen, sublen, mutant=None):
+    v = np.asarray(v)
+    words = bk4[v.astype(np.int64)].astype(np.uint64)
+    lens = (words >> np.uint64(27)).astype(np.int64)
+    codes = words & np.uint64(0x07FFFFFF)
+    pardeg = -(-v.size // sublen)
+    nbits, cells = [], []
+    for p in range(pardeg):
+        L, cw = lens[p * sublen:(p + 1) * sublen], codes[p * sublen:(p + 1) * sublen]
+        M = (cw[:, None] >> np.arange(26, -1, -1, dtype=np.uint64)[None, :]) & np.uint64(1)
+        bits = M[np.arange(27)[None, :] >= 27 - L[:, None]].astype(np.uint8)
+        packed = np.packbits(bits, bitorder="big" if mutant != "lsb_first" else "little")
+        packed = np.concatenate([packed, np.zeros(-packed.size % 4, np.uint8)])
+        cells.append(np.frombuffer(packed.tobytes(), ">u4").astype("<u4"))
+        nbits.append(bits.size)
+    ncell = np.array([c.size for c in cells], np.int64)
+    entry = np.r_[0, np.cumsum(ncell)[:-1]].astype(np.uint32)
+    tot_cells = int(ncell.sum())
+    h = np.zeros(128, np.uint8)
+    h[0:2] = np.frombuffer(np.uint16(bklen).tobytes(), np.uint8)
+    h[4:12] = np.array([sublen, pardeg], np.int32).view(np.uint8)
+    h[16:40] = np.array([v.size, sum(nbits), tot_cells], np.uint64).view(np.uint8)
+    e = np.cumsum([0, 128, rev.size, 4 * pardeg, 4 * pardeg, 4 * tot_cells]).astype(np.uint32)
+    h[40:64] = e.view(np.uint8)
+    return np.concatenate([h, rev, np.array(nbits, np.uint32).view(np.uint8), entry.view(np.uint8)]
+                          + [c.view(np.uint8) for c in cells])
+
+
+def test_huffman_stream_known_answer(orc):
+    """Four symbols with frequencies 4, 2, 1, 1: canonical codes 1, 01, 000,
+    001 (the compiled reference builder's book), so 0 1 2 3 0 0 1 0 packs
+    to the 14 bits 1 01 000 001 1 1 01 1 = cell 0xA0EC0000."""
+    v = np.array([0, 1, 2, 3, 0, 0, 1, 0], np.uint16)
+    f = orc.histogram(v, 4)
+    bk4, rev = orc.huffman_book(f, 2)
+    assert [(int(w) >> 27, int(w) & 0x7FFFFFF) for w in bk4] == [(1, 1), (2, 1), (3, 0), (3, 1)]
+    ref = orc.ref_huffman_book(f, 2)
+    if ref is not None:
+        np.testing.assert_array_equal(ref[0], bk4)
+    s = orc.huffman_encode(v, 4)
+    ent = s[40:64].view(np.uint32)
+    assert ent.tolist() == [0, 128, 128 + rev.size, 132 + rev.size, 136 + rev.size, 140 + rev.size]
+    assert s[ent[2]:ent[3]].view(np.uint32).tolist() == [14]
+    assert s[ent[3]:ent[4]].view(np.uint32).tolist() == [0]
+    assert s[ent[4]:ent[5]].view(np.uint32).tolist() == [0xA0EC0000]
+    np.testing.assert_array_equal(s, huffman_stream_def(v, bk4, rev, 4, 768))
+    assert not np.array_equal(s, huffman_stream_def(v, bk4, rev, 4, 768, mutant="lsb_first"))
+
+
+@pytest.mark.parametrize("case", ["coarse_1024", "coarse_256_u8", "fine_256_u8", "long_codes"])
+def test_huffman_stream_matches_definition(orc, case):
+    rng = np.random.default_rng(len(case))
+    if case == "coarse_1024":
+        v = np.minimum(np.abs(rng.normal(0, 30, 50000)), 1023).astype(np.uint16)
+        bklen, fine, sub = 1024, False, 768
+    elif case == "coarse_256_u8":
+        v = np.minimum(np.abs(rng.laplace(0, 6, 40001)), 255).astype(np.uint8)
+        bklen, fine, sub = 256, False, 768
+    elif case == "fine_256_u8":
+        # every symbol used, every codeword 8 bits: the fine layout applies
+        v = np.random.default_rng(21).integers(0, 256, 1024 * 50 + 77).astype(np.uint8)
+        bklen, fine, sub = 256, True, 1024
+    else:  # geometric frequencies: codewords well past 16 bits
+        v = np.minimum(rng.geometric(0.45, 200000) - 1, 1023).astype(np.uint16)
+        bklen, fine, sub = 1024, False, 768
+    f = orc.histogram(v, bklen)
+    bk4, rev = orc.huffman_book(f, v.dtype.itemsize)
+    if case == "long_codes":
+        assert int((bk4[f > 0] >> 27).max()) > 16
+    s = orc.huffman_encode(v, bklen, fine=fine)
+    assert int(s[4:8].view(np.int32)[0]) == sub
+    np.testing.assert_array_equal(s, huffman_stream_def(v, bk4, rev, bklen, sub))
 
 
 # ---------------------------------------------------------------------------
