@@ -66,6 +66,8 @@ def parse():
                     help="fields of the batched workload (0 skips it)")
     ap.add_argument("--selector-train", type=int, default=8192,
                     help="training chunks of the learned-selector workload (0 skips it)")
+    ap.add_argument("--no-cfg1", dest="cfg1", action="store_false", help="skip the 2^24 sinusoid -> RLE workload")
+    ap.add_argument("--no-cfg3", dest="cfg3", action="store_false", help="skip the 512^3 turbulence -> Huffman workload")
     return ap.parse_args()
 
 
@@ -215,8 +217,20 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    # ---- the GPU blob against the oracle chain at full size ------------------
+    # (strict mode adds the patch list; the first four parts are the plain
+    # chain's and must equal the oracle's byte for byte)
+    if cpu is not None:
+        got = split_blob(tmp)
+        ref = cpu.pop("_parts")
+        res["bit_exact_vs_oracle"] = bool(len(got) == 7 and all(np.array_equal(got[i], ref[i]) for i in range(4)))
+        log(f"[bench] compressed parts equal to the oracle chain: {res['bit_exact_vs_oracle']}")
     p.close()
     d_in.free()
+    if args.cfg1:
+        res["sinusoid_1d"] = chain_workload("1d", args)
+    if args.cfg3:
+        res["turbulence_3d"] = chain_workload("3d", args)
     if args.batch_fields > 0:
         res["batched"] = batched_workload(args)
     if args.selector_train > 0:
@@ -611,14 +625,52 @@ def selector_workload(args):
     return res
 
 
+def split_blob(blob):
+    """Parts of a pipeline result ([u32 n][u64 size x n], 16-byte slots)."""
+    n = int(blob[:4].view(np.uint32)[0])
+    sizes = blob[4:4 + 8 * n].view(np.uint64).astype(np.int64)
+    off = (4 + 8 * n + 15) // 16 * 16
+    parts = []
+    for sz in sizes:
+        parts.append(blob[off:off + sz])
+        off += (int(sz) + 15) // 16 * 16
+    return parts
+
+
+def cpu_info():
+    """Host CPU: model name, logical CPUs, CPUs this process may run on, and
+    the thread share the oracle uses (OMP_NUM_THREADS when set, else the
+    affinity set)."""
+    model = None
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    if model is None:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    affinity = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or affinity
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity, "threads": min(share, affinity)}
+
+
 def cpu_baseline(x, runs):
     """The oracle's value base + Lorenzo2D + Bitshuffle + RZE on the host
-    cores (OpenMP over tiles/blocks/chunks), median of `runs` after 1 warm-up."""
+    cores (OpenMP over tiles/blocks/chunks), median of `runs` after 1 warm-up,
+    on every thread of the process's share and on one thread."""
     import oracle
 
-    oracle.lib()
-    cores = min(os.cpu_count() or 1, 16)
-    oracle.set_threads(cores)
+    info = cpu_info()
     L = oracle.lib()
     n = int(math.ceil(x.nbytes / 16384) * 16384) // 4
     xp = np.zeros(n, np.float32)
@@ -638,23 +690,209 @@ def cpu_baseline(x, runs):
         return L.orc_chain_l2d_bs_rze(xp.ctypes.data, n, NX, NY, r, 32768, 1, codes.ctypes.data, shuf.ctypes.data,
                                       out.ctypes.data, err.ctypes.data, idx.ctypes.data, m, C.byref(no))
 
-    run()
-    ts = []
-    for _ in range(runs):
-        t0 = time.perf_counter()
-        run()
-        ts.append(time.perf_counter() - t0)
-    med = float(np.median(ts))
-    gbs = x.nbytes / med / 1e9
-    log(f"[bench] cpu oracle chain: {med * 1e3:.1f} ms  {gbs:.2f} GB/s on {cores} threads")
+    def timed(threads, k):
+        oracle.set_threads(threads)
+        sz = run()
+        ts = []
+        for _ in range(k):
+            t0 = time.perf_counter()
+            run()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), sz
+
+    med, sz = timed(info["threads"], runs)
+    med1, _ = timed(1, max(3, runs // 2))
+    oracle.set_threads(info["threads"])
+    gbs, gbs1 = x.nbytes / med / 1e9, x.nbytes / med1 / 1e9
+    log(f"[bench] cpu oracle chain: {med * 1e3:.1f} ms  {gbs:.2f} GB/s on {info['threads']} threads; "
+        f"{med1 * 1e3:.1f} ms  {gbs1:.3f} GB/s on 1 thread ({info['model']}, nproc {info['nproc']})")
+    k = min(int(no.value), m)
     return {
         "value": round(gbs, 3),
         "unit": "GB/s",
-        "cores": cores,
+        "cores": info["threads"],
         "kind": "port",
         "sample": f"whole 1800x3600 field (value base + Lorenzo2D + Bitshuffle + RZE), median of {runs} runs "
-                  f"after 1 warm-up, OpenMP {cores} threads",
+                  f"after 1 warm-up, OpenMP over tiles/blocks/chunks on the process's {info['threads']}-thread "
+                  "share of the host",
+        "value_1thread": round(gbs1, 4),
+        "cpu_model": info["model"],
+        "nproc": info["nproc"],
+        "affinity_cpus": info["affinity"],
+        "_parts": [out[:sz].copy(), err[:k].copy(), idx[:k].copy(), np.array([no.value], np.uint32).view(np.uint8)],
     }
+
+
+def chain_workload(kind, args):
+    """One of the other two single-field workloads, strict bound on:
+      1d: sinusoid_1d(2^24), Lorenzo ABS 1e-4 r32768 cap 0.2 -> RLE<u16>
+      3d: turbulence_3d(512), Lorenzo REL 1e-4 r512 zigzag cap 0.2 ->
+          Huffman<u16> (bklen 1024, coarse partitions of 768 symbols)
+    Wall-time compress/decompress GB/s, ratio, max error, the first four
+    parts against the oracle chain, per-kernel device times (library
+    dispatch timestamps) with each main kernel's algorithmic bytes and HBM
+    fraction, and the oracle's time on the host."""
+    import oracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import chains
+    from fzgpumodules_amd import fields
+    from fzgpumodules_amd._fzh import DeviceBuffer, call, lib
+    from fzgpumodules_amd.pipeline import Pipeline
+
+    t = time.time()
+    if kind == "1d":
+        x = fields.sinusoid_1d()
+        dims = (x.size, 1, 1)
+        name = "sinusoid_1d float32 2^24 (8 sinusoids + N(0,1e-3)); Lorenzo1D ABS 1e-4 r32768 cap0.2 strict-eb -> RLE<u16>"
+    else:
+        x = fields.turbulence_3d(512)
+        dims = (512, 512, 512)
+        name = ("turbulence_3d float32 512^3 (GRF k^-5/3 + 1% spikes x100); Lorenzo3D REL 1e-4 r512 zigzag cap0.2 "
+                "strict-eb -> Huffman<u16> bklen 1024 coarse")
+    log(f"[bench] {kind}: field {x.shape} generated in {time.time() - t:.1f}s sha256={fields.sha256(x)[:16]}")
+    nbytes = x.nbytes
+    nv = x.size
+    d_in = DeviceBuffer.from_array(x)
+    p = Pipeline(nbytes)
+    if kind == "1d":
+        lz = p.add_lorenzo(1e-4, mode="ABS", quant_radius=32768, outlier_capacity=0.2)
+        co = p.add_rle(2)
+    else:
+        lz = p.add_lorenzo(1e-4, mode="REL", quant_radius=512, outlier_capacity=0.2, zigzag=True)
+        co = p.add_huffman(2, 1024)
+        p.set_dims(*dims)
+    p.set_strict(lz, True)
+    p.connect(co, lz, "codes")
+    p.finalize()
+    p.set_pool_managed_decomp_output(True)
+    steps = max(3, min(args.steps, 20))
+    for _ in range(2):
+        out_ptr, out_n = p.compress(d_in, nbytes)
+    call("fzh_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out_ptr, out_n = p.compress(d_in, nbytes)
+    call("fzh_synchronize")
+    comp_s = (time.perf_counter() - t0) / steps
+    outs = dict(p.outputs())
+    blob_h = np.empty(out_n, np.uint8)
+    call("fzh_memcpy_dtoh", blob_h.ctypes.data, out_ptr, out_n)
+    blob = DeviceBuffer.from_array(blob_h)
+    for _ in range(2):
+        y_ptr, y_n = p.decompress(blob, out_n)
+    call("fzh_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        y_ptr, y_n = p.decompress(blob, out_n)
+    call("fzh_synchronize")
+    dec_s = (time.perf_counter() - t0) / steps
+    y = np.empty(nv, np.float32)
+    call("fzh_memcpy_dtoh", y.ctypes.data, y_ptr, y.nbytes)
+    abs_eb = p.abs_error_bound(lz)
+    max_err = float(np.max(np.abs(y.astype(np.float64) - x.ravel().astype(np.float64))))
+    del y
+
+    # per-kernel device times over `reps` compress + decompress calls
+    reps = 10
+    call("fzh_kernel_timing_enable", 1)
+    for _ in range(reps):
+        p.compress(d_in, nbytes)
+    for _ in range(reps):
+        p.decompress(blob, out_n)
+    call("fzh_synchronize")
+    n = nv  # no padding: RLE and Huffman need no alignment
+    k = outs["outlier_errors"] // 4
+    kp = outs["patch_values"] // 4
+    Cc = outs["output"]
+    nd = "1d" if kind == "1d" else "3d"
+    main = {
+        "minmax_partial_kernel": 4 * nv,
+        f"lz{nd}_fwd_kernel": 4 * nv + 2 * n + 8 * k + 8 * kp + 8,
+        f"lz{nd}_inv_kernel": 2 * n + 8 * k + 4 * nv,
+    }
+    if kind == "1d":
+        R = int(blob_h[split_offset(blob_h, 0):][:4].view(np.uint32)[0])
+        main["rle_enc_runs_kernel"] = 2 * n
+        helpers = ["place_kernel", "scan_small_kernel", "scan_reduce_kernel", "scan_partials_kernel",
+                   "scan_apply_kernel", "rle_enc_lengths_kernel", "rle_dec_prep_kernel", "rle_dec_reduce_kernel",
+                   "rle_dec_apply_kernel", "rle_dec_expand_kernel", "concat_readback_kernel",
+                   "readback_kernel", "apply_patches_kernel", "bin_count_kernel", "bin_place_kernel"]
+    else:
+        R = None
+        main["hist_kernel"] = 2 * n
+        main["huf_enc_kernel"] = 2 * n + Cc
+        main["huf_dec_kernel"] = Cc + 2 * n
+        helpers = ["minmax_final_kernel", "place_kernel", "scan_small_kernel", "scan_reduce_kernel",
+                   "scan_partials_kernel", "scan_apply_kernel", "huf_pack_kernel", "huf_header_kernel",
+                   "concat_readback_kernel", "readback_kernel", "apply_patches_kernel", "bin_count_kernel",
+                   "bin_place_kernel"]
+    kern = {}
+    tot, cnt_ = C.c_double(), C.c_uint64()
+    for kname in list(main) + helpers:
+        call("fzh_kernel_timing_get", kname.encode(), C.byref(tot), C.byref(cnt_))
+        if cnt_.value == 0:
+            continue
+        ms = tot.value / cnt_.value
+        e = {"us": round(ms * 1e3, 2), "launches_per_call": round(cnt_.value / reps, 2)}
+        if kname in main:
+            gbs = main[kname] / (ms * 1e-3) / 1e9
+            e.update(bytes=int(main[kname]), gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4))
+        kern[kname] = e
+        log(f"[bench] {kind}: kernel {kname:24s} {ms * 1e3:9.2f} us x {e['launches_per_call']}"
+            + (f"  {e['gbs']:8.1f} GB/s  {100 * e['frac']:5.1f}% of HBM peak" if "gbs" in e else ""))
+    call("fzh_kernel_timing_enable", 0)
+    p.close()
+    d_in.free()
+    blob.free()
+
+    # the first four parts against the oracle chain; the oracle's time on the host
+    info = cpu_info()
+    oracle.set_threads(info["threads"])
+    t0 = time.perf_counter()
+    if kind == "1d":
+        want, oinfo = chains.lorenzo_rle(x)
+    else:
+        want, oinfo = chains.lorenzo_huffman(x, dims)
+    cpu_s = time.perf_counter() - t0
+    got, ref = split_blob(blob_h), split_blob(want)
+    exact = len(got) == 7 and all(np.array_equal(got[i], ref[i]) for i in range(4))
+    res = {
+        "workload": name,
+        "input_bytes": int(nbytes),
+        "compressed_bytes": int(out_n),
+        "ratio": round(nbytes / out_n, 3),
+        "compress_gbs": round(nbytes / comp_s / 1e9, 2),
+        "compress_ms": round(comp_s * 1e3, 4),
+        "decompress_gbs": round(nbytes / dec_s / 1e9, 2),
+        "decompress_ms": round(dec_s * 1e3, 4),
+        "outputs": outs,
+        "outliers": int(oinfo["cnt"]),
+        "outlier_capacity": chains.max_outliers(n, 0.2),
+        "abs_eb": abs_eb,
+        "max_abs_err": max_err,
+        "within_eb": bool(max_err <= abs_eb),
+        "bit_exact_vs_oracle": bool(exact),
+        "kernels": kern,
+        "cpu_oracle": {"gbs": round(nbytes / cpu_s / 1e9, 3), "threads": info["threads"],
+                       "sample": "whole field, one run of the oracle chain (value base, Lorenzo, coder) "
+                                 "including its numpy padding copy"},
+    }
+    if R is not None:
+        res["runs"] = R
+    log(f"[bench] {kind}: compress {res['compress_gbs']} GB/s, decompress {res['decompress_gbs']} GB/s, ratio "
+        f"{res['ratio']}, max|err| {max_err:.3e} (eb {abs_eb:.3e}), outliers {res['outliers']}/"
+        f"{res['outlier_capacity']}, equal to the oracle: {exact}, cpu oracle {res['cpu_oracle']['gbs']} GB/s")
+    return res
+
+
+def split_offset(blob, i):
+    """Byte offset of part i of a pipeline result."""
+    n = int(blob[:4].view(np.uint32)[0])
+    sizes = blob[4:4 + 8 * n].view(np.uint64).astype(np.int64)
+    off = (4 + 8 * n + 15) // 16 * 16
+    for sz in sizes[:i]:
+        off += (int(sz) + 15) // 16 * 16
+    return off
 
 
 if __name__ == "__main__":

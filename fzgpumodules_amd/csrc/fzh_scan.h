@@ -295,9 +295,9 @@ inline int scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, size_t n, 
     const size_t nseg = std::max<size_t>(cdiv(n, SCAN_SEG), 1);
     if (nseg > SCAN_MAX_SEGS) return fail(FZH_E_INVALID, "scan: more than 2^28 elements");
     uint32_t* seg = (uint32_t*)ws;
-    scan_reduce_kernel<WRAP><<<(unsigned)nseg, 256, 0, s>>>(in, n, d_n, seg);
-    scan_partials_kernel<WRAP><<<1, 1024, 0, s>>>(seg, (uint32_t)nseg, d_total);
-    if (n) scan_apply_kernel<WRAP, CLEAR><<<(unsigned)nseg, 256, 0, s>>>(const_cast<uint32_t*>(in), out, n, d_n, seg);
+    launch_k("scan_reduce_kernel", scan_reduce_kernel<WRAP>, dim3((unsigned)nseg), dim3(256), 0, s, in, n, d_n, seg);
+    launch_k("scan_partials_kernel", scan_partials_kernel<WRAP>, dim3(1), dim3(1024), 0, s, seg, (uint32_t)nseg, d_total);
+    if (n) launch_k("scan_apply_kernel", scan_apply_kernel<WRAP, CLEAR>, dim3((unsigned)nseg), dim3(256), 0, s, const_cast<uint32_t*>(in), out, n, d_n, seg);
     return FZH_OK;
 }
 

@@ -378,12 +378,12 @@ int enc_impl(hipStream_t s, const void* in, size_t n, uint32_t bklen, const uint
     const int book_in_lds = bklen <= 8192 ? 1 : 0;
     const size_t lds = (size_t)4 * CELLS * 4 + (book_in_lds ? 4 * (size_t)bklen : 0);
     const unsigned blocks = (unsigned)cdiv(pardeg, 4);
-    huf_enc_kernel<T, SL><<<blocks, 256, lds, s>>>((const T*)in, n, bklen, bk4, rev, rev_bytes, o8, scells, ncell,
-                                                   pardeg, book_in_lds);
+    launch_k("huf_enc_kernel", huf_enc_kernel<T, SL>, dim3(blocks), dim3(256), (uint32_t)lds, s, (const T*)in, n, bklen, bk4, rev,
+             rev_bytes, o8, scells, ncell, pardeg, book_in_lds);
     if (int rc = scan_u32(s, ncell, reinterpret_cast<uint32_t*>(o8 + ent3), pardeg, nullptr, tot, w + L.o_scan))
         return rc;
-    huf_pack_kernel<<<blocks, 256, 0, s>>>(scells, ncell, o8, ent3, ent4, pardeg, (uint32_t)CELLS);
-    huf_header_kernel<<<1, 1024, 0, s>>>(o8, n, bklen, (uint32_t)SL, ent1, ent2, ent3, ent4, pardeg, tot, out_size);
+    launch_k("huf_pack_kernel", huf_pack_kernel, dim3(blocks), dim3(256), 0, s, scells, ncell, o8, ent3, ent4, pardeg, (uint32_t)CELLS);
+    launch_k("huf_header_kernel", huf_header_kernel, dim3(1), dim3(1024), 0, s, o8, n, bklen, (uint32_t)SL, ent1, ent2, ent3, ent4, pardeg, tot, out_size);
     FZH_CHECK_LAUNCH("huffman encode");
     return FZH_OK;
 }
@@ -418,9 +418,9 @@ int fzh_histogram(void* stream, const void* d_in, size_t n, int elem_bytes, uint
     hipStream_t s = S(stream);
 #define FZH_HIST(T)                                                                                       \
     if (replicas)                                                                                         \
-        hist_kernel<T><<<blocks, 256, 4 * (size_t)bklen * replicas, s>>>((const T*)d_in, n, bklen, d_freq, replicas); \
+        launch_k("hist_kernel", hist_kernel<T>, dim3(blocks), dim3(256), (uint32_t)(4 * bklen * replicas), s, (const T*)d_in, n, bklen, d_freq, replicas); \
     else                                                                                                  \
-        hist_global_kernel<T><<<blocks, 256, 0, s>>>((const T*)d_in, n, bklen, d_freq);
+        launch_k("hist_global_kernel", hist_global_kernel<T>, dim3(blocks), dim3(256), 0, s, (const T*)d_in, n, bklen, d_freq);
     switch (elem_bytes) {
     case 1: FZH_HIST(uint8_t) break;
     case 2: FZH_HIST(uint16_t) break;
@@ -465,9 +465,9 @@ int fzh_huffman_decode(void* stream, const void* d_in, size_t in_bytes, int elem
     const unsigned blocks = (unsigned)cdiv(pardeg, 256);
     hipStream_t s = S(stream);
     switch (elem_bytes) {
-    case 1: huf_dec_kernel<uint8_t><<<blocks, 256, 0, s>>>((const uint8_t*)d_in, in_bytes, (uint8_t*)d_out, n, d_status); break;
-    case 2: huf_dec_kernel<uint16_t><<<blocks, 256, 0, s>>>((const uint8_t*)d_in, in_bytes, (uint16_t*)d_out, n, d_status); break;
-    case 4: huf_dec_kernel<uint32_t><<<blocks, 256, 0, s>>>((const uint8_t*)d_in, in_bytes, (uint32_t*)d_out, n, d_status); break;
+    case 1: launch_k("huf_dec_kernel", huf_dec_kernel<uint8_t>, dim3(blocks), dim3(256), 0, s, (const uint8_t*)d_in, in_bytes, (uint8_t*)d_out, n, d_status); break;
+    case 2: launch_k("huf_dec_kernel", huf_dec_kernel<uint16_t>, dim3(blocks), dim3(256), 0, s, (const uint8_t*)d_in, in_bytes, (uint16_t*)d_out, n, d_status); break;
+    case 4: launch_k("huf_dec_kernel", huf_dec_kernel<uint32_t>, dim3(blocks), dim3(256), 0, s, (const uint8_t*)d_in, in_bytes, (uint32_t*)d_out, n, d_status); break;
     default: return fail(FZH_E_INVALID, "huffman dec: elem_bytes must be 1, 2 or 4");
     }
     FZH_CHECK_LAUNCH("huf_dec_kernel");

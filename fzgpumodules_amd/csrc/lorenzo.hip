@@ -990,7 +990,7 @@ int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode, floa
     float2* partial = (float2*)((uint8_t*)d_ws + 256);
     launch_k("minmax_partial_kernel", minmax_partial_kernel, dim3(blocks), dim3(256), 0, S(stream), d_in, n, partial,
              zero_pad);
-    minmax_final_kernel<<<1, 256, 0, S(stream)>>>(partial, blocks, mode, user_eb, d_params);
+    launch_k("minmax_final_kernel", minmax_final_kernel, dim3(1), dim3(256), 0, S(stream), partial, blocks, mode, user_eb, d_params);
     FZH_CHECK_LAUNCH("minmax");
     return FZH_OK;
 }

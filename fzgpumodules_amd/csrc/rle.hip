@@ -303,7 +303,7 @@ int enc_impl(hipStream_t s, const void* in, size_t n, void* out, size_t cap, uin
     if (ws_bytes < L.total) return fail(FZH_E_CAPACITY, "RLE enc: workspace too small");
     if (cap < fzh_rle_enc_bound(n, sizeof(T))) return fail(FZH_E_CAPACITY, "RLE enc: output below fzh_rle_enc_bound()");
     if (n == 0) {
-        rle_enc_empty_kernel<<<1, 64, 0, s>>>((uint8_t*)out, out_size);
+        launch_k("rle_enc_empty_kernel", rle_enc_empty_kernel, dim3(1), dim3(64), 0, s, (uint8_t*)out, out_size);
         FZH_CHECK_LAUNCH("rle_enc_empty_kernel");
         return FZH_OK;
     }
@@ -316,11 +316,11 @@ int enc_impl(hipStream_t s, const void* in, size_t n, void* out, size_t cap, uin
     uint32_t* pos = (uint32_t*)(w + L.o_pos);
     uint32_t* dR = (uint32_t*)(w + L.o_R);
     const unsigned blocks = (unsigned)cdiv(ntiles, 4);
-    rle_enc_runs_kernel<T><<<blocks, 256, 0, s>>>((const T*)in, n, spos, scnt, ntiles);
+    launch_k("rle_enc_runs_kernel", rle_enc_runs_kernel<T>, dim3(blocks), dim3(256), 0, s, (const T*)in, n, spos, scnt, ntiles);
     CompactList L0{nullptr, spos, scnt, nullptr, pos, dR, n};
     if (int rc = launch_compact(s, w + L.o_cst, ntiles, (uint32_t)TILE, L0, nullptr)) return rc;
     const unsigned lb = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256), 2048));
-    rle_enc_lengths_kernel<T><<<lb, 256, 0, s>>>((const T*)in, (uint8_t*)out, pos, dR, n, out_size);
+    launch_k("rle_enc_lengths_kernel", rle_enc_lengths_kernel<T>, dim3(lb), dim3(256), 0, s, (const T*)in, (uint8_t*)out, pos, dR, n, out_size);
     FZH_CHECK_LAUNCH("rle encode");
     return FZH_OK;
 }
@@ -358,12 +358,12 @@ int dec_impl(hipStream_t s, const void* in, size_t in_bytes, void* out, size_t n
     uint32_t* start = (uint32_t*)(w + L.o_start);
     uint32_t* tf = (uint32_t*)(w + L.o_tf);
     const uint8_t* bin = (const uint8_t*)in;
-    rle_dec_prep_kernel<T><<<1, 64, 0, s>>>(bin, in_bytes, n, Rv, d_status);
-    rle_dec_reduce_kernel<T><<<(unsigned)L.nseg, 256, 0, s>>>(bin, Rv, seg);
-    scan_partials_kernel<false><<<1, 1024, 0, s>>>(seg, (uint32_t)L.nseg, tot);
-    rle_dec_apply_kernel<T><<<(unsigned)L.nseg, 256, 0, s>>>(bin, n, Rv, seg, tot, start, tf, d_status);
+    launch_k("rle_dec_prep_kernel", rle_dec_prep_kernel<T>, dim3(1), dim3(64), 0, s, bin, in_bytes, n, Rv, d_status);
+    launch_k("rle_dec_reduce_kernel", rle_dec_reduce_kernel<T>, dim3((unsigned)L.nseg), dim3(256), 0, s, bin, Rv, seg);
+    launch_k("scan_partials_kernel", scan_partials_kernel<false>, dim3(1), dim3(1024), 0, s, seg, (uint32_t)L.nseg, tot);
+    launch_k("rle_dec_apply_kernel", rle_dec_apply_kernel<T>, dim3((unsigned)L.nseg), dim3(256), 0, s, bin, n, Rv, seg, tot, start, tf, d_status);
     const unsigned eb = (unsigned)cdiv(L.ntiles, 2);
-    rle_dec_expand_kernel<T><<<eb, 128, 0, s>>>((const uint8_t*)in, n, start, tf, d_status, (T*)out, (uint32_t)L.ntiles);
+    launch_k("rle_dec_expand_kernel", rle_dec_expand_kernel<T>, dim3(eb), dim3(128), 0, s, (const uint8_t*)in, n, start, tf, d_status, (T*)out, (uint32_t)L.ntiles);
     FZH_CHECK_LAUNCH("rle decode");
     return FZH_OK;
 }

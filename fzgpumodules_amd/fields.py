@@ -61,7 +61,7 @@ def power_law_field(shape, expo: float, seed: int) -> np.ndarray:
     amp[(0,) * len(shape)] = 0.0
     F *= amp
     del amp, k2
-    g = np.fft.irfftn(F, s=shape)
+    g = np.fft.irfftn(F, s=shape, axes=tuple(range(len(shape))))
     sd = g.std()
     return g / (sd if sd > 0 else 1.0)
 
