@@ -99,9 +99,7 @@ def test_difference_float_inverse_exact_sums(dt):
 
 def test_difference_float_association_is_not_sequential():
     """The raking association differs from a left-to-right running sum on
-    general floats (so the order is actually exercised): for some input the
-    two disagree, and the raking one equals an element-by-element restatement
-    of one tile."""
+    general floats, so the order is actually exercised."""
     rng = np.random.default_rng(6)
     d = (rng.standard_normal(512) * 10 ** rng.uniform(-3, 3, 512)).astype(np.float32)
     rak = T.diff_inv(d, np.float32)
@@ -111,22 +109,127 @@ def test_difference_float_association_is_not_sequential():
         acc = np.float32(acc + v) if i else v
         seq[i] = acc
     assert not np.array_equal(rak, seq)
-    # scalar restatement of one tile
-    part = [d[16 * s] for s in range(32)]
-    for s in range(32):
+
+
+# Known answers of the inverse difference's float association
+# (diff.cu:75-103: cub::BlockScan<float, 512>::InclusiveSum per 512-element
+# tile -- raking: 32 segments of 16 summed left to right, a Kogge-Stone scan
+# of the segment sums with offsets 1, 2, 4, 8, 16, each segment's elements
+# added left to right onto the previous segments' sum -- then
+# `scan + prefix` with prefix = 0 + the earlier tiles' aggregates, left to
+# right).  B = 2^24: B + 1 rounds to B, B + 3 and B + 5 round to B + 4 (ties
+# to even), B + 2 and B + 4 are exact.  Every expected value below is worked
+# out by hand from those rules; each case differs from what a plausible
+# wrong association gives (noted per case).
+B = np.float32(2.0 ** 24)
+
+
+def _inv(d):
+    return T.diff_inv(np.asarray(d, np.float32), np.float32)
+
+
+def test_difference_association_serial_inside_a_segment():
+    # segment 0 = [B, 1, 1]: B + 1 = B, then B + 1 = B (a tree inside the
+    # segment would give (1 + 1) + B = B + 2 at index 2)
+    d = np.zeros(512, np.float32)
+    d[0:3] = [B, 1, 1]
+    out = _inv(d)
+    assert np.array_equal(out, np.full(512, B, np.float32))
+
+
+def test_difference_association_kogge_stone_over_segments():
+    # segment sums p = [B, 1, 1, 0, ...] (d[0] = B, d[16] = d[32] = 1).
+    # Offset 1: [B, B, 2, 1, 0 ...]; offset 2: [B, B, B+2, B, 2, 1, 0 ...];
+    # offset 4: [B, B, B+2, B, B+2, B, B+2, B, 2, 1, 0 ...]; offsets 8 and 16
+    # carry the same pattern on, so inc[s] = B for s = 0, 1 and every odd s,
+    # B + 2 for every even s >= 2.  Segment s >= 1 outputs inc[s-1] + its
+    # elements: s = 1, 2: B + 1 = B; s >= 3: B + 2 when s is odd, else B.
+    # (A sequential scan gives B everywhere; the exact sums are B + 2.)
+    d = np.zeros(512, np.float32)
+    d[0], d[16], d[32] = B, 1, 1
+    want = np.full(512, B, np.float32)
+    for s in range(3, 32, 2):
+        want[16 * s:16 * s + 16] = B + 2
+    assert np.array_equal(_inv(d), want)
+
+
+def test_difference_association_segment_elements_after_the_prefix():
+    # segment 1 = [1, 1]: its outputs are (inc[0] + 1) = B, then B + 1 = B;
+    # inc[1] = B + 2 (exact), so segments 2.. output B + 2.  (Scanning the
+    # segment locally and adding the prefix last gives B + 2 at index 17.)
+    d = np.zeros(512, np.float32)
+    d[0], d[16], d[17] = B, 1, 1
+    want = np.full(512, B + 2, np.float32)
+    want[:32] = B
+    assert np.array_equal(_inv(d), want)
+
+
+def test_difference_association_tile_prefix():
+    # tiles: [B, 0..] [1, 1, 0..] [1, 0..] [1, 0..] [0..]
+    # tile 1: local scan [1, 2, 2 ..] + prefix B -> [B, B+2, B+2 ..];
+    # prefix B + 2; tile 2: 1 + (B+2) = B+3 -> B+4; prefix (B+2) + 1 -> B+4;
+    # tile 3: 1 + (B+4) = B+5 -> B+4; prefix -> B+4; tile 4: B+4.
+    # (A running sum over the whole chunk stays at B; updating the prefix
+    # before adding it gives B+4 already at index 512.)
+    d = np.zeros(5 * 512, np.float32)
+    d[0] = B
+    d[512], d[513] = 1, 1
+    d[1024] = 1
+    d[1536] = 1
+    want = np.empty_like(d)
+    want[:512] = B
+    want[512] = B
+    want[513:1024] = B + 2
+    want[1024:] = B + 4
+    assert np.array_equal(_inv(d), want)
+    # chunks restart the prefix: the same tiles as chunks of 512 elements
+    # are each scanned on their own
+    got = T.diff_inv(d, np.float32, 512)
+    want_c = np.zeros_like(d)
+    want_c[:512] = B
+    want_c[512] = 1
+    want_c[513:1024] = 2
+    want_c[1024:1536] = 1
+    want_c[1536:2048] = 1
+    assert np.array_equal(got, want_c)
+
+
+def test_difference_association_negative_zero():
+    # every tile adds its prefix (+0.0 for the first): -0.0 + 0.0 = +0.0
+    got = _inv(np.full(1024, -0.0, np.float32))
+    assert np.all(got == 0) and not np.any(np.signbit(got))
+    got = T.diff_inv(np.full(1024, -0.0, np.float32), np.float32, 512)
+    assert not np.any(np.signbit(got))
+
+
+def test_difference_association_known_answers_reject_mutants(monkeypatch):
+    """The known answers above fail an oracle whose segment sums are combined
+    by a sequential scan instead of Kogge-Stone, and one whose tiles add the
+    prefix inside the scan (the mistakes the cases are built to catch)."""
+    real = T._raking_tile_scan
+
+    def sequential_partials(t):
+        seg = t.reshape(t.shape[0], 32, 16)
+        part = seg[:, :, 0].copy()
         for j in range(1, 16):
-            part[s] = np.float32(part[s] + d[16 * s + j])
-    inc = list(part)
-    for o in (1, 2, 4, 8, 16):
-        inc = [np.float32(inc[s - o] + inc[s]) if s >= o else inc[s] for s in range(32)]
-    want = np.empty(512, np.float32)
-    for s in range(32):
-        run = d[16 * s] if s == 0 else np.float32(inc[s - 1] + d[16 * s])
-        want[16 * s] = np.float32(run + np.float32(0))
+            part = part + seg[:, :, j]
+        inc = part.copy()
+        for s_ in range(1, 32):
+            inc[:, s_] = inc[:, s_ - 1] + part[:, s_]
+        out = np.empty_like(seg)
+        run = seg[:, :, 0].copy()
+        run[:, 1:] = inc[:, :-1] + seg[:, 1:, 0]
+        out[:, :, 0] = run
         for j in range(1, 16):
-            run = np.float32(run + d[16 * s + j])
-            want[16 * s + j] = np.float32(run + np.float32(0))
-    assert np.array_equal(rak, want)
+            run = run + seg[:, :, j]
+            out[:, :, j] = run
+        return out.reshape(t.shape[0], 512), inc[:, 31].copy()
+
+    monkeypatch.setattr(T, "_raking_tile_scan", sequential_partials)
+    with pytest.raises(AssertionError):
+        test_difference_association_kogge_stone_over_segments()
+    monkeypatch.setattr(T, "_raking_tile_scan", real)
+    test_difference_association_kogge_stone_over_segments()
 
 
 @pytest.mark.parametrize("dims", [(1000, 1, 1), (64, 64, 1), (37, 23, 1), (16, 16, 16), (9, 7, 5)])
