@@ -41,6 +41,7 @@ SIGNATURES = {
     "fzh_lorenzo_quant_inv_apply_f32": (_I, [_P, _P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _F, _I, _I, _P, _P, _Z]),
     "fzh_value_base_ws_bytes": (_Z, [_Z]),
     "fzh_value_base_f32": (_I, [_P, _P, _Z, _I, _F, _P, _P, _Z]),
+    "fzh_value_base_f32_noalloc": (_I, [_P, _P, _Z, _I, _F, _P]),
     "fzh_bitshuffle_enc": (_I, [_P, _P, _P, _Z, _U32, _I]),
     "fzh_bitshuffle_dec": (_I, [_P, _P, _P, _Z, _U32, _I]),
     "fzh_rze_enc_bound": (_Z, [_Z]),

@@ -16,6 +16,8 @@
 // kernel places the lists, so the outlier list is deterministic: tile-major,
 // and sorted by element index inside each tile (the reference appends them
 // with atomics in arbitrary order).
+#include <mutex>
+
 #include "fzh_compact.h"
 #include "fzh_lorenzo_tile.h"
 
@@ -110,6 +112,9 @@ lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t
 // reduction starts from them.  Pass 2: one workgroup reduces the partials
 // and resolves the bound -- two launches instead of a last-workgroup atomic,
 // whose device-scope fences are costly across the XCDs' separate L2s.
+// Most workgroups of pass 1 (so at most this many partial pairs).
+constexpr unsigned VB_MAX_PARTIALS = FZH_VB_NOALLOC_PARTIALS;
+
 __global__ void __launch_bounds__(256)
 minmax_partial_kernel(const float* __restrict__ in, uint64_t n, float2* __restrict__ partial, bool zero_pad) {
     __shared__ float smn[4], smx[4];
@@ -800,6 +805,66 @@ int inv_apply(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny, 
     return FZH_OK;
 }
 
+// Pass 1 writes one (min, max) pair per workgroup into `partial` (every
+// pair pass 2 reads is written by the same call, so nothing needs clearing);
+// about 16 float4 per thread, at most VB_MAX_PARTIALS workgroups.
+int value_base_launch(hipStream_t s, const float* d_in, size_t n, int mode, float user_eb, float* d_params,
+                      float2* partial) {
+    const bool zero_pad = (mode & FZH_VB_ZERO_PADDED) != 0;
+    mode &= ~FZH_VB_ZERO_PADDED;
+    if (mode != 1 && mode != 2) return fail(FZH_E_INVALID, "value base: mode must be 1 (REL) or 2 (NOA)");
+    if (n == 0) return fail(FZH_E_INVALID, "value base: empty input");
+    if (!d_in || !d_params) return fail(FZH_E_INVALID, "value base: null pointer");
+    if (reinterpret_cast<uintptr_t>(d_in) % 16 != 0) return fail(FZH_E_INVALID, "value base: input must be 16-byte aligned");
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256 * 64), VB_MAX_PARTIALS));
+    launch_k("minmax_partial_kernel", minmax_partial_kernel, dim3(blocks), dim3(256), 0, s, d_in, n, partial,
+             zero_pad);
+    launch_k("minmax_final_kernel", minmax_final_kernel, dim3(1), dim3(256), 0, s, partial, blocks, mode, user_eb,
+             d_params);
+    FZH_CHECK_LAUNCH("minmax");
+    return FZH_OK;
+}
+
+}  // namespace
+}  // namespace fzh
+
+// The partials of fzh_value_base_f32_noalloc: one fixed array per GPU, held
+// until the process ends.  Host code never passes this name to a kernel (on
+// the host it is only a handle); it resolves the array's device address per
+// device with hipGetSymbolAddress and uses that.
+static __device__ __align__(256) float2 scratch[fzh::VB_MAX_PARTIALS];
+
+namespace fzh {
+namespace {
+
+constexpr int VB_MAX_DEVICES = 64;
+std::mutex g_vb_mu;
+float2* g_vb_scratch[VB_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the device that runs stream s (the current
+// device for the null stream), resolved once per device with that device
+// current.
+int vb_scratch_address(hipStream_t s, float2** out) {
+    int cur = 0;
+    FZH_TRY_HIP(hipGetDevice(&cur));
+    int dev = cur;
+    if (s) FZH_TRY_HIP(hipStreamGetDevice(s, &dev));
+    if (dev < 0 || dev >= VB_MAX_DEVICES) return fail(FZH_E_INVALID, "value base (no-alloc): device ordinal out of range");
+    std::lock_guard<std::mutex> g(g_vb_mu);
+    if (!g_vb_scratch[dev]) {
+        if (dev != cur) FZH_TRY_HIP(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (dev != cur) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(FZH_E_RUNTIME, std::string("value base (no-alloc): hipGetSymbolAddress(scratch): ") +
+                                           hipGetErrorString(e));
+        g_vb_scratch[dev] = static_cast<float2*>(p);
+    }
+    *out = g_vb_scratch[dev];
+    return FZH_OK;
+}
+
 }  // namespace
 }  // namespace fzh
 
@@ -980,19 +1045,15 @@ size_t fzh_value_base_ws_bytes(size_t n) {
 int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode, float user_eb, float* d_params,
                        void* d_ws, size_t ws_bytes) {
     if (ws_bytes < fzh_value_base_ws_bytes(n)) return fail(FZH_E_CAPACITY, "value base: workspace too small");
-    const bool zero_pad = (mode & FZH_VB_ZERO_PADDED) != 0;
-    mode &= ~FZH_VB_ZERO_PADDED;
-    if (mode != 1 && mode != 2) return fail(FZH_E_INVALID, "value base: mode must be 1 (REL) or 2 (NOA)");
-    if (n == 0) return fail(FZH_E_INVALID, "value base: empty input");
-    if (reinterpret_cast<uintptr_t>(d_in) % 16 != 0) return fail(FZH_E_INVALID, "value base: input must be 16-byte aligned");
-    // about 16 float4 per thread, at most two workgroups per CU
-    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256 * 64), 512));
-    float2* partial = (float2*)((uint8_t*)d_ws + 256);
-    launch_k("minmax_partial_kernel", minmax_partial_kernel, dim3(blocks), dim3(256), 0, S(stream), d_in, n, partial,
-             zero_pad);
-    launch_k("minmax_final_kernel", minmax_final_kernel, dim3(1), dim3(256), 0, S(stream), partial, blocks, mode, user_eb, d_params);
-    FZH_CHECK_LAUNCH("minmax");
-    return FZH_OK;
+    if (!d_ws) return fail(FZH_E_INVALID, "value base: null workspace");
+    return value_base_launch(S(stream), d_in, n, mode, user_eb, d_params, (float2*)((uint8_t*)d_ws + 256));
+}
+
+int fzh_value_base_f32_noalloc(void* stream, const float* d_in, size_t n, int mode, float user_eb,
+                               float* d_params) {
+    float2* partial = nullptr;
+    if (int rc = vb_scratch_address(S(stream), &partial)) return rc;
+    return value_base_launch(S(stream), d_in, n, mode, user_eb, d_params, partial);
 }
 
 }  // extern "C"

@@ -169,6 +169,20 @@ int fzh_lorenzo_quant_inv_apply_f32(void* stream, const void* d_codes, size_t n,
 size_t fzh_value_base_ws_bytes(size_t n);
 int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode,
                        float user_eb, float* d_params, void* d_ws, size_t ws_bytes);
+/* fzh_value_base_f32 without a workspace argument and without any
+ * allocation (no counterpart in the reference; for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same kernels, same d_params.  Its scratch is a
+ * fixed 4 KiB device array per GPU (FZH_VB_NOALLOC_PARTIALS min/max pairs,
+ * one per workgroup of the first pass; the grid is capped at that many
+ * workgroups, so every n fits), kept until the process ends; every pair the
+ * second pass reads is written by the same call, so nothing is cleared.
+ * All calls on one GPU share that array: calls that may overlap on one GPU
+ * (different streams, or host threads) race and must be serialised by the
+ * caller.  The stream's device (the current device for NULL) is used.      */
+#define FZH_VB_NOALLOC_PARTIALS 512
+int fzh_value_base_f32_noalloc(void* stream, const float* d_in, size_t n, int mode,
+                               float user_eb, float* d_params);
 
 /* ---- Bitshuffle (block_size multiple of 1024*elem_width; tail copied) ---- */
 int fzh_bitshuffle_enc(void* stream, const void* d_in, void* d_out,
