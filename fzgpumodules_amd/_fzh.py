@@ -161,6 +161,7 @@ SIGNATURES = {
     "fzh_scan_ws_bytes": (C.c_size_t, [C.c_uint64]),
     "fzh_scan_u32": (_I, [_P, _P, _P, C.c_uint64, _P, C.c_int, _P, C.c_size_t]),
     "fzh_periodic_window_f32": (_I, [_P, _P] + [C.c_uint64] * 6 + [_P] + [C.c_uint64] * 3),
+    "fzh_synth_power_law_f32": (_I, [_P, _P, C.c_uint64, C.c_uint64, C.c_uint64, _F, C.c_uint64]),
     "fzh_device_alloc": (_I, [C.POINTER(_P), _Z]),
     "fzh_device_free": (_I, [_P]),
     "fzh_memcpy_htod": (_I, [_P, _P, _Z]),

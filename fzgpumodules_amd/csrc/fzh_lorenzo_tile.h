@@ -50,7 +50,18 @@ __device__ __forceinline__ float dec_code(TCode c) {
     }
 }
 
-__device__ __forceinline__ float quant_round(float x, float r) { return __builtin_roundf(x * r); }
+// round(x * r), halves away from zero (lorenzo_quant.cu:120-135 uses
+// round()).  For every float v, round(v) == trunc(v + copysign(c, v)) with c
+// the largest float below 1/2 under round-to-nearest-even addition: a
+// fraction below 1/2 stays below the next integer (its distance to it is at
+// least one ulp of v plus 2^-25), a fraction of 1/2 or more reaches it, and
+// |v| >= 2^23 is unchanged -- three instructions where roundf takes six.
+// Checked against round() over all 2^32 floats.  The product is rounded
+// first (no fused multiply-add: the build disables contraction).
+__device__ __forceinline__ float quant_round(float x, float r) {
+    const float v = x * r;
+    return __builtin_truncf(v + __builtin_copysignf(0.49999997f, v));
+}
 
 // Strict error bound (an extension; no counterpart in the reference).  The
 // inverse reconstructs element i as fl(S_i * ebx2), S_i the exact integer
@@ -104,6 +115,15 @@ __device__ __forceinline__ bool needs_patch_fast(float x, float s, StrictEb se, 
     if (ef < b.lo) return false;
     if (ef > b.hi) return true;
     return needs_patch(x, s, se.eb, se.ebx2);
+}
+
+// The same test split for wave-wide use: `sure` = patched by the FP32 test
+// alone; `unsure` = the FP32 difference is within [lo, hi] or NaN, where the
+// exact FP64 test decides.  needs_patch_fast == sure || (unsure && needs_patch).
+__device__ __forceinline__ void patch_fast(float x, float s, StrictEb se, StrictBand b, bool& sure, bool& unsure) {
+    const float ef = fabsf(s * se.ebx2 - x);
+    sure = ef > b.hi;
+    unsure = !(ef < b.lo) && !sure;
 }
 
 // Zero codes[from, to) (tail beyond nx*ny[*nz] when the input was padded).
@@ -236,8 +256,43 @@ __device__ __forceinline__ TileCounts lz1d_tile(const float* __restrict__ in, ui
 // than 2^32 bytes) run a variant with no bounds tests whose loads and code
 // stores address from the tile's scalar base with 32-bit lane offsets.
 // ---------------------------------------------------------------------------
+// Whether a 2D tile lies inside the field with rows spanning less than 2^32
+// bytes (32-bit lane offsets from the tile's scalar base).
+__device__ __forceinline__ bool lz2d_interior(uint32_t nx, uint32_t ny, uint32_t bx, uint32_t by) {
+    return bx * 32 + 32 <= nx && by * 32 + 32 <= ny && nx < (1u << 25);
+}
+
+// The 16 values of tile `tile` this lane holds (column l & 31, rows
+// 16 (l >> 5) .. +15 of the tile; zero outside the field).
+template <bool INTERIOR>
+__device__ __forceinline__ void lz2d_load(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t tile,
+                                          uint32_t gx, float (&xv)[16]) {
+    const int l = lane_id();
+    const int x = l & 31, h = l >> 5;
+    const uint32_t bx = tile % gx, by = tile / gx;
+    const uint32_t X = bx * 32 + x;
+    const uint32_t Y0 = by * 32 + 16 * h;
+    const uint64_t tbase = (uint64_t)(by * 32) * nx + bx * 32;
+    if constexpr (INTERIOR) {
+        // row i of both halves from the row's scalar base: no per-load address arithmetic
+        const uint32_t lb = 4u * ((uint32_t)(16 * h) * nx + (uint32_t)x);
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const char* rp = reinterpret_cast<const char*>(in + tbase + (uint64_t)i * nx);
+            xv[i] = *reinterpret_cast<const float*>(rp + lb);
+        }
+    } else {
+        const bool xin = X < nx;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t Y = Y0 + i;
+            xv[i] = (xin && Y < ny) ? in[(uint64_t)Y * nx + X] : 0.0f;
+        }
+    }
+}
+
 template <typename TCode, bool ZZ, bool STRICT, bool INTERIOR, bool CODES>
-__device__ __forceinline__ TileCounts lz2d_tile_v(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n,
+__device__ __forceinline__ TileCounts lz2d_tile_c(const float (&xin16)[16], uint32_t nx, uint32_t ny, uint64_t n,
                                                   uint32_t tile, uint32_t gx, bool last, float r, float rad,
                                                   TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
     const int l = lane_id();
@@ -249,18 +304,8 @@ __device__ __forceinline__ TileCounts lz2d_tile_v(const float* __restrict__ in, 
     const uint64_t tbase = (uint64_t)(by * 32) * nx + bx * 32;  // the tile's first element (scalar)
     const uint32_t loff = (uint32_t)(16 * h) * nx + (uint32_t)x;  // this lane's first element in the tile
     float s[16], xv[16];
-    if constexpr (INTERIOR) {
-        const char* tp = reinterpret_cast<const char*>(in + tbase);
 #pragma unroll
-        for (int i = 0; i < 16; i++)
-            xv[i] = *reinterpret_cast<const float*>(tp + 4u * (loff + (uint32_t)i * nx));
-    } else {
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t Y = Y0 + i;
-            xv[i] = (xin && Y < ny) ? in[(uint64_t)Y * nx + X] : 0.0f;
-        }
-    }
+    for (int i = 0; i < 16; i++) xv[i] = xin16[i];
 #pragma unroll
     for (int i = 0; i < 16; i++) s[i] = quant_round(xv[i], r);
     // t = s(y) - s(y-1) inside the tile (row 0 of the tile subtracts 0).
@@ -269,9 +314,8 @@ __device__ __forceinline__ TileCounts lz2d_tile_v(const float* __restrict__ in, 
     t[0] = s[0] - (h ? north : 0.0f);
 #pragma unroll
     for (int i = 1; i < 16; i++) t[i] = s[i] - s[i - 1];
-    uint32_t mask = 0;
     uint64_t bal[16];
-    char* ctp = CODES ? reinterpret_cast<char*>(codes + tbase) : nullptr;
+    const uint32_t cb = (uint32_t)sizeof(TCode) * loff;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         const float w = wave_shr1(t[i]);  // lane x - 1 (x > 0; lane 32's is unused)
@@ -281,18 +325,18 @@ __device__ __forceinline__ TileCounts lz2d_tile_v(const float* __restrict__ in, 
         if constexpr (CODES) {
             const TCode c = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
             if constexpr (INTERIOR) {
-                *reinterpret_cast<TCode*>(ctp + (uint32_t)sizeof(TCode) * (loff + (uint32_t)i * nx)) = c;
+                char* cp = reinterpret_cast<char*>(codes + tbase + (uint64_t)i * nx);
+                *reinterpret_cast<TCode*>(cp + cb) = c;
             } else if (valid) {
                 codes[(uint64_t)(Y0 + i) * nx + X] = c;
             }
         }
-        const bool is_out = valid && !ok;
-        mask |= (uint32_t)is_out << i;
-        bal[i] = __ballot(is_out);
+        bal[i] = __ballot(valid && !ok);
         t[i] = d;  // keep the delta for the outlier pass
     }
-    // Row-major order inside the tile: rows 0..15 (h=0) then 16..31 (h=1).
-    auto emit = [&](const uint64_t (&bb)[16], uint32_t m, bool patch, const ListOut& o) {
+    // Row-major order inside the tile: rows 0..15 (h=0) then 16..31 (h=1);
+    // a lane's own flags are its bits of the ballots.
+    auto emit = [&](const uint64_t (&bb)[16], bool patch, const ListOut& o) {
         uint32_t tot = 0, lw = 0;
 #pragma unroll
         for (int i = 0; i < 16; i++) {
@@ -305,7 +349,7 @@ __device__ __forceinline__ TileCounts lz2d_tile_v(const float* __restrict__ in, 
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const uint32_t half = h ? (uint32_t)(bb[i] >> 32) : (uint32_t)bb[i];
-                if (m & (1u << i)) {
+                if ((half >> x) & 1u) {
                     const uint32_t pos = run + __popc(half & below);
                     if (pos < o.cap) {
                         o.val[pos] = patch ? xv[i] : t[i];
@@ -317,26 +361,46 @@ __device__ __forceinline__ TileCounts lz2d_tile_v(const float* __restrict__ in, 
         }
         return tot;
     };
-    TileCounts tc{emit(bal, mask, false, lo), 0u};
+    TileCounts tc{emit(bal, false, lo), 0u};
     if (CODES && last) zero_code_tail(codes, (uint64_t)nx * ny, n);
     if constexpr (STRICT) {
         bool huge = false;
 #pragma unroll
         for (int i = 0; i < 16; i++) huge |= !(fabsf(s[i]) < kExactS);
         const bool tile_huge = __ballot(huge) != 0ull;
-        uint32_t pm = 0;
         uint64_t pbal[16];
+        uint64_t unsure = 0;
         const StrictBand band = strict_band(se);
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const bool valid = INTERIOR || (xin && (Y0 + i) < ny);
-            const bool p = valid && (tile_huge || needs_patch_fast(xv[i], s[i], se, band));
-            pm |= (uint32_t)p << i;
-            pbal[i] = __ballot(p);
+            bool sure, uns;
+            patch_fast(xv[i], s[i], se, band, sure, uns);
+            pbal[i] = __ballot(valid && (tile_huge || sure));
+            unsure |= __ballot(valid && !tile_huge && uns);
         }
-        tc.pcnt = emit(pbal, pm, true, lp);
+        if (unsure) {  // rare: elements within 2^-22 of the bound take the exact test
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const bool valid = INTERIOR || (xin && (Y0 + i) < ny);
+                bool sure, uns;
+                patch_fast(xv[i], s[i], se, band, sure, uns);
+                pbal[i] = __ballot(valid && (tile_huge || sure || (uns && needs_patch(xv[i], s[i], se.eb, se.ebx2))));
+            }
+        }
+        tc.pcnt = emit(pbal, true, lp);
     }
     return tc;
+}
+
+
+template <typename TCode, bool ZZ, bool STRICT, bool INTERIOR, bool CODES>
+__device__ __forceinline__ TileCounts lz2d_tile_v(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t n,
+                                                  uint32_t tile, uint32_t gx, bool last, float r, float rad,
+                                                  TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
+    float xv[16];
+    lz2d_load<INTERIOR>(in, nx, ny, tile, gx, xv);
+    return lz2d_tile_c<TCode, ZZ, STRICT, INTERIOR, CODES>(xv, nx, ny, n, tile, gx, last, r, rad, codes, lo, lp, se);
 }
 
 template <typename TCode, bool ZZ, bool STRICT>
@@ -344,8 +408,7 @@ __device__ __forceinline__ TileCounts lz2d_tile(const float* __restrict__ in, ui
                                                 uint32_t tile, uint32_t gx, bool last, float r, float rad,
                                                 TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
     const uint32_t bx = tile % gx, by = tile / gx;
-    // 32 rows of the tile must span less than 2^32 bytes for 32-bit offsets
-    const bool interior = bx * 32 + 32 <= nx && by * 32 + 32 <= ny && nx < (1u << 25);
+    const bool interior = lz2d_interior(nx, ny, bx, by);
     if (interior) {
         if (codes) return lz2d_tile_v<TCode, ZZ, STRICT, true, true>(in, nx, ny, n, tile, gx, last, r, rad, codes, lo, lp, se);
         return lz2d_tile_v<TCode, ZZ, STRICT, true, false>(in, nx, ny, n, tile, gx, last, r, rad, codes, lo, lp, se);
@@ -372,20 +435,23 @@ __device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, 
     const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
     const bool xz = X < nx && Z < nz;
     const uint64_t plane = (uint64_t)nx * ny;
-    // interior: the tile's scalar base plus 32-bit lane offsets, no tests
+    // interior: row y of the tile from its scalar base plus a 32-bit lane offset
     const uint64_t tbase = (uint64_t)(bz * 8) * plane + (uint64_t)(by * 8) * nx + bx * 8;
     const uint32_t loff = (uint32_t)z * (uint32_t)plane + (uint32_t)x;
     float s[8], xv[8];
+    if constexpr (INTERIOR) {
+        const uint32_t lb = 4u * loff;
 #pragma unroll
-    for (int y = 0; y < 8; y++) {
-        if constexpr (INTERIOR) {
-            xv[y] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(in + tbase) +
-                                                     4u * (loff + (uint32_t)y * nx));
-        } else {
-            xv[y] = (xz && Y0 + y < ny) ? in[Z * plane + (uint64_t)(Y0 + y) * nx + X] : 0.0f;
+        for (int y = 0; y < 8; y++) {
+            const char* rp = reinterpret_cast<const char*>(in + tbase + (uint64_t)y * nx);
+            xv[y] = *reinterpret_cast<const float*>(rp + lb);
         }
-        s[y] = quant_round(xv[y], r);
+    } else {
+#pragma unroll
+        for (int y = 0; y < 8; y++) xv[y] = (xz && Y0 + y < ny) ? in[Z * plane + (uint64_t)(Y0 + y) * nx + X] : 0.0f;
     }
+#pragma unroll
+    for (int y = 0; y < 8; y++) s[y] = quant_round(xv[y], r);
     // a = s - s(z-1); b = a - a(x-1); d = b - b(y-1)   (lorenzo_quant_nd.cu:234-246)
     float b[8];
 #pragma unroll
@@ -396,8 +462,8 @@ __device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, 
         b[y] = (x > 0) ? a - aw : a;
     }
     float d[8];
-    uint32_t mask = 0;
     uint64_t bal[8];
+    const uint32_t cb = (uint32_t)sizeof(TCode) * loff;
 #pragma unroll
     for (int y = 0; y < 8; y++) {
         d[y] = (y > 0) ? b[y] - b[y - 1] : b[y];
@@ -405,18 +471,17 @@ __device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, 
         const bool ok = fabsf(d[y]) < rad;
         if constexpr (CODES) {
             const TCode cv = ok ? enc_code<TCode, ZZ>(d[y]) : (TCode)0;
-            if constexpr (INTERIOR)
-                *reinterpret_cast<TCode*>(reinterpret_cast<char*>(codes + tbase) +
-                                          (uint32_t)sizeof(TCode) * (loff + (uint32_t)y * nx)) = cv;
-            else if (valid)
+            if constexpr (INTERIOR) {
+                char* cp = reinterpret_cast<char*>(codes + tbase + (uint64_t)y * nx);
+                *reinterpret_cast<TCode*>(cp + cb) = cv;
+            } else if (valid) {
                 codes[Z * plane + (uint64_t)(Y0 + y) * nx + X] = cv;
+            }
         }
-        const bool is_out = valid && !ok;
-        mask |= (uint32_t)is_out << y;
-        bal[y] = __ballot(is_out);
+        bal[y] = __ballot(valid && !ok);
     }
-    // Order inside the tile: (z, y, x).
-    auto emit = [&](const uint64_t (&bb)[8], uint32_t m, bool patch, const ListOut& o) {
+    // Order inside the tile: (z, y, x); a lane's own flags are its ballot bits.
+    auto emit = [&](const uint64_t (&bb)[8], bool patch, const ListOut& o) {
         uint32_t tot = 0;
 #pragma unroll
         for (int y = 0; y < 8; y++) tot += __popcll(bb[y]);
@@ -429,7 +494,7 @@ __device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, 
 #pragma unroll
             for (int y = 0; y < 8; y++) {
                 const uint32_t row = (uint32_t)(bb[y] >> (8 * z)) & 0xFFu;
-                if (m & (1u << y)) {
+                if ((row >> x) & 1u) {
                     const uint32_t pos = run + __popc(row & below);
                     if (pos < o.cap) {
                         o.val[pos] = patch ? xv[y] : d[y];
@@ -441,7 +506,7 @@ __device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, 
         }
         return tot;
     };
-    TileCounts tc{emit(bal, mask, false, lo), 0u};
+    TileCounts tc{emit(bal, false, lo), 0u};
     if (CODES && last) zero_code_tail(codes, (uint64_t)nx * ny * nz, n);
     if constexpr (STRICT) {
         bool huge = false;
@@ -449,16 +514,26 @@ __device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, 
         for (int y = 0; y < 8; y++) huge |= !(fabsf(s[y]) < kExactS);
         const bool tile_huge = __ballot(huge) != 0ull;
         const StrictBand band = strict_band(se);
-        uint32_t pm = 0;
         uint64_t pbal[8];
+        uint64_t unsure = 0;
 #pragma unroll
         for (int y = 0; y < 8; y++) {
             const bool valid = INTERIOR || (xz && (Y0 + y) < ny);
-            const bool p = valid && (tile_huge || needs_patch_fast(xv[y], s[y], se, band));
-            pm |= (uint32_t)p << y;
-            pbal[y] = __ballot(p);
+            bool sure, uns;
+            patch_fast(xv[y], s[y], se, band, sure, uns);
+            pbal[y] = __ballot(valid && (tile_huge || sure));
+            unsure |= __ballot(valid && !tile_huge && uns);
         }
-        tc.pcnt = emit(pbal, pm, true, lp);
+        if (unsure) {  // rare: elements within 2^-22 of the bound take the exact test
+#pragma unroll
+            for (int y = 0; y < 8; y++) {
+                const bool valid = INTERIOR || (xz && (Y0 + y) < ny);
+                bool sure, uns;
+                patch_fast(xv[y], s[y], se, band, sure, uns);
+                pbal[y] = __ballot(valid && (tile_huge || sure || (uns && needs_patch(xv[y], s[y], se.eb, se.ebx2))));
+            }
+        }
+        tc.pcnt = emit(pbal, true, lp);
     }
     return tc;
 }

@@ -54,256 +54,337 @@ __device__ __forceinline__ void unshuffled_store(const uint32_t (&d)[16], uint8_
 }
 
 // ---------------------------------------------------------------------------
-// The encoder: one 256-thread workgroup (four waves) per chunk; thread t,
-// step p holds the chunk's dword at byte 1024 p + 4 t, so every
-// load and store of a step covers 1 KiB contiguously and a thread keeps 16
-// dwords instead of 64.  For bit-shuffled blocks thread t transposes the
-// element group(s) whose plane words land at exactly those positions.
-// Level 2 gives thread t bm1 bytes 8t..8t+7 (so bm2 byte t is its 8-bit
-// repeat mask), level 3 one bm2 byte per thread (bm3 bytes from per-wave
-// ballots), level 4 runs in wave 0.
+// Byte-mask helpers (v_dot4_u32_u8 gathers one flag bit per byte)
+// ---------------------------------------------------------------------------
+// 0x80 in every nonzero byte of v, 0 in every zero byte.
+__device__ __forceinline__ uint32_t nz_bytes(uint32_t v) {
+    return (((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;
+}
+// Bit i set <=> byte i of the 8-byte little-endian pair (lo, hi) is nonzero.
+__device__ __forceinline__ uint32_t nz_mask8(uint32_t lo, uint32_t hi) {
+    return __builtin_amdgcn_udot4(nz_bytes(hi), 0x80402010u,
+                                  __builtin_amdgcn_udot4(nz_bytes(lo), 0x08040201u, 0u, false), false) >> 7;
+}
+// Repeat-elimination flags of 8 consecutive bitmap bytes (lo, hi): bit k set
+// <=> byte k differs from byte k-1, with byte -1 = prevb.
+__device__ __forceinline__ uint32_t repeat_flags8(uint32_t lo, uint32_t hi, uint32_t prevb) {
+    const uint32_t ylo = (lo << 8) | prevb;
+    const uint32_t yhi = __builtin_amdgcn_alignbyte(hi, lo, 3);
+    return nz_mask8(lo ^ ylo, hi ^ yhi);
+}
+// Low min(max(k, 0), 8) bits set.
+__device__ __forceinline__ uint32_t low_bits8(int k) { return (1u << (k < 0 ? 0 : k > 8 ? 8 : k)) - 1u; }
+
+// v_perm selectors that pack the bytes of an 8-byte pair (lo = bytes 0-3,
+// hi = 4-7, v_perm(hi, lo, sel)) named by the set bits of a mask byte to the
+// low end in order: [m][0] gives packed bytes 0-3, [m][1] bytes 4-7; 0x0C
+// selects a zero byte.  kExpand8 is the inverse: output byte k takes packed
+// byte (number of set bits below k) when bit k is set, else zero.
+struct alignas(16) Sel8Table {
+    uint32_t v[256][2];
+};
+constexpr Sel8Table make_pack8() {
+    Sel8Table T{};
+    for (int m = 0; m < 256; m++) {
+        uint32_t s[2] = {0x0C0C0C0Cu, 0x0C0C0C0Cu};
+        int c = 0;
+        for (int i = 0; i < 8; i++)
+            if (m & (1 << i)) {
+                s[c >> 2] = (s[c >> 2] & ~(0xFFu << (8 * (c & 3)))) | ((uint32_t)i << (8 * (c & 3)));
+                c++;
+            }
+        T.v[m][0] = s[0];
+        T.v[m][1] = s[1];
+    }
+    return T;
+}
+constexpr Sel8Table make_expand8() {
+    Sel8Table T{};
+    for (int m = 0; m < 256; m++) {
+        uint32_t s[2] = {0x0C0C0C0Cu, 0x0C0C0C0Cu};
+        int c = 0;
+        for (int k = 0; k < 8; k++)
+            if (m & (1 << k)) {
+                s[k >> 2] = (s[k >> 2] & ~(0xFFu << (8 * (k & 3)))) | ((uint32_t)c << (8 * (k & 3)));
+                c++;
+            }
+        T.v[m][0] = s[0];
+        T.v[m][1] = s[1];
+    }
+    return T;
+}
+// Repeat expansion: output byte k takes packed byte r - 1, r = set bits of
+// the mask at positions <= k, or zero when r = 0 (the caller fills those
+// leading bytes with the byte before the run).
+constexpr Sel8Table make_repeat8() {
+    Sel8Table T{};
+    for (int m = 0; m < 256; m++) {
+        uint32_t s[2] = {0x0C0C0C0Cu, 0x0C0C0C0Cu};
+        int r = 0;
+        for (int k = 0; k < 8; k++) {
+            if (m & (1 << k)) r++;
+            if (r > 0) s[k >> 2] = (s[k >> 2] & ~(0xFFu << (8 * (k & 3)))) | ((uint32_t)(r - 1) << (8 * (k & 3)));
+        }
+        T.v[m][0] = s[0];
+        T.v[m][1] = s[1];
+    }
+    return T;
+}
+__constant__ Sel8Table kPack8 = make_pack8();
+__constant__ Sel8Table kExpand8 = make_expand8();
+__constant__ Sel8Table kRepeat8 = make_repeat8();
+
+// ORs the 8 bytes (lo, hi) into the byte stream `st` at byte position pos
+// (bytes past the packed count are zero, so neighbours' bytes that share a
+// dword are untouched).
+__device__ __forceinline__ void or_bytes8(uint32_t* st, uint32_t pos, uint32_t lo, uint32_t hi) {
+    const uint32_t sh = 8u * (pos & 3u);
+    const uint64_t r = ((uint64_t)hi << 32 | lo) << sh;
+    const uint32_t r2 = (uint32_t)(((uint64_t)hi << sh) >> 32);
+    uint32_t* d = st + (pos >> 2);
+    atomicOr(d, (uint32_t)r);
+    if ((uint32_t)(r >> 32)) atomicOr(d + 1, (uint32_t)(r >> 32));
+    if (r2) atomicOr(d + 2, r2);
+}
+
+// Word w of a chunk image staged in LDS sits at img_swz(w): the 16-byte
+// groups of every 64-byte row are rotated by the row's bits 4-5, so a wave
+// reading one 16-byte group per lane at a 64-byte stride spreads over the
+// banks.  A bijection that keeps 16-byte groups and 64-word blocks intact.
+__device__ __forceinline__ uint32_t img_swz(uint32_t w) { return w ^ (((w >> 4) & 3u) << 2); }
+
+// ---------------------------------------------------------------------------
+// The encoder: one 256-thread workgroup (four waves) per chunk.  Thread t
+// owns the chunk image's bytes [64 t, 64 t + 64) (16 dwords), so its
+// level-1 bitmap is bm1 bytes 8t..8t+7 (one 64-bit word in registers), its
+// level-2 repeat mask is bm2 byte t, and one block scan places its nonzero
+// bytes.  Bit-shuffled blocks are transposed per element group (thread t:
+// group t, or groups t and 256 + t for 1-byte elements) and exchanged through
+// LDS into that layout first.  Four barriers: the exchange; the data-byte
+// totals; the level-2/3 totals and ballots; the assembled stream.  The
+// compressed chunk is assembled in LDS with byte-exact ORs and copied to
+// `slot` (a raw bit-shuffled block is written from registers; other raw
+// chunks are left to the caller, which copies the input).  With slot ==
+// nullptr only the size is computed.  Returns the size (bit 31 = raw).  One
+// call per workgroup launch: LDS is still in use by other waves on return.
 // ---------------------------------------------------------------------------
 struct RzeWgLds {
-    uint8_t st[LDS_STREAM];   // assembled stream (dword aligned: offset 0)
-    uint32_t sel[16];         // v_perm selectors packing the nonzero bytes of a dword, by nibble
-    uint8_t bm1[LDS_BM1];     // level-1 bitmap
-    uint32_t wsum[4][16];     // per-wave partial sums (scratch for block scans)
-    uint32_t tot[16];         // broadcast totals
-    uint8_t bm2[256];
-    uint64_t bal[4];          // level-3 ballots (bm3 bytes 8w..8w+7)
+    uint32_t st[LDS_STREAM / 4];  // the assembled stream; first the shuffled image (img_swz order)
+    alignas(8) uint32_t sel[256][2];  // kPack8
+    uint32_t wsum[4][2];          // per-wave totals: data bytes, non-repeated bm1 bytes
+    uint32_t edge[4][3];          // per-wave lane 63: bm1 words, the bm1 byte before them
+    uint64_t bal[4];              // level-3 flags: bm3 bytes 8w..8w+7
 };
-
-// Block-wide exclusive scan of one u32 per thread (256 threads); returns
-// the thread's exclusive prefix, *total gets the sum.  Uses L.wsum[..][slot].
-__device__ __forceinline__ uint32_t wg_excl_scan(RzeWgLds& L, uint32_t v, int slot, uint32_t& total) {
-    const int w = threadIdx.x >> 6;
-    const uint32_t incl = wave_incl_scan(v);
-    if (lane_id() == 63) L.wsum[w][slot] = incl;
-    __syncthreads();
-    uint32_t base = 0;
-    for (int j = 0; j < w; j++) base += L.wsum[j][slot];
-    total = L.wsum[0][slot] + L.wsum[1][slot] + L.wsum[2][slot] + L.wsum[3][slot];
-    return base + incl - v;
-}
 
 template <int SEW>
 __device__ __forceinline__ uint32_t rze_encode_chunk_wg(const uint8_t* __restrict__ src, int in_size, bool shuffled,
                                                         RzeWgLds& L, uint8_t* __restrict__ slot) {
     const int t = threadIdx.x;
-    const int w = t >> 6;
+    const int w = (int)wave_in_wg();
     const int l = lane_id();
-    const bool full = in_size == CS;
-    if (t < 16) {  // selector for nibble t: byte slot c <- c-th nonzero byte, zeros after
-        uint32_t sel = 0x0C0C0C0Cu;
-        int c = 0;
-        for (int k = 0; k < 4; k++)
-            if (t & (1 << k)) {
-                sel = (sel & ~(0xFFu << (8 * c))) | ((uint32_t)k << (8 * c));
-                c++;
-            }
-        L.sel[t] = sel;
+    if (slot) {
+        const uint2 s = *reinterpret_cast<const uint2*>(kPack8.v[t]);
+        L.sel[t][0] = s.x;
+        L.sel[t][1] = s.y;
     }
-    // -- load: dword d[p] = chunk bytes [1024 p + 4 t, +4)
-    uint32_t d[16];
-    if (SEW == 2 && shuffled) {
-        const uint8_t* g = src + 64 * t;  // element group t (32 x u16)
-        uint32_t wd[16];
+    // -- the chunk image, thread t: D[j] = bytes [64 t + 4 j, +4)
+    uint32_t D[16];
+    if (SEW != 0 && shuffled) {
+        uint32_t d[16];  // d[p] = image word 256 p + t
+        if constexpr (SEW == 2) {
+            const uint8_t* g = src + 64 * t;  // element group t (32 x u16)
+            uint32_t wd[16];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint4 x = ld16(g + 16 * q);
+                wd[4 * q] = x.x; wd[4 * q + 1] = x.y; wd[4 * q + 2] = x.z; wd[4 * q + 3] = x.w;
+            }
+            uint32_t A[16];
+            planes16(wd, A);
+#pragma unroll
+            for (int p = 0; p < 16; p++) d[p] = A[15 - p];
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint8_t* g = src + 32 * (256 * h + t);  // element group 256 h + t (32 x u8)
+                const uint4 x0 = ld16(g), x1 = ld16(g + 16);
+                const uint32_t wd[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+                uint32_t A[8];
+                planes8(wd, A);
+#pragma unroll
+                for (int P = 0; P < 8; P++) d[2 * P + h] = A[7 - P];
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < 16; p++) L.st[img_swz(256 * p + t)] = d[p];
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const uint4 x = ld16(g + 16 * q);
-            wd[4 * q] = x.x; wd[4 * q + 1] = x.y; wd[4 * q + 2] = x.z; wd[4 * q + 3] = x.w;
+            const uint4 v = *reinterpret_cast<const uint4*>(&L.st[img_swz(16 * t + 4 * q)]);
+            D[4 * q] = v.x; D[4 * q + 1] = v.y; D[4 * q + 2] = v.z; D[4 * q + 3] = v.w;
         }
-        uint32_t A[16];
-        planes16(wd, A);
+    } else if (in_size == CS) {
 #pragma unroll
-        for (int p = 0; p < 16; p++) d[p] = A[15 - p];
-    } else if (SEW == 1 && shuffled) {
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const uint8_t* g = src + 32 * (256 * h + t);  // element group 256 h + t (32 x u8)
-            const uint4 x0 = ld16(g), x1 = ld16(g + 16);
-            const uint32_t wd[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-            uint32_t A[8];
-            planes8(wd, A);
-#pragma unroll
-            for (int P = 0; P < 8; P++) d[2 * P + h] = A[7 - P];
+        for (int q = 0; q < 4; q++) {
+            const uint4 v = ld16(src + 64 * t + 16 * q);
+            D[4 * q] = v.x; D[4 * q + 1] = v.y; D[4 * q + 2] = v.z; D[4 * q + 3] = v.w;
         }
     } else {
 #pragma unroll
-        for (int p = 0; p < 16; p++) {
-            const int b = 1024 * p + 4 * t;
-            if (full || b + 4 <= in_size) {
-                d[p] = *reinterpret_cast<const uint32_t*>(src + b);
+        for (int j = 0; j < 16; j++) {
+            const int b = 64 * t + 4 * j;
+            if (b + 4 <= in_size) {
+                D[j] = *reinterpret_cast<const uint32_t*>(src + b);
             } else {
                 uint32_t x = 0;
                 for (int k = 0; k < 4; k++)
                     if (b + k < in_size) x |= (uint32_t)src[b + k] << (8 * k);
-                d[p] = x;
+                D[j] = x;
             }
         }
     }
-    // -- level 1: nibble of nonzero bytes per step; bm1 byte 128 p + t/2
-    uint64_t nib = 0;  // 4 bits per step
-    uint32_t my_nz = 0;
+    // -- level 1: bm1 bytes 8t..8t+7 and the data byte count
+    uint32_t Xlo = 0, Xhi = 0;
 #pragma unroll
-    for (int p = 0; p < 16; p++) {
-        uint32_t m = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) m |= (((d[p] >> (8 * k)) & 0xFFu) != 0u ? 1u : 0u) << k;
-        nib |= (uint64_t)m << (4 * p);
-        my_nz += __popc(m);
-        const uint32_t other = lane_xor1(m);
-        if ((t & 1) == 0) L.bm1[128 * p + (t >> 1)] = (uint8_t)(m | (other << 4));
+    for (int b = 0; b < 4; b++) {
+        Xlo |= nz_mask8(D[2 * b], D[2 * b + 1]) << (8 * b);
+        Xhi |= nz_mask8(D[8 + 2 * b], D[9 + 2 * b]) << (8 * b);
     }
-    uint32_t data_nz;
-    (void)wg_excl_scan(L, my_nz, 0, data_nz);  // barrier: bm1 complete
-    // zero the data part of the stream: the data bytes are ORed in below
-    for (uint32_t i = t; i < (data_nz + 3) / 4; i += 256) reinterpret_cast<uint32_t*>(L.st)[i] = 0u;
-
-    const int avail = CS - 2;
+    const uint32_t cnt = __popc(Xlo) + __popc(Xhi);
+    const uint32_t cnt_incl = wave_incl_scan(cnt);
+    const uint32_t prevb_lane = wave_shr1(Xhi >> 24);  // bm1 byte 8t - 1 (lane 0: not yet known)
+    if (l == 63) {
+        L.wsum[w][0] = cnt_incl;
+        L.edge[w][0] = Xlo;
+        L.edge[w][1] = Xhi;
+        L.edge[w][2] = prevb_lane;
+    }
+    __syncthreads();
+    const uint32_t s0 = L.wsum[0][0], s1 = L.wsum[1][0], s2 = L.wsum[2][0], s3 = L.wsum[3][0];
+    const uint32_t data_nz = s0 + s1 + s2 + s3;
+    const uint32_t data_pos = (w > 0 ? s0 : 0u) + (w > 1 ? s1 : 0u) + (w > 2 ? s2 : 0u) + cnt_incl - cnt;
     const int bm1_bytes = (in_size + 7) / 8;
     const int bm2_bytes = (bm1_bytes + 7) / 8;
     const int bm3_bytes = (bm2_bytes + 7) / 8;
-    uint32_t size = 0;
+    // -- level 2: bm2 byte t = repeat flags of bm1 bytes 8t..8t+7 (byte -1 := 0)
+    uint32_t e_lo = 0, e_hi = 0, e_prev = 0;  // the previous wave's lane 63
+    if (w > 0) {
+        e_lo = L.edge[w - 1][0];
+        e_hi = L.edge[w - 1][1];
+        e_prev = L.edge[w - 1][2];
+    }
+    const uint32_t prevb = l == 0 ? (e_hi >> 24) : prevb_lane;
+    const uint32_t rm = repeat_flags8(Xlo, Xhi, prevb) & low_bits8(bm1_bytes - 8 * t);
+    // -- level 3: bm3 bit t = bm2 byte t differs from bm2 byte t-1 (byte -1 := 0)
+    const uint32_t rm_edge = w > 0 ? (repeat_flags8(e_lo, e_hi, e_prev) & low_bits8(bm1_bytes - 8 * (64 * w - 1))) : 0u;
+    const uint32_t rm_shifted = wave_shr1(rm);  // every lane takes part in the lane shift
+    const uint32_t rm_prev = l == 0 ? rm_edge : rm_shifted;
+    const bool nr2 = t < bm2_bytes && rm != rm_prev;
+    const uint64_t bal = __ballot(nr2);
+    const uint32_t rm_cnt = __popc(rm);
+    const uint32_t rm_incl = wave_incl_scan(rm_cnt);
+    if (l == 63) L.wsum[w][1] = rm_incl;
+    if (l == 0) L.bal[w] = bal;
+    if (slot) {  // clear the stream for the ORs (the staged image is in registers by now)
+        uint4* z = reinterpret_cast<uint4*>(L.st);
+#pragma unroll
+        for (int q = 0; q < 4; q++) z[4 * t + q] = make_uint4(0, 0, 0, 0);
+        if (t < (LDS_STREAM - CS) / 16) z[4 * 256 + t] = make_uint4(0, 0, 0, 0);
+    }
+    __syncthreads();
+    const uint32_t r0 = L.wsum[0][1], r1 = L.wsum[1][1], r2 = L.wsum[2][1], r3 = L.wsum[3][1];
+    const uint32_t bm2_nr = r0 + r1 + r2 + r3;
+    const uint64_t b0 = L.bal[0], b1 = L.bal[1], b2 = L.bal[2], b3 = L.bal[3];
+    const uint32_t bm3_nr = __popcll(b0) + __popcll(b1) + __popcll(b2) + __popcll(b3);
+    // -- level 4: bm4 bit j = bm3 byte j differs from bm3 byte j-1 (byte -1 := 0)
+    uint32_t bm4;
+    {
+        const uint64_t d0 = b0 ^ (b0 << 8), d1 = b1 ^ (b1 << 8 | b0 >> 56), d2 = b2 ^ (b2 << 8 | b1 >> 56),
+                       d3 = b3 ^ (b3 << 8 | b2 >> 56);
+        bm4 = nz_mask8((uint32_t)d0, (uint32_t)(d0 >> 32)) | nz_mask8((uint32_t)d1, (uint32_t)(d1 >> 32)) << 8 |
+              nz_mask8((uint32_t)d2, (uint32_t)(d2 >> 32)) << 16 | nz_mask8((uint32_t)d3, (uint32_t)(d3 >> 32)) << 24;
+        if (bm3_bytes < 32) bm4 &= (1u << bm3_bytes) - 1u;
+    }
+    const uint32_t bm4_nr = __popc(bm4);
+    // -- size and the raw fallback, with the reference's thresholds
+    const int avail = CS - 2;
+    uint32_t size;
     bool raw = false;
-    uint32_t bm2_nr = 0, bm3_nr = 0, bm4_nr = 0;
-    uint32_t rmask1 = 0;  // bm1 bytes 8t..8t+7 non-repeated (= bm2 byte t)
-    uint32_t nr2 = 0;     // bm2 byte t non-repeated
-    uint32_t rmask3 = 0;  // bm4 (wave 0 computes it; broadcast through LDS)
     if (data_nz == 0) {
         size = 2;
     } else {
         int wpos = (int)data_nz;
         if (wpos >= avail || wpos > avail - bm1_bytes) raw = true;
         if (!raw) {
-            // -- level 2: RE over bm1; thread t owns bytes 8t..8t+7
-            const uint32_t lo = *reinterpret_cast<const uint32_t*>(L.bm1 + 8 * t);
-            const uint32_t hi = *reinterpret_cast<const uint32_t*>(L.bm1 + 8 * t + 4);
-            uint32_t prev = t > 0 ? L.bm1[8 * t - 1] : 0u;
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const uint32_t cur = ((k < 4 ? lo : hi) >> (8 * (k & 3))) & 0xFFu;
-                if (cur != prev && 8 * t + k < bm1_bytes) rmask1 |= 1u << k;
-                prev = cur;
-            }
-            L.bm2[t] = (uint8_t)rmask1;
-            (void)wg_excl_scan(L, __popc(rmask1), 1, bm2_nr);  // barrier: bm2 complete
             wpos += (int)bm2_nr;
-            if (wpos >= avail) raw = true;
-            if (!raw && (wpos >= avail || wpos > avail - bm2_bytes)) raw = true;
+            if (wpos >= avail || wpos > avail - bm2_bytes) raw = true;
         }
         if (!raw) {
-            // -- level 3: RE over bm2; thread t owns bm2 byte t
-            const uint32_t prev = t > 0 ? L.bm2[t - 1] : 0u;
-            nr2 = (t < bm2_bytes && rmask1 != prev) ? 1u : 0u;
-            const uint64_t b = __ballot(nr2 != 0u);
-            if (l == 0) L.bal[w] = b;
-            (void)wg_excl_scan(L, nr2, 2, bm3_nr);  // barrier: ballots visible
             wpos += (int)bm3_nr;
-            if (wpos >= avail) raw = true;
-            if (!raw && (wpos >= avail - 4 || wpos > (avail - 4) - bm3_bytes)) raw = true;
+            if (wpos >= avail || wpos >= avail - 4 || wpos > (avail - 4) - bm3_bytes) raw = true;
         }
         if (!raw) {
-            // -- level 4: RE over bm3 (32 bytes: ballot bytes), in wave 0
-            if (w == 0) {
-                const uint32_t cur = l < 32 ? (uint32_t)((L.bal[l >> 3] >> (8 * (l & 7))) & 0xFFu) : 0u;
-                const uint32_t prev = wave_shr1(cur);  // lane 0: 0
-                const uint32_t m3 = (uint32_t)__ballot(l < bm3_bytes && cur != prev);
-                if (l == 0) L.tot[0] = m3;
-            }
-            __syncthreads();
-            rmask3 = L.tot[0];
-            bm4_nr = __popc(rmask3);
             wpos += (int)bm4_nr;
             if (wpos >= avail - 4) raw = true;
         }
         size = raw ? ((1u << 31) | (uint32_t)in_size) : (uint32_t)(wpos + 4 + 2);
     }
-    const uint32_t clean = size & 0x7FFFFFFFu;
-    const bool from_scratch = !raw || (SEW != 0 && shuffled);
+    if (!slot) return size;
     if (raw) {
         // stored bytes = the (shuffled) chunk image, straight from registers
-        if (slot && from_scratch) {
+        if (SEW != 0 && shuffled) {
 #pragma unroll
-            for (int p = 0; p < 16; p++) reinterpret_cast<uint32_t*>(slot)[256 * p + t] = d[p];
+            for (int q = 0; q < 4; q++)
+                st16(slot + 64 * t + 16 * q, make_uint4(D[4 * q], D[4 * q + 1], D[4 * q + 2], D[4 * q + 3]));
         }
-        __syncthreads();
         return size;
     }
+    uint8_t* stb = reinterpret_cast<uint8_t*>(L.st);
     if (size == 2) {
-        if (t == 0) { L.st[0] = (uint8_t)(in_size & 0xFF); L.st[1] = (uint8_t)((in_size >> 8) & 0xFF); }
+        if (t == 0) { stb[0] = (uint8_t)(in_size & 0xFF); stb[1] = (uint8_t)((in_size >> 8) & 0xFF); }
     } else {
-        // nonzero bytes in chunk order: step-major, then thread, then byte
-        uint32_t incl[16];
+        // nonzero data bytes in chunk order
+        uint32_t pos = data_pos;
 #pragma unroll
-        for (int p = 0; p < 16; p++) {
-            incl[p] = wave_incl_scan((uint32_t)__popcll((nib >> (4 * p)) & 0xFull));
-            if (l == 63) L.wsum[w][p] = incl[p];
-        }
-        __syncthreads();
-        // lane q < 16 of each wave: where this wave's bytes of step q start =
-        // bytes of all steps before q + bytes of step q in earlier waves
-        uint32_t qb = 0;
-        {
-            uint32_t before = 0, all = 0;
-            if (l < 16) {
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const uint32_t v = L.wsum[j][l];
-                    before += j < w ? v : 0u;
-                    all += v;
-                }
-            }
-            qb = wave_incl_scan(all) - all + before;  // lanes 0..15 form one DPP row
-        }
-#pragma unroll
-        for (int p = 0; p < 16; p++) {
-            const uint32_t m = (uint32_t)((nib >> (4 * p)) & 0xFull);
-            const uint32_t base = (uint32_t)__builtin_amdgcn_readlane((int)qb, p);
-            const uint32_t pos = base + incl[p] - __popc(m);
+        for (int b = 0; b < 8; b++) {
+            const uint32_t m = ((b < 4 ? Xlo : Xhi) >> (8 * (b & 3))) & 0xFFu;
             if (m) {
-                // the dword's nonzero bytes packed to its low end, ORed into
-                // the zeroed stream at [pos, pos + popc(m)) with two aligned
-                // dword ORs (neighbouring threads' bytes share the boundary
-                // dwords, never a byte)
-                const uint32_t packed = __builtin_amdgcn_perm(0u, d[p], L.sel[m]);
-                const uint64_t sh = (uint64_t)packed << (8 * (pos & 3));
-                uint32_t* sw = reinterpret_cast<uint32_t*>(L.st) + (pos >> 2);
-                atomicOr(sw, (uint32_t)sh);
-                if ((uint32_t)(sh >> 32)) atomicOr(sw + 1, (uint32_t)(sh >> 32));
+                const uint2 sel = *reinterpret_cast<const uint2*>(L.sel[m]);
+                or_bytes8(L.st, pos, __builtin_amdgcn_perm(D[2 * b + 1], D[2 * b], sel.x),
+                          __builtin_amdgcn_perm(D[2 * b + 1], D[2 * b], sel.y));
             }
+            pos += __popc(m);
         }
-        __syncthreads();  // wsum reused below
-        // bm1 non-repeated bytes (thread t: bm1 bytes 8t..8t+7)
-        uint32_t dummy;
-        uint32_t pos1 = data_nz + wg_excl_scan(L, __popc(rmask1), 3, dummy);
-#pragma unroll
-        for (int k = 0; k < 8; k++)
-            if (rmask1 & (1u << k)) L.st[pos1++] = L.bm1[8 * t + k];
-        // bm2 non-repeated bytes (one per thread at most)
-        const uint32_t pos2 = data_nz + bm2_nr + wg_excl_scan(L, nr2, 4, dummy);
-        if (nr2) L.st[pos2] = (uint8_t)rmask1;
-        // bm3 non-repeated bytes, bm4, tag
+        // non-repeated bm1 bytes (this thread's: the set bits of rm)
+        if (rm) {
+            const uint32_t pos1 = data_nz + (w > 0 ? r0 : 0u) + (w > 1 ? r1 : 0u) + (w > 2 ? r2 : 0u) + rm_incl - rm_cnt;
+            const uint2 sel = *reinterpret_cast<const uint2*>(L.sel[rm]);
+            or_bytes8(L.st, pos1, __builtin_amdgcn_perm(Xhi, Xlo, sel.x), __builtin_amdgcn_perm(Xhi, Xlo, sel.y));
+        }
+        // non-repeated bm2 bytes (this thread's byte rm when nr2)
+        if (nr2) {
+            const uint32_t before = (w > 0 ? (uint32_t)__popcll(b0) : 0u) + (w > 1 ? (uint32_t)__popcll(b1) : 0u) +
+                                    (w > 2 ? (uint32_t)__popcll(b2) : 0u) + lanes_below(bal);
+            stb[data_nz + bm2_nr + before] = (uint8_t)rm;
+        }
+        // non-repeated bm3 bytes, bm4, the size tag
         const uint32_t base3 = data_nz + bm2_nr + bm3_nr;
-        if (w == 0 && l < 32 && (rmask3 & (1u << l))) {
-            const uint32_t b3 = (uint32_t)((L.bal[l >> 3] >> (8 * (l & 7))) & 0xFFu);
-            L.st[base3 + __popc(rmask3 & ((1u << l) - 1u))] = (uint8_t)b3;
+        if (t < 32 && (bm4 & (1u << t))) {
+            const uint64_t bw = t < 8 ? b0 : t < 16 ? b1 : t < 24 ? b2 : b3;
+            stb[base3 + __popc(bm4 & ((1u << t) - 1u))] = (uint8_t)((bw >> (8 * (t & 7))) & 0xFFu);
         }
         const uint32_t base4 = base3 + bm4_nr;
-        if (t < 4) L.st[base4 + t] = (uint8_t)((rmask3 >> (8 * t)) & 0xFFu);
+        if (t < 4) stb[base4 + t] = (uint8_t)((bm4 >> (8 * t)) & 0xFFu);
         if (t == 0) {
-            L.st[base4 + 4] = (uint8_t)(in_size & 0xFF);
-            L.st[base4 + 5] = (uint8_t)((in_size >> 8) & 0xFF);
+            stb[base4 + 4] = (uint8_t)(in_size & 0xFF);
+            stb[base4 + 5] = (uint8_t)((in_size >> 8) & 0xFF);
         }
     }
     __syncthreads();
-    if (slot && clean > 0) {
-        const uint32_t nd = (clean + 3) / 4;
-        const uint32_t* sw = reinterpret_cast<const uint32_t*>(L.st);
-        uint32_t* dw = reinterpret_cast<uint32_t*>(slot);
-        for (uint32_t i = t; i < nd; i += 256) dw[i] = sw[i];
-    }
-    __syncthreads();  // LDS free for the workgroup's next chunk
+    const uint32_t nq = (size + 15) / 16;  // size < CS here
+    const uint4* sq = reinterpret_cast<const uint4*>(L.st);
+    uint4* dq = reinterpret_cast<uint4*>(slot);
+    for (uint32_t i = t; i < nq; i += 256) dq[i] = sq[i];
     return size;
 }
 

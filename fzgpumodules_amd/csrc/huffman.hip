@@ -21,9 +21,12 @@
 //     the cell counts writes par_entry into the stream; a pack kernel moves
 //     the cells and a one-workgroup kernel writes the header: no host prefix
 //     sum, no host sync (reference phases 1-4 + two syncs).
-//   decode: one lane per partition with a 1024-entry prefix lookup table in
-//     LDS (codes up to 10 bits resolve in one probe), canonical bit loop only
-//     for longer codes; outputs leave in 16-byte stores.
+//   decode: one wave per partition: 64 lanes decode equal spans of its bits
+//     speculatively and settle the span starts in rounds (a prefix code
+//     resynchronises within a few codewords), a 4096-entry prefix table
+//     yields the symbol of any code of up to 12 bits in one LDS probe and
+//     longer codes' lengths come from per-length limit compares; symbols are
+//     staged in LDS and leave in 16-byte stores.
 #include "fzh_scan.h"
 
 namespace fzh {
@@ -32,7 +35,6 @@ namespace {
 constexpr int SUBLEN = 768;       // coarse partitions
 constexpr int SUBLEN_FINE = 1024; // fine partitions
 constexpr int HDR_BYTES = 128;
-constexpr int LUT_BITS = 10;
 
 template <typename T>
 __device__ __forceinline__ uint32_t ld_sym(const T* p, uint64_t i) { return (uint32_t)p[i]; }
@@ -162,36 +164,52 @@ huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint3
     }
 }
 
-// Cells of partition p -> stream cell par_entry[p] (one wave per partition).
+// Cells of partition p -> stream cell par_entry[p] (one wave per partition);
+// each workgroup adds its partitions' bit counts to one of NBIT_SLOTS
+// accumulators (one atomic per workgroup) for the header.
+constexpr uint32_t NBIT_SLOTS = 256;
+
 __global__ void __launch_bounds__(256)
 huf_pack_kernel(const uint32_t* __restrict__ scells, const uint32_t* __restrict__ ncell,
-                uint8_t* __restrict__ out, uint32_t ent3, uint32_t ent4, uint32_t pardeg, uint32_t cells) {
-    const uint32_t p = blockIdx.x * 4 + wave_in_wg();
-    if (p >= pardeg) return;
+                uint8_t* __restrict__ out, uint32_t ent2, uint32_t ent3, uint32_t ent4, uint32_t pardeg,
+                uint32_t cells, unsigned long long* __restrict__ nbit_acc) {
+    __shared__ uint32_t s_nb[4];
+    const int w = (int)wave_in_wg();
+    const uint32_t p = blockIdx.x * 4 + w;
     const int l = lane_id();
-    const uint32_t e = reinterpret_cast<const uint32_t*>(out + ent3)[p];
-    const uint32_t c = ncell[p];
-    const uint32_t* src = scells + (uint64_t)p * cells;
-    uint32_t* dst = reinterpret_cast<uint32_t*>(out + ent4) + e;
-    for (uint32_t i = l; i < c; i += 64) dst[i] = src[i];
+    uint32_t nb = 0;
+    if (p < pardeg) {
+        const uint32_t e = reinterpret_cast<const uint32_t*>(out + ent3)[p];
+        const uint32_t c = ncell[p];
+        const uint32_t* src = scells + (uint64_t)p * cells;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(out + ent4) + e;
+        for (uint32_t i = l; i < c; i += 64) dst[i] = src[i];
+        nb = reinterpret_cast<const uint32_t*>(out + ent2)[p];
+    }
+    if (l == 0) s_nb[w] = nb;
+    __syncthreads();
+    if (threadIdx.x == 0)  // spread over NBIT_SLOTS words: one address serialises every atomic
+        atomicAdd(nbit_acc + (blockIdx.x % NBIT_SLOTS), (unsigned long long)s_nb[0] + s_nb[1] + s_nb[2] + s_nb[3]);
 }
 
-// phf_header (hf.h:38-44) once every partition is placed.
-__global__ void __launch_bounds__(1024)
+// phf_header (hf.h:38-44) once every partition is placed; sums the bit
+// total from the NBIT_SLOTS accumulators and clears them for the next call.
+__global__ void __launch_bounds__(256)
 huf_header_kernel(uint8_t* __restrict__ out, uint64_t n, uint32_t bklen, uint32_t sublen, uint32_t ent1, uint32_t ent2,
                   uint32_t ent3, uint32_t ent4, uint32_t pardeg, const uint32_t* __restrict__ total_ncell,
-                  uint64_t* __restrict__ out_size) {
-    __shared__ unsigned long long s_w[16];
-    const uint32_t* nb = reinterpret_cast<const uint32_t*>(out + ent2);
-    unsigned long long t = 0;
-    for (uint32_t p = threadIdx.x; p < pardeg; p += 1024) t += nb[p];
+                  uint64_t* __restrict__ out_size, unsigned long long* __restrict__ nbit_acc) {
+    __shared__ unsigned long long s_w[4];
+    unsigned long long a = 0;
+    for (uint32_t i = threadIdx.x; i < NBIT_SLOTS; i += 256) {
+        a += nbit_acc[i];
+        nbit_acc[i] = 0;
+    }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-    if (lane_id() == 0) s_w[threadIdx.x >> 6] = t;
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane_id() == 0) s_w[threadIdx.x >> 6] = a;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long tnb = 0;
-        for (int i = 0; i < 16; i++) tnb += s_w[i];
+        const unsigned long long tnb = s_w[0] + s_w[1] + s_w[2] + s_w[3];
         const uint32_t tc = *total_ncell;
         const uint32_t ent5 = ent4 + 4 * tc;
         uint32_t* h32 = reinterpret_cast<uint32_t*>(out);
@@ -215,14 +233,247 @@ huf_header_kernel(uint8_t* __restrict__ out, uint64_t n, uint32_t bklen, uint32_
 // ---------------------------------------------------------------------------
 // Decode
 // ---------------------------------------------------------------------------
+// One wave per partition.  Codeword boundaries inside a partition are only
+// known by decoding from its first bit, so the wave cuts the partition's
+// bits into 64 equal spans (of at least 32 bits, longer than any codeword)
+// and decodes them speculatively, all lanes at once: lane j starts at the
+// first bit of span j and decodes the codewords that start inside its span,
+// which leaves it at the end of its last codeword -- the first codeword
+// boundary at or past span j+1's first bit, i.e. where lane j+1 really
+// starts.  Lanes whose start moves decode again from the corrected start,
+// and stop as soon as they reach a boundary of their previous decode (the
+// first 64 bits of each span keep a boundary mask): from there on both
+// decodes see the same codewords, so the count follows from the mask and
+// the end is unchanged.  A prefix code falls back into step within a few
+// codewords, so one short round nearly always settles the wave; lane 0 is
+// always right and a lane whose predecessor is right becomes right in the
+// next round, so 64 rounds bound it.  A prefix sum of the per-lane counts
+// gives each lane its output offset; a last decode of each span writes its
+// symbols to LDS and the wave stores the partition with 16-byte stores.
+//
+// Per codeword: a probe of a 2^12-entry table (length and symbol -- the
+// symbol itself for 1- and 2-byte keys -- of every code of up to 12 bits);
+// the 12-bit prefixes of longer codes point into a second table indexed by
+// the next B2 bits (B2 chosen so the table fits; all long codes of typical
+// books); codes longer still take the canonical comparison against the
+// per-length limits first[l] << (32 - l), which do not increase with l
+// (first[l] = round((first[l+1] + numl[l+1]) / 2)), so the length is one
+// more than the number of limits above the window (the bit loop of
+// hf_kernels.cu:258-312 in a few compares).  A window with no code of the
+// book decodes as a 31-bit code with a flag that fails the partition, so
+// the loops never branch on it.  Bits come from a 64-bit register buffer
+// refilled from the partition's cells in LDS.
+//
+// A partition's cells are staged in LDS when they fit (DEC_CELLS cells and
+// partitions of at most DEC_SUBLEN_MAX symbols, which covers the coarse 768
+// and fine 1024 layouts); any other partition is decoded by the same steps
+// reading its cells from memory and writing its symbols there.
+constexpr int DEC_LUT_BITS = 12;
+constexpr uint32_t DEC_LUT2 = 1024;  // second-level entries
+constexpr uint32_t DEC_CELLS = 448;
+constexpr uint32_t DEC_SUBLEN_MAX = 1024;
+constexpr uint32_t DEC_INVALID = 0xFFFFFFFFu;
+// table entry: value (symbol, or key index with DEC_KEY) in bits 0-23,
+// code length in bits 24-28
+constexpr uint32_t DEC_KEY = 1u << 29;
+constexpr uint32_t DEC_BAD = (1u << 30) | (31u << 24);  // no code: skip 31 bits, fail
+constexpr uint32_t DEC_LONG = 1u << 31;                 // first level: prefix of a longer code
+
+// waves per workgroup: the tables are built once per workgroup, so
+// workgroups are large; 4-byte keys halve it to keep the LDS under 64 KiB
 template <typename T>
-__global__ void __launch_bounds__(256)
+constexpr int dec_waves() { return sizeof(T) == 4 ? 4 : 8; }
+
+struct DecTables {
+    const uint32_t* lut;   // DEC_LUT_BITS-bit prefix -> entry, or DEC_LONG | prefix
+    const uint32_t* lut2;  // long prefix, next b2 bits -> entry; 0 = longer code or none
+    const uint64_t* lim;   // first[l] << (32 - l)
+    const int32_t* diff;   // entry[l] - first[l]
+    uint32_t maxl, nkeys, b2;
+};
+
+// Entry of a code longer than DEC_LUT_BITS bits at the head of W, given its
+// first-level entry e.
+__device__ __forceinline__ uint32_t dec_long(const DecTables& t, uint32_t W, uint32_t e) {
+    if (t.b2) {
+        const uint32_t e2 = t.lut2[((e & 0xFFFFu) << t.b2) | ((W << DEC_LUT_BITS) >> (32 - t.b2))];
+        if (e2) return e2;
+    }
+    uint32_t lv = DEC_LUT_BITS + 1;
+    for (uint32_t l = DEC_LUT_BITS + 1; l < t.maxl; l++) lv += (uint64_t)W < t.lim[l] ? 1u : 0u;
+    const uint32_t k = (W >> (32 - lv)) + (uint32_t)t.diff[lv];
+    if (lv > t.maxl || (uint64_t)W < t.lim[lv] || k >= t.nkeys) return DEC_BAD;
+    return (lv << 24) | DEC_KEY | k;
+}
+
+__device__ __forceinline__ uint32_t dec_entry(const DecTables& t, uint32_t W) {
+    uint32_t e = t.lut[W >> (32 - DEC_LUT_BITS)];
+    if ((int32_t)e < 0) e = dec_long(t, W, e);
+    return e;
+}
+
+// MSB-first reader of a partition's cells: staged in LDS (with zero cells
+// past the end) or read from memory with the end checked.  `buf` holds bits
+// [p, 32 next) of the partition at its top; it is refilled whenever fewer
+// than 32 remain, i.e. when p > thr = 32 (next - 1).
+template <bool STAGED>
+struct DecBits {
+    const uint32_t* c;
+    uint32_t ncell;
+    uint64_t buf;
+    uint32_t next, thr;
+    __device__ __forceinline__ uint32_t at(uint32_t k) const { return STAGED ? c[k] : (k < ncell ? c[k] : 0u); }
+    __device__ __forceinline__ void seek(uint32_t p) {
+        const uint32_t k = p >> 5;
+        buf = (((uint64_t)at(k) << 32) | at(k + 1)) << (p & 31u);
+        next = k + 2;
+        thr = 32 * k + 32;
+    }
+    __device__ __forceinline__ uint32_t window() const { return (uint32_t)(buf >> 32); }
+    // advance from p to p + L (L < 32)
+    __device__ __forceinline__ void consume(uint32_t p, uint32_t L) {
+        buf <<= L;
+        p += L;
+        if (p > thr) {
+            buf |= (uint64_t)at(next) << (p - thr);
+            next++;
+            thr += 32;
+        }
+    }
+};
+
+// One partition of nbit bits and cnt symbols, decoded by one wave into
+// `ost` (LDS staging, STAGED) or straight into `dst`.  Returns false if the
+// bits do not decode to exactly cnt symbols ending at bit nbit.
+template <typename T, bool STAGED>
+__device__ __forceinline__ bool dec_partition(const uint32_t* cells, uint32_t ncell, const DecTables& t, uint32_t nbit,
+                                              uint32_t cnt, const T* __restrict__ keys, T* ost, T* __restrict__ dst) {
+    const uint32_t l = (uint32_t)lane_id();
+    // spans of at least 32 bits: longer than any codeword, so every span
+    // holds the start of a codeword and a correction moves on by whole spans
+    const uint32_t seg = max((nbit + 63) / 64, 32u);
+    const uint32_t act = (nbit + seg - 1) / seg;  // lanes with a span (wave-uniform, <= 64)
+    if (act == 0) return cnt == 0;
+    const bool on = l < act;
+    const uint32_t lo = min(l * seg, nbit), lim = min(l * seg + seg, nbit);
+    const uint32_t lim64 = min(lim, lo + 64);
+    DecBits<STAGED> bits{cells, ncell, 0, 0, 0};
+    // speculative decode from lo: count, end, boundaries in [lo, lo + 64)
+    uint32_t b = lo, c = 0, e = nbit;
+    uint64_t m = 0;
+    if (on) {
+        uint32_t p = lo, acc = 0;
+        bits.seek(p);
+        while (p < lim64) {
+            const uint32_t en = dec_entry(t, bits.window());
+            const uint32_t L = (en >> 24) & 31u;
+            acc |= en;
+            m |= 1ull << (p - lo);
+            bits.consume(p, L);
+            p += L;
+            c++;
+        }
+        while (p < lim) {
+            const uint32_t en = dec_entry(t, bits.window());
+            const uint32_t L = (en >> 24) & 31u;
+            acc |= en;
+            bits.consume(p, L);
+            p += L;
+            c++;
+        }
+        e = (acc & (1u << 30)) ? DEC_INVALID : p;
+    }
+    for (int round = 0; round < 64; round++) {
+        const uint32_t prev = wave_shr1(e);  // every lane takes part in the lane shift
+        const uint32_t nb = l == 0 ? 0u : prev;
+        const bool moved = on && nb != b;
+        if (!__ballot(moved)) break;
+        if (moved) {
+            // decode again from nb, up to the first boundary of the previous decode
+            b = nb;
+            uint32_t p = nb, steps = 0, acc = 0;
+            uint64_t mn = 0;
+            bool synced = false;
+            if (nb != DEC_INVALID) {
+                bits.seek(p);
+                while (p < lim64) {
+                    if ((m >> (p - lo)) & 1ull) {
+                        synced = true;
+                        break;
+                    }
+                    const uint32_t en = dec_entry(t, bits.window());
+                    const uint32_t L = (en >> 24) & 31u;
+                    acc |= en;
+                    mn |= 1ull << (p - lo);
+                    bits.consume(p, L);
+                    p += L;
+                    steps++;
+                }
+            }
+            if (nb == DEC_INVALID) {
+                e = DEC_INVALID;
+                c = 0;
+                m = 0;
+            } else if (synced && !(acc & (1u << 30))) {
+                // from p on both decodes see the same codewords: e stays
+                const uint64_t below = (1ull << (p - lo)) - 1ull;
+                c = steps + c - (uint32_t)__popcll(m & below);
+                m = mn | (m & ~below);
+            } else {
+                while (p < lim) {
+                    const uint32_t en = dec_entry(t, bits.window());
+                    const uint32_t L = (en >> 24) & 31u;
+                    acc |= en;
+                    bits.consume(p, L);
+                    p += L;
+                    steps++;
+                }
+                e = (acc & (1u << 30)) ? DEC_INVALID : p;
+                c = steps;
+                m = mn;
+            }
+        }
+    }
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t last_e = (uint32_t)__builtin_amdgcn_readlane((int)e, (int)(act - 1));
+    if (last_e != nbit || lane63(incl) != cnt) return false;
+    // every span settled: decode it once more, writing its symbols
+    uint32_t o = incl - c;
+    if (b < lim) {
+        uint32_t p = b;
+        bits.seek(p);
+        while (p < lim) {
+            const uint32_t en = dec_entry(t, bits.window());
+            const uint32_t L = (en >> 24) & 31u;
+            uint32_t v = en & 0xFFFFFFu;
+            if (en & DEC_KEY) v = (uint32_t)keys[v];
+            if (STAGED)
+                ost[o] = (T)v;
+            else
+                dst[o] = (T)v;
+            o++;
+            bits.consume(p, L);
+            p += L;
+        }
+    }
+    return true;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64 * dec_waves<T>())
 huf_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, T* __restrict__ out, uint64_t n,
                int* __restrict__ d_status) {
+    constexpr bool FOLD = sizeof(T) <= 2;
+    constexpr int DEC_WAVES = dec_waves<T>();
     __shared__ uint32_t s_first[32], s_entry[32];
-    __shared__ uint32_t s_lut[1 << LUT_BITS];  // (len << 24) | key index ; 0 = long code
+    __shared__ uint64_t s_lim[32];
+    __shared__ int32_t s_diff[32];
+    __shared__ uint32_t s_lut[1 << DEC_LUT_BITS];
+    __shared__ uint32_t s_lut2[DEC_LUT2];
+    __shared__ uint32_t s_cells[DEC_WAVES][DEC_CELLS + 4];
+    __shared__ uint4 s_out[DEC_WAVES][DEC_SUBLEN_MAX * sizeof(T) / 16];
     if (in_bytes < HDR_BYTES) {
-        if (threadIdx.x == 0) *d_status = 1;
+        if (blockIdx.x == 0 && threadIdx.x == 0) *d_status = 1;
         return;
     }
     const int32_t sublen = *reinterpret_cast<const int32_t*>(in + 4);
@@ -230,8 +481,9 @@ huf_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, T* __restrict_
     const uint32_t* ent = reinterpret_cast<const uint32_t*>(in + 40);
     const uint32_t e1 = ent[1], e2 = ent[2], e3 = ent[3], e4 = ent[4], e5 = ent[5];
     const bool hdr_ok = sublen > 0 && pardeg >= 0 && (uint64_t)pardeg == (n + sublen - 1) / sublen &&
-                        e1 <= e2 && e2 - e1 >= 256 && e2 <= e3 && e3 <= e4 && e4 <= e5 && e5 <= in_bytes &&
-                        e3 - e2 == 4ull * pardeg && e4 - e3 == 4ull * pardeg && (e4 % 4) == 0 && (e1 % 4) == 0;
+                        e1 <= e2 && e2 - e1 >= 256 && (e2 - e1 - 256) / sizeof(T) <= 65536 && e2 <= e3 &&
+                        e3 <= e4 && e4 <= e5 && e5 <= in_bytes && e3 - e2 == 4ull * pardeg &&
+                        e4 - e3 == 4ull * pardeg && (e4 % 4) == 0 && (e1 % 4) == 0;
     if (!hdr_ok) {
         if (blockIdx.x == 0 && threadIdx.x == 0) *d_status = 1;
         return;
@@ -239,117 +491,106 @@ huf_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, T* __restrict_
     const uint32_t nkeys = (e2 - e1 - 256) / sizeof(T);
     const T* keys = reinterpret_cast<const T*>(in + e1 + 256);
     if (threadIdx.x < 32) {
-        s_first[threadIdx.x] = reinterpret_cast<const uint32_t*>(in + e1)[threadIdx.x];
-        s_entry[threadIdx.x] = reinterpret_cast<const uint32_t*>(in + e1 + 128)[threadIdx.x];
+        const uint32_t f = reinterpret_cast<const uint32_t*>(in + e1)[threadIdx.x];
+        const uint32_t en = reinterpret_cast<const uint32_t*>(in + e1 + 128)[threadIdx.x];
+        s_first[threadIdx.x] = f;
+        s_entry[threadIdx.x] = en;
+        s_lim[threadIdx.x] = threadIdx.x ? (uint64_t)f << (32 - threadIdx.x) : ~0ull;
+        s_diff[threadIdx.x] = (int32_t)(en - f);
     }
     __syncthreads();
-    // prefix table: simulate the canonical loop on every LUT_BITS-bit prefix
-    for (uint32_t pat = threadIdx.x; pat < (1u << LUT_BITS); pat += blockDim.x) {
-        uint32_t v = pat >> (LUT_BITS - 1), lv = 1, res = 0;
-        for (int i = 1;; i++) {
+    // longest code of the book: the last length with a symbol (entry[l+1] > entry[l])
+    uint32_t maxl = 1;
+    for (uint32_t l = 1; l < 31; l++)
+        if (s_entry[l + 1] > s_entry[l]) maxl = l;
+    if (s_entry[31] < nkeys) maxl = 31;
+    // The canonical loop on the window W from length l0 up to length lmax:
+    // the entry of the code, DEC_BAD when the window starts no code of the
+    // book, or (first level) DEC_LONG | (W's 12-bit prefix) for a code longer
+    // than 12 bits; 0 (second level) for a code longer than lmax.
+    auto canon = [&](uint32_t W, uint32_t l0, uint32_t lmax, bool first_level) -> uint32_t {
+        for (uint32_t lv = l0; lv <= lmax; lv++) {
+            const uint32_t v = W >> (32 - lv);
             if (!(v < s_first[lv])) {
                 const uint32_t k = s_entry[lv] + v - s_first[lv];
-                res = (k < nkeys && k < (1u << 24)) ? ((lv << 24) | k) : 0u;
-                break;
+                if (k >= nkeys) return DEC_BAD;
+                return (lv << 24) | (FOLD ? (uint32_t)keys[k] : (DEC_KEY | k));
             }
-            if (i == LUT_BITS) break;
-            v = (v << 1) | ((pat >> (LUT_BITS - 1 - i)) & 1u);
-            lv++;
         }
-        s_lut[pat] = res;
+        return first_level ? (DEC_LONG | (W >> (32 - DEC_LUT_BITS))) : 0u;
+    };
+    for (uint32_t pat = threadIdx.x; pat < (1u << DEC_LUT_BITS); pat += blockDim.x)
+        s_lut[pat] = canon(pat << (32 - DEC_LUT_BITS), 1, DEC_LUT_BITS, true);
+    // second level: the long codes' 12-bit prefixes are [0, first[12]) (the
+    // smallest values of their lengths); B2 more bits index it
+    const uint32_t n_long = maxl > (uint32_t)DEC_LUT_BITS ? s_first[DEC_LUT_BITS] : 0u;
+    uint32_t b2 = 0;
+    if (n_long > 0)
+        while (b2 < maxl - DEC_LUT_BITS && (n_long << (b2 + 1)) <= DEC_LUT2) b2++;
+    for (uint32_t i = threadIdx.x; i < (b2 ? n_long << b2 : 0u); i += blockDim.x) {
+        const uint32_t W = ((i >> b2) << (32 - DEC_LUT_BITS)) | ((i & ((1u << b2) - 1u)) << (32 - DEC_LUT_BITS - b2));
+        s_lut2[i] = canon(W, DEC_LUT_BITS + 1, DEC_LUT_BITS + b2, false);
     }
     __syncthreads();
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (uint32_t)pardeg) return;
-    const uint32_t nbit = reinterpret_cast<const uint32_t*>(in + e2)[p];
-    const uint32_t pe = reinterpret_cast<const uint32_t*>(in + e3)[p];
+    const DecTables tb{s_lut, s_lut2, s_lim, s_diff, maxl, nkeys, b2};
+    const uint32_t w = wave_in_wg();
+    const uint32_t l = (uint32_t)lane_id();
     const uint32_t ncells_total = (e5 - e4) / 4;
-    const uint32_t ncell = (nbit + 31) / 32;
-    const uint64_t o0 = (uint64_t)p * sublen;
-    const uint32_t cnt = (uint32_t)((n - o0) < (uint64_t)sublen ? (n - o0) : (uint64_t)sublen);
-    T* dst = out + o0;
-    if ((uint64_t)pe + ncell > ncells_total) {
-        *d_status = 1;
-        return;
-    }
-    if (s_entry[1] == 1 && nbit == 0) {  // one symbol with a zero-length code
-        const T k0 = nkeys ? keys[0] : (T)0;
-        for (uint32_t j = 0; j < cnt; j++) dst[j] = k0;
-        return;
-    }
-    const uint32_t* cells = reinterpret_cast<const uint32_t*>(in + e4) + pe;
-    // 64-bit window: `buf` holds the next `avail` bits MSB-aligned
-    uint64_t buf = 0;
-    int avail = 0;
-    uint32_t next_cell = 0;
-    auto refill = [&]() {
-        while (avail <= 32) {
-            const uint32_t c = next_cell < ncell ? cells[next_cell] : 0u;
-            next_cell++;
-            buf |= (uint64_t)c << (32 - avail);
-            avail += 32;
+    const uint32_t* par_nbit = reinterpret_cast<const uint32_t*>(in + e2);
+    const uint32_t* par_entry = reinterpret_cast<const uint32_t*>(in + e3);
+    const uint32_t* cells = reinterpret_cast<const uint32_t*>(in + e4);
+    uint32_t* cst = s_cells[w];
+    T* ost = reinterpret_cast<T*>(s_out[w]);
+    for (uint32_t p = blockIdx.x * DEC_WAVES + w; p < (uint32_t)pardeg; p += gridDim.x * DEC_WAVES) {
+        const uint32_t nbit = par_nbit[p];
+        const uint32_t pe = par_entry[p];
+        const uint32_t ncell = (uint32_t)(((uint64_t)nbit + 31) / 32);
+        const uint64_t o0 = (uint64_t)p * (uint32_t)sublen;
+        const uint32_t cnt = (uint32_t)((n - o0) < (uint64_t)sublen ? (n - o0) : (uint64_t)sublen);
+        T* dst = out + o0;
+        if ((uint64_t)pe + ncell > ncells_total) {
+            if (l == 0) *d_status = 1;
+            continue;
         }
-    };
-    uint32_t used = 0;
-    bool bad = false;
-    auto decode_one = [&]() -> T {
-        refill();
-        const uint32_t pat = (uint32_t)(buf >> (64 - LUT_BITS));
-        const uint32_t e = s_lut[pat];
-        uint32_t key, L;
-        if (e) {
-            L = e >> 24;
-            key = e & 0xFFFFFFu;
-        } else {
-            // long code: continue the canonical loop from LUT_BITS bits
-            uint32_t v = pat, lv = LUT_BITS;
-            int i = LUT_BITS;
-            bool done = false;
-            while (lv < 31) {
-                v = (v << 1) | (uint32_t)((buf >> (63 - i)) & 1u);
-                lv++;
-                i++;
-                if (!(v < s_first[lv])) { done = true; break; }
+        if (s_entry[1] == 1 && nbit == 0) {  // one symbol with a zero-length code
+            const T k0 = nkeys ? keys[0] : (T)0;
+            for (uint32_t j = l; j < cnt; j += 64) dst[j] = k0;
+            continue;
+        }
+        bool ok;
+        if (ncell <= DEC_CELLS && (uint32_t)sublen <= DEC_SUBLEN_MAX) {
+            wave_lds_sync();  // the previous partition's staging reads are done
+            for (uint32_t i = l; i < ncell + 4; i += 64) cst[i] = i < ncell ? cells[pe + i] : 0u;
+            wave_lds_sync();
+            ok = dec_partition<T, true>(cst, ncell, tb, nbit, cnt, keys, ost, dst);
+            wave_lds_sync();
+            if (ok) {
+                const uint32_t bytes = cnt * (uint32_t)sizeof(T);
+                uint32_t done = 0;
+                if (((uintptr_t)dst & 15u) == 0) {
+                    const uint32_t nv = bytes / 16;
+                    for (uint32_t i = l; i < nv; i += 64) reinterpret_cast<uint4*>(dst)[i] = s_out[w][i];
+                    done = nv * 16 / (uint32_t)sizeof(T);
+                }
+                for (uint32_t i = done + l; i < cnt; i += 64) dst[i] = ost[i];
             }
-            const uint32_t k = s_entry[lv] + v - s_first[lv];
-            if (!done || k >= nkeys) {
-                bad = true;
-                return (T)0;
-            }
-            L = lv;
-            key = k;
-        }
-        buf <<= L;
-        avail -= (int)L;
-        used += L;
-        return keys[key];
-    };
-    constexpr int E = 16 / sizeof(T);
-    const bool vec = ((uint64_t)sublen * sizeof(T)) % 16 == 0;
-    for (uint32_t g = 0; g < cnt && !bad; g += E) {
-        T pack[E];
-#pragma unroll
-        for (int q = 0; q < E; q++) pack[q] = (g + q < cnt && !bad) ? decode_one() : (T)0;
-        if (vec && g + E <= cnt) {
-            *reinterpret_cast<uint4*>(dst + g) = *reinterpret_cast<const uint4*>(pack);
         } else {
-#pragma unroll
-            for (int q = 0; q < E; q++)
-                if (g + q < cnt) dst[g + q] = pack[q];
+            ok = dec_partition<T, false>(cells + pe, ncell, tb, nbit, cnt, keys, ost, dst);
         }
+        if (!ok && l == 0) *d_status = 1;
     }
-    if (bad || used != nbit) *d_status = 1;
 }
 
 struct EncWs {
-    size_t pardeg, o_ncell, o_tot, o_scan, o_cells, total;
+    size_t pardeg, o_ncell, o_tot, o_nbit, o_scan, o_cells, total;
 };
 EncWs enc_layout(size_t n, int sl) {
     EncWs w;
     w.pardeg = cdiv(std::max<size_t>(n, 1), (size_t)sl);
     size_t o = 0;
     w.o_ncell = o; o = align_up(o + 4 * w.pardeg, 256);
-    w.o_tot = o;   o = align_up(o + 16, 256);
+    w.o_tot = o;   o = align_up(o + 256, 256);  // total cells (u32)
+    w.o_nbit = o;  o = align_up(o + 8 * NBIT_SLOTS, 256);  // bit-total accumulators (u64)
     w.o_scan = o;  o = align_up(o + scan_ws_bytes(w.pardeg), 256);
     w.o_cells = o; o = align_up(o + 4 * (size_t)cells_of(sl) * w.pardeg, 256);
     w.total = o;
@@ -382,8 +623,11 @@ int enc_impl(hipStream_t s, const void* in, size_t n, uint32_t bklen, const uint
              rev_bytes, o8, scells, ncell, pardeg, book_in_lds);
     if (int rc = scan_u32(s, ncell, reinterpret_cast<uint32_t*>(o8 + ent3), pardeg, nullptr, tot, w + L.o_scan))
         return rc;
-    launch_k("huf_pack_kernel", huf_pack_kernel, dim3(blocks), dim3(256), 0, s, scells, ncell, o8, ent3, ent4, pardeg, (uint32_t)CELLS);
-    launch_k("huf_header_kernel", huf_header_kernel, dim3(1), dim3(1024), 0, s, o8, n, bklen, (uint32_t)SL, ent1, ent2, ent3, ent4, pardeg, tot, out_size);
+    unsigned long long* nbit_acc = reinterpret_cast<unsigned long long*>(w + L.o_nbit);
+    launch_k("huf_pack_kernel", huf_pack_kernel, dim3(blocks), dim3(256), 0, s, scells, ncell, o8, ent2, ent3, ent4,
+             pardeg, (uint32_t)CELLS, nbit_acc);
+    launch_k("huf_header_kernel", huf_header_kernel, dim3(1), dim3(256), 0, s, o8, n, bklen, (uint32_t)SL, ent1, ent2,
+             ent3, ent4, pardeg, tot, out_size, nbit_acc);
     FZH_CHECK_LAUNCH("huffman encode");
     return FZH_OK;
 }
@@ -461,15 +705,24 @@ int fzh_huffman_decode(void* stream, const void* d_in, size_t in_bytes, int elem
     if (n == 0) return FZH_OK;
     if (!d_status) return fail(FZH_E_INVALID, "huffman dec: null status");
     if ((uintptr_t)d_in % 4 || (uintptr_t)d_out % 16) return fail(FZH_E_INVALID, "huffman dec: input 4-byte, output 16-byte alignment required");
+    // persistent workgroups: each builds the decode tables once and then
+    // takes partitions in turn (the partition count is in the stream; the
+    // coarse 768-symbol partitions bound it)
     const size_t pardeg = cdiv(n, SUBLEN);
-    const unsigned blocks = (unsigned)cdiv(pardeg, 256);
     hipStream_t s = S(stream);
+#define FZH_HUF_DEC(T)                                                                                         \
+    {                                                                                                          \
+        const unsigned blocks = (unsigned)std::min<size_t>(cdiv(pardeg, dec_waves<T>()), 768);                \
+        launch_k("huf_dec_kernel", huf_dec_kernel<T>, dim3(blocks), dim3(64 * dec_waves<T>()), 0, s,          \
+                 (const uint8_t*)d_in, in_bytes, (T*)d_out, n, d_status);                                      \
+    }
     switch (elem_bytes) {
-    case 1: launch_k("huf_dec_kernel", huf_dec_kernel<uint8_t>, dim3(blocks), dim3(256), 0, s, (const uint8_t*)d_in, in_bytes, (uint8_t*)d_out, n, d_status); break;
-    case 2: launch_k("huf_dec_kernel", huf_dec_kernel<uint16_t>, dim3(blocks), dim3(256), 0, s, (const uint8_t*)d_in, in_bytes, (uint16_t*)d_out, n, d_status); break;
-    case 4: launch_k("huf_dec_kernel", huf_dec_kernel<uint32_t>, dim3(blocks), dim3(256), 0, s, (const uint8_t*)d_in, in_bytes, (uint32_t*)d_out, n, d_status); break;
+    case 1: FZH_HUF_DEC(uint8_t) break;
+    case 2: FZH_HUF_DEC(uint16_t) break;
+    case 4: FZH_HUF_DEC(uint32_t) break;
     default: return fail(FZH_E_INVALID, "huffman dec: elem_bytes must be 1, 2 or 4");
     }
+#undef FZH_HUF_DEC
     FZH_CHECK_LAUNCH("huf_dec_kernel");
     return FZH_OK;
 }

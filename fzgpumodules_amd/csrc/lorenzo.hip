@@ -16,6 +16,7 @@
 // kernel places the lists, so the outlier list is deterministic: tile-major,
 // and sorted by element index inside each tile (the reference appends them
 // with atomics in arbitrary order).
+#include <atomic>
 #include <mutex>
 
 #include "fzh_compact.h"
@@ -226,9 +227,37 @@ __device__ __forceinline__ bool tile_of(const TileMap& m, uint32_t idx, uint32_t
     return true;
 }
 
+// Outlier binning.  A list in tile order (what the forward writes) is
+// binned without atomics: bin_check_kernel finds out whether the list is in
+// tile order with valid indices; if it is, the counting kernels return at
+// once, the scan sees a count of zero, and bin_place_kernel takes each
+// tile's range from the tile changes along the list.  Otherwise (the
+// reference's atomic order) a counting sort runs: per-tile counts with
+// atomics, their scan, and placement.  `flag` == `tag` marks an unordered
+// list; `tag` is new for every call, so the flag needs no clearing, and
+// `dn` (the scan's element count) is T for an unordered list and is reset
+// to 0 by the ordered path.
+__global__ void __launch_bounds__(256)
+bin_check_kernel(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ count, uint64_t max_out,
+                 TileMap m, uint32_t T, uint32_t* __restrict__ flag, uint32_t* __restrict__ dn, uint32_t tag) {
+    const uint64_t k = min((uint64_t)*count, max_out);
+    bool unordered = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t t, loc, tp, lp;
+        if (!tile_of(m, idx[i], t, loc)) unordered = true;
+        else if (i > 0 && (!tile_of(m, idx[i - 1], tp, lp) || tp > t)) unordered = true;
+    }
+    if (__ballot(unordered) && lane_id() == 0) {
+        *flag = tag;
+        *dn = T;
+    }
+}
+
 __global__ void __launch_bounds__(256)
 bin_count_kernel(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ count, uint64_t max_out,
-                 TileMap m, uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ slot) {
+                 TileMap m, uint32_t* __restrict__ tile_cnt, uint32_t* __restrict__ slot,
+                 const uint32_t* __restrict__ flag, uint32_t tag) {
+    if (*flag != tag) return;  // in tile order: bin_place_kernel bins it
     const uint64_t k = min((uint64_t)*count, max_out);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t t, loc;
@@ -239,33 +268,62 @@ bin_count_kernel(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ 
 __global__ void __launch_bounds__(256)
 bin_place_kernel(const float* __restrict__ err, const uint32_t* __restrict__ idx,
                  const uint32_t* __restrict__ count, uint64_t max_out, TileMap m,
-                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ slot,
-                 float* __restrict__ serr, uint32_t* __restrict__ sloc) {
+                 uint32_t* __restrict__ off, const uint32_t* __restrict__ slot,
+                 float* __restrict__ serr, uint32_t* __restrict__ sloc, uint32_t T,
+                 const uint32_t* __restrict__ flag, uint32_t* __restrict__ dn, uint32_t tag) {
     const uint64_t k = min((uint64_t)*count, max_out);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t t, loc;
-        if (tile_of(m, idx[i], t, loc)) {
-            const uint32_t p = off[t] + slot[i];
-            serr[p] = err[i];
-            sloc[p] = loc;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (*flag == tag) {
+        for (uint64_t i = gid; i < k; i += stride) {
+            uint32_t t, loc;
+            if (tile_of(m, idx[i], t, loc)) {
+                const uint32_t p = off[t] + slot[i];
+                serr[p] = err[i];
+                sloc[p] = loc;
+            }
+        }
+        return;
+    }
+    // in tile order: entry i starts the range of every tile after its
+    // predecessor's tile up to its own (entry k: every tile up to T)
+    for (uint64_t i = gid; i <= k; i += stride) {
+        uint32_t t = T, loc = 0, tp = 0, lp;
+        if (i < k) (void)tile_of(m, idx[i], t, loc);
+        const bool head = i == 0 || (tile_of(m, idx[i - 1], tp, lp), tp != t);
+        if (head)
+            for (uint32_t u = i == 0 ? 0u : tp + 1; u <= t; u++) off[u] = (uint32_t)i;
+        if (i < k) {
+            serr[i] = err[i];
+            sloc[i] = loc;
         }
     }
+    if (gid == 0) *dn = 0;
 }
 
-// Adds the tile's outliers into the per-lane registers.  `reg_of(loc, lane, r)`
-// maps a local index to (lane, register); registers are updated with a select
-// chain so no dynamic register indexing is needed.
+// Adds the tile's binned outliers into the per-lane registers: lane k
+// loads entry o0 + j0 + k of the tile's range (64 independent loads per
+// round, not a dependent chain), then the wave walks the round's entries
+// through cross-lane reads.  LANE_OF / REG_OF map the local index `loc` to
+// (lane, register); registers are updated with a select chain so no dynamic
+// register indexing is needed.
 #define FZH_APPLY_OUTLIERS(NREG, DVAR, LANE_OF, REG_OF)                      \
     do {                                                                     \
         const uint32_t o0 = off[tile], o1 = off[tile + 1];                   \
-        for (uint32_t j = o0; j < o1; j++) {                                 \
-            const uint32_t loc = sloc[j];                                    \
-            const float e = serr[j];                                         \
-            const int tl = (LANE_OF);                                        \
-            const int tr = (REG_OF);                                         \
-            if (l == tl) {                                                   \
-                _Pragma("unroll") for (int rr = 0; rr < NREG; rr++)          \
-                    if (rr == tr) DVAR[rr] = e + DVAR[rr];                   \
+        for (uint32_t j0 = o0; j0 < o1; j0 += 64) {                          \
+            const uint32_t jl = j0 + (uint32_t)l;                            \
+            const uint32_t loc_ = jl < o1 ? sloc[jl] : 0u;                   \
+            const float e_ = jl < o1 ? serr[jl] : 0.0f;                      \
+            const uint32_t m_ = min(o1 - j0, 64u);                           \
+            for (uint32_t k = 0; k < m_; k++) {                              \
+                const uint32_t loc = (uint32_t)__builtin_amdgcn_readlane((int)loc_, (int)k); \
+                const float e = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e_), (int)k)); \
+                const int tl = (LANE_OF);                                    \
+                const int tr = (REG_OF);                                     \
+                if (l == tl) {                                               \
+                    _Pragma("unroll") for (int rr = 0; rr < NREG; rr++)      \
+                        if (rr == tr) DVAR[rr] = e + DVAR[rr];               \
+                }                                                            \
             }                                                                \
         }                                                                    \
     } while (0)
@@ -526,49 +584,138 @@ lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float
         lz2d_inv_tile<TCode, ZZ, DIRECT, PATCH, false>(codes, nx, ny, ebx2, off, serr, sloc, out, gx, tile, dout, pin);
 }
 
-// 3D inverse, scan tree of lorenzo_quant_nd.cu:281-364 (y sequential, x
-// Hillis-Steele over 8, z Hillis-Steele over 8).
+// 3D inverse, the addition tree of lorenzo_quant_nd.cu:282-360: outlier +
+// decoded code, a sequential prefix along y, then Hillis-Steele prefixes
+// (offsets 1, 2, 4) along x and then along z, times ebx2.  One wave per
+// tile: lane (x, z) = (l & 7, l >> 3) holds rows y = 0..7 for the y prefix
+// and the x prefix (DPP within 8-lane groups); the tile then passes through
+// LDS ([z][y][x], 72 floats per z plane: 2 lanes per bank) to lane
+// (x, y) = (l & 7, l >> 3) holding z = 0..7, where the z prefix is register
+// arithmetic.  Outliers are added in the same LDS tile, one entry per lane;
+// strict-mode patches of a short list are stored after the tile's values.
 template <typename TCode, bool ZZ, bool DIRECT, bool PATCH>
 __global__ void __launch_bounds__(256)
 lz3d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, uint32_t nz, float ebx2,
                 const uint32_t* __restrict__ off, const float* __restrict__ serr,
                 const uint32_t* __restrict__ sloc, float* __restrict__ out, Tiles tg, uint32_t T,
                 DirectOut dout, DirectOut pin) {
+    constexpr uint32_t ZS = 72;  // LDS floats per z plane of the tile
+    __shared__ float s_tile[4][8 * ZS];
     const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (tile >= T) return;
+    float* st = s_tile[wave_in_wg()];
     const int l = lane_id();
     const int x = l & 7, z = l >> 3;
     const uint32_t bx = tile % tg.gx, by = (tile / tg.gx) % tg.gy, bz = tile / (tg.gx * tg.gy);
-    const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
-    const bool xz = X < nx && Z < nz;
     const uint64_t plane = (uint64_t)nx * ny;
+    const uint64_t tbase = (uint64_t)(bz * 8) * plane + (uint64_t)(by * 8) * nx + bx * 8;
+    const bool interior = bx * 8 + 8 <= nx && by * 8 + 8 <= ny && bz * 8 + 8 <= nz && plane < (1ull << 26);
+    // -- decoded codes, lane (x, z), row y
     float d[8];
+    {
+        const bool xz = bx * 8 + x < nx && bz * 8 + z < nz;
+        const uint32_t lo = (uint32_t)z * (uint32_t)plane + (uint32_t)x;
+        if (interior) {
 #pragma unroll
-    for (int y = 0; y < 8; y++)
-        d[y] = (xz && Y0 + y < ny) ? dec_code<TCode, ZZ>(codes[Z * plane + (uint64_t)(Y0 + y) * nx + X]) : 0.0f;
-    // local = (z << 6) | (y << 3) | x ; lane = x + 8z, register = y
-    FZH_APPLY_ANY(8, d, (int)((loc & 7u) | ((loc >> 3) & 56u)), (int)((loc >> 3) & 7u));
+            for (int y = 0; y < 8; y++)
+                d[y] = dec_code<TCode, ZZ>(*reinterpret_cast<const TCode*>(
+                    reinterpret_cast<const char*>(codes + tbase + (uint64_t)y * nx) + (uint32_t)sizeof(TCode) * lo));
+        } else {
+#pragma unroll
+            for (int y = 0; y < 8; y++)
+                d[y] = (xz && by * 8 + y < ny) ? dec_code<TCode, ZZ>(codes[tbase + (uint64_t)z * plane + (uint64_t)y * nx + x])
+                                               : 0.0f;
+        }
+    }
+    // -- outliers: value + decoded code, in the LDS tile (one entry per lane)
+    bool any = false;
+    uint32_t o0 = 0, o1 = 0, kk = 0;
+    if constexpr (DIRECT) {
+        kk = dout.count ? min(*dout.count, dout.max_out) : 0u;
+        any = kk > 0;
+    } else {
+        o0 = off[tile];
+        o1 = off[tile + 1];
+        any = o1 > o0;
+    }
+    if (any) {
+#pragma unroll
+        for (int y = 0; y < 8; y++) st[z * ZS + y * 8 + x] = d[y];
+        wave_lds_sync();
+        if constexpr (DIRECT) {
+            for (uint32_t j = (uint32_t)l; j < kk; j += 64) {
+                uint32_t t, loc;
+                if (tile_of(dout.m, dout.idx[j], t, loc) && t == tile) {
+                    const uint32_t a = (loc >> 6) * ZS + (loc & 63u);
+                    st[a] = dout.err[j] + st[a];
+                }
+            }
+        } else {
+            for (uint32_t j = o0 + (uint32_t)l; j < o1; j += 64) {
+                const uint32_t loc = sloc[j];
+                const uint32_t a = (loc >> 6) * ZS + (loc & 63u);
+                st[a] = serr[j] + st[a];
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int y = 0; y < 8; y++) d[y] = st[z * ZS + y * 8 + x];
+        wave_lds_sync();
+    }
+    // -- y prefix (sequential), x prefix (offsets 1, 2, 4 inside the 8-lane group)
 #pragma unroll
     for (int y = 1; y < 8; y++) d[y] += d[y - 1];
 #pragma unroll
     for (int y = 0; y < 8; y++) {
         float v = d[y];
-#pragma unroll
-        for (int o = 1; o < 8; o <<= 1) {
-            const float u = __shfl_up(v, o, 8);
-            if (x >= o) v += u;
-        }
-#pragma unroll
-        for (int o = 1; o < 8; o <<= 1) {
-            const float u = __shfl_up(v, 8 * o, 64);
-            if (z >= o) v += u;
-        }
-        d[y] = v * ebx2;
+        float u = __uint_as_float(dpp_u32<0x111>(__float_as_uint(v)));  // lane x - 1 (row_shr:1)
+        if (x >= 1) v += u;
+        u = __uint_as_float(dpp_u32<0x112>(__float_as_uint(v)));        // x - 2
+        if (x >= 2) v += u;
+        u = __uint_as_float(dpp_u32<0x114>(__float_as_uint(v)));        // x - 4
+        if (x >= 4) v += u;
+        st[z * ZS + y * 8 + x] = v;
     }
-    FZH_PATCH_ANY(8, d, (int)((loc & 7u) | ((loc >> 3) & 56u)), (int)((loc >> 3) & 7u));
+    wave_lds_sync();
+    // -- lane (x, y): z = 0..7; z prefix in registers (descending, so each
+    // stage adds the previous stage's values)
+    const int yy = l >> 3;
+    float v[8];
 #pragma unroll
-    for (int y = 0; y < 8; y++)
-        if (xz && Y0 + y < ny) out[Z * plane + (uint64_t)(Y0 + y) * nx + X] = d[y];
+    for (int zz = 0; zz < 8; zz++) v[zz] = st[zz * ZS + yy * 8 + x];
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1)
+#pragma unroll
+        for (int zz = 7; zz >= o; zz--) v[zz] += v[zz - o];
+    {
+        const bool xy = bx * 8 + x < nx && by * 8 + yy < ny;
+        const uint32_t lo = (uint32_t)yy * nx + (uint32_t)x;
+        if (interior) {
+#pragma unroll
+            for (int zz = 0; zz < 8; zz++)
+                *reinterpret_cast<float*>(reinterpret_cast<char*>(out + tbase + (uint64_t)zz * plane) + 4u * lo) =
+                    v[zz] * ebx2;
+        } else {
+#pragma unroll
+            for (int zz = 0; zz < 8; zz++)
+                if (xy && bz * 8 + zz < nz) out[tbase + (uint64_t)zz * plane + lo] = v[zz] * ebx2;
+        }
+    }
+    // -- strict-mode patches of a short list: stored after the tile's values
+    // (a wave's stores to one address land in program order)
+    if constexpr (PATCH) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // the tile's stores first
+        const uint32_t kp = pin.count ? min(*pin.count, pin.max_out) : 0u;
+        for (uint32_t j = (uint32_t)l; j < kp; j += 64) {
+            const uint32_t q = pin.idx[j];
+            uint32_t t, loc;
+            if (tile_of(pin.m, q, t, loc)) {
+                if (t == tile) out[q] = pin.err[j];
+            } else if (tile == 0 && (uint64_t)q >= pin.m.total && (uint64_t)q < pin.n_out) {
+                out[q] = pin.err[j];
+            }
+        }
+    }
 }
 
 // Strict-mode patches: out[idx[i]] = val[i] for i < min(*count, cap).
@@ -675,7 +822,7 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
         launch_k("lz2d_fwd_kernel", lz2d_fwd_kernel<TCode, ZZV, STRICT>, dim3(blocks), dim3(256), 0, s, in, \
                  (uint32_t)nx, (uint32_t)ny, n, r, d_r, rad, c, tl, (uint32_t)cdiv(nx, 32), (uint32_t)T);  \
     else                                                                                                \
-        launch_k("lz3d_fwd_kernel", lz3d_fwd_kernel<TCode, ZZV, STRICT>, dim3(blocks), dim3(256), 0, s, in, \
+        launch_k("lz3d_fwd_kernel", lz3d_fwd_kernel<TCode, ZZV, STRICT>, dim3(blocks), dim3(256), 0, s, in,  \
                  (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, n, r, d_r, rad, c, tl,                       \
                  Tiles{(uint32_t)cdiv(nx, 8), (uint32_t)cdiv(ny, 8), 0}, (uint32_t)T);
     if (zz) {
@@ -696,7 +843,7 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
 // the outliers binned by tile.
 struct InvWs {
     size_t T, max_out;
-    size_t o_scan, o_cnt, o_off, o_slot, o_err, o_loc, total;
+    size_t o_scan, o_cnt, o_off, o_slot, o_err, o_loc, o_flag, total;
 };
 
 InvWs inv_layout(size_t n, size_t nx, size_t ny, size_t nz, size_t max_out) {
@@ -710,6 +857,7 @@ InvWs inv_layout(size_t n, size_t nx, size_t ny, size_t nz, size_t max_out) {
     w.o_slot = o; o = align_up(o + 4 * std::max<size_t>(max_out, 1), 256);
     w.o_err = o;  o = align_up(o + 4 * std::max<size_t>(max_out, 1), 256);
     w.o_loc = o;  o = align_up(o + 4 * std::max<size_t>(max_out, 1), 256);
+    w.o_flag = o; o = align_up(o + 8, 256);
     w.total = o;
     return w;
 }
@@ -740,15 +888,19 @@ int inv_bin(hipStream_t s, size_t n, size_t nx, size_t ny, size_t nz, const floa
     if (max_out <= kDirectOutliers) return FZH_OK;  // the inverse reads the list itself
     const TileMap m = tile_map(n, nx, ny, nz);
     const bool has_out = count != nullptr && max_out > 0;
+    if (!has_out) return zero_bytes(s, off, 4 * (L.T + 1));
+    uint32_t* flag = (uint32_t*)(w + L.o_flag);
+    uint32_t* dn = flag + 1;
+    static std::atomic<uint32_t> epoch{0};
+    const uint32_t tag = 2u * (epoch.fetch_add(1, std::memory_order_relaxed) + 1u) + 1u;
     const unsigned ob = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(max_out, 256), 1024));
-    if (has_out) {
-        launch_k("bin_count_kernel", bin_count_kernel, dim3(ob), dim3(256), 0, s, idx, count, max_out, m, cnt, slot);
-    }
-    if (int rc = scan_u32<false, true>(s, cnt, off, L.T, nullptr, off + L.T, w + L.o_scan)) return rc;
-    if (has_out) {
-        launch_k("bin_place_kernel", bin_place_kernel, dim3(ob), dim3(256), 0, s, err, idx, count, max_out, m, off,
-                 slot, serr, sloc);
-    }
+    launch_k("bin_check_kernel", bin_check_kernel, dim3(ob), dim3(256), 0, s, idx, count, max_out, m, (uint32_t)L.T,
+             flag, dn, tag);
+    launch_k("bin_count_kernel", bin_count_kernel, dim3(ob), dim3(256), 0, s, idx, count, max_out, m, cnt, slot,
+             (const uint32_t*)flag, tag);
+    if (int rc = scan_u32<false, true>(s, cnt, off, L.T, dn, off + L.T, w + L.o_scan)) return rc;
+    launch_k("bin_place_kernel", bin_place_kernel, dim3(ob), dim3(256), 0, s, err, idx, count, max_out, m, off,
+             (const uint32_t*)slot, serr, sloc, (uint32_t)L.T, (const uint32_t*)flag, dn, tag);
     FZH_CHECK_LAUNCH("lorenzo inv binning");
     return FZH_OK;
 }

@@ -6,12 +6,16 @@
 // x[i] != x[i-1] (rle.cu:45-55).
 //
 // MI355X design
-//   encode: K1 — one wave per 8 x 64 x (16/sizeof T) elements finds the run
-//     starts (boundary flags from 16-byte loads) and writes their positions to
-//     the tile's scratch slot; K2 — the shared compaction kernel packs the
-//     positions in order (R = their number); K3 — values = input at each
-//     start, lengths = next start - start, header and padding.  (The
-//     reference does 5 passes plus a CUB scan through ~15 B/element of
+//   encode: one wave per tile of 8 x 64 x (16/sizeof T) elements.  K1
+//     counts the tile's run starts (boundary flags from 16-byte loads); a
+//     scan of the counts gives each tile's first run index and R; K2 finds
+//     the starts again and writes every run's value and every length whose
+//     run ends inside the tile straight to the output (the previous start
+//     from a running maximum across lanes), leaving the start of the tile's
+//     last run in its length slot; K3, one thread per tile, turns those into
+//     lengths with the next tile's first start and writes the header.  The
+//     input is read twice and the output written once, no position scratch.
+//     (The reference does 5 passes plus a CUB scan through ~15 B/element of
 //     scratch.)
 //   decode: a three-launch segmented scan of the run lengths gives the run
 //     starts and a map "output tile -> first run overlapping it" (every run
@@ -19,7 +23,6 @@
 //     output tile stages that tile's runs in LDS, each lane binary-searches
 //     its first run and fills 64 consecutive outputs.  No host round trip
 //     (the reference syncs twice and fills with one thread per run).
-#include "fzh_compact.h"
 #include "fzh_scan.h"
 
 namespace fzh {
@@ -28,17 +31,13 @@ namespace {
 constexpr int STEPS = 8;
 constexpr uint32_t DTILE = 4096;  // decode output tile (elements)
 
+// Run-start flags of one wave's tile: mask[j] bit k <=> element
+// base + 64 E j + E l + k starts a run (k < E, E elements per 16-byte load).
 template <typename T>
-__global__ void __launch_bounds__(256)
-rle_enc_runs_kernel(const T* __restrict__ in, uint64_t n, uint32_t* __restrict__ spos, uint32_t* __restrict__ scnt,
-                    uint32_t ntiles) {
-    constexpr int E = 16 / sizeof(T);       // elements per 16-byte load
-    constexpr uint32_t TILE = 64 * E * STEPS;
-    const uint32_t tile = blockIdx.x * 4 + wave_in_wg();
-    if (tile >= ntiles) return;
+__device__ __forceinline__ void rle_tile_starts(const T* __restrict__ in, uint64_t n, uint64_t base,
+                                                T (&v)[STEPS][16 / sizeof(T)], uint32_t (&mask)[STEPS]) {
+    constexpr int E = 16 / sizeof(T);
     const int l = lane_id();
-    const uint64_t base = (uint64_t)tile * TILE;
-    T v[STEPS][E];
 #pragma unroll
     for (int j = 0; j < STEPS; j++) {
         const uint64_t e0 = base + (uint64_t)j * 64 * E + (uint64_t)l * E;
@@ -53,7 +52,6 @@ rle_enc_runs_kernel(const T* __restrict__ in, uint64_t n, uint32_t* __restrict__
         }
     }
     const T before = (base > 0) ? in[base - 1] : (T)0;
-    uint32_t mask[STEPS];
 #pragma unroll
     for (int j = 0; j < STEPS; j++) {
         T prev = __shfl_up(v[j][E - 1], 1, 64);
@@ -70,42 +68,108 @@ rle_enc_runs_kernel(const T* __restrict__ in, uint64_t n, uint32_t* __restrict__
         }
         mask[j] = m;
     }
-    uint32_t run = 0;
-    uint32_t* dst = spos + base;
-#pragma unroll
-    for (int j = 0; j < STEPS; j++) {
-        const uint32_t c = __popc(mask[j]);
-        const uint32_t incl = wave_incl_scan(c);
-        uint32_t r = run + incl - c;
-        const uint64_t e0 = base + (uint64_t)j * 64 * E + (uint64_t)l * E;
-#pragma unroll
-        for (int k = 0; k < E; k++)
-            if (mask[j] & (1u << k)) dst[r++] = (uint32_t)(e0 + k);
-        run += lane63(incl);
-    }
-    if (l == 0) scnt[tile] = run;
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-rle_enc_lengths_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const uint32_t* __restrict__ pos,
-                       const uint32_t* __restrict__ d_R, uint64_t n, uint64_t* __restrict__ out_size) {
+rle_enc_count_kernel(const T* __restrict__ in, uint64_t n, uint32_t* __restrict__ cnt, uint32_t ntiles) {
+    constexpr int E = 16 / sizeof(T);
+    constexpr uint32_t TILE = 64 * E * STEPS;
+    const uint32_t tile = blockIdx.x * 4 + wave_in_wg();
+    if (tile >= ntiles) return;
+    T v[STEPS][E];
+    uint32_t mask[STEPS];
+    rle_tile_starts<T>(in, n, (uint64_t)tile * TILE, v, mask);
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < STEPS; j++) c += __popc(mask[j]);
+    c = wave_sum(c);
+    if (lane_id() == 0) cnt[tile] = c;
+}
+
+// Layout of the stream for R runs: values at 4, lengths at 4 + va.
+__device__ __forceinline__ uint64_t rle_values_bytes(uint32_t R, int eb) { return ((uint64_t)R * eb + 3) & ~3ull; }
+
+// pre[tile] = index of the tile's first run (exclusive scan of the counts),
+// *d_R = the run count.  Writes values[r] for every run of the tile and
+// lengths[r] for every run that ends inside it; the tile's last run gets its
+// start position in its length slot and first[tile] the tile's first start,
+// for rle_enc_fix_kernel.
+template <typename T>
+__global__ void __launch_bounds__(256)
+rle_enc_write_kernel(const T* __restrict__ in, uint64_t n, const uint32_t* __restrict__ pre,
+                     const uint32_t* __restrict__ d_R, uint32_t* __restrict__ first, uint8_t* __restrict__ out,
+                     uint32_t ntiles) {
+    constexpr int E = 16 / sizeof(T);
+    constexpr uint32_t TILE = 64 * E * STEPS;
+    const uint32_t tile = blockIdx.x * 4 + wave_in_wg();
+    if (tile >= ntiles) return;
+    const int l = lane_id();
+    const uint64_t base = (uint64_t)tile * TILE;
+    T v[STEPS][E];
+    uint32_t mask[STEPS];
+    rle_tile_starts<T>(in, n, base, v, mask);
     const uint32_t R = *d_R;
-    if (R == 0xFFFFFFFFu) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) *out_size = 0;
-        return;
-    }
-    const uint64_t vb = (uint64_t)R * sizeof(T);
-    const uint64_t va = (vb + 3) & ~3ull;
     T* vals = reinterpret_cast<T*>(out + 4);
-    uint32_t* lens = reinterpret_cast<uint32_t*>(out + 4 + va);
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R; r += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t s = pos[r];
-        const uint32_t e = (r + 1 < R) ? pos[r + 1] : (uint32_t)n;
-        vals[r] = in[s];
-        lens[r] = e - s;
+    uint32_t* lens = reinterpret_cast<uint32_t*>(out + 4 + rle_values_bytes(R, sizeof(T)));
+    uint32_t r = pre[tile];     // wave-uniform: index of the next run start in tile order
+    uint32_t last1 = 0;         // wave-uniform: (last start so far in this tile) + 1, 0 if none
+    uint32_t first1 = 0;        // wave-uniform: (first start of the tile) + 1
+#pragma unroll
+    for (int j = 0; j < STEPS; j++) {
+        const uint32_t m = mask[j];
+        const uint32_t c = __popc(m);
+        const uint32_t incl = wave_incl_scan(c);
+        const uint32_t e0 = (uint32_t)(base + (uint64_t)j * 64 * E + (uint64_t)l * E);
+        const uint32_t lp1 = m ? e0 + (31u - (uint32_t)__clz(m)) + 1u : 0u;  // last start in this lane + 1
+        const uint32_t pmax = wave_incl_max(lp1);
+        uint32_t prev1 = max(wave_shr1(pmax), last1);  // previous start in tile order + 1
+        uint32_t rr = r + incl - c;
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            if (m & (1u << k)) {
+                const uint32_t e = e0 + (uint32_t)k;
+                vals[rr] = v[j][k];
+                if (prev1) lens[rr - 1] = e - (prev1 - 1u);
+                prev1 = e + 1u;
+                rr++;
+            }
+        }
+        const uint64_t any = __ballot(m != 0u);
+        if (!first1 && any) {
+            const int fl = __builtin_ctzll(any);
+            first1 = (uint32_t)__builtin_amdgcn_readlane((int)(m ? e0 + (uint32_t)__builtin_ctz(m) + 1u : 0u), fl);
+        }
+        last1 = max(last1, lane63(pmax));
+        r += lane63(incl);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (l == 0) {
+        first[tile] = first1;
+        if (last1) lens[r - 1] = last1 - 1u;
+    }
+}
+
+// One thread per tile: the length of the run before the tile's first start
+// (the previous tile's last run, whose slot holds its start), and of the
+// last run; thread 0 writes the header and the values' zero padding.
+template <typename T>
+__global__ void __launch_bounds__(256)
+rle_enc_fix_kernel(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ pre,
+                   const uint32_t* __restrict__ first, const uint32_t* __restrict__ d_R, uint64_t n,
+                   uint8_t* __restrict__ out, uint64_t* __restrict__ out_size, uint32_t ntiles) {
+    const uint32_t R = *d_R;
+    const uint64_t vb = (uint64_t)R * sizeof(T);
+    const uint64_t va = rle_values_bytes(R, sizeof(T));
+    uint32_t* lens = reinterpret_cast<uint32_t*>(out + 4 + va);
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ntiles) {
+        const uint32_t c = cnt[k], p = pre[k];
+        if (c) {
+            if (p) lens[p - 1] = (first[k] - 1u) - lens[p - 1];
+            if (p + c == R) lens[R - 1] = (uint32_t)n - lens[R - 1];
+        }
+    }
+    if (k == 0) {
         *reinterpret_cast<uint32_t*>(out) = R;
         for (uint64_t b = vb; b < va; b++) out[4 + b] = 0;
         *out_size = 4 + va + 4ull * R;
@@ -279,7 +343,7 @@ rle_dec_expand_kernel(const uint8_t* __restrict__ in, uint64_t n, const uint32_t
 }
 
 struct EncWs {
-    size_t ntiles, o_cst, o_cnt, o_spos, o_pos, o_R, total;
+    size_t ntiles, o_cnt, o_pre, o_first, o_R, o_scan, total;
 };
 template <typename T>
 EncWs enc_layout(size_t n) {
@@ -287,11 +351,11 @@ EncWs enc_layout(size_t n) {
     EncWs w;
     w.ntiles = std::max<size_t>(cdiv(n, TILE), 1);
     size_t o = 0;
-    w.o_cst = o;  o = align_up(o + compact_state_bytes(w.ntiles), 256);
-    w.o_cnt = o;  o = align_up(o + 4 * w.ntiles, 256);
-    w.o_spos = o; o = align_up(o + 4 * w.ntiles * TILE, 256);
-    w.o_pos = o;  o = align_up(o + 4 * std::max<size_t>(n, 1), 256);
-    w.o_R = o;    o = align_up(o + 16, 256);
+    w.o_cnt = o;   o = align_up(o + 4 * w.ntiles, 256);
+    w.o_pre = o;   o = align_up(o + 4 * w.ntiles, 256);
+    w.o_first = o; o = align_up(o + 4 * w.ntiles, 256);
+    w.o_R = o;     o = align_up(o + 16, 256);
+    w.o_scan = o;  o = align_up(o + scan_ws_bytes(w.ntiles), 256);
     w.total = o;
     return w;
 }
@@ -311,16 +375,18 @@ int enc_impl(hipStream_t s, const void* in, size_t n, void* out, size_t cap, uin
     constexpr size_t TILE = 64 * (16 / sizeof(T)) * STEPS;
     const uint32_t ntiles = (uint32_t)cdiv(n, TILE);
     uint8_t* w = (uint8_t*)ws;
-    uint32_t* scnt = (uint32_t*)(w + L.o_cnt);
-    uint32_t* spos = (uint32_t*)(w + L.o_spos);
-    uint32_t* pos = (uint32_t*)(w + L.o_pos);
+    uint32_t* cnt = (uint32_t*)(w + L.o_cnt);
+    uint32_t* pre = (uint32_t*)(w + L.o_pre);
+    uint32_t* first = (uint32_t*)(w + L.o_first);
     uint32_t* dR = (uint32_t*)(w + L.o_R);
     const unsigned blocks = (unsigned)cdiv(ntiles, 4);
-    launch_k("rle_enc_runs_kernel", rle_enc_runs_kernel<T>, dim3(blocks), dim3(256), 0, s, (const T*)in, n, spos, scnt, ntiles);
-    CompactList L0{nullptr, spos, scnt, nullptr, pos, dR, n};
-    if (int rc = launch_compact(s, w + L.o_cst, ntiles, (uint32_t)TILE, L0, nullptr)) return rc;
-    const unsigned lb = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256), 2048));
-    launch_k("rle_enc_lengths_kernel", rle_enc_lengths_kernel<T>, dim3(lb), dim3(256), 0, s, (const T*)in, (uint8_t*)out, pos, dR, n, out_size);
+    launch_k("rle_enc_count_kernel", rle_enc_count_kernel<T>, dim3(blocks), dim3(256), 0, s, (const T*)in, n, cnt, ntiles);
+    if (int rc = scan_u32(s, cnt, pre, ntiles, nullptr, dR, w + L.o_scan)) return rc;
+    launch_k("rle_enc_write_kernel", rle_enc_write_kernel<T>, dim3(blocks), dim3(256), 0, s, (const T*)in, n,
+             (const uint32_t*)pre, (const uint32_t*)dR, first, (uint8_t*)out, ntiles);
+    launch_k("rle_enc_fix_kernel", rle_enc_fix_kernel<T>, dim3((unsigned)cdiv(ntiles, 256)), dim3(256), 0, s,
+             (const uint32_t*)cnt, (const uint32_t*)pre, (const uint32_t*)first, (const uint32_t*)dR, (uint64_t)n,
+             (uint8_t*)out, out_size, ntiles);
     FZH_CHECK_LAUNCH("rle encode");
     return FZH_OK;
 }

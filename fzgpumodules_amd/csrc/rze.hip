@@ -178,29 +178,34 @@ rze_offsets_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint32_t* 
     }
 }
 
-// Decoder, one 256-thread workgroup per chunk.  Thread t, step p produces
-// the chunk's dword at byte 1024 p + 4 t (the encoder's layout): bm3 in wave
-// 0, bm2 byte t and bm1 bytes 8t..8t+7 in thread t (repeat-elimination
-// decode by prefix popcounts: byte i = the last stored byte at or before
-// i), then the zero-eliminated data by one scan in chunk order.  Bit-shuffled
-// blocks are un-shuffled in registers (thread t owns element group t, or
-// groups t and 256 + t for 1-byte elements).
+// Decoder, one 256-thread workgroup per chunk, in the encoder's layout:
+// thread t rebuilds bm1 bytes 8t..8t+7 and produces the chunk image's bytes
+// [64 t, 64 t + 64).  bm3 is rebuilt by every wave on its own (lanes 0..31,
+// no barrier), bm2 byte t by thread t, bm1 bytes 8t..8t+7 by thread t from
+// its run of stored bytes with one repeat-expansion v_perm, and the data
+// bytes by one block scan and an expansion v_perm per 8 bytes.  Bit-shuffled
+// blocks go back through an LDS exchange to the element-group layout and are
+// un-shuffled in registers (thread t owns element group t, or groups t and
+// 256 + t for 1-byte elements).
+template <int SEW>
 struct RzeDecLds {
-    uint32_t stw[(CS + 64) / 4 + 4];  // the chunk's stored bytes, dword aligned
-    uint8_t bm1[LDS_BM1];
-    uint8_t bm2[256];
-    uint8_t bm3[32];
-    uint32_t hsum[4][3];   // per-wave sums of the bitmap stages
-    uint32_t wsum[4][16];  // per-wave sums of the 16 data steps
-    uint32_t osum[4];      // per-wave sums of the sizes before the chunk
-    uint32_t xsel[16];     // v_perm selectors spreading packed bytes over a dword, by nibble
+    uint32_t stw[(CS + 64) / 4 + 8];      // the chunk's stored bytes, dword aligned, zero tail
+    uint32_t xch[SEW != 0 ? CS / 4 : 4];  // decoded image of a shuffled block (img_swz order)
+    alignas(8) uint32_t rep[256][2];      // kRepeat8
+    alignas(8) uint32_t exp[256][2];      // kExpand8
+    uint32_t wsum[4][2];                  // per-wave sums: stored bm1 bytes, data bytes
+    uint32_t osum[4];                     // per-wave sums of the sizes before the chunk
 };
+constexpr uint32_t STW_LAST = (CS + 64) / 4 + 8 - 3;  // staged reads start at or below this word
 
-__device__ __forceinline__ uint32_t dec_block_sum(RzeDecLds& L, uint32_t v, int slot) {
-    const uint32_t ws = wave_sum(v);
-    if (lane_id() == 0) L.hsum[threadIdx.x >> 6][slot] = ws;
-    __syncthreads();
-    return L.hsum[0][slot] + L.hsum[1][slot] + L.hsum[2][slot] + L.hsum[3][slot];
+// The 8 stream bytes starting at staged byte q (any alignment).
+__device__ __forceinline__ void staged_pair(const uint32_t* stw, uint32_t q, uint32_t& lo, uint32_t& hi) {
+    uint32_t i = q >> 2;
+    i = i < STW_LAST ? i : STW_LAST;
+    const uint32_t s = q & 3u;
+    const uint32_t a = stw[i], b = stw[i + 1], c = stw[i + 2];
+    lo = __builtin_amdgcn_alignbyte(b, a, s);
+    hi = __builtin_amdgcn_alignbyte(c, b, s);
 }
 
 // stream dword starting at chunk byte q (any alignment) from the staged words
@@ -215,10 +220,10 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
                uint64_t orig_bytes, int* __restrict__ d_status, const uint32_t* __restrict__ offs, uint32_t nc,
                uint64_t nfull) {
     constexpr int SE = SEW == 0 ? 2 : SEW;
-    __shared__ RzeDecLds L;
+    __shared__ RzeDecLds<SEW> L;
     const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x);
     if (c >= nc) return;
-    const int t = threadIdx.x, w = t >> 6, l = lane_id();
+    const int t = threadIdx.x, w = (int)wave_in_wg(), l = lane_id();
     const uint32_t hdr = 8u + 4u * nc;
     bool bad = in_bytes < hdr;
     uint32_t off_c = 0;
@@ -255,12 +260,13 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
         }
         return;
     }
-    if (t < 16) {  // selector for nibble t: byte k <- packed byte (rank of k) if bit k set, else 0
-        uint32_t sel = 0x0C0C0C0Cu;
-        int c = 0;
-        for (int k = 0; k < 4; k++)
-            if (t & (1 << k)) sel = (sel & ~(0xFFu << (8 * k))) | ((uint32_t)(c++) << (8 * k));
-        L.xsel[t] = sel;
+    if (!israw) {
+        const uint2 a = *reinterpret_cast<const uint2*>(kRepeat8.v[t]);
+        const uint2 b = *reinterpret_cast<const uint2*>(kExpand8.v[t]);
+        L.rep[t][0] = a.x;
+        L.rep[t][1] = a.y;
+        L.exp[t][0] = b.x;
+        L.exp[t][1] = b.y;
     }
     // -- stage the chunk's bytes into LDS (dword aligned start)
     const uint64_t a0 = src & ~3ull;
@@ -278,13 +284,13 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
         }
         L.stw[i] = v;
     }
-    if (t < 4) L.stw[nwd + t] = 0u;  // the dword after the last one read by staged_dword
+    if (t < 8) L.stw[nwd + t] = 0u;  // the words after the last one the staged reads may touch
     __syncthreads();
     const uint8_t* cs = reinterpret_cast<const uint8_t*>(L.stw) + shift;  // chunk byte 0
     const bool unshuf = SEW != 0 && c < nfull;
-    uint32_t d[16];
     if (israw) {
         if (unshuf) {
+            uint32_t d[16];
 #pragma unroll
             for (int p = 0; p < 16; p++) d[p] = staged_dword(L.stw, 1024 * p + 4 * t + shift);
             unshuffled_store<SE>(d, dst);
@@ -308,108 +314,113 @@ rze_dec_kernel(const uint8_t* __restrict__ in, uint64_t in_bytes, uint8_t* __res
     const uint32_t bm4 = (uint32_t)cs[bm4_start] | ((uint32_t)cs[bm4_start + 1] << 8) |
                          ((uint32_t)cs[bm4_start + 2] << 16) | ((uint32_t)cs[bm4_start + 3] << 24);
     const int bm3_start = bm4_start - __popc(bm4);
-    // -- bm3: thread t < 32 owns byte t
-    if (t < 32) {
-        const uint32_t rank = __popc(bm4 & (t >= 31 ? 0xFFFFFFFFu : ((2u << t) - 1u)));
+    // -- bm3, in every wave: lane j < 32 holds byte j = the last stored byte at or before j
+    uint32_t b3 = 0;
+    if (l < 32) {
+        const uint32_t rank = __popc(bm4 & (l >= 31 ? 0xFFFFFFFFu : ((2u << l) - 1u)));
         const int p = bm3_start + (int)rank - 1;
-        uint32_t b3 = 0;
-        if (t < bm3_bytes && rank > 0) {
+        if (l < bm3_bytes && rank > 0) {
             if (p < 0) bad = true;
             else b3 = cs[p];
         }
-        L.bm3[t] = (uint8_t)b3;
     }
-    __syncthreads();
-    const uint64_t m3[4] = {*reinterpret_cast<const uint64_t*>(L.bm3), *reinterpret_cast<const uint64_t*>(L.bm3 + 8),
-                            *reinterpret_cast<const uint64_t*>(L.bm3 + 16),
-                            *reinterpret_cast<const uint64_t*>(L.bm3 + 24)};
-    const uint32_t bm3_nr = __popcll(m3[0]) + __popcll(m3[1]) + __popcll(m3[2]) + __popcll(m3[3]);
-    const int bm2_start = bm3_start - (int)bm3_nr;
-    // -- bm2: thread t owns byte t; its flag is bm3 bit t
+    const uint32_t b3_cnt = __popc(b3);
+    const uint32_t b3_incl = wave_incl_scan(b3_cnt);
+    const int bm2_start = bm3_start - (int)lane63(b3_incl);
+    // -- bm2 byte t: its flag is bm3 bit t
     uint32_t b2 = 0;
     {
-        uint32_t rank = 0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint64_t m = m3[j];
-            const int lo = 64 * j;
-            if (t >= lo + 63) rank += __popcll(m);
-            else if (t >= lo) rank += __popcll(m & ((2ull << (t - lo)) - 1ull));
-        }
+        const int j = t >> 3;  // 8 w + (l >> 3): a lane below 32 of this wave
+        const uint32_t byte_j = (uint32_t)__shfl((int)b3, j & 63, 64);
+        const uint32_t before_j = (uint32_t)__shfl((int)(b3_incl - b3_cnt), j & 63, 64);
+        const uint32_t rank = before_j + __popc(byte_j & ((2u << (t & 7)) - 1u));
         const int p = bm2_start + (int)rank - 1;
         if (t < bm2_bytes && rank > 0) {
             if (p < 0) bad = true;
             else b2 = cs[p];
         }
     }
-    const uint32_t bm2_nr = dec_block_sum(L, __popc(b2), 0);
-    const int bm1_start = bm2_start - (int)bm2_nr;
-    // -- bm1: thread t owns bytes 8t..8t+7; their flags are the bits of bm2 byte t
-    {
-        const uint32_t cnt = __popc(b2);
-        const uint32_t incl = wave_incl_scan(cnt);
-        if (l == 63) L.hsum[w][1] = incl;
-        __syncthreads();
-        uint32_t r = incl - cnt;
-        for (int j = 0; j < w; j++) r += L.hsum[j][1];
-        uint32_t lo = 0, hi = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            if (b2 & (1u << k)) r++;
-            const int p = bm1_start + (int)r - 1;
-            uint32_t bv = 0;
-            if (8 * t + k < bm1_bytes && r > 0) {
-                if (p < 0) bad = true;
-                else bv = cs[p];
-            }
-            if (k < 4) lo |= bv << (8 * k);
-            else hi |= bv << (8 * (k - 4));
-        }
-        *reinterpret_cast<uint32_t*>(L.bm1 + 8 * t) = lo;
-        *reinterpret_cast<uint32_t*>(L.bm1 + 8 * t + 4) = hi;
-        const uint32_t data_nz = dec_block_sum(L, __popc(lo) + __popc(hi), 2);  // barrier: bm1 complete
-        if ((int)data_nz > bm1_start) bad = true;
-    }
-    // -- zero elimination: dword d[p] = chunk bytes [1024 p + 4 t, +4)
-    uint32_t incl[16];
-#pragma unroll
-    for (int p = 0; p < 16; p++) {
-        const uint32_t m = (L.bm1[128 * p + (t >> 1)] >> (4 * (t & 1))) & 0xFu;
-        incl[p] = wave_incl_scan(__popc(m));
-        if (l == 63) L.wsum[w][p] = incl[p];
-    }
+    const uint32_t b2_cnt = __popc(b2);
+    const uint32_t b2_incl = wave_incl_scan(b2_cnt);
+    if (l == 63) L.wsum[w][0] = b2_incl;
     __syncthreads();
-    // lane q < 16 of each wave: first stored byte of this wave's step q
-    uint32_t qb = 0;
+    const uint32_t u0 = L.wsum[0][0], u1 = L.wsum[1][0], u2 = L.wsum[2][0], u3 = L.wsum[3][0];
+    const int bm1_start = bm2_start - (int)(u0 + u1 + u2 + u3);
+    const uint32_t pre2 = (w > 0 ? u0 : 0u) + (w > 1 ? u1 : 0u) + (w > 2 ? u2 : 0u) + b2_incl - b2_cnt;
+    // -- bm1 bytes 8t..8t+7: byte k = stored bm1 byte (pre2 + r_k - 1), r_k = set bits of b2 at
+    //    positions <= k; the bytes before b2's first set bit repeat the byte before the run
+    uint32_t Xlo, Xhi;
     {
-        uint32_t before = 0, all = 0;
-        if (l < 16) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t v = L.wsum[j][l];
-                before += j < w ? v : 0u;
-                all += v;
-            }
+        const int q = bm1_start + (int)pre2;
+        if ((b2 && q < 0) || (pre2 > 0 && q - 1 < 0)) bad = true;
+        const uint32_t qq = q < 0 ? 0u : (uint32_t)q;
+        uint32_t plo, phi;
+        staged_pair(L.stw, qq + shift, plo, phi);
+        const uint32_t prev = (pre2 > 0 && q > 0) ? (uint32_t)cs[q - 1] : 0u;
+        const uint2 rs = *reinterpret_cast<const uint2*>(L.rep[b2]);
+        Xlo = __builtin_amdgcn_perm(phi, plo, rs.x);
+        Xhi = __builtin_amdgcn_perm(phi, plo, rs.y);
+        const uint32_t tz = b2 ? (uint32_t)__builtin_ctz(b2) : 8u;  // leading bytes without a stored byte
+        const uint64_t lead = tz >= 8 ? ~0ull : ((1ull << (8 * tz)) - 1ull);
+        const uint32_t pr = prev * 0x01010101u;
+        Xlo |= pr & (uint32_t)lead;
+        Xhi |= pr & (uint32_t)(lead >> 32);
+        const int nb = bm1_bytes - 8 * t;  // bytes of bm1 this thread covers
+        if (nb < 8) {
+            const uint64_t keep = nb <= 0 ? 0ull : ((1ull << (8 * nb)) - 1ull);
+            Xlo &= (uint32_t)keep;
+            Xhi &= (uint32_t)(keep >> 32);
         }
-        qb = wave_incl_scan(all) - all + before;  // lanes 0..15 form one DPP row
     }
+    // -- data: one scan places this thread's nonzero bytes
+    const uint32_t cnt = __popc(Xlo) + __popc(Xhi);
+    const uint32_t cnt_incl = wave_incl_scan(cnt);
+    if (l == 63) L.wsum[w][1] = cnt_incl;
+    __syncthreads();
+    const uint32_t v0 = L.wsum[0][1], v1 = L.wsum[1][1], v2 = L.wsum[2][1], v3 = L.wsum[3][1];
+    if ((int)(v0 + v1 + v2 + v3) > bm1_start) bad = true;
+    uint32_t D[16];
+    {
+        uint32_t p = (w > 0 ? v0 : 0u) + (w > 1 ? v1 : 0u) + (w > 2 ? v2 : 0u) + cnt_incl - cnt;
 #pragma unroll
-    for (int p = 0; p < 16; p++) {
-        const uint32_t m = (L.bm1[128 * p + (t >> 1)] >> (4 * (t & 1))) & 0xFu;
-        const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)qb, p) + incl[p] - __popc(m);
-        // the popc(m) stored bytes at r (one unaligned dword from the staged
-        // stream, inside its slack) spread to the set positions of m; r stays
-        // below data_nz <= bm1_start unless the stream is malformed (flagged)
-        d[p] = m ? __builtin_amdgcn_perm(0u, staged_dword(L.stw, r + shift), L.xsel[m]) : 0u;
+        for (int b = 0; b < 8; b++) {
+            const uint32_t m = ((b < 4 ? Xlo : Xhi) >> (8 * (b & 3))) & 0xFFu;
+            if (m) {
+                uint32_t lo, hi;
+                staged_pair(L.stw, p + shift, lo, hi);
+                const uint2 es = *reinterpret_cast<const uint2*>(L.exp[m]);
+                D[2 * b] = __builtin_amdgcn_perm(hi, lo, es.x);
+                D[2 * b + 1] = __builtin_amdgcn_perm(hi, lo, es.y);
+            } else {
+                D[2 * b] = 0u;
+                D[2 * b + 1] = 0u;
+            }
+            p += __popc(m);
+        }
     }
     if (unshuf) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            *reinterpret_cast<uint4*>(&L.xch[img_swz(16 * t + 4 * q)]) =
+                make_uint4(D[4 * q], D[4 * q + 1], D[4 * q + 2], D[4 * q + 3]);
+        __syncthreads();
+        uint32_t d[16];
+#pragma unroll
+        for (int p = 0; p < 16; p++) d[p] = L.xch[img_swz(256 * p + t)];
         unshuffled_store<SE>(d, dst);
     } else {
+        const uint32_t b0 = 64u * t;
+        if (b0 + 64 <= o) {
 #pragma unroll
-        for (int p = 0; p < 16; p++) {
-            const uint32_t b = 1024 * p + 4 * t;
-            if (b + 4 <= o) *reinterpret_cast<uint32_t*>(dst + b) = d[p];
-            else for (uint32_t k = b; k < o; k++) dst[k] = (uint8_t)(d[p] >> (8 * (k - b)));
+            for (int q = 0; q < 4; q++)
+                st16(dst + b0 + 16 * q, make_uint4(D[4 * q], D[4 * q + 1], D[4 * q + 2], D[4 * q + 3]));
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const uint32_t b = b0 + 4 * j;
+                if (b + 4 <= o) *reinterpret_cast<uint32_t*>(dst + b) = D[j];
+                else for (uint32_t k = b; k < o; k++) dst[k] = (uint8_t)(D[j] >> (8 * (k - b)));
+            }
         }
     }
     if (bad) *d_status = 1;

@@ -301,6 +301,14 @@ int fzh_periodic_window_f32(void* stream, const float* d_base, uint64_t bx, uint
                             uint64_t x0, uint64_t y0, uint64_t z0, float* d_out, uint64_t nx, uint64_t ny,
                             uint64_t nz);
 
+/* Synthetic-data helper (no counterpart in the reference): a seeded
+ * Gaussian random field with power spectrum ~ k^expo in nx*ny*nz (x fastest;
+ * d = 3 if nz > 1, else 2 if ny > 1, else 1): octaves of smoothly blended
+ * standard-normal lattice noise at spacings 2^o, amplitude 2^(-o(expo+d)/2),
+ * the largest amplitude scaled to 1.  Deterministic for a given seed.      */
+int fzh_synth_power_law_f32(void* stream, float* d_out, uint64_t nx, uint64_t ny, uint64_t nz,
+                            float expo, uint64_t seed);
+
 /* Per-kernel device time (no counterpart in the reference, whose
  * benchmarks time whole stages).  While enabled, each launch of the chain's
  * main kernels outside stream capture is bracketed by a HIP event pair on

@@ -147,6 +147,74 @@ def test_huffman_rejects_out_of_range(gpu):
         gpu.huffman_encode(np.array([1, 2, 2000], np.uint16), 1024)
 
 
+# ---------------------------------------------------------------------------
+# Huffman decoder on streams the encoders here never write, built with the
+# numpy stream definition that pins the oracle's layout: partition lengths
+# (phf_header.sublen) other than 768 and 1024, partitions denser than the
+# decoder's LDS staging, and corrupt partition metadata.  The reference
+# decoder takes any sublen (hf_kernels.cu:258-312).
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.uint32])
+@pytest.mark.parametrize("sublen", [1, 64, 333, 768, 1024, 2048, 5000])
+def test_huffman_decode_any_sublen(gpu, orc, dtype, sublen):
+    from test_cpu_oracle import huffman_stream_def
+    rng = np.random.default_rng(sublen)
+    bklen = 256 if dtype == np.uint8 else 1024
+    n = 3001 if sublen < 100 else 40_011
+    v = np.minimum(np.abs(rng.laplace(0, 8, n)), bklen - 1).astype(dtype)
+    f = orc.histogram(v, bklen)
+    bk4, rev = orc.huffman_book(f, np.dtype(dtype).itemsize)
+    s = huffman_stream_def(v, bk4, rev, bklen, sublen)
+    np.testing.assert_array_equal(orc.huffman_decode(s, dtype, v.size), v)
+    np.testing.assert_array_equal(gpu.huffman_decode(s, dtype, v.size), v)
+
+
+def _fib_book(orc):
+    """Fibonacci frequencies on 24 of 1024 symbols: codewords of 1 to 23 bits."""
+    fib = [1, 1]
+    while len(fib) < 24:
+        fib.append(fib[-1] + fib[-2])
+    f = np.zeros(1024, np.uint32)
+    f[:24] = fib[::-1]
+    bk4, rev = orc.huffman_book(f, 2)
+    lens = (bk4[:24] >> 27).astype(np.int64)
+    return bk4, rev, lens
+
+
+def test_huffman_decode_dense_partitions(gpu, orc):
+    """Partitions of mostly 20+-bit codewords (over 448 cells per 768-symbol
+    partition, past the decoder's LDS staging) next to ordinary ones."""
+    from test_cpu_oracle import huffman_stream_def
+    bk4, rev, lens = _fib_book(orc)
+    rng = np.random.default_rng(5)
+    rare = np.nonzero(lens >= 20)[0]
+    common = np.nonzero(lens <= 6)[0]
+    v = np.concatenate([rng.choice(common, 768 * 3), rng.choice(rare, 768 * 4 + 100),
+                        rng.choice(common, 500)]).astype(np.uint16)
+    s = huffman_stream_def(v, bk4, rev, 1024, 768)
+    ent = s[40:64].view(np.uint32)
+    nbit = s[ent[2]:ent[3]].view(np.uint32)
+    assert (nbit[3:7] > 448 * 32).all() and (nbit[:3] < 448 * 32).all()
+    np.testing.assert_array_equal(orc.huffman_decode(s, np.uint16, v.size), v)
+    np.testing.assert_array_equal(gpu.huffman_decode(s, np.uint16, v.size), v)
+
+
+@pytest.mark.parametrize("part", [0, 3, 389])
+def test_huffman_decode_rejects_wrong_bit_count(gpu, orc, part):
+    """A partition whose par_nbit is one more than its codewords' bits does
+    not end on a codeword boundary with its symbol count: status set."""
+    v = _huffman_inputs()["gauss"]
+    s = orc.huffman_encode(v, 1024)
+    ent = s[40:64].view(np.uint32)
+    nb = s[ent[2]:ent[3]].view(np.uint32)
+    assert nb.size == 391
+    bad = s.copy()
+    bad[ent[2]:ent[3]].view(np.uint32)[part] = int(nb[part]) + (1 if nb[part] % 32 else -1)
+    with pytest.raises(ValueError):
+        gpu.huffman_decode(bad, np.uint16, v.size)
+    np.testing.assert_array_equal(gpu.huffman_decode(s, np.uint16, v.size), v)
+
+
 @pytest.mark.parametrize("zigzag", [False, True])
 @pytest.mark.parametrize("code_bytes,radius", [(2, 32768), (1, 64), (4, 1 << 20)])
 def test_quantizer_matches_oracle(gpu, orc, zigzag, code_bytes, radius):
