@@ -38,13 +38,13 @@ USER_EB = 1e-3
 CAP = 0.10
 # HBM bytes per launch of the chain's main kernels on this workload, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the
-# same counters read for a plain copy of known size (FETCH x 1.998, WRITE x
-# 1.000); profiles/cloud2d_1800x3600_chain_hbm_traffic.md holds the counter
-# runs, their commands and the scaling.
+# same counters read for a plain copy of known size (FETCH x 1.9996, WRITE x
+# 1.0000); profiles/cloud2d_sinusoid1d_turbulence3d_hbm_traffic.md holds the
+# counter runs, their commands and the scaling.
 MEASURED_TRAFFIC = {
-    "minmax_partial_kernel": 25951380,
-    "lz2d_fwd_kernel": 39264659,
-    "rze_enc_kernel": 16288073,
+    "minmax_partial_kernel": 25974889,
+    "lz2d_fwd_kernel": 39296664,
+    "rze_enc_kernel": 16230164,
 }
 # Reference CLI example, benchmark mean of 10, host time (docs/cli.md:66-70)
 REF_COMPRESS_HOST_GBS = 18.85
