@@ -64,8 +64,8 @@ def parse():
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--batch-fields", type=int, default=4096,
                     help="fields of the batched workload (0 skips it)")
-    ap.add_argument("--selector-train", type=int, default=8192,
-                    help="training chunks of the learned-selector workload (0 skips it)")
+    ap.add_argument("--selector-train", type=int, default=1000000,
+                    help="training chunks of the learned-selector workload, generated on the device (0 skips it)")
     ap.add_argument("--no-cfg1", dest="cfg1", action="store_false", help="skip the 2^24 sinusoid -> RLE workload")
     ap.add_argument("--no-cfg3", dest="cfg3", action="store_false", help="skip the 512^3 turbulence -> Huffman workload")
     return ap.parse_args()
@@ -538,45 +538,54 @@ def selector_workload(args):
     """Learned chain selection on 256 KiB chunks of five regimes (smooth
     k^-4 GRF, white noise, 80% zeros, piecewise constant, Poisson 20), 2D
     256 x 256 and 3D 16 x 64 x 64, each chunk bounded by 1e-3 of its parent
-    field's max |x|: exhaustive labels -> device training -> held-out regret
-    and selection throughput (learned: features + MLP; exhaustive: the
-    labeller, i.e. every chain's exact size)."""
+    field's max |x|, all generated on the device from seeds
+    (fzs_synth_chunks): exhaustive labels (the labeller: every chain's exact
+    compressed size) -> device training -> held-out regret, selection
+    throughput, and whole-field compress throughput of the selecting
+    compressor (learned; exhaustive, i.e. labeller then compress; every
+    chunk with every chain; one fixed chain)."""
     from fzgpumodules_amd import fields
-    from fzgpumodules_amd._fzh import DeviceBuffer, Event, call
+    from fzgpumodules_amd._fzh import DeviceBuffer, call
     from fzgpumodules_amd import selector as S
 
     t = time.time()
-    per_dim = max(args.selector_train // 32, 5)  # 16 chunks per parent field, two dimensionalities
+    n_train = max(args.selector_train // 80 * 80, 80)  # whole parent fields of every regime, both dims
+    n_test = max(min(n_train // 4, 8192) // 80 * 80, 80)
+    batch = 8192
+    bases = {nd: S.synth_base(nd, fields.BASE_SEED + 50 + nd) for nd in (2, 3)}
+    shapes = {2: (256, 256), 3: (16, 64, 64)}
 
-    def make(nfields, seed):
-        return fields.selector_set(nfields, seed, fields_shape_2d=(1024, 1024), fields_shape_3d=(64, 128, 128))
-
-    def featurize(data):
-        feats, sizes, dev = [], [], []
+    def labelled(n_total, seed, keep=False):
+        feats, sizes, kept = [], [], []
         for nd in (2, 3):
-            ch, eb, _ = data[nd]
-            d_c, d_eb = DeviceBuffer.from_array(ch), DeviceBuffer.from_array(eb)
-            f = S.features(d_c.ptr, len(ch), ch.shape[1:], d_eb.ptr)
-            feats.append(f.download(np.float32, 32 * len(ch)).reshape(-1, 32))
-            f.free()
-            sizes.append(S.label_sizes(d_c.ptr, len(ch), ch.shape[1:], d_eb.ptr))
-            dev.append((d_c, d_eb, len(ch), ch.shape[1:]))
-        return np.concatenate(feats), np.concatenate(sizes), dev
+            n_nd = n_total // 2
+            for k, off in enumerate(range(0, n_nd, batch)):
+                nb = min(batch, n_nd - off)
+                d_c, d_eb = S.synth_chunks(nd, nb, seed * 1000003 + 7919 * nd + k, bases[nd])
+                f = S.features(d_c.ptr, nb, shapes[nd], d_eb.ptr)
+                feats.append(f.download(np.float32, 32 * nb).reshape(-1, 32))
+                f.free()
+                sizes.append(S.label_sizes(d_c.ptr, nb, shapes[nd], d_eb.ptr))
+                if keep:
+                    kept.append((d_c, d_eb, nb, nd))
+                else:
+                    d_c.free()
+                    d_eb.free()
+                if k % 16 == 15:
+                    log(f"[bench] selector: {nd}D, {off + nb} of {n_nd} chunks generated and labelled")
+        return np.concatenate(feats), np.concatenate(sizes), kept
 
-    train = make(per_dim, fields.BASE_SEED + 5)
-    f_tr, s_tr, dev_tr = featurize(train)
-    for d_c, d_eb, _, _ in dev_tr:
-        d_c.free()
-        d_eb.free()
-    del train
-    test = make(max(per_dim // 4, 5), fields.BASE_SEED + 500)
-    f_te, s_te, dev_te = featurize(test)
-    log(f"[bench] selector: {len(f_tr)} training / {len(f_te)} held-out chunks generated and labelled in "
-        f"{time.time() - t:.1f}s")
+    f_tr, s_tr, _ = labelled(n_train, fields.BASE_SEED + 5)
+    gen_s = time.time() - t
+    f_te, s_te, dev_te = labelled(n_test, fields.BASE_SEED + 500, keep=True)
+    log(f"[bench] selector: {len(f_tr)} training / {len(f_te)} held-out chunks generated on the device and "
+        f"labelled in {time.time() - t:.1f}s")
     sel = S.Selector(seed=11)
     t = time.time()
-    hist = sel.fit(f_tr, S.regret_costs(s_tr), epochs=40, batch=256, lr=3e-3, tau=0.02)
+    hist = sel.fit(f_tr, S.regret_costs(s_tr), epochs=20, batch=4096, lr=3e-3, tau=0.02)
+    call("fzh_synchronize")
     train_s = time.time() - t
+    log(f"[bench] selector: trained in {train_s:.1f}s, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
     d_f = DeviceBuffer.from_array(f_te)
     choice = sel.select(d_f.ptr, len(f_te))
     best = s_te.min(axis=1).astype(np.float64)
@@ -584,7 +593,6 @@ def selector_workload(args):
     in_bytes = 262144 * len(f_te)
     single = [float(s_te[:, k].astype(np.float64).sum()) for k in range(4)]
     kfix = int(np.argmin([s_tr[:, k].astype(np.float64).sum() for k in range(4)]))  # best chain on the training set
-    # selection throughput on the device-resident held-out chunks
     st = C.c_void_p()
     call("fzh_stream_create", C.byref(st))
 
@@ -598,28 +606,56 @@ def selector_workload(args):
         return (time.perf_counter() - t0) / reps
 
     def learned_select():
-        for d_c, d_eb, n, shape in dev_te:
-            f = S.features(d_c.ptr, n, shape, d_eb.ptr, stream=st.value)
+        for d_c, d_eb, n, nd in dev_te:
+            f = S.features(d_c.ptr, n, shapes[nd], d_eb.ptr, stream=st.value)
             sel.select(f.ptr, n, stream=st.value)
             f.free()
 
     def exhaustive_select():
-        for d_c, d_eb, n, shape in dev_te:
-            S.label_sizes(d_c.ptr, n, shape, d_eb.ptr, stream=st.value)
+        for d_c, d_eb, n, nd in dev_te:
+            S.label_sizes(d_c.ptr, n, shapes[nd], d_eb.ptr, stream=st.value)
 
     t_learn = timed(learned_select)
     t_exh = timed(exhaustive_select)
+    # whole-field compress: the held-out chunks of each dimensionality as one
+    # field (chunks stacked along the slowest axis), the selecting compressor
+    # in each mode; the bound is the field's REL 1e-3
+    params, norm = sel.get_params(), sel.get_norm()
+    comp = {"learned": 0.0, "exhaustive_labeller": 0.0, "fixed": [0.0] * 4}
+    comp_bytes = {"learned": 0, "exhaustive_labeller": 0, "fixed": [0] * 4}
+    for nd in (2, 3):
+        sc = S.SelectingCompressor(shapes[nd])
+        sc.set_model(params, norm)
+        parts = [(d_c, n) for d_c, _, n, d in dev_te if d == nd]
+        for d_c, n in parts:
+            shape = (256 * n, 256) if nd == 2 else (16 * n, 64, 64)
+            for mode, key in ((S.LEARNED, "learned"), (S.EXHAUSTIVE, "exhaustive_labeller")):
+                sc.compress(d_c.ptr, shape, 1e-3, mode)  # warm: chains built, buffers sized
+                blob = sc.compress(d_c.ptr, shape, 1e-3, mode)
+                comp[key] += sc.timing()["total_ms"]
+                comp_bytes[key] += blob.size
+            for k in range(4):
+                sc.compress(d_c.ptr, shape, 1e-3, S.FIXED, k)
+                blob = sc.compress(d_c.ptr, shape, 1e-3, S.FIXED, k)
+                comp["fixed"][k] += sc.timing()["total_ms"]
+                comp_bytes["fixed"][k] += blob.size
+        sc.close()
     call("fzh_stream_destroy", st.value)
     for d_c, d_eb, _, _ in dev_te:
         d_c.free()
         d_eb.free()
+    for b in bases.values():
+        b.free()
     d_f.free()
     sel.close()
+    gbs = lambda ms: round(in_bytes / (ms * 1e-3) / 1e9, 2)
+    all_chains_ms = sum(comp["fixed"])
     res = {
-        "workload": "256 KiB chunks (2D 256x256, 3D 16x64x64) of 5 regimes, eb = 1e-3 max|parent|; chains "
-                    + ", ".join(S.CHAINS),
+        "workload": "256 KiB chunks (2D 256x256, 3D 16x64x64) of 5 regimes generated on the device, eb = 1e-3 "
+                    "max|parent|; chains " + ", ".join(S.CHAINS),
         "train_chunks": int(len(f_tr)),
         "heldout_chunks": int(len(f_te)),
+        "generate_and_label_seconds": round(gen_s, 2),
         "train_seconds": round(train_s, 2),
         "train_loss_first_last": [round(hist[0], 4), round(hist[-1], 4)],
         "ratio_exhaustive": round(in_bytes / best.sum(), 3),
@@ -632,6 +668,13 @@ def selector_workload(args):
         "select_gbs_learned": round(in_bytes / t_learn / 1e9, 2),
         "select_gbs_exhaustive": round(in_bytes / t_exh / 1e9, 2),
         "select_speedup": round(t_exh / t_learn, 2),
+        "compress_gbs_learned": gbs(comp["learned"]),
+        "compress_gbs_exhaustive_labeller": gbs(comp["exhaustive_labeller"]),
+        "compress_gbs_exhaustive_all_chains": gbs(all_chains_ms),
+        "compress_gbs_single_chain": gbs(comp["fixed"][kfix]),
+        "compress_speedup_vs_all_chains": round(all_chains_ms / comp["learned"], 2),
+        "compress_ratio_learned": round(in_bytes / comp_bytes["learned"], 3),
+        "compress_ratio_single_chain": round(in_bytes / comp_bytes["fixed"][kfix], 3),
     }
     log(f"[bench] selector: {res}")
     return res

@@ -133,6 +133,7 @@ SIGNATURES = {
     "fzs_label_ws_bytes": (_Z, [_Z, _U32, _U32, _U32]),
     "fzs_label_sizes": (_I, [_P, _P, _Z, _U32, _U32, _U32, _P, _P, _P, _Z, _I]),
     "fzs_select": (_I, [_P, _P, _Z, _P]),
+    "fzs_synth_chunks": (_I, [_P, _I, C.c_uint64, C.c_uint64, _P, _U32, _U32, _U32, _P, _P, _P]),
     "fzs_gather_chunks": (_I, [_P, _P, C.c_uint64, C.c_uint64, C.c_uint64, _U32, _U32, _U32, _P]),
     "fzs_scatter_chunks": (_I, [_P, _P, C.c_uint64, C.c_uint64, C.c_uint64, _U32, _U32, _U32, _P]),
     "fzs_create": (_I, [_U32, _U32, _U32, C.POINTER(_P)]),

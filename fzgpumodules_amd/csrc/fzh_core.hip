@@ -1,5 +1,6 @@
 // fzh_core.hip — library plumbing: error strings, workspace init, device
 // helpers used by the bindings, and the multi-output gather kernel.
+#include "fzh_synth.h"
 #include "fzh_internal.h"
 #include "fzh_scan.h"
 #include "fzh_concat.h"
@@ -219,67 +220,12 @@ periodic_window_kernel(const float* __restrict__ base, uint64_t bx, uint64_t by,
     }
 }
 
-// Seeded Gaussian random field with spectrum ~ k^expo (synthetic input
-// for benchmarks): a sum of octaves o = 0 .. O-1 of lattice noise -- iid
-// standard-normal values on a lattice of spacing 2^o, blended between the
-// 2^d surrounding lattice points with the quintic weight
-// t^3 (t (6t - 15) + 10) -- with octave amplitude 2^(-o (expo + d) / 2)
-// (a power law P(k) ~ k^expo puts variance ~ k^(expo + d) in the octave of
-// wavenumbers around 2^-o).  The value is a linear combination of normal
-// variables, so the field is Gaussian; amplitudes are scaled so the largest
-// is 1.
-__device__ __forceinline__ uint64_t mix64(uint64_t z) {
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-__device__ __forceinline__ float lattice_normal(uint64_t seed, uint32_t o, uint64_t i, uint64_t j, uint64_t k) {
-    const uint64_t h = mix64(seed ^ mix64(((uint64_t)o << 56) ^ (i * 0xD1B54A32D192ED03ull) ^
-                                          (j * 0xABC98388FB8FAC03ull) ^ (k * 0x8CB92BA72F3D8DD7ull)));
-    const float u1 = ((float)(uint32_t)(h >> 40) + 0.5f) * (1.0f / 16777216.0f);  // (0, 1)
-    const float u2 = (float)(uint32_t)(h & 0xFFFFFFu) * (1.0f / 16777216.0f);     // [0, 1)
-    return sqrtf(-2.0f * __logf(u1)) * __cosf(6.28318531f * u2);
-}
-
-__device__ __forceinline__ float quintic(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
-
 __global__ void __launch_bounds__(256)
 synth_power_law_kernel(float* __restrict__ out, uint64_t nx, uint64_t ny, uint64_t nz, uint64_t total, int d,
                        int octaves, float ea, uint64_t seed) {
     for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (uint64_t)gridDim.x * 256) {
         const uint64_t x = e % nx, y = (e / nx) % ny, z = e / (nx * ny);
-        float acc = 0.0f;
-        for (int o = 0; o < octaves; o++) {
-            // 2^(ea o), the largest of them (o = 0 or o = octaves - 1) scaled to 1
-            const float amp = exp2f(ea * (float)(ea >= 0.0f ? o - (octaves - 1) : o));
-            const uint64_t ix = x >> o, iy = y >> o, iz = z >> o;
-            const float inv = 1.0f / (float)(1u << o);
-            const float tx = quintic((float)(x & ((1u << o) - 1u)) * inv);
-            const float ty = quintic((float)(y & ((1u << o) - 1u)) * inv);
-            const float tz = quintic((float)(z & ((1u << o) - 1u)) * inv);
-            float v;
-            if (d == 1) {
-                v = lattice_normal(seed, o, ix, 0, 0) * (1.0f - tx) + lattice_normal(seed, o, ix + 1, 0, 0) * tx;
-            } else if (d == 2) {
-                const float a = lattice_normal(seed, o, ix, iy, 0) * (1.0f - tx) + lattice_normal(seed, o, ix + 1, iy, 0) * tx;
-                const float b = lattice_normal(seed, o, ix, iy + 1, 0) * (1.0f - tx) + lattice_normal(seed, o, ix + 1, iy + 1, 0) * tx;
-                v = a * (1.0f - ty) + b * ty;
-            } else {
-                float c[2];
-                for (int dz = 0; dz < 2; dz++) {
-                    const float a = lattice_normal(seed, o, ix, iy, iz + dz) * (1.0f - tx) +
-                                    lattice_normal(seed, o, ix + 1, iy, iz + dz) * tx;
-                    const float b = lattice_normal(seed, o, ix, iy + 1, iz + dz) * (1.0f - tx) +
-                                    lattice_normal(seed, o, ix + 1, iy + 1, iz + dz) * tx;
-                    c[dz] = a * (1.0f - ty) + b * ty;
-                }
-                v = c[0] * (1.0f - tz) + c[1] * tz;
-            }
-            acc += amp * v;
-        }
-        out[e] = acc;
+        out[e] = power_law_value(x, y, z, d, octaves, ea, seed);
     }
 }
 
@@ -352,11 +298,10 @@ int fzh_synth_power_law_f32(void* stream, float* d_out, uint64_t nx, uint64_t ny
     const uint64_t total = nx * ny * nz;
     if (total == 0) return FZH_OK;
     const int d = nz > 1 ? 3 : (ny > 1 ? 2 : 1);
-    uint64_t m = std::max(nx, std::max(ny, nz)), octaves = 1;
-    while ((1ull << octaves) < m && octaves < 24) octaves++;
+    const int octaves = power_law_octaves(std::max(nx, std::max(ny, nz)));
     const float ea = -0.5f * (expo + (float)d);  // log2 of the amplitude ratio between octaves o+1 and o
     const unsigned blocks = (unsigned)std::min<uint64_t>(cdiv(total, 256), 65536);
-    synth_power_law_kernel<<<blocks, 256, 0, S(stream)>>>(d_out, nx, ny, nz, total, d, (int)octaves, ea, seed);
+    synth_power_law_kernel<<<blocks, 256, 0, S(stream)>>>(d_out, nx, ny, nz, total, d, octaves, ea, seed);
     FZH_CHECK_LAUNCH("synth_power_law_kernel");
     return FZH_OK;
 }

@@ -26,6 +26,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fz/pipeline.h"
@@ -211,32 +212,60 @@ class ChainSelectCompressor {
             }
         }
         const double t1 = now_ms();
-        // chains in use, built with this field's bound
+        // compress every chunk with its chain: kWorkers host threads, each
+        // with its own stream and its own pipeline per chain, take the chunks
+        // in turn (each fz::Pipeline::compress ends with a host wait, so one
+        // thread would leave the GPU idle between chunks)
         uint32_t mask = 0;
         for (uint8_t k : choice_) mask |= 1u << k;
-        for (int k = 0; k < FZS_NCHAINS; k++)
-            if ((mask >> k) & 1u)
-                if (!chain_[k].p || chain_eb_[k] != abs_eb) {
-                    chain_[k] = make_chain(k, abs_eb, cx_, cy_, cz_);
-                    chain_eb_[k] = abs_eb;
-                }
-        // compress every chunk with its chain
-        std::vector<std::vector<uint8_t>> blobs(n);
-        for (uint64_t c = 0; c < n; c++) {
-            void* d_out = nullptr;
-            size_t bytes = 0;
-            chain_[choice_[c]].p->compress((const float*)chunks_.p + c * E, 4 * E, &d_out, &bytes, s);
-            blobs[c].resize(bytes);
-            hipck(hipMemcpyAsync(blobs[c].data(), d_out, bytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-            hipck(hipStreamSynchronize(s), "hipStreamSynchronize");
+        hipck(hipStreamSynchronize(s), "hipStreamSynchronize");  // chunks_ and the choices are ready
+        const int W = (int)std::max<uint64_t>(1, std::min<uint64_t>(kWorkers, n));
+        while ((int)workers_.size() < W) {
+            workers_.push_back(std::make_unique<Worker>());
+            hipck(hipStreamCreate(&workers_.back()->s), "hipStreamCreate");
         }
+        for (int w = 0; w < W; w++)
+            for (int k = 0; k < FZS_NCHAINS; k++)
+                if ((mask >> k) & 1u) {
+                    Worker& wk = *workers_[w];
+                    if (!wk.chain[k].p || wk.chain_eb[k] != abs_eb) {
+                        wk.chain[k] = make_chain(k, abs_eb, cx_, cy_, cz_);
+                        wk.chain_eb[k] = abs_eb;
+                    }
+                }
+        std::vector<std::vector<uint8_t>> blobs(n);
+        std::vector<std::exception_ptr> errs(W);
+        std::vector<std::thread> th;
+        for (int w = 0; w < W; w++)
+            th.emplace_back([&, w] {
+                try {
+                    Worker& wk = *workers_[w];
+                    for (uint64_t c = (uint64_t)w; c < n; c += (uint64_t)W) {
+                        void* d_out = nullptr;
+                        size_t bytes = 0;
+                        wk.chain[choice_[c]].p->compress((const float*)chunks_.p + c * E, 4 * E, &d_out, &bytes, wk.s);
+                        blobs[c].resize(bytes);
+                        hipck(hipMemcpyAsync(blobs[c].data(), d_out, bytes, hipMemcpyDeviceToHost, wk.s),
+                              "hipMemcpyAsync");
+                        hipck(hipStreamSynchronize(wk.s), "hipStreamSynchronize");
+                    }
+                } catch (...) {
+                    errs[w] = std::current_exception();
+                }
+            });
+        for (auto& t : th) t.join();
+        for (auto& e : errs)
+            if (e) std::rethrow_exception(e);
         const double t2 = now_ms();
-        // container
+        // container: chain k's stages are described by the pipeline that
+        // compressed the last chunk choosing k (chunk c went to worker c % W)
+        std::vector<uint64_t> last(FZS_NCHAINS, 0);
+        for (uint64_t c = 0; c < n; c++) last[choice_[c]] = c;
         std::vector<std::vector<FZMStageInfo>> recs(FZS_NCHAINS);
         size_t hdr = 64 + 4;
         for (int k = 0; k < FZS_NCHAINS; k++)
             if ((mask >> k) & 1u) {
-                recs[k] = chain_[k].p->buildHeader().stages;
+                recs[k] = workers_[last[k] % (uint64_t)W]->chain[k].p->buildHeader().stages;
                 hdr += 4 + recs[k].size() * sizeof(FZMStageInfo);
             }
         const size_t o_choice = hdr;
@@ -361,13 +390,22 @@ class ChainSelectCompressor {
     }
 
   private:
+    static constexpr int kWorkers = 8;
+    struct Worker {
+        hipStream_t s = nullptr;
+        Chain chain[FZS_NCHAINS];
+        float chain_eb[FZS_NCHAINS] = {0, 0, 0, 0};
+        ~Worker() {
+            for (auto& c : chain) c.p.reset();
+            if (s) (void)hipStreamDestroy(s);
+        }
+    };
+    std::vector<std::unique_ptr<Worker>> workers_;
     uint32_t cx_, cy_, cz_;
     bool has_model_ = false;
     DevBuf params_, norm_, chunks_, eb_, feat_, logits_, dchoice_, label_ws_, vb_ws_, blob_;
     std::vector<uint8_t> choice_, out_;
     std::vector<uint64_t> sizes_;
-    Chain chain_[FZS_NCHAINS];
-    float chain_eb_[FZS_NCHAINS] = {0, 0, 0, 0};
     float select_ms_ = 0, compress_ms_ = 0, total_ms_ = 0;
 };
 

@@ -29,6 +29,7 @@
 #include <cmath>
 
 #include "fzh_internal.h"
+#include "fzh_synth.h"
 #include "fzs.h"
 
 namespace fzh {
@@ -361,38 +362,55 @@ mlp_backward_kernel(const float* __restrict__ p, const float* __restrict__ norm,
     r.loss[i] = loss;
 }
 
-// One thread per parameter: mean over the batch, samples in order.
+// Gradient: the batch is cut into segments of GRAD_SEG samples; thread
+// (parameter q, segment g) sums its segment's samples in order
+// (mlp_grad_part_kernel), then one thread per parameter adds the segment
+// sums in order and divides by n (mlp_grad_final_kernel) -- a fixed order,
+// so the result is deterministic, and the work spreads over the GPU for
+// large batches.  Index NP carries the loss.
+constexpr uint32_t GRAD_SEG = 256;
+
 __global__ void __launch_bounds__(256)
-mlp_grad_kernel(Rec r, uint64_t n, float* __restrict__ grad, float* __restrict__ loss_out) {
+mlp_grad_part_kernel(Rec r, uint64_t n, float* __restrict__ part) {
     const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q == NP) {
-        float s = 0.0f;
-        for (uint64_t i = 0; i < n; i++) s = s + r.loss[i];
-        *loss_out = s / (float)n;
-        return;
-    }
     if (q > NP) return;
+    const uint64_t i0 = (uint64_t)blockIdx.y * GRAD_SEG, i1 = min(n, i0 + GRAD_SEG);
     float acc = 0.0f;
-    if (q < OB1) {
+    if (q == NP) {
+        for (uint64_t i = i0; i < i1; i++) acc = acc + r.loss[i];
+    } else if (q < OB1) {
         const int o = q / NF, k = q % NF;
-        for (uint64_t i = 0; i < n; i++) acc = acc + r.d1b[i * H1 + o] * r.xb[i * NF + k];
+        for (uint64_t i = i0; i < i1; i++) acc = acc + r.d1b[i * H1 + o] * r.xb[i * NF + k];
     } else if (q < OW2) {
         const int o = q - OB1;
-        for (uint64_t i = 0; i < n; i++) acc = acc + r.d1b[i * H1 + o];
+        for (uint64_t i = i0; i < i1; i++) acc = acc + r.d1b[i * H1 + o];
     } else if (q < OB2) {
         const int o = (q - OW2) / H1, k = (q - OW2) % H1;
-        for (uint64_t i = 0; i < n; i++) acc = acc + r.d2b[i * H2 + o] * r.h1b[i * H1 + k];
+        for (uint64_t i = i0; i < i1; i++) acc = acc + r.d2b[i * H2 + o] * r.h1b[i * H1 + k];
     } else if (q < OW3) {
         const int o = q - OB2;
-        for (uint64_t i = 0; i < n; i++) acc = acc + r.d2b[i * H2 + o];
+        for (uint64_t i = i0; i < i1; i++) acc = acc + r.d2b[i * H2 + o];
     } else if (q < OB3) {
         const int o = (q - OW3) / H2, k = (q - OW3) % H2;
-        for (uint64_t i = 0; i < n; i++) acc = acc + r.d3b[i * NC + o] * r.h2b[i * H2 + k];
+        for (uint64_t i = i0; i < i1; i++) acc = acc + r.d3b[i * NC + o] * r.h2b[i * H2 + k];
     } else {
         const int o = q - OB3;
-        for (uint64_t i = 0; i < n; i++) acc = acc + r.d3b[i * NC + o];
+        for (uint64_t i = i0; i < i1; i++) acc = acc + r.d3b[i * NC + o];
     }
-    grad[q] = acc / (float)n;
+    part[(uint64_t)blockIdx.y * (NP + 1) + q] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+mlp_grad_final_kernel(const float* __restrict__ part, uint32_t nseg, uint64_t n, float* __restrict__ grad,
+                      float* __restrict__ loss_out) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q > NP) return;
+    float acc = 0.0f;
+    for (uint32_t g = 0; g < nseg; g++) acc = acc + part[(uint64_t)g * (NP + 1) + q];
+    if (q == NP)
+        *loss_out = acc / (float)n;
+    else
+        grad[q] = acc / (float)n;
 }
 
 __global__ void __launch_bounds__(256)
@@ -442,6 +460,143 @@ __global__ void __launch_bounds__(256) select_kernel(const float* __restrict__ l
     choice[i] = (uint8_t)best;
 }
 
+
+// ---------------------------------------------------------------------------
+// Selector training/evaluation chunks generated on the device.  Chunk c
+// belongs to parent field f = c / CPP (CPP = 4 chunks of 256 x 256 in a
+// 512 x 512 field, or 8 chunks of 16 x 64 x 64 in a 32 x 128 x 128 field, in
+// row-major order of the chunk grid) of regime f % 5, seeded by (seed, nd, f):
+//   0  Gaussian random field, power ~ k^-4: a window of the caller's base
+//      field (fzh_synth_power_law_f32 output) at a seeded offset, wrapping
+//      periodically
+//   1  white noise, N(0, 1)
+//   2  80% exact zeros, the rest N(0, 1)
+//   3  piecewise constant: 48 random axis-aligned boxes painted in order,
+//      each a N(0, 1) value, on zeros
+//   4  Poisson(20) integers (inverse CDF)
+// and its bound is float(1e-3f * max |parent|) (1e-3 for an all-zero
+// parent).  Deterministic in (seed, nd, chunk index).
+// ---------------------------------------------------------------------------
+constexpr int SYN_BOXES = 48;
+constexpr int POIS_K = 64;
+
+struct PoissonCdf {
+    double c[POIS_K];
+};
+constexpr PoissonCdf make_poisson_cdf() {
+    PoissonCdf t{};
+    double p = 2.061153622438557828e-09;  // e^-20
+    double acc = 0.0;
+    for (int k = 0; k < POIS_K; k++) {
+        acc += p;
+        t.c[k] = acc;
+        p = p * 20.0 / (double)(k + 1);
+    }
+    return t;
+}
+__constant__ PoissonCdf kPoisCdf = make_poisson_cdf();
+
+struct SynGeom {
+    uint32_t cx, cy, cz, gx, gy, cpp, px, py, pz;
+};
+
+__device__ __forceinline__ float wave_max_f(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__host__ __device__ inline SynGeom syn_geom(int nd) {
+    // chunk (cx, cy, cz), chunk grid of a parent (gx, gy, gz = cpp / (gx gy))
+    if (nd == 2) return SynGeom{256, 256, 1, 2, 2, 4, 512, 512, 1};
+    return SynGeom{64, 64, 16, 2, 2, 8, 128, 128, 32};
+}
+
+__device__ __forceinline__ uint64_t parent_seed(uint64_t seed, int nd, uint64_t f) {
+    return mix64(seed ^ mix64(0x5E1EC7ull * (uint64_t)nd + f * 0x9E3779B97F4A7C15ull));
+}
+
+__global__ void __launch_bounds__(256)
+synth_chunks_kernel(float* __restrict__ out, float* __restrict__ cmax, uint8_t* __restrict__ regime, int nd,
+                    uint64_t seed, const float* __restrict__ base, uint32_t bsx, uint32_t bsy, uint32_t bsz) {
+    const SynGeom g = syn_geom(nd);
+    const uint64_t c = blockIdx.x;
+    const uint64_t f = c / g.cpp;
+    const uint32_t q = (uint32_t)(c % g.cpp);
+    const uint32_t bx = q % g.gx, by = (q / g.gx) % g.gy, bz = q / (g.gx * g.gy);
+    const int r = (int)(f % 5);
+    const uint64_t ps = parent_seed(seed, nd, f);
+    __shared__ int s_box[SYN_BOXES][6];
+    __shared__ float s_val[SYN_BOXES];
+    __shared__ float s_max[4];
+    if (r == 3 && threadIdx.x < SYN_BOXES) {
+        const uint32_t b = threadIdx.x;
+        const uint32_t dims[3] = {g.px, g.py, g.pz};
+        for (int a = 0; a < 3; a++) {
+            const uint64_t h = mix64(ps ^ mix64(0xB0C5ull + 8 * b + a));
+            uint32_t lo = (uint32_t)((h & 0xFFFFFFFFull) % (dims[a] + 1)), hi = (uint32_t)((h >> 32) % (dims[a] + 1));
+            if (lo > hi) { const uint32_t t = lo; lo = hi; hi = t; }
+            if (hi == lo) hi = lo + 1;
+            s_box[b][2 * a] = (int)lo;
+            s_box[b][2 * a + 1] = (int)hi;
+        }
+        s_val[b] = hash_normal(mix64(ps ^ mix64(0xBA1ull + b)));
+    }
+    __syncthreads();
+    const uint32_t E = g.cx * g.cy * g.cz;
+    const uint64_t wo = mix64(ps ^ 0x3141592653589793ull);  // the window's offset in the base
+    const uint32_t ox = (uint32_t)(wo % bsx), oy = (uint32_t)((wo >> 21) % bsy), oz = (uint32_t)((wo >> 42) % bsz);
+    float m = 0.0f;
+    float* dst = out + c * E;
+    for (uint32_t e = threadIdx.x; e < E; e += 256) {
+        const uint32_t x = bx * g.cx + e % g.cx, y = by * g.cy + (e / g.cx) % g.cy, z = bz * g.cz + e / (g.cx * g.cy);
+        const uint64_t p = ((uint64_t)z * g.py + y) * g.px + x;
+        float v;
+        if (r == 0) {
+            v = base[((uint64_t)((oz + z) % bsz) * bsy + (oy + y) % bsy) * bsx + (ox + x) % bsx];
+        } else if (r == 1) {
+            v = hash_normal(mix64(ps ^ mix64(p)));
+        } else if (r == 2) {
+            const uint64_t h = mix64(ps ^ mix64(p));
+            v = u01(h >> 24) >= 0.8f ? hash_normal(mix64(h)) : 0.0f;
+        } else if (r == 3) {
+            v = 0.0f;
+            for (int b = 0; b < SYN_BOXES; b++)
+                if ((int)x >= s_box[b][0] && (int)x < s_box[b][1] && (int)y >= s_box[b][2] && (int)y < s_box[b][3] &&
+                    (int)z >= s_box[b][4] && (int)z < s_box[b][5])
+                    v = s_val[b];
+        } else {
+            const double u = (double)(mix64(ps ^ mix64(p)) >> 11) * (1.0 / 9007199254740992.0);
+            int lo = 0, hi = POIS_K - 1;  // smallest k with CDF(k) > u
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (kPoisCdf.c[mid] > u) hi = mid;
+                else lo = mid + 1;
+            }
+            v = (float)lo;
+        }
+        dst[e] = v;
+        m = fmaxf(m, fabsf(v));
+    }
+    m = wave_max_f(m);
+    if (lane_id() == 0) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cmax[c] = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+        if (regime) regime[c] = (uint8_t)r;
+    }
+}
+
+// One thread per parent: its chunks' bound from their maxima (in place).
+__global__ void __launch_bounds__(256) synth_eb_kernel(float* __restrict__ eb, uint64_t n_parents, uint32_t cpp) {
+    const uint64_t f = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (f >= n_parents) return;
+    float vb = 0.0f;
+    for (uint32_t k = 0; k < cpp; k++) vb = fmaxf(vb, eb[f * cpp + k]);
+    const float e = vb > 0.0f ? 1e-3f * vb : 1e-3f;
+    for (uint32_t k = 0; k < cpp; k++) eb[f * cpp + k] = e;
+}
+
 }  // namespace
 }  // namespace sel
 }  // namespace fzh
@@ -473,7 +628,9 @@ int fzs_mlp_forward(void* stream, const float* d_params, const float* d_norm, co
     return FZH_OK;
 }
 
-size_t fzs_mlp_train_ws_bytes(size_t n) { return 4 * n * (NF + 2 * H1 + 2 * H2 + NC + 1) + 256; }
+size_t fzs_mlp_train_ws_bytes(size_t n) {
+    return 4 * n * (NF + 2 * H1 + 2 * H2 + NC + 1) + 256 + 4 * cdiv(n, GRAD_SEG) * (NP + 1);
+}
 
 int fzs_mlp_train_step(void* stream, const float* d_params, const float* d_norm, const float* d_feat,
                        const float* d_cost, size_t n, float tau, float* d_grad, float* d_loss, void* d_ws,
@@ -485,7 +642,10 @@ int fzs_mlp_train_step(void* stream, const float* d_params, const float* d_norm,
     if (ws_bytes < fzs_mlp_train_ws_bytes(n)) return fail(FZH_E_CAPACITY, "mlp train: workspace too small");
     const Rec r = rec_layout(d_ws, n);
     mlp_backward_kernel<<<(unsigned)cdiv(n, 256), 256, 0, S(stream)>>>(d_params, d_norm, d_feat, d_cost, n, tau, r);
-    mlp_grad_kernel<<<(unsigned)cdiv(NP + 1, 256), 256, 0, S(stream)>>>(r, n, d_grad, d_loss);
+    const uint32_t nseg = (uint32_t)cdiv(n, GRAD_SEG);
+    float* part = reinterpret_cast<float*>(reinterpret_cast<uint8_t*>(d_ws) + 4 * n * (NF + 2 * H1 + 2 * H2 + NC + 1) + 256);
+    mlp_grad_part_kernel<<<dim3((unsigned)cdiv(NP + 1, 256), nseg), 256, 0, S(stream)>>>(r, n, part);
+    mlp_grad_final_kernel<<<(unsigned)cdiv(NP + 1, 256), 256, 0, S(stream)>>>(part, nseg, n, d_grad, d_loss);
     FZH_CHECK_LAUNCH("mlp train");
     return FZH_OK;
 }
@@ -528,6 +688,23 @@ int fzs_gather_chunks(void* stream, const float* d_field, uint64_t nx, uint64_t 
 int fzs_scatter_chunks(void* stream, const float* d_chunks, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t cx,
                        uint32_t cy, uint32_t cz, float* d_field) {
     return chunk_copy(stream, d_chunks, d_field, nx, ny, nz, cx, cy, cz, false);
+}
+
+int fzs_synth_chunks(void* stream, int nd, uint64_t n_chunks, uint64_t seed, const float* d_base, uint32_t bx,
+                     uint32_t by, uint32_t bz, float* d_chunks, float* d_eb, uint8_t* d_regime) {
+    if (nd != 2 && nd != 3) return fail(FZH_E_INVALID, "synth chunks: nd must be 2 or 3");
+    if (n_chunks == 0) return FZH_OK;
+    if (!d_chunks || !d_eb || !d_base) return fail(FZH_E_INVALID, "synth chunks: null pointer");
+    if (!bx || !by || !bz || (nd == 2 && bz != 1)) return fail(FZH_E_INVALID, "synth chunks: bad base field shape");
+    const SynGeom g = syn_geom(nd);
+    if (n_chunks % g.cpp) return fail(FZH_E_INVALID, "synth chunks: a whole number of parent fields is required");
+    if (n_chunks > 0x7FFFFFFFull) return fail(FZH_E_INVALID, "synth chunks: too many chunks");
+    synth_chunks_kernel<<<(unsigned)n_chunks, 256, 0, S(stream)>>>(d_chunks, d_eb, d_regime, nd, seed, d_base, bx, by,
+                                                                   bz);
+    const uint64_t np = n_chunks / g.cpp;
+    synth_eb_kernel<<<(unsigned)cdiv(np, 256), 256, 0, S(stream)>>>(d_eb, np, g.cpp);
+    FZH_CHECK_LAUNCH("synth chunks");
+    return FZH_OK;
 }
 
 int fzs_select(void* stream, const float* d_logits, size_t n, uint8_t* d_choice) {

@@ -73,6 +73,28 @@ def label_sizes(d_chunks, n: int, shape, d_eb, stream=None, host_threads=0) -> n
     return out
 
 
+# Base fields for the smooth regime of the device-generated chunks (parents
+# of regime 0 are seeded windows of them): k^-4 Gaussian random fields.
+SYNTH_BASE = {2: (2048, 2048, 1), 3: (256, 256, 128)}
+
+
+def synth_base(nd: int, seed: int, stream=None) -> DeviceBuffer:
+    bx, by, bz = SYNTH_BASE[nd]
+    base = DeviceBuffer(4 * bx * by * bz)
+    call("fzh_synth_power_law_f32", stream, base.ptr, bx, by, bz, np.float32(-4.0), seed)
+    return base
+
+
+def synth_chunks(nd: int, n: int, seed: int, base: DeviceBuffer, stream=None, regime=False):
+    """n device chunks of nd dimensions from `seed` (fzs_synth_chunks):
+    (chunks, bounds[, regimes]) as DeviceBuffers."""
+    bx, by, bz = SYNTH_BASE[nd]
+    d_c, d_eb = DeviceBuffer(4 * 65536 * n), DeviceBuffer(max(4 * n, 16))
+    d_r = DeviceBuffer(max(n, 16)) if regime else None
+    call("fzs_synth_chunks", stream, nd, n, seed, base.ptr, bx, by, bz, d_c.ptr, d_eb.ptr, d_r.ptr if d_r else None)
+    return (d_c, d_eb, d_r) if regime else (d_c, d_eb)
+
+
 def regret_costs(sizes: np.ndarray) -> np.ndarray:
     s = sizes.astype(np.float64)
     return (s / s.min(axis=1, keepdims=True) - 1.0).astype(np.float32)
@@ -148,27 +170,38 @@ class Selector:
              np.float32(lr), np.float32(b1), np.float32(b2), np.float32(eps), np.float32(wd))
 
     def fit(self, feat: np.ndarray, cost: np.ndarray, epochs=20, batch=1024, lr=3e-3, tau=0.02, seed=0,
-            wd=1e-4) -> list:
-        """Mini-batch training on the device; returns the mean loss per epoch."""
+            wd=1e-4, stream=None) -> list:
+        """Mini-batch training on the device; returns the mean loss per epoch.
+        Each epoch uploads one shuffled copy of the set; batches are slices
+        of it and each step's loss lands in a device array read once per
+        epoch (no host round trip per step)."""
         n = feat.shape[0]
         self.set_norm_from(feat)
         rng = np.random.default_rng(seed)
-        d_feat = DeviceBuffer(4 * NFEAT * batch)
-        d_cost = DeviceBuffer(4 * NCHAINS * batch)
+        nb = max(1, n // batch) if n >= batch else 1
+        bsz = batch if n >= batch else n
+        d_feat = DeviceBuffer(4 * NFEAT * nb * bsz)
+        d_cost = DeviceBuffer(4 * NCHAINS * nb * bsz)
+        d_loss = DeviceBuffer(max(4 * nb, 16))
+        need = lib().fzs_mlp_train_ws_bytes(bsz)
+        if self._ws is None or self._ws.nbytes < need:
+            if self._ws is not None:
+                self._ws.free()
+            self._ws = DeviceBuffer(need)
         hist = []
         for _ in range(epochs):
-            perm = rng.permutation(n)
-            tot, cnt = 0.0, 0
-            for s in range(0, n - batch + 1 if n >= batch else 1, batch):
-                idx = perm[s:s + batch]
-                d_feat.upload(np.ascontiguousarray(feat[idx], np.float32))
-                d_cost.upload(np.ascontiguousarray(cost[idx], np.float32))
-                tot += self.train_step(d_feat.ptr, d_cost.ptr, len(idx), tau) * len(idx)
-                cnt += len(idx)
-                self.adam(lr, wd=wd)
-            hist.append(tot / max(cnt, 1))
+            perm = rng.permutation(n)[: nb * bsz]
+            d_feat.upload(np.ascontiguousarray(feat[perm], np.float32))
+            d_cost.upload(np.ascontiguousarray(cost[perm], np.float32))
+            for b in range(nb):
+                call("fzs_mlp_train_step", stream, self.params.ptr, self.norm.ptr, d_feat.ptr + 4 * NFEAT * bsz * b,
+                     d_cost.ptr + 4 * NCHAINS * bsz * b, bsz, np.float32(tau), self.grad.ptr, d_loss.ptr + 4 * b,
+                     self._ws.ptr, self._ws.nbytes)
+                self.adam(lr, wd=wd, stream=stream)
+            hist.append(float(d_loss.download(np.float32, nb).astype(np.float64).mean()))
         d_feat.free()
         d_cost.free()
+        d_loss.free()
         return hist
 
     def close(self):

@@ -99,6 +99,19 @@ int fzs_gather_chunks(void* stream, const float* d_field, uint64_t nx, uint64_t 
 int fzs_scatter_chunks(void* stream, const float* d_chunks, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t cx,
                        uint32_t cy, uint32_t cz, float* d_field);
 
+/* Seeded training/evaluation chunks generated on the device (no
+ * counterpart in the reference): n_chunks 256 KiB chunks of nd = 2 (256 x
+ * 256, 4 per 512 x 512 parent field) or nd = 3 (16 x 64 x 64, 8 per
+ * 32 x 128 x 128 parent), parent f of regime f % 5 (0 GRF k^-4: a window at
+ * a seeded offset of the base field d_base, bx x by x bz, e.g. from
+ * fzh_synth_power_law_f32 with expo -4, wrapping periodically; 1 white
+ * noise; 2 80% zeros; 3 piecewise-constant boxes; 4 Poisson(20)),
+ * d_eb[c] = 1e-3 * max |parent| (1e-3 for an all-zero parent),
+ * d_regime[c] (optional) = the regime.  n_chunks must be a multiple of the
+ * chunks per parent.  Deterministic in (nd, seed, base, chunk index).      */
+int fzs_synth_chunks(void* stream, int nd, uint64_t n_chunks, uint64_t seed, const float* d_base, uint32_t bx,
+                     uint32_t by, uint32_t bz, float* d_chunks, float* d_eb, uint8_t* d_regime);
+
 /* ---- selecting compressor -------------------------------------------------
  * Compresses a field chunk by chunk, each chunk with the chain the selector
  * picks (FZS_MODE_LEARNED: the MLP; FZS_MODE_EXHAUSTIVE: the smallest size

@@ -195,3 +195,65 @@ def test_selecting_compressor_round_trip(trained):
         sc.close()
         d.free()
         out.free()
+
+
+# ---------------------------------------------------------------------------
+# Device-generated selector chunks (fzs_synth_chunks)
+# ---------------------------------------------------------------------------
+def _synth(nd, n, seed):
+    E = 65536
+    base = S.synth_base(nd, 99)
+    d_c, d_eb, d_r = S.synth_chunks(nd, n, seed, base, regime=True)
+    base.free()
+    shape = (n, 256, 256) if nd == 2 else (n, 16, 64, 64)
+    out = (d_c.download(np.float32, E * n).reshape(shape), d_eb.download(np.float32, n),
+           d_r.download(np.uint8, n))
+    for b in (d_c, d_eb, d_r):
+        b.free()
+    return out
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_synth_chunks_regimes_and_bounds(nd):
+    cpp = 4 if nd == 2 else 8
+    n = 10 * cpp  # two parent fields of every regime
+    c, eb, reg = _synth(nd, n, 7)
+    np.testing.assert_array_equal(reg, (np.arange(n) // cpp) % 5)
+    for f in range(n // cpp):
+        ch = c[f * cpp:(f + 1) * cpp]
+        vb = float(np.max(np.abs(ch)))
+        want = np.float32(np.float32(1e-3) * np.float32(vb)) if vb > 0 else np.float32(1e-3)
+        assert (eb[f * cpp:(f + 1) * cpp] == want).all()
+        r = f % 5
+        v = ch.astype(np.float64).ravel()
+        if r == 0:  # smooth: neighbours along x nearly equal
+            a = ch.reshape(-1, ch.shape[-1])
+            cor = np.corrcoef(a[:, :-1].ravel(), a[:, 1:].ravel())[0, 1]
+            assert cor > 0.9
+        elif r == 1:
+            assert abs(v.mean()) < 0.02 and abs(v.std() - 1) < 0.02
+        elif r == 2:
+            z = np.mean(v == 0.0)
+            assert 0.78 < z < 0.82
+            assert abs(v[v != 0].std() - 1) < 0.03
+        elif r == 3:
+            a = ch.reshape(-1, ch.shape[-1])
+            assert np.mean(a[:, 1:] == a[:, :-1]) > 0.9
+        else:
+            assert np.array_equal(v, np.round(v)) and v.min() >= 0
+            assert abs(v.mean() - 20) < 0.1 and abs(v.var() - 20) < 0.5
+
+
+def test_synth_chunks_deterministic_and_seeded():
+    a = _synth(2, 8, 3)[0]
+    b = _synth(2, 8, 3)[0]
+    c = _synth(2, 8, 4)[0]
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(a, c)
+
+
+def test_synth_chunks_rejects_partial_parent():
+    d = DeviceBuffer(16)
+    with pytest.raises(FzhError, match="whole number of parent"):
+        call("fzs_synth_chunks", None, 3, 4, 1, d.ptr, 4, 4, 4, d.ptr, d.ptr, None)  # 3D parents hold 8 chunks
+    d.free()
