@@ -21,6 +21,7 @@ _Z = C.c_size_t
 _I = C.c_int
 _U32 = C.c_uint32
 _F = C.c_float
+_D = C.c_double
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -42,6 +43,12 @@ SIGNATURES = {
     "fzh_value_base_ws_bytes": (_Z, [_Z]),
     "fzh_value_base_f32": (_I, [_P, _P, _Z, _I, _F, _P, _P, _Z]),
     "fzh_value_base_f32_noalloc": (_I, [_P, _P, _Z, _I, _F, _P]),
+    "fzh_lorenzo_quant_fwd_f64_ws_bytes": (_Z, [_Z, _Z, _Z, _Z]),
+    "fzh_lorenzo_quant_fwd_f64": (_I, [_P, _P, _Z, _Z, _Z, _Z, _D, _P, _U32, _I, _I, _P, _P, _P, _P, _Z, _P, _Z]),
+    "fzh_lorenzo_quant_inv_f64_ws_bytes": (_Z, [_Z, _Z, _Z, _Z, _Z]),
+    "fzh_lorenzo_quant_inv_f64": (_I, [_P, _P, _Z, _Z, _Z, _Z, _P, _P, _P, _Z, _D, _I, _I, _P, _P, _Z]),
+    "fzh_value_base_f64_ws_bytes": (_Z, [_Z]),
+    "fzh_value_base_f64": (_I, [_P, _P, _Z, _I, _F, _P, _P, _Z]),
     "fzh_bitshuffle_enc": (_I, [_P, _P, _P, _Z, _U32, _I]),
     "fzh_bitshuffle_dec": (_I, [_P, _P, _P, _Z, _U32, _I]),
     "fzh_rze_enc_bound": (_Z, [_Z]),
@@ -85,6 +92,8 @@ SIGNATURES = {
     "fzp_destroy": (_I, [_P]),
     "fzp_add_lorenzo": (_I, [_P, _F, _I, _U32, _F, _I, _I, _F, C.POINTER(_I)]),
     "fzp_set_lorenzo_strict": (_I, [_P, _I, _I, _F]),
+    "fzp_add_lorenzo_f64": (_I, [_P, _F, _I, _U32, _F, _I, _I, _F, C.POINTER(_I)]),
+    "fzp_stage_abs_error_bound_f64": (_I, [_P, _I, C.POINTER(_D)]),
     "fzp_add_bitshuffle": (_I, [_P, _U32, _I, C.POINTER(_I)]),
     "fzp_add_rze": (_I, [_P, _U32, _I, C.POINTER(_I)]),
     "fzp_add_rle": (_I, [_P, _I, C.POINTER(_I)]),

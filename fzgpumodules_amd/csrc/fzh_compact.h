@@ -19,23 +19,26 @@ namespace {
 // Up to two lists (e.g. outliers and patches) per launch; a list with a null
 // s_val moves indices only.  For two lists the second list's counts must
 // directly follow the first's (cnt1 = cnt0 + T), so one scan covers both.
-struct CompactList {
-    const float* s_val;
+// V: the value type (float, or double for FP64 outlier errors).
+template <typename V>
+struct CompactListT {
+    const V* s_val;
     const uint32_t* s_idx;
     const uint32_t* cnt;
-    float* val;
+    V* val;
     uint32_t* idx;
     uint32_t* total;
     uint64_t cap;
 };
+using CompactList = CompactListT<float>;
 
 constexpr uint32_t COMPACT_INLINE = 16384;
 
 // excl: exclusive prefix (mod 2^32) of the NL*T counts, excl[NL*T] = sum
 // (INL: unused, the prefixes are summed in the kernel).
-template <int NL, bool INL>
+template <int NL, bool INL, typename V>
 __global__ void __launch_bounds__(256)
-place_kernel(CompactList L0, CompactList L1, uint32_t T, uint32_t E, const uint32_t* __restrict__ excl) {
+place_kernel(CompactListT<V> L0, CompactListT<V> L1, uint32_t T, uint32_t E, const uint32_t* __restrict__ excl) {
     const int w = (int)wave_in_wg();
     const uint32_t wave = blockIdx.x * 4 + (uint32_t)w;
     const int l = lane_id();
@@ -92,9 +95,9 @@ place_kernel(CompactList L0, CompactList L1, uint32_t T, uint32_t E, const uint3
         if constexpr (INL) dst0 = (uint32_t)__shfl((int)ex, j, 64);
         else dst0 = excl[(uint64_t)lj * T + tile] - (lj ? excl[T] : 0u);
         const uint64_t src0 = (uint64_t)tile * E;
-        const float* sv = lj ? L1.s_val : L0.s_val;
+        const V* sv = lj ? L1.s_val : L0.s_val;
         const uint32_t* si = lj ? L1.s_idx : L0.s_idx;
-        float* dv = lj ? L1.val : L0.val;
+        V* dv = lj ? L1.val : L0.val;
         uint32_t* di = lj ? L1.idx : L0.idx;
         const uint64_t cap = lj ? L1.cap : L0.cap;
         for (uint32_t k = (uint32_t)l; k < cj; k += 64) {
@@ -112,7 +115,8 @@ inline size_t compact_state_bytes(size_t T) {
     return align_up(4 * (2 * T + 1), 256) + scan_ws_bytes(2 * std::max<size_t>(T, 1));
 }
 
-inline int launch_compact(hipStream_t s, void* state, size_t T, uint32_t E, CompactList L0, CompactList* L1) {
+template <typename V>
+inline int launch_compact(hipStream_t s, void* state, size_t T, uint32_t E, CompactListT<V> L0, CompactListT<V>* L1) {
     if (T == 0) return FZH_OK;
     const int nl = L1 ? 2 : 1;
     if (L1 && L1->cnt != L0.cnt + T) return fail(FZH_E_INVALID, "compact: second count list must follow the first");
@@ -122,9 +126,10 @@ inline int launch_compact(hipStream_t s, void* state, size_t T, uint32_t E, Comp
     if (!inl)
         if (int rc = scan_u32<true>(s, L0.cnt, excl, nl * T, nullptr, excl + nl * T, ws)) return rc;
     const unsigned blocks = (unsigned)cdiv(cdiv(T, 32), 4);
-    auto* k = inl ? (L1 ? place_kernel<2, true> : place_kernel<1, true>)
-                  : (L1 ? place_kernel<2, false> : place_kernel<1, false>);
-    launch_k("place_kernel", k, dim3(blocks), dim3(256), 0, s, L0, L1 ? *L1 : CompactList{}, (uint32_t)T, E, excl);
+    auto* k = inl ? (L1 ? place_kernel<2, true, V> : place_kernel<1, true, V>)
+                  : (L1 ? place_kernel<2, false, V> : place_kernel<1, false, V>);
+    launch_k("place_kernel", k, dim3(blocks), dim3(256), 0, s, L0, L1 ? *L1 : CompactListT<V>{}, (uint32_t)T, E,
+             excl);
     FZH_CHECK_LAUNCH("compact");
     return FZH_OK;
 }

@@ -51,6 +51,16 @@ void need(const void* p) {
     if (!p) throw std::invalid_argument("null argument");
 }
 
+// The resolved absolute bound of a LorenzoQuant stage of any instantiation.
+bool lorenzo_abs_eb(fz::Stage* s, double* v) {
+    if (auto* a = dynamic_cast<fz::LorenzoQuantStage<float, uint16_t>*>(s)) *v = a->getComputedErrorBound();
+    else if (auto* b = dynamic_cast<fz::LorenzoQuantStage<float, uint8_t>*>(s)) *v = b->getComputedErrorBound();
+    else if (auto* c = dynamic_cast<fz::LorenzoQuantStage<double, uint16_t>*>(s)) *v = c->getComputedErrorBound();
+    else if (auto* d = dynamic_cast<fz::LorenzoQuantStage<double, uint32_t>*>(s)) *v = d->getComputedErrorBound();
+    else return false;
+    return true;
+}
+
 }  // namespace
 
 static int add_code(fzp_pipeline* p, int eb, bool nb, int* id);
@@ -105,6 +115,34 @@ int fzp_add_lorenzo(fzp_pipeline* p, float eb, int eb_mode, uint32_t radius, flo
     });
 }
 
+int fzp_add_lorenzo_f64(fzp_pipeline* p, float eb, int eb_mode, uint32_t radius, float cap, int zigzag,
+                        int code_bytes, float value_base, int* id) {
+    return guard([&] {
+        need(p);
+        auto setup = [&](auto* l, auto r) {
+            l->setErrorBound(eb);
+            l->setErrorBoundMode(mode_of(eb_mode));
+            l->setQuantRadius(r);
+            l->setOutlierCapacity(cap);
+            l->setZigzagCodes(zigzag != 0);
+            l->setValueBase(value_base);
+        };
+        fz::Stage* s;
+        if (code_bytes == 2) {
+            auto* l = new fz::LorenzoQuantStage<double, uint16_t>();
+            setup(l, (uint16_t)radius);
+            s = l;
+        } else if (code_bytes == 4) {
+            auto* l = new fz::LorenzoQuantStage<double, uint32_t>();
+            setup(l, (uint32_t)radius);
+            s = l;
+        } else {
+            throw std::invalid_argument("code_bytes must be 2 or 4 for FP64 input");
+        }
+        add(p, s, id);
+    });
+}
+
 int fzp_set_lorenzo_strict(fzp_pipeline* p, int id, int on, float cap) {
     return guard([&] {
         fz::Stage* s = stage_at(p, id);
@@ -114,6 +152,9 @@ int fzp_set_lorenzo_strict(fzp_pipeline* p, int id, int on, float cap) {
         } else if (auto* b = dynamic_cast<fz::LorenzoQuantStage<float, uint8_t>*>(s)) {
             b->setStrictErrorBound(on != 0);
             if (cap > 0) b->setPatchCapacity(cap);
+        } else if (dynamic_cast<fz::LorenzoQuantStage<double, uint16_t>*>(s) ||
+                   dynamic_cast<fz::LorenzoQuantStage<double, uint32_t>*>(s)) {
+            throw std::invalid_argument("the strict error bound is available for float input only");
         } else {
             throw std::invalid_argument("stage is not a LorenzoQuant stage");
         }
@@ -438,10 +479,16 @@ int fzp_output_info(fzp_pipeline* p, int i, char* name, size_t cap, size_t* byte
 int fzp_stage_abs_error_bound(fzp_pipeline* p, int id, float* abs_eb) {
     return guard([&] {
         need(abs_eb);
-        fz::Stage* s = stage_at(p, id);
-        if (auto* a = dynamic_cast<fz::LorenzoQuantStage<float, uint16_t>*>(s)) *abs_eb = a->getComputedErrorBound();
-        else if (auto* b = dynamic_cast<fz::LorenzoQuantStage<float, uint8_t>*>(s)) *abs_eb = b->getComputedErrorBound();
-        else throw std::invalid_argument("stage is not a LorenzoQuant stage");
+        double v;
+        if (!lorenzo_abs_eb(stage_at(p, id), &v)) throw std::invalid_argument("stage is not a LorenzoQuant stage");
+        *abs_eb = (float)v;
+    });
+}
+
+int fzp_stage_abs_error_bound_f64(fzp_pipeline* p, int id, double* abs_eb) {
+    return guard([&] {
+        need(abs_eb);
+        if (!lorenzo_abs_eb(stage_at(p, id), abs_eb)) throw std::invalid_argument("stage is not a LorenzoQuant stage");
     });
 }
 

@@ -102,7 +102,17 @@ void LorenzoQuantStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, co
         size_t nx, ny, nz;
         resolveDims(n, nx, ny, nz);
         const size_t max_out = sizes[1] / sizeof(TI);
-        const float ebx2 = 2.0f * computed_abs_eb_;
+        if constexpr (std::is_same_v<TI, double>) {
+            // ebx2 = 2 * abs_eb in double (lorenzo_quant.cu:509)
+            inv_ws_.ensure(fzh_lorenzo_quant_inv_f64_ws_bytes(n, nx, ny, nz, max_out), pool, stream);
+            throwIfFzh(fzh_lorenzo_quant_inv_f64(stream, inputs[0], n, nx, ny, nz, (const double*)inputs[1],
+                                                 (const uint32_t*)inputs[2], (const uint32_t*)inputs[3], max_out,
+                                                 2.0 * computed_abs_eb_, config_.zigzag_codes ? 1 : 0, (int)sizeof(TC),
+                                                 (double*)outputs[0], inv_ws_.ptr, inv_ws_.bytes),
+                       "LorenzoQuantStage (inverse)");
+            return;
+        }
+        const float ebx2 = 2.0f * (float)computed_abs_eb_;
         inv_ws_.ensure(fzh_lorenzo_quant_inv_ws_bytes(n, nx, ny, nz, max_out), pool, stream);
         if (config_.strict_error_bound) {
             // the patches go in with the inverse (one launch for short lists)
@@ -142,58 +152,106 @@ void LorenzoQuantStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, co
     if (ndim() == 1) nx = valid;
     else if (nx * ny * nz > valid)
         throw std::runtime_error("LorenzoQuantStage: dims exceed the unpadded input");
-    // Resolve the absolute bound (lorenzo_quant.cu:612-644).
-    const float* d_r = nullptr;
-    float r = 0.0f;
-    params_pending_ = false;
-    if (config_.eb_mode == ErrorBoundMode::ABS) {
-        computed_abs_eb_ = config_.error_bound;
-        computed_value_base_ = 0.0f;
-        r = 1.0f / (2.0f * computed_abs_eb_);
-    } else if (config_.precomputed_value_base > 0.0f) {
-        computed_value_base_ = config_.precomputed_value_base;
-        computed_abs_eb_ = config_.error_bound * computed_value_base_;
-        r = 1.0f / (2.0f * computed_abs_eb_);
+    if constexpr (std::is_same_v<TI, double>) {
+        executeF64(stream, pool, inputs, outputs, n, valid, nx, ny, nz, max_out);
     } else {
-        // value base reduced on the device; no host round trip before the kernel
-        params_.ensure(256 + fzh_value_base_ws_bytes(n), pool, stream);
-        float* d_params = (float*)params_.ptr;
-        const int mode = (config_.eb_mode == ErrorBoundMode::NOA ? 2 : 1) | (valid < n ? FZH_VB_ZERO_PADDED : 0);
-        throwIfFzh(fzh_value_base_f32(stream, (const float*)inputs[0], valid, mode, config_.error_bound, d_params,
-                                      (uint8_t*)params_.ptr + 256, params_.bytes - 256),
-                   "LorenzoQuantStage value base");
-        d_r = d_params + 2;
-        params_pending_ = true;
-        addReadback(d_params, h_params_, sizeof(h_params_));
+        // Resolve the absolute bound (lorenzo_quant.cu:612-644).
+        const float* d_r = nullptr;
+        float r = 0.0f;
+        params_pending_ = false;
+        if (config_.eb_mode == ErrorBoundMode::ABS) {
+            computed_abs_eb_ = config_.error_bound;
+            computed_value_base_ = 0.0f;
+            r = 1.0f / (2.0f * computed_abs_eb_);
+        } else if (config_.precomputed_value_base > 0.0f) {
+            computed_value_base_ = config_.precomputed_value_base;
+            computed_abs_eb_ = config_.error_bound * computed_value_base_;
+            r = 1.0f / (2.0f * computed_abs_eb_);
+        } else {
+            // value base reduced on the device; no host round trip before the kernel
+            params_.ensure(256 + fzh_value_base_ws_bytes(n), pool, stream);
+            float* d_params = (float*)params_.ptr;
+            const int mode = (config_.eb_mode == ErrorBoundMode::NOA ? 2 : 1) | (valid < n ? FZH_VB_ZERO_PADDED : 0);
+            throwIfFzh(fzh_value_base_f32(stream, (const float*)inputs[0], valid, mode, config_.error_bound, d_params,
+                                          (uint8_t*)params_.ptr + 256, params_.bytes - 256),
+                       "LorenzoQuantStage value base");
+            d_r = d_params + 2;
+            params_pending_ = true;
+            addReadback(d_params, h_params_, sizeof(h_params_));
+        }
+        ws_.ensure(config_.strict_error_bound ? fzh_lorenzo_quant_fwd_strict_ws_bytes(n, nx, ny, nz)
+                                              : fzh_lorenzo_quant_fwd_ws_bytes(n, nx, ny, nz),
+                   pool, stream);
+        if (!config_.strict_error_bound) {
+            throwIfFzh(fzh_lorenzo_quant_fwd_f32(stream, (const float*)inputs[0], n, nx, ny, nz, r, d_r,
+                                                 (uint32_t)config_.quant_radius, config_.zigzag_codes ? 1 : 0,
+                                                 (int)sizeof(TC), outputs[0], (float*)outputs[1], (uint32_t*)outputs[2],
+                                                 (uint32_t*)outputs[3], max_out, ws_.ptr, ws_.bytes),
+                       "LorenzoQuantStage");
+        } else {
+            const size_t pm = maxPatches(n);
+            const float eb = computed_abs_eb_;
+            throwIfFzh(fzh_lorenzo_quant_fwd_strict_f32(
+                           stream, (const float*)inputs[0], n, nx, ny, nz, r, d_r, (uint32_t)config_.quant_radius,
+                           config_.zigzag_codes ? 1 : 0, (int)sizeof(TC), outputs[0], (float*)outputs[1],
+                           (uint32_t*)outputs[2], (uint32_t*)outputs[3], max_out, eb, 2.0f * eb,
+                           params_pending_ ? (const float*)params_.ptr : nullptr, (float*)outputs[4],
+                           (uint32_t*)outputs[5], (uint32_t*)outputs[6], pm, ws_.ptr, ws_.bytes),
+                       "LorenzoQuantStage (strict)");
+            d_pcount_ptr_ = outputs[6];
+            addReadback(d_pcount_ptr_, &h_pcount_, 4);
+        }
+        d_count_ptr_ = outputs[3];
+        addReadback(d_count_ptr_, &h_count_, 4);
+        actual_output_sizes_ = {n * sizeof(TC), max_out * sizeof(TI), max_out * sizeof(uint32_t), sizeof(uint32_t)};
+        if (config_.strict_error_bound) {
+            const size_t pm = maxPatches(n);
+            actual_output_sizes_.insert(actual_output_sizes_.end(), {pm * sizeof(TI), pm * sizeof(uint32_t), sizeof(uint32_t)});
+        }
     }
-    ws_.ensure(config_.strict_error_bound ? fzh_lorenzo_quant_fwd_strict_ws_bytes(n, nx, ny, nz)
-                                          : fzh_lorenzo_quant_fwd_ws_bytes(n, nx, ny, nz),
-               pool, stream);
-    if (!config_.strict_error_bound) {
-        throwIfFzh(fzh_lorenzo_quant_fwd_f32(stream, (const float*)inputs[0], n, nx, ny, nz, r, d_r,
-                                             (uint32_t)config_.quant_radius, config_.zigzag_codes ? 1 : 0,
-                                             (int)sizeof(TC), outputs[0], (float*)outputs[1], (uint32_t*)outputs[2],
-                                             (uint32_t*)outputs[3], max_out, ws_.ptr, ws_.bytes),
+}
+
+// FP64 forward: the bound resolved in double (lorenzo_quant.cu:612-644 with
+// TInput = double: abs_eb = (double)user_eb * (double)value_base, value_base
+// a float; ebx2_r = 1 / (2 abs_eb)), then the FP64 kernels.
+template <typename TI, typename TC>
+void LorenzoQuantStage<TI, TC>::executeF64(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                                           const std::vector<void*>& outputs, size_t n, size_t valid, size_t nx,
+                                           size_t ny, size_t nz, size_t max_out) {
+    if constexpr (std::is_same_v<TI, double>) {
+        if (config_.strict_error_bound)
+            throw std::runtime_error("LorenzoQuantStage: the strict error bound is available for float input only");
+        const double* d_r = nullptr;
+        double r = 0.0;
+        params_pending_ = false;
+        if (config_.eb_mode == ErrorBoundMode::ABS) {
+            computed_abs_eb_ = (TI)config_.error_bound;
+            computed_value_base_ = 0.0f;
+            r = 1.0 / (2.0 * (double)computed_abs_eb_);
+        } else if (config_.precomputed_value_base > 0.0f) {
+            computed_value_base_ = config_.precomputed_value_base;
+            computed_abs_eb_ = (TI)config_.error_bound * (TI)computed_value_base_;
+            r = 1.0 / (2.0 * (double)computed_abs_eb_);
+        } else {
+            params_.ensure(256 + fzh_value_base_f64_ws_bytes(n), pool, stream);
+            double* d_params = (double*)params_.ptr;
+            const int mode = (config_.eb_mode == ErrorBoundMode::NOA ? 2 : 1) | (valid < n ? FZH_VB_ZERO_PADDED : 0);
+            throwIfFzh(fzh_value_base_f64(stream, (const double*)inputs[0], valid, mode, config_.error_bound, d_params,
+                                          (uint8_t*)params_.ptr + 256, params_.bytes - 256),
+                       "LorenzoQuantStage value base");
+            d_r = d_params + 2;
+            params_pending_ = true;
+            addReadback(d_params, h_params_, sizeof(h_params_));
+        }
+        ws_.ensure(fzh_lorenzo_quant_fwd_f64_ws_bytes(n, nx, ny, nz), pool, stream);
+        throwIfFzh(fzh_lorenzo_quant_fwd_f64(stream, (const double*)inputs[0], n, nx, ny, nz, r, d_r,
+                                             (uint32_t)config_.quant_radius, config_.zigzag_codes ? 1 : 0, (int)sizeof(TC),
+                                             outputs[0], (double*)outputs[1], (uint32_t*)outputs[2], (uint32_t*)outputs[3],
+                                             max_out, ws_.ptr, ws_.bytes),
                    "LorenzoQuantStage");
-    } else {
-        const size_t pm = maxPatches(n);
-        const float eb = computed_abs_eb_;
-        throwIfFzh(fzh_lorenzo_quant_fwd_strict_f32(
-                       stream, (const float*)inputs[0], n, nx, ny, nz, r, d_r, (uint32_t)config_.quant_radius,
-                       config_.zigzag_codes ? 1 : 0, (int)sizeof(TC), outputs[0], (float*)outputs[1],
-                       (uint32_t*)outputs[2], (uint32_t*)outputs[3], max_out, eb, 2.0f * eb,
-                       params_pending_ ? (const float*)params_.ptr : nullptr, (float*)outputs[4],
-                       (uint32_t*)outputs[5], (uint32_t*)outputs[6], pm, ws_.ptr, ws_.bytes),
-                   "LorenzoQuantStage (strict)");
-        d_pcount_ptr_ = outputs[6];
-        addReadback(d_pcount_ptr_, &h_pcount_, 4);
-    }
-    d_count_ptr_ = outputs[3];
-    addReadback(d_count_ptr_, &h_count_, 4);
-    actual_output_sizes_ = {n * sizeof(TC), max_out * sizeof(TI), max_out * sizeof(uint32_t), sizeof(uint32_t)};
-    if (config_.strict_error_bound) {
-        const size_t pm = maxPatches(n);
-        actual_output_sizes_.insert(actual_output_sizes_.end(), {pm * sizeof(TI), pm * sizeof(uint32_t), sizeof(uint32_t)});
+        d_count_ptr_ = outputs[3];
+        addReadback(d_count_ptr_, &h_count_, 4);
+        actual_output_sizes_ = {n * sizeof(TC), max_out * sizeof(TI), max_out * sizeof(uint32_t), sizeof(uint32_t)};
     }
 }
 
@@ -202,7 +260,7 @@ void LorenzoQuantStage<TI, TC>::postStreamSync(hipStream_t) {
     if (is_inverse_ || !d_count_ptr_) return;
     completeReadbacks();
     if (params_pending_) {
-        computed_value_base_ = h_params_[0];
+        computed_value_base_ = (float)h_params_[0];
         computed_abs_eb_ = h_params_[1];
     }
     uint32_t cnt = h_count_;
@@ -238,11 +296,11 @@ template <typename TI, typename TC>
 size_t LorenzoQuantStage<TI, TC>::serializeHeader(size_t, uint8_t* buf, size_t max_size) const {
     if (max_size < sizeof(LorenzoQuantConfig)) throw std::runtime_error("Insufficient buffer for Lorenzo config");
     LorenzoQuantConfig c;
-    c.error_bound = computed_abs_eb_;
+    c.error_bound = (float)computed_abs_eb_;  // lorenzo_quant.h:218
     c.quant_radius = (uint32_t)config_.quant_radius;
     c.num_elements = (uint32_t)num_elements_;
     c.outlier_count = actual_outlier_count_;
-    c.input_type = DataType::FLOAT32;
+    c.input_type = dataTypeOf<TI>();
     c.code_type = dataTypeOf<TC>();
     c.ndim = (uint8_t)ndim();
     c.eb_mode = (uint8_t)config_.eb_mode;
@@ -264,7 +322,7 @@ void LorenzoQuantStage<TI, TC>::deserializeHeader(const uint8_t* buf, size_t siz
     LorenzoQuantConfig c;
     std::memcpy(&c, buf, std::min(size, sizeof(LorenzoQuantConfig)));
     config_.error_bound = c.error_bound;
-    computed_abs_eb_ = c.error_bound;
+    computed_abs_eb_ = (TI)c.error_bound;  // lorenzo_quant.h:255
     config_.quant_radius = (TC)c.quant_radius;
     num_elements_ = c.num_elements;
     actual_outlier_count_ = c.outlier_count;
@@ -289,6 +347,8 @@ void LorenzoQuantStage<TI, TC>::deserializeHeader(const uint8_t* buf, size_t siz
 
 template class LorenzoQuantStage<float, uint16_t>;
 template class LorenzoQuantStage<float, uint8_t>;
+template class LorenzoQuantStage<double, uint16_t>;
+template class LorenzoQuantStage<double, uint32_t>;
 
 // ===========================================================================
 // Bitshuffle
@@ -739,6 +799,8 @@ Stage* createStage(StageType type, const uint8_t* config, size_t config_size) {
         std::memcpy(&lc, config, sizeof(lc));
         if (lc.input_type == DataType::FLOAT32 && lc.code_type == DataType::UINT16) s = new LorenzoQuantStage<float, uint16_t>();
         else if (lc.input_type == DataType::FLOAT32 && lc.code_type == DataType::UINT8) s = new LorenzoQuantStage<float, uint8_t>();
+        else if (lc.input_type == DataType::FLOAT64 && lc.code_type == DataType::UINT16) s = new LorenzoQuantStage<double, uint16_t>();
+        else if (lc.input_type == DataType::FLOAT64 && lc.code_type == DataType::UINT32) s = new LorenzoQuantStage<double, uint32_t>();
         else throw std::runtime_error("Unsupported Lorenzo template instantiation: input_type=" +
                                       std::to_string((int)lc.input_type) + " code_type=" + std::to_string((int)lc.code_type));
         break;

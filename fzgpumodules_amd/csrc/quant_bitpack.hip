@@ -282,7 +282,7 @@ int fzh_quant_abs_fwd_f32(void* stream, const float* d_in, size_t n, float ebx2_
 #undef FZH_Q
     CompactList L0{sval, sidx, scnt, d_oval, d_oidx, d_ocount, max_outliers};
     FZH_CHECK_LAUNCH("quant_fwd_kernel");
-    if (int rc = launch_compact(s, w, T, QTILE, L0, nullptr)) return rc;
+    if (int rc = launch_compact<float>(s, w, T, QTILE, L0, nullptr)) return rc;
     return FZH_OK;
 }
 

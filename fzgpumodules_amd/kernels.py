@@ -66,6 +66,58 @@ def value_base_params(x, mode, user_eb):
     return par.download(np.float32, 4)
 
 
+def lorenzo_fwd64(x, dims, ebx2_r, radius, zigzag, code_bytes=2, max_outliers=None, n=None, d_params=None):
+    """FP64 Lorenzo forward (fzh_lorenzo_quant_fwd_f64): codes, double outlier
+    errors, indices, total count."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    n = x.size if n is None else n
+    nx, ny, nz = dims
+    if max_outliers is None:
+        max_outliers = n
+    din = DeviceBuffer.from_array(x if x.size else np.zeros(1))
+    codes = DeviceBuffer(max(n, 1) * code_bytes)
+    err = DeviceBuffer(max(max_outliers, 1) * 8)
+    idx = DeviceBuffer(max(max_outliers, 1) * 4)
+    cnt = DeviceBuffer(4)
+    ws = workspace(lib().fzh_lorenzo_quant_fwd_f64_ws_bytes(n, nx, ny, nz))
+    call("fzh_lorenzo_quant_fwd_f64", None, din.ptr, n, nx, ny, nz, float(ebx2_r), d_params, radius, int(zigzag),
+         code_bytes, codes.ptr, err.ptr, idx.ptr, cnt.ptr, max_outliers, ws.ptr, ws.nbytes)
+    c = int(cnt.download(np.uint32, 1)[0])
+    k = min(c, max_outliers)
+    return (codes.download(_CODE_DT[code_bytes], n), err.download(np.float64, k),
+            idx.download(np.uint32, k), c)
+
+
+def lorenzo_inv64(codes, dims, err, idx, count, ebx2, zigzag, max_outliers=None):
+    """FP64 Lorenzo inverse (fzh_lorenzo_quant_inv_f64)."""
+    codes = np.ascontiguousarray(codes)
+    n = codes.size
+    nx, ny, nz = dims
+    err = np.ascontiguousarray(err, np.float64)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    if max_outliers is None:
+        max_outliers = err.size
+    dc = DeviceBuffer.from_array(codes)
+    de = DeviceBuffer.from_array(err if err.size else np.zeros(1, np.float64))
+    di = DeviceBuffer.from_array(idx if idx.size else np.zeros(1, np.uint32))
+    dn = DeviceBuffer.from_array(np.array([count], np.uint32))
+    out = DeviceBuffer(n * 8)
+    ws = workspace(lib().fzh_lorenzo_quant_inv_f64_ws_bytes(n, nx, ny, nz, max_outliers))
+    call("fzh_lorenzo_quant_inv_f64", None, dc.ptr, n, nx, ny, nz, de.ptr, di.ptr, dn.ptr, max_outliers,
+         float(ebx2), int(zigzag), codes.dtype.itemsize, out.ptr, ws.ptr, ws.nbytes)
+    return out.download(np.float64, n)
+
+
+def value_base_params64(x, mode, user_eb):
+    """fzh_value_base_f64: [value_base, abs_eb, ebx2_r, ebx2] as doubles."""
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    din = DeviceBuffer.from_array(x)
+    par = DeviceBuffer(32)
+    ws = workspace(lib().fzh_value_base_f64_ws_bytes(x.size))
+    call("fzh_value_base_f64", None, din.ptr, x.size, mode, np.float32(user_eb), par.ptr, ws.ptr, ws.nbytes)
+    return par.download(np.float64, 4)
+
+
 def bitshuffle(b, block_size=16384, elem_width=2, inverse=False):
     b = np.ascontiguousarray(b).view(np.uint8).ravel()
     din = DeviceBuffer.from_array(b)

@@ -72,8 +72,11 @@ class Pipeline:
         return i.value
 
     def add_lorenzo(self, error_bound, mode="ABS", quant_radius=32768, outlier_capacity=0.2, zigzag=False,
-                    code_bytes=2, value_base=0.0) -> int:
-        return self._id("fzp_add_lorenzo", float(error_bound), _mode(mode), int(quant_radius),
+                    code_bytes=2, value_base=0.0, dtype="f32") -> int:
+        """LorenzoQuantStage<float, u8|u16> (dtype "f32", code_bytes 1 or 2) or
+        LorenzoQuantStage<double, u16|u32> (dtype "f64", code_bytes 2 or 4)."""
+        fn = {"f32": "fzp_add_lorenzo", "f64": "fzp_add_lorenzo_f64"}[dtype]
+        return self._id(fn, float(error_bound), _mode(mode), int(quant_radius),
                         float(outlier_capacity), int(bool(zigzag)), int(code_bytes), float(value_base))
 
     def set_strict(self, stage_id: int, on: bool = True, patch_capacity: float = 0.0) -> None:
@@ -216,8 +219,10 @@ class Pipeline:
         return res
 
     def abs_error_bound(self, stage_id: int) -> float:
-        v = C.c_float()
-        call("fzp_stage_abs_error_bound", self.h, stage_id, C.byref(v))
+        """The stage's resolved absolute bound in its input precision (a
+        float's value for FP32 stages, the double for FP64 stages)."""
+        v = C.c_double()
+        call("fzp_stage_abs_error_bound_f64", self.h, stage_id, C.byref(v))
         return v.value
 
     def last_perf(self):

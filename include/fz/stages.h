@@ -83,11 +83,16 @@ struct LorenzoQuantConfig {
 };
 static_assert(sizeof(LorenzoQuantConfig) == 44, "LorenzoQuantConfig is 44 bytes");
 
+// Instantiations (lorenzo_quant.h:335-339): <float, uint16_t>,
+// <float, uint8_t>, <double, uint16_t>, <double, uint32_t>.  With double
+// input every step runs in double and outlier errors are doubles; the
+// strict-error-bound extension is FP32 only.
 template <typename TInput = float, typename TCode = uint16_t>
 class LorenzoQuantStage : public Stage {
-    static_assert(std::is_same_v<TInput, float>, "LorenzoQuantStage: float input on gfx950");
-    static_assert(std::is_same_v<TCode, uint16_t> || std::is_same_v<TCode, uint8_t>,
-                  "LorenzoQuantStage: uint8_t or uint16_t codes");
+    static_assert((std::is_same_v<TInput, float> && (std::is_same_v<TCode, uint16_t> || std::is_same_v<TCode, uint8_t>)) ||
+                      (std::is_same_v<TInput, double> &&
+                       (std::is_same_v<TCode, uint16_t> || std::is_same_v<TCode, uint32_t>)),
+                  "LorenzoQuantStage: <float, uint8_t|uint16_t> or <double, uint16_t|uint32_t>");
 
   public:
     struct Config {
@@ -152,7 +157,7 @@ class LorenzoQuantStage : public Stage {
     float getValueBase() const { return config_.precomputed_value_base; }
     bool getZigzagCodes() const { return config_.zigzag_codes; }
     // Absolute bound of the last compression (REL/NOA resolved) and its base.
-    float getComputedErrorBound() const { return computed_abs_eb_; }
+    TInput getComputedErrorBound() const { return computed_abs_eb_; }
     float getComputedValueBase() const { return computed_value_base_; }
     uint32_t getOutlierCount() const { return actual_outlier_count_; }
 
@@ -163,8 +168,10 @@ class LorenzoQuantStage : public Stage {
     }
     DeviceSize getDeviceOutputSize(int index) const override {
         if (is_inverse_) return {};
-        if ((index == 1 || index == 2) && d_count_ptr_) return {d_count_ptr_, 2, 4};
-        if ((index == 4 || index == 5) && d_pcount_ptr_) return {d_pcount_ptr_, 2, 4};
+        if (index == 1 && d_count_ptr_) return {d_count_ptr_, 2, (uint32_t)sizeof(TInput)};
+        if (index == 2 && d_count_ptr_) return {d_count_ptr_, 2, 4};
+        if (index == 4 && d_pcount_ptr_) return {d_pcount_ptr_, 2, (uint32_t)sizeof(TInput)};
+        if (index == 5 && d_pcount_ptr_) return {d_pcount_ptr_, 2, 4};
         return {};
     }
     void setInverse(bool inv) override { is_inverse_ = inv; }
@@ -174,12 +181,12 @@ class LorenzoQuantStage : public Stage {
     uint8_t getOutputDataType(size_t i) const override {
         switch (i) {
         case 0: return (uint8_t)dataTypeOf<TCode>();
-        case 1: case 4: return (uint8_t)DataType::FLOAT32;
+        case 1: case 4: return (uint8_t)dataTypeOf<TInput>();
         case 2: case 3: case 5: case 6: return (uint8_t)DataType::UINT32;
         default: return (uint8_t)DataType::UINT8;
         }
     }
-    uint8_t getInputDataType(size_t) const override { return (uint8_t)DataType::FLOAT32; }
+    uint8_t getInputDataType(size_t) const override { return (uint8_t)dataTypeOf<TInput>(); }
     size_t serializeHeader(size_t output_index, uint8_t* buf, size_t max_size) const override;
     void deserializeHeader(const uint8_t* buf, size_t size) override;
     size_t getMaxHeaderSize(size_t) const override { return sizeof(LorenzoQuantConfig); }
@@ -191,20 +198,23 @@ class LorenzoQuantStage : public Stage {
     }
     size_t numPorts() const { return config_.strict_error_bound ? 7 : 4; }
     void resolveDims(size_t n, size_t& nx, size_t& ny, size_t& nz) const;
+    void executeF64(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
+                    const std::vector<void*>& outputs, size_t n, size_t valid, size_t nx, size_t ny, size_t nz,
+                    size_t max_out);
 
     Config config_;
     bool is_inverse_ = false;
     std::vector<size_t> actual_output_sizes_ = std::vector<size_t>(4, 0), saved_output_sizes_;
     size_t num_elements_ = 0;
     uint32_t actual_outlier_count_ = 0;
-    float computed_abs_eb_ = 0.0f;
+    TInput computed_abs_eb_ = 0;  // in the input type (lorenzo_quant.h:310)
     float computed_value_base_ = 0.0f;
     size_t valid_elems_ = 0;             // > 0: input elements past this read as zero (one call)
     const void* d_count_ptr_ = nullptr;  // forward: outlier count output
     const void* d_pcount_ptr_ = nullptr; // forward (strict): patch count output
     bool params_pending_ = false;        // REL/NOA resolved on the device
     uint32_t h_count_ = 0, h_pcount_ = 0;
-    float h_params_[4] = {0, 0, 0, 0};
+    TInput h_params_[4] = {0, 0, 0, 0};
     DeviceScratch ws_, inv_ws_, params_;
 };
 

@@ -184,6 +184,47 @@ int fzh_value_base_f32(void* stream, const float* d_in, size_t n, int mode,
 int fzh_value_base_f32_noalloc(void* stream, const float* d_in, size_t n, int mode,
                                float user_eb, float* d_params);
 
+/* ---- Lorenzo predict + quantize, FP64 input --------------------------------
+ * LorenzoQuantStage<double, uint16_t | uint32_t> (lorenzo_quant.h:335-339;
+ * kernels lorenzo_quant.cu:262-364, lorenzo_quant_nd.cu:29-105,199-274 with
+ * TInput = double).  The same tiles, dims rules, zero padding and outlier
+ * order as the FP32 entry points; every step runs in double
+ * (s = round(x * ebx2_r), the differences, |delta| < radius) and outlier
+ * errors are doubles.  code_bytes: 2 (u16) or 4 (u32).  d_in and d_codes
+ * 16-byte aligned.  If d_ebx2_r is non-NULL the kernel reads ebx2_r (a
+ * double) from device memory (fzh_value_base_f64's d_params + 2).           */
+size_t fzh_lorenzo_quant_fwd_f64_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz);
+int fzh_lorenzo_quant_fwd_f64(void* stream, const double* d_in, size_t n,
+                              size_t nx, size_t ny, size_t nz,
+                              double ebx2_r, const double* d_ebx2_r,
+                              uint32_t radius, int zigzag, int code_bytes,
+                              void* d_codes, double* d_outlier_err,
+                              uint32_t* d_outlier_idx, uint32_t* d_outlier_count,
+                              size_t max_outliers, void* d_ws, size_t ws_bytes);
+/* Inverse (lorenzo_quant.cu:17-87,155-244; lorenzo_quant_nd.cu:112-190,
+ * 281-364 in double): min(*d_outlier_count, max_outliers) outliers with
+ * distinct indices, in any order; ebx2 = 2*abs_eb in double.  d_codes and
+ * d_out 16-byte aligned.                                                    */
+size_t fzh_lorenzo_quant_inv_f64_ws_bytes(size_t n, size_t nx, size_t ny, size_t nz,
+                                          size_t max_outliers);
+int fzh_lorenzo_quant_inv_f64(void* stream, const void* d_codes, size_t n,
+                              size_t nx, size_t ny, size_t nz,
+                              const double* d_outlier_err,
+                              const uint32_t* d_outlier_idx,
+                              const uint32_t* d_outlier_count,
+                              size_t max_outliers, double ebx2, int zigzag,
+                              int code_bytes, double* d_out, void* d_ws,
+                              size_t ws_bytes);
+/* REL / NOA value base of FP64 input (predictor_utils.cuh:75-116 with
+ * TInput = double: min/max in double, the base cast to float) and the bound
+ * in double (lorenzo_quant.cu:612-644).  d_params (4 doubles):
+ *   [0] (double)value_base, [1] abs_eb = (double)user_eb * [0] (user_eb when
+ *   value_base <= 0), [2] ebx2_r = 1/(2*abs_eb), [3] ebx2 = 2*abs_eb.
+ * mode as fzh_value_base_f32 (| FZH_VB_ZERO_PADDED).  d_in 16-byte aligned. */
+size_t fzh_value_base_f64_ws_bytes(size_t n);
+int fzh_value_base_f64(void* stream, const double* d_in, size_t n, int mode,
+                       float user_eb, double* d_params, void* d_ws, size_t ws_bytes);
+
 /* ---- Bitshuffle (block_size multiple of 1024*elem_width; tail copied) ---- */
 int fzh_bitshuffle_enc(void* stream, const void* d_in, void* d_out,
                        size_t nbytes, uint32_t block_size, int elem_width);

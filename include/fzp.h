@@ -48,6 +48,11 @@ int fzp_destroy(fzp_pipeline* p);
 int fzp_add_lorenzo(fzp_pipeline* p, float error_bound, int eb_mode, uint32_t quant_radius,
                     float outlier_capacity, int zigzag, int code_bytes, float value_base,
                     int* stage_id);
+/* FP64 input: LorenzoQuantStage<double, uint16_t> (code_bytes 2) or
+ * <double, uint32_t> (code_bytes 4) (lorenzo_quant.h:335-339). */
+int fzp_add_lorenzo_f64(fzp_pipeline* p, float error_bound, int eb_mode, uint32_t quant_radius,
+                        float outlier_capacity, int zigzag, int code_bytes, float value_base,
+                        int* stage_id);
 /* Extension (no counterpart in the reference): strict error bound on a
  * LorenzoQuant stage -- codes and outliers unchanged, plus a patch list so
  * every decompressed value is within eb (LorenzoQuantStage::setStrictErrorBound). */
@@ -120,6 +125,9 @@ int fzp_last_stage_perf(fzp_pipeline* p, int i, char* name, size_t name_cap, flo
 int fzp_num_outputs(fzp_pipeline* p, int* n);
 int fzp_output_info(fzp_pipeline* p, int i, char* name, size_t name_cap, size_t* bytes);
 int fzp_stage_abs_error_bound(fzp_pipeline* p, int stage_id, float* abs_eb);
+/* The same bound in double: exact for an FP64 Lorenzo stage, whose bound is
+ * a double (lorenzo_quant.h:310). */
+int fzp_stage_abs_error_bound_f64(fzp_pipeline* p, int stage_id, double* abs_eb);
 
 #ifdef __cplusplus
 }

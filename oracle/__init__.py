@@ -57,6 +57,9 @@ def _declare(L):
         "orc_lorenzo_fwd_f32": (C.c_uint32, [_P, _Z, _Z, _Z, _Z, C.c_float, C.c_uint32, C.c_int, C.c_int, _P, _P, _P, _Z]),
         "orc_lorenzo_inv_f32": (None, [_P, _Z, _Z, _Z, _Z, _P, _P, C.c_uint32, _Z, C.c_float, C.c_int, C.c_int, _P]),
         "orc_value_base_f32": (C.c_float, [_P, _Z, C.c_int]),
+        "orc_lorenzo_fwd_f64": (C.c_uint32, [_P, _Z, _Z, _Z, _Z, C.c_double, C.c_uint32, C.c_int, C.c_int, _P, _P, _P, _Z]),
+        "orc_lorenzo_inv_f64": (None, [_P, _Z, _Z, _Z, _Z, _P, _P, C.c_uint32, _Z, C.c_double, C.c_int, C.c_int, _P]),
+        "orc_value_base_f64": (C.c_float, [_P, _Z, C.c_int]),
         "orc_bitshuffle_enc": (None, [_P, _P, _Z, C.c_uint32, C.c_int]),
         "orc_bitshuffle_dec": (None, [_P, _P, _Z, C.c_uint32, C.c_int]),
         "orc_rze_enc": (_Z, [_P, _Z, _P]),
@@ -98,41 +101,53 @@ _CODE_DT = {1: np.uint8, 2: np.uint16, 4: np.uint32}
 # ---------------------------------------------------------------------------
 def lorenzo_fwd(x: np.ndarray, dims, ebx2_r: float, radius: int, zigzag: bool,
                 code_bytes: int = 2, max_outliers: int | None = None, n: int | None = None):
-    """Returns (codes, err, idx, count).  ``dims`` = (nx, ny, nz)."""
-    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    """Returns (codes, err, idx, count).  ``dims`` = (nx, ny, nz).  A float64
+    ``x`` runs the FP64 path (double arithmetic, double outlier errors);
+    anything else is taken as float32."""
+    f64 = np.asarray(x).dtype == np.float64
+    ft = np.float64 if f64 else np.float32
+    x = np.ascontiguousarray(x, dtype=ft).ravel()
     n = x.size if n is None else n
     nx, ny, nz = dims
     if max_outliers is None:
         max_outliers = n
     codes = np.zeros(n, dtype=_CODE_DT[code_bytes])
-    err = np.zeros(max(max_outliers, 1), dtype=np.float32)
+    err = np.zeros(max(max_outliers, 1), dtype=ft)
     idx = np.zeros(max(max_outliers, 1), dtype=np.uint32)
-    cnt = lib().orc_lorenzo_fwd_f32(_ptr(x), n, nx, ny, nz, np.float32(ebx2_r), radius,
-                                    int(zigzag), code_bytes, _ptr(codes), _ptr(err), _ptr(idx),
-                                    max_outliers)
+    fn = lib().orc_lorenzo_fwd_f64 if f64 else lib().orc_lorenzo_fwd_f32
+    cnt = fn(_ptr(x), n, nx, ny, nz, ft(ebx2_r), radius, int(zigzag), code_bytes, _ptr(codes), _ptr(err),
+             _ptr(idx), max_outliers)
     k = min(cnt, max_outliers)
     return codes, err[:k].copy(), idx[:k].copy(), int(cnt)
 
 
 def lorenzo_inv(codes: np.ndarray, dims, err, idx, count: int, ebx2: float, zigzag: bool,
                 max_outliers: int | None = None) -> np.ndarray:
+    """Inverse; float64 outlier errors select the FP64 path (float64 output)."""
     codes = np.ascontiguousarray(codes)
     cb = codes.dtype.itemsize
     n = codes.size
     nx, ny, nz = dims
-    err = np.ascontiguousarray(err, dtype=np.float32)
+    f64 = np.asarray(err).dtype == np.float64
+    ft = np.float64 if f64 else np.float32
+    err = np.ascontiguousarray(err, dtype=ft)
     idx = np.ascontiguousarray(idx, dtype=np.uint32)
     if max_outliers is None:
         max_outliers = err.size
-    out = np.zeros(n, dtype=np.float32)
-    e = err if err.size else np.zeros(1, np.float32)
+    out = np.zeros(n, dtype=ft)
+    e = err if err.size else np.zeros(1, ft)
     i = idx if idx.size else np.zeros(1, np.uint32)
-    lib().orc_lorenzo_inv_f32(_ptr(codes), n, nx, ny, nz, _ptr(e), _ptr(i), count,
-                              min(max_outliers, err.size), np.float32(ebx2), int(zigzag), cb, _ptr(out))
+    fn = lib().orc_lorenzo_inv_f64 if f64 else lib().orc_lorenzo_inv_f32
+    fn(_ptr(codes), n, nx, ny, nz, _ptr(e), _ptr(i), count, min(max_outliers, err.size), ft(ebx2), int(zigzag), cb,
+       _ptr(out))
     return out
 
 
 def value_base(x: np.ndarray, mode: int) -> float:
+    """REL/NOA value base (float), in the input's precision for float64."""
+    if np.asarray(x).dtype == np.float64:
+        x = np.ascontiguousarray(x).ravel()
+        return float(np.float32(lib().orc_value_base_f64(_ptr(x), x.size, mode)))
     x = np.ascontiguousarray(x, dtype=np.float32).ravel()
     return float(np.float32(lib().orc_value_base_f32(_ptr(x), x.size, mode)))
 

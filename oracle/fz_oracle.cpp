@@ -39,19 +39,31 @@ inline S zz_dec(typename std::make_unsigned<S>::type u) {
     return (S)((U)(u >> 1) ^ (U)(-(S)(u & 1u)));
 }
 
-inline uint32_t encode_code(float delta, int zigzag, int code_bytes) {
-    int q = (int)delta;
+// (int)delta: exact for |delta| < 2^31; beyond that (reachable only with u32
+// codes and a radius above 2^31) the conversion saturates, as the GPU's
+// float-to-int conversion instructions do.
+template <typename T>
+inline int to_int_sat(T d) {
+    if (d >= (T)2147483647.0) return 2147483647;
+    if (d <= (T)-2147483648.0) return (int)(-2147483647 - 1);
+    return (int)d;
+}
+
+template <typename T>
+inline uint32_t encode_code(T delta, int zigzag, int code_bytes) {
+    int q = to_int_sat(delta);
     switch (code_bytes) {
     case 1: return zigzag ? (uint32_t)zz_enc<int8_t>((int8_t)q) : (uint32_t)(uint8_t)q;
     case 2: return zigzag ? (uint32_t)zz_enc<int16_t>((int16_t)q) : (uint32_t)(uint16_t)q;
     default: return zigzag ? (uint32_t)zz_enc<int32_t>((int32_t)q) : (uint32_t)q;
     }
 }
-inline float decode_code(uint32_t c, int zigzag, int code_bytes) {
+template <typename T>
+inline T decode_code(uint32_t c, int zigzag, int code_bytes) {
     switch (code_bytes) {
-    case 1: return zigzag ? (float)zz_dec<int8_t>((uint8_t)c) : (float)(int8_t)(uint8_t)c;
-    case 2: return zigzag ? (float)zz_dec<int16_t>((uint16_t)c) : (float)(int16_t)(uint16_t)c;
-    default: return zigzag ? (float)zz_dec<int32_t>(c) : (float)(int32_t)c;
+    case 1: return zigzag ? (T)zz_dec<int8_t>((uint8_t)c) : (T)(int8_t)(uint8_t)c;
+    case 2: return zigzag ? (T)zz_dec<int16_t>((uint16_t)c) : (T)(int16_t)(uint16_t)c;
+    default: return zigzag ? (T)zz_dec<int32_t>(c) : (T)(int32_t)c;
     }
 }
 inline uint32_t load_code(const void* codes, size_t i, int code_bytes) {
@@ -70,19 +82,24 @@ inline void store_code(void* codes, size_t i, uint32_t c, int code_bytes) {
 }
 // Radius is passed to the kernels as TCode (lorenzo_quant.cu:369-403), so it
 // wraps for narrow code types.
-inline float radius_as_tcode(uint32_t radius, int code_bytes) {
+template <typename T>
+inline T radius_as_tcode(uint32_t radius, int code_bytes) {
     switch (code_bytes) {
-    case 1: return (float)(uint8_t)radius;
-    case 2: return (float)(uint16_t)radius;
-    default: return (float)radius;
+    case 1: return (T)(uint8_t)radius;
+    case 2: return (T)(uint16_t)radius;
+    default: return (T)radius;
     }
 }
 
-struct Outlier { float err; uint32_t idx; };
+// One outlier: the prediction error in the input type (TInput in
+// lorenzo_quant.cu:270) and its element index.
+template <typename T>
+struct OutlierT { T err; uint32_t idx; };
 
 // Quantize one predicted delta (lorenzo_quant.cu:321-350; _nd.cu:82-102).
-inline void quantize_delta(float delta, float rad, int zigzag, int code_bytes,
-                           void* codes, size_t gid, std::vector<Outlier>& outl) {
+template <typename T>
+inline void quantize_delta(T delta, T rad, int zigzag, int code_bytes,
+                           void* codes, size_t gid, std::vector<OutlierT<T>>& outl) {
     if (std::fabs(delta) < rad) {
         store_code(codes, gid, encode_code(delta, zigzag, code_bytes), code_bytes);
     } else {
@@ -101,8 +118,9 @@ inline int nthreads() {
 }
 
 // Collect per-tile outlier vectors in tile order into the output arrays.
-uint32_t flush_outliers(const std::vector<std::vector<Outlier>>& per_tile,
-                        float* out_err, uint32_t* out_idx, size_t max_outliers) {
+template <typename T>
+uint32_t flush_outliers(const std::vector<std::vector<OutlierT<T>>>& per_tile,
+                        T* out_err, uint32_t* out_idx, size_t max_outliers) {
     size_t pos = 0;
     for (const auto& v : per_tile) {
         for (const auto& o : v) {
@@ -123,18 +141,19 @@ uint32_t flush_outliers(const std::vector<std::vector<Outlier>>& per_tile,
 // 1D: 1024-element tiles, chain restarts at every tile start
 // (lorenzo_quant.cu:287-312); s = round(x * r) with round() = half away
 // from zero (lorenzo_quant.cu:292).
-void lorenzo_fwd_1d(const float* in, size_t n, float r, float rad, int zz,
-                    int cb, void* codes, std::vector<std::vector<Outlier>>& per_tile) {
-    const size_t T = (n + 1023) / 1024;
-    per_tile.assign(T, {});
+template <typename T>
+void lorenzo_fwd_1d(const T* in, size_t n, T r, T rad, int zz,
+                    int cb, void* codes, std::vector<std::vector<OutlierT<T>>>& per_tile) {
+    const size_t nt = (n + 1023) / 1024;
+    per_tile.assign(nt, {});
 #pragma omp parallel for schedule(static) num_threads(nthreads())
-    for (long long t = 0; t < (long long)T; t++) {
+    for (long long t = 0; t < (long long)nt; t++) {
         const size_t base = (size_t)t * 1024;
         const size_t end = std::min(n, base + 1024);
-        float prev = 0.0f;
+        T prev = 0;
         for (size_t i = base; i < end; i++) {
-            float cur = std::round(in[i] * r);
-            float delta = cur - prev;
+            T cur = std::round(in[i] * r);
+            T delta = cur - prev;
             quantize_delta(delta, rad, zz, cb, codes, i, per_tile[t]);
             prev = cur;
         }
@@ -144,29 +163,30 @@ void lorenzo_fwd_1d(const float* in, size_t n, float r, float rad, int zz,
 // 2D: 32x32 tiles (lorenzo_quant_nd.cu:29-105).  Operation order per element:
 //   t(x,y) = s(x,y) - s(x,y-1)        (y-1 inside the tile, else 0)
 //   delta  = t(x,y) - t(x-1,y)        (only if x-1 inside the tile)
-void lorenzo_fwd_2d(const float* in, size_t nx, size_t ny, float r, float rad,
+template <typename T>
+void lorenzo_fwd_2d(const T* in, size_t nx, size_t ny, T r, T rad,
                     int zz, int cb, void* codes,
-                    std::vector<std::vector<Outlier>>& per_tile) {
+                    std::vector<std::vector<OutlierT<T>>>& per_tile) {
     const size_t gx = (nx + 31) / 32, gy = (ny + 31) / 32;
     per_tile.assign(gx * gy, {});
 #pragma omp parallel for schedule(static) num_threads(nthreads())
     for (long long t = 0; t < (long long)(gx * gy); t++) {
         const size_t bx = (size_t)t % gx, by = (size_t)t / gx;
-        float s[32][32];
+        T s[32][32];
         for (int ty = 0; ty < 32; ty++)
             for (int tx = 0; tx < 32; tx++) {
                 size_t x = bx * 32 + tx, y = by * 32 + ty;
-                s[ty][tx] = (x < nx && y < ny) ? std::round(in[y * nx + x] * r) : 0.0f;
+                s[ty][tx] = (x < nx && y < ny) ? std::round(in[y * nx + x] * r) : (T)0;
             }
-        float tv[32][32];
+        T tv[32][32];
         for (int ty = 0; ty < 32; ty++)
             for (int tx = 0; tx < 32; tx++)
-                tv[ty][tx] = s[ty][tx] - (ty > 0 ? s[ty - 1][tx] : 0.0f);
+                tv[ty][tx] = s[ty][tx] - (ty > 0 ? s[ty - 1][tx] : (T)0);
         for (int ty = 0; ty < 32; ty++)
             for (int tx = 0; tx < 32; tx++) {
                 size_t x = bx * 32 + tx, y = by * 32 + ty;
                 if (x >= nx || y >= ny) continue;
-                float d = tv[ty][tx];
+                T d = tv[ty][tx];
                 if (tx > 0) d = d - tv[ty][tx - 1];
                 quantize_delta(d, rad, zz, cb, codes, y * nx + x, per_tile[t]);
             }
@@ -179,26 +199,27 @@ void lorenzo_fwd_2d(const float* in, size_t nx, size_t ny, float r, float rad,
 //   d = b(x,y,z) - b(x,y-1,z)       (y-1 in tile; the reference multiplies
 //                                    the ghost row by 0, i.e. subtracts 0)
 // Canonical tile order: ((tz*GY + ty)*GX + tx) over 8-wide tiles.
-void lorenzo_fwd_3d(const float* in, size_t nx, size_t ny, size_t nz, float r,
-                    float rad, int zz, int cb, void* codes,
-                    std::vector<std::vector<Outlier>>& per_tile) {
+template <typename T>
+void lorenzo_fwd_3d(const T* in, size_t nx, size_t ny, size_t nz, T r,
+                    T rad, int zz, int cb, void* codes,
+                    std::vector<std::vector<OutlierT<T>>>& per_tile) {
     const size_t gx = (nx + 7) / 8, gy = (ny + 7) / 8, gz = (nz + 7) / 8;
     per_tile.assign(gx * gy * gz, {});
 #pragma omp parallel for schedule(static) num_threads(nthreads())
     for (long long t = 0; t < (long long)(gx * gy * gz); t++) {
         const size_t bx = (size_t)t % gx, by = ((size_t)t / gx) % gy, bz = (size_t)t / (gx * gy);
-        float s[8][8][8], a[8][8][8], b[8][8][8];
+        T s[8][8][8], a[8][8][8], b[8][8][8];
         for (int z = 0; z < 8; z++)
             for (int y = 0; y < 8; y++)
                 for (int x = 0; x < 8; x++) {
                     size_t X = bx * 8 + x, Y = by * 8 + y, Z = bz * 8 + z;
                     s[z][y][x] = (X < nx && Y < ny && Z < nz)
-                                     ? std::round(in[(Z * ny + Y) * nx + X] * r) : 0.0f;
+                                     ? std::round(in[(Z * ny + Y) * nx + X] * r) : (T)0;
                 }
         for (int z = 0; z < 8; z++)
             for (int y = 0; y < 8; y++)
                 for (int x = 0; x < 8; x++)
-                    a[z][y][x] = s[z][y][x] - (z > 0 ? s[z - 1][y][x] : 0.0f);
+                    a[z][y][x] = s[z][y][x] - (z > 0 ? s[z - 1][y][x] : (T)0);
         for (int z = 0; z < 8; z++)
             for (int y = 0; y < 8; y++)
                 for (int x = 0; x < 8; x++)
@@ -208,7 +229,7 @@ void lorenzo_fwd_3d(const float* in, size_t nx, size_t ny, size_t nz, float r,
                 for (int x = 0; x < 8; x++) {
                     size_t X = bx * 8 + x, Y = by * 8 + y, Z = bz * 8 + z;
                     if (X >= nx || Y >= ny || Z >= nz) continue;
-                    float d = y > 0 ? b[z][y][x] - b[z][y - 1][x] : b[z][y][x];
+                    T d = y > 0 ? b[z][y][x] - b[z][y - 1][x] : b[z][y][x];
                     quantize_delta(d, rad, zz, cb, codes, (Z * ny + Y) * nx + X, per_tile[t]);
                 }
     }
@@ -216,8 +237,9 @@ void lorenzo_fwd_3d(const float* in, size_t nx, size_t ny, size_t nz, float r,
 
 // Hillis-Steele inclusive scan with shuffle-up semantics:
 //   v'[i] = v[i] + v[i-off] for i >= off, offsets 1,2,4,... < len.
-inline void hs_scan(float* v, int len) {
-    float tmp[64];
+template <typename T>
+inline void hs_scan(T* v, int len) {
+    T tmp[64];
     for (int off = 1; off < len; off *= 2) {
         for (int i = 0; i < len; i++) tmp[i] = v[i];
         for (int i = off; i < len; i++) v[i] = tmp[i] + tmp[i - off];
@@ -233,20 +255,21 @@ inline void hs_scan(float* v, int len) {
 //   per thread: sequential sum of 4 consecutive deltas
 //   per warp:   Hillis-Steele over 32 thread totals, add lane-1's inclusive
 //   per block:  Hillis-Steele over 8 warp totals, add warp-1's inclusive
-void lorenzo_inv_1d_tile(float* d /*1024*/, float ebx2) {
-    float v[256][4];
+template <typename T>
+void lorenzo_inv_1d_tile(T* d /*1024*/, T ebx2) {
+    T v[256][4];
     for (int t = 0; t < 256; t++) {
         v[t][0] = d[4 * t];
         for (int i = 1; i < 4; i++) v[t][i] = v[t][i - 1] + d[4 * t + i];
     }
     for (int w = 0; w < 8; w++) {
-        float ts[32];
+        T ts[32];
         for (int l = 0; l < 32; l++) ts[l] = v[w * 32 + l][3];
         hs_scan(ts, 32);
         for (int l = 1; l < 32; l++)
             for (int i = 0; i < 4; i++) v[w * 32 + l][i] += ts[l - 1];
     }
-    float wt[8];
+    T wt[8];
     for (int w = 0; w < 8; w++) wt[w] = v[w * 32 + 31][3];
     hs_scan(wt, 8);
     for (int w = 1; w < 8; w++)
@@ -259,14 +282,15 @@ void lorenzo_inv_1d_tile(float* d /*1024*/, float ebx2) {
 // 2D tile scan of lorenzo_quant_nd.cu:112-190: y sequential within each
 // 8-row warp group, warp totals accumulated sequentially, then an x
 // Hillis-Steele scan over 32 lanes, then * ebx2.
-void lorenzo_inv_2d_tile(float d[32][32], float ebx2) {
+template <typename T>
+void lorenzo_inv_2d_tile(T d[32][32], T ebx2) {
     for (int x = 0; x < 32; x++) {
-        float T[4];
+        T tot[4];
         for (int w = 0; w < 4; w++) {
             for (int r = 1; r < 8; r++) d[w * 8 + r][x] += d[w * 8 + r - 1][x];
-            T[w] = d[w * 8 + 7][x];
+            tot[w] = d[w * 8 + 7][x];
         }
-        float acc[3] = {T[0], T[1], T[2]};
+        T acc[3] = {tot[0], tot[1], tot[2]};
         for (int i = 1; i < 3; i++) acc[i] += acc[i - 1];
         for (int w = 1; w < 4; w++)
             for (int r = 0; r < 8; r++) d[w * 8 + r][x] += acc[w - 1];
@@ -279,14 +303,15 @@ void lorenzo_inv_2d_tile(float d[32][32], float ebx2) {
 
 // 3D tile scan of lorenzo_quant_nd.cu:281-364: y sequential, then per y an
 // x Hillis-Steele scan over 8, then a z Hillis-Steele scan over 8, * ebx2.
-void lorenzo_inv_3d_tile(float d[8][8][8] /*[z][y][x]*/, float ebx2) {
+template <typename T>
+void lorenzo_inv_3d_tile(T d[8][8][8] /*[z][y][x]*/, T ebx2) {
     for (int z = 0; z < 8; z++)
         for (int x = 0; x < 8; x++)
             for (int y = 1; y < 8; y++) d[z][y][x] += d[z][y - 1][x];
     for (int y = 0; y < 8; y++) {
         for (int z = 0; z < 8; z++) hs_scan(d[z][y], 8);
         for (int x = 0; x < 8; x++) {
-            float col[8];
+            T col[8];
             for (int z = 0; z < 8; z++) col[z] = d[z][y][x];
             hs_scan(col, 8);
             for (int z = 0; z < 8; z++) d[z][y][x] = col[z];
@@ -295,6 +320,117 @@ void lorenzo_inv_3d_tile(float d[8][8][8] /*[z][y][x]*/, float ebx2) {
     for (int z = 0; z < 8; z++)
         for (int y = 0; y < 8; y++)
             for (int x = 0; x < 8; x++) d[z][y][x] *= ebx2;
+}
+
+template <typename T>
+uint32_t lorenzo_fwd_any(const T* in, size_t n, size_t nx, size_t ny, size_t nz, T ebx2_r, uint32_t radius,
+                         int zigzag, int code_bytes, void* codes, T* out_err, uint32_t* out_idx,
+                         size_t max_outliers) {
+    std::vector<std::vector<OutlierT<T>>> per_tile;
+    const T rad = radius_as_tcode<T>(radius, code_bytes);
+    if (n == 0) return 0;
+    if (nz > 1)
+        lorenzo_fwd_3d(in, nx, ny, nz, ebx2_r, rad, zigzag, code_bytes, codes, per_tile);
+    else if (ny > 1)
+        lorenzo_fwd_2d(in, nx, ny, ebx2_r, rad, zigzag, code_bytes, codes, per_tile);
+    else
+        lorenzo_fwd_1d(in, n, ebx2_r, rad, zigzag, code_bytes, codes, per_tile);
+    return flush_outliers(per_tile, out_err, out_idx, max_outliers);
+}
+
+template <typename T>
+void lorenzo_inv_any(const void* codes, size_t n, size_t nx, size_t ny, size_t nz, const T* err,
+                     const uint32_t* idx, uint32_t count, size_t max_outliers, T ebx2, int zigzag, int cb,
+                     T* out) {
+    if (n == 0) return;
+    // Scattered outlier deltas (lorenzo_quant.cu:426-450): zero + scatter of
+    // min(count, capacity) entries.
+    size_t total = (nz > 1) ? nx * ny * nz : (ny > 1) ? nx * ny : n;
+    std::vector<T> sc(total, (T)0);
+    const size_t k = std::min<size_t>(count, max_outliers);
+    for (size_t t = 0; t < k; t++)
+        if (idx[t] < total) sc[idx[t]] = err[t];
+
+    if (nz > 1) {
+        const size_t gx = (nx + 7) / 8, gy = (ny + 7) / 8, gz = (nz + 7) / 8;
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+        for (long long t = 0; t < (long long)(gx * gy * gz); t++) {
+            const size_t bx = (size_t)t % gx, by = ((size_t)t / gx) % gy, bz = (size_t)t / (gx * gy);
+            T d[8][8][8];
+            for (int z = 0; z < 8; z++)
+                for (int y = 0; y < 8; y++)
+                    for (int x = 0; x < 8; x++) {
+                        size_t X = bx * 8 + x, Y = by * 8 + y, Z = bz * 8 + z;
+                        if (X < nx && Y < ny && Z < nz) {
+                            size_t g = (Z * ny + Y) * nx + X;
+                            d[z][y][x] = sc[g] + decode_code<T>(load_code(codes, g, cb), zigzag, cb);
+                        } else {
+                            d[z][y][x] = (T)0;
+                        }
+                    }
+            lorenzo_inv_3d_tile(d, ebx2);
+            for (int z = 0; z < 8; z++)
+                for (int y = 0; y < 8; y++)
+                    for (int x = 0; x < 8; x++) {
+                        size_t X = bx * 8 + x, Y = by * 8 + y, Z = bz * 8 + z;
+                        if (X < nx && Y < ny && Z < nz) out[(Z * ny + Y) * nx + X] = d[z][y][x];
+                    }
+        }
+    } else if (ny > 1) {
+        const size_t gx = (nx + 31) / 32, gy = (ny + 31) / 32;
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+        for (long long t = 0; t < (long long)(gx * gy); t++) {
+            const size_t bx = (size_t)t % gx, by = (size_t)t / gx;
+            T d[32][32];
+            for (int y = 0; y < 32; y++)
+                for (int x = 0; x < 32; x++) {
+                    size_t X = bx * 32 + x, Y = by * 32 + y;
+                    if (X < nx && Y < ny) {
+                        size_t g = Y * nx + X;
+                        d[y][x] = sc[g] + decode_code<T>(load_code(codes, g, cb), zigzag, cb);
+                    } else {
+                        d[y][x] = (T)0;
+                    }
+                }
+            lorenzo_inv_2d_tile(d, ebx2);
+            for (int y = 0; y < 32; y++)
+                for (int x = 0; x < 32; x++) {
+                    size_t X = bx * 32 + x, Y = by * 32 + y;
+                    if (X < nx && Y < ny) out[Y * nx + X] = d[y][x];
+                }
+        }
+    } else {
+        const size_t nt = (n + 1023) / 1024;
+#pragma omp parallel for schedule(static) num_threads(nthreads())
+        for (long long t = 0; t < (long long)nt; t++) {
+            T d[1024];
+            const size_t base = (size_t)t * 1024;
+            for (int i = 0; i < 1024; i++) {
+                size_t g = base + i;
+                d[i] = g < n ? decode_code<T>(load_code(codes, g, cb), zigzag, cb) + sc[g] : (T)0;
+            }
+            lorenzo_inv_1d_tile(d, ebx2);
+            for (int i = 0; i < 1024; i++)
+                if (base + i < n) out[base + i] = d[i];
+        }
+    }
+}
+
+// predictor_utils.cuh:27-117 — min/max with '<' / '>' comparisons in the
+// input type, the value base returned as float (predictor_utils.cuh:110-115).
+template <typename T>
+float value_base_any(const T* in, size_t n, int mode) {
+    if (n == 0) return 0.0f;
+    T mn = in[0], mx = in[0];
+    for (size_t i = 1; i < n; i++) {
+        T v = in[i];
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+    }
+    if (mode == 2) return (float)(mx - mn);
+    T amin = mn < (T)0 ? -mn : mn;
+    T amax = mx < (T)0 ? -mx : mx;
+    return (float)(amin > amax ? amin : amax);
 }
 
 }  // namespace
@@ -312,110 +448,35 @@ uint32_t orc_lorenzo_fwd_f32(const float* in, size_t n, size_t nx, size_t ny,
                              int zigzag, int code_bytes, void* codes,
                              float* out_err, uint32_t* out_idx,
                              size_t max_outliers) {
-    std::vector<std::vector<Outlier>> per_tile;
-    const float rad = radius_as_tcode(radius, code_bytes);
-    if (n == 0) return 0;
-    if (nz > 1)
-        lorenzo_fwd_3d(in, nx, ny, nz, ebx2_r, rad, zigzag, code_bytes, codes, per_tile);
-    else if (ny > 1)
-        lorenzo_fwd_2d(in, nx, ny, ebx2_r, rad, zigzag, code_bytes, codes, per_tile);
-    else
-        lorenzo_fwd_1d(in, n, ebx2_r, rad, zigzag, code_bytes, codes, per_tile);
-    return flush_outliers(per_tile, out_err, out_idx, max_outliers);
+    return lorenzo_fwd_any<float>(in, n, nx, ny, nz, ebx2_r, radius, zigzag, code_bytes, codes, out_err, out_idx,
+                                  max_outliers);
+}
+
+uint32_t orc_lorenzo_fwd_f64(const double* in, size_t n, size_t nx, size_t ny,
+                             size_t nz, double ebx2_r, uint32_t radius,
+                             int zigzag, int code_bytes, void* codes,
+                             double* out_err, uint32_t* out_idx,
+                             size_t max_outliers) {
+    return lorenzo_fwd_any<double>(in, n, nx, ny, nz, ebx2_r, radius, zigzag, code_bytes, codes, out_err, out_idx,
+                                   max_outliers);
 }
 
 void orc_lorenzo_inv_f32(const void* codes, size_t n, size_t nx, size_t ny,
                          size_t nz, const float* err, const uint32_t* idx,
                          uint32_t count, size_t max_outliers, float ebx2,
                          int zigzag, int cb, float* out) {
-    if (n == 0) return;
-    // Scattered outlier deltas (lorenzo_quant.cu:426-450): zero + scatter of
-    // min(count, capacity) entries.
-    size_t total = (nz > 1) ? nx * ny * nz : (ny > 1) ? nx * ny : n;
-    std::vector<float> sc(total, 0.0f);
-    const size_t k = std::min<size_t>(count, max_outliers);
-    for (size_t t = 0; t < k; t++)
-        if (idx[t] < total) sc[idx[t]] = err[t];
-
-    if (nz > 1) {
-        const size_t gx = (nx + 7) / 8, gy = (ny + 7) / 8, gz = (nz + 7) / 8;
-#pragma omp parallel for schedule(static) num_threads(nthreads())
-        for (long long t = 0; t < (long long)(gx * gy * gz); t++) {
-            const size_t bx = (size_t)t % gx, by = ((size_t)t / gx) % gy, bz = (size_t)t / (gx * gy);
-            float d[8][8][8];
-            for (int z = 0; z < 8; z++)
-                for (int y = 0; y < 8; y++)
-                    for (int x = 0; x < 8; x++) {
-                        size_t X = bx * 8 + x, Y = by * 8 + y, Z = bz * 8 + z;
-                        if (X < nx && Y < ny && Z < nz) {
-                            size_t g = (Z * ny + Y) * nx + X;
-                            d[z][y][x] = sc[g] + decode_code(load_code(codes, g, cb), zigzag, cb);
-                        } else {
-                            d[z][y][x] = 0.0f;
-                        }
-                    }
-            lorenzo_inv_3d_tile(d, ebx2);
-            for (int z = 0; z < 8; z++)
-                for (int y = 0; y < 8; y++)
-                    for (int x = 0; x < 8; x++) {
-                        size_t X = bx * 8 + x, Y = by * 8 + y, Z = bz * 8 + z;
-                        if (X < nx && Y < ny && Z < nz) out[(Z * ny + Y) * nx + X] = d[z][y][x];
-                    }
-        }
-    } else if (ny > 1) {
-        const size_t gx = (nx + 31) / 32, gy = (ny + 31) / 32;
-#pragma omp parallel for schedule(static) num_threads(nthreads())
-        for (long long t = 0; t < (long long)(gx * gy); t++) {
-            const size_t bx = (size_t)t % gx, by = (size_t)t / gx;
-            float d[32][32];
-            for (int y = 0; y < 32; y++)
-                for (int x = 0; x < 32; x++) {
-                    size_t X = bx * 32 + x, Y = by * 32 + y;
-                    if (X < nx && Y < ny) {
-                        size_t g = Y * nx + X;
-                        d[y][x] = sc[g] + decode_code(load_code(codes, g, cb), zigzag, cb);
-                    } else {
-                        d[y][x] = 0.0f;
-                    }
-                }
-            lorenzo_inv_2d_tile(d, ebx2);
-            for (int y = 0; y < 32; y++)
-                for (int x = 0; x < 32; x++) {
-                    size_t X = bx * 32 + x, Y = by * 32 + y;
-                    if (X < nx && Y < ny) out[Y * nx + X] = d[y][x];
-                }
-        }
-    } else {
-        const size_t T = (n + 1023) / 1024;
-#pragma omp parallel for schedule(static) num_threads(nthreads())
-        for (long long t = 0; t < (long long)T; t++) {
-            float d[1024];
-            const size_t base = (size_t)t * 1024;
-            for (int i = 0; i < 1024; i++) {
-                size_t g = base + i;
-                d[i] = g < n ? decode_code(load_code(codes, g, cb), zigzag, cb) + sc[g] : 0.0f;
-            }
-            lorenzo_inv_1d_tile(d, ebx2);
-            for (int i = 0; i < 1024; i++)
-                if (base + i < n) out[base + i] = d[i];
-        }
-    }
+    lorenzo_inv_any<float>(codes, n, nx, ny, nz, err, idx, count, max_outliers, ebx2, zigzag, cb, out);
 }
 
-// predictor_utils.cuh:27-117 — min/max with '<' / '>' comparisons.
-float orc_value_base_f32(const float* in, size_t n, int mode) {
-    if (n == 0) return 0.0f;
-    float mn = in[0], mx = in[0];
-    for (size_t i = 1; i < n; i++) {
-        float v = in[i];
-        if (v < mn) mn = v;
-        if (v > mx) mx = v;
-    }
-    if (mode == 2) return mx - mn;
-    float amin = mn < 0.0f ? -mn : mn;
-    float amax = mx < 0.0f ? -mx : mx;
-    return amin > amax ? amin : amax;
+void orc_lorenzo_inv_f64(const void* codes, size_t n, size_t nx, size_t ny,
+                         size_t nz, const double* err, const uint32_t* idx,
+                         uint32_t count, size_t max_outliers, double ebx2,
+                         int zigzag, int cb, double* out) {
+    lorenzo_inv_any<double>(codes, n, nx, ny, nz, err, idx, count, max_outliers, ebx2, zigzag, cb, out);
 }
+
+float orc_value_base_f32(const float* in, size_t n, int mode) { return value_base_any<float>(in, n, mode); }
+float orc_value_base_f64(const double* in, size_t n, int mode) { return value_base_any<double>(in, n, mode); }
 
 // ---------------------------------------------------------------------------
 // Bitshuffle (bitshuffle_stage.cu).  Element widths 1/2 use the ballot

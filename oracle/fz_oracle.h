@@ -59,8 +59,23 @@ void orc_lorenzo_inv_f32(const void* codes, size_t n, size_t nx, size_t ny,
                          uint32_t count, size_t max_outliers, float ebx2,
                          int zigzag, int code_bytes, float* out);
 
-/* mode 1 = REL (max |x|), 2 = NOA (max - min). */
+/* FP64 input (LorenzoQuantStage<double, TCode>, lorenzo_quant.h:335-339):
+ * the same tiles, order and scan trees with every step in double; outlier
+ * errors are doubles. */
+uint32_t orc_lorenzo_fwd_f64(const double* in, size_t n, size_t nx, size_t ny,
+                             size_t nz, double ebx2_r, uint32_t radius,
+                             int zigzag, int code_bytes, void* codes,
+                             double* out_err, uint32_t* out_idx,
+                             size_t max_outliers);
+void orc_lorenzo_inv_f64(const void* codes, size_t n, size_t nx, size_t ny,
+                         size_t nz, const double* err, const uint32_t* idx,
+                         uint32_t count, size_t max_outliers, double ebx2,
+                         int zigzag, int code_bytes, double* out);
+
+/* mode 1 = REL (max |x|), 2 = NOA (max - min); computed in the input type
+ * and returned as float (predictor_utils.cuh:110-115). */
 float orc_value_base_f32(const float* in, size_t n, int mode);
+float orc_value_base_f64(const double* in, size_t n, int mode);
 
 /* ---- Bitshuffle ---------------------------------------------------------- */
 void orc_bitshuffle_enc(const uint8_t* in, uint8_t* out, size_t nbytes,

@@ -103,6 +103,51 @@ def lorenzo_huffman(x, dims, mode=1, user_eb=1e-4, radius=512, cap=0.2, zigzag=T
     return blob, dict(xp=xp, codes=codes, err=err, idx=idx, cnt=cnt, abs_eb=abs_eb, ebx2=ebx2)
 
 
+def resolve_eb64(x_padded: np.ndarray, mode: int, user_eb: float):
+    """(abs_eb, ebx2_r, ebx2) for FP64 input: the value base reduced in
+    double and cast to float, abs_eb = (double)user_eb * (double)base
+    (lorenzo_quant.cu:612-644 with TInput = double)."""
+    eb = float(np.float32(user_eb))
+    abs_eb = eb
+    if mode != 0:
+        vb = float(np.float32(orc.value_base(x_padded, mode)))
+        if vb > 0:
+            abs_eb = eb * vb
+    return abs_eb, 1.0 / (2.0 * abs_eb), 2.0 * abs_eb
+
+
+def lorenzo_part64(x, dims, mode, user_eb, radius, cap, zigzag, code_bytes, alignment):
+    """lorenzo_part for FP64 input (LorenzoQuantStage<double, TCode>)."""
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    padded = align(x.nbytes, alignment) // 8
+    xp = np.zeros(padded, np.float64)
+    xp[: x.size] = x
+    abs_eb, r, ebx2 = resolve_eb64(xp, mode, user_eb)
+    nx, ny, nz = dims
+    if ny == 1 and nz == 1:
+        nx = padded
+    m = max_outliers(padded, cap)
+    codes, err, idx, cnt = orc.lorenzo_fwd(xp, (nx, ny, nz), r, radius, zigzag, code_bytes, m)
+    return xp, codes, err, idx, cnt, abs_eb, ebx2
+
+
+def lorenzo64_chain(x, dims, coder, mode=1, user_eb=1e-3, radius=512, cap=0.2, zigzag=True, code_bytes=2,
+                    bklen=1024):
+    """FP64 LorenzoQuant -> coder ("rze": Bitshuffle(16 KiB, 2) + RZE;
+    "huffman": Huffman coarse; "rle": RLE), the pipeline's blob."""
+    alignment = 16384 if coder == "rze" else 1
+    xp, codes, err, idx, cnt, abs_eb, ebx2 = lorenzo_part64(x, dims, mode, user_eb, radius, cap, zigzag,
+                                                            code_bytes, alignment)
+    if coder == "rze":
+        head = orc.rze_enc(orc.bitshuffle_enc(codes.view(np.uint8), 16384, 2))
+    elif coder == "huffman":
+        head = orc.huffman_encode(codes, bklen)
+    else:
+        head = orc.rle_enc(codes)
+    blob = concat([head, err, idx, np.array([cnt], np.uint32)])
+    return blob, dict(xp=xp, codes=codes, err=err, idx=idx, cnt=cnt, abs_eb=abs_eb, ebx2=ebx2)
+
+
 def reconstruct(parts_info, dims, zigzag=True):
     """Oracle inverse of the Lorenzo part (padded length)."""
     p = parts_info
