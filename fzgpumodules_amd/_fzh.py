@@ -43,6 +43,9 @@ SIGNATURES = {
     "fzh_value_base_ws_bytes": (_Z, [_Z]),
     "fzh_value_base_f32": (_I, [_P, _P, _Z, _I, _F, _P, _P, _Z]),
     "fzh_value_base_f32_noalloc": (_I, [_P, _P, _Z, _I, _F, _P]),
+    "fzh_quant_rel_params": (None, [_F, _P]),
+    "fzh_quant_rel_fwd_f32": (_I, [_P, _P, _Z, _F, _F, _U32, _I, _P, _P, _P, _P, _Z, _P, _Z]),
+    "fzh_quant_rel_inv_f32": (_I, [_P, _P, _Z, _F, _I, _P, _P, _P, _Z, _P]),
     "fzh_lorenzo_quant_fwd_f64_ws_bytes": (_Z, [_Z, _Z, _Z, _Z]),
     "fzh_lorenzo_quant_fwd_f64": (_I, [_P, _P, _Z, _Z, _Z, _Z, _D, _P, _U32, _I, _I, _P, _P, _P, _P, _Z, _P, _Z]),
     "fzh_lorenzo_quant_inv_f64_ws_bytes": (_Z, [_Z, _Z, _Z, _Z, _Z]),

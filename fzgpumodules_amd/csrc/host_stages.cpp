@@ -614,8 +614,6 @@ std::unordered_map<std::string, size_t> QuantizerStage<TI, TC>::getActualOutputS
 template <typename TI, typename TC>
 void QuantizerStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
                                      const std::vector<void*>& outputs, const std::vector<size_t>& sizes) {
-    if (config_.eb_mode == ErrorBoundMode::REL)
-        throw std::runtime_error("QuantizerStage: REL (log-space) mode is not provided by this build");
     if (isInplaceMode()) {  // quantizer.cu:396-452 / 492-615, one port each way
         if (!config_.zigzag_codes)
             throw std::runtime_error("QuantizerStage: setInplaceOutliers(true) requires setZigzagCodes(true)");
@@ -638,6 +636,13 @@ void QuantizerStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const
             throw std::runtime_error("QuantizerStage (inverse): requires 4 inputs");
         const size_t n = sizes[0] / sizeof(TC);
         actual_output_sizes_.assign(1, n * sizeof(TI));
+        if (config_.eb_mode == ErrorBoundMode::REL) {  // quantizer.cu:426-436: log2eb from the user bound
+            throwIfFzh(fzh_quant_rel_inv_f32(stream, inputs[0], n, config_.error_bound, (int)sizeof(TC),
+                                             (const float*)inputs[1], (const uint32_t*)inputs[2],
+                                             (const uint32_t*)inputs[3], sizes[1] / sizeof(TI), (float*)outputs[0]),
+                       "QuantizerStage (inverse, REL)");
+            return;
+        }
         throwIfFzh(fzh_quant_abs_inv_f32(stream, inputs[0], n, 2.0f * computed_abs_eb_, config_.zigzag_codes ? 1 : 0,
                                          (int)sizeof(TC), (const float*)inputs[1], (const uint32_t*)inputs[2],
                                          (const uint32_t*)inputs[3], sizes[1] / sizeof(TI), (float*)outputs[0]),
@@ -653,6 +658,20 @@ void QuantizerStage<TI, TC>::execute(hipStream_t stream, MemoryPool* pool, const
     clearReadbacks();
     d_count_ptr_ = nullptr;
     if (n == 0) { actual_output_sizes_.assign(isInplaceMode() ? 1 : 4, 0); return; }
+    if (config_.eb_mode == ErrorBoundMode::REL) {  // quantizer.cu:530-533, 563-580
+        computed_abs_eb_ = 0.0f;
+        computed_value_base_ = 0.0f;
+        ws_.ensure(256 + fzh_quant_abs_fwd_ws_bytes(n), pool, stream);
+        throwIfFzh(fzh_quant_rel_fwd_f32(stream, (const float*)inputs[0], n, config_.error_bound,
+                                         config_.outlier_threshold, (uint32_t)config_.quant_radius, (int)sizeof(TC),
+                                         outputs[0], (float*)outputs[1], (uint32_t*)outputs[2], (uint32_t*)outputs[3],
+                                         m, (uint8_t*)ws_.ptr + 256, ws_.bytes - 256),
+                   "QuantizerStage (REL)");
+        d_count_ptr_ = outputs[3];
+        addReadback(d_count_ptr_, &h_count_, 4);
+        actual_output_sizes_ = {n * sizeof(TC), m * sizeof(TI), m * 4, 4};
+        return;
+    }
     if (config_.eb_mode == ErrorBoundMode::ABS) {
         computed_abs_eb_ = config_.error_bound;
         computed_value_base_ = 0.0f;

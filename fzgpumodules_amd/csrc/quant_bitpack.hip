@@ -1,7 +1,7 @@
 // quant_bitpack.hip — direct-value quantizer (ABS/NOA) and power-of-two bit
 // packing (gfx950).
 //
-// Quantizer (modules/quantizers/quantizer/quantizer.cu:30-67, 116-137):
+// Quantizer ABS/NOA (modules/quantizers/quantizer/quantizer.cu:30-67, 116-137):
 //   q = __float2int_rn(x * ebx2_r)  (round half to even, saturating)
 //   |q| < radius and |x| < threshold -> code = q (two's complement in TCode)
 //   or zigzag(q); otherwise code 0 and an outlier (x, i).
@@ -112,6 +112,113 @@ quant_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float ebx2, float*
         if constexpr (ZZ) q = (int)(SC)((TCode)(c >> 1) ^ (TCode)(-(SC)(c & 1u)));
         else q = (int)(SC)c;
         out[i] = (float)q * ebx2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// REL: log2-space bins (quantizer.cu:171-352).  For x normal, finite and
+// below the threshold: log_bin = __float2int_rn(log2|x| * log2eb_r), the
+// code packs (|log_bin|, sign of log_bin, sign of x) + 1 (0 marks an
+// outlier), and it is kept when |log_bin| < radius, it fits TCode and
+// 2^(log_bin * log2eb) lies in [|x| / (1+eps), |x| (1+eps)]; everything else
+// is an outlier.  The reference takes log2 and 2^t from __log2f / exp2f
+// (within 2 ulp); here both are the correctly rounded float results (the
+// double-precision functions rounded once), which is what the bin and the
+// bound test are defined on.
+// ---------------------------------------------------------------------------
+struct RelParams {
+    float log2eb, log2eb_r, opp, oopp;
+};
+
+__device__ __forceinline__ float log2_cr(float x) { return (float)log2((double)x); }
+__device__ __forceinline__ float exp2_cr(float t) { return (float)exp2((double)t); }
+
+// Code of one element (0: outlier).
+template <typename TCode>
+__device__ __forceinline__ uint32_t rel_code(float x, const RelParams& p, float threshold, uint32_t rad) {
+    const uint32_t bits = __float_as_uint(x);
+    const uint32_t expo = (bits >> 23) & 0xFFu;
+    if (fabsf(x) >= threshold || expo == 0u || expo == 255u) return 0u;
+    const bool neg = x < 0.0f;
+    const float ax = neg ? -x : x;
+    const int lb = f2i_rn(log2_cr(ax) * p.log2eb_r);
+    const uint32_t alb = lb < 0 ? 0u - (uint32_t)lb : (uint32_t)lb;
+    const uint32_t stored = ((alb << 2) | ((lb < 0 ? 1u : 0u) << 1) | (neg ? 1u : 0u)) + 1u;
+    constexpr uint32_t cmax = sizeof(TCode) == 1 ? 0xFFu : sizeof(TCode) == 2 ? 0xFFFFu : 0xFFFFFFFFu;
+    if (!(alb < rad && stored <= cmax)) return 0u;
+    const float rec = exp2_cr((float)lb * p.log2eb);
+    return (rec >= ax * p.oopp && rec <= ax * p.opp) ? stored : 0u;
+}
+
+template <typename TCode>
+__global__ void __launch_bounds__(256)
+quant_rel_fwd_kernel(const float* __restrict__ in, uint64_t n, RelParams p, float threshold, uint32_t rad,
+                     TCode* __restrict__ codes, float* __restrict__ sval, uint32_t* __restrict__ sidx,
+                     uint32_t* __restrict__ scnt, uint32_t T) {
+    const uint32_t tile = blockIdx.x * 4 + wave_in_wg();
+    if (tile >= T) return;
+    const int l = lane_id();
+    const uint64_t base = (uint64_t)tile * QTILE;
+    float x[16];
+    uint32_t mask = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
+        if (e0 + 4 <= n) {
+            const float4 v = *reinterpret_cast<const float4*>(in + e0);
+            x[4 * q] = v.x; x[4 * q + 1] = v.y; x[4 * q + 2] = v.z; x[4 * q + 3] = v.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) x[4 * q + k] = (e0 + k < n) ? in[e0 + k] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint64_t e = e0 + k;
+            const uint32_t c = rel_code<TCode>(x[4 * q + k], p, threshold, rad);
+            if (e < n) {
+                codes[e] = (TCode)c;
+                if (c == 0u) mask |= 1u << (4 * q + k);
+            }
+        }
+    }
+    const uint32_t agg = wave_sum(__popc(mask));
+    if (agg) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t mq = (mask >> (4 * q)) & 0xFu;
+            const uint32_t c = __popc(mq);
+            const uint32_t incl = wave_incl_scan(c);
+            uint32_t pos = run + incl - c;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (mq & (1u << k)) {
+                    sval[base + pos] = x[4 * q + k];
+                    sidx[base + pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
+                    pos++;
+                }
+            run += lane63(incl);
+        }
+    }
+    if (l == 0) scnt[tile] = agg;
+}
+
+// Inverse (quantizer.cu:328-352): code 0 -> 0 (an outlier's value is
+// assigned afterwards), else +-2^(log_bin * log2eb).
+template <typename TCode>
+__global__ void __launch_bounds__(256)
+quant_rel_inv_kernel(const TCode* __restrict__ codes, uint64_t n, float log2eb, float* __restrict__ out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t st = (uint32_t)codes[i];
+        float v = 0.0f;
+        if (st != 0u) {
+            const uint32_t cv = st - 1u;
+            const int alb = (int)(cv >> 2);
+            const int lb = ((cv >> 1) & 1u) ? -alb : alb;
+            const float rec = exp2_cr((float)lb * log2eb);
+            v = (cv & 1u) ? -rec : rec;
+        }
+        out[i] = v;
     }
 }
 
@@ -305,6 +412,89 @@ int fzh_quant_abs_inv_f32(void* stream, const void* d_codes, size_t n, float ebx
         scatter_assign_kernel<<<ob, 256, 0, s>>>(d_oval, d_oidx, d_ocount, max_outliers, n, d_out);
     }
     FZH_CHECK_LAUNCH("quantizer inverse");
+    return FZH_OK;
+}
+
+void fzh_quant_rel_params(float eps, float* params) {
+    const float log2eb = 2.0f * std::log2(1.0f + eps);  // quantizer.cu:563-568
+    params[0] = log2eb;
+    params[1] = 1.0f / log2eb;
+    params[2] = 1.0f + eps;
+    params[3] = 1.0f / (1.0f + eps);
+}
+
+int fzh_quant_rel_fwd_f32(void* stream, const float* d_in, size_t n, float eps, float threshold, uint32_t radius,
+                          int code_bytes, void* d_codes, float* d_oval, uint32_t* d_oidx, uint32_t* d_ocount,
+                          size_t max_outliers, void* d_ws, size_t ws_bytes) {
+    if (ws_bytes < fzh_quant_abs_fwd_ws_bytes(n)) return fail(FZH_E_CAPACITY, "quantizer (REL): workspace too small");
+    if (n == 0) return FZH_OK;
+    if (n > 0xFFFFFFFFull) return fail(FZH_E_INVALID, "quantizer (REL): more than 2^32 elements");
+    if (!d_in || !d_codes || !d_ocount) return fail(FZH_E_INVALID, "quantizer (REL): null pointer");
+    if ((uintptr_t)d_in % 16) return fail(FZH_E_INVALID, "quantizer (REL): input must be 16-byte aligned");
+    if (max_outliers > 0 && (!d_oval || !d_oidx)) return fail(FZH_E_INVALID, "quantizer (REL): null outlier buffers");
+    float pf[4];
+    fzh_quant_rel_params(eps, pf);
+    const RelParams p{pf[0], pf[1], pf[2], pf[3]};
+    const uint32_t T = (uint32_t)cdiv(n, QTILE);
+    const unsigned blocks = (unsigned)cdiv(T, 4);
+    uint8_t* w = (uint8_t*)d_ws;
+    const size_t o_cnt = compact_state_bytes(T);
+    const size_t o_val = o_cnt + align_up(4 * (size_t)T, 256);
+    const size_t o_idx = o_val + align_up(4 * (size_t)T * QTILE, 256);
+    uint32_t* scnt = (uint32_t*)(w + o_cnt);
+    float* sval = (float*)(w + o_val);
+    uint32_t* sidx = (uint32_t*)(w + o_idx);
+    hipStream_t s = S(stream);
+    switch (code_bytes) {
+    case 1:
+        launch_k("quant_rel_fwd_kernel", quant_rel_fwd_kernel<uint8_t>, dim3(blocks), dim3(256), 0, s, d_in, n, p,
+                 threshold, (uint32_t)(uint8_t)radius, (uint8_t*)d_codes, sval, sidx, scnt, T);
+        break;
+    case 2:
+        launch_k("quant_rel_fwd_kernel", quant_rel_fwd_kernel<uint16_t>, dim3(blocks), dim3(256), 0, s, d_in, n, p,
+                 threshold, (uint32_t)(uint16_t)radius, (uint16_t*)d_codes, sval, sidx, scnt, T);
+        break;
+    case 4:
+        launch_k("quant_rel_fwd_kernel", quant_rel_fwd_kernel<uint32_t>, dim3(blocks), dim3(256), 0, s, d_in, n, p,
+                 threshold, radius, (uint32_t*)d_codes, sval, sidx, scnt, T);
+        break;
+    default: return fail(FZH_E_INVALID, "quantizer (REL): code_bytes must be 1, 2 or 4");
+    }
+    FZH_CHECK_LAUNCH("quant_rel_fwd_kernel");
+    CompactList L0{sval, sidx, scnt, d_oval, d_oidx, d_ocount, max_outliers};
+    return launch_compact<float>(s, w, T, QTILE, L0, nullptr);
+}
+
+int fzh_quant_rel_inv_f32(void* stream, const void* d_codes, size_t n, float eps, int code_bytes,
+                          const float* d_oval, const uint32_t* d_oidx, const uint32_t* d_ocount, size_t max_outliers,
+                          float* d_out) {
+    if (n == 0) return FZH_OK;
+    if (!d_codes || !d_out) return fail(FZH_E_INVALID, "quantizer (REL) inverse: null pointer");
+    float pf[4];
+    fzh_quant_rel_params(eps, pf);
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256), 4096));
+    hipStream_t s = S(stream);
+    switch (code_bytes) {
+    case 1:
+        launch_k("quant_rel_inv_kernel", quant_rel_inv_kernel<uint8_t>, dim3(blocks), dim3(256), 0, s,
+                 (const uint8_t*)d_codes, n, pf[0], d_out);
+        break;
+    case 2:
+        launch_k("quant_rel_inv_kernel", quant_rel_inv_kernel<uint16_t>, dim3(blocks), dim3(256), 0, s,
+                 (const uint16_t*)d_codes, n, pf[0], d_out);
+        break;
+    case 4:
+        launch_k("quant_rel_inv_kernel", quant_rel_inv_kernel<uint32_t>, dim3(blocks), dim3(256), 0, s,
+                 (const uint32_t*)d_codes, n, pf[0], d_out);
+        break;
+    default: return fail(FZH_E_INVALID, "quantizer (REL): code_bytes must be 1, 2 or 4");
+    }
+    if (d_ocount && max_outliers > 0) {
+        const unsigned ob = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(max_outliers, 256), 1024));
+        launch_k("scatter_assign_kernel", scatter_assign_kernel, dim3(ob), dim3(256), 0, s, d_oval, d_oidx, d_ocount,
+                 (uint64_t)max_outliers, (uint64_t)n, d_out);
+    }
+    FZH_CHECK_LAUNCH("quantizer (REL) inverse");
     return FZH_OK;
 }
 

@@ -262,6 +262,37 @@ def quant_abs_inv(codes, ebx2, zigzag, vals, idx, count):
     return out.download(np.float32, codes.size)
 
 
+def quant_rel_fwd(x, eps, radius, code_bytes=2, threshold=np.inf, max_outliers=None):
+    """Quantizer REL forward (fzh_quant_rel_fwd_f32)."""
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    if max_outliers is None:
+        max_outliers = x.size
+    din = DeviceBuffer.from_array(x)
+    codes = DeviceBuffer(x.size * code_bytes)
+    val = DeviceBuffer(max(max_outliers, 1) * 4)
+    idx = DeviceBuffer(max(max_outliers, 1) * 4)
+    cnt = DeviceBuffer(4)
+    ws = workspace(lib().fzh_quant_abs_fwd_ws_bytes(x.size))
+    call("fzh_quant_rel_fwd_f32", None, din.ptr, x.size, np.float32(eps), np.float32(threshold), radius, code_bytes,
+         codes.ptr, val.ptr, idx.ptr, cnt.ptr, max_outliers, ws.ptr, ws.nbytes)
+    c = int(cnt.download(np.uint32, 1)[0])
+    k = min(c, max_outliers)
+    return codes.download(_CODE_DT[code_bytes], x.size), val.download(np.float32, k), idx.download(np.uint32, k), c
+
+
+def quant_rel_inv(codes, eps, vals, idx, count):
+    """Quantizer REL inverse (fzh_quant_rel_inv_f32)."""
+    codes = np.ascontiguousarray(codes)
+    dc = DeviceBuffer.from_array(codes)
+    dv = DeviceBuffer.from_array(np.ascontiguousarray(vals, np.float32) if len(vals) else np.zeros(1, np.float32))
+    di = DeviceBuffer.from_array(np.ascontiguousarray(idx, np.uint32) if len(idx) else np.zeros(1, np.uint32))
+    dn = DeviceBuffer.from_array(np.array([count], np.uint32))
+    out = DeviceBuffer(codes.size * 4)
+    call("fzh_quant_rel_inv_f32", None, dc.ptr, codes.size, np.float32(eps), codes.dtype.itemsize, dv.ptr, di.ptr,
+         dn.ptr, len(vals), out.ptr)
+    return out.download(np.float32, codes.size)
+
+
 def scan_u32(v, wrap=False, clear=False, in_place=False):
     """Exclusive scan through fzh_scan_u32: (prefixes, total, input after the call)."""
     v = np.ascontiguousarray(v, np.uint32)

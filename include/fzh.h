@@ -319,6 +319,31 @@ int fzh_quant_abs_fwd_inplace_f32(void* stream, const float* d_in, size_t n,
 int fzh_quant_abs_inv_inplace_f32(void* stream, const uint32_t* d_codes, size_t n,
                                   float ebx2, uint32_t radius, float* d_out);
 
+/* ---- Quantizer (REL, log2-space bins) --------------------------------------
+ * quantizer_rel_fwd_kernel / quantizer_rel_inv_kernel (quantizer.cu:171-352)
+ * with the stage's constants (quantizer.cu:563-568): log2eb =
+ * 2*log2f(1+eps), 1/log2eb, 1+eps, 1/(1+eps) (fzh_quant_rel_params).  A
+ * normal finite x with |x| < threshold gets log_bin = rint(log2|x| *
+ * (1/log2eb)) and the code ((|log_bin| << 2) | (log_bin < 0) << 1 |
+ * (x < 0)) + 1 when |log_bin| < radius, the code fits the code type and
+ * 2^(log_bin*log2eb) is within [|x|/(1+eps), |x|(1+eps)]; otherwise code 0
+ * and an outlier (x, i), listed in index order.  log2 and 2^t are the
+ * correctly rounded float results (the reference's __log2f / exp2f are
+ * within 2 ulp of them).  Same workspace as fzh_quant_abs_fwd_f32.  The
+ * inverse writes +-2^(log_bin*log2eb), 0 for code 0, then assigns the
+ * outliers.                                                                */
+void fzh_quant_rel_params(float eps, float* params4);
+int fzh_quant_rel_fwd_f32(void* stream, const float* d_in, size_t n, float eps,
+                          float threshold, uint32_t radius, int code_bytes,
+                          void* d_codes, float* d_outlier_val,
+                          uint32_t* d_outlier_idx, uint32_t* d_outlier_count,
+                          size_t max_outliers, void* d_ws, size_t ws_bytes);
+int fzh_quant_rel_inv_f32(void* stream, const void* d_codes, size_t n, float eps,
+                          int code_bytes, const float* d_outlier_val,
+                          const uint32_t* d_outlier_idx,
+                          const uint32_t* d_outlier_count, size_t max_outliers,
+                          float* d_out);
+
 /* ---- Bitpack -------------------------------------------------------------- */
 size_t fzh_max_u32_ws_bytes(void);
 int fzh_max_u32(void* stream, const void* d_in, size_t n, int elem_bytes,

@@ -75,6 +75,9 @@ def _declare(L):
         "orc_huffman_decode": (C.c_int, [_P, _Z, C.c_int, _P, _Z]),
         "orc_quant_abs_fwd_f32": (C.c_uint32, [_P, _Z, C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_int, _P, _P, _P, _Z]),
         "orc_quant_abs_inv_f32": (None, [_P, _Z, C.c_float, C.c_int, C.c_int, _P, _P, C.c_uint32, _Z, _P]),
+        "orc_quant_rel_params": (None, [C.c_float, _P]),
+        "orc_quant_rel_fwd_f32": (C.c_uint32, [_P, _Z, C.c_float, C.c_float, C.c_uint32, C.c_int, _P, _P, _P, _Z]),
+        "orc_quant_rel_inv_f32": (None, [_P, _Z, C.c_float, C.c_int, _P, _P, C.c_uint32, _Z, _P]),
         "orc_bitpack_nbits_for_max": (C.c_int, [C.c_uint32, C.c_int]),
         "orc_bitpack_enc": (_Z, [_P, _Z, C.c_int, C.c_int, _P]),
         "orc_bitpack_dec": (None, [_P, _Z, C.c_int, C.c_int, _P]),
@@ -282,6 +285,39 @@ def quant_abs_inv(codes, ebx2, zigzag, vals, idx, count):
     i = idx if idx.size else np.zeros(1, np.uint32)
     lib().orc_quant_abs_inv_f32(_ptr(codes), codes.size, np.float32(ebx2), int(zigzag), codes.dtype.itemsize,
                                 _ptr(v), _ptr(i), count, vals.size, _ptr(out))
+    return out
+
+
+def quant_rel_params(eps: float) -> np.ndarray:
+    """[log2eb, 1/log2eb, 1+eps, 1/(1+eps)] as float32 (quantizer.cu:563-568)."""
+    p = np.zeros(4, np.float32)
+    lib().orc_quant_rel_params(np.float32(eps), _ptr(p))
+    return p
+
+
+def quant_rel_fwd(x, eps, radius, code_bytes=2, threshold=np.inf, max_outliers=None):
+    """Quantizer REL forward: (codes, outlier values, indices, count)."""
+    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    if max_outliers is None:
+        max_outliers = x.size
+    codes = np.zeros(x.size, dtype=_CODE_DT[code_bytes])
+    val = np.zeros(max(max_outliers, 1), np.float32)
+    idx = np.zeros(max(max_outliers, 1), np.uint32)
+    cnt = lib().orc_quant_rel_fwd_f32(_ptr(x), x.size, np.float32(eps), np.float32(threshold), radius, code_bytes,
+                                      _ptr(codes), _ptr(val), _ptr(idx), max_outliers)
+    k = min(cnt, max_outliers)
+    return codes, val[:k].copy(), idx[:k].copy(), int(cnt)
+
+
+def quant_rel_inv(codes, eps, vals, idx, count):
+    codes = np.ascontiguousarray(codes)
+    out = np.zeros(codes.size, np.float32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    v = vals if vals.size else np.zeros(1, np.float32)
+    i = idx if idx.size else np.zeros(1, np.uint32)
+    lib().orc_quant_rel_inv_f32(_ptr(codes), codes.size, np.float32(eps), codes.dtype.itemsize, _ptr(v), _ptr(i),
+                                count, vals.size, _ptr(out))
     return out
 
 

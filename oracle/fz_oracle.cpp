@@ -1191,6 +1191,79 @@ void orc_quant_abs_inv_f32(const void* codes, size_t n, float ebx2, int zigzag, 
         if (idx[t] < n) out[idx[t]] = vals[t];
 }
 
+// ---------------------------------------------------------------------------
+// Quantizer REL (quantizer.cu:171-352, 426-436, 563-580): log2-space bins.
+// The reference evaluates log2 with __log2f and 2^t with exp2f (CUDA,
+// within 2 ulp); this restatement uses the correctly rounded float results
+// (the double-precision functions, rounded once to float), as the HIP
+// kernels do.  The host constants are the reference's float expressions.
+// ---------------------------------------------------------------------------
+void orc_quant_rel_params(float eps, float* p /*[4]*/) {
+    const float log2eb = 2.0f * std::log2(1.0f + eps);  // quantizer.cu:565
+    p[0] = log2eb;
+    p[1] = 1.0f / log2eb;
+    p[2] = 1.0f + eps;
+    p[3] = 1.0f / (1.0f + eps);
+}
+
+static inline float log2_cr(float x) { return (float)std::log2((double)x); }
+static inline float exp2_cr(float t) { return (float)std::exp2((double)t); }
+
+uint32_t orc_quant_rel_fwd_f32(const float* in, size_t n, float eps, float threshold, uint32_t radius, int cb,
+                               void* codes, float* out_val, uint32_t* out_idx, size_t max_outliers) {
+    float p[4];
+    orc_quant_rel_params(eps, p);
+    const float log2eb = p[0], log2eb_r = p[1], opp = p[2], oopp = p[3];
+    const uint32_t rad = cb == 1 ? (uint8_t)radius : cb == 2 ? (uint16_t)radius : radius;
+    const uint32_t cmax = cb == 1 ? 0xFFu : cb == 2 ? 0xFFFFu : 0xFFFFFFFFu;
+    uint32_t cnt = 0;
+    for (size_t i = 0; i < n; i++) {
+        const float x = in[i];
+        uint32_t bits;
+        std::memcpy(&bits, &x, 4);
+        const uint32_t expo = (bits >> 23) & 0xFFu;
+        bool ok = false;
+        uint32_t stored = 0;
+        if (!(std::fabs(x) >= threshold) && expo != 0 && expo != 255) {  // quantizer.cu:249-275
+            const bool neg = x < 0.0f;
+            const float ax = neg ? -x : x;
+            const int lb = f2i_rn(log2_cr(ax) * log2eb_r);
+            const uint32_t alb = lb < 0 ? 0u - (uint32_t)lb : (uint32_t)lb;
+            stored = ((alb << 2) | ((lb < 0 ? 1u : 0u) << 1) | (neg ? 1u : 0u)) + 1u;  // quantizer.cu:224-230
+            if (alb < rad && stored <= cmax) {
+                const float rec = exp2_cr((float)lb * log2eb);
+                ok = rec >= ax * oopp && rec <= ax * opp;  // quantizer.cu:300-311
+            }
+        }
+        if (ok) {
+            store_code(codes, i, stored, cb);
+        } else {
+            store_code(codes, i, 0, cb);
+            if (cnt < max_outliers) { out_val[cnt] = x; out_idx[cnt] = (uint32_t)i; }
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
+void orc_quant_rel_inv_f32(const void* codes, size_t n, float eps, int cb, const float* vals, const uint32_t* idx,
+                           uint32_t count, size_t max_outliers, float* out) {
+    float p[4];
+    orc_quant_rel_params(eps, p);
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t st = ld(codes, i, cb);
+        if (st == 0) { out[i] = 0.0f; continue; }  // quantizer.cu:340-342
+        const uint32_t cv = st - 1u;
+        const int alb = (int)(cv >> 2);
+        const int lb = ((cv >> 1) & 1u) ? -alb : alb;
+        const float rec = exp2_cr((float)lb * p[0]);
+        out[i] = (cv & 1u) ? -rec : rec;
+    }
+    const size_t k = std::min<size_t>(count, max_outliers);
+    for (size_t t = 0; t < k; t++)
+        if (idx[t] < n) out[idx[t]] = vals[t];
+}
+
 int orc_bitpack_nbits_for_max(uint32_t max_val, int eb) {
     const int full = 8 * eb;
     for (int b = 1; b < full; b *= 2)

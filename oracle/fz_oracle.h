@@ -26,6 +26,8 @@
  *                          phf/hf_buf.cc:151-176
  *   orc_huffman_decode     phf/hf_kernels.cu:258-312
  *   orc_quant_abs_*        modules/quantizers/quantizer/quantizer.cu:30-67, 116-137
+ *   orc_quant_rel_*        quantizer.cu:171-352, 426-436, 563-580 (log2/exp2 correctly
+ *                          rounded to float where the reference uses __log2f/exp2f)
  *   orc_bitpack_*          modules/coders/bitpack/bitpack_stage.cu:30-124
  *   orc_crc32              src/pipeline/compressor_io.cpp:22-68
  *
@@ -125,6 +127,14 @@ uint32_t orc_quant_abs_fwd_f32(const float* in, size_t n, float ebx2_r,
 void orc_quant_abs_inv_f32(const void* codes, size_t n, float ebx2, int zigzag,
                            int code_bytes, const float* vals,
                            const uint32_t* idx, uint32_t count,
+                           size_t max_outliers, float* out);
+/* Quantizer REL (log2-space bins): [log2eb, 1/log2eb, 1+eps, 1/(1+eps)]. */
+void orc_quant_rel_params(float eps, float* params4);
+uint32_t orc_quant_rel_fwd_f32(const float* in, size_t n, float eps, float threshold,
+                               uint32_t radius, int code_bytes, void* codes,
+                               float* out_val, uint32_t* out_idx, size_t max_outliers);
+void orc_quant_rel_inv_f32(const void* codes, size_t n, float eps, int code_bytes,
+                           const float* vals, const uint32_t* idx, uint32_t count,
                            size_t max_outliers, float* out);
 int orc_bitpack_nbits_for_max(uint32_t max_val, int elem_bytes);
 size_t orc_bitpack_enc(const void* in, size_t n, int elem_bytes, int nbits,
