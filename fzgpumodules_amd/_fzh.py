@@ -69,6 +69,7 @@ SIGNATURES = {
     "fzh_histogram": (_I, [_P, _P, _Z, _I, _U32, _P]),
     "fzh_huffman_revbook_bytes": (_Z, [_U32, _I]),
     "fzh_huffman_build_book": (_I, [_P, _U32, _I, _P, _P]),
+    "fzh_huffman_build_book_device": (_I, [_P, _P, _U32, _I, C.c_uint64, _P, _P, _P]),
     "fzh_huffman_enc_bound": (_Z, [_Z, _U32, _I]),
     "fzh_huffman_enc_ws_bytes": (_Z, [_Z]),
     "fzh_huffman_encode": (_I, [_P, _P, _Z, _I, _U32, _P, _P, _P, _Z, _P, _P, _Z]),
@@ -160,6 +161,7 @@ SIGNATURES = {
     "fzh_kernel_timing_get": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "fzp_add_difference": (_I, [_P, _I, _I, _Z, C.POINTER(_I)]),
     "fzp_set_huffman_fine": (_I, [_P, _I, _I]),
+    "fzp_set_huffman_device_book": (_I, [_P, _I, _I]),
     "fzp_add_zigzag": (_I, [_P, _I, C.POINTER(_I)]),
     "fzp_add_negabinary": (_I, [_P, _I, C.POINTER(_I)]),
     "fzp_add_lorenzo_int": (_I, [_P, _I, C.POINTER(_I)]),

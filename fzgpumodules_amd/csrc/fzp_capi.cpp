@@ -326,6 +326,17 @@ int fzp_set_huffman_fine(fzp_pipeline* p, int sid, int fine) {
     });
 }
 
+int fzp_set_huffman_device_book(fzp_pipeline* p, int sid, int on) {
+    return guard([&] {
+        need(p);
+        fz::Stage* s = stage_at(p, sid);
+        if (auto* h = dynamic_cast<fz::HuffmanStage<uint8_t>*>(s)) h->setDeviceCodebook(on != 0);
+        else if (auto* h2 = dynamic_cast<fz::HuffmanStage<uint16_t>*>(s)) h2->setDeviceCodebook(on != 0);
+        else if (auto* h4 = dynamic_cast<fz::HuffmanStage<uint32_t>*>(s)) h4->setDeviceCodebook(on != 0);
+        else throw std::invalid_argument("stage is not a Huffman stage");
+    });
+}
+
 int fzp_connect(fzp_pipeline* p, int dep, int prod, const char* name) {
     return guard([&] {
         need(p);

@@ -517,10 +517,36 @@ void HuffmanStage<T>::execute(hipStream_t stream, MemoryPool* pool, const std::v
     if (inputs.empty() || outputs.empty() || sizes.empty())
         throw std::runtime_error("HuffmanStage: inputs, outputs, and sizes must be non-empty");
     clearReadbacks();
-    size_pending_ = status_pending_ = false;
+    size_pending_ = status_pending_ = book_pending_ = false;
     if (sizes[0] == 0) { actual_output_size_ = 0; return; }
     small_.ensure(256, pool, stream);
-    if (!is_inverse_) {
+    if (!is_inverse_ && device_book_) {
+        // histogram -> codebook -> encode, all on the stream; the codebook
+        // status (the host path's out-of-range check) is read after the sync
+        if (encode_mode_ == HuffmanEncodeMode::Fine)
+            throw std::runtime_error("HuffmanStage: the device codebook serves the coarse encoding only "
+                                     "(the fine/coarse choice reads the codebook on the host)");
+        const size_t n = sizes[0] / sizeof(T);
+        const size_t rev_bytes = fzh_huffman_revbook_bytes(bklen_, sizeof(T));
+        freq_.ensure(4 * (size_t)bklen_, pool, stream);
+        book_.ensure(4 * (size_t)bklen_ + rev_bytes, pool, stream);
+        ws_.ensure(fzh_huffman_enc_ws_bytes(n), pool, stream);
+        throwIfFzh(fzh_histogram(stream, inputs[0], n, (int)sizeof(T), bklen_, (uint32_t*)freq_.ptr), "HuffmanStage histogram");
+        throwIfFzh(fzh_huffman_build_book_device(stream, (const uint32_t*)freq_.ptr, bklen_, (int)sizeof(T), n,
+                                                 (uint32_t*)book_.ptr, (uint8_t*)book_.ptr + 4 * (size_t)bklen_,
+                                                 (int*)((uint8_t*)small_.ptr + 16)),
+                   "HuffmanStage device codebook");
+        throwIfFzh(fzh_huffman_encode(stream, inputs[0], n, (int)sizeof(T), bklen_, (const uint32_t*)book_.ptr,
+                                      (const uint8_t*)book_.ptr + 4 * (size_t)bklen_, outputs[0],
+                                      fzh_huffman_enc_bound(n, bklen_, sizeof(T)), (uint64_t*)small_.ptr, ws_.ptr,
+                                      ws_.bytes),
+                   "HuffmanStage encode");
+        original_len_ = n;
+        actual_output_size_ = fzh_huffman_enc_bound(n, bklen_, sizeof(T));
+        size_pending_ = book_pending_ = true;
+        addReadback(small_.ptr, &h_size_, 8);
+        addReadback((uint8_t*)small_.ptr + 16, &h_book_status_, 4);
+    } else if (!is_inverse_) {
         const size_t n = sizes[0] / sizeof(T);
         freq_.ensure(4 * (size_t)bklen_, pool, stream);
         throwIfFzh(fzh_histogram(stream, inputs[0], n, (int)sizeof(T), bklen_, (uint32_t*)freq_.ptr), "HuffmanStage histogram");
@@ -573,6 +599,14 @@ template <typename T>
 void HuffmanStage<T>::postStreamSync(hipStream_t) {
     if (!size_pending_ && !status_pending_) return;
     completeReadbacks();
+    if (book_pending_ && h_book_status_ != 0) {
+        size_pending_ = book_pending_ = false;
+        if (h_book_status_ == 1)
+            throw std::runtime_error("HuffmanStage: out-of-range symbol(s) detected; all symbols must be in [0, " +
+                                     std::to_string(bklen_) +
+                                     "). Increase bklen or use setZigzagCodes(true) with LorenzoQuantStage.");
+        throw std::runtime_error("HuffmanStage: device codebook: a code would exceed 58 bits");
+    }
     if (size_pending_) {
         if (h_size_ == 0) throw std::runtime_error("HuffmanStage: device prefix failed (workspace state)");
         actual_output_size_ = h_size_;

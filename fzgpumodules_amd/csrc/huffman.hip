@@ -127,7 +127,10 @@ huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint3
     for (int k = 0; k < K; k++) {
         const uint32_t j = l + 64 * k;
         uint32_t w = 0;
-        if (j < cnt) w = book[ld_sym(in, s0 + j)];
+        if (j < cnt) {
+            const uint32_t sym = ld_sym(in, s0 + j);
+            if (sym < bklen) w = book[sym];  // an out-of-range symbol (reported by its caller) gets no code
+        }
         len[k] = j < cnt ? (w >> 27) : 0u;
         code[k] = w & 0x07FFFFFFu;
     }

@@ -200,6 +200,21 @@ def huffman_book(freq, elem_bytes=2):
     return bk4, rev
 
 
+def huffman_book_device(freq, elem_bytes=2, n_expected=None):
+    """The codebook built on the device from a histogram: (bk4, revbook, status)."""
+    freq = np.ascontiguousarray(freq, np.uint32)
+    if n_expected is None:
+        n_expected = int(freq.sum(dtype=np.uint64))
+    df = DeviceBuffer.from_array(freq)
+    dbk = DeviceBuffer(freq.size * 4)
+    nrev = lib().fzh_huffman_revbook_bytes(freq.size, elem_bytes)
+    drev = DeviceBuffer(nrev)
+    st = DeviceBuffer.zeros(4)
+    call("fzh_huffman_build_book_device", None, df.ptr, freq.size, elem_bytes, int(n_expected),
+         dbk.ptr, drev.ptr, st.ptr)
+    return dbk.download(np.uint32, freq.size), drev.download(np.uint8, nrev), int(st.download(np.int32, 1)[0])
+
+
 def huffman_encode(v, bklen, fine=False):
     """Histogram on the GPU, codebook on the host, encode on the GPU (fine:
     1024-symbol partitions when no codeword exceeds 8 bits, as HuffmanStage's

@@ -110,6 +110,10 @@ class Pipeline:
     def set_huffman_fine(self, stage_id: int, on: bool = True) -> None:
         call("fzp_set_huffman_fine", self.h, int(stage_id), int(bool(on)))
 
+    def set_huffman_device_book(self, stage_id: int, on: bool = True) -> None:
+        """Build the Huffman codebook on the device (coarse encoding)."""
+        call("fzp_set_huffman_device_book", self.h, int(stage_id), int(bool(on)))
+
     def add_difference(self, dtype="f32", negabinary=False, chunk_bytes=0) -> int:
         return self._id("fzp_add_difference", dtype_code(dtype), int(bool(negabinary)), int(chunk_bytes))
 

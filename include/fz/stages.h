@@ -427,8 +427,16 @@ class HuffmanStage : public Stage {
     HuffmanEncodeMode getEncodeMode() const { return encode_mode_; }
     void setInverse(bool inv) override { is_inverse_ = inv; }
     bool isInverse() const override { return is_inverse_; }
-    // The histogram has to reach the host-side codebook builder.
-    bool isGraphCompatible() const override { return is_inverse_; }
+    // Extension: build the codebook on the device (coarse encoding, bklen <=
+    // 1024), so the forward path enqueues device work only and can be
+    // captured in a graph; the stream is byte for byte the host-book one.
+    // Out-of-range symbols are then reported after the stream synchronises.
+    // The device build is a serial heap replay: slower than the host
+    // builder (ms for a full 1024-symbol alphabet), chosen for capture.
+    void setDeviceCodebook(bool on) { device_book_ = on; }
+    bool getDeviceCodebook() const { return device_book_; }
+    // With the host codebook the histogram has to reach the host builder.
+    bool isGraphCompatible() const override { return is_inverse_ || device_book_; }
 
     void execute(hipStream_t stream, MemoryPool* pool, const std::vector<void*>& inputs,
                  const std::vector<void*>& outputs, const std::vector<size_t>& sizes) override;
@@ -475,9 +483,10 @@ class HuffmanStage : public Stage {
     HuffmanEncodeMode encode_mode_ = HuffmanEncodeMode::Coarse;
     uint64_t original_len_ = 0;
     size_t actual_output_size_ = 0;
-    bool size_pending_ = false, status_pending_ = false;
+    bool device_book_ = false;
+    bool size_pending_ = false, status_pending_ = false, book_pending_ = false;
     uint64_t h_size_ = 0;
-    int32_t h_status_ = 0;
+    int32_t h_status_ = 0, h_book_status_ = 0;
     struct Saved { uint32_t bk; uint64_t ol; size_t out; } saved_{1024, 0, 0};
     DeviceScratch ws_, small_, book_, freq_;
   public:

@@ -275,6 +275,22 @@ int fzh_histogram(void* stream, const void* d_in, size_t n, int elem_bytes,
 size_t fzh_huffman_revbook_bytes(uint32_t bklen, int elem_bytes);
 int fzh_huffman_build_book(const uint32_t* h_freq, uint32_t bklen,
                            int elem_bytes, uint32_t* h_bk4, uint8_t* h_revbook);
+/* The same codebook built on the device (an extension; the reference builds
+ * it on the host, huffman_stage.h:108): the reference builder's binary-heap
+ * merges replayed in order by one lane with the heap in LDS, leaf depths
+ * from parent links, canonisation in parallel; d_bk4 and d_revbook are byte
+ * for byte those of fzh_huffman_build_book on the same histogram.  No host
+ * round trip, so a Huffman encode can be captured in a graph.  bklen <= 1024
+ * (256 for 1-byte symbols).  *d_status: 0, 1 when the histogram does not sum
+ * to n_expected (symbols outside [0, bklen); the encoder then gives them no
+ * code), 2 when a code would exceed 58 bits.  One workgroup whose time is
+ * the replay's chain of dependent LDS accesses (3 heap operations of about
+ * log2(m) steps per merge, m symbols present): on MI355X about 130 us for
+ * 83 symbols and 2.3 ms for 1024, where the host builder takes 7-22 us --
+ * it serves callers that cannot take a host round trip, not speed.        */
+int fzh_huffman_build_book_device(void* stream, const uint32_t* d_freq, uint32_t bklen,
+                                  int elem_bytes, uint64_t n_expected, uint32_t* d_bk4,
+                                  uint8_t* d_revbook, int* d_status);
 size_t fzh_huffman_enc_bound(size_t n, uint32_t bklen, int elem_bytes);
 size_t fzh_huffman_enc_ws_bytes(size_t n);
 int fzh_huffman_encode(void* stream, const void* d_in, size_t n, int elem_bytes,

@@ -63,6 +63,12 @@ int fzp_add_rle(fzp_pipeline* p, int elem_bytes, int* stage_id);
 int fzp_add_huffman(fzp_pipeline* p, int elem_bytes, uint32_t bklen, int* stage_id);
 /* HuffmanStage::setEncodeMode: fine = 1 for HuffmanEncodeMode::Fine. */
 int fzp_set_huffman_fine(fzp_pipeline* p, int stage_id, int fine);
+/* Extension (no counterpart in the reference, which builds the codebook on
+ * the host, huffman_stage.h:108): HuffmanStage::setDeviceCodebook -- the
+ * codebook is built on the device, so a compress enqueues device work only
+ * and graph mode captures it; coarse encoding, bklen <= 1024, the stream is
+ * byte for byte that of the host codebook. */
+int fzp_set_huffman_device_book(fzp_pipeline* p, int stage_id, int on);
 int fzp_add_quantizer(fzp_pipeline* p, float error_bound, int eb_mode, uint32_t quant_radius,
                       float outlier_capacity, int zigzag, int code_bytes, float value_base,
                       int* stage_id);
