@@ -10,7 +10,8 @@
 //
 // Launch sequence of one compress (all on one stream, captured as a graph):
 //   value_base  per-unit min/max, then one wave per field resolves eb
-//   lorenzo     codes + per-tile lists (kListSlots entries) + per-tile counts
+//   lorenzo     codes + per-tile lists (kListSlots entries) + per-tile counts,
+//               one launch per tile shape (3D, 2D, 1D fields)
 //   rze         fused Bitshuffle+RZE, one workgroup per chunk, into a chunk
 //               scratch slot
 //   scan        one exclusive scan (mod 2^32) over [tile outliers | tile
@@ -138,27 +139,108 @@ __device__ __forceinline__ TileCounts run_tile(const Plan& p, const Field& fd, u
                                            p.rad, codes, lo, lp, se);
 }
 
-template <bool ZZ, bool STRICT>
+// Wave-uniform copies of table values (the tables are read with vector
+// loads, so without this every field parameter and pointer would take
+// vector registers for the whole tile).
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ float rflf(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+__device__ __forceinline__ Field uniform_field(const Field* fp) {
+    static_assert(sizeof(Field) % 16 == 0, "Field is read as 16-byte words");
+    Field f;
+    const uint4* src = reinterpret_cast<const uint4*>(fp);
+    uint4* dst = reinterpret_cast<uint4*>(&f);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(Field) / 16); i++) {
+        const uint4 v = src[i];
+        dst[i] = make_uint4(rfl(v.x), rfl(v.y), rfl(v.z), rfl(v.w));
+    }
+    return f;
+}
+
+// One launch per tile shape (ND = 1, 2, 3): a wave takes units of G
+// consecutive tiles of its class and walks them field by field; tile
+// indices, list slots and counts are the global ones.
+template <int ND, bool ZZ, bool STRICT>
+__device__ __forceinline__ TileCounts run_tile_nd(const Plan& p, const Field& fd, uint32_t f, uint32_t lt,
+                                                  uint16_t* codes, ListOut lo, ListOut lp) {
+    const float* prm = p.params + 4 * (size_t)f;
+    const float r = rflf(prm[2]);
+    const StrictEb se{rflf(prm[1]), rflf(prm[3])};
+    const bool last = lt + 1 == fd.ntiles;
+    // codes are always written here: only the code-writing tile variants
+    if constexpr (ND == 1) {
+        return lz1d_tile<uint16_t, ZZ, STRICT>(fd.in, fd.n, fd.nv, lt, r, p.rad, codes, lo, lp, se);
+    } else if constexpr (ND == 2) {
+        const uint32_t bx = lt % fd.gx, by = lt / fd.gx;
+        if (lz2d_interior(fd.nx, fd.ny, bx, by))
+            return lz2d_tile_v<uint16_t, ZZ, STRICT, true, true>(fd.in, fd.nx, fd.ny, fd.n, lt, fd.gx, last, r, p.rad,
+                                                                 codes, lo, lp, se);
+        return lz2d_tile_v<uint16_t, ZZ, STRICT, false, true>(fd.in, fd.nx, fd.ny, fd.n, lt, fd.gx, last, r, p.rad,
+                                                              codes, lo, lp, se);
+    } else {
+        const Tiles tg{fd.gx, fd.gy, 0};
+        const uint32_t bx = lt % tg.gx, by = (lt / tg.gx) % tg.gy, bz = lt / (tg.gx * tg.gy);
+        if (bx * 8 + 8 <= fd.nx && by * 8 + 8 <= fd.ny && bz * 8 + 8 <= fd.nz && (uint64_t)fd.nx * fd.ny < (1ull << 26))
+            return lz3d_tile_v<uint16_t, ZZ, STRICT, true, true>(fd.in, fd.nx, fd.ny, fd.nz, fd.n, lt, tg, last, r,
+                                                                 p.rad, codes, lo, lp, se);
+        return lz3d_tile_v<uint16_t, ZZ, STRICT, false, true>(fd.in, fd.nx, fd.ny, fd.nz, fd.n, lt, tg, last, r,
+                                                              p.rad, codes, lo, lp, se);
+    }
+}
+
+template <int ND, bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256) lz_batch_kernel(const Plan* __restrict__ pp, uint32_t G) {
     const Plan& p = *pp;
+    constexpr int d = ND - 1;
+    const uint32_t* __restrict__ cf = p.cls_field[d];
+    const uint32_t* __restrict__ ct0 = p.cls_tile0[d];
+    const uint32_t T = p.cls_T[d];
     const uint32_t waves = gridDim.x * 4;
-    const uint32_t nu = (p.T + G - 1) / G;
+    const uint32_t nu = (T + G - 1) / G;
     for (uint32_t u = blockIdx.x * 4 + wave_in_wg(); u < nu; u += waves) {
-        const uint32_t g0 = u * G, g1 = min(p.T, g0 + G);
-        uint32_t f = p.tfield[g0 >> 6];
-        while (g0 >= p.tile0[f + 1]) f++;
-        for (uint32_t g = g0; g < g1; g++) {
-            while (g >= p.tile0[f + 1]) f++;
-            const Field& fd = p.fields[f];
+        const uint32_t c0 = u * G, c1 = min(T, c0 + G);
+        uint32_t j = rfl(p.cls_tfield[d][c0 >> 6]);
+        while (c0 >= rfl(ct0[j + 1])) j++;
+        for (uint32_t c = c0; c < c1; c++) {
+            while (c >= rfl(ct0[j + 1])) j++;
+            const uint32_t f = rfl(cf[j]);
+            const Field fd = uniform_field(p.fields + f);
+            const uint32_t lt = c - rfl(ct0[j]);
+            const uint32_t g = fd.tile0 + lt;
             const uint64_t sl = (uint64_t)g * kListSlots;
             const ListOut lo{p.lerr + sl, p.lidx + sl, kListSlots};
             const ListOut lp = STRICT ? ListOut{p.lpval + sl, p.lpidx + sl, kListSlots} : ListOut{nullptr, nullptr, 0};
-            const TileCounts tc = run_tile<ZZ, STRICT>(p, fd, f, g - fd.tile0, p.codes + fd.codes_off, lo, lp);
+            const TileCounts tc = run_tile_nd<ND, ZZ, STRICT>(p, fd, f, lt, p.codes + fd.codes_off, lo, lp);
             if (lane_id() == 0) {
                 p.counts[g] = tc.cnt;
                 if constexpr (STRICT) p.counts[p.T + g] = tc.pcnt;
             }
         }
+    }
+}
+
+// The same with one tile per wave and no loop (a grid covering the class):
+// the registers of the single-field kernels, so more waves per SIMD.
+template <int ND, bool ZZ, bool STRICT>
+__global__ void __launch_bounds__(256) lz_batch_tile_kernel(const Plan* __restrict__ pp) {
+    const Plan& p = *pp;
+    constexpr int d = ND - 1;
+    const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
+    if (c >= p.cls_T[d]) return;
+    const uint32_t* __restrict__ ct0 = p.cls_tile0[d];
+    uint32_t j = rfl(p.cls_tfield[d][c >> 6]);
+    while (c >= rfl(ct0[j + 1])) j++;
+    const uint32_t f = rfl(p.cls_field[d][j]);
+    const Field fd = uniform_field(p.fields + f);
+    const uint32_t lt = c - rfl(ct0[j]);
+    const uint32_t g = fd.tile0 + lt;
+    const uint64_t sl = (uint64_t)g * kListSlots;
+    const ListOut lo{p.lerr + sl, p.lidx + sl, kListSlots};
+    const ListOut lp = STRICT ? ListOut{p.lpval + sl, p.lpidx + sl, kListSlots} : ListOut{nullptr, nullptr, 0};
+    const TileCounts tc = run_tile_nd<ND, ZZ, STRICT>(p, fd, f, lt, p.codes + fd.codes_off, lo, lp);
+    if (lane_id() == 0) {
+        p.counts[g] = tc.cnt;
+        if constexpr (STRICT) p.counts[p.T + g] = tc.pcnt;
     }
 }
 
@@ -174,8 +256,8 @@ __global__ void __launch_bounds__(256) rze_batch_kernel(const Plan* __restrict__
     const uint32_t gc = xcd_swizzle(blockIdx.x, gridDim.x);
     if (gc >= p.NC) return;
     const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
-    const uint32_t f = p.cfield[gc];
-    const Field& fd = p.fields[f];
+    const uint32_t f = rfl(p.cfield[gc]);
+    const Field fd = uniform_field(p.fields + f);
     const uint32_t lc = gc - fd.chunk0;
     const uint64_t cb = 2 * fd.n;  // code bytes
     const uint64_t off = (uint64_t)lc * kChunk;
@@ -379,16 +461,31 @@ int launch_value_base(hipStream_t s, const Plan& p, const Plan* dp, uint32_t gri
 
 int launch_lorenzo(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid, uint32_t G) {
     G = std::max<uint32_t>(1, G);
-    const uint32_t nu = (uint32_t)cdiv(p.T, G);
-    const unsigned g = std::max<uint32_t>(1, std::min<uint64_t>(grid, cdiv(nu, 4)));
-    if (p.zigzag) {
-        if (p.strict) lz_batch_kernel<true, true><<<g, 256, 0, s>>>(dp, G);
-        else lz_batch_kernel<true, false><<<g, 256, 0, s>>>(dp, G);
-    } else {
-        if (p.strict) lz_batch_kernel<false, true><<<g, 256, 0, s>>>(dp, G);
-        else lz_batch_kernel<false, false><<<g, 256, 0, s>>>(dp, G);
+#define FZH_LZB(ND)                                                                                   \
+    if (p.cls_T[ND - 1]) {                                                                            \
+        const uint32_t nu = (uint32_t)cdiv(p.cls_T[ND - 1], G);                                       \
+        const unsigned g = std::max<uint32_t>(1, std::min<uint64_t>(grid, cdiv(nu, 4)));             \
+        if (G == 1 && (uint64_t)grid >= cdiv(nu, 4)) { /* one tile per wave */                      \
+            if (p.zigzag) {                                                                           \
+                if (p.strict) lz_batch_tile_kernel<ND, true, true><<<g, 256, 0, s>>>(dp);              \
+                else lz_batch_tile_kernel<ND, true, false><<<g, 256, 0, s>>>(dp);                      \
+            } else {                                                                                  \
+                if (p.strict) lz_batch_tile_kernel<ND, false, true><<<g, 256, 0, s>>>(dp);             \
+                else lz_batch_tile_kernel<ND, false, false><<<g, 256, 0, s>>>(dp);                     \
+            }                                                                                         \
+        } else if (p.zigzag) {                                                                        \
+            if (p.strict) lz_batch_kernel<ND, true, true><<<g, 256, 0, s>>>(dp, G);                    \
+            else lz_batch_kernel<ND, true, false><<<g, 256, 0, s>>>(dp, G);                            \
+        } else {                                                                                      \
+            if (p.strict) lz_batch_kernel<ND, false, true><<<g, 256, 0, s>>>(dp, G);                   \
+            else lz_batch_kernel<ND, false, false><<<g, 256, 0, s>>>(dp, G);                           \
+        }                                                                                             \
+        FZH_CHECK_LAUNCH("batch lorenzo");                                                            \
     }
-    FZH_CHECK_LAUNCH("batch lorenzo");
+    FZH_LZB(3)
+    FZH_LZB(2)
+    FZH_LZB(1)
+#undef FZH_LZB
     return FZH_OK;
 }
 

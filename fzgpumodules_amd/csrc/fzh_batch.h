@@ -61,7 +61,14 @@ struct Plan {
     const uint32_t* tile0;   // F + 1
     const uint32_t* chunk0;  // F + 1
     const uint32_t* unit0;   // F + 1
-    const uint32_t* tfield;  // ceil(T / 64): the field holding tile 64 k (a start for the tile -> field walk)
+    // Tiles by dimensionality (class d = ndim - 1), so each Lorenzo launch
+    // holds one tile shape's registers: the class's fields in field order,
+    // the prefix of their tile counts, and the class field holding class
+    // tile 64 k (a start for the tile -> field walk).
+    const uint32_t* cls_field[3];  // cls_F[d]
+    const uint32_t* cls_tile0[3];  // cls_F[d] + 1
+    const uint32_t* cls_tfield[3]; // ceil(cls_T[d] / 64)
+    uint32_t cls_F[3], cls_T[3];
     const uint32_t* cfield;  // NC: the field of each chunk
     const uint32_t* ufield;  // U: the field of each value-base unit
     uint32_t F, T, NC, U;

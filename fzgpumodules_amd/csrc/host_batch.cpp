@@ -196,17 +196,36 @@ void BatchCompressor::setFields(const std::vector<BatchField>& fields) {
     hipck(hipMemcpy(d_t0, tile0.data(), 4 * (F + 1), hipMemcpyHostToDevice), "hipMemcpy (tiles)");
     hipck(hipMemcpy(d_c0, chunk0.data(), 4 * (F + 1), hipMemcpyHostToDevice), "hipMemcpy (chunks)");
     hipck(hipMemcpy(d_u0, unit0.data(), 4 * (F + 1), hipMemcpyHostToDevice), "hipMemcpy (units)");
-    // field of every 64th tile: the Lorenzo kernel starts its walk there
-    // instead of a binary search over all fields
-    const size_t nb = std::max<size_t>((T + 63) / 64, 1);
-    std::vector<uint32_t> tfield(nb, 0);
-    for (size_t k = 0, f = 0; k < nb; k++) {
-        while (f + 1 < F && tile0[f + 1] <= 64 * k) f++;
-        tfield[k] = (uint32_t)f;
+    // tiles by dimensionality: per class, its fields, their tile prefix and
+    // the class field of every 64th class tile (the Lorenzo kernels start
+    // their walk there instead of a binary search)
+    for (int d = 0; d < 3; d++) {
+        std::vector<uint32_t> cf, ct0{0};
+        for (size_t i = 0; i < F; i++)
+            if (tab[i].ndim == (uint32_t)d + 1) {
+                cf.push_back((uint32_t)i);
+                ct0.push_back(ct0.back() + tab[i].ntiles);
+            }
+        const uint32_t Td = ct0.back();
+        const size_t nb = std::max<size_t>((Td + 63) / 64, 1);
+        std::vector<uint32_t> tf(nb, 0);
+        for (size_t k = 0, j = 0; k < nb && !cf.empty(); k++) {
+            while (j + 1 < cf.size() && ct0[j + 1] <= 64 * k) j++;
+            tf[k] = (uint32_t)j;
+        }
+        if (cf.empty()) cf.push_back(0);
+        uint32_t* d_cf = (uint32_t*)impl_->dev(4 * cf.size());
+        uint32_t* d_ct0 = (uint32_t*)impl_->dev(4 * ct0.size());
+        uint32_t* d_ctf = (uint32_t*)impl_->dev(4 * nb);
+        hipck(hipMemcpy(d_cf, cf.data(), 4 * cf.size(), hipMemcpyHostToDevice), "hipMemcpy (class fields)");
+        hipck(hipMemcpy(d_ct0, ct0.data(), 4 * ct0.size(), hipMemcpyHostToDevice), "hipMemcpy (class tiles)");
+        hipck(hipMemcpy(d_ctf, tf.data(), 4 * nb, hipMemcpyHostToDevice), "hipMemcpy (class tile fields)");
+        p.cls_field[d] = d_cf;
+        p.cls_tile0[d] = d_ct0;
+        p.cls_tfield[d] = d_ctf;
+        p.cls_F[d] = (uint32_t)(ct0.size() - 1);
+        p.cls_T[d] = Td;
     }
-    uint32_t* d_tf = (uint32_t*)impl_->dev(4 * nb);
-    hipck(hipMemcpy(d_tf, tfield.data(), 4 * nb, hipMemcpyHostToDevice), "hipMemcpy (tile fields)");
-    p.tfield = d_tf;
     std::vector<uint32_t> cfield(std::max<uint64_t>(NC, 1), 0);
     for (uint32_t f = 0; f < F; f++)
         for (uint32_t c = chunk0[f]; c < chunk0[f + 1]; c++) cfield[c] = f;

@@ -36,11 +36,13 @@ struct BatchConfig {
 
 // Persistent-grid geometry (workgroups per CU per kernel; Lorenzo tiles per
 // wave step; the RZE encoder takes one workgroup per chunk).  Changes timing
-// only.
+// only.  The Lorenzo default, a grid covering every tile with one tile per
+// wave (lz_tiles_per_unit 1 and a grid at least the tile count), runs the
+// loop-free kernels.
 struct BatchLaunch {
     uint32_t vb_wg_per_cu = 4;
-    uint32_t lz_wg_per_cu = 8;
-    uint32_t lz_tiles_per_unit = 4;
+    uint32_t lz_wg_per_cu = 1u << 20;
+    uint32_t lz_tiles_per_unit = 1;
     uint32_t pack_wg_per_cu = 4;
     uint32_t scatter_wg_per_cu = 2;
 };

@@ -54,7 +54,9 @@ typedef struct fzb_config {
 /* Launch geometry of the persistent kernels (changes timing only, never
  * bytes).  0 = library default.  wg_per_cu: workgroups per compute unit of
  * each persistent grid (a grid larger than the work is clamped to one unit
- * per wave); tiles_per_unit: Lorenzo tiles taken per wave step.
+ * per wave); tiles_per_unit: Lorenzo tiles taken per wave step.  Lorenzo
+ * runs one launch per tile shape (1D, 2D, 3D fields); tiles_per_unit 1 with
+ * a grid covering the tiles (the default) takes its loop-free kernels.
  * (The RZE encoder takes one workgroup per chunk and has no parameter.)    */
 typedef struct fzb_launch {
     uint32_t vb_wg_per_cu;

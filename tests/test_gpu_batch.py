@@ -130,6 +130,7 @@ def test_batch_fields_match_the_oracle_chain(mixed, mixed_reference):
     dict(vb_wg_per_cu=1, lz_wg_per_cu=1, lz_tiles_per_unit=1, pack_wg_per_cu=1, scatter_wg_per_cu=1),
     dict(vb_wg_per_cu=8, lz_wg_per_cu=16, lz_tiles_per_unit=16, pack_wg_per_cu=8, scatter_wg_per_cu=8),
     dict(lz_tiles_per_unit=3, pack_wg_per_cu=3),
+    dict(lz_wg_per_cu=1 << 20, lz_tiles_per_unit=1),  # one tile per wave, no loop
 ])
 def test_batch_bytes_do_not_depend_on_launch_geometry(mixed, mixed_reference, launch):
     bt = _Batch(mixed)
@@ -140,6 +141,22 @@ def test_batch_bytes_do_not_depend_on_launch_geometry(mixed, mixed_reference, la
             np.testing.assert_array_equal(got[i], want, err_msg=f"field {i} {launch}")
     finally:
         bt.close()
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_batch_of_one_dimensionality(nd):
+    """Only one tile shape present: the other shapes' launches are skipped."""
+    xs = [x for x in _mixed_fields(9, 11, max_elems=1 << 16) if x.ndim == nd]
+    want = [_single(x)[0] for x in xs]
+    for launch in (dict(lz_wg_per_cu=1 << 20, lz_tiles_per_unit=1), dict(lz_wg_per_cu=2, lz_tiles_per_unit=4)):
+        bt = _Batch(xs)
+        try:
+            bt.b.set_launch(**launch)
+            got = bt.run()
+            for i in range(len(xs)):
+                np.testing.assert_array_equal(got[i], want[i], err_msg=f"field {i} {launch}")
+        finally:
+            bt.close()
 
 
 def test_batch_graph_replay_and_autotune(mixed, mixed_reference):
