@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(256) vb_partial_kernel(const Plan* __restrict_
         const float* in = fd.in;
         const uint64_t b = (uint64_t)(u - fd.unit0) * kVbUnit;
         const uint64_t e = min(fd.nv, b + kVbUnit);
-        const float first = in[0];
+        const float first = gp(in)[0];
         float mn = first, mx = first;
         if (fd.nv < fd.n) {
             if (0.0f < mn) mn = 0.0f;
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(256) vb_partial_kernel(const Plan* __restrict_
         }
         const uint64_t e4 = b + ((e - b) & ~3ull);
         for (uint64_t i = b + 4 * (uint64_t)threadIdx.x; i < e4; i += 1024) {
-            const float4 v = *reinterpret_cast<const float4*>(in + i);
+            const float4 v = gld_f4(in + i);
             if (v.x < mn) mn = v.x;
             if (v.x > mx) mx = v.x;
             if (v.y < mn) mn = v.y;
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256) vb_partial_kernel(const Plan* __restrict_
             if (v.w > mx) mx = v.w;
         }
         for (uint64_t i = e4 + threadIdx.x; i < e; i += 256) {
-            const float v = in[i];
+            const float v = gp(in)[i];
             if (v < mn) mn = v;
             if (v > mx) mx = v;
         }
@@ -151,7 +151,7 @@ __device__ __forceinline__ Field uniform_field(const Field* fp) {
     uint4* dst = reinterpret_cast<uint4*>(&f);
 #pragma unroll
     for (int i = 0; i < (int)(sizeof(Field) / 16); i++) {
-        const uint4 v = src[i];
+        const uint4 v = gld_u4(src + i);
         dst[i] = make_uint4(rfl(v.x), rfl(v.y), rfl(v.z), rfl(v.w));
     }
     return f;
@@ -164,8 +164,8 @@ template <int ND, bool ZZ, bool STRICT>
 __device__ __forceinline__ TileCounts run_tile_nd(const Plan& p, const Field& fd, uint32_t f, uint32_t lt,
                                                   uint16_t* codes, ListOut lo, ListOut lp) {
     const float* prm = p.params + 4 * (size_t)f;
-    const float r = rflf(prm[2]);
-    const StrictEb se{rflf(prm[1]), rflf(prm[3])};
+    const float r = rflf(gp(prm)[2]);
+    const StrictEb se{rflf(gp(prm)[1]), rflf(gp(prm)[3])};
     const bool last = lt + 1 == fd.ntiles;
     // codes are always written here: only the code-writing tile variants
     if constexpr (ND == 1) {
@@ -199,21 +199,21 @@ __global__ void __launch_bounds__(256) lz_batch_kernel(const Plan* __restrict__ 
     const uint32_t nu = (T + G - 1) / G;
     for (uint32_t u = blockIdx.x * 4 + wave_in_wg(); u < nu; u += waves) {
         const uint32_t c0 = u * G, c1 = min(T, c0 + G);
-        uint32_t j = rfl(p.cls_tfield[d][c0 >> 6]);
-        while (c0 >= rfl(ct0[j + 1])) j++;
+        uint32_t j = rfl(gp(p.cls_tfield[d])[c0 >> 6]);
+        while (c0 >= rfl(gp(ct0)[j + 1])) j++;
         for (uint32_t c = c0; c < c1; c++) {
-            while (c >= rfl(ct0[j + 1])) j++;
-            const uint32_t f = rfl(cf[j]);
+            while (c >= rfl(gp(ct0)[j + 1])) j++;
+            const uint32_t f = rfl(gp(cf)[j]);
             const Field fd = uniform_field(p.fields + f);
-            const uint32_t lt = c - rfl(ct0[j]);
+            const uint32_t lt = c - rfl(gp(ct0)[j]);
             const uint32_t g = fd.tile0 + lt;
             const uint64_t sl = (uint64_t)g * kListSlots;
             const ListOut lo{p.lerr + sl, p.lidx + sl, kListSlots};
             const ListOut lp = STRICT ? ListOut{p.lpval + sl, p.lpidx + sl, kListSlots} : ListOut{nullptr, nullptr, 0};
             const TileCounts tc = run_tile_nd<ND, ZZ, STRICT>(p, fd, f, lt, p.codes + fd.codes_off, lo, lp);
             if (lane_id() == 0) {
-                p.counts[g] = tc.cnt;
-                if constexpr (STRICT) p.counts[p.T + g] = tc.pcnt;
+                gp(p.counts)[g] = tc.cnt;
+                if constexpr (STRICT) gp(p.counts)[p.T + g] = tc.pcnt;
             }
         }
     }
@@ -228,20 +228,74 @@ __global__ void __launch_bounds__(256) lz_batch_tile_kernel(const Plan* __restri
     const uint32_t c = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
     if (c >= p.cls_T[d]) return;
     const uint32_t* __restrict__ ct0 = p.cls_tile0[d];
-    uint32_t j = rfl(p.cls_tfield[d][c >> 6]);
-    while (c >= rfl(ct0[j + 1])) j++;
-    const uint32_t f = rfl(p.cls_field[d][j]);
+    uint32_t j = rfl(gp(p.cls_tfield[d])[c >> 6]);
+    while (c >= rfl(gp(ct0)[j + 1])) j++;
+    const uint32_t f = rfl(gp(p.cls_field[d])[j]);
     const Field fd = uniform_field(p.fields + f);
-    const uint32_t lt = c - rfl(ct0[j]);
+    const uint32_t lt = c - rfl(gp(ct0)[j]);
     const uint32_t g = fd.tile0 + lt;
     const uint64_t sl = (uint64_t)g * kListSlots;
     const ListOut lo{p.lerr + sl, p.lidx + sl, kListSlots};
     const ListOut lp = STRICT ? ListOut{p.lpval + sl, p.lpidx + sl, kListSlots} : ListOut{nullptr, nullptr, 0};
     const TileCounts tc = run_tile_nd<ND, ZZ, STRICT>(p, fd, f, lt, p.codes + fd.codes_off, lo, lp);
     if (lane_id() == 0) {
-        p.counts[g] = tc.cnt;
-        if constexpr (STRICT) p.counts[p.T + g] = tc.pcnt;
+        gp(p.counts)[g] = tc.cnt;
+        if constexpr (STRICT) gp(p.counts)[p.T + g] = tc.pcnt;
     }
+}
+
+// 3D fields: two consecutive tiles of one field per wave, both tiles' loads
+// issued before either is computed, so a wave keeps 4 KiB of input in
+// flight as a 2D or 1D tile's wave does (one 8x8x8 tile holds only 2 KiB).
+template <bool ZZ, bool STRICT>
+__device__ __forceinline__ void tile3_finish(const Plan& p, const Field& fd, uint32_t f, uint32_t lt, bool interior,
+                                             const float (&xv)[8]) {
+    const float* prm = p.params + 4 * (size_t)f;
+    const float r = rflf(gp(prm)[2]);
+    const StrictEb se{rflf(gp(prm)[1]), rflf(gp(prm)[3])};
+    const Tiles tg{fd.gx, fd.gy, 0};
+    const uint32_t g = fd.tile0 + lt;
+    const uint64_t sl = (uint64_t)g * kListSlots;
+    const ListOut lo{p.lerr + sl, p.lidx + sl, kListSlots};
+    const ListOut lp = STRICT ? ListOut{p.lpval + sl, p.lpidx + sl, kListSlots} : ListOut{nullptr, nullptr, 0};
+    uint16_t* codes = p.codes + fd.codes_off;
+    const bool last = lt + 1 == fd.ntiles;
+    const TileCounts tc =
+        interior ? lz3d_tile_c<uint16_t, ZZ, STRICT, true, true>(xv, fd.nx, fd.ny, fd.nz, fd.n, lt, tg, last, r, p.rad,
+                                                                 codes, lo, lp, se)
+                 : lz3d_tile_c<uint16_t, ZZ, STRICT, false, true>(xv, fd.nx, fd.ny, fd.nz, fd.n, lt, tg, last, r,
+                                                                  p.rad, codes, lo, lp, se);
+    if (lane_id() == 0) {
+        gp(p.counts)[g] = tc.cnt;
+        if constexpr (STRICT) gp(p.counts)[p.T + g] = tc.pcnt;
+    }
+}
+
+template <bool ZZ, bool STRICT>
+__global__ void __launch_bounds__(256) lz_batch_tile3_kernel(const Plan* __restrict__ pp) {
+    const Plan& p = *pp;
+    const uint32_t u = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
+    if (u >= p.P) return;
+    const uint32_t* __restrict__ pr0 = p.pair0;
+    uint32_t j = rfl(gp(p.pfield)[u >> 6]);
+    while (u >= rfl(gp(pr0)[j + 1])) j++;
+    const uint32_t f = rfl(gp(p.cls_field[2])[j]);
+    const Field fd = uniform_field(p.fields + f);
+    const Tiles tg{fd.gx, fd.gy, 0};
+    const uint32_t lta = 2 * (u - rfl(gp(pr0)[j])), ltb = lta + 1;
+    const bool two = ltb < fd.ntiles;
+    float xa[8], xb[8];
+    const bool ia = lz3d_interior(fd.nx, fd.ny, fd.nz, lta, tg);
+    if (ia) lz3d_load_tile<true>(fd.in, fd.nx, fd.ny, fd.nz, lta, tg, xa);
+    else lz3d_load_tile<false>(fd.in, fd.nx, fd.ny, fd.nz, lta, tg, xa);
+    bool ib = false;
+    if (two) {
+        ib = lz3d_interior(fd.nx, fd.ny, fd.nz, ltb, tg);
+        if (ib) lz3d_load_tile<true>(fd.in, fd.nx, fd.ny, fd.nz, ltb, tg, xb);
+        else lz3d_load_tile<false>(fd.in, fd.nx, fd.ny, fd.nz, ltb, tg, xb);
+    }
+    tile3_finish<ZZ, STRICT>(p, fd, f, lta, ia, xa);
+    if (two) tile3_finish<ZZ, STRICT>(p, fd, f, ltb, ib, xb);
 }
 
 // ---------------------------------------------------------------------------
@@ -256,7 +310,7 @@ __global__ void __launch_bounds__(256) rze_batch_kernel(const Plan* __restrict__
     const uint32_t gc = xcd_swizzle(blockIdx.x, gridDim.x);
     if (gc >= p.NC) return;
     const uint32_t cbase = p.T * (p.strict ? 2u : 1u);
-    const uint32_t f = rfl(p.cfield[gc]);
+    const uint32_t f = rfl(gp(p.cfield)[gc]);
     const Field fd = uniform_field(p.fields + f);
     const uint32_t lc = gc - fd.chunk0;
     const uint64_t cb = 2 * fd.n;  // code bytes
@@ -465,7 +519,16 @@ int launch_lorenzo(hipStream_t s, const Plan& p, const Plan* dp, uint32_t grid, 
     if (p.cls_T[ND - 1]) {                                                                            \
         const uint32_t nu = (uint32_t)cdiv(p.cls_T[ND - 1], G);                                       \
         const unsigned g = std::max<uint32_t>(1, std::min<uint64_t>(grid, cdiv(nu, 4)));             \
-        if (G == 1 && (uint64_t)grid >= cdiv(nu, 4)) { /* one tile per wave */                      \
+        if (G == 1 && (uint64_t)grid >= cdiv(nu, 4) && ND == 3) { /* two tiles per wave */          \
+            const unsigned g3 = (unsigned)std::max<uint64_t>(1, cdiv(p.P, 4));                        \
+            if (p.zigzag) {                                                                           \
+                if (p.strict) lz_batch_tile3_kernel<true, true><<<g3, 256, 0, s>>>(dp);                \
+                else lz_batch_tile3_kernel<true, false><<<g3, 256, 0, s>>>(dp);                        \
+            } else {                                                                                  \
+                if (p.strict) lz_batch_tile3_kernel<false, true><<<g3, 256, 0, s>>>(dp);               \
+                else lz_batch_tile3_kernel<false, false><<<g3, 256, 0, s>>>(dp);                       \
+            }                                                                                         \
+        } else if (G == 1 && (uint64_t)grid >= cdiv(nu, 4)) { /* one tile per wave */               \
             if (p.zigzag) {                                                                           \
                 if (p.strict) lz_batch_tile_kernel<ND, true, true><<<g, 256, 0, s>>>(dp);              \
                 else lz_batch_tile_kernel<ND, true, false><<<g, 256, 0, s>>>(dp);                      \

@@ -86,8 +86,9 @@ __device__ __forceinline__ void unplanes8(uint32_t (&A)[8], uint32_t (&w)[8]) {
     }
 }
 
-__device__ __forceinline__ uint4 ld16(const uint8_t* p) { return *reinterpret_cast<const uint4*>(p); }
-__device__ __forceinline__ void st16(uint8_t* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
+// 16-byte global-memory load / store
+__device__ __forceinline__ uint4 ld16(const uint8_t* p) { return gld_u4(p); }
+__device__ __forceinline__ void st16(uint8_t* p, uint4 v) { gst_u4(p, v); }
 
 }  // namespace
 }  // namespace fzh

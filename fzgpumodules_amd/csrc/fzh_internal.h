@@ -98,6 +98,38 @@ __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 // This thread's wave within its workgroup, as a wave-uniform value: work
 // indices derived from it (tiles, chunks) then live in scalar registers, so
 // their divisions, bounds tests and base addresses are scalar work.
+// The same pointer in the global address space.  Device code that reads a
+// pointer out of memory (a table of fields, a plan) holds a generic pointer,
+// and the compiler then emits FLAT instructions, which are counted by both
+// the vector-memory and the LDS/scalar wait counters; accessing through this
+// cast gives global instructions.  Only for pointers into device memory.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gp(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
+// 8- and 16-byte global accesses through gp() (clang vector types: HIP's
+// vector classes have no address-space-qualified members).
+typedef uint32_t fzh_u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t fzh_u32x4 __attribute__((ext_vector_type(4)));
+typedef float fzh_f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 gld_u4(const void* p) {
+    const fzh_u32x4 v = *gp(reinterpret_cast<const fzh_u32x4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 gld_f4(const void* p) {
+    const fzh_f32x4 v = *gp(reinterpret_cast<const fzh_f32x4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst_u4(void* p, uint4 v) {
+    const fzh_u32x4 w = {v.x, v.y, v.z, v.w};
+    *gp(reinterpret_cast<fzh_u32x4*>(p)) = w;
+}
+__device__ __forceinline__ void gst_u2(void* p, uint2 v) {
+    const fzh_u32x2 w = {v.x, v.y};
+    *gp(reinterpret_cast<fzh_u32x2*>(p)) = w;
+}
+
 __device__ __forceinline__ uint32_t wave_in_wg() {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }

@@ -129,7 +129,7 @@ __device__ __forceinline__ void patch_fast(float x, float s, StrictEb se, Strict
 // Zero codes[from, to) (tail beyond nx*ny[*nz] when the input was padded).
 template <typename TCode>
 __device__ __forceinline__ void zero_code_tail(TCode* codes, uint64_t from, uint64_t to) {
-    for (uint64_t i = from + lane_id(); i < to; i += 64) codes[i] = (TCode)0;
+    for (uint64_t i = from + lane_id(); i < to; i += 64) gp(codes)[i] = (TCode)0;
 }
 
 // ---------------------------------------------------------------------------
@@ -148,14 +148,14 @@ __device__ __forceinline__ TileCounts lz1d_tile(const float* __restrict__ in, ui
     for (int q = 0; q < 4; q++) {
         const uint64_t e0 = base + 4 * (uint64_t)(l + 64 * q);
         if (full) {
-            const float4 v = *reinterpret_cast<const float4*>(in + e0);
+            const float4 v = gld_f4(in + e0);
             xv[q][0] = v.x;
             xv[q][1] = v.y;
             xv[q][2] = v.z;
             xv[q][3] = v.w;
         } else {
 #pragma unroll
-            for (int k = 0; k < 4; k++) xv[q][k] = (e0 + k < nv) ? in[e0 + k] : 0.0f;
+            for (int k = 0; k < 4; k++) xv[q][k] = (e0 + k < nv) ? gp(in)[e0 + k] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) s[q][k] = quant_round(xv[q][k], r);
@@ -191,16 +191,16 @@ __device__ __forceinline__ TileCounts lz1d_tile(const float* __restrict__ in, ui
                     uint2 w;
                     w.x = (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 16);
                     w.y = (uint32_t)c[q][2] | ((uint32_t)c[q][3] << 16);
-                    *reinterpret_cast<uint2*>(codes + e0) = w;
+                    gst_u2(codes + e0, w);
                 } else {
-                    *reinterpret_cast<uint32_t*>(codes + e0) =
+                    *gp(reinterpret_cast<uint32_t*>(codes + e0)) =
                         (uint32_t)c[q][0] | ((uint32_t)c[q][1] << 8) | ((uint32_t)c[q][2] << 16) |
                         ((uint32_t)c[q][3] << 24);
                 }
             } else {
 #pragma unroll
                 for (int k = 0; k < 4; k++)
-                    if (e0 + k < n) codes[e0 + k] = c[q][k];
+                    if (e0 + k < n) gp(codes)[e0 + k] = c[q][k];
             }
         }
     }
@@ -217,8 +217,8 @@ __device__ __forceinline__ TileCounts lz1d_tile(const float* __restrict__ in, ui
             for (int k = 0; k < 4; k++)
                 if (mq & (1u << k)) {
                     if (pos < o.cap) {
-                        o.val[pos] = patch ? xv[q][k] : d[q][k];
-                        o.idx[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
+                        gp(o.val)[pos] = patch ? xv[q][k] : d[q][k];
+                        gp(o.idx)[pos] = (uint32_t)(base + 4 * (uint64_t)(l + 64 * q) + k);
                     }
                     pos++;
                 }
@@ -279,14 +279,14 @@ __device__ __forceinline__ void lz2d_load(const float* __restrict__ in, uint32_t
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const char* rp = reinterpret_cast<const char*>(in + tbase + (uint64_t)i * nx);
-            xv[i] = *reinterpret_cast<const float*>(rp + lb);
+            xv[i] = *gp(reinterpret_cast<const float*>(rp + lb));
         }
     } else {
         const bool xin = X < nx;
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const uint32_t Y = Y0 + i;
-            xv[i] = (xin && Y < ny) ? in[(uint64_t)Y * nx + X] : 0.0f;
+            xv[i] = (xin && Y < ny) ? gp(in)[(uint64_t)Y * nx + X] : 0.0f;
         }
     }
 }
@@ -326,9 +326,9 @@ __device__ __forceinline__ TileCounts lz2d_tile_c(const float (&xin16)[16], uint
             const TCode c = ok ? enc_code<TCode, ZZ>(d) : (TCode)0;
             if constexpr (INTERIOR) {
                 char* cp = reinterpret_cast<char*>(codes + tbase + (uint64_t)i * nx);
-                *reinterpret_cast<TCode*>(cp + cb) = c;
+                *gp(reinterpret_cast<TCode*>(cp + cb)) = c;
             } else if (valid) {
-                codes[(uint64_t)(Y0 + i) * nx + X] = c;
+                gp(codes)[(uint64_t)(Y0 + i) * nx + X] = c;
             }
         }
         bal[i] = __ballot(valid && !ok);
@@ -352,8 +352,8 @@ __device__ __forceinline__ TileCounts lz2d_tile_c(const float (&xin16)[16], uint
                 if ((half >> x) & 1u) {
                     const uint32_t pos = run + __popc(half & below);
                     if (pos < o.cap) {
-                        o.val[pos] = patch ? xv[i] : t[i];
-                        o.idx[pos] = (Y0 + i) * nx + X;
+                        gp(o.val)[pos] = patch ? xv[i] : t[i];
+                        gp(o.idx)[pos] = (Y0 + i) * nx + X;
                     }
                 }
                 run += __popc(half);
@@ -425,31 +425,74 @@ struct Tiles {
 // 3D: tile (bx, by, bz) = (tile % gx, (tile / gx) % gy, tile / (gx gy)) of an
 // nx x ny x nz field; `last` zeroes the code tail.  codes == nullptr: lists only.
 // ---------------------------------------------------------------------------
-template <typename TCode, bool ZZ, bool STRICT, bool INTERIOR, bool CODES>
-__device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
-                                                uint64_t n, uint32_t tile, Tiles tg, bool last, float r, float rad,
-                                                TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
+// Tile geometry of this lane: column X, plane Z, first row Y0; the tile's
+// first element tbase and the lane's 32-bit offset from it (interior tiles).
+struct Lz3Geo {
+    uint32_t X, Z, Y0;
+    bool xz;
+    uint64_t plane, tbase;
+    uint32_t loff;
+};
+__device__ __forceinline__ Lz3Geo lz3d_geo(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t tile, Tiles tg) {
     const int l = lane_id();
     const int x = l & 7, z = l >> 3;
     const uint32_t bx = tile % tg.gx, by = (tile / tg.gx) % tg.gy, bz = tile / (tg.gx * tg.gy);
-    const uint32_t X = bx * 8 + x, Z = bz * 8 + z, Y0 = by * 8;
-    const bool xz = X < nx && Z < nz;
-    const uint64_t plane = (uint64_t)nx * ny;
-    // interior: row y of the tile from its scalar base plus a 32-bit lane offset
-    const uint64_t tbase = (uint64_t)(bz * 8) * plane + (uint64_t)(by * 8) * nx + bx * 8;
-    const uint32_t loff = (uint32_t)z * (uint32_t)plane + (uint32_t)x;
-    float s[8], xv[8];
+    Lz3Geo g;
+    g.X = bx * 8 + x;
+    g.Z = bz * 8 + z;
+    g.Y0 = by * 8;
+    g.xz = g.X < nx && g.Z < nz;
+    g.plane = (uint64_t)nx * ny;
+    g.tbase = (uint64_t)(bz * 8) * g.plane + (uint64_t)(by * 8) * nx + bx * 8;
+    g.loff = (uint32_t)z * (uint32_t)g.plane + (uint32_t)x;
+    return g;
+}
+// True when the tile lies inside the field and its 8 planes span less than
+// 2^32 bytes (32-bit lane offsets from the tile base).
+__device__ __forceinline__ bool lz3d_interior(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t tile, Tiles tg) {
+    const uint32_t bx = tile % tg.gx, by = (tile / tg.gx) % tg.gy, bz = tile / (tg.gx * tg.gy);
+    return bx * 8 + 8 <= nx && by * 8 + 8 <= ny && bz * 8 + 8 <= nz && (uint64_t)nx * ny < (1ull << 26);
+}
+
+// The tile's 8 values of this lane (rows Y0..Y0+7 of column X, plane Z;
+// zero outside the field).
+template <bool INTERIOR>
+__device__ __forceinline__ void lz3d_load(const float* __restrict__ in, uint32_t nx, uint32_t ny, const Lz3Geo& g,
+                                          float (&xv)[8]) {
     if constexpr (INTERIOR) {
-        const uint32_t lb = 4u * loff;
+        // row y of the tile from its scalar base plus a 32-bit lane offset
+        const uint32_t lb = 4u * g.loff;
 #pragma unroll
         for (int y = 0; y < 8; y++) {
-            const char* rp = reinterpret_cast<const char*>(in + tbase + (uint64_t)y * nx);
-            xv[y] = *reinterpret_cast<const float*>(rp + lb);
+            const char* rp = reinterpret_cast<const char*>(in + g.tbase + (uint64_t)y * nx);
+            xv[y] = *gp(reinterpret_cast<const float*>(rp + lb));
         }
     } else {
 #pragma unroll
-        for (int y = 0; y < 8; y++) xv[y] = (xz && Y0 + y < ny) ? in[Z * plane + (uint64_t)(Y0 + y) * nx + X] : 0.0f;
+        for (int y = 0; y < 8; y++)
+            xv[y] = (g.xz && g.Y0 + y < ny) ? gp(in)[g.Z * g.plane + (uint64_t)(g.Y0 + y) * nx + g.X] : 0.0f;
     }
+}
+
+template <bool INTERIOR>
+__device__ __forceinline__ void lz3d_load_tile(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+                                               uint32_t tile, Tiles tg, float (&xv)[8]) {
+    lz3d_load<INTERIOR>(in, nx, ny, lz3d_geo(nx, ny, nz, tile, tg), xv);
+}
+
+// The tile from its loaded values xv (lz3d_load).
+template <typename TCode, bool ZZ, bool STRICT, bool INTERIOR, bool CODES>
+__device__ __forceinline__ TileCounts lz3d_tile_c(const float (&xv)[8], uint32_t nx, uint32_t ny, uint32_t nz,
+                                                  uint64_t n, uint32_t tile, Tiles tg, bool last, float r, float rad,
+                                                  TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
+    const int l = lane_id();
+    const int x = l & 7, z = l >> 3;
+    const Lz3Geo g = lz3d_geo(nx, ny, nz, tile, tg);
+    const uint32_t X = g.X, Z = g.Z, Y0 = g.Y0;
+    const bool xz = g.xz;
+    const uint64_t plane = g.plane, tbase = g.tbase;
+    const uint32_t loff = g.loff;
+    float s[8];
 #pragma unroll
     for (int y = 0; y < 8; y++) s[y] = quant_round(xv[y], r);
     // a = s - s(z-1); b = a - a(x-1); d = b - b(y-1)   (lorenzo_quant_nd.cu:234-246)
@@ -473,9 +516,9 @@ __device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, 
             const TCode cv = ok ? enc_code<TCode, ZZ>(d[y]) : (TCode)0;
             if constexpr (INTERIOR) {
                 char* cp = reinterpret_cast<char*>(codes + tbase + (uint64_t)y * nx);
-                *reinterpret_cast<TCode*>(cp + cb) = cv;
+                *gp(reinterpret_cast<TCode*>(cp + cb)) = cv;
             } else if (valid) {
-                codes[Z * plane + (uint64_t)(Y0 + y) * nx + X] = cv;
+                gp(codes)[Z * plane + (uint64_t)(Y0 + y) * nx + X] = cv;
             }
         }
         bal[y] = __ballot(valid && !ok);
@@ -497,8 +540,8 @@ __device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, 
                 if ((row >> x) & 1u) {
                     const uint32_t pos = run + __popc(row & below);
                     if (pos < o.cap) {
-                        o.val[pos] = patch ? xv[y] : d[y];
-                        o.idx[pos] = (uint32_t)(Z * plane + (uint64_t)(Y0 + y) * nx + X);
+                        gp(o.val)[pos] = patch ? xv[y] : d[y];
+                        gp(o.idx)[pos] = (uint32_t)(Z * plane + (uint64_t)(Y0 + y) * nx + X);
                     }
                 }
                 run += __popc(row);
@@ -538,14 +581,21 @@ __device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, 
     return tc;
 }
 
+template <typename TCode, bool ZZ, bool STRICT, bool INTERIOR, bool CODES>
+__device__ __forceinline__ TileCounts lz3d_tile_v(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+                                                  uint64_t n, uint32_t tile, Tiles tg, bool last, float r, float rad,
+                                                  TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
+    float xv[8];
+    lz3d_load_tile<INTERIOR>(in, nx, ny, nz, tile, tg, xv);
+    return lz3d_tile_c<TCode, ZZ, STRICT, INTERIOR, CODES>(xv, nx, ny, nz, n, tile, tg, last, r, rad, codes, lo, lp,
+                                                           se);
+}
+
 template <typename TCode, bool ZZ, bool STRICT>
 __device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
                                                 uint64_t n, uint32_t tile, Tiles tg, bool last, float r, float rad,
                                                 TCode* __restrict__ codes, ListOut lo, ListOut lp, StrictEb se) {
-    const uint32_t bx = tile % tg.gx, by = (tile / tg.gx) % tg.gy, bz = tile / (tg.gx * tg.gy);
-    // inside the field, and 8 planes of the tile spanning less than 2^32 bytes
-    const bool interior = bx * 8 + 8 <= nx && by * 8 + 8 <= ny && bz * 8 + 8 <= nz &&
-                          (uint64_t)nx * ny < (1ull << 26);
+    const bool interior = lz3d_interior(nx, ny, nz, tile, tg);
 #define FZH_LZ3(I, C) lz3d_tile_v<TCode, ZZ, STRICT, I, C>(in, nx, ny, nz, n, tile, tg, last, r, rad, codes, lo, lp, se)
     if (interior) return codes ? FZH_LZ3(true, true) : FZH_LZ3(true, false);
     return codes ? FZH_LZ3(false, true) : FZH_LZ3(false, false);

@@ -234,11 +234,11 @@ __device__ __forceinline__ uint32_t rze_encode_chunk_wg(const uint8_t* __restric
         for (int j = 0; j < 16; j++) {
             const int b = 64 * t + 4 * j;
             if (b + 4 <= in_size) {
-                D[j] = *reinterpret_cast<const uint32_t*>(src + b);
+                D[j] = *gp(reinterpret_cast<const uint32_t*>(src + b));
             } else {
                 uint32_t x = 0;
                 for (int k = 0; k < 4; k++)
-                    if (b + k < in_size) x |= (uint32_t)src[b + k] << (8 * k);
+                    if (b + k < in_size) x |= (uint32_t)gp(src)[b + k] << (8 * k);
                 D[j] = x;
             }
         }
@@ -384,7 +384,7 @@ __device__ __forceinline__ uint32_t rze_encode_chunk_wg(const uint8_t* __restric
     const uint32_t nq = (size + 15) / 16;  // size < CS here
     const uint4* sq = reinterpret_cast<const uint4*>(L.st);
     uint4* dq = reinterpret_cast<uint4*>(slot);
-    for (uint32_t i = t; i < nq; i += 256) dq[i] = sq[i];
+    for (uint32_t i = t; i < nq; i += 256) gst_u4(dq + i, sq[i]);
     return size;
 }
 
@@ -400,22 +400,22 @@ __device__ __forceinline__ void copy_to_unaligned16(const uint8_t* src, uint64_t
     const int l = NT == 64 ? lane_id() : (int)threadIdx.x;
     const uint32_t head = (uint32_t)((16 - ((uintptr_t)dst & 15)) & 15);
     const uint32_t hb = head < size ? head : size;
-    if ((uint32_t)l < hb) dst[l] = src[l];
+    if ((uint32_t)l < hb) gp(dst)[l] = gp(src)[l];
     if (size <= hb) return;
     const uint32_t sh = hb & 3;  // source byte hb sits at this offset in its dword
     const uint32_t* sw = reinterpret_cast<const uint32_t*>(src);
     auto word = [&](uint32_t q) -> uint32_t {
-        if (4ull * q + 4 <= limit) return sw[q];
+        if (4ull * q + 4 <= limit) return gp(sw)[q];
         uint32_t v = 0;
         for (uint32_t k = 0; k < 4; k++)
-            if (4ull * q + k < limit) v |= (uint32_t)src[4 * q + k] << (8 * k);
+            if (4ull * q + k < limit) v |= (uint32_t)gp(src)[4 * q + k] << (8 * k);
         return v;
     };
     uint4* dq = reinterpret_cast<uint4*>(dst + hb);
     const uint32_t n16 = (size - hb) / 16;
     for (uint32_t i = l; i < n16; i += NT) {
         const uint32_t q = (hb + 16 * i) >> 2;
-        const uint32_t w0 = sw[q], w1 = sw[q + 1], w2 = sw[q + 2], w3 = sw[q + 3];
+        const uint32_t w0 = gp(sw)[q], w1 = gp(sw)[q + 1], w2 = gp(sw)[q + 2], w3 = gp(sw)[q + 3];
         uint4 v;
         if (sh == 0) {
             v = make_uint4(w0, w1, w2, w3);
@@ -424,10 +424,10 @@ __device__ __forceinline__ void copy_to_unaligned16(const uint8_t* src, uint64_t
             v = make_uint4(__builtin_amdgcn_alignbyte(w1, w0, sh), __builtin_amdgcn_alignbyte(w2, w1, sh),
                            __builtin_amdgcn_alignbyte(w3, w2, sh), __builtin_amdgcn_alignbyte(w4, w3, sh));
         }
-        dq[i] = v;
+        gst_u4(dq + i, v);
     }
     const uint32_t done = hb + 16 * n16;
-    if ((uint32_t)l < size - done) dst[done + l] = src[done + l];
+    if ((uint32_t)l < size - done) gp(dst)[done + l] = gp(src)[done + l];
 }
 
 }  // namespace

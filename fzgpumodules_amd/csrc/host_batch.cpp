@@ -225,6 +225,24 @@ void BatchCompressor::setFields(const std::vector<BatchField>& fields) {
         p.cls_tfield[d] = d_ctf;
         p.cls_F[d] = (uint32_t)(ct0.size() - 1);
         p.cls_T[d] = Td;
+        if (d == 2) {  // 3D tiles in pairs of one field
+            std::vector<uint32_t> pr0{0};
+            for (size_t k = 0; k + 1 < ct0.size(); k++) pr0.push_back(pr0.back() + (ct0[k + 1] - ct0[k] + 1) / 2);
+            const uint32_t P = pr0.back();
+            const size_t pb = std::max<size_t>((P + 63) / 64, 1);
+            std::vector<uint32_t> pf(pb, 0);
+            for (size_t k = 0, j = 0; k < pb && P; k++) {
+                while (j + 2 < pr0.size() && pr0[j + 1] <= 64 * k) j++;
+                pf[k] = (uint32_t)j;
+            }
+            uint32_t* d_pr0 = (uint32_t*)impl_->dev(4 * pr0.size());
+            uint32_t* d_pf = (uint32_t*)impl_->dev(4 * pb);
+            hipck(hipMemcpy(d_pr0, pr0.data(), 4 * pr0.size(), hipMemcpyHostToDevice), "hipMemcpy (tile pairs)");
+            hipck(hipMemcpy(d_pf, pf.data(), 4 * pb, hipMemcpyHostToDevice), "hipMemcpy (pair fields)");
+            p.pair0 = d_pr0;
+            p.pfield = d_pf;
+            p.P = P;
+        }
     }
     std::vector<uint32_t> cfield(std::max<uint64_t>(NC, 1), 0);
     for (uint32_t f = 0; f < F; f++)
