@@ -121,11 +121,10 @@ __global__ void __launch_bounds__(256) gather_kernel(const fzh_copy_desc* __rest
     if (!src || !dst || !bytes) return;
     if (((uintptr_t)src | (uintptr_t)dst) % 16 == 0) {
         const size_t nw = bytes / 16;
-        for (size_t i = threadIdx.x; i < nw; i += blockDim.x)
-            reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
-        for (size_t i = nw * 16 + threadIdx.x; i < bytes; i += blockDim.x) dst[i] = src[i];
+        for (size_t i = threadIdx.x; i < nw; i += blockDim.x) gst_u4(dst + 16 * i, gld_u4(src + 16 * i));
+        for (size_t i = nw * 16 + threadIdx.x; i < bytes; i += blockDim.x) gp(dst)[i] = gp(src)[i];
     } else {
-        for (size_t i = threadIdx.x; i < bytes; i += blockDim.x) dst[i] = src[i];
+        for (size_t i = threadIdx.x; i < bytes; i += blockDim.x) gp(dst)[i] = gp(src)[i];
     }
 }
 

@@ -98,22 +98,25 @@ hist_global_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, uint32_
 // cells per partition slot (every codeword is at most 28 bits)
 constexpr int cells_of(int sl) { return (sl * 28 + 31) / 32 + 2; }
 
-template <typename T, int SL>
+// BOOK_LDS: the codebook is staged in LDS (bklen <= 8192), else read from
+// global memory.  (A template parameter, not a pointer chosen at run time:
+// a pointer that may be either LDS or global makes every lookup a FLAT
+// access.)
+template <typename T, int SL, bool BOOK_LDS>
 __global__ void __launch_bounds__(256)
 huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint32_t* __restrict__ bk4,
                const uint8_t* __restrict__ revbook, uint32_t rev_bytes, uint8_t* __restrict__ out,
-               uint32_t* __restrict__ scells, uint32_t* __restrict__ ncell_out, uint32_t pardeg, int book_in_lds) {
+               uint32_t* __restrict__ scells, uint32_t* __restrict__ ncell_out, uint32_t pardeg) {
     constexpr int CELLS = cells_of(SL);
     constexpr int K = SL / 64;  // symbols per lane
     extern __shared__ uint32_t smem[];
     const int wv = threadIdx.x >> 6;
     uint32_t* cells = smem + wv * CELLS;
     uint32_t* s_bk = smem + 4 * CELLS;
-    if (book_in_lds) {
+    if constexpr (BOOK_LDS) {
         for (uint32_t i = threadIdx.x; i < bklen; i += blockDim.x) s_bk[i] = bk4[i];
         __syncthreads();
     }
-    const uint32_t* book = book_in_lds ? s_bk : bk4;
     const uint32_t ent1 = HDR_BYTES, ent2 = ent1 + rev_bytes;
     const uint32_t p = blockIdx.x * 4 + wv;
     const int l = lane_id();
@@ -129,7 +132,7 @@ huf_enc_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, const uint3
         uint32_t w = 0;
         if (j < cnt) {
             const uint32_t sym = ld_sym(in, s0 + j);
-            if (sym < bklen) w = book[sym];  // an out-of-range symbol (reported by its caller) gets no code
+            if (sym < bklen) w = BOOK_LDS ? s_bk[sym] : bk4[sym];  // an out-of-range symbol (reported by its caller) gets no code
         }
         len[k] = j < cnt ? (w >> 27) : 0u;
         code[k] = w & 0x07FFFFFFu;
@@ -619,11 +622,15 @@ int enc_impl(hipStream_t s, const void* in, size_t n, uint32_t bklen, const uint
     uint32_t* tot = (uint32_t*)(w + L.o_tot);
     uint32_t* scells = (uint32_t*)(w + L.o_cells);
     uint8_t* o8 = (uint8_t*)out;
-    const int book_in_lds = bklen <= 8192 ? 1 : 0;
+    const bool book_in_lds = bklen <= 8192;
     const size_t lds = (size_t)4 * CELLS * 4 + (book_in_lds ? 4 * (size_t)bklen : 0);
     const unsigned blocks = (unsigned)cdiv(pardeg, 4);
-    launch_k("huf_enc_kernel", huf_enc_kernel<T, SL>, dim3(blocks), dim3(256), (uint32_t)lds, s, (const T*)in, n, bklen, bk4, rev,
-             rev_bytes, o8, scells, ncell, pardeg, book_in_lds);
+    if (book_in_lds)
+        launch_k("huf_enc_kernel", huf_enc_kernel<T, SL, true>, dim3(blocks), dim3(256), (uint32_t)lds, s, (const T*)in, n,
+                 bklen, bk4, rev, rev_bytes, o8, scells, ncell, pardeg);
+    else
+        launch_k("huf_enc_kernel", huf_enc_kernel<T, SL, false>, dim3(blocks), dim3(256), (uint32_t)lds, s, (const T*)in,
+                 n, bklen, bk4, rev, rev_bytes, o8, scells, ncell, pardeg);
     if (int rc = scan_u32(s, ncell, reinterpret_cast<uint32_t*>(o8 + ent3), pardeg, nullptr, tot, w + L.o_scan))
         return rc;
     unsigned long long* nbit_acc = reinterpret_cast<unsigned long long*>(w + L.o_nbit);
