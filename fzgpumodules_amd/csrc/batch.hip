@@ -244,33 +244,9 @@ __global__ void __launch_bounds__(256) lz_batch_tile_kernel(const Plan* __restri
     }
 }
 
-// 3D fields: two consecutive tiles of one field per wave, both tiles' loads
-// issued before either is computed, so a wave keeps 4 KiB of input in
-// flight as a 2D or 1D tile's wave does (one 8x8x8 tile holds only 2 KiB).
-template <bool ZZ, bool STRICT>
-__device__ __forceinline__ void tile3_finish(const Plan& p, const Field& fd, uint32_t f, uint32_t lt, bool interior,
-                                             const float (&xv)[8]) {
-    const float* prm = p.params + 4 * (size_t)f;
-    const float r = rflf(gp(prm)[2]);
-    const StrictEb se{rflf(gp(prm)[1]), rflf(gp(prm)[3])};
-    const Tiles tg{fd.gx, fd.gy, 0};
-    const uint32_t g = fd.tile0 + lt;
-    const uint64_t sl = (uint64_t)g * kListSlots;
-    const ListOut lo{p.lerr + sl, p.lidx + sl, kListSlots};
-    const ListOut lp = STRICT ? ListOut{p.lpval + sl, p.lpidx + sl, kListSlots} : ListOut{nullptr, nullptr, 0};
-    uint16_t* codes = p.codes + fd.codes_off;
-    const bool last = lt + 1 == fd.ntiles;
-    const TileCounts tc =
-        interior ? lz3d_tile_c<uint16_t, ZZ, STRICT, true, true>(xv, fd.nx, fd.ny, fd.nz, fd.n, lt, tg, last, r, p.rad,
-                                                                 codes, lo, lp, se)
-                 : lz3d_tile_c<uint16_t, ZZ, STRICT, false, true>(xv, fd.nx, fd.ny, fd.nz, fd.n, lt, tg, last, r,
-                                                                  p.rad, codes, lo, lp, se);
-    if (lane_id() == 0) {
-        gp(p.counts)[g] = tc.cnt;
-        if constexpr (STRICT) gp(p.counts)[p.T + g] = tc.pcnt;
-    }
-}
-
+// 3D fields: one wave per pair of x-adjacent tiles of one field
+// (lz3d_pair: a load instruction covers 4 rows of 64 bytes instead of 8 of
+// 32), tile lists, counts and codes as lz3d_tile gives them.
 template <bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256) lz_batch_tile3_kernel(const Plan* __restrict__ pp) {
     const Plan& p = *pp;
@@ -281,21 +257,32 @@ __global__ void __launch_bounds__(256) lz_batch_tile3_kernel(const Plan* __restr
     while (u >= rfl(gp(pr0)[j + 1])) j++;
     const uint32_t f = rfl(gp(p.cls_field[2])[j]);
     const Field fd = uniform_field(p.fields + f);
-    const Tiles tg{fd.gx, fd.gy, 0};
-    const uint32_t lta = 2 * (u - rfl(gp(pr0)[j])), ltb = lta + 1;
-    const bool two = ltb < fd.ntiles;
-    float xa[8], xb[8];
-    const bool ia = lz3d_interior(fd.nx, fd.ny, fd.nz, lta, tg);
-    if (ia) lz3d_load_tile<true>(fd.in, fd.nx, fd.ny, fd.nz, lta, tg, xa);
-    else lz3d_load_tile<false>(fd.in, fd.nx, fd.ny, fd.nz, lta, tg, xa);
-    bool ib = false;
-    if (two) {
-        ib = lz3d_interior(fd.nx, fd.ny, fd.nz, ltb, tg);
-        if (ib) lz3d_load_tile<true>(fd.in, fd.nx, fd.ny, fd.nz, ltb, tg, xb);
-        else lz3d_load_tile<false>(fd.in, fd.nx, fd.ny, fd.nz, ltb, tg, xb);
+    const uint32_t lp = u - rfl(gp(pr0)[j]);
+    const uint32_t gx2 = (fd.gx + 1) / 2;
+    const uint32_t bx2 = lp % gx2, by = (lp / gx2) % fd.gy, bz = lp / (gx2 * fd.gy);
+    const uint32_t ta = 2 * bx2 + fd.gx * (by + fd.gy * bz), tb = ta + 1;
+    const bool has_b = 2 * bx2 + 1 < fd.gx;
+    const float* prm = p.params + 4 * (size_t)f;
+    const float r = rflf(gp(prm)[2]);
+    const StrictEb se{rflf(gp(prm)[1]), rflf(gp(prm)[3])};
+    const uint32_t ga = fd.tile0 + ta, gb = fd.tile0 + tb;
+    const uint64_t sa = (uint64_t)ga * kListSlots, sb = (uint64_t)gb * kListSlots;
+    Lz3PairOut o;
+    o.loA = ListOut{p.lerr + sa, p.lidx + sa, kListSlots};
+    o.lpA = STRICT ? ListOut{p.lpval + sa, p.lpidx + sa, kListSlots} : ListOut{nullptr, nullptr, 0};
+    o.loB = has_b ? ListOut{p.lerr + sb, p.lidx + sb, kListSlots} : ListOut{nullptr, nullptr, 0};
+    o.lpB = (STRICT && has_b) ? ListOut{p.lpval + sb, p.lpidx + sb, kListSlots} : ListOut{nullptr, nullptr, 0};
+    const bool last = ta + 1 == fd.ntiles || (has_b && tb + 1 == fd.ntiles);
+    lz3d_pair<uint16_t, ZZ, STRICT>(fd.in, fd.nx, fd.ny, fd.nz, fd.n, bx2, by, bz, last, r, p.rad,
+                                    p.codes + fd.codes_off, se, o);
+    if (lane_id() == 0) {
+        gp(p.counts)[ga] = o.A.cnt;
+        if constexpr (STRICT) gp(p.counts)[p.T + ga] = o.A.pcnt;
+        if (has_b) {
+            gp(p.counts)[gb] = o.B.cnt;
+            if constexpr (STRICT) gp(p.counts)[p.T + gb] = o.B.pcnt;
+        }
     }
-    tile3_finish<ZZ, STRICT>(p, fd, f, lta, ia, xa);
-    if (two) tile3_finish<ZZ, STRICT>(p, fd, f, ltb, ib, xb);
 }
 
 // ---------------------------------------------------------------------------

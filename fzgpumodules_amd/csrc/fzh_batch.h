@@ -69,9 +69,9 @@ struct Plan {
     const uint32_t* cls_tile0[3];  // cls_F[d] + 1
     const uint32_t* cls_tfield[3]; // ceil(cls_T[d] / 64)
     uint32_t cls_F[3], cls_T[3];
-    // 3D fields' tiles in pairs (tiles 2k, 2k + 1 of one field; a field with
-    // an odd tile count ends with a single): prefix of the pair counts over
-    // the 3D class fields, and the class field holding pair 64 k.
+    // 3D fields' tiles in x-adjacent pairs (tiles (2 bx2, by, bz) and the
+    // next, ceil(gx / 2) gy gz pairs per field): prefix of the pair counts
+    // over the 3D class fields, and the class field holding pair 64 k.
     const uint32_t* pair0;   // cls_F[2] + 1
     const uint32_t* pfield;  // ceil(P / 64)
     uint32_t P;

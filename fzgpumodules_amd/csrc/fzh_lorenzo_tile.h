@@ -602,6 +602,173 @@ __device__ __forceinline__ TileCounts lz3d_tile(const float* __restrict__ in, ui
 #undef FZH_LZ3
 }
 
+// ---------------------------------------------------------------------------
+// 3D, two x-adjacent tiles per wave: tiles A = (2 bx2, by, bz) and B =
+// A + 1 of the tile grid (B absent when 2 bx2 + 1 == gx), for pair
+// bx2 + gx2 (by + gy bz), gx2 = ceil(gx / 2).  Lane (x 0..15, yl 0..3)
+// holds column X = 16 bx2 + x of rows Y0 + yl (h = 0) and Y0 + 4 + yl
+// (h = 1) of the 8 planes: v[z][h] = element (X, Y0 + 4 h + yl, Z0 + z).
+// A load instruction then covers 4 rows of 64 contiguous bytes; a lone
+// tile's wave (lz3d_tile_v) covers 8 rows of 32 bytes, which the memory
+// pipeline delivers at about 60% of the rate.  Codes, outlier and patch
+// lists and counts are those of lz3d_tile_v for A and for B.
+// ---------------------------------------------------------------------------
+struct Lz3PairOut {
+    ListOut loA, lpA, loB, lpB;
+    TileCounts A, B;
+};
+
+__device__ __forceinline__ bool lz3d_pair_interior(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t bx2, uint32_t by,
+                                                   uint32_t bz) {
+    return bx2 * 16 + 16 <= nx && by * 8 + 8 <= ny && bz * 8 + 8 <= nz && (uint64_t)nx * ny < (1ull << 26);
+}
+
+template <typename TCode, bool ZZ, bool STRICT, bool INTERIOR>
+__device__ __forceinline__ void lz3d_pair_v(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+                                            uint64_t n, uint32_t bx2, uint32_t by, uint32_t bz, bool last, float r,
+                                            float rad, TCode* __restrict__ codes, StrictEb se, Lz3PairOut& o) {
+    const int l = lane_id();
+    const int x = l & 15, yl = l >> 4;
+    const uint32_t X = bx2 * 16 + x, Y0 = by * 8, Z0 = bz * 8;
+    const bool xin = X < nx;
+    const uint64_t plane = (uint64_t)nx * ny;
+    const uint64_t tbase = (uint64_t)Z0 * plane + (uint64_t)Y0 * nx + bx2 * 16;
+    const uint32_t loff = (uint32_t)yl * nx + (uint32_t)x;
+    float xv[8][2];
+#pragma unroll
+    for (int z = 0; z < 8; z++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if constexpr (INTERIOR) {
+                const char* rp = reinterpret_cast<const char*>(in + tbase + (uint64_t)z * plane + (uint64_t)(4 * h) * nx);
+                xv[z][h] = *gp(reinterpret_cast<const float*>(rp + 4u * loff));
+            } else {
+                const uint32_t Y = Y0 + 4 * h + yl, Z = Z0 + z;
+                xv[z][h] = (xin && Y < ny && Z < nz) ? gp(in)[(uint64_t)Z * plane + (uint64_t)Y * nx + X] : 0.0f;
+            }
+        }
+    // a = s - s(z-1); b = a - a(x-1); d = b - b(y-1)   (lorenzo_quant_nd.cu:234-246)
+    float s[8][2], b[8][2];
+#pragma unroll
+    for (int z = 0; z < 8; z++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) s[z][h] = quant_round(xv[z][h], r);
+#pragma unroll
+    for (int z = 0; z < 8; z++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const float a = s[z][h] - (z > 0 ? s[z > 0 ? z - 1 : 0][h] : 0.0f);
+            const float aw = wave_shr1(a);  // lane x - 1 (x & 7 > 0: same tile)
+            b[z][h] = (x & 7) ? a - aw : a;
+        }
+    float d[8][2];
+    uint64_t bal[8][2];
+    const uint32_t cb = (uint32_t)sizeof(TCode) * loff;
+    const int src_h1 = yl == 0 ? l + 48 : l - 16;  // row y - 1 for h = 1: (h 0, yl 3) or (h 1, yl - 1)
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        const float up0 = __shfl_up(b[z][0], 16, 64);
+        const float m = yl == 3 ? b[z][0] : b[z][1];
+        const float up1 = __shfl(m, src_h1, 64);
+        d[z][0] = yl > 0 ? b[z][0] - up0 : b[z][0];
+        d[z][1] = b[z][1] - up1;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t Y = Y0 + 4 * h + yl, Z = Z0 + z;
+            const bool valid = INTERIOR || (xin && Y < ny && Z < nz);
+            const bool ok = fabsf(d[z][h]) < rad;
+            const TCode cv = ok ? enc_code<TCode, ZZ>(d[z][h]) : (TCode)0;
+            if constexpr (INTERIOR) {
+                char* cp = reinterpret_cast<char*>(codes + tbase + (uint64_t)z * plane + (uint64_t)(4 * h) * nx);
+                *gp(reinterpret_cast<TCode*>(cp + cb)) = cv;
+            } else if (valid) {
+                gp(codes)[(uint64_t)Z * plane + (uint64_t)Y * nx + X] = cv;
+            }
+            bal[z][h] = __ballot(valid && !ok);
+        }
+    }
+    // Order inside a tile: (z, y, x); registers in (z, h) order and lanes in
+    // (yl, x) order walk it, the tile's lanes being those with x & 8 equal.
+    constexpr uint64_t kMaskA = 0x00FF00FF00FF00FFull;
+    const uint64_t below = (1ull << l) - 1ull;
+    auto emit = [&](const uint64_t (&bb)[8][2], bool patch, uint64_t mask, const ListOut& lo) {
+        uint32_t tot = 0;
+#pragma unroll
+        for (int z = 0; z < 8; z++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) tot += __popcll(bb[z][h] & mask);
+        if (tot) {
+            uint32_t run = 0;
+#pragma unroll
+            for (int z = 0; z < 8; z++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const uint64_t mm = bb[z][h] & mask;
+                    if ((mm >> l) & 1ull) {
+                        const uint32_t pos = run + __popcll(mm & below);
+                        if (pos < lo.cap) {
+                            gp(lo.val)[pos] = patch ? xv[z][h] : d[z][h];
+                            gp(lo.idx)[pos] = (uint32_t)((uint64_t)(Z0 + z) * plane + (uint64_t)(Y0 + 4 * h + yl) * nx + X);
+                        }
+                    }
+                    run += __popcll(mm);
+                }
+        }
+        return tot;
+    };
+    o.A.cnt = emit(bal, false, kMaskA, o.loA);
+    o.B.cnt = emit(bal, false, ~kMaskA, o.loB);
+    if (last) zero_code_tail(codes, (uint64_t)nx * ny * nz, n);
+    if constexpr (STRICT) {
+        bool huge = false;
+#pragma unroll
+        for (int z = 0; z < 8; z++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) huge |= !(fabsf(s[z][h]) < kExactS);
+        const uint64_t hb = __ballot(huge);
+        const bool tile_huge = (x & 8) ? (hb & ~kMaskA) != 0ull : (hb & kMaskA) != 0ull;
+        const StrictBand band = strict_band(se);
+        uint64_t pbal[8][2];
+        uint64_t unsure = 0;
+#pragma unroll
+        for (int z = 0; z < 8; z++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const bool valid = INTERIOR || (xin && Y0 + 4 * h + yl < ny && Z0 + z < nz);
+                bool sure, uns;
+                patch_fast(xv[z][h], s[z][h], se, band, sure, uns);
+                pbal[z][h] = __ballot(valid && (tile_huge || sure));
+                unsure |= __ballot(valid && !tile_huge && uns);
+            }
+        if (unsure) {  // rare: elements within 2^-22 of the bound take the exact test
+#pragma unroll
+            for (int z = 0; z < 8; z++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const bool valid = INTERIOR || (xin && Y0 + 4 * h + yl < ny && Z0 + z < nz);
+                    bool sure, uns;
+                    patch_fast(xv[z][h], s[z][h], se, band, sure, uns);
+                    pbal[z][h] = __ballot(valid && (tile_huge || sure ||
+                                                    (uns && needs_patch(xv[z][h], s[z][h], se.eb, se.ebx2))));
+                }
+        }
+        o.A.pcnt = emit(pbal, true, kMaskA, o.lpA);
+        o.B.pcnt = emit(pbal, true, ~kMaskA, o.lpB);
+    } else {
+        o.A.pcnt = o.B.pcnt = 0;
+    }
+}
+
+template <typename TCode, bool ZZ, bool STRICT>
+__device__ __forceinline__ void lz3d_pair(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz,
+                                          uint64_t n, uint32_t bx2, uint32_t by, uint32_t bz, bool last, float r,
+                                          float rad, TCode* __restrict__ codes, StrictEb se, Lz3PairOut& o) {
+    if (lz3d_pair_interior(nx, ny, nz, bx2, by, bz))
+        lz3d_pair_v<TCode, ZZ, STRICT, true>(in, nx, ny, nz, n, bx2, by, bz, last, r, rad, codes, se, o);
+    else
+        lz3d_pair_v<TCode, ZZ, STRICT, false>(in, nx, ny, nz, n, bx2, by, bz, last, r, rad, codes, se, o);
+}
+
 // Device-side resolution of the error bound from the value base, shared by
 // the single-field and batched reductions (lorenzo_quant.cu:612-644):
 //   params = [value_base, abs_eb, 1/(2 abs_eb), 2 abs_eb]

@@ -225,9 +225,13 @@ void BatchCompressor::setFields(const std::vector<BatchField>& fields) {
         p.cls_tfield[d] = d_ctf;
         p.cls_F[d] = (uint32_t)(ct0.size() - 1);
         p.cls_T[d] = Td;
-        if (d == 2) {  // 3D tiles in pairs of one field
+        if (d == 2) {  // 3D tiles in x-adjacent pairs of one field: ceil(gx / 2) gy gz per field
             std::vector<uint32_t> pr0{0};
-            for (size_t k = 0; k + 1 < ct0.size(); k++) pr0.push_back(pr0.back() + (ct0[k + 1] - ct0[k] + 1) / 2);
+            for (size_t k = 0; k + 1 < ct0.size(); k++) {
+                const Field& fk = tab[cf[k]];
+                const uint64_t gz = fzh::cdiv(fk.nz, 8);
+                pr0.push_back(pr0.back() + (uint32_t)((uint64_t)((fk.gx + 1) / 2) * fk.gy * gz));
+            }
             const uint32_t P = pr0.back();
             const size_t pb = std::max<size_t>((P + 63) / 64, 1);
             std::vector<uint32_t> pf(pb, 0);

@@ -95,20 +95,31 @@ lz2d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint64_t
     store_counts<STRICT>(tl, tile, tc);
 }
 
+// 3D: one wave per pair of x-adjacent tiles (lz3d_pair), four pairs per
+// workgroup; P = ceil(gx / 2) gy gz pairs.
 template <typename TCode, bool ZZ, bool STRICT>
 __global__ void __launch_bounds__(256)
 lz3d_fwd_kernel(const float* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t nz, uint64_t n,
                 float r_host, const float* __restrict__ d_r, float rad, TCode* __restrict__ codes, TileLists tl,
-                Tiles tg, uint32_t T) {
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
-    if (tile >= T) return;
+                Tiles tg, uint32_t T, uint32_t P) {
+    const uint32_t pair = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
+    if (pair >= P) return;
+    const uint32_t gx2 = (tg.gx + 1) / 2;
+    const uint32_t bx2 = pair % gx2, by = (pair / gx2) % tg.gy, bz = pair / (gx2 * tg.gy);
+    const uint32_t ta = 2 * bx2 + tg.gx * (by + tg.gy * bz), tb = ta + 1;
+    const bool has_b = 2 * bx2 + 1 < tg.gx;
     const float r = d_r ? *d_r : r_host;
-    const uint64_t b = (uint64_t)tile * 512;
-    const TileCounts tc = lz3d_tile<TCode, ZZ, STRICT>(in, nx, ny, nz, n, tile, tg, tile == T - 1, r, rad, codes,
-                                                       ListOut{tl.err + b, tl.idx + b, 512u},
-                                                       ListOut{tl.pval + b, tl.pidx + b, 512u},
-                                                       STRICT ? strict_eb(tl) : StrictEb{0.0f, 0.0f});
-    store_counts<STRICT>(tl, tile, tc);
+    const uint64_t ba = (uint64_t)ta * 512, bb = (uint64_t)tb * 512;
+    Lz3PairOut o;
+    o.loA = ListOut{tl.err + ba, tl.idx + ba, 512u};
+    o.lpA = ListOut{tl.pval + ba, tl.pidx + ba, 512u};
+    o.loB = has_b ? ListOut{tl.err + bb, tl.idx + bb, 512u} : ListOut{nullptr, nullptr, 0u};
+    o.lpB = has_b ? ListOut{tl.pval + bb, tl.pidx + bb, 512u} : ListOut{nullptr, nullptr, 0u};
+    const bool last = ta == T - 1 || (has_b && tb == T - 1);
+    lz3d_pair<TCode, ZZ, STRICT>(in, nx, ny, nz, n, bx2, by, bz, last, r, rad, codes,
+                                 STRICT ? strict_eb(tl) : StrictEb{0.0f, 0.0f}, o);
+    store_counts<STRICT>(tl, ta, o.A);
+    if (has_b) store_counts<STRICT>(tl, tb, o.B);
 }
 
 // ===========================================================================
@@ -666,6 +677,7 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
     TCode* c = (TCode*)codes;
     const int nd = ndim_of(ny, nz);
     const size_t nv = (nd == 1 && nx > 0 && nx < n) ? nx : n;
+    const size_t P3 = nd == 3 ? cdiv(cdiv(nx, 8), 2) * cdiv(ny, 8) * cdiv(nz, 8) : 0;  // 3D tile pairs
 #define FZH_LZ_FWD(ZZV)                                                                                 \
     if (nd == 1)                                                                                        \
         launch_k("lz1d_fwd_kernel", lz1d_fwd_kernel<TCode, ZZV, STRICT>, dim3(blocks), dim3(256), 0, s, in, n, \
@@ -674,9 +686,9 @@ int launch_fwd(hipStream_t s, const float* in, size_t n, size_t nx, size_t ny, s
         launch_k("lz2d_fwd_kernel", lz2d_fwd_kernel<TCode, ZZV, STRICT>, dim3(blocks), dim3(256), 0, s, in, \
                  (uint32_t)nx, (uint32_t)ny, n, r, d_r, rad, c, tl, (uint32_t)cdiv(nx, 32), (uint32_t)T);  \
     else                                                                                                \
-        launch_k("lz3d_fwd_kernel", lz3d_fwd_kernel<TCode, ZZV, STRICT>, dim3(blocks), dim3(256), 0, s, in,  \
-                 (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, n, r, d_r, rad, c, tl,                       \
-                 Tiles{(uint32_t)cdiv(nx, 8), (uint32_t)cdiv(ny, 8), 0}, (uint32_t)T);
+        launch_k("lz3d_fwd_kernel", lz3d_fwd_kernel<TCode, ZZV, STRICT>, dim3((unsigned)cdiv(P3, 4)), dim3(256), 0, \
+                 s, in, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, n, r, d_r, rad, c, tl,               \
+                 Tiles{(uint32_t)cdiv(nx, 8), (uint32_t)cdiv(ny, 8), 0}, (uint32_t)T, (uint32_t)P3);
     if (zz) {
         FZH_LZ_FWD(true)
     } else {
