@@ -498,131 +498,175 @@ lz2d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, float
 // 3D inverse, the addition tree of lorenzo_quant_nd.cu:282-360: outlier +
 // decoded code, a sequential prefix along y, then Hillis-Steele prefixes
 // (offsets 1, 2, 4) along x and then along z, times ebx2.  One wave per
-// tile: lane (x, z) = (l & 7, l >> 3) holds rows y = 0..7 for the y prefix
-// and the x prefix (DPP within 8-lane groups); the tile then passes through
-// LDS ([z][y][x], 72 floats per z plane: 2 lanes per bank) to lane
-// (x, y) = (l & 7, l >> 3) holding z = 0..7, where the z prefix is register
-// arithmetic.  Outliers are added in the same LDS tile, one entry per lane;
-// strict-mode patches of a short list are stored after the tile's values.
+// pair of x-adjacent tiles (as the forward): lane (x, zl) = (l & 15,
+// l >> 4) holds column 16 bx2 + x, rows y = 0..7 of planes zl and 4 + zl,
+// so loads and stores cover 4 rows of contiguous bytes.  The y prefix is
+// register arithmetic, the x prefix DPP inside each tile's 8 lanes of a
+// 16-lane row, the z prefix lane shuffles by 16 and 32 (offsets 1, 2) and
+// the other register (offset 4).  Outliers are added in LDS tiles
+// ([tile][z][y][x], 72 floats per z plane), one entry per lane;
+// strict-mode patches of a short list are stored after the tiles' values.
 template <typename TCode, bool ZZ, bool DIRECT, bool PATCH>
 __global__ void __launch_bounds__(256)
 lz3d_inv_kernel(const TCode* __restrict__ codes, uint32_t nx, uint32_t ny, uint32_t nz, float ebx2,
                 const uint32_t* __restrict__ off, const float* __restrict__ serr,
-                const uint32_t* __restrict__ sloc, float* __restrict__ out, Tiles tg, uint32_t T,
+                const uint32_t* __restrict__ sloc, float* __restrict__ out, Tiles tg, uint32_t T, uint32_t P,
                 DirectOut dout, DirectOut pin) {
-    constexpr uint32_t ZS = 72;  // LDS floats per z plane of the tile
-    __shared__ float s_tile[4][8 * ZS];
-    const uint32_t tile = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
-    if (tile >= T) return;
+    constexpr uint32_t ZS = 72;  // LDS floats per z plane of a tile
+    __shared__ float s_tile[4][2 * 8 * ZS];
+    const uint32_t pair = xcd_swizzle(blockIdx.x, gridDim.x) * 4 + wave_in_wg();
+    if (pair >= P) return;
     float* st = s_tile[wave_in_wg()];
     const int l = lane_id();
-    const int x = l & 7, z = l >> 3;
-    const uint32_t bx = tile % tg.gx, by = (tile / tg.gx) % tg.gy, bz = tile / (tg.gx * tg.gy);
+    const int x = l & 15, zl = l >> 4, xt = x & 7;
+    const uint32_t gx2 = (tg.gx + 1) / 2;
+    const uint32_t bx2 = pair % gx2, by = (pair / gx2) % tg.gy, bz = pair / (gx2 * tg.gy);
+    const uint32_t ta = 2 * bx2 + tg.gx * (by + tg.gy * bz), tb = ta + 1;
+    const bool has_b = 2 * bx2 + 1 < tg.gx;
+    const uint32_t X = bx2 * 16 + x, Y0 = by * 8, Z0 = bz * 8;
     const uint64_t plane = (uint64_t)nx * ny;
-    const uint64_t tbase = (uint64_t)(bz * 8) * plane + (uint64_t)(by * 8) * nx + bx * 8;
-    const bool interior = bx * 8 + 8 <= nx && by * 8 + 8 <= ny && bz * 8 + 8 <= nz && plane < (1ull << 26);
-    // -- decoded codes, lane (x, z), row y
-    float d[8];
-    {
-        const bool xz = bx * 8 + x < nx && bz * 8 + z < nz;
-        const uint32_t lo = (uint32_t)z * (uint32_t)plane + (uint32_t)x;
-        if (interior) {
+    const uint64_t tbase = (uint64_t)Z0 * plane + (uint64_t)Y0 * nx + bx2 * 16;
+    const bool interior = bx2 * 16 + 16 <= nx && Y0 + 8 <= ny && Z0 + 8 <= nz && plane < (1ull << 26);
+    const bool xin = X < nx;
+    const uint32_t lo = (uint32_t)zl * (uint32_t)plane + (uint32_t)x;  // lane offset in plane Z0 + zl, row Y0
+    // -- decoded codes: v[y][h] = element (X, Y0 + y, Z0 + 4 h + zl)
+    float v[8][2];
+    if (interior) {
+#pragma unroll
+        for (int h = 0; h < 2; h++)
 #pragma unroll
             for (int y = 0; y < 8; y++)
-                d[y] = dec_code<TCode, ZZ>(*reinterpret_cast<const TCode*>(
-                    reinterpret_cast<const char*>(codes + tbase + (uint64_t)y * nx) + (uint32_t)sizeof(TCode) * lo));
-        } else {
+                v[y][h] = dec_code<TCode, ZZ>(*gp(reinterpret_cast<const TCode*>(
+                    reinterpret_cast<const char*>(codes + tbase + (uint64_t)(4 * h) * plane + (uint64_t)y * nx) +
+                    (uint32_t)sizeof(TCode) * lo)));
+    } else {
 #pragma unroll
-            for (int y = 0; y < 8; y++)
-                d[y] = (xz && by * 8 + y < ny) ? dec_code<TCode, ZZ>(codes[tbase + (uint64_t)z * plane + (uint64_t)y * nx + x])
-                                               : 0.0f;
-        }
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int y = 0; y < 8; y++) {
+                const uint32_t Z = Z0 + 4 * h + zl, Y = Y0 + y;
+                v[y][h] = (xin && Y < ny && Z < nz)
+                              ? dec_code<TCode, ZZ>(gp(codes)[(uint64_t)Z * plane + (uint64_t)Y * nx + X])
+                              : 0.0f;
+            }
     }
-    // -- outliers: value + decoded code, in the LDS tile (one entry per lane)
+    // -- outliers: value + decoded code, in the LDS tiles [t][z][y][x]
+    const uint32_t lbase = (uint32_t)(x >> 3) * (8 * ZS) + (uint32_t)zl * ZS + (uint32_t)xt;  // + 4 h ZS + 8 y
     bool any = false;
-    uint32_t o0 = 0, o1 = 0, kk = 0;
+    uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0, kk = 0;
     if constexpr (DIRECT) {
         kk = dout.count ? min(*dout.count, dout.max_out) : 0u;
         any = kk > 0;
     } else {
-        o0 = off[tile];
-        o1 = off[tile + 1];
-        any = o1 > o0;
+        a0 = off[ta];
+        a1 = off[ta + 1];
+        if (has_b) {
+            b0 = a1;
+            b1 = off[tb + 1];
+        }
+        any = a1 > a0 || b1 > b0;
     }
     if (any) {
 #pragma unroll
-        for (int y = 0; y < 8; y++) st[z * ZS + y * 8 + x] = d[y];
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int y = 0; y < 8; y++) st[lbase + 4 * h * ZS + 8 * y] = v[y][h];
         wave_lds_sync();
         if constexpr (DIRECT) {
             for (uint32_t j = (uint32_t)l; j < kk; j += 64) {
                 uint32_t t, loc;
-                if (tile_of(dout.m, dout.idx[j], t, loc) && t == tile) {
-                    const uint32_t a = (loc >> 6) * ZS + (loc & 63u);
+                if (tile_of(dout.m, dout.idx[j], t, loc) && (t == ta || (has_b && t == tb))) {
+                    const uint32_t a = (t == ta ? 0u : 8 * ZS) + (loc >> 6) * ZS + (loc & 63u);
                     st[a] = dout.err[j] + st[a];
                 }
             }
         } else {
-            for (uint32_t j = o0 + (uint32_t)l; j < o1; j += 64) {
+            for (uint32_t j = a0 + (uint32_t)l; j < a1; j += 64) {
                 const uint32_t loc = sloc[j];
                 const uint32_t a = (loc >> 6) * ZS + (loc & 63u);
+                st[a] = serr[j] + st[a];
+            }
+            for (uint32_t j = b0 + (uint32_t)l; j < b1; j += 64) {
+                const uint32_t loc = sloc[j];
+                const uint32_t a = 8 * ZS + (loc >> 6) * ZS + (loc & 63u);
                 st[a] = serr[j] + st[a];
             }
         }
         wave_lds_sync();
 #pragma unroll
-        for (int y = 0; y < 8; y++) d[y] = st[z * ZS + y * 8 + x];
-        wave_lds_sync();
-    }
-    // -- y prefix (sequential), x prefix (offsets 1, 2, 4 inside the 8-lane group)
+        for (int h = 0; h < 2; h++)
 #pragma unroll
-    for (int y = 1; y < 8; y++) d[y] += d[y - 1];
+            for (int y = 0; y < 8; y++) v[y][h] = st[lbase + 4 * h * ZS + 8 * y];
+    }
+    // -- y prefix (sequential, registers)
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int y = 1; y < 8; y++) v[y][h] += v[y - 1][h];
+    // -- x prefix (offsets 1, 2, 4 inside each tile's 8-lane group of a 16-lane row)
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            float w = v[y][h];
+            float u = __uint_as_float(dpp_u32<0x111>(__float_as_uint(w)));  // lane x - 1 (row_shr:1)
+            if (xt >= 1) w += u;
+            u = __uint_as_float(dpp_u32<0x112>(__float_as_uint(w)));        // x - 2
+            if (xt >= 2) w += u;
+            u = __uint_as_float(dpp_u32<0x114>(__float_as_uint(w)));        // x - 4
+            if (xt >= 4) w += u;
+            v[y][h] = w;
+        }
+    // -- z prefix, offsets 1, 2, 4 (each stage adds the previous stage's
+    // values): z = 4 h + zl, lanes 16 apart per plane; offset 4 is h - 1
+    const int src1 = zl == 0 ? l + 48 : l - 16;  // z - 1 for h = 1: (h 0, zl 3) or (h 1, zl - 1)
+    const int src2 = zl < 2 ? l + 32 : l - 32;   // z - 2 for h = 1: (h 0, zl + 2) or (h 1, zl - 2)
 #pragma unroll
     for (int y = 0; y < 8; y++) {
-        float v = d[y];
-        float u = __uint_as_float(dpp_u32<0x111>(__float_as_uint(v)));  // lane x - 1 (row_shr:1)
-        if (x >= 1) v += u;
-        u = __uint_as_float(dpp_u32<0x112>(__float_as_uint(v)));        // x - 2
-        if (x >= 2) v += u;
-        u = __uint_as_float(dpp_u32<0x114>(__float_as_uint(v)));        // x - 4
-        if (x >= 4) v += u;
-        st[z * ZS + y * 8 + x] = v;
-    }
-    wave_lds_sync();
-    // -- lane (x, y): z = 0..7; z prefix in registers (descending, so each
-    // stage adds the previous stage's values)
-    const int yy = l >> 3;
-    float v[8];
-#pragma unroll
-    for (int zz = 0; zz < 8; zz++) v[zz] = st[zz * ZS + yy * 8 + x];
-#pragma unroll
-    for (int o = 1; o < 8; o <<= 1)
-#pragma unroll
-        for (int zz = 7; zz >= o; zz--) v[zz] += v[zz - o];
-    {
-        const bool xy = bx * 8 + x < nx && by * 8 + yy < ny;
-        const uint32_t lo = (uint32_t)yy * nx + (uint32_t)x;
-        if (interior) {
-#pragma unroll
-            for (int zz = 0; zz < 8; zz++)
-                *reinterpret_cast<float*>(reinterpret_cast<char*>(out + tbase + (uint64_t)zz * plane) + 4u * lo) =
-                    v[zz] * ebx2;
-        } else {
-#pragma unroll
-            for (int zz = 0; zz < 8; zz++)
-                if (xy && bz * 8 + zz < nz) out[tbase + (uint64_t)zz * plane + lo] = v[zz] * ebx2;
+        float v0 = v[y][0], v1 = v[y][1];
+        {
+            const float u0 = __shfl_up(v0, 16, 64);
+            const float u1 = __shfl(zl == 3 ? v0 : v1, src1, 64);
+            v0 = zl >= 1 ? v0 + u0 : v0;
+            v1 = v1 + u1;
         }
+        {
+            const float u0 = __shfl_up(v0, 32, 64);
+            const float u1 = __shfl(zl >= 2 ? v0 : v1, src2, 64);
+            v0 = zl >= 2 ? v0 + u0 : v0;
+            v1 = v1 + u1;
+        }
+        v1 = v1 + v0;
+        v[y][0] = v0;
+        v[y][1] = v1;
     }
-    // -- strict-mode patches of a short list: stored after the tile's values
+    if (interior) {
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int y = 0; y < 8; y++)
+                *gp(reinterpret_cast<float*>(reinterpret_cast<char*>(out + tbase + (uint64_t)(4 * h) * plane +
+                                                                    (uint64_t)y * nx) + 4u * lo)) = v[y][h] * ebx2;
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int y = 0; y < 8; y++) {
+                const uint32_t Z = Z0 + 4 * h + zl, Y = Y0 + y;
+                if (xin && Y < ny && Z < nz) gp(out)[(uint64_t)Z * plane + (uint64_t)Y * nx + X] = v[y][h] * ebx2;
+            }
+    }
+    // -- strict-mode patches of a short list: stored after the tiles' values
     // (a wave's stores to one address land in program order)
     if constexpr (PATCH) {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // the tile's stores first
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // the tiles' stores first
         const uint32_t kp = pin.count ? min(*pin.count, pin.max_out) : 0u;
         for (uint32_t j = (uint32_t)l; j < kp; j += 64) {
             const uint32_t q = pin.idx[j];
             uint32_t t, loc;
             if (tile_of(pin.m, q, t, loc)) {
-                if (t == tile) out[q] = pin.err[j];
-            } else if (tile == 0 && (uint64_t)q >= pin.m.total && (uint64_t)q < pin.n_out) {
+                if (t == ta || (has_b && t == tb)) out[q] = pin.err[j];
+            } else if (ta == 0 && (uint64_t)q >= pin.m.total && (uint64_t)q < pin.n_out) {
                 out[q] = pin.err[j];
             }
         }
@@ -718,6 +762,7 @@ int inv_apply(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny, 
     const unsigned blocks = (unsigned)cdiv(L.T, 4);
     const TCode* c = (const TCode*)codes;
     const bool direct = max_out <= kDirectOutliers;
+    const size_t P3 = nd == 3 ? cdiv(cdiv(nx, 8), 2) * cdiv(ny, 8) * cdiv(nz, 8) : 0;  // 3D tile pairs
     const DirectOut dout{err, idx, (err && idx) ? count : nullptr, (uint32_t)max_out, m, 0};
     // short patch lists ride along with short outlier lists (one launch)
     const bool fold = patches && patches->max_out > 0 && direct && patches->max_out <= kDirectOutliers;
@@ -731,9 +776,9 @@ int inv_apply(hipStream_t s, const void* codes, size_t n, size_t nx, size_t ny, 
         launch_k("lz2d_inv_kernel", lz2d_inv_kernel<TCode, ZZV, DV, PV>, dim3(blocks), dim3(256), 0, s, c, \
                  (uint32_t)nx, (uint32_t)ny, ebx2, off, serr, sloc, out, m.gx, (uint32_t)L.T, dout, pin); \
     else                                                                                                \
-        launch_k("lz3d_inv_kernel", lz3d_inv_kernel<TCode, ZZV, DV, PV>, dim3(blocks), dim3(256), 0, s, c, \
-                 (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, ebx2, off, serr, sloc, out, Tiles{m.gx, m.gy, 0}, \
-                 (uint32_t)L.T, dout, pin);
+        launch_k("lz3d_inv_kernel", lz3d_inv_kernel<TCode, ZZV, DV, PV>, dim3((unsigned)cdiv(P3, 4)), dim3(256), 0, s, \
+                 c, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, ebx2, off, serr, sloc, out, Tiles{m.gx, m.gy, 0}, \
+                 (uint32_t)L.T, (uint32_t)P3, dout, pin);
     if (zz) {
         if (fold) { FZH_LZ_INV(true, true, true) }
         else if (direct) { FZH_LZ_INV(true, true, false) }
