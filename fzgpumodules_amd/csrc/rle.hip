@@ -19,9 +19,11 @@
 //     scratch.)
 //   decode: a three-launch segmented scan of the run lengths gives the run
 //     starts and a map "output tile -> first run overlapping it" (every run
-//     marks the tile boundaries it covers); then one wave per 4096-element
-//     output tile stages that tile's runs in LDS, each lane binary-searches
-//     its first run and fills 64 consecutive outputs.  No host round trip
+//     marks the tile boundaries it covers; the lengths and starts pass
+//     through LDS so global accesses stay coalesced); then one wave per
+//     1024-element output tile stages that tile's runs in LDS (33 KiB per
+//     four-wave workgroup), each lane binary-searches its first run and
+//     fills 16 consecutive outputs.  No host round trip
 //     (the reference syncs twice and fills with one thread per run).
 #include "fzh_scan.h"
 
@@ -29,7 +31,7 @@ namespace fzh {
 namespace {
 
 constexpr int STEPS = 8;
-constexpr uint32_t DTILE = 4096;  // decode output tile (elements)
+constexpr uint32_t DTILE = 1024;  // decode output tile (elements), one wave each
 
 // Run-start flags of one wave's tile: mask[j] bit k <=> element
 // base + 64 E j + E l + k starts a run (k < E, E elements per 16-byte load).
@@ -241,14 +243,26 @@ rle_dec_apply_kernel(const uint8_t* __restrict__ in, uint64_t n, const uint32_t*
                      const uint32_t* __restrict__ seg_base, const uint32_t* __restrict__ total,
                      uint32_t* __restrict__ start, uint32_t* __restrict__ tile_first, int* __restrict__ d_status) {
     __shared__ uint32_t s_w[4];
+    // the segment's lengths, then its starts, in LDS: loaded and stored
+    // coalesced, read by thread t as its 16 consecutive runs (one padding
+    // word per 16 keeps those reads on distinct banks)
+    __shared__ uint32_t s_x[SCAN_SEG + SCAN_SEG / 16];
+    auto px = [](uint32_t i) { return i + (i >> 4); };
     const uint32_t R = *Rv;
     const uint32_t* lens = rle_lengths(in, R, sizeof(T));
     const int tid = threadIdx.x, w = tid >> 6;
-    const uint64_t b = (uint64_t)blockIdx.x * SCAN_SEG + 16ull * tid;
+    const uint64_t seg0 = (uint64_t)blockIdx.x * SCAN_SEG;
+    const uint64_t b = seg0 + 16ull * tid;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint32_t i = (uint32_t)(256 * k + tid);
+        s_x[px(i)] = (seg0 + i < R) ? lens[seg0 + i] : 0u;
+    }
+    __syncthreads();
     uint32_t v[16], sum = 0;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        v[k] = (b + k < R) ? lens[b + k] : 0u;
+        v[k] = s_x[px(16 * tid + k)];
         sum += v[k];
     }
     const uint32_t incl = wave_incl_scan(sum);
@@ -259,15 +273,19 @@ rle_dec_apply_kernel(const uint8_t* __restrict__ in, uint64_t n, const uint32_t*
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         const uint64_t r = b + k;
-        if (r < R) {
-            const uint64_t s0 = run, e = run + v[k];
-            start[r] = (uint32_t)(s0 < n ? s0 : n);
-            if (s0 < n && v[k] > 0) {
-                const uint64_t e2 = e < n ? e : n;
-                for (uint64_t t = (s0 + DTILE - 1) / DTILE; t * DTILE < e2; t++) tile_first[t] = (uint32_t)r;
-            }
+        const uint64_t s0 = run, e = run + v[k];
+        s_x[px(16 * tid + k)] = (uint32_t)(s0 < n ? s0 : n);  // after this thread's own reads
+        if (r < R && s0 < n && v[k] > 0) {
+            const uint64_t e2 = e < n ? e : n;
+            for (uint64_t t = (s0 + DTILE - 1) / DTILE; t * DTILE < e2; t++) tile_first[t] = (uint32_t)r;
         }
         run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint32_t i = (uint32_t)(256 * k + tid);
+        if (seg0 + i < R) start[seg0 + i] = s_x[px(i)];
     }
     if (blockIdx.x == 0 && tid == 0) {
         const uint32_t tot = *total;
@@ -277,14 +295,14 @@ rle_dec_apply_kernel(const uint8_t* __restrict__ in, uint64_t n, const uint32_t*
 }
 
 template <typename T>
-__global__ void __launch_bounds__(128)
+__global__ void __launch_bounds__(256)
 rle_dec_expand_kernel(const uint8_t* __restrict__ in, uint64_t n, const uint32_t* __restrict__ start,
                       const uint32_t* __restrict__ tile_first, const int* __restrict__ d_status, T* __restrict__ out,
                       uint32_t ntiles) {
-    __shared__ uint32_t s_start[2][DTILE + 2];
-    __shared__ T s_val[2][DTILE + 2];
+    __shared__ uint32_t s_start[4][DTILE + 2];
+    __shared__ T s_val[4][DTILE + 2];
     const int wv = threadIdx.x >> 6;
-    const uint32_t t = blockIdx.x * 2 + wv;
+    const uint32_t t = blockIdx.x * 4 + wv;
     if (t >= ntiles || *d_status != 0) return;
     const int l = lane_id();
     const uint32_t R = *reinterpret_cast<const uint32_t*>(in);
@@ -300,8 +318,9 @@ rle_dec_expand_kernel(const uint8_t* __restrict__ in, uint64_t n, const uint32_t
         s_val[wv][i] = vals[r0 + i];
     }
     wave_lds_sync();
-    // lane l fills [o0 + 64 l, o0 + 64 l + 64)
-    const uint64_t my0 = o0 + 64ull * l;
+    // lane l fills [o0 + LSPAN l, o0 + LSPAN l + LSPAN)
+    constexpr uint32_t LSPAN = DTILE / 64;
+    const uint64_t my0 = o0 + (uint64_t)LSPAN * l;
     if (my0 < o1) {
         // last run with start <= my0
         uint32_t lo = 0, hi = mm;  // invariant: s_start[lo] <= my0 < s_start[hi] (hi may be mm)
@@ -312,10 +331,10 @@ rle_dec_expand_kernel(const uint8_t* __restrict__ in, uint64_t n, const uint32_t
         uint32_t r = lo;
         uint64_t next = (r + 1 < mm) ? s_start[wv][r + 1] : (r0 + r + 1 <= R ? (uint64_t)start[r0 + r + 1] : n);
         T cur = s_val[wv][r];
-        const uint64_t my1 = (my0 + 64 < o1) ? my0 + 64 : o1;
+        const uint64_t my1 = (my0 + LSPAN < o1) ? my0 + LSPAN : o1;
         constexpr int E = 16 / sizeof(T);
-        if (my1 - my0 == 64) {
-            for (int q = 0; q < 64 / E; q++) {
+        if (my1 - my0 == LSPAN) {
+            for (int q = 0; q < (int)LSPAN / E; q++) {
                 T buf[E];
 #pragma unroll
                 for (int k = 0; k < E; k++) {
@@ -428,8 +447,8 @@ int dec_impl(hipStream_t s, const void* in, size_t in_bytes, void* out, size_t n
     launch_k("rle_dec_reduce_kernel", rle_dec_reduce_kernel<T>, dim3((unsigned)L.nseg), dim3(256), 0, s, bin, Rv, seg);
     launch_k("scan_partials_kernel", scan_partials_kernel<false>, dim3(1), dim3(1024), 0, s, seg, (uint32_t)L.nseg, tot);
     launch_k("rle_dec_apply_kernel", rle_dec_apply_kernel<T>, dim3((unsigned)L.nseg), dim3(256), 0, s, bin, n, Rv, seg, tot, start, tf, d_status);
-    const unsigned eb = (unsigned)cdiv(L.ntiles, 2);
-    launch_k("rle_dec_expand_kernel", rle_dec_expand_kernel<T>, dim3(eb), dim3(128), 0, s, (const uint8_t*)in, n, start, tf, d_status, (T*)out, (uint32_t)L.ntiles);
+    const unsigned eb = (unsigned)cdiv(L.ntiles, 4);
+    launch_k("rle_dec_expand_kernel", rle_dec_expand_kernel<T>, dim3(eb), dim3(256), 0, s, (const uint8_t*)in, n, start, tf, d_status, (T*)out, (uint32_t)L.ntiles);
     FZH_CHECK_LAUNCH("rle decode");
     return FZH_OK;
 }
