@@ -104,9 +104,15 @@ rle_enc_write_kernel(const T* __restrict__ in, uint64_t n, const uint32_t* __res
                      uint32_t ntiles) {
     constexpr int E = 16 / sizeof(T);
     constexpr uint32_t TILE = 64 * E * STEPS;
+    constexpr uint32_t SR = 64 * E;  // most run starts of one step
+    // a step's runs staged in LDS, then stored contiguously (coalesced):
+    // value of run r + i, length of run r - 1 + i
+    __shared__ T s_v[4][SR];
+    __shared__ uint32_t s_l[4][SR];
     const uint32_t tile = blockIdx.x * 4 + wave_in_wg();
     if (tile >= ntiles) return;
     const int l = lane_id();
+    const int wv = (int)wave_in_wg();
     const uint64_t base = (uint64_t)tile * TILE;
     T v[STEPS][E];
     uint32_t mask[STEPS];
@@ -126,17 +132,26 @@ rle_enc_write_kernel(const T* __restrict__ in, uint64_t n, const uint32_t* __res
         const uint32_t lp1 = m ? e0 + (31u - (uint32_t)__clz(m)) + 1u : 0u;  // last start in this lane + 1
         const uint32_t pmax = wave_incl_max(lp1);
         uint32_t prev1 = max(wave_shr1(pmax), last1);  // previous start in tile order + 1
-        uint32_t rr = r + incl - c;
+        uint32_t q = incl - c;  // this lane's first run within the step
 #pragma unroll
         for (int k = 0; k < E; k++) {
             if (m & (1u << k)) {
                 const uint32_t e = e0 + (uint32_t)k;
-                vals[rr] = v[j][k];
-                if (prev1) lens[rr - 1] = e - (prev1 - 1u);
+                s_v[wv][q] = v[j][k];
+                s_l[wv][q] = prev1 ? e - (prev1 - 1u) : 0u;
                 prev1 = e + 1u;
-                rr++;
+                q++;
             }
         }
+        const uint32_t ctot = lane63(incl);
+        wave_lds_sync();
+        // the step's first run closes the run before it only when that run
+        // started in this tile (else the fix kernel closes it)
+        for (uint32_t i = (uint32_t)l; i < ctot; i += 64) {
+            vals[r + i] = s_v[wv][i];
+            if (i > 0 || last1) lens[r - 1 + i] = s_l[wv][i];
+        }
+        wave_lds_sync();
         const uint64_t any = __ballot(m != 0u);
         if (!first1 && any) {
             const int fl = __builtin_ctzll(any);
