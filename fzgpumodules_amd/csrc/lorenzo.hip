@@ -156,7 +156,17 @@ minmax_partial_kernel(const float* __restrict__ in, uint64_t n, float2* __restri
         if (v.w > mx) mx = v.w;
     };
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // four independent loads in flight per thread
+    // large fields: eight independent 16-byte loads in flight per thread
+    // (the grid is at most 512 workgroups: 16 MiB in flight, what the HBM
+    // latency needs); small ones keep four (fewer, shorter rounds)
+    if (n4 >= 64 * stride)
+    for (; i + 7 * stride < n4; i += 8 * stride) {
+        float4 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = in4[i + k * stride];
+#pragma unroll
+        for (int k = 0; k < 8; k++) fold(v[k]);
+    }
     for (; i + 3 * stride < n4; i += 4 * stride) {
         const float4 a = in4[i], b = in4[i + stride], c = in4[i + 2 * stride], d = in4[i + 3 * stride];
         fold(a);
