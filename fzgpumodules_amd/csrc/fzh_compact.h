@@ -79,9 +79,14 @@ place_kernel(CompactListT<V> L0, CompactListT<V> L1, uint32_t T, uint32_t E, con
             *L0.total = q0;
             if constexpr (NL == 2) *L1.total = q1;
         }
-    } else if (wave == 0 && l == 0) {
-        *L0.total = excl[T];
-        if constexpr (NL == 2) *L1.total = excl[2 * T] - excl[T];
+    } else {
+        // every lane's destination in one load (not one dependent load per
+        // tile inside the placement loop)
+        if (c) ex = excl[(uint64_t)li * T + t] - (li ? excl[T] : 0u);
+        if (wave == 0 && l == 0) {
+            *L0.total = excl[T];
+            if constexpr (NL == 2) *L1.total = excl[2 * T] - excl[T];
+        }
     }
     if (tb >= T) return;
     uint64_t todo = __ballot(c != 0u);
@@ -91,9 +96,7 @@ place_kernel(CompactListT<V> L0, CompactListT<V> L1, uint32_t T, uint32_t E, con
         const uint32_t cj = __shfl(c, j, 64);
         const uint32_t lj = (uint32_t)j >> 5;
         const uint32_t tile = tb + (uint32_t)(j & 31);
-        uint32_t dst0;
-        if constexpr (INL) dst0 = (uint32_t)__shfl((int)ex, j, 64);
-        else dst0 = excl[(uint64_t)lj * T + tile] - (lj ? excl[T] : 0u);
+        const uint32_t dst0 = (uint32_t)__shfl((int)ex, j, 64);
         const uint64_t src0 = (uint64_t)tile * E;
         const V* sv = lj ? L1.s_val : L0.s_val;
         const uint32_t* si = lj ? L1.s_idx : L0.s_idx;

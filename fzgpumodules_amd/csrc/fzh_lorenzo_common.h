@@ -69,11 +69,36 @@ FwdWs fwd_layout(size_t n, size_t nx, size_t ny, size_t nz, bool strict, size_t 
 // ===========================================================================
 // Inverse: outlier binning by tile (counting sort), then per-tile decode
 // ===========================================================================
+// Division of any 32-bit n by a divisor d fixed for a launch, as a multiply
+// (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, fig. 4.1): with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1, n / d = (t + ((n - t) >> s1)) >> s2,
+// t = mulhi(m, n), s1 = min(l, 1), s2 = max(l - 1, 0).  A few instructions
+// where a division by a run-time value takes several dozen.
+struct FastDiv {
+    uint32_t d, m, s1, s2;
+};
+inline FastDiv fast_div(uint32_t d) {
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < (uint64_t)d) l++;
+    FastDiv f;
+    f.d = d;
+    f.m = (uint32_t)(((((1ull << l) - d) << 32) / d) + 1);
+    f.s1 = l ? 1u : 0u;
+    f.s2 = l ? l - 1u : 0u;
+    return f;
+}
+__device__ __forceinline__ uint32_t div_by(uint32_t n, const FastDiv& f) {
+    const uint32_t t = __umulhi(f.m, n);
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
 struct TileMap {
     int ndim;
     uint32_t nx, ny, nz;
     uint32_t gx, gy;
     uint64_t total;  // elements covered by the tiles' valid region
+    FastDiv dx, dy;  // division by nx and by ny
 };
 
 __device__ __forceinline__ bool tile_of(const TileMap& m, uint32_t idx, uint32_t& tile, uint32_t& local) {
@@ -82,12 +107,12 @@ __device__ __forceinline__ bool tile_of(const TileMap& m, uint32_t idx, uint32_t
         tile = idx >> 10;
         local = idx & 1023u;
     } else if (m.ndim == 2) {
-        const uint32_t x = idx % m.nx, y = idx / m.nx;
+        const uint32_t y = div_by(idx, m.dx), x = idx - y * m.nx;
         tile = (y >> 5) * m.gx + (x >> 5);
         local = ((y & 31u) << 5) | (x & 31u);
     } else {
-        const uint32_t x = idx % m.nx, yz = idx / m.nx;
-        const uint32_t y = yz % m.ny, z = yz / m.ny;
+        const uint32_t yz = div_by(idx, m.dx), x = idx - yz * m.nx;
+        const uint32_t z = div_by(yz, m.dy), y = yz - z * m.ny;
         tile = ((z >> 3) * m.gy + (y >> 3)) * m.gx + (x >> 3);
         local = ((z & 7u) << 6) | ((y & 7u) << 3) | (x & 7u);
     }
@@ -207,6 +232,8 @@ TileMap tile_map(size_t n, size_t nx, size_t ny, size_t nz) {
     m.gx = (uint32_t)(nd == 2 ? cdiv(nx, 32) : cdiv(nx, 8));
     m.gy = (uint32_t)cdiv(ny, 8);
     m.total = nd == 3 ? (uint64_t)nx * ny * nz : nd == 2 ? (uint64_t)nx * ny : n;
+    m.dx = fast_div((uint32_t)std::max<size_t>(nx, 1));
+    m.dy = fast_div((uint32_t)std::max<size_t>(ny, 1));
     return m;
 }
 
