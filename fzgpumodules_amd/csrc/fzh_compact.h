@@ -47,7 +47,8 @@ place_kernel(CompactListT<V> L0, CompactListT<V> L1, uint32_t T, uint32_t E, con
     const uint32_t t = tb + (uint32_t)(l & 31);
     uint32_t c = 0;
     if (t < T && li < (uint32_t)NL) c = L0.cnt[(uint64_t)li * T + t];
-    uint32_t ex = 0;  // INL: this lane's tile's exclusive prefix within its list
+    uint32_t ex = 0;  // this lane's tile's exclusive prefix within its list
+    const uint32_t inc = wave_incl_scan(c);
     if constexpr (INL) {
         __shared__ uint32_t s_sum[4 * NL];
         __shared__ uint32_t s_wave[4][2];
@@ -56,7 +57,6 @@ place_kernel(CompactListT<V> L0, CompactListT<V> L1, uint32_t T, uint32_t E, con
 #pragma unroll
         for (int k = 0; k < NL; k++) lists[k] = L0.cnt + (uint64_t)k * T;
         wg256_sums_before_u32<NL>(lists, min(blockIdx.x * 128u, T), s_sum, before);
-        const uint32_t inc = wave_incl_scan(c);
         const uint32_t h0 = (uint32_t)__builtin_amdgcn_readlane((int)inc, 31);  // list 0 part of the wave
         const uint32_t h1 = lane63(inc) - h0;                                    // list 1 part
         if (l == 0) {
@@ -89,25 +89,34 @@ place_kernel(CompactListT<V> L0, CompactListT<V> L1, uint32_t T, uint32_t E, con
         }
     }
     if (tb >= T) return;
-    uint64_t todo = __ballot(c != 0u);
-    while (todo) {
-        const int j = __builtin_ctzll(todo);
-        todo &= todo - 1;
-        const uint32_t cj = __shfl(c, j, 64);
-        const uint32_t lj = (uint32_t)j >> 5;
-        const uint32_t tile = tb + (uint32_t)(j & 31);
+    // The wave's entries (its lanes' tiles' lists, in lane order) spread
+    // over the lanes, 64 at a time: entry g belongs to the first lane j
+    // whose inclusive count exceeds g (a binary search over the lanes'
+    // counts), so a round's 64 copies are independent.
+    const uint32_t total = lane63(inc);
+    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+        const uint32_t g = g0 + (uint32_t)l;
+        int j = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if ((uint32_t)__shfl((int)inc, j + step - 1, 64) <= g) j += step;
+        const uint32_t cj = (uint32_t)__shfl((int)c, j, 64);
+        const uint32_t ij = (uint32_t)__shfl((int)inc, j, 64);
         const uint32_t dst0 = (uint32_t)__shfl((int)ex, j, 64);
-        const uint64_t src0 = (uint64_t)tile * E;
-        const V* sv = lj ? L1.s_val : L0.s_val;
-        const uint32_t* si = lj ? L1.s_idx : L0.s_idx;
-        V* dv = lj ? L1.val : L0.val;
-        uint32_t* di = lj ? L1.idx : L0.idx;
-        const uint64_t cap = lj ? L1.cap : L0.cap;
-        for (uint32_t k = (uint32_t)l; k < cj; k += 64) {
+        if (g < total) {
+            const uint32_t k = g - (ij - cj);
+            const uint32_t lj = (uint32_t)j >> 5;
+            const uint32_t tile = tb + (uint32_t)(j & 31);
+            const uint64_t src = (uint64_t)tile * E + k;
+            const V* sv = lj ? L1.s_val : L0.s_val;
+            const uint32_t* si = lj ? L1.s_idx : L0.s_idx;
+            V* dv = lj ? L1.val : L0.val;
+            uint32_t* di = lj ? L1.idx : L0.idx;
+            const uint64_t cap = lj ? L1.cap : L0.cap;
             const uint64_t pos = (uint64_t)dst0 + k;
             if (pos < cap) {
-                if (sv) dv[pos] = sv[src0 + k];
-                di[pos] = si[src0 + k];
+                if (sv) dv[pos] = sv[src];
+                di[pos] = si[src];
             }
         }
     }
