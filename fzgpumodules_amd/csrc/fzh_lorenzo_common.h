@@ -133,11 +133,25 @@ __global__ void __launch_bounds__(256)
 bin_check_kernel(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ count, uint64_t max_out,
                  TileMap m, uint32_t T, uint32_t* __restrict__ flag, uint32_t* __restrict__ dn, uint32_t tag) {
     const uint64_t k = min((uint64_t)*count, max_out);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     bool unordered = false;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t t, loc, tp, lp;
-        if (!tile_of(m, idx[i], t, loc)) unordered = true;
-        else if (i > 0 && (!tile_of(m, idx[i - 1], tp, lp) || tp > t)) unordered = true;
+    // four entries per step, their loads issued together
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < k; i0 += 4 * stride) {
+        uint32_t q[4], qp[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint64_t i = i0 + u * stride;
+            q[u] = i < k ? idx[i] : 0u;
+            qp[u] = (i < k && i > 0) ? idx[i - 1] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint64_t i = i0 + u * stride;
+            if (i >= k) continue;
+            uint32_t t, loc, tp, lp;
+            if (!tile_of(m, q[u], t, loc)) unordered = true;
+            else if (i > 0 && (!tile_of(m, qp[u], tp, lp) || tp > t)) unordered = true;
+        }
     }
     if (__ballot(unordered) && lane_id() == 0) {
         *flag = tag;
@@ -179,16 +193,31 @@ bin_place_kernel(const V* __restrict__ err, const uint32_t* __restrict__ idx,
         return;
     }
     // in tile order: entry i starts the range of every tile after its
-    // predecessor's tile up to its own (entry k: every tile up to T)
-    for (uint64_t i = gid; i <= k; i += stride) {
-        uint32_t t = T, loc = 0, tp = 0, lp;
-        if (i < k) (void)tile_of(m, idx[i], t, loc);
-        const bool head = i == 0 || (tile_of(m, idx[i - 1], tp, lp), tp != t);
-        if (head)
-            for (uint32_t u = i == 0 ? 0u : tp + 1; u <= t; u++) off[u] = (uint32_t)i;
-        if (i < k) {
-            serr[i] = err[i];
-            sloc[i] = loc;
+    // predecessor's tile up to its own (entry k: every tile up to T); four
+    // entries per step, their loads issued together
+    for (uint64_t i0 = gid; i0 <= k; i0 += 4 * stride) {
+        uint32_t q[4], qp[4];
+        V e[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint64_t i = i0 + u * stride;
+            q[u] = i < k ? idx[i] : 0u;
+            qp[u] = (i <= k && i > 0) ? idx[i - 1] : 0u;
+            e[u] = i < k ? err[i] : V(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint64_t i = i0 + u * stride;
+            if (i > k) continue;
+            uint32_t t = T, loc = 0, tp = 0, lp;
+            if (i < k) (void)tile_of(m, q[u], t, loc);
+            const bool head = i == 0 || (tile_of(m, qp[u], tp, lp), tp != t);
+            if (head)
+                for (uint32_t w = i == 0 ? 0u : tp + 1; w <= t; w++) off[w] = (uint32_t)i;
+            if (i < k) {
+                serr[i] = e[u];
+                sloc[i] = loc;
+            }
         }
     }
     if (gid == 0) *dn = 0;
