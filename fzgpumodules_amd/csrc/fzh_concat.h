@@ -74,10 +74,20 @@ __device__ __forceinline__ void concat_readback_body(const ConcatArgs& a, int n,
             continue;
         }
         uint4* dst = reinterpret_cast<uint4*>(d8);
+        const uint4* src4 = reinterpret_cast<const uint4*>(src);
         const uint64_t full = sz / 16;
-        for (uint64_t w = g; w < nw; w += G) {
+        uint64_t w = g;
+        // four words per step, loads issued before the stores
+        for (; w + 3 * G < full; w += 4 * G) {
+            const uint4 v0 = src4[w], v1 = src4[w + G], v2 = src4[w + 2 * G], v3 = src4[w + 3 * G];
+            dst[w] = v0;
+            dst[w + G] = v1;
+            dst[w + 2 * G] = v2;
+            dst[w + 3 * G] = v3;
+        }
+        for (; w < nw; w += G) {
             if (w < full) {
-                dst[w] = reinterpret_cast<const uint4*>(src)[w];
+                dst[w] = src4[w];
             } else {
                 uint32_t v[4] = {0, 0, 0, 0};
                 for (uint64_t b = 16 * w; b < sz; b++) v[(b & 15) >> 2] |= (uint32_t)src[b] << (8 * (b & 3));

@@ -189,7 +189,15 @@ huf_pack_kernel(const uint32_t* __restrict__ scells, const uint32_t* __restrict_
         const uint32_t c = ncell[p];
         const uint32_t* src = scells + (uint64_t)p * cells;
         uint32_t* dst = reinterpret_cast<uint32_t*>(out + ent4) + e;
-        for (uint32_t i = l; i < c; i += 64) dst[i] = src[i];
+        // up to four loads in flight per lane before their stores
+        for (uint32_t i = l; i < c; i += 256) {
+            uint32_t v[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) v[k] = i + 64 * k < c ? src[i + 64 * k] : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (i + 64 * k < c) dst[i + 64 * k] = v[k];
+        }
         nb = reinterpret_cast<const uint32_t*>(out + ent2)[p];
     }
     if (l == 0) s_nb[w] = nb;
