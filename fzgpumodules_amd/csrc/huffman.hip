@@ -54,22 +54,32 @@ hist_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, uint32_t* __re
     constexpr int E = 16 / sizeof(T);
     const uint64_t nv = n / E;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
-        const uint4 w = reinterpret_cast<const uint4*>(in)[v];
-        const T* p = reinterpret_cast<const T*>(&w);
-        uint32_t cur = (uint32_t)p[0], cnt = 1;
+    // four 16-byte loads per thread and step, issued before the atomics
+    for (uint64_t v0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v0 < nv; v0 += 4 * stride) {
+        uint4 w[4];
 #pragma unroll
-        for (int k = 1; k < E; k++) {
-            const uint32_t s = (uint32_t)p[k];
-            if (s == cur) {
-                cnt++;
-            } else {
-                if (cur < bklen) atomicAdd(&h[cur], cnt);
-                cur = s;
-                cnt = 1;
-            }
+        for (int u = 0; u < 4; u++) {
+            const uint64_t v = v0 + (uint64_t)u * stride;
+            w[u] = v < nv ? reinterpret_cast<const uint4*>(in)[v] : make_uint4(0, 0, 0, 0);
         }
-        if (cur < bklen) atomicAdd(&h[cur], cnt);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (v0 + (uint64_t)u * stride >= nv) continue;
+            const T* p = reinterpret_cast<const T*>(&w[u]);
+            uint32_t cur = (uint32_t)p[0], cnt = 1;
+#pragma unroll
+            for (int k = 1; k < E; k++) {
+                const uint32_t s = (uint32_t)p[k];
+                if (s == cur) {
+                    cnt++;
+                } else {
+                    if (cur < bklen) atomicAdd(&h[cur], cnt);
+                    cur = s;
+                    cnt = 1;
+                }
+            }
+            if (cur < bklen) atomicAdd(&h[cur], cnt);
+        }
     }
     for (uint64_t i = nv * E + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t s = (uint32_t)in[i];
