@@ -665,6 +665,7 @@ def selector_workload(args):
         "single_chain_baseline": S.CHAINS[kfix],
         "regret_single_chain": round(single[kfix] / best.sum() - 1, 5),
         "choice_accuracy": round(float(np.mean(choice == np.argmin(s_te, axis=1))), 4),
+        "learned_choice_counts": [int(v) for v in np.bincount(choice, minlength=4)],
         "select_gbs_learned": round(in_bytes / t_learn / 1e9, 2),
         "select_gbs_exhaustive": round(in_bytes / t_exh / 1e9, 2),
         "select_speedup": round(t_exh / t_learn, 2),

@@ -29,6 +29,7 @@
 #include <thread>
 #include <vector>
 
+#include "fz/batch.h"
 #include "fz/pipeline.h"
 #include "fz/stages.h"
 #include "fzh_internal.h"
@@ -233,7 +234,19 @@ class ChainSelectCompressor {
                         wk.chain_eb[k] = abs_eb;
                     }
                 }
+        // the chunks on chain 1 (LorenzoQuant -> Bitshuffle -> RZE) go through
+        // fz::BatchCompressor, one launch per stage for all of them, whose
+        // per-field bytes equal fz::Pipeline::compress of each chunk; the
+        // last such chunk still runs through its pipeline on a worker, so the
+        // container's stage records come from a real run and its bytes check
+        // the batch's
+        std::vector<uint64_t> c1;
+        for (uint64_t c = 0; c < n; c++)
+            if (choice_[c] == 1) c1.push_back(c);
+        const bool batch1 = c1.size() >= kBatchMin;
+        const uint64_t c1_last = batch1 ? c1.back() : 0;
         std::vector<std::vector<uint8_t>> blobs(n);
+        std::vector<uint8_t> batch_last;
         std::vector<std::exception_ptr> errs(W);
         std::vector<std::thread> th;
         for (int w = 0; w < W; w++)
@@ -241,6 +254,7 @@ class ChainSelectCompressor {
                 try {
                     Worker& wk = *workers_[w];
                     for (uint64_t c = (uint64_t)w; c < n; c += (uint64_t)W) {
+                        if (batch1 && choice_[c] == 1 && c != c1_last) continue;
                         void* d_out = nullptr;
                         size_t bytes = 0;
                         wk.chain[choice_[c]].p->compress((const float*)chunks_.p + c * E, 4 * E, &d_out, &bytes, wk.s);
@@ -253,9 +267,20 @@ class ChainSelectCompressor {
                     errs[w] = std::current_exception();
                 }
             });
+        std::exception_ptr batch_err;
+        if (batch1) {
+            try {
+                run_batch1(s, abs_eb, c1, blobs, batch_last);
+            } catch (...) {
+                batch_err = std::current_exception();
+            }
+        }
         for (auto& t : th) t.join();
         for (auto& e : errs)
             if (e) std::rethrow_exception(e);
+        if (batch_err) std::rethrow_exception(batch_err);
+        if (batch1 && batch_last != blobs[c1_last])
+            throw std::runtime_error("selector: batched chain-1 bytes differ from the pipeline's");
         const double t2 = now_ms();
         // container: chain k's stages are described by the pipeline that
         // compressed the last chunk choosing k (chunk c went to worker c % W)
@@ -390,7 +415,65 @@ class ChainSelectCompressor {
     }
 
   private:
+    // Compresses the chunks `c1` (all on chain 1) in one batch on stream s:
+    // every result but the last into blobs, the last into `last`.  The batch
+    // plan is kept while the chunk list and the bound stay the same.
+    void run_batch1(hipStream_t s, float abs_eb, const std::vector<uint64_t>& c1,
+                    std::vector<std::vector<uint8_t>>& blobs, std::vector<uint8_t>& last) {
+        const uint64_t E = (uint64_t)cx_ * cy_ * cz_;
+        if (!batch_ || batch_eb_ != abs_eb || batch_idx_ != c1 || batch_base_ != chunks_.p) {
+            batch_.reset();
+            BatchConfig cfg;
+            cfg.error_bound = abs_eb;
+            cfg.eb_mode = ErrorBoundMode::ABS;
+            cfg.quant_radius = FZS_LZ_RADIUS;
+            cfg.outlier_capacity = FZS_OUTLIER_CAP;
+            cfg.zigzag = true;
+            cfg.strict = false;
+            auto b = std::make_unique<BatchCompressor>(cfg);
+            std::vector<BatchField> f(c1.size());
+            for (size_t i = 0; i < c1.size(); i++) {
+                f[i].d_in = (const float*)chunks_.p + c1[i] * E;
+                f[i].nx = cx_;
+                f[i].ny = cy_;
+                f[i].nz = cz_;
+            }
+            b->setFields(f);
+            b->enableGraphMode(false);  // the plan changes with the choices; no capture to amortise
+            batch_ = std::move(b);
+            batch_eb_ = abs_eb;
+            batch_idx_ = c1;
+            batch_base_ = chunks_.p;
+        }
+        batch_->compress(s);
+        // the results packed back to back (16-byte steps) by one gather
+        // launch, then one copy to the host
+        const uint8_t* d = batch_->output();
+        const size_t F = c1.size();
+        std::vector<fzh_copy_desc> desc(F);
+        std::vector<uint64_t> pos(F + 1, 0);
+        for (size_t i = 0; i < F; i++) {
+            const uint64_t b = batch_->fieldBytes(i);
+            pos[i + 1] = pos[i] + ((b + 15) & ~15ull);
+        }
+        pack_.ensure(pos[F]);
+        pack_desc_.ensure(sizeof(fzh_copy_desc) * F);
+        for (size_t i = 0; i < F; i++)
+            desc[i] = {d + batch_->fieldOffset(i), (uint8_t*)pack_.p + pos[i], batch_->fieldBytes(i)};
+        hipck(hipMemcpyAsync(pack_desc_.p, desc.data(), sizeof(fzh_copy_desc) * F, hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync (batch descriptors)");
+        fzck(fzh_gather(s, (const fzh_copy_desc*)pack_desc_.p, (int)F), "gather (batch)");
+        pack_host_.resize(pos[F]);
+        hipck(hipMemcpyAsync(pack_host_.data(), pack_.p, pos[F], hipMemcpyDeviceToHost, s), "hipMemcpyAsync (batch)");
+        hipck(hipStreamSynchronize(s), "hipStreamSynchronize (batch)");
+        for (size_t i = 0; i < F; i++) {
+            std::vector<uint8_t>& dst = i + 1 < F ? blobs[c1[i]] : last;
+            dst.assign(pack_host_.data() + pos[i], pack_host_.data() + pos[i] + batch_->fieldBytes(i));
+        }
+    }
+
     static constexpr int kWorkers = 8;
+    static constexpr size_t kBatchMin = 2;  // chain-1 chunks below this run per chunk
     struct Worker {
         hipStream_t s = nullptr;
         Chain chain[FZS_NCHAINS];
@@ -407,6 +490,12 @@ class ChainSelectCompressor {
     std::vector<uint8_t> choice_, out_;
     std::vector<uint64_t> sizes_;
     float select_ms_ = 0, compress_ms_ = 0, total_ms_ = 0;
+    std::unique_ptr<BatchCompressor> batch_;
+    DevBuf pack_, pack_desc_;
+    std::vector<uint8_t> pack_host_;
+    float batch_eb_ = 0;
+    std::vector<uint64_t> batch_idx_;
+    const void* batch_base_ = nullptr;
 };
 
 }  // namespace fz

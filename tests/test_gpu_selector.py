@@ -8,6 +8,7 @@ import pytest
 from fzgpumodules_amd import fields
 from fzgpumodules_amd._fzh import DeviceBuffer, FzhError, call
 from fzgpumodules_amd import selector as S
+from fzgpumodules_amd.pipeline import Pipeline
 from oracle import selector_ref as R
 
 pytestmark = pytest.mark.gpu
@@ -195,6 +196,45 @@ def test_selecting_compressor_round_trip(trained):
         sc.close()
         d.free()
         out.free()
+
+
+def test_chain1_chunks_equal_the_pipeline(trained):
+    """Chunks on chain 1 (LorenzoQuant ABS -> Bitshuffle(16384, 2) ->
+    RZE(16384, 4)) are compressed as one batch; every chunk's bytes in the
+    container equal fz::Pipeline::compress of that chunk alone with the
+    chain's settings (radius 512, outlier capacity 0.2, zigzag)."""
+    x = _composite_field()
+    d = _dev(x)
+    sc = S.SelectingCompressor(fields.CHUNK_2D)
+    try:
+        for _ in range(2):  # the second call reuses the batch plan
+            blob = sc.compress(d.ptr, x.shape, 1e-3, mode=S.FIXED, chain=1)
+        assert np.all(sc.choices() == 1)
+    finally:
+        sc.close()
+        d.free()
+    abs_eb = float(blob[56:60].view(np.float32)[0])
+    want = []
+    for cy in range(x.shape[0] // 256):
+        for cx in range(x.shape[1] // 256):
+            tile = np.ascontiguousarray(x[256 * cy:256 * (cy + 1), 256 * cx:256 * (cx + 1)])
+            with Pipeline(tile.nbytes) as p:
+                lz = p.add_lorenzo(abs_eb, mode="ABS", quant_radius=512, outlier_capacity=0.2, zigzag=True)
+                bs = p.add_bitshuffle(16384, 2)
+                rz = p.add_rze(16384, 4)
+                p.connect(bs, lz, "codes")
+                p.connect(rz, bs, "output")
+                p.set_dims(256, 256, 1)
+                p.finalize()
+                want.append(p.compress_array(tile))
+    pad = [(w.size + 15) & ~15 for w in want]
+    start = blob.size - sum(pad)
+    sizes = blob[start - 8 * len(want):start].view(np.uint64)
+    np.testing.assert_array_equal(sizes, [w.size for w in want])
+    o = start
+    for c, w in enumerate(want):
+        np.testing.assert_array_equal(blob[o:o + w.size], w, err_msg=f"chunk {c}")
+        o += pad[c]
 
 
 # ---------------------------------------------------------------------------
