@@ -47,10 +47,14 @@ __global__ void __launch_bounds__(256)
 hist_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, uint32_t* __restrict__ freq, int replicas) {
     extern __shared__ uint32_t sh[];
     const int wv = threadIdx.x >> 6;
-    const int rep = replicas == 4 ? wv : 0;
-    for (uint32_t i = threadIdx.x; i < bklen * (uint32_t)replicas; i += blockDim.x) sh[i] = 0;
+    // 8 replicas: two per wave, by lane parity (half the same-bin lanes of
+    // one LDS atomic); replica stride bklen + 1 words, so one bin of two
+    // replicas sits in different banks
+    const uint32_t rs = replicas == 8 ? bklen + 1 : bklen;
+    const int rep = replicas == 8 ? 2 * wv + (threadIdx.x & 1) : (replicas == 4 ? wv : 0);
+    for (uint32_t i = threadIdx.x; i < rs * (uint32_t)replicas; i += blockDim.x) sh[i] = 0;
     __syncthreads();
-    uint32_t* h = sh + (size_t)rep * bklen;
+    uint32_t* h = sh + (size_t)rep * rs;
     constexpr int E = 16 / sizeof(T);
     const uint64_t nv = n / E;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -88,7 +92,7 @@ hist_kernel(const T* __restrict__ in, uint64_t n, uint32_t bklen, uint32_t* __re
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < bklen; b += blockDim.x) {
         uint32_t s = 0;
-        for (int r = 0; r < replicas; r++) s += sh[(size_t)r * bklen + b];
+        for (int r = 0; r < replicas; r++) s += sh[(size_t)r * rs + b];
         if (s) atomicAdd(&freq[b], s);
     }
 }
@@ -685,12 +689,13 @@ int fzh_histogram(void* stream, const void* d_in, size_t n, int elem_bytes, uint
     if (int rc = zero_bytes(S(stream), d_freq, 4 * (size_t)bklen)) return rc;
     if (n == 0) return FZH_OK;
     if ((uintptr_t)d_in % 16) return fail(FZH_E_INVALID, "histogram: input must be 16-byte aligned");
-    const int replicas = bklen <= 4096 ? 4 : (bklen <= 16384 ? 1 : 0);
+    const int replicas = bklen <= 1024 ? 8 : (bklen <= 4096 ? 4 : (bklen <= 16384 ? 1 : 0));
+    const uint32_t rs = replicas == 8 ? bklen + 1 : bklen;  // replica stride in words (LDS <= 64 KiB)
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(cdiv(n, 256 * 64), 1024));
     hipStream_t s = S(stream);
 #define FZH_HIST(T)                                                                                       \
     if (replicas)                                                                                         \
-        launch_k("hist_kernel", hist_kernel<T>, dim3(blocks), dim3(256), (uint32_t)(4 * bklen * replicas), s, (const T*)d_in, n, bklen, d_freq, replicas); \
+        launch_k("hist_kernel", hist_kernel<T>, dim3(blocks), dim3(256), (uint32_t)(4 * rs * replicas), s, (const T*)d_in, n, bklen, d_freq, replicas); \
     else                                                                                                  \
         launch_k("hist_global_kernel", hist_global_kernel<T>, dim3(blocks), dim3(256), 0, s, (const T*)d_in, n, bklen, d_freq);
     switch (elem_bytes) {
